@@ -1,0 +1,377 @@
+"""geye_amd -- MI355X (gfx950) DPM detection path behind the reference's entry points.
+
+Python mirror of the C ABI in include/geye_dpm.h (ctypes over libgeye_amd.so).  The names
+follow the reference's C++ entry points (CT = CAR_TRACKING/ of the reference repository):
+
+    resize(img, scale)                     CT/resize.cc:135
+    calc_feature(img, sbin)                CT/featurepyramid.cpp.new2:263
+    calc_f_pyramid(frame, sbin, ...)       CT/featurepyramid.cpp.new2:952
+    fconvsMT(feat, filters, h, w)          CT/get_boxes_func.h:28 (CT/fconvsMT.cpp:310)
+    dt(vals, ax, bx, ay, by)               CT/dt.cpp:101
+    get_boxes(frame, thresh)               CT/get_boxes.cpp:583 (with detect, CT/detect.cpp:151)
+    nms(boxes, overlap)                    CT/nms.cpp:94
+    car_detection(frame, thresh, overlap)  CT/detect.cpp:232 (detect + nms)
+
+Arrays use the reference's layouts: images and features are column-major planes
+([c][x][y], y fastest), maps are [x][y].  Every call runs the HIP kernels; there is no CPU
+fallback, and a missing library raises at import of the first call.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgeye_amd.so")
+
+GEYE_FP32, GEYE_BF16, GEYE_FP16 = 0, 1, 2
+PRECISIONS = {"fp32": GEYE_FP32, "bf16": GEYE_BF16, "fp16": GEYE_FP16}
+
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+_llp = C.POINTER(C.c_longlong)
+_dp = C.POINTER(C.c_double)
+
+
+class GeyeError(RuntimeError):
+    pass
+
+
+class ModelDesc(C.Structure):
+    """geye_model_desc (include/geye_dpm.h)."""
+
+    _fields_ = [
+        ("numcomponent", C.c_int), ("sbin", C.c_int), ("interval", C.c_int),
+        ("max_Y", C.c_int), ("max_X", C.c_int), ("padx", C.c_int), ("pady", C.c_int),
+        ("maxpart", C.c_int),
+        ("ridx", _ip), ("oidx", _ip), ("offw", _fp), ("rsize", _ip), ("numpart", _ip),
+        ("pidx", _ip), ("didx", _ip), ("psize", _ip),
+        ("DefL", C.c_int), ("deff", _fp), ("anchor", _ip),
+        ("NoR", C.c_int), ("root_size", _ip), ("rootfilter", _fp), ("root_off", _llp),
+        ("NoP", C.c_int), ("part_size", _ip), ("partfilter", _fp), ("part_off", _llp),
+    ]
+
+
+# (name, restype, argtypes) of every C ABI entry point
+_SIGS = [
+    ("geye_last_error", C.c_char_p, []),
+    ("geye_device_count", C.c_int, [_ip]),
+    ("geye_init", C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    ("geye_shutdown", None, [C.c_void_p]),
+    ("geye_model_load_csv", C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_char_p,
+                                      C.POINTER(C.c_void_p)]),
+    ("geye_model_create", C.c_int, [C.c_void_p, C.POINTER(ModelDesc), C.POINTER(C.c_void_p)]),
+    ("geye_model_free", None, [C.c_void_p]),
+    ("geye_model_info", C.c_int, [C.c_void_p, _ip, _ip, _ip, _ip, _ip, _ip, _ip]),
+    ("geye_level_plan", C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _fp, _ip, _ip]),
+    ("geye_get_boxes", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                 C.c_float, C.c_int, C.POINTER(_fp), _ip, _ip]),
+    ("geye_detect", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                              C.c_float, C.c_float, C.c_int, C.POINTER(_fp), _ip, _ip]),
+    ("geye_detect_dev", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                  C.c_float, C.c_float, C.c_int, C.POINTER(_fp), _ip, _ip]),
+    ("geye_last_scores", C.c_int, [C.c_void_p, _fp, C.c_longlong, _llp]),
+    ("geye_free", None, [C.c_void_p]),
+    ("geye_resize", C.c_int, [C.c_void_p, _fp, _ip, C.c_float, _fp, _ip]),
+    ("geye_calc_feature", C.c_int, [C.c_void_p, _fp, C.c_int, C.c_int, C.c_int, _fp, _ip, _ip]),
+    ("geye_pyramid", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                               C.c_int, C.c_int, _fp, C.c_longlong, _ip, _ip, _fp]),
+    ("geye_fconv", C.c_int, [C.c_void_p, _fp, C.c_int, C.c_int, _fp, C.c_int, C.c_int, C.c_int,
+                             C.c_int, _fp]),
+    ("geye_dt", C.c_int, [C.c_void_p, _fp, C.c_float, C.c_float, C.c_float, C.c_float, _ip, _fp,
+                          _ip, _ip]),
+    ("geye_nms", C.c_int, [C.c_void_p, _fp, C.c_int, C.c_int, C.c_float, _fp, _ip]),
+    ("geye_dev_alloc", C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
+    ("geye_dev_free", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("geye_memcpy_h2d", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("geye_synchronize", C.c_int, [C.c_void_p]),
+    ("geye_profile_enable", C.c_int, [C.c_void_p, C.c_int]),
+    ("geye_kernel_stats", C.c_int, [C.c_void_p, C.c_char_p, C.c_int, _dp, _llp, C.c_int, _ip]),
+]
+
+_lib = None
+
+
+def lib():
+    """Load libgeye_amd.so (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GeyeError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
+                            "(python -c 'import __graft_entry__ as g; g.build()')")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in _SIGS:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return [s[0] for s in _SIGS]
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().geye_last_error()
+        raise GeyeError(f"geye error {rc}: {msg.decode() if msg else ''}")
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def desc_from_model(model):
+    """Build a ModelDesc (and the arrays it points into) from a synth.DPMModel."""
+    keep = []
+
+    def arr(a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a
+
+    m = model
+    d = ModelDesc()
+    d.numcomponent, d.sbin, d.interval = m.numcomponent, m.sbin, m.interval
+    d.max_Y, d.max_X, d.padx, d.pady, d.maxpart = m.max_Y, m.max_X, m.padx, m.pady, m.maxpart
+    d.ridx = _p(arr(m.ridx, np.int32), _ip)
+    d.oidx = _p(arr(m.oidx, np.int32), _ip)
+    d.offw = _p(arr(m.offw, np.float32), _fp)
+    d.rsize = _p(arr(m.rsize, np.int32), _ip)
+    d.numpart = _p(arr(m.numpart, np.int32), _ip)
+    d.pidx = _p(arr(m.pidx, np.int32), _ip)
+    d.didx = _p(arr(m.didx, np.int32), _ip)
+    d.psize = _p(arr(m.psize, np.int32), _ip)
+    d.DefL = int(m.deff.shape[0])
+    d.deff = _p(arr(m.deff, np.float32), _fp)
+    d.anchor = _p(arr(m.anchor, np.int32), _ip)
+    d.NoR = m.NoR
+    d.root_size = _p(arr(m.root_size, np.int32), _ip)
+    d.rootfilter = _p(arr(np.concatenate(m.rootfilters), np.float32), _fp)
+    d.root_off = _p(arr(np.cumsum([0] + [len(f) for f in m.rootfilters[:-1]]), np.int64), _llp)
+    d.NoP = m.NoP
+    if m.NoP:
+        d.part_size = _p(arr(m.part_size, np.int32), _ip)
+        d.partfilter = _p(arr(np.concatenate(m.partfilters), np.float32), _fp)
+        d.part_off = _p(arr(np.cumsum([0] + [len(f) for f in m.partfilters[:-1]]), np.int64), _llp)
+    return d, keep
+
+
+class Context:
+    """One device context (geye_init / geye_shutdown)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib().geye_init(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().geye_shutdown(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- models
+    def model(self, model) -> "Model":
+        d, keep = desc_from_model(model)
+        h = C.c_void_p()
+        _check(lib().geye_model_create(self.h, C.byref(d), C.byref(h)))
+        return Model(self, h, getattr(model, "name", "model"))
+
+    def load_model_csv(self, comp: str, root: str, part: str) -> "Model":
+        h = C.c_void_p()
+        _check(lib().geye_model_load_csv(self.h, comp.encode(), root.encode(), part.encode(),
+                                         C.byref(h)))
+        return Model(self, h, os.path.basename(comp))
+
+    # ---- stage entry points (reference names)
+    def resize(self, img: np.ndarray, scale: float) -> np.ndarray:
+        """img float32 [3][w][h] (column-major planes) -> resized, same layout."""
+        c, w, h = img.shape
+        img = np.ascontiguousarray(img, dtype=np.float32)
+        sd = np.array([h, w, c], np.int32)
+        od = np.zeros(3, np.int32)
+        _check(lib().geye_resize(self.h, _p(img, _fp), _p(sd, _ip), C.c_float(scale), None,
+                                 _p(od, _ip)))
+        out = np.zeros((od[2], od[1], od[0]), np.float32)
+        _check(lib().geye_resize(self.h, _p(img, _fp), _p(sd, _ip), C.c_float(scale),
+                                 _p(out, _fp), _p(od, _ip)))
+        return out
+
+    def calc_feature(self, img: np.ndarray, sbin: int) -> np.ndarray:
+        c, w, h = img.shape
+        img = np.ascontiguousarray(img, dtype=np.float32)
+        F0, F1 = C.c_int(), C.c_int()
+        _check(lib().geye_calc_feature(self.h, _p(img, _fp), h, w, sbin, None, C.byref(F0),
+                                       C.byref(F1)))
+        out = np.zeros((31, F1.value, F0.value), np.float32)
+        if out.size:
+            _check(lib().geye_calc_feature(self.h, _p(img, _fp), h, w, sbin, _p(out, _fp),
+                                           C.byref(F0), C.byref(F1)))
+        return out
+
+    def calc_f_pyramid(self, frame: np.ndarray, sbin: int, interval: int, max_scale: int):
+        H, W, _ = frame.shape
+        frame = np.ascontiguousarray(frame, dtype=np.uint8)
+        LEN = max_scale + interval
+        fh, fw = np.zeros(LEN, np.int32), np.zeros(LEN, np.int32)
+        sc = np.zeros(LEN, np.float32)
+        _check(lib().geye_pyramid(self.h, frame.ctypes.data, W, H, 3 * W, sbin, interval,
+                                  max_scale, None, 0, _p(fh, _ip), _p(fw, _ip), _p(sc, _fp)))
+        total = int(sum(31 * int(a) * int(b) for a, b in zip(fh, fw)))
+        buf = np.zeros(max(total, 1), np.float32)
+        _check(lib().geye_pyramid(self.h, frame.ctypes.data, W, H, 3 * W, sbin, interval,
+                                  max_scale, _p(buf, _fp), total, _p(fh, _ip), _p(fw, _ip),
+                                  _p(sc, _fp)))
+        feats, off = [], 0
+        for l in range(LEN):
+            n = 31 * int(fh[l]) * int(fw[l])
+            feats.append(buf[off:off + n].reshape(31, int(fw[l]), int(fh[l])))
+            off += n
+        return feats, fh, fw, sc
+
+    def fconvsMT(self, feat: np.ndarray, filters, h: int, w: int,
+                 precision: str = "fp32") -> np.ndarray:
+        """feat [31][F1][F0]; filters: list of flat [31][w][h] arrays -> [n][W][H]."""
+        _, F1, F0 = feat.shape
+        feat = np.ascontiguousarray(feat, dtype=np.float32)
+        fl = np.ascontiguousarray(np.concatenate([np.asarray(f, np.float32).ravel() for f in filters]))
+        n = len(filters)
+        out = np.zeros((n, F1 - w + 1, F0 - h + 1), np.float32)
+        _check(lib().geye_fconv(self.h, _p(feat, _fp), F0, F1, _p(fl, _fp), n, h, w,
+                                PRECISIONS[precision], _p(out, _fp)))
+        return out
+
+    def dt(self, vals: np.ndarray, ax, bx, ay, by):
+        """vals [dims1][dims0] -> (M, Ix, Iy) in the same layout."""
+        d1, d0 = vals.shape
+        v = np.ascontiguousarray(vals, dtype=np.float32)
+        dims = np.array([d0, d1], np.int32)
+        M = np.zeros_like(v)
+        Ix = np.zeros((d1, d0), np.int32)
+        Iy = np.zeros((d1, d0), np.int32)
+        _check(lib().geye_dt(self.h, _p(v, _fp), C.c_float(ax), C.c_float(bx), C.c_float(ay),
+                             C.c_float(by), _p(dims, _ip), _p(M, _fp), _p(Ix, _ip), _p(Iy, _ip)))
+        return M, Ix, Iy
+
+    def nms(self, boxes: np.ndarray, overlap: float) -> np.ndarray:
+        boxes = np.ascontiguousarray(boxes, dtype=np.float32)
+        n, GL = boxes.shape
+        out = np.zeros_like(boxes)
+        k = C.c_int()
+        _check(lib().geye_nms(self.h, _p(boxes, _fp), n, GL, C.c_float(overlap), _p(out, _fp),
+                              C.byref(k)))
+        return out[:k.value].copy()
+
+    # ---- profiling
+    def profile(self, on: bool = True):
+        _check(lib().geye_profile_enable(self.h, 1 if on else 0))
+
+    def kernel_stats(self) -> dict:
+        n = C.c_int()
+        cap = 64
+        names = C.create_string_buffer(8192)
+        ms = np.zeros(cap, np.float64)
+        cnt = np.zeros(cap, np.int64)
+        _check(lib().geye_kernel_stats(self.h, names, 8192, _p(ms, _dp), _p(cnt, _llp), cap,
+                                       C.byref(n)))
+        keys = names.value.decode().split("\n")[: n.value]
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(keys)}
+
+    def synchronize(self):
+        _check(lib().geye_synchronize(self.h))
+
+    # ---- device memory
+    def dev_upload(self, arr: np.ndarray) -> C.c_void_p:
+        arr = np.ascontiguousarray(arr)
+        p = C.c_void_p()
+        _check(lib().geye_dev_alloc(self.h, arr.nbytes, C.byref(p)))
+        _check(lib().geye_memcpy_h2d(self.h, p, arr.ctypes.data, arr.nbytes))
+        return p
+
+    def dev_free(self, p):
+        _check(lib().geye_dev_free(self.h, p))
+
+
+class Model:
+    def __init__(self, ctx: Context, h, name: str):
+        self.ctx, self.h, self.name = ctx, h, name
+
+    def free(self):
+        if self.h:
+            lib().geye_model_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        v = [C.c_int() for _ in range(7)]
+        _check(lib().geye_model_info(self.h, *[C.byref(x) for x in v]))
+        keys = ["numcomponent", "sbin", "interval", "max_Y", "max_X", "NoR", "NoP"]
+        return {k: x.value for k, x in zip(keys, v)}
+
+    def level_plan(self, width: int, height: int) -> dict:
+        ms, LEN = C.c_int(), C.c_int()
+        _check(lib().geye_level_plan(self.h, width, height, C.byref(ms), C.byref(LEN), None,
+                                     None, None))
+        n = LEN.value
+        sc = np.zeros(n, np.float32)
+        fh, fw = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        _check(lib().geye_level_plan(self.h, width, height, C.byref(ms), C.byref(LEN),
+                                     _p(sc, _fp), _p(fh, _ip), _p(fw, _ip)))
+        return dict(max_scale=ms.value, LEN=n, scales=sc, feat_h=fh, feat_w=fw)
+
+    def _rows(self, fn, *args) -> np.ndarray:
+        bp = _fp()
+        n, GL = C.c_int(), C.c_int()
+        _check(fn(*args, C.byref(bp), C.byref(n), C.byref(GL)))
+        try:
+            k = n.value * GL.value
+            out = np.ctypeslib.as_array(bp, shape=(max(k, 1),))[:k].copy()
+        finally:
+            lib().geye_free(bp)
+        return out.reshape(n.value, GL.value)
+
+    def get_boxes(self, frame: np.ndarray, thresh: float = -0.5,
+                  precision: str = "fp32") -> np.ndarray:
+        H, W, _ = frame.shape
+        frame = np.ascontiguousarray(frame, dtype=np.uint8)
+        return self._rows(lib().geye_get_boxes, self.ctx.h, self.h, frame.ctypes.data, W, H,
+                          3 * W, C.c_float(thresh), PRECISIONS[precision])
+
+    def car_detection(self, frame: np.ndarray, thresh: float = -0.5, overlap: float = 0.5,
+                      precision: str = "fp32") -> np.ndarray:
+        H, W, _ = frame.shape
+        frame = np.ascontiguousarray(frame, dtype=np.uint8)
+        return self._rows(lib().geye_detect, self.ctx.h, self.h, frame.ctypes.data, W, H, 3 * W,
+                          C.c_float(thresh), C.c_float(overlap), PRECISIONS[precision])
+
+    def detect_dev(self, d_frame, W: int, H: int, thresh: float = -0.5, overlap: float = 0.5,
+                   precision: str = "fp32") -> np.ndarray:
+        return self._rows(lib().geye_detect_dev, self.ctx.h, self.h, d_frame, W, H, 3 * W,
+                          C.c_float(thresh), C.c_float(overlap), PRECISIONS[precision])
+
+    def last_scores(self) -> np.ndarray:
+        total = C.c_longlong()
+        _check(lib().geye_last_scores(self.ctx.h, None, 0, C.byref(total)))
+        out = np.zeros(max(total.value, 1), np.float32)
+        _check(lib().geye_last_scores(self.ctx.h, _p(out, _fp), total.value, C.byref(total)))
+        return out[: total.value]

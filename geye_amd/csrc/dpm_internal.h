@@ -1,0 +1,194 @@
+// dpm_internal.h -- shared host/device definitions of the gfx950 DPM detection path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/geye_dpm.h"
+
+namespace geye {
+
+// ---------------------------------------------------------------------------------------
+// Device work descriptors (plain structs, copied to HBM once per plan).
+// ---------------------------------------------------------------------------------------
+
+// One separable-resize pass over one image: out[c][dy][x] (rows pass) or
+// out[c][y][dx] (columns pass) as a weighted sum of input samples; (start, count) index
+// the interpolation table per destination index.
+struct ResizeOp {
+  long long in_off, out_off;   // element offsets into the image arena
+  int in_h, in_w;              // input image (row-major [3][in_h][in_w])
+  int out_h, out_w;            // output image
+  int tab_off;                 // offset of this op's (start, count) pairs
+  int blocks_begin;            // first flat block of this op in the launch
+};
+
+struct HogLevel {
+  long long img_off;           // image arena offset ([3][h][w] row-major)
+  long long hist_off;          // hist arena offset ([18][b1][b0])
+  long long norm_off;          // norm arena offset ([b1][b0])
+  long long feat_off;          // feature arena offset ([31][F1][F0])
+  long long featb_off;         // bf16 channel-last arena offset ([F1][F0][32])
+  int h, w, sbin, b0, b1, F0, F1;
+  int tiles_x, tiles_y;        // hist tiles
+  int blocks_begin;            // first flat block (hist or feat launch)
+  int T;                       // cells per hist tile side
+};
+
+// One convolution tile: NF filters of one shape against one feature level.
+struct ConvTile {
+  long long feat_off;          // [31][F1][F0]
+  int F0, F1;
+  int H, W;                    // output map size
+  int x0, y0;
+  int filt_base;               // group index into the packed filter buffer
+  int out_slot;                // first output map slot of the group
+};
+
+// One DT job: a block of lines (columns or rows) of one map.
+struct DtJob {
+  long long src_off, dst_off;  // map base offsets (part score arena / DT arenas)
+  int n, nlines, line0;        // line length, lines in this job, first line
+  int line_stride, elem_stride;
+  int node_off, depth_off, depth_count;  // node table for length n
+  int G;                       // lanes per line
+  int neg;                     // negate the source on load
+  float a, b;
+};
+
+struct ScoreMap {              // one (valid level, component) root score map
+  long long root_off;          // root conv output offset
+  long long score_off;         // flat score array offset (reference output order)
+  int R0, R1;
+  int level, comp;
+  int npart;
+  int part_first;              // index into ScorePart[]
+  float off;                   // offw[ridx[comp]]
+  float scale;                 // sbin / scales[level]
+};
+
+struct ScorePart {
+  long long M_off;             // DT output map
+  long long Ix_off;            // tmpIx arena offset (same map geometry)
+  long long Iy_off;            // tmpIy arena offset
+  int P0, P1, ax, ay, ps0, ps1;
+};
+
+// ---------------------------------------------------------------------------------------
+// Host-side plan for one (frame size, model) pair.
+// ---------------------------------------------------------------------------------------
+struct Level {
+  int img_h, img_w, sbin, F0, F1;
+  float scale;
+  int img_index;               // which arena image feeds this level
+  bool valid;                  // root level with F0 >= max_Y && F1 >= max_X
+};
+
+struct DeviceBuffer {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+struct Plan {
+  int W = 0, H = 0, ws = 0;
+  int max_scale = 0, LEN = 0, interval = 0, sbin = 0;
+  std::vector<Level> lv;
+  // images: index -> (h, w, arena offset)
+  std::vector<int> img_h, img_w;
+  std::vector<long long> img_off;
+  long long img_total = 0;
+  // resize rounds: each round is (rows ops, cols ops) over flat blocks
+  struct Round {
+    std::vector<ResizeOp> rows, cols;
+    int rows_blocks = 0, cols_blocks = 0;
+    long long tmp_total = 0;
+  };
+  std::vector<Round> rounds;
+  std::vector<int2> resize_tab;      // (start, count) per destination index, all ops
+  std::vector<int> resize_si;
+  std::vector<float> resize_alpha;
+  // HOG
+  std::vector<HogLevel> hog;          // [T16 levels][T8 levels][feature levels]
+  int hog16_n = 0, hog8_n = 0, hogf_n = 0;
+  int hog_hist_blocks = 0, hog8_blocks = 0, hog_feat_blocks = 0;
+  long long hist_total = 0, norm_total = 0, feat_total = 0, featb_total = 0;
+  std::vector<long long> feat_off;    // per level, -1 when the level has no features
+  // convolution
+  std::vector<ConvTile> conv_tiles;    // all tiles, grouped by shape class
+  struct ConvClass {
+    int h, w, nf, tile_begin, tile_count, filt_off;
+  };
+  std::vector<ConvClass> conv_classes;
+  std::vector<long long> map_off;      // output map slot -> offset in conv arena
+  std::vector<int2> map_dims;          // output map slot -> (H, W)
+  long long conv_total = 0;
+  std::vector<long long> root_slot;    // [level][NoR] -> slot (or -1)
+  std::vector<long long> part_slot;    // [L][NoP] -> slot (or -1)
+  // DT
+  std::vector<DtJob> dt_pass1, dt_pass2;
+  std::vector<int4> dt_nodes;          // (d, lo, hi, 0) per node, per length
+  std::vector<int> dt_depths;          // node index where each depth starts (+ end)
+  std::unordered_map<int, std::pair<int, int>> dt_len_tab;  // n -> (node_off, depth_off)
+  long long dt_total = 0;              // cells per DT arena
+  size_t dt_lds1 = 0, dt_lds2 = 0;     // dynamic LDS of the two DT launches
+  // scoring
+  std::vector<ScoreMap> smaps;
+  std::vector<ScorePart> sparts;
+  long long score_total = 0;
+  int GL = 0;
+  // device copies of the tables
+  DeviceBuffer d_resize_ops, d_resize_tab, d_resize_si, d_resize_alpha, d_hog, d_conv_tiles,
+      d_map_off, d_dt1, d_dt2, d_nodes, d_depths, d_smaps, d_sparts;
+  // device arenas
+  DeviceBuffer d_frame, d_img, d_tmp, d_hist, d_norm, d_feat, d_featb, d_conv, d_tmpM,
+      d_tmpIy, d_M, d_tmpIx, d_score, d_flags, d_blocksum, d_cand, d_count, d_rows;
+  long long cand_cap = 0;
+};
+
+struct Model {
+  geye_ctx *ctx = nullptr;
+  int numcomponent = 0, sbin = 0, interval = 0, max_Y = 0, max_X = 0, padx = 0, pady = 0;
+  int maxpart = 0, NoR = 0, NoP = 0, DefL = 0;
+  std::vector<int> ridx, oidx, rsize, numpart, pidx, didx, psize, anchor, root_size, part_size;
+  std::vector<float> offw, def;
+  std::vector<std::vector<float>> rootf, partf;   // reference layout [31][w][h]
+  // packed filters per shape class (device): [group][31][w][h][NF]
+  struct FilterClass {
+    int h, w, nf;
+    bool root;
+    std::vector<int> members;        // filter ids (padded with -1 to a multiple of nf)
+    long long off = 0;               // float offset in d_filters
+  };
+  std::vector<FilterClass> classes;
+  DeviceBuffer d_filters, d_rsize;
+  std::unordered_map<long long, Plan *> plans;   // key = (W << 20) | H
+};
+
+struct KernelStat {
+  double ms = 0;
+  long long count = 0;
+};
+
+}  // namespace geye
+
+struct geye_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  bool profile = false;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::unordered_map<std::string, geye::KernelStat> stats;
+  std::vector<hipEvent_t> event_pool;
+  std::vector<float> last_scores;
+  std::vector<geye::DeviceBuffer> scratch_bufs;
+  geye::Plan *last_plan = nullptr;
+};
+
+struct geye_model {
+  geye::Model m;
+};

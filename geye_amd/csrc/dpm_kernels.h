@@ -1,0 +1,34 @@
+// dpm_kernels.h -- launchers of the gfx950 DPM kernels (defined in the .hip files).
+#pragma once
+
+#include "dpm_internal.h"
+
+namespace geye {
+
+void launch_ingest(hipStream_t s, const unsigned char *bgr, int W, int H, int ws, float *out);
+void launch_resize_rows(hipStream_t s, const ResizeOp *ops, int nops, int blocks,
+                        const int2 *tab, const int *si, const float *alpha, const float *src,
+                        float *dst);
+void launch_resize_cols(hipStream_t s, const ResizeOp *ops, int nops, int blocks,
+                        const int2 *tab, const int *si, const float *alpha, const float *src,
+                        float *dst);
+void launch_hog_hist(hipStream_t s, int T, const HogLevel *lv, int nlv, int blocks,
+                     const float *img, float *hist, float *norm);
+void launch_hog_feat(hipStream_t s, const HogLevel *lv, int nlv, int blocks, const float *hist,
+                     const float *norm, float *feat, void *featb);
+void launch_conv_f32(hipStream_t s, int nf, const ConvTile *tiles, int ntiles, int fh, int fw,
+                     const float *feat, const float *filt, const long long *map_off,
+                     float *out);
+void launch_dt(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, const int4 *nodes,
+               const int *depths, const float *src, float *dst, int *ptr);
+void launch_score(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,
+                  const float *conv, const float *M, float thresh, float *score, int *flags,
+                  long long total);
+int compact_blocks(long long n);
+void launch_compact(hipStream_t s, const int *flags, long long n, int *blocksum, int *total,
+                    int *out);
+void launch_rows(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,
+                 const float *score, const int *cand, int ncand, const int *tmpIx,
+                 const int *tmpIy, int GL, int padx, int pady, const int *rsize, float *rows);
+
+}  // namespace geye

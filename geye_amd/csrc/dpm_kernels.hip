@@ -1,0 +1,743 @@
+// dpm_kernels.hip -- gfx950 kernels of the FP32 DPM detection path.
+//
+// Every kernel reproduces the reference's per-element arithmetic order (the library is
+// built with -ffp-contract=off, so `a += b * c` stays a multiply and an add), which is what
+// makes the FP32 path bit-identical to the CPU restatement of the reference:
+//   resize      CT/resize.cc:52-124        gather form of alphacopy, table order per output
+//   HOG hist    CT/featurepyramid.cpp.new2:323-378  gather per cell, pixels in x-major order
+//   HOG feat    CT/featurepyramid.cpp.new2:395-492  double-promoted normaliser and texture
+//   conv        CT/GPU_function.cu:302-358 per output: f outer, (xp, yp) inner, fused MAC
+//   DT          CT/dt.cpp:61-136           divide-and-conquer, level-synchronous
+//   score       CT/get_boxes.cpp:1022-1041 (+bias) and CT/dt.cpp:38-55 (part add)
+//   boxes       CT/get_boxes.cpp:185-258, 1497-1546
+#include "dpm_kernels.h"
+
+#include <hip/hip_bf16.h>
+
+namespace geye {
+
+// ---------------------------------------------------------------------------------------
+// Ingest: BGR interleaved uint8 -> planar float [3][H][W] (row-major).  Plane 0 is B, as in
+// Ipl_to_FLOAT (CT/featurepyramid.cpp.new2:816-850); the conversion is exact.
+// ---------------------------------------------------------------------------------------
+__global__ void k_ingest(const unsigned char *__restrict__ bgr, int W, int H, int ws,
+                         float *__restrict__ out) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long SQ = (long long)W * H;
+  if (i >= SQ) return;
+  int y = (int)(i / W), x = (int)(i - (long long)y * W);
+  const unsigned char *p = bgr + (long long)y * ws + 3 * x;
+  out[i] = (float)p[0];
+  out[SQ + i] = (float)p[1];
+  out[2 * SQ + i] = (float)p[2];
+}
+
+// ---------------------------------------------------------------------------------------
+// Resize, one separable pass.  Each output sums alpha*src over its table entries in table
+// order starting from 0, exactly the per-destination sequence alphacopy produces.
+// rows pass: out[c][dy][x] = sum_k alpha_k * in[c][si_k][x]   (in: [3][in_h][in_w])
+// cols pass: out[c][y][dx] = sum_k alpha_k * in[c][y][si_k]
+// ---------------------------------------------------------------------------------------
+__device__ inline int find_op(const ResizeOp *ops, int nops, int b) {
+  int k = 0;
+  while (k + 1 < nops && ops[k + 1].blocks_begin <= b) k++;
+  return k;
+}
+
+__global__ __launch_bounds__(256) void k_resize_rows(const ResizeOp *__restrict__ ops, int nops,
+                                                     const int2 *__restrict__ tab,
+                                                     const int *__restrict__ si,
+                                                     const float *__restrict__ alpha,
+                                                     const float *__restrict__ src,
+                                                     float *__restrict__ dst) {
+  const ResizeOp op = ops[find_op(ops, nops, blockIdx.x)];
+  long long e = (long long)(blockIdx.x - op.blocks_begin) * blockDim.x + threadIdx.x;
+  long long per_c = (long long)op.out_h * op.in_w;
+  if (e >= 3 * per_c) return;
+  int c = (int)(e / per_c);
+  long long r = e - c * per_c;
+  int dy = (int)(r / op.in_w), x = (int)(r - (long long)dy * op.in_w);
+  const float *in = src + op.in_off + (long long)c * op.in_h * op.in_w + x;
+  int2 t = tab[op.tab_off + dy];
+  float acc = 0.0f;
+  for (int k = 0; k < t.y; k++) acc += alpha[t.x + k] * in[(long long)si[t.x + k] * op.in_w];
+  dst[op.out_off + e] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_resize_cols(const ResizeOp *__restrict__ ops, int nops,
+                                                     const int2 *__restrict__ tab,
+                                                     const int *__restrict__ si,
+                                                     const float *__restrict__ alpha,
+                                                     const float *__restrict__ src,
+                                                     float *__restrict__ dst) {
+  const ResizeOp op = ops[find_op(ops, nops, blockIdx.x)];
+  long long e = (long long)(blockIdx.x - op.blocks_begin) * blockDim.x + threadIdx.x;
+  long long per_c = (long long)op.out_h * op.out_w;
+  if (e >= 3 * per_c) return;
+  int c = (int)(e / per_c);
+  long long r = e - c * per_c;
+  int y = (int)(r / op.out_w), dx = (int)(r - (long long)y * op.out_w);
+  const float *in = src + op.in_off + ((long long)c * op.in_h + y) * op.in_w;
+  int2 t = tab[op.tab_off + dx];
+  float acc = 0.0f;
+  for (int k = 0; k < t.y; k++) acc += alpha[t.x + k] * in[si[t.x + k]];
+  dst[op.out_off + e] = acc;
+}
+
+// ---------------------------------------------------------------------------------------
+// HOG orientation histogram + block energy.
+// A workgroup owns a TxT tile of cells.  Stage 1 computes, once per pixel of the tile's
+// support, the strongest-channel gradient, its snapped orientation and magnitude
+// (CT/featurepyramid.cpp.new2:325-364) into LDS.  Stage 2 gives each thread one cell and
+// accumulates the bilinear votes of its pixels in the reference's x-major, y-minor order
+// (CT/featurepyramid.cpp.new2:323-378), so each histogram bin sees the same sequence of
+// float additions as the reference's scatter loop.  Stage 3 writes hist[o][x][y] and
+// norm[x][y] = sum_{o<9} (h_o + h_{o+9})^2 (CT/featurepyramid.cpp.new2:391-404).
+// ---------------------------------------------------------------------------------------
+__constant__ float c_hcos[9] = {1.0000f, 0.9397f, 0.7660f, 0.5000f, 0.1736f,
+                                -0.1736f, -0.5000f, -0.7660f, -0.9397f};
+__constant__ float c_hsin[9] = {0.0000f, 0.3420f, 0.6428f, 0.8660f, 0.9848f,
+                                0.9848f, 0.8660f, 0.6428f, 0.3420f};
+
+__device__ inline int floor_div(int a, int b) {  // b > 0
+  int q = a / b;
+  return (a % b != 0 && a < 0) ? q - 1 : q;
+}
+
+// first pixel coordinate whose left interpolation cell is >= c:
+// ixp(x) = floor((2x + 1 - sbin) / (2 sbin)) >= c  <=>  x >= ceil((2 sbin c + sbin - 1) / 2)
+__device__ inline int first_pix(int c, int sbin) { return floor_div(2 * sbin * c + sbin, 2); }
+
+__device__ inline void interp_coord(int x, int sbin, int *ix, float *v0, float *v1) {
+  float xp = (float)(((double)(float)x + 0.5) / (double)(float)sbin - 0.5);
+  int i = (int)floorf(xp);
+  float a = xp - (float)i;
+  *ix = i;
+  *v0 = a;
+  *v1 = (float)(1.0 - (double)a);
+}
+
+template <int T>
+__global__ __launch_bounds__(T *T) void k_hog_hist(const HogLevel *__restrict__ lv, int nlv,
+                                                   const float *__restrict__ img_arena,
+                                                   float *__restrict__ hist_arena,
+                                                   float *__restrict__ norm_arena) {
+  // find the level of this block
+  int li = 0;
+  while (li + 1 < nlv && lv[li + 1].blocks_begin <= (int)blockIdx.x) li++;
+  const HogLevel L = lv[li];
+  const int tile = blockIdx.x - L.blocks_begin;
+  const int tcx = tile / L.tiles_y, tcy = tile - tcx * L.tiles_y;
+  const int cx0 = tcx * T, cy0 = tcy * T;
+  const int sbin = L.sbin;
+  const int vis0 = L.b0 * sbin, vis1 = L.b1 * sbin;
+  // pixel support of the tile, clipped to the reference's loop range [1, vis-2]
+  const int xa = max(1, first_pix(cx0 - 1, sbin));
+  const int xb = min(vis1 - 2, first_pix(cx0 + T, sbin) - 1);
+  const int ya = max(1, first_pix(cy0 - 1, sbin));
+  const int yb = min(vis0 - 2, first_pix(cy0 + T, sbin) - 1);
+  constexpr int RMAX = (T + 1) * (T == 16 ? 4 : 8);
+  constexpr int RS = RMAX + 1;
+  __shared__ float s_v[RMAX * RS];
+  __shared__ int s_o[RMAX * RS];
+  __shared__ float s_acc[18 * T * T];
+  __shared__ int s_ix[RMAX], s_iy[RMAX];
+  __shared__ float s_vx0[RMAX], s_vx1[RMAX], s_vy0[RMAX], s_vy1[RMAX];
+  const int tid = threadIdx.x;
+  const int nx = xb - xa + 1, ny = yb - ya + 1;
+  for (int i = tid; i < 18 * T * T; i += T * T) s_acc[i] = 0.0f;
+  if (nx > 0 && ny > 0) {
+    for (int i = tid; i < nx; i += T * T) interp_coord(xa + i, sbin, &s_ix[i], &s_vx0[i], &s_vx1[i]);
+    for (int i = tid; i < ny; i += T * T) interp_coord(ya + i, sbin, &s_iy[i], &s_vy0[i], &s_vy1[i]);
+    const int h = L.h, w = L.w;
+    const long long SQ = (long long)h * w;
+    const float *img = img_arena + L.img_off;
+    for (int i = tid; i < nx * ny; i += T * T) {
+      int ry = i / nx, rx = i - ry * nx;
+      int x = min(xa + rx, w - 2), y = min(ya + ry, h - 2);
+      const float *s = img + (long long)y * w + x;
+      float dy = s[w] - s[-w];
+      float dx = s[1] - s[-1];
+      float v = dx * dx + dy * dy;
+      s += SQ;
+      float dy2 = s[w] - s[-w];
+      float dx2 = s[1] - s[-1];
+      float v2 = dx2 * dx2 + dy2 * dy2;
+      s += SQ;
+      float dy3 = s[w] - s[-w];
+      float dx3 = s[1] - s[-1];
+      float v3 = dx3 * dx3 + dy3 * dy3;
+      if (v2 > v) { v = v2; dx = dx2; dy = dy2; }
+      if (v3 > v) { v = v3; dx = dx3; dy = dy3; }
+      float best_dot = 0.0f;
+      int best_o = 0;
+#pragma unroll
+      for (int o = 0; o < 9; o++) {
+        float dot = c_hcos[o] * dx + c_hsin[o] * dy;
+        if (dot > best_dot) {
+          best_dot = dot;
+          best_o = o;
+        } else if (-dot > best_dot) {
+          best_dot = -dot;
+          best_o = o + 9;
+        }
+      }
+      s_v[ry * RS + rx] = sqrtf(v);
+      s_o[ry * RS + rx] = best_o;
+    }
+  }
+  __syncthreads();
+  const int ci = tid / T, cj = tid - ci * T;  // cell (cx0 + ci, cy0 + cj)
+  const int cx = cx0 + ci, cy = cy0 + cj;
+  if (cx < L.b1 && cy < L.b0) {
+    // pixels voting into this cell: ixp in {cx-1, cx}
+    const int px0 = max(xa, first_pix(cx - 1, sbin)) - xa;
+    const int px1 = min(xb, first_pix(cx + 1, sbin) - 1) - xa;
+    const int py0 = max(ya, first_pix(cy - 1, sbin)) - ya;
+    const int py1 = min(yb, first_pix(cy + 1, sbin) - 1) - ya;
+    float *acc = s_acc + tid;
+    for (int rx = px0; rx <= px1; rx++) {
+      const float wx = (s_ix[rx] == cx) ? s_vx1[rx] : s_vx0[rx];
+      for (int ry = py0; ry <= py1; ry++) {
+        const float wy = (s_iy[ry] == cy) ? s_vy1[ry] : s_vy0[ry];
+        const float v = s_v[ry * RS + rx];
+        const int o = s_o[ry * RS + rx];
+        acc[o * T * T] += wx * wy * v;
+      }
+    }
+    const long long BSQ = (long long)L.b0 * L.b1;
+    float *hist = hist_arena + L.hist_off + (long long)cx * L.b0 + cy;
+    float nrm = 0.0f;
+#pragma unroll
+    for (int o = 0; o < 18; o++) hist[o * BSQ] = acc[o * T * T];
+#pragma unroll
+    for (int o = 0; o < 9; o++) {
+      float t = acc[o * T * T] + acc[(o + 9) * T * T];
+      nrm += t * t;
+    }
+    norm_arena[L.norm_off + (long long)cx * L.b0 + cy] = nrm;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// HOG features: 18 contrast-sensitive, 9 insensitive, 4 texture (CT/featurepyramid.cpp.new2:
+// 407-492).  The four normalisers are 1/sqrt((double)sum + 1e-4) rounded to float and the
+// texture features 0.2357*t in double, as the reference's promotions compute them.
+// Writes [31][F1][F0] float and, when featb_arena is non-null, [F1][F0][32] bf16.
+// ---------------------------------------------------------------------------------------
+__device__ inline float min_2(float x) { return (x < 0.2f) ? x : 0.2f; }
+
+__device__ inline float norm_factor(const float *p, int b0) {
+  float s = p[0] + p[1] + p[b0] + p[b0 + 1];
+  return (float)(1.0 / sqrt((double)s + 0.0001));
+}
+
+__global__ __launch_bounds__(256) void k_hog_feat(const HogLevel *__restrict__ lv, int nlv,
+                                                  const float *__restrict__ hist_arena,
+                                                  const float *__restrict__ norm_arena,
+                                                  float *__restrict__ feat_arena,
+                                                  __hip_bfloat16 *__restrict__ featb_arena) {
+  int li = 0;
+  while (li + 1 < nlv && lv[li + 1].blocks_begin <= (int)blockIdx.x) li++;
+  const HogLevel L = lv[li];
+  long long e = (long long)(blockIdx.x - L.blocks_begin) * blockDim.x + threadIdx.x;
+  const long long OSQ = (long long)L.F0 * L.F1;
+  if (e >= OSQ) return;
+  const int x = (int)(e / L.F0), y = (int)(e - (long long)x * L.F0);
+  const int b0 = L.b0;
+  const long long BSQ = (long long)b0 * L.b1;
+  const float *norm = norm_arena + L.norm_off;
+  const float n1 = norm_factor(norm + (long long)(x + 1) * b0 + y + 1, b0);
+  const float n2 = norm_factor(norm + (long long)(x + 1) * b0 + y, b0);
+  const float n3 = norm_factor(norm + (long long)x * b0 + y + 1, b0);
+  const float n4 = norm_factor(norm + (long long)x * b0 + y, b0);
+  const float *src = hist_arena + L.hist_off + (long long)(x + 1) * b0 + (y + 1);
+  float *dst = feat_arena + L.feat_off + e;
+  __hip_bfloat16 *db = featb_arena ? featb_arena + L.featb_off + e * 32 : nullptr;
+  float t1 = 0.0f, t2 = 0.0f, t3 = 0.0f, t4 = 0.0f;
+  float hs[18];
+#pragma unroll
+  for (int o = 0; o < 18; o++) {
+    float hv = src[o * BSQ];
+    hs[o] = hv;
+    float h1 = min_2(hv * n1), h2 = min_2(hv * n2), h3 = min_2(hv * n3), h4 = min_2(hv * n4);
+    float f = 0.5f * (h1 + h2 + h3 + h4);
+    dst[o * OSQ] = f;
+    if (db) db[o] = __float2bfloat16(f);
+    t1 += h1; t2 += h2; t3 += h3; t4 += h4;
+  }
+#pragma unroll
+  for (int o = 0; o < 9; o++) {
+    float sum = hs[o] + hs[o + 9];
+    float h1 = min_2(sum * n1), h2 = min_2(sum * n2), h3 = min_2(sum * n3), h4 = min_2(sum * n4);
+    float f = 0.5f * (h1 + h2 + h3 + h4);
+    dst[(18 + o) * OSQ] = f;
+    if (db) db[18 + o] = __float2bfloat16(f);
+  }
+  float tx[4] = {(float)(0.2357 * (double)t1), (float)(0.2357 * (double)t2),
+                 (float)(0.2357 * (double)t3), (float)(0.2357 * (double)t4)};
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    dst[(27 + q) * OSQ] = tx[q];
+    if (db) db[27 + q] = __float2bfloat16(tx[q]);
+  }
+  if (db) db[31] = __float2bfloat16(0.0f);
+}
+
+// ---------------------------------------------------------------------------------------
+// FP32 filter convolution, LDS-tiled.  A workgroup computes a 16 (x) x 64 (y) tile of
+// output positions for NF filters of one shape; each thread owns 4 consecutive y x NF
+// filters.  Per output the arithmetic is exactly CT/GPU_function.cu:302-358:
+//   add = 0; for f: { val = 0; for xp: for yp: val = fma(A, B, val); add += val; }
+// Channels stream through double-buffered LDS tiles.
+// ---------------------------------------------------------------------------------------
+template <int NF>
+__global__ __launch_bounds__(256) void k_conv_f32(const ConvTile *__restrict__ tiles, int fh,
+                                                  int fw, const float *__restrict__ feat_arena,
+                                                  const float *__restrict__ filt,
+                                                  const long long *__restrict__ map_off,
+                                                  float *__restrict__ out_arena) {
+  constexpr int TX = 16, TY = 64, PY = 4;
+  constexpr int AXM = TX + 15, AYP = TY + 16;  // fw, fh <= 16
+  __shared__ float As[2][AXM * AYP];
+  __shared__ float Bs[2][16 * 16 * NF];
+  const ConvTile t = tiles[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int tx = tid >> 4, ty = tid & 15;
+  const int AX = TX + fw - 1, AY = TY + fh - 1;
+  const long long FSQ = (long long)t.F0 * t.F1;
+  const float *A0 = feat_arena + t.feat_off;
+  const float *B0 = filt + (long long)t.filt_base * 31 * fw * fh * NF;
+  const int nb = fw * fh * NF;
+
+  auto stage = [&](int f, int buf) {
+    const float *A = A0 + f * FSQ;
+    for (int i = tid; i < AX * AY; i += 256) {
+      int ax = i / AY, ay = i - ax * AY;
+      int gx = t.x0 + ax, gy = t.y0 + ay;
+      As[buf][ax * AYP + ay] = (gx < t.F1 && gy < t.F0) ? A[(long long)gx * t.F0 + gy] : 0.0f;
+    }
+    const float *B = B0 + (long long)f * nb;
+    for (int i = tid; i < nb; i += 256) Bs[buf][i] = B[i];
+  };
+
+  float acc[PY][NF];
+#pragma unroll
+  for (int i = 0; i < PY; i++)
+#pragma unroll
+    for (int n = 0; n < NF; n++) acc[i][n] = 0.0f;
+
+  stage(0, 0);
+  __syncthreads();
+  for (int f = 0; f < 31; f++) {
+    const int cur = f & 1;
+    if (f + 1 < 31) stage(f + 1, cur ^ 1);
+    float val[PY][NF];
+#pragma unroll
+    for (int i = 0; i < PY; i++)
+#pragma unroll
+      for (int n = 0; n < NF; n++) val[i][n] = 0.0f;
+    const float *Ab = &As[cur][tx * AYP + ty * PY];
+    const float *Bb = &Bs[cur][0];
+    for (int xp = 0; xp < fw; xp++) {
+      for (int yp = 0; yp < fh; yp++) {
+        float b[NF];
+#pragma unroll
+        for (int n = 0; n < NF; n++) b[n] = Bb[(xp * fh + yp) * NF + n];
+#pragma unroll
+        for (int i = 0; i < PY; i++) {
+          float a = Ab[xp * AYP + i + yp];
+#pragma unroll
+          for (int n = 0; n < NF; n++) val[i][n] = __builtin_fmaf(a, b[n], val[i][n]);
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < PY; i++)
+#pragma unroll
+      for (int n = 0; n < NF; n++) acc[i][n] += val[i][n];
+    __syncthreads();
+  }
+  const int x = t.x0 + tx;
+  if (x < t.W) {
+#pragma unroll
+    for (int n = 0; n < NF; n++) {
+      float *o = out_arena + map_off[t.out_slot + n] + (long long)x * t.H;
+#pragma unroll
+      for (int i = 0; i < PY; i++) {
+        int y = t.y0 + ty * PY + i;
+        if (y < t.H) o[y] = 0.0f + acc[i][n];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Distance transform, one 1-D pass over a block of lines (columns or rows) of one map.
+// The reference's recursion (CT/dt.cpp:61-85) visits node d of interval [d1, d2] with the
+// source range [s1, s2] = [ptr[d1-1] or 0, ptr[d2+1] or n-1] and keeps the first index of
+// minimal cost ((src + (a*t)*t) + b*t), t = d - s.  Processing the recursion tree depth by
+// depth gives the same (value, index) at every node; a group of G lanes owns one line, and
+// at depths with fewer nodes than lanes a node's range is split across lanes and the
+// partial results are merged lexicographically on (cost, index), which is exactly "first
+// minimum" of the sequential scan.
+// ---------------------------------------------------------------------------------------
+__device__ inline float dt_cost(float src, float a, float b, int t) {
+  float ft = (float)t;
+  return src + a * ft * ft + b * ft;
+}
+
+__global__ __launch_bounds__(256) void k_dt(const DtJob *__restrict__ jobs,
+                                            const int4 *__restrict__ nodes,
+                                            const int *__restrict__ depths,
+                                            const float *__restrict__ src_arena,
+                                            float *__restrict__ dst_arena,
+                                            int *__restrict__ ptr_arena) {
+  extern __shared__ float smem[];
+  const DtJob J = jobs[blockIdx.x];
+  const int n = J.n, G = J.G, NLn = 256 / G;
+  const int tid = threadIdx.x;
+  const int line = tid / G, g = tid - line * G;
+  float *s_src = smem;                                  // [NLn][n+1]
+  int *s_ptr = (int *)(smem + NLn * (n + 1));           // [NLn][n]
+  const float *src = src_arena + J.src_off + (long long)J.line0 * J.line_stride;
+  // stage the lines
+  if (J.elem_stride == 1) {
+    for (int i = tid; i < J.nlines * n; i += 256) {
+      int l = i / n, p = i - l * n;
+      float v = src[(long long)l * J.line_stride + p];
+      s_src[l * (n + 1) + p] = J.neg ? v * -1.0f : v;
+    }
+  } else {
+    for (int i = tid; i < J.nlines * n; i += 256) {
+      int p = i / J.nlines, l = i - p * J.nlines;
+      float v = src[(long long)l * J.line_stride + (long long)p * J.elem_stride];
+      s_src[l * (n + 1) + p] = J.neg ? v * -1.0f : v;
+    }
+  }
+  __syncthreads();
+  const bool active_line = line < J.nlines;
+  const float *ls = s_src + line * (n + 1);
+  int *lp = s_ptr + line * n;
+  float *dst = dst_arena + J.dst_off + (long long)(J.line0 + line) * J.line_stride;
+  int *pdst = ptr_arena + J.dst_off + (long long)(J.line0 + line) * J.line_stride;
+  const float a = J.a, b = J.b;
+  for (int k = 0; k < J.depth_count; k++) {
+    const int nb = depths[J.depth_off + k], ne = depths[J.depth_off + k + 1];
+    const int nk = ne - nb;
+    if (nk >= G) {
+      for (int i = nb + g; i < ne; i += G) {
+        if (!active_line) break;
+        const int4 nd = nodes[J.node_off + i];
+        const int d = nd.x;
+        const int s1 = nd.y < 0 ? 0 : lp[nd.y];
+        const int s2 = nd.z >= n ? n - 1 : lp[nd.z];
+        int s = s1;
+        float best = dt_cost(ls[s1], a, b, d - s1);
+        for (int p = s1 + 1; p <= s2; p++) {
+          float c = dt_cost(ls[p], a, b, d - p);
+          if (best > c) {
+            best = c;
+            s = p;
+          }
+        }
+        lp[d] = s;
+        dst[(long long)d * J.elem_stride] = best;
+        pdst[(long long)d * J.elem_stride] = s;
+      }
+    } else {
+      int S = 1;
+      while (S * 2 * nk <= G) S *= 2;
+      const int ni = g / S, part = g - ni * S;
+      const bool act = active_line && ni < nk;
+      float best = __builtin_inff();
+      int bi = 0x7fffffff;
+      int d = 0;
+      if (act) {
+        const int4 nd = nodes[J.node_off + nb + ni];
+        d = nd.x;
+        const int s1 = nd.y < 0 ? 0 : lp[nd.y];
+        const int s2 = nd.z >= n ? n - 1 : lp[nd.z];
+        const int len = s2 - s1 + 1;
+        const int chunk = (len + S - 1) / S;
+        const int p0 = s1 + part * chunk;
+        const int p1 = min(s2, p0 + chunk - 1);
+        if (p0 <= p1) {
+          best = dt_cost(ls[p0], a, b, d - p0);
+          bi = p0;
+          for (int p = p0 + 1; p <= p1; p++) {
+            float c = dt_cost(ls[p], a, b, d - p);
+            if (best > c) {
+              best = c;
+              bi = p;
+            }
+          }
+        }
+      }
+      for (int m = 1; m < S; m <<= 1) {
+        float ob = __shfl_xor(best, m);
+        int oi = __shfl_xor(bi, m);
+        if (ob < best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if (act && part == 0) {
+        lp[d] = bi;
+        dst[(long long)d * J.elem_stride] = best;
+        pdst[(long long)d * J.elem_stride] = bi;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Scores: root + bias, minus each part's DT value at (ay-1+2y, ax-1+2x) in part order
+// (CT/get_boxes.cpp:1022-1041, CT/dt.cpp:38-55), written in the reference's output order;
+// flags mark score > thresh (CT/get_boxes.cpp:185-220).
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_score(const ScoreMap *__restrict__ maps, int nmaps,
+                                               const ScorePart *__restrict__ parts,
+                                               const float *__restrict__ conv_arena,
+                                               const float *__restrict__ M_arena, float thresh,
+                                               float *__restrict__ score,
+                                               int *__restrict__ flags, long long total) {
+  long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  int lo = 0, hi = nmaps - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (maps[mid].score_off <= e) lo = mid; else hi = mid - 1;
+  }
+  const ScoreMap m = maps[lo];
+  const long long i = e - m.score_off;
+  const int x = (int)(i / m.R0), y = (int)(i - (long long)x * m.R0);
+  float s = conv_arena[m.root_off + i] + m.off;
+  for (int k = 0; k < m.npart; k++) {
+    const ScorePart q = parts[m.part_first + k];
+    s -= M_arena[q.M_off + (q.ay - 1 + 2 * y) + (long long)(q.ax - 1 + 2 * x) * q.P0];
+  }
+  score[e] = s;
+  flags[e] = s > thresh ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------------------
+// Order-preserving stream compaction of flagged indices (block counts, scan, scatter).
+// ---------------------------------------------------------------------------------------
+constexpr int kCompactBlock = 1024;
+
+__global__ __launch_bounds__(256) void k_flag_count(const int *__restrict__ flags, long long n,
+                                                    int *__restrict__ blocksum) {
+  __shared__ int s[256];
+  long long base = (long long)blockIdx.x * kCompactBlock;
+  int c = 0;
+  for (int i = threadIdx.x; i < kCompactBlock; i += 256)
+    if (base + i < n) c += flags[base + i];
+  s[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) blocksum[blockIdx.x] = s[0];
+}
+
+// exclusive scan of nb block sums in one workgroup; writes the total at blocksum[nb]
+__global__ __launch_bounds__(1024) void k_scan_blocks(int *__restrict__ blocksum, int nb,
+                                                      int *__restrict__ total) {
+  __shared__ int s[1024];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += 1024) {
+    int i = base + threadIdx.x;
+    int v = i < nb ? blocksum[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      int t = (int)threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nb) blocksum[i] = carry + s[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += s[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(256) void k_flag_scatter(const int *__restrict__ flags,
+                                                      long long n,
+                                                      const int *__restrict__ blocksum,
+                                                      int *__restrict__ out) {
+  __shared__ int s[256];
+  long long base = (long long)blockIdx.x * kCompactBlock;
+  // each thread owns 4 consecutive elements
+  int f[4], c = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    long long i = base + threadIdx.x * 4 + k;
+    f[k] = (i < n) ? flags[i] : 0;
+    c += f[k];
+  }
+  s[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    int t = (int)threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int pos = blocksum[blockIdx.x] + s[threadIdx.x] - c;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (f[k]) out[pos++] = (int)(base + threadIdx.x * 4 + k);
+}
+
+// ---------------------------------------------------------------------------------------
+// Candidate rows (CT/get_boxes.cpp:1497-1546): root box (rootbox, :227-235), part boxes
+// (partbox, :243-258, including its probe at (2(y-1)+ay, 2(x-1)+ax)), component, score,
+// scale.  Ix/Iy are composed at the probe from the two DT passes' argmins
+// (CT/dt.cpp:120-131).
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_rows(const ScoreMap *__restrict__ maps, int nmaps,
+                                              const ScorePart *__restrict__ parts,
+                                              const float *__restrict__ score,
+                                              const int *__restrict__ cand, int ncand,
+                                              const int *__restrict__ tmpIx,
+                                              const int *__restrict__ tmpIy, int GL, int padx,
+                                              int pady, const int *__restrict__ rsize,
+                                              float *__restrict__ rows) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncand) return;
+  const long long e = cand[c];
+  int lo = 0, hi = nmaps - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (maps[mid].score_off <= e) lo = mid; else hi = mid - 1;
+  }
+  const ScoreMap m = maps[lo];
+  const long long i = e - m.score_off;
+  const int x = (int)(i / m.R0), y = (int)(i - (long long)x * m.R0);
+  const float scale = m.scale;
+  float *r = rows + (long long)c * GL;
+  float r0 = ((float)y - (float)pady + 1.0f) * scale;
+  float r1 = ((float)x - (float)padx + 1.0f) * scale;
+  r[0] = r0;
+  r[1] = r1;
+  r[2] = (float)((double)(r0 + (float)rsize[m.comp * 2] * scale) - 1.0);
+  r[3] = (float)((double)(r1 + (float)rsize[m.comp * 2 + 1] * scale) - 1.0);
+  for (int k = 0; k < m.npart; k++) {
+    const ScorePart q = parts[m.part_first + k];
+    const int probex = (x - 1) * 2 + q.ax, probey = (y - 1) * 2 + q.ay;
+    const long long P = probey + (long long)probex * q.P0;
+    const int ix = tmpIx[q.Ix_off + P];
+    const int iy = tmpIy[q.Iy_off + (long long)ix * q.P0 + probey];
+    const float px = (float)ix + 1.0f, py = (float)iy + 1.0f;
+    float b0 = (float)((((double)py - 2.0) / 2.0 + 1.0 - (double)(float)pady) * (double)scale);
+    float b1 = (float)((((double)px - 2.0) / 2.0 + 1.0 - (double)(float)padx) * (double)scale);
+    float *o = r + 4 + 4 * k;
+    o[0] = b0;
+    o[1] = b1;
+    o[2] = (float)(((double)b0 + (double)((float)q.ps0 * scale) / 2.0) - 1.0);
+    o[3] = (float)(((double)b1 + (double)((float)q.ps1 * scale) / 2.0) - 1.0);
+  }
+  r[4 + 4 * m.npart] = (float)m.comp;
+  r[5 + 4 * m.npart] = score[e];
+  r[6 + 4 * m.npart] = scale;
+}
+
+// ---------------------------------------------------------------------------------------
+// Host-side launchers
+// ---------------------------------------------------------------------------------------
+void launch_ingest(hipStream_t s, const unsigned char *bgr, int W, int H, int ws, float *out) {
+  long long n = (long long)W * H;
+  hipLaunchKernelGGL(k_ingest, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, bgr, W, H, ws,
+                     out);
+}
+
+void launch_resize_rows(hipStream_t s, const ResizeOp *ops, int nops, int blocks,
+                        const int2 *tab, const int *si, const float *alpha, const float *src,
+                        float *dst) {
+  hipLaunchKernelGGL(k_resize_rows, dim3(blocks), dim3(256), 0, s, ops, nops, tab, si, alpha,
+                     src, dst);
+}
+
+void launch_resize_cols(hipStream_t s, const ResizeOp *ops, int nops, int blocks,
+                        const int2 *tab, const int *si, const float *alpha, const float *src,
+                        float *dst) {
+  hipLaunchKernelGGL(k_resize_cols, dim3(blocks), dim3(256), 0, s, ops, nops, tab, si, alpha,
+                     src, dst);
+}
+
+void launch_hog_hist(hipStream_t s, int T, const HogLevel *lv, int nlv, int blocks,
+                     const float *img, float *hist, float *norm) {
+  if (T == 16)
+    hipLaunchKernelGGL(k_hog_hist<16>, dim3(blocks), dim3(256), 0, s, lv, nlv, img, hist, norm);
+  else
+    hipLaunchKernelGGL(k_hog_hist<8>, dim3(blocks), dim3(64), 0, s, lv, nlv, img, hist, norm);
+}
+
+void launch_hog_feat(hipStream_t s, const HogLevel *lv, int nlv, int blocks, const float *hist,
+                     const float *norm, float *feat, void *featb) {
+  hipLaunchKernelGGL(k_hog_feat, dim3(blocks), dim3(256), 0, s, lv, nlv, hist, norm, feat,
+                     (__hip_bfloat16 *)featb);
+}
+
+void launch_conv_f32(hipStream_t s, int nf, const ConvTile *tiles, int ntiles, int fh, int fw,
+                     const float *feat, const float *filt, const long long *map_off,
+                     float *out) {
+  switch (nf) {
+    case 8:
+      hipLaunchKernelGGL(k_conv_f32<8>, dim3(ntiles), dim3(256), 0, s, tiles, fh, fw, feat, filt,
+                         map_off, out);
+      break;
+    case 4:
+      hipLaunchKernelGGL(k_conv_f32<4>, dim3(ntiles), dim3(256), 0, s, tiles, fh, fw, feat, filt,
+                         map_off, out);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_conv_f32<2>, dim3(ntiles), dim3(256), 0, s, tiles, fh, fw, feat, filt,
+                         map_off, out);
+      break;
+    default:
+      hipLaunchKernelGGL(k_conv_f32<1>, dim3(ntiles), dim3(256), 0, s, tiles, fh, fw, feat, filt,
+                         map_off, out);
+      break;
+  }
+}
+
+void launch_dt(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, const int4 *nodes,
+               const int *depths, const float *src, float *dst, int *ptr) {
+  hipLaunchKernelGGL(k_dt, dim3(njobs), dim3(256), lds, s, jobs, nodes, depths, src, dst, ptr);
+}
+
+void launch_score(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,
+                  const float *conv, const float *M, float thresh, float *score, int *flags,
+                  long long total) {
+  hipLaunchKernelGGL(k_score, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, maps, nmaps,
+                     parts, conv, M, thresh, score, flags, total);
+}
+
+int compact_blocks(long long n) { return (int)((n + kCompactBlock - 1) / kCompactBlock); }
+
+void launch_compact(hipStream_t s, const int *flags, long long n, int *blocksum, int *total,
+                    int *out) {
+  int nb = compact_blocks(n);
+  hipLaunchKernelGGL(k_flag_count, dim3(nb), dim3(256), 0, s, flags, n, blocksum);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, blocksum, nb, total);
+  hipLaunchKernelGGL(k_flag_scatter, dim3(nb), dim3(256), 0, s, flags, n, blocksum, out);
+}
+
+void launch_rows(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,
+                 const float *score, const int *cand, int ncand, const int *tmpIx,
+                 const int *tmpIy, int GL, int padx, int pady, const int *rsize, float *rows) {
+  if (ncand <= 0) return;
+  hipLaunchKernelGGL(k_rows, dim3((ncand + 255) / 256), dim3(256), 0, s, maps, nmaps, parts,
+                     score, cand, ncand, tmpIx, tmpIy, GL, padx, pady, rsize, rows);
+}
+
+}  // namespace geye

@@ -1,0 +1,454 @@
+// dpm_plan.cpp -- per-(model, frame size) execution plan of the gfx950 DPM path.
+//
+// The plan fixes every geometric decision on the host once per frame size, in the same
+// integer/float expressions as the reference, and packs the work descriptors the kernels
+// consume: the level plan (CT/featurepyramid.cpp.new2:174-239, CT/featurepyramid.cpp.org:
+// 570-630), the resize interpolation tables (CT/resize.cc:68-108), HOG level geometry
+// (CT/featurepyramid.cpp.new2:271-289), convolution output sizes and the level skip rule
+// (CT/fconvsMT.cpp:409-495), DT jobs and the score-map order of get_boxes
+// (CT/get_boxes.cpp:1410-1570).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "dpm_runtime.h"
+
+namespace geye {
+
+int compute_max_scale(const Model &m, int W, int H) {
+  const int interval = m.interval;
+  const float sc = (float)std::pow(2.0, (1 / (double)interval));
+  int max_scale = 36;
+  const int L_NUM = interval + max_scale;
+  float MRY = (float)m.rsize[0], MRX = (float)m.rsize[1];
+  for (int kk = 1; kk < m.numcomponent; kk++) {
+    if ((float)m.rsize[kk * 2] < MRY) MRY = (float)m.rsize[kk * 2];
+    if ((float)m.rsize[kk * 2 + 1] < MRX) MRX = (float)m.rsize[kk * 2 + 1];
+  }
+  MRY /= 2;
+  MRX /= 2;
+  float height = (float)H / (float)m.sbin;
+  float width = (float)W / (float)m.sbin;
+  const float sc_step = 1 / sc;
+  for (int kk = 0; kk < L_NUM; kk++) {
+    height *= sc_step;
+    width *= sc_step;
+    if (height < MRY || width < MRX) {
+      max_scale = kk - interval - 1;
+      break;
+    }
+  }
+  if (max_scale < interval) max_scale = interval;
+  return max_scale;
+}
+
+static int blocks_of(int n, int sbin) { return (int)std::floor(double(n) / double(sbin) + 0.5); }
+
+void level_geometry(int max_scale, int sbin, int interval, int W, int H, std::vector<Level> &lv,
+                    std::vector<int> &img_h, std::vector<int> &img_w,
+                    std::vector<int> &img_src, std::vector<float> &img_scale) {
+  const int LEN = max_scale + interval;
+  lv.assign(LEN, Level{});
+  img_h.clear(); img_w.clear(); img_src.clear(); img_scale.clear();
+  const int sbin2 = (int)std::floor((double)sbin / 2.0);
+  const float sc = (float)std::pow(2, (1.0 / (double)interval));
+  std::vector<int> level_img(LEN, -1);
+  for (int ii = 0; ii < interval && ii < LEN; ii++) {
+    const float st = (float)(1.0 / std::pow((double)sc, (double)ii));
+    int h = H, w = W;
+    if (st != 1.0f) {
+      h = (int)((double)((float)H * st) + 0.5);
+      w = (int)((double)((float)W * st) + 0.5);
+    }
+    int im = (int)img_h.size();
+    img_h.push_back(h); img_w.push_back(w);
+    img_src.push_back(ii == 0 ? -1 : 0);
+    img_scale.push_back(st);
+    lv[ii] = Level{h, w, sbin2, 0, 0, st * 2, im, false};
+    if (ii + interval < LEN) lv[ii + interval] = Level{h, w, sbin, 0, 0, st, im, false};
+  }
+  for (int ii = 0; ii < interval && ii < LEN; ii++) {
+    int prev_img = lv[ii].img_index;
+    int ph = lv[ii].img_h, pw = lv[ii].img_w;
+    for (int jj = ii + interval; jj < max_scale; jj += interval) {
+      const int l = jj + interval;
+      const int nh = (int)((double)((float)ph * 0.5f) + 0.5);
+      const int nw = (int)((double)((float)pw * 0.5f) + 0.5);
+      int im = (int)img_h.size();
+      img_h.push_back(nh); img_w.push_back(nw);
+      img_src.push_back(prev_img);
+      img_scale.push_back(0.5f);
+      lv[l] = Level{nh, nw, sbin, 0, 0, (float)(0.5 * (double)lv[jj].scale), im, false};
+      prev_img = im;
+      ph = nh;
+      pw = nw;
+    }
+  }
+  for (int l = 0; l < LEN; l++) {
+    Level &L = lv[l];
+    L.F0 = std::max(blocks_of(L.img_h, L.sbin) - 2, 0);
+    L.F1 = std::max(blocks_of(L.img_w, L.sbin) - 2, 0);
+  }
+}
+
+// resize1dtran's interpolation table (CT/resize.cc:68-108), grouped per destination index.
+void resize_tables(int sheight, int dheight, std::vector<int2> &tab, std::vector<int> &si,
+                         std::vector<float> &alpha) {
+  const float scale = (float)dheight / (float)sheight;
+  const float invscale = (float)sheight / (float)dheight;
+  for (int dy = 0; dy < dheight; dy++) {
+    const int start = (int)si.size();
+    const float fsy1 = (float)dy * invscale;
+    const float fsy2 = fsy1 + invscale;
+    const int sy1 = (int)((double)fsy1 + 0.99);
+    const int sy2 = (int)fsy2;
+    const float e1 = (float)sy1 - fsy1;
+    if ((double)e1 > 1e-3) { si.push_back(sy1 - 1); alpha.push_back(e1 * scale); }
+    for (int sy = sy1; sy < sy2; sy++) { si.push_back(sy); alpha.push_back(scale); }
+    const float e2 = fsy2 - (float)sy2;
+    if ((double)e2 > 1e-3) { si.push_back(sy2); alpha.push_back(e2 * scale); }
+    tab.push_back(int2{start, (int)si.size() - start});
+  }
+  // the reference's table never indexes outside the source; keep that an invariant
+  for (size_t k = 0; k < si.size(); k++)
+    if (si[k] < 0 || si[k] >= sheight) si[k] = si[k] < 0 ? 0 : sheight - 1;
+}
+
+static void dt_nodes_for(Plan &P, int n) {
+  if (P.dt_len_tab.count(n)) return;
+  const int node_off = (int)P.dt_nodes.size();
+  const int depth_off = (int)P.dt_depths.size();
+  std::vector<std::pair<int, int>> cur{{0, n - 1}}, next;
+  int idx = 0;
+  while (!cur.empty()) {
+    P.dt_depths.push_back(idx);
+    next.clear();
+    for (auto [d1, d2] : cur) {
+      const int d = (d1 + d2) >> 1;
+      P.dt_nodes.push_back(int4{d, d1 - 1, d2 + 1 >= n ? n : d2 + 1, 0});
+      idx++;
+      if (d - 1 >= d1) next.push_back({d1, d - 1});
+      if (d2 >= d + 1) next.push_back({d + 1, d2});
+    }
+    std::swap(cur, next);
+  }
+  P.dt_depths.push_back(idx);
+  P.dt_len_tab[n] = {node_off, depth_off};
+}
+
+static int dt_depth_count(const Plan &P, int n) {
+  // depths between consecutive entries; recomputed from the table
+  auto it = P.dt_len_tab.find(n);
+  int depth_off = it->second.second;
+  int k = 0;
+  int total = n;
+  while (P.dt_depths[depth_off + k] < total) k++;
+  return k;
+}
+
+static size_t dt_lds_bytes(int n, int G) { return (size_t)(256 / G) * ((size_t)(n + 1) * 4 + (size_t)n * 4); }
+
+static int dt_pick_G(int n) {
+  for (int G = 1; G <= 64; G *= 2)
+    if (dt_lds_bytes(n, G) <= 48 * 1024) return G;
+  return -1;
+}
+
+static void add_dt_jobs(Plan &P, std::vector<DtJob> &jobs, long long src_off, long long dst_off,
+                        int n, int nlines, int line_stride, int elem_stride, int neg, float a,
+                        float b) {
+  dt_nodes_for(P, n);
+  const auto tabs = P.dt_len_tab[n];
+  const int G = dt_pick_G(n);
+  const int NL = 256 / G;
+  const int dc = dt_depth_count(P, n);
+  for (int l0 = 0; l0 < nlines; l0 += NL) {
+    DtJob J;
+    J.src_off = src_off;
+    J.dst_off = dst_off;
+    J.n = n;
+    J.nlines = std::min(NL, nlines - l0);
+    J.line0 = l0;
+    J.line_stride = line_stride;
+    J.elem_stride = elem_stride;
+    J.node_off = tabs.first;
+    J.depth_off = tabs.second;
+    J.depth_count = dc;
+    J.G = G;
+    J.neg = neg;
+    J.a = a;
+    J.b = b;
+    jobs.push_back(J);
+  }
+}
+
+int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_override) {
+  if (m.sbin > 8 || m.sbin < 2) {
+    err = "sbin outside [2, 8] is not supported";
+    return GEYE_ERR_UNSUPPORTED;
+  }
+  if (W < 8 || H < 8) {
+    err = "frame too small";
+    return GEYE_ERR_ARG;
+  }
+  P.W = W;
+  P.H = H;
+  P.interval = m.interval;
+  P.sbin = m.sbin;
+  P.max_scale = max_scale_override >= 0 ? max_scale_override : compute_max_scale(m, W, H);
+  P.LEN = P.max_scale + m.interval;
+  std::vector<int> img_src;
+  std::vector<float> img_scale;
+  level_geometry(P.max_scale, m.sbin, m.interval, W, H, P.lv, P.img_h, P.img_w, img_src,
+                 img_scale);
+  const int LEN = P.LEN, interval = m.interval;
+  for (int l = 0; l < LEN; l++) {
+    Level &L = P.lv[l];
+    if (L.F0 > 0 && L.F1 > 0 && (L.img_h < 3 || L.img_w < 3)) {
+      err = "level image smaller than 3 pixels";
+      return GEYE_ERR_ARG;
+    }
+    L.valid = (l >= interval) && !(L.F0 + 2 * m.pady < m.max_Y || L.F1 + 2 * m.padx < m.max_X);
+  }
+  // ---- images and resize rounds
+  const int nimg = (int)P.img_h.size();
+  P.img_off.assign(nimg, 0);
+  long long off = 0;
+  for (int i = 0; i < nimg; i++) {
+    P.img_off[i] = off;
+    off += 3LL * P.img_h[i] * P.img_w[i];
+  }
+  P.img_total = off;
+  // round of an image = depth of its resize chain
+  std::vector<int> round(nimg, 0);
+  int maxr = 0;
+  for (int i = 0; i < nimg; i++) {
+    if (img_src[i] < 0) round[i] = -1;
+    else if (img_src[i] == 0) round[i] = 0;
+    else round[i] = round[img_src[i]] + 1;
+    maxr = std::max(maxr, round[i]);
+  }
+  P.rounds.assign(maxr + 1, Plan::Round{});
+  P.resize_tab.clear();
+  P.resize_si.clear();
+  P.resize_alpha.clear();
+  for (int r = 0; r <= maxr; r++) {
+    Plan::Round &R = P.rounds[r];
+    long long toff = 0;
+    for (int i = 0; i < nimg; i++) {
+      if (round[i] != r) continue;
+      const int s = img_src[i];
+      const int ih = P.img_h[s], iw = P.img_w[s], oh = P.img_h[i], ow = P.img_w[i];
+      ResizeOp rows{P.img_off[s], toff, ih, iw, oh, iw, (int)P.resize_tab.size(), R.rows_blocks};
+      resize_tables(ih, oh, P.resize_tab, P.resize_si, P.resize_alpha);
+      R.rows_blocks += (int)((3LL * oh * iw + 255) / 256);
+      R.rows.push_back(rows);
+      ResizeOp cols{toff, P.img_off[i], oh, iw, oh, ow, (int)P.resize_tab.size(), R.cols_blocks};
+      resize_tables(iw, ow, P.resize_tab, P.resize_si, P.resize_alpha);
+      R.cols_blocks += (int)((3LL * oh * ow + 255) / 256);
+      R.cols.push_back(cols);
+      toff += 3LL * oh * iw;
+    }
+    R.tmp_total = toff;
+  }
+  // ---- HOG levels (two hist launches: T = 16 for sbin <= 4, T = 8 otherwise)
+  P.hog.clear();
+  P.feat_off.assign(LEN, -1);
+  long long ho = 0, no = 0, fo = 0, fbo = 0;
+  std::vector<HogLevel> h16, h8, hf;
+  int b16 = 0, b8 = 0, bf = 0;
+  for (int l = 0; l < LEN; l++) {
+    const Level &L = P.lv[l];
+    if (L.F0 <= 0 || L.F1 <= 0) continue;
+    HogLevel h{};
+    h.img_off = P.img_off[L.img_index];
+    h.h = L.img_h;
+    h.w = L.img_w;
+    h.sbin = L.sbin;
+    h.b0 = blocks_of(L.img_h, L.sbin);
+    h.b1 = blocks_of(L.img_w, L.sbin);
+    h.F0 = L.F0;
+    h.F1 = L.F1;
+    h.hist_off = ho;
+    h.norm_off = no;
+    h.feat_off = fo;
+    h.featb_off = fbo;
+    P.feat_off[l] = fo;
+    ho += 18LL * h.b0 * h.b1;
+    no += (long long)h.b0 * h.b1;
+    fo += 31LL * L.F0 * L.F1;
+    fbo += 32LL * L.F0 * L.F1;
+    h.T = (L.sbin <= 4) ? 16 : 8;
+    h.tiles_x = (h.b1 + h.T - 1) / h.T;
+    h.tiles_y = (h.b0 + h.T - 1) / h.T;
+    HogLevel hh = h;
+    if (h.T == 16) { hh.blocks_begin = b16; b16 += h.tiles_x * h.tiles_y; h16.push_back(hh); }
+    else { hh.blocks_begin = b8; b8 += h.tiles_x * h.tiles_y; h8.push_back(hh); }
+    HogLevel hft = h;
+    hft.blocks_begin = bf;
+    bf += (int)((1LL * L.F0 * L.F1 + 255) / 256);
+    hf.push_back(hft);
+  }
+  P.hist_total = ho;
+  P.norm_total = no;
+  P.feat_total = fo;
+  P.featb_total = fbo;
+  P.hog = h16;
+  P.hog.insert(P.hog.end(), h8.begin(), h8.end());
+  P.hog.insert(P.hog.end(), hf.begin(), hf.end());
+  P.hog_hist_blocks = b16;  // T16 blocks; T8 count stored below
+  P.hog_feat_blocks = bf;
+  P.hog16_n = (int)h16.size();
+  P.hog8_n = (int)h8.size();
+  P.hog8_blocks = b8;
+  P.hogf_n = (int)hf.size();
+  // ---- convolution tiles and output maps
+  P.conv_tiles.clear();
+  P.conv_classes.clear();
+  P.map_off.clear();
+  P.map_dims.clear();
+  P.root_slot.assign((size_t)LEN * m.NoR, -1);
+  P.part_slot.assign((size_t)LEN * m.NoP, -1);
+  long long co = 0;
+  for (size_t ci = 0; ci < m.classes.size(); ci++) {
+    const Model::FilterClass &fc = m.classes[ci];
+    Plan::ConvClass cc{fc.h, fc.w, fc.nf, (int)P.conv_tiles.size(), 0, (int)ci};
+    const int ngroups = (int)fc.members.size() / fc.nf;
+    for (int l = interval; l < LEN; l++) {
+      if (!P.lv[l].valid) continue;
+      const int fl = fc.root ? l : l - interval;
+      const Level &FL = P.lv[fl];
+      const int Hh = FL.F0 - fc.h + 1, Ww = FL.F1 - fc.w + 1;
+      if (Hh < 1 || Ww < 1) {
+        err = "filter larger than a valid feature level";
+        return GEYE_ERR_ARG;
+      }
+      for (int g = 0; g < ngroups; g++) {
+        const int slot0 = (int)P.map_off.size();
+        for (int n = 0; n < fc.nf; n++) {
+          const int fid = fc.members[g * fc.nf + n];
+          P.map_off.push_back(co);
+          P.map_dims.push_back(int2{Hh, Ww});
+          co += (long long)Hh * Ww;
+          if (fid >= 0) {
+            if (fc.root) P.root_slot[(size_t)fl * m.NoR + fid] = slot0 + n;
+            else P.part_slot[(size_t)fl * m.NoP + fid] = slot0 + n;
+          }
+        }
+        for (int x0 = 0; x0 < Ww; x0 += 16)
+          for (int y0 = 0; y0 < Hh; y0 += 64) {
+            ConvTile t;
+            t.feat_off = P.feat_off[fl];
+            t.F0 = FL.F0;
+            t.F1 = FL.F1;
+            t.H = Hh;
+            t.W = Ww;
+            t.x0 = x0;
+            t.y0 = y0;
+            t.filt_base = (int)(fc.off / (31LL * fc.w * fc.h * fc.nf)) + g;
+            t.out_slot = slot0;
+            P.conv_tiles.push_back(t);
+          }
+      }
+    }
+    cc.tile_count = (int)P.conv_tiles.size() - cc.tile_begin;
+    P.conv_classes.push_back(cc);
+  }
+  P.conv_total = co;
+  // ---- DT jobs and score maps
+  P.dt_pass1.clear();
+  P.dt_pass2.clear();
+  P.dt_nodes.clear();
+  P.dt_depths.clear();
+  P.dt_len_tab.clear();
+  P.smaps.clear();
+  P.sparts.clear();
+  long long dto = 0, so = 0;
+  const int MP = m.maxpart;
+  for (int l = interval; l < LEN; l++) {
+    if (!P.lv[l].valid) continue;
+    const int L = l - interval;
+    for (int jj = 0; jj < m.numcomponent; jj++) {
+      if (jj >= m.NoR) {
+        err = "component without a root filter";
+        return GEYE_ERR_ARG;
+      }
+      const long long rslot = P.root_slot[(size_t)l * m.NoR + jj];
+      ScoreMap sm{};
+      sm.root_off = P.map_off[rslot];
+      sm.R0 = P.map_dims[rslot].x;
+      sm.R1 = P.map_dims[rslot].y;
+      sm.score_off = so;
+      sm.level = l;
+      sm.comp = jj;
+      sm.npart = m.numpart[jj];
+      sm.part_first = (int)P.sparts.size();
+      sm.off = m.offw[m.ridx[jj]];
+      sm.scale = (float)m.sbin / P.lv[l].scale;
+      so += (long long)sm.R0 * sm.R1;
+      for (int kk = 0; kk < m.numpart[jj]; kk++) {
+        const int pid = m.pidx[jj * MP + kk], did = m.didx[jj * MP + kk];
+        const long long pslot = P.part_slot[(size_t)L * m.NoP + pid];
+        const int P0 = P.map_dims[pslot].x, P1 = P.map_dims[pslot].y;
+        ScorePart q{};
+        q.M_off = dto;
+        q.Ix_off = dto;
+        q.Iy_off = dto;
+        q.P0 = P0;
+        q.P1 = P1;
+        q.ax = m.anchor[did * 2] + 1;
+        q.ay = m.anchor[did * 2 + 1] + 1;
+        q.ps0 = m.psize[(jj * MP + kk) * 2];
+        q.ps1 = m.psize[(jj * MP + kk) * 2 + 1];
+        if (q.ay < 2 || q.ax < 2 || q.ay - 1 + 2 * (sm.R0 - 1) >= P0 ||
+            q.ax - 1 + 2 * (sm.R1 - 1) >= P1) {
+          err = "part anchor outside the part score map";
+          return GEYE_ERR_ARG;
+        }
+        const float *df = &m.def[did * 4];
+        if (dt_pick_G(P0) < 0 || dt_pick_G(P1) < 0) {
+          err = "part score map too large for the DT kernel";
+          return GEYE_ERR_UNSUPPORTED;
+        }
+        // pass 1: columns (along y, n = P0) with (ay, by); source negated (inverse_Q)
+        add_dt_jobs(P, P.dt_pass1, P.map_off[pslot], dto, P0, P1, P0, 1, 1, df[2], df[3]);
+        // pass 2: rows (along x, n = P1) with (ax, bx)
+        add_dt_jobs(P, P.dt_pass2, dto, dto, P1, P0, 1, P0, 0, df[0], df[1]);
+        P.sparts.push_back(q);
+        dto += (long long)P0 * P1;
+      }
+      P.smaps.push_back(sm);
+    }
+  }
+  P.dt_total = dto;
+  P.score_total = so;
+  P.dt_lds1 = 0;
+  P.dt_lds2 = 0;
+  for (auto &j : P.dt_pass1) P.dt_lds1 = std::max(P.dt_lds1, dt_lds_bytes(j.n, j.G));
+  for (auto &j : P.dt_pass2) P.dt_lds2 = std::max(P.dt_lds2, dt_lds_bytes(j.n, j.G));
+  P.GL = m.numcomponent > 0 ? (m.numpart[0] + 1) * 4 + 3 : 0;
+  for (int jj = 1; jj < m.numcomponent; jj++)
+    if (m.numpart[jj] != m.numpart[0]) {
+      err = "components with different part counts are not supported (row width)";
+      return GEYE_ERR_UNSUPPORTED;
+    }
+  return GEYE_OK;
+}
+
+int dt_single_jobs(int n0, int n1, float ax, float bx, float ay, float by,
+                   std::vector<DtJob> &p1, std::vector<DtJob> &p2, std::vector<int4> &nodes,
+                   std::vector<int> &depths, size_t &lds1, size_t &lds2) {
+  if (dt_pick_G(n0) < 0 || dt_pick_G(n1) < 0) return GEYE_ERR_UNSUPPORTED;
+  Plan P;
+  add_dt_jobs(P, P.dt_pass1, 0, 0, n0, n1, n0, 1, 0, ay, by);
+  add_dt_jobs(P, P.dt_pass2, 0, 0, n1, n0, 1, n0, 0, ax, bx);
+  p1 = P.dt_pass1;
+  p2 = P.dt_pass2;
+  nodes = P.dt_nodes;
+  depths = P.dt_depths;
+  lds1 = dt_lds_bytes(n0, dt_pick_G(n0));
+  lds2 = dt_lds_bytes(n1, dt_pick_G(n1));
+  return GEYE_OK;
+}
+
+}  // namespace geye

@@ -1,0 +1,1151 @@
+// dpm_runtime.cpp -- context, model residency, the per-frame pipeline and the C ABI.
+//
+// Replaces the reference's host orchestration: init_cuda/clean_cuda (CT/GPU_init.cpp:50-356),
+// load_model (CT/load_model.cpp:541), calc_f_pyramid (CT/featurepyramid.cpp.new2:952),
+// fconvsMT_GPU (CT/fconvsMT.cpp:310), dt_GPU (CT/dt_GPU.cpp:763) and get_boxes
+// (CT/get_boxes.cpp:583).  Unlike the reference, filters, plans and every intermediate stay
+// resident in HBM: a frame crosses PCIe once (in) and only its boxes come back.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "dpm_kernels.h"
+#include "dpm_runtime.h"
+
+using namespace geye;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(call)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(GEYE_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+#define CHK(expr)            \
+  do {                       \
+    int r_ = (expr);         \
+    if (r_ != GEYE_OK) return r_; \
+  } while (0)
+
+int ensure(DeviceBuffer &b, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  if (b.p && b.bytes >= bytes) return GEYE_OK;
+  if (b.p) {
+    HIPCHK(hipFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return fail(GEYE_ERR_NOMEM, std::string("hipMalloc(") + std::to_string(bytes) +
+                                    "): " + hipGetErrorString(e));
+  }
+  b.bytes = bytes;
+  return GEYE_OK;
+}
+
+void release(DeviceBuffer &b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+template <typename T>
+int upload(DeviceBuffer &b, const std::vector<T> &v) {
+  CHK(ensure(b, sizeof(T) * v.size()));
+  if (!v.empty()) HIPCHK(hipMemcpy(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return GEYE_OK;
+}
+
+template <typename T>
+T *dp(DeviceBuffer &b) {
+  return (T *)b.p;
+}
+
+// ---------------------------------------------------------------------------------------
+// Per-kernel timing: events around launches on the context's stream.
+// ---------------------------------------------------------------------------------------
+struct KTimer {
+  geye_ctx *c;
+  std::string name;
+  hipEvent_t a = nullptr, b = nullptr;
+  KTimer(geye_ctx *ctx, const char *n) : c(ctx), name(n) {
+    if (!c->profile) return;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, c->stream);
+  }
+  ~KTimer() {
+    if (!c->profile) return;
+    (void)hipEventRecord(b, c->stream);
+    c->pending.push_back({name, {a, b}});
+  }
+};
+
+void flush_stats(geye_ctx *c) {
+  for (auto &p : c->pending) {
+    float ms = 0;
+    (void)hipEventSynchronize(p.second.second);
+    (void)hipEventElapsedTime(&ms, p.second.first, p.second.second);
+    auto &st = c->stats[p.first];
+    st.ms += ms;
+    st.count += 1;
+    (void)hipEventDestroy(p.second.first);
+    (void)hipEventDestroy(p.second.second);
+  }
+  c->pending.clear();
+}
+
+// ---------------------------------------------------------------------------------------
+// Model
+// ---------------------------------------------------------------------------------------
+int pick_nf(int count) {
+  if (count >= 8) return 8;
+  if (count >= 4) return 4;
+  if (count >= 2) return 2;
+  return 1;
+}
+
+int model_from_desc(geye_ctx *ctx, const geye_model_desc *d, Model &m) {
+  if (!d || d->numcomponent <= 0 || d->NoR <= 0 || d->sbin <= 0 || d->interval <= 0)
+    return fail(GEYE_ERR_ARG, "model: empty or malformed description");
+  m.ctx = ctx;
+  m.numcomponent = d->numcomponent;
+  m.sbin = d->sbin;
+  m.interval = d->interval;
+  m.max_Y = d->max_Y;
+  m.max_X = d->max_X;
+  m.padx = d->padx;
+  m.pady = d->pady;
+  m.maxpart = std::max(d->maxpart, 1);
+  m.NoR = d->NoR;
+  m.NoP = d->NoP;
+  m.DefL = d->DefL;
+  const int C = m.numcomponent, MP = m.maxpart;
+  m.ridx.assign(d->ridx, d->ridx + C);
+  m.oidx.assign(d->oidx, d->oidx + C);
+  m.offw.assign(d->offw, d->offw + C);
+  m.rsize.assign(d->rsize, d->rsize + 2 * C);
+  m.numpart.assign(d->numpart, d->numpart + C);
+  m.pidx.assign(C * MP, 0);
+  m.didx.assign(C * MP, 0);
+  m.psize.assign(C * MP * 2, 0);
+  for (int i = 0; i < C; i++) {
+    if (m.numpart[i] > MP || m.numpart[i] < 0) return fail(GEYE_ERR_ARG, "model: numpart > maxpart");
+    if (m.ridx[i] < 0 || m.ridx[i] >= C) return fail(GEYE_ERR_ARG, "model: ridx out of range");
+    for (int k = 0; k < m.numpart[i]; k++) {
+      m.pidx[i * MP + k] = d->pidx[i * d->maxpart + k];
+      m.didx[i * MP + k] = d->didx[i * d->maxpart + k];
+      m.psize[(i * MP + k) * 2] = d->psize[(i * d->maxpart + k) * 2];
+      m.psize[(i * MP + k) * 2 + 1] = d->psize[(i * d->maxpart + k) * 2 + 1];
+      if (m.pidx[i * MP + k] < 0 || m.pidx[i * MP + k] >= m.NoP)
+        return fail(GEYE_ERR_ARG, "model: pidx out of range");
+      if (m.didx[i * MP + k] < 0 || m.didx[i * MP + k] >= m.DefL)
+        return fail(GEYE_ERR_ARG, "model: didx out of range");
+    }
+  }
+  // each part filter may be used by one (component, part) pair: the DT negates per use
+  std::vector<int> uses(m.NoP, 0);
+  for (int i = 0; i < C; i++)
+    for (int k = 0; k < m.numpart[i]; k++)
+      if (++uses[m.pidx[i * MP + k]] > 1)
+        return fail(GEYE_ERR_UNSUPPORTED, "model: a part filter shared by two parts");
+  m.def.assign(d->def, d->def + 4 * m.DefL);
+  m.anchor.assign(d->anchor, d->anchor + 2 * m.DefL);
+  m.root_size.assign(d->root_size, d->root_size + 3 * m.NoR);
+  m.part_size.assign(d->part_size, d->part_size + 3 * m.NoP);
+  m.rootf.resize(m.NoR);
+  m.partf.resize(m.NoP);
+  for (int r = 0; r < m.NoR; r++) {
+    const int *s = &m.root_size[3 * r];
+    if (s[2] != 31 || s[0] < 1 || s[1] < 1 || s[0] > 16 || s[1] > 16)
+      return fail(GEYE_ERR_UNSUPPORTED, "model: root filters must be h,w <= 16 with 31 channels");
+    m.rootf[r].assign(d->rootfilter + d->root_off[r], d->rootfilter + d->root_off[r] + 31LL * s[0] * s[1]);
+  }
+  for (int p = 0; p < m.NoP; p++) {
+    const int *s = &m.part_size[3 * p];
+    if (s[2] != 31 || s[0] < 1 || s[1] < 1 || s[0] > 16 || s[1] > 16)
+      return fail(GEYE_ERR_UNSUPPORTED, "model: part filters must be h,w <= 16 with 31 channels");
+    m.partf[p].assign(d->partfilter + d->part_off[p], d->partfilter + d->part_off[p] + 31LL * s[0] * s[1]);
+  }
+  for (int i = 0; i < C; i++)
+    if (m.rsize[2 * i] != m.root_size[3 * i] || m.rsize[2 * i + 1] != m.root_size[3 * i + 1])
+      return fail(GEYE_ERR_ARG, "model: rsize disagrees with the root filter size");
+  // shape classes and packed filters [group][31][w][h][NF]
+  m.classes.clear();
+  std::vector<float> packed;
+  for (int pass = 0; pass < 2; pass++) {
+    const bool root = pass == 0;
+    const int nfil = root ? m.NoR : m.NoP;
+    const std::vector<int> &sz = root ? m.root_size : m.part_size;
+    std::vector<bool> done(nfil, false);
+    for (int i = 0; i < nfil; i++) {
+      if (done[i]) continue;
+      Model::FilterClass fc;
+      fc.h = sz[3 * i];
+      fc.w = sz[3 * i + 1];
+      fc.root = root;
+      for (int j = i; j < nfil; j++)
+        if (!done[j] && sz[3 * j] == fc.h && sz[3 * j + 1] == fc.w) {
+          fc.members.push_back(j);
+          done[j] = true;
+        }
+      fc.nf = pick_nf((int)fc.members.size());
+      while (fc.members.size() % fc.nf) fc.members.push_back(-1);
+      fc.off = (long long)packed.size();
+      const int ng = (int)fc.members.size() / fc.nf;
+      for (int g = 0; g < ng; g++)
+        for (int f = 0; f < 31; f++)
+          for (int xp = 0; xp < fc.w; xp++)
+            for (int yp = 0; yp < fc.h; yp++)
+              for (int n = 0; n < fc.nf; n++) {
+                const int fid = fc.members[g * fc.nf + n];
+                const std::vector<float> &src = root ? m.rootf[std::max(fid, 0)] : m.partf[std::max(fid, 0)];
+                packed.push_back(fid >= 0 ? src[(size_t)f * fc.h * fc.w + (size_t)xp * fc.h + yp] : 0.0f);
+              }
+      m.classes.push_back(fc);
+    }
+  }
+  CHK(upload(m.d_filters, packed));
+  CHK(upload(m.d_rsize, m.rsize));
+  return GEYE_OK;
+}
+
+bool read_csv(const char *path, std::vector<float> &v) {
+  FILE *f = std::fopen(path, "r");
+  if (!f) return false;
+  std::string s;
+  char buf[1 << 16];
+  size_t n;
+  while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) s.append(buf, n);
+  std::fclose(f);
+  const char *p = s.c_str();
+  while (*p) {
+    while (*p && (*p == ',' || *p == ' ' || *p == '\n' || *p == '\r' || *p == '\t')) p++;
+    if (!*p) break;
+    char *e;
+    float x = std::strtof(p, &e);
+    if (e == p) return false;
+    v.push_back(x);
+    p = e;
+  }
+  return true;
+}
+
+// CSV model format of CT/load_model.cpp:57-558; the pads are zero as load_model forces
+// (CT/load_model.cpp:552-553).
+int load_csv(geye_ctx *ctx, const char *comp, const char *root, const char *part, Model &m) {
+  std::vector<float> c, r, p;
+  if (!read_csv(comp, c)) return fail(GEYE_ERR_IO, std::string("cannot read ") + comp);
+  if (!read_csv(root, r)) return fail(GEYE_ERR_IO, std::string("cannot read ") + root);
+  if (!read_csv(part, p)) return fail(GEYE_ERR_IO, std::string("cannot read ") + part);
+  size_t k = 0;
+  bool bad = false;
+  auto next = [&](const std::vector<float> &v) -> float {
+    if (k >= v.size()) { bad = true; return 0.f; }
+    return v[k++];
+  };
+  geye_model_desc d{};
+  d.numcomponent = (int)next(c);
+  d.sbin = (int)next(c);
+  d.interval = (int)next(c);
+  d.max_Y = (int)next(c);
+  d.max_X = (int)next(c);
+  const int C = d.numcomponent;
+  if (bad || C <= 0 || C > 1024) return fail(GEYE_ERR_IO, "component file malformed");
+  std::vector<int> ridx(C), oidx(C), rsize(2 * C), numpart(C);
+  std::vector<float> offw(C);
+  std::vector<std::vector<int>> pi(C), di(C), ps(C);
+  int MP = 1;
+  for (int i = 0; i < C && !bad; i++) {
+    ridx[i] = (int)next(c) - 1;
+    oidx[i] = (int)next(c) - 1;
+    offw[i] = next(c);
+    rsize[2 * i] = (int)next(c);
+    rsize[2 * i + 1] = (int)next(c);
+    numpart[i] = (int)next(c);
+    if (numpart[i] < 0 || numpart[i] > 4096) { bad = true; break; }
+    MP = std::max(MP, numpart[i]);
+    for (int j = 0; j < numpart[i]; j++) {
+      pi[i].push_back((int)next(c) - 1);
+      di[i].push_back((int)next(c) - 1);
+      ps[i].push_back((int)next(c));
+      ps[i].push_back((int)next(c));
+    }
+  }
+  const int DefL = (int)next(c);
+  if (bad || DefL < 0 || DefL > 1 << 20) return fail(GEYE_ERR_IO, "component file malformed");
+  std::vector<float> def(4 * DefL);
+  std::vector<int> anchor(2 * DefL);
+  for (int q = 0; q < DefL; q++) {
+    for (int t = 0; t < 4; t++) def[4 * q + t] = next(c);
+    anchor[2 * q] = (int)next(c);
+    anchor[2 * q + 1] = (int)next(c);
+  }
+  if (bad) return fail(GEYE_ERR_IO, "component file truncated");
+  std::vector<int> pidx(C * MP, 0), didx(C * MP, 0), psize(C * MP * 2, 0);
+  for (int i = 0; i < C; i++)
+    for (int j = 0; j < numpart[i]; j++) {
+      pidx[i * MP + j] = pi[i][j];
+      didx[i * MP + j] = di[i][j];
+      psize[(i * MP + j) * 2] = ps[i][2 * j];
+      psize[(i * MP + j) * 2 + 1] = ps[i][2 * j + 1];
+    }
+  auto filters = [&](const std::vector<float> &v, bool partner, int &N, std::vector<int> &size,
+                     std::vector<float> &w, std::vector<long long> &off) -> bool {
+    k = 0;
+    N = (int)next(v);
+    if (bad || N < 0 || N > 1 << 16) return false;
+    for (int i = 0; i < N; i++) {
+      int h = (int)next(v), ww = (int)next(v), ch = (int)next(v);
+      if (bad || h <= 0 || ww <= 0 || ch <= 0) return false;
+      size.insert(size.end(), {h, ww, ch});
+      off.push_back((long long)w.size());
+      for (long long q = 0; q < (long long)h * ww * ch; q++) w.push_back(next(v));
+      if (partner) (void)next(v);
+      if (bad) return false;
+    }
+    return true;
+  };
+  int NoR = 0, NoP = 0;
+  std::vector<int> rs, pss;
+  std::vector<float> rw, pw;
+  std::vector<long long> ro, po;
+  if (!filters(r, false, NoR, rs, rw, ro)) return fail(GEYE_ERR_IO, "root filter file malformed");
+  if (!filters(p, true, NoP, pss, pw, po)) return fail(GEYE_ERR_IO, "part filter file malformed");
+  d.padx = 0;
+  d.pady = 0;
+  d.maxpart = MP;
+  d.ridx = ridx.data();
+  d.oidx = oidx.data();
+  d.offw = offw.data();
+  d.rsize = rsize.data();
+  d.numpart = numpart.data();
+  d.pidx = pidx.data();
+  d.didx = didx.data();
+  d.psize = psize.data();
+  d.DefL = DefL;
+  d.def = def.data();
+  d.anchor = anchor.data();
+  d.NoR = NoR;
+  d.root_size = rs.data();
+  d.rootfilter = rw.data();
+  d.root_off = ro.data();
+  d.NoP = NoP;
+  d.part_size = pss.data();
+  d.partfilter = pw.data();
+  d.part_off = po.data();
+  return model_from_desc(ctx, &d, m);
+}
+
+// ---------------------------------------------------------------------------------------
+// Plans: built once per frame size and kept with the model.
+// ---------------------------------------------------------------------------------------
+int plan_upload(Plan &P) {
+  std::vector<ResizeOp> ops;
+  for (auto &R : P.rounds) {
+    ops.insert(ops.end(), R.rows.begin(), R.rows.end());
+    ops.insert(ops.end(), R.cols.begin(), R.cols.end());
+  }
+  CHK(upload(P.d_resize_ops, ops));
+  CHK(upload(P.d_resize_tab, P.resize_tab));
+  CHK(upload(P.d_resize_si, P.resize_si));
+  CHK(upload(P.d_resize_alpha, P.resize_alpha));
+  CHK(upload(P.d_hog, P.hog));
+  CHK(upload(P.d_conv_tiles, P.conv_tiles));
+  CHK(upload(P.d_map_off, P.map_off));
+  CHK(upload(P.d_dt1, P.dt_pass1));
+  CHK(upload(P.d_dt2, P.dt_pass2));
+  CHK(upload(P.d_nodes, P.dt_nodes));
+  CHK(upload(P.d_depths, P.dt_depths));
+  CHK(upload(P.d_smaps, P.smaps));
+  CHK(upload(P.d_sparts, P.sparts));
+  long long tmp = 0;
+  for (auto &R : P.rounds) tmp = std::max(tmp, R.tmp_total);
+  CHK(ensure(P.d_img, sizeof(float) * P.img_total));
+  CHK(ensure(P.d_tmp, sizeof(float) * tmp));
+  CHK(ensure(P.d_hist, sizeof(float) * P.hist_total));
+  CHK(ensure(P.d_norm, sizeof(float) * P.norm_total));
+  CHK(ensure(P.d_feat, sizeof(float) * P.feat_total));
+  CHK(ensure(P.d_conv, sizeof(float) * P.conv_total));
+  CHK(ensure(P.d_tmpM, sizeof(float) * P.dt_total));
+  CHK(ensure(P.d_tmpIy, sizeof(int) * P.dt_total));
+  CHK(ensure(P.d_M, sizeof(float) * P.dt_total));
+  CHK(ensure(P.d_tmpIx, sizeof(int) * P.dt_total));
+  CHK(ensure(P.d_score, sizeof(float) * P.score_total));
+  CHK(ensure(P.d_flags, sizeof(int) * P.score_total));
+  CHK(ensure(P.d_blocksum, sizeof(int) * (compact_blocks(P.score_total) + 1)));
+  CHK(ensure(P.d_cand, sizeof(int) * P.score_total));
+  CHK(ensure(P.d_count, sizeof(int) * 4));
+  return GEYE_OK;
+}
+
+void plan_release(Plan &P) {
+  for (DeviceBuffer *b : {&P.d_resize_ops, &P.d_resize_tab, &P.d_resize_si, &P.d_resize_alpha,
+                          &P.d_hog, &P.d_conv_tiles, &P.d_map_off, &P.d_dt1, &P.d_dt2,
+                          &P.d_nodes, &P.d_depths, &P.d_smaps, &P.d_sparts, &P.d_frame,
+                          &P.d_img, &P.d_tmp, &P.d_hist, &P.d_norm, &P.d_feat, &P.d_featb,
+                          &P.d_conv, &P.d_tmpM, &P.d_tmpIy, &P.d_M, &P.d_tmpIx, &P.d_score,
+                          &P.d_flags, &P.d_blocksum, &P.d_cand, &P.d_count, &P.d_rows})
+    release(*b);
+}
+
+int get_plan(Model &m, int W, int H, Plan **out) {
+  const long long key = ((long long)W << 20) | (long long)H;
+  auto it = m.plans.find(key);
+  if (it != m.plans.end()) {
+    *out = it->second;
+    return GEYE_OK;
+  }
+  std::unique_ptr<Plan> P(new Plan());
+  std::string err;
+  int rc = build_plan(m, W, H, *P, err);
+  if (rc != GEYE_OK) return fail(rc, err);
+  rc = plan_upload(*P);
+  if (rc != GEYE_OK) {
+    plan_release(*P);
+    return rc;
+  }
+  *out = P.release();
+  m.plans[key] = *out;
+  return GEYE_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// Pipeline stages
+// ---------------------------------------------------------------------------------------
+int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, bool want_bf16) {
+  hipStream_t s = c->stream;
+  {
+    KTimer t(c, "ingest");
+    launch_ingest(s, d_bgr, P.W, P.H, ws, dp<float>(P.d_img) + P.img_off[0]);
+  }
+  int op_base = 0;
+  for (auto &R : P.rounds) {
+    const ResizeOp *ops = dp<ResizeOp>(P.d_resize_ops) + op_base;
+    const int nr = (int)R.rows.size(), nc = (int)R.cols.size();
+    {
+      KTimer t(c, "resize_rows");
+      launch_resize_rows(s, ops, nr, R.rows_blocks, dp<int2>(P.d_resize_tab),
+                         dp<int>(P.d_resize_si), dp<float>(P.d_resize_alpha), dp<float>(P.d_img),
+                         dp<float>(P.d_tmp));
+    }
+    {
+      KTimer t(c, "resize_cols");
+      launch_resize_cols(s, ops + nr, nc, R.cols_blocks, dp<int2>(P.d_resize_tab),
+                         dp<int>(P.d_resize_si), dp<float>(P.d_resize_alpha), dp<float>(P.d_tmp),
+                         dp<float>(P.d_img));
+    }
+    op_base += nr + nc;
+  }
+  const HogLevel *hog = dp<HogLevel>(P.d_hog);
+  if (P.hog16_n > 0) {
+    KTimer t(c, "hog_hist");
+    launch_hog_hist(s, 16, hog, P.hog16_n, P.hog_hist_blocks, dp<float>(P.d_img),
+                    dp<float>(P.d_hist), dp<float>(P.d_norm));
+  }
+  if (P.hog8_n > 0) {
+    KTimer t(c, "hog_hist");
+    launch_hog_hist(s, 8, hog + P.hog16_n, P.hog8_n, P.hog8_blocks, dp<float>(P.d_img),
+                    dp<float>(P.d_hist), dp<float>(P.d_norm));
+  }
+  if (want_bf16) CHK(ensure(P.d_featb, 2 * P.featb_total));
+  if (P.hogf_n > 0) {
+    KTimer t(c, "hog_feat");
+    launch_hog_feat(s, hog + P.hog16_n + P.hog8_n, P.hogf_n, P.hog_feat_blocks,
+                    dp<float>(P.d_hist), dp<float>(P.d_norm), dp<float>(P.d_feat),
+                    want_bf16 ? P.d_featb.p : nullptr);
+  }
+  HIPCHK(hipGetLastError());
+  return GEYE_OK;
+}
+
+int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int *ncand) {
+  hipStream_t s = c->stream;
+  if (precision != GEYE_FP32) return fail(GEYE_ERR_UNSUPPORTED, "precision not available");
+  for (auto &cc : P.conv_classes) {
+    if (cc.tile_count == 0) continue;
+    KTimer t(c, "conv_f32");
+    launch_conv_f32(s, cc.nf, dp<ConvTile>(P.d_conv_tiles) + cc.tile_begin, cc.tile_count, cc.h,
+                    cc.w, dp<float>(P.d_feat), dp<float>(m.d_filters), dp<long long>(P.d_map_off),
+                    dp<float>(P.d_conv));
+  }
+  if (!P.dt_pass1.empty()) {
+    KTimer t(c, "dt_cols");
+    launch_dt(s, dp<DtJob>(P.d_dt1), (int)P.dt_pass1.size(), P.dt_lds1, dp<int4>(P.d_nodes),
+              dp<int>(P.d_depths), dp<float>(P.d_conv), dp<float>(P.d_tmpM), dp<int>(P.d_tmpIy));
+  }
+  if (!P.dt_pass2.empty()) {
+    KTimer t(c, "dt_rows");
+    launch_dt(s, dp<DtJob>(P.d_dt2), (int)P.dt_pass2.size(), P.dt_lds2, dp<int4>(P.d_nodes),
+              dp<int>(P.d_depths), dp<float>(P.d_tmpM), dp<float>(P.d_M), dp<int>(P.d_tmpIx));
+  }
+  *ncand = 0;
+  if (P.score_total > 0) {
+    {
+      KTimer t(c, "score");
+      launch_score(s, dp<ScoreMap>(P.d_smaps), (int)P.smaps.size(), dp<ScorePart>(P.d_sparts),
+                   dp<float>(P.d_conv), dp<float>(P.d_M), thresh, dp<float>(P.d_score),
+                   dp<int>(P.d_flags), P.score_total);
+    }
+    {
+      KTimer t(c, "compact");
+      launch_compact(s, dp<int>(P.d_flags), P.score_total, dp<int>(P.d_blocksum),
+                     dp<int>(P.d_count), dp<int>(P.d_cand));
+    }
+    HIPCHK(hipGetLastError());
+    int n = 0;
+    HIPCHK(hipMemcpyAsync(&n, P.d_count.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *ncand = n;
+  }
+  HIPCHK(hipGetLastError());
+  return GEYE_OK;
+}
+
+int run_rows(geye_ctx *c, Model &m, Plan &P, int ncand, std::vector<float> &rows) {
+  rows.assign((size_t)ncand * P.GL, 0.0f);
+  if (ncand == 0) return GEYE_OK;
+  CHK(ensure(P.d_rows, sizeof(float) * (size_t)ncand * P.GL));
+  {
+    KTimer t(c, "rows");
+    launch_rows(c->stream, dp<ScoreMap>(P.d_smaps), (int)P.smaps.size(), dp<ScorePart>(P.d_sparts),
+                dp<float>(P.d_score), dp<int>(P.d_cand), ncand, dp<int>(P.d_tmpIx),
+                dp<int>(P.d_tmpIy), P.GL, m.padx, m.pady, dp<int>(m.d_rsize), dp<float>(P.d_rows));
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(rows.data(), P.d_rows.p, sizeof(float) * rows.size(),
+                        hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GEYE_OK;
+}
+
+int frame_in(geye_ctx *c, Plan &P, const unsigned char *bgr, int W, int H, int ws,
+             const unsigned char **d_bgr) {
+  if (!bgr || ws < 3 * W) return fail(GEYE_ERR_ARG, "frame: null pointer or widthStep < 3*width");
+  CHK(ensure(P.d_frame, (size_t)ws * H));
+  HIPCHK(hipMemcpyAsync(P.d_frame.p, bgr, (size_t)ws * H, hipMemcpyHostToDevice, c->stream));
+  *d_bgr = (const unsigned char *)P.d_frame.p;
+  return GEYE_OK;
+}
+
+float *to_malloc(const std::vector<float> &v) {
+  float *p = (float *)std::malloc(sizeof(float) * std::max<size_t>(v.size(), 1));
+  if (p && !v.empty()) std::memcpy(p, v.data(), sizeof(float) * v.size());
+  return p;
+}
+
+int detect_common(geye_ctx *c, const geye_model *model, const unsigned char *bgr, bool dev,
+                  int W, int H, int ws, float thresh, float overlap, int precision, bool do_nms,
+                  float **boxes, int *num, int *GL) {
+  if (!c || !model || !boxes || !num || !GL) return fail(GEYE_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(hipSetDevice(c->device));
+  Model &m = const_cast<Model &>(model->m);
+  Plan *P = nullptr;
+  CHK(get_plan(m, W, H, &P));
+  const unsigned char *d_bgr = bgr;
+  if (!dev) CHK(frame_in(c, *P, bgr, W, H, ws, &d_bgr));
+  CHK(run_pyramid(c, *P, d_bgr, ws, precision != GEYE_FP32));
+  int ncand = 0;
+  CHK(run_scores(c, m, *P, thresh, precision, &ncand));
+  std::vector<float> rows;
+  CHK(run_rows(c, m, *P, ncand, rows));
+  c->last_plan = P;
+  if (do_nms) {
+    std::vector<float> out(rows.size());
+    int k = nms_host(rows.data(), ncand, P->GL, overlap, out.data());
+    out.resize((size_t)k * P->GL);
+    *boxes = to_malloc(out);
+    *num = k;
+  } else {
+    *boxes = to_malloc(rows);
+    *num = ncand;
+  }
+  *GL = P->GL;
+  flush_stats(c);
+  return *boxes ? GEYE_OK : fail(GEYE_ERR_NOMEM, "malloc");
+}
+
+// single-level plan pieces used by the stage entry points
+struct Scratch {
+  std::vector<DeviceBuffer> bufs;
+  ~Scratch() {
+    for (auto &b : bufs) release(b);
+  }
+  DeviceBuffer &get() {
+    bufs.emplace_back();
+    return bufs.back();
+  }
+};
+
+}  // namespace
+
+// =======================================================================================
+// Host NMS (CT/nms.cpp:70-217), exact: the reference's middle-pivot quicksort, the
+// min-area overlap, the `while (cnum > 0)` loop that never emits the last survivor, and the
+// output in original candidate order.
+// =======================================================================================
+namespace geye {
+
+void quicksort_host(float *ary, int *order, int first, int last) {
+  while (true) {
+    int i = first, j = last;
+    const float key = ary[(first + last) / 2];
+    while (true) {
+      while (ary[i] > key) i++;
+      while (ary[j] < key) j--;
+      if (i >= j) break;
+      std::swap(ary[i], ary[j]);
+      std::swap(order[i], order[j]);
+      i++;
+      j--;
+    }
+    // recurse into the smaller side first to bound the stack; the order of the two
+    // recursive calls does not change the result (they touch disjoint ranges)
+    const bool left = first < i - 1, right = last > j + 1;
+    if (left && right) {
+      if ((i - 1 - first) < (last - j - 1)) {
+        quicksort_host(ary, order, first, i - 1);
+        first = j + 1;
+      } else {
+        quicksort_host(ary, order, j + 1, last);
+        last = i - 1;
+      }
+    } else if (left) {
+      last = i - 1;
+    } else if (right) {
+      first = j + 1;
+    } else {
+      return;
+    }
+  }
+}
+
+int nms_host(const float *boxes, int NUM, int GL, float overlap, float *out) {
+  if (NUM <= 0) return 0;
+  std::vector<float> area(NUM), scores(NUM);
+  std::vector<int> order(NUM), checked(NUM, 0);
+  for (int i = 0; i < NUM; i++) {
+    const float *P = boxes + (size_t)GL * i;
+    area[i] = (float)(((double)(P[3] - P[1]) + 1.0) * ((double)(P[2] - P[0]) + 1.0));
+    scores[i] = P[GL - 2];
+    order[i] = i;
+  }
+  quicksort_host(scores.data(), order.data(), 0, NUM - 1);
+  int cnum = NUM - 1, cur = 0, picked = 0;
+  while (cnum > 0) {
+    const int A = order[cur];
+    const float *P = boxes + (size_t)GL * A;
+    const float Ay1 = P[0], Ax1 = P[1], Ay2 = P[2], Ax2 = P[3];
+    checked[A] = 1;
+    cnum--;
+    cur++;
+    picked++;
+    for (int kk = cur; kk < NUM; kk++) {
+      const int B = order[kk];
+      if (checked[B] != 0) continue;
+      P = boxes + (size_t)GL * B;
+      const float yy1 = Ay1 >= P[0] ? Ay1 : P[0];
+      const float xx1 = Ax1 >= P[1] ? Ax1 : P[1];
+      const float yy2 = Ay2 <= P[2] ? Ay2 : P[2];
+      const float xx2 = Ax2 <= P[3] ? Ax2 : P[3];
+      const float w = (float)((double)(xx2 - xx1) + 1.0);
+      const float h = (float)((double)(yy2 - yy1) + 1.0);
+      const float R = area[A] <= area[B] ? area[A] : area[B];
+      if (w > 0 && h > 0) {
+        if (w * h / R > overlap) {
+          checked[B] = -1;
+          cnum--;
+        }
+      } else if ((Ay1 < P[0] && Ax1 < P[1] && Ay2 > P[2] && Ax2 > P[3]) ||
+                 (Ay1 > P[0] && Ax1 > P[1] && Ay2 < P[2] && Ax2 < P[3])) {
+        checked[B] = -1;
+        cnum--;
+      }
+    }
+    for (int kk = cur; kk < NUM; kk++)
+      if (checked[order[kk]] == 0) {
+        cur = kk;
+        break;
+      }
+  }
+  int count = 0;
+  for (int ii = 0; count < picked; ii++)
+    if (checked[ii] == 1) {
+      std::memcpy(out + (size_t)count * GL, boxes + (size_t)GL * ii, sizeof(float) * GL);
+      count++;
+    }
+  return picked;
+}
+
+}  // namespace geye
+
+// =======================================================================================
+// C ABI
+// =======================================================================================
+extern "C" {
+
+const char *geye_last_error(void) { return g_err.c_str(); }
+
+int geye_device_count(int *n) {
+  if (!n) return fail(GEYE_ERR_ARG, "null");
+  HIPCHK(hipGetDeviceCount(n));
+  return GEYE_OK;
+}
+
+int geye_init(int device, geye_ctx **ctx) {
+  if (!ctx) return fail(GEYE_ERR_ARG, "null ctx");
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(GEYE_ERR_ARG, "no such device");
+  HIPCHK(hipSetDevice(device));
+  geye_ctx *c = new geye_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(GEYE_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  *ctx = c;
+  return GEYE_OK;
+}
+
+void geye_shutdown(geye_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  flush_stats(c);
+  for (auto &b : c->scratch_bufs) release(b);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int geye_model_create(geye_ctx *c, const geye_model_desc *desc, geye_model **model) {
+  if (!c || !desc || !model) return fail(GEYE_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(hipSetDevice(c->device));
+  geye_model *gm = new geye_model();
+  int rc = model_from_desc(c, desc, gm->m);
+  if (rc != GEYE_OK) {
+    geye_model_free(gm);
+    return rc;
+  }
+  *model = gm;
+  return GEYE_OK;
+}
+
+int geye_model_load_csv(geye_ctx *c, const char *comp, const char *root, const char *part,
+                        geye_model **model) {
+  if (!c || !comp || !root || !part || !model) return fail(GEYE_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(hipSetDevice(c->device));
+  geye_model *gm = new geye_model();
+  int rc = load_csv(c, comp, root, part, gm->m);
+  if (rc != GEYE_OK) {
+    geye_model_free(gm);
+    return rc;
+  }
+  *model = gm;
+  return GEYE_OK;
+}
+
+void geye_model_free(geye_model *gm) {
+  if (!gm) return;
+  Model &m = gm->m;
+  if (m.ctx && m.ctx->last_plan) {
+    for (auto &kv : m.plans)
+      if (kv.second == m.ctx->last_plan) m.ctx->last_plan = nullptr;
+  }
+  for (auto &kv : m.plans) {
+    plan_release(*kv.second);
+    delete kv.second;
+  }
+  release(m.d_filters);
+  release(m.d_rsize);
+  delete gm;
+}
+
+int geye_model_info(const geye_model *gm, int *numcomponent, int *sbin, int *interval,
+                    int *max_Y, int *max_X, int *NoR, int *NoP) {
+  if (!gm) return fail(GEYE_ERR_ARG, "null model");
+  const Model &m = gm->m;
+  if (numcomponent) *numcomponent = m.numcomponent;
+  if (sbin) *sbin = m.sbin;
+  if (interval) *interval = m.interval;
+  if (max_Y) *max_Y = m.max_Y;
+  if (max_X) *max_X = m.max_X;
+  if (NoR) *NoR = m.NoR;
+  if (NoP) *NoP = m.NoP;
+  return GEYE_OK;
+}
+
+int geye_level_plan(const geye_model *gm, int width, int height, int *max_scale, int *LEN,
+                    float *scales, int *feat_h, int *feat_w) {
+  if (!gm || !max_scale || !LEN) return fail(GEYE_ERR_ARG, "null argument");
+  const Model &m = gm->m;
+  int ms = compute_max_scale(m, width, height);
+  std::vector<Level> lv;
+  std::vector<int> ih, iw, is;
+  std::vector<float> isc;
+  level_geometry(ms, m.sbin, m.interval, width, height, lv, ih, iw, is, isc);
+  *max_scale = ms;
+  *LEN = (int)lv.size();
+  for (size_t l = 0; l < lv.size(); l++) {
+    if (scales) scales[l] = lv[l].scale;
+    if (feat_h) feat_h[l] = lv[l].F0;
+    if (feat_w) feat_w[l] = lv[l].F1;
+  }
+  return GEYE_OK;
+}
+
+int geye_get_boxes(geye_ctx *c, const geye_model *model, const unsigned char *bgr, int width,
+                   int height, int widthStep, float thresh, int precision, float **boxes,
+                   int *num, int *GL) {
+  return detect_common(c, model, bgr, false, width, height, widthStep, thresh, 0.0f, precision,
+                       false, boxes, num, GL);
+}
+
+int geye_detect(geye_ctx *c, const geye_model *model, const unsigned char *bgr, int width,
+                int height, int widthStep, float thresh, float overlap, int precision,
+                float **boxes, int *num, int *GL) {
+  return detect_common(c, model, bgr, false, width, height, widthStep, thresh, overlap,
+                       precision, true, boxes, num, GL);
+}
+
+int geye_detect_dev(geye_ctx *c, const geye_model *model, const void *d_bgr, int width,
+                    int height, int widthStep, float thresh, float overlap, int precision,
+                    float **boxes, int *num, int *GL) {
+  return detect_common(c, model, (const unsigned char *)d_bgr, true, width, height, widthStep,
+                       thresh, overlap, precision, true, boxes, num, GL);
+}
+
+int geye_last_scores(geye_ctx *c, float *out, long long cap, long long *total) {
+  if (!c || !total) return fail(GEYE_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  Plan *P = c->last_plan;
+  if (!P) return fail(GEYE_ERR_ARG, "no detection has run on this context");
+  *total = P->score_total;
+  if (out && cap > 0) {
+    long long n = std::min(cap, P->score_total);
+    HIPCHK(hipMemcpy(out, P->d_score.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+  }
+  return GEYE_OK;
+}
+
+void geye_free(void *p) { std::free(p); }
+
+int geye_nms(geye_ctx *c, const float *boxes, int num, int GL, float overlap, float *out,
+             int *out_num) {
+  (void)c;
+  if (!out_num || (num > 0 && (!boxes || !out)) || GL < 7) return fail(GEYE_ERR_ARG, "bad argument");
+  *out_num = nms_host(boxes, num, GL, overlap, out);
+  return GEYE_OK;
+}
+
+// ---- stage entry points ----------------------------------------------------------------
+
+int geye_resize(geye_ctx *c, const float *src, const int *sdims, float scale, float *dst,
+                int *odims) {
+  if (!c || !src || !sdims || !odims) return fail(GEYE_ERR_ARG, "null argument");
+  const int h = sdims[0], w = sdims[1], ch = sdims[2];
+  if (h < 1 || w < 1 || ch != 3) return fail(GEYE_ERR_ARG, "resize: sdims must be (h, w, 3)");
+  if (scale == 1.0f) {
+    odims[0] = h; odims[1] = w; odims[2] = ch;
+    if (dst) std::memcpy(dst, src, sizeof(float) * 3 * h * w);
+    return GEYE_OK;
+  }
+  const int oh = (int)((double)((float)h * scale) + 0.5);
+  const int ow = (int)((double)((float)w * scale) + 0.5);
+  odims[0] = oh; odims[1] = ow; odims[2] = ch;
+  if (!dst) return GEYE_OK;
+  if (oh < 1 || ow < 1) return fail(GEYE_ERR_ARG, "resize: empty output");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(hipSetDevice(c->device));
+  // column-major [c][x][y] -> row-major [c][y][x]
+  std::vector<float> in((size_t)3 * h * w);
+  for (int cc = 0; cc < 3; cc++)
+    for (int x = 0; x < w; x++)
+      for (int y = 0; y < h; y++) in[((size_t)cc * h + y) * w + x] = src[((size_t)cc * w + x) * h + y];
+  std::vector<int2> tab;
+  std::vector<int> si;
+  std::vector<float> al;
+  resize_tables(h, oh, tab, si, al);
+  const int tab2 = (int)tab.size();
+  resize_tables(w, ow, tab, si, al);
+  ResizeOp ops[2] = {{0, 0, h, w, oh, w, 0, 0}, {0, 3LL * h * w, oh, w, oh, ow, tab2, 0}};
+  Scratch S;
+  DeviceBuffer &dops = S.get(), &dtab = S.get(), &dsi = S.get(), &dal = S.get(),
+               &dimg = S.get(), &dtmp = S.get();
+  CHK(upload(dops, std::vector<ResizeOp>(ops, ops + 2)));
+  CHK(upload(dtab, tab));
+  CHK(upload(dsi, si));
+  CHK(upload(dal, al));
+  CHK(ensure(dimg, sizeof(float) * (3LL * h * w + 3LL * oh * ow)));
+  CHK(ensure(dtmp, sizeof(float) * 3LL * oh * w));
+  HIPCHK(hipMemcpy(dimg.p, in.data(), sizeof(float) * in.size(), hipMemcpyHostToDevice));
+  launch_resize_rows(c->stream, dp<ResizeOp>(dops), 1, (int)((3LL * oh * w + 255) / 256),
+                     dp<int2>(dtab), dp<int>(dsi), dp<float>(dal), dp<float>(dimg), dp<float>(dtmp));
+  launch_resize_cols(c->stream, dp<ResizeOp>(dops) + 1, 1, (int)((3LL * oh * ow + 255) / 256),
+                     dp<int2>(dtab), dp<int>(dsi), dp<float>(dal), dp<float>(dtmp), dp<float>(dimg));
+  HIPCHK(hipGetLastError());
+  std::vector<float> outr((size_t)3 * oh * ow);
+  HIPCHK(hipMemcpyAsync(outr.data(), dp<float>(dimg) + 3LL * h * w, sizeof(float) * outr.size(),
+                        hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int cc = 0; cc < 3; cc++)
+    for (int x = 0; x < ow; x++)
+      for (int y = 0; y < oh; y++) dst[((size_t)cc * ow + x) * oh + y] = outr[((size_t)cc * oh + y) * ow + x];
+  return GEYE_OK;
+}
+
+int geye_calc_feature(geye_ctx *c, const float *img, int height, int width, int sbin,
+                      float *feat, int *F0, int *F1) {
+  if (!c || !img || !F0 || !F1) return fail(GEYE_ERR_ARG, "null argument");
+  if (sbin < 2 || sbin > 8) return fail(GEYE_ERR_UNSUPPORTED, "sbin outside [2, 8]");
+  if (height < 3 || width < 3) return fail(GEYE_ERR_ARG, "image smaller than 3x3");
+  const int b0 = (int)std::floor(double(height) / double(sbin) + 0.5);
+  const int b1 = (int)std::floor(double(width) / double(sbin) + 0.5);
+  *F0 = std::max(b0 - 2, 0);
+  *F1 = std::max(b1 - 2, 0);
+  if (!feat || *F0 == 0 || *F1 == 0) return GEYE_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(hipSetDevice(c->device));
+  const int h = height, w = width;
+  std::vector<float> in((size_t)3 * h * w);
+  for (int cc = 0; cc < 3; cc++)
+    for (int x = 0; x < w; x++)
+      for (int y = 0; y < h; y++) in[((size_t)cc * h + y) * w + x] = img[((size_t)cc * w + x) * h + y];
+  HogLevel L{};
+  L.img_off = 0;
+  L.h = h; L.w = w; L.sbin = sbin; L.b0 = b0; L.b1 = b1; L.F0 = *F0; L.F1 = *F1;
+  L.T = sbin <= 4 ? 16 : 8;
+  L.tiles_x = (b1 + L.T - 1) / L.T;
+  L.tiles_y = (b0 + L.T - 1) / L.T;
+  L.blocks_begin = 0;
+  Scratch S;
+  DeviceBuffer &dl = S.get(), &dimg = S.get(), &dh = S.get(), &dn = S.get(), &df = S.get();
+  CHK(upload(dl, std::vector<HogLevel>{L}));
+  CHK(upload(dimg, in));
+  CHK(ensure(dh, sizeof(float) * 18LL * b0 * b1));
+  CHK(ensure(dn, sizeof(float) * (size_t)b0 * b1));
+  CHK(ensure(df, sizeof(float) * 31LL * *F0 * *F1));
+  launch_hog_hist(c->stream, L.T, dp<HogLevel>(dl), 1, L.tiles_x * L.tiles_y, dp<float>(dimg),
+                  dp<float>(dh), dp<float>(dn));
+  launch_hog_feat(c->stream, dp<HogLevel>(dl), 1, (int)((1LL * *F0 * *F1 + 255) / 256),
+                  dp<float>(dh), dp<float>(dn), dp<float>(df), nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(feat, df.p, sizeof(float) * 31LL * *F0 * *F1, hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GEYE_OK;
+}
+
+int geye_pyramid(geye_ctx *c, const unsigned char *bgr, int width, int height, int widthStep,
+                 int sbin, int interval, int max_scale, float *feat, long long cap, int *feat_h,
+                 int *feat_w, float *scales) {
+  if (!c || !bgr || !feat_h || !feat_w || !scales) return fail(GEYE_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(hipSetDevice(c->device));
+  Model m;
+  m.sbin = sbin;
+  m.interval = interval;
+  m.numcomponent = 0;
+  Plan P;
+  std::string err;
+  int rc = build_plan(m, width, height, P, err, max_scale);
+  if (rc != GEYE_OK) return fail(rc, err);
+  rc = plan_upload(P);
+  if (rc == GEYE_OK) {
+    const unsigned char *d_bgr = nullptr;
+    rc = frame_in(c, P, bgr, width, height, widthStep, &d_bgr);
+    if (rc == GEYE_OK) rc = run_pyramid(c, P, d_bgr, widthStep, false);
+  }
+  if (rc == GEYE_OK) {
+    for (int l = 0; l < P.LEN; l++) {
+      feat_h[l] = P.lv[l].F0;
+      feat_w[l] = P.lv[l].F1;
+      scales[l] = P.lv[l].scale;
+    }
+    if (feat) {
+      // the feature arena holds the levels with F0*F1 > 0 in level order, i.e. exactly the
+      // reference's concatenation of all levels
+      long long n = std::min(cap, P.feat_total);
+      hipError_t e = hipMemcpyAsync(feat, P.d_feat.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e != hipSuccess) rc = fail(GEYE_ERR_HIP, hipGetErrorString(e));
+    }
+  }
+  plan_release(P);
+  return rc;
+}
+
+int geye_fconv(geye_ctx *c, const float *feat, int F0, int F1, const float *filters, int n,
+               int h, int w, int precision, float *out) {
+  if (!c || !feat || !filters || !out || n <= 0) return fail(GEYE_ERR_ARG, "null argument");
+  if (h < 1 || w < 1 || h > 16 || w > 16) return fail(GEYE_ERR_UNSUPPORTED, "filter size outside 1..16");
+  const int H = F0 - h + 1, W = F1 - w + 1;
+  if (H < 1 || W < 1) return fail(GEYE_ERR_ARG, "filter larger than the feature map");
+  if (precision != GEYE_FP32) return fail(GEYE_ERR_UNSUPPORTED, "precision not available");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(hipSetDevice(c->device));
+  const int nf = pick_nf(n);
+  const int ng = (n + nf - 1) / nf;
+  std::vector<float> packed((size_t)ng * 31 * w * h * nf, 0.0f);
+  for (int g = 0; g < ng; g++)
+    for (int f = 0; f < 31; f++)
+      for (int xp = 0; xp < w; xp++)
+        for (int yp = 0; yp < h; yp++)
+          for (int k = 0; k < nf; k++) {
+            int fid = g * nf + k;
+            if (fid < n)
+              packed[((((size_t)g * 31 + f) * w + xp) * h + yp) * nf + k] =
+                  filters[(size_t)fid * 31 * h * w + (size_t)f * h * w + (size_t)xp * h + yp];
+          }
+  std::vector<ConvTile> tiles;
+  std::vector<long long> moff;
+  for (int g = 0; g < ng; g++) {
+    for (int k = 0; k < nf; k++) moff.push_back((long long)(g * nf + k) * H * W);
+    for (int x0 = 0; x0 < W; x0 += 16)
+      for (int y0 = 0; y0 < H; y0 += 64) tiles.push_back(ConvTile{0, F0, F1, H, W, x0, y0, g, g * nf});
+  }
+  Scratch S;
+  DeviceBuffer &dfe = S.get(), &dfi = S.get(), &dt = S.get(), &dm = S.get(), &dout = S.get();
+  CHK(upload(dfe, std::vector<float>(feat, feat + 31LL * F0 * F1)));
+  CHK(upload(dfi, packed));
+  CHK(upload(dt, tiles));
+  CHK(upload(dm, moff));
+  CHK(ensure(dout, sizeof(float) * (size_t)ng * nf * H * W));
+  launch_conv_f32(c->stream, nf, dp<ConvTile>(dt), (int)tiles.size(), h, w, dp<float>(dfe),
+                  dp<float>(dfi), dp<long long>(dm), dp<float>(dout));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, dout.p, sizeof(float) * (size_t)n * H * W, hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GEYE_OK;
+}
+
+int geye_dt(geye_ctx *c, const float *vals, float ax, float bx, float ay, float by,
+            const int *dims, float *M, int *Ix, int *Iy) {
+  if (!c || !vals || !dims || !M || !Ix || !Iy) return fail(GEYE_ERR_ARG, "null argument");
+  const int d0 = dims[0], d1 = dims[1];
+  if (d0 < 1 || d1 < 1) return fail(GEYE_ERR_ARG, "dt: empty map");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<DtJob> p1, p2;
+  std::vector<int4> nodes;
+  std::vector<int> depths;
+  size_t lds1 = 0, lds2 = 0;
+  if (dt_single_jobs(d0, d1, ax, bx, ay, by, p1, p2, nodes, depths, lds1, lds2) != GEYE_OK)
+    return fail(GEYE_ERR_UNSUPPORTED, "dt: map dimension too large");
+  const long long SQ = (long long)d0 * d1;
+  Scratch S;
+  DeviceBuffer &dv = S.get(), &dtm = S.get(), &diy = S.get(), &dm = S.get(), &dix = S.get(),
+               &dj1 = S.get(), &dj2 = S.get(), &dn = S.get(), &dd = S.get();
+  CHK(upload(dv, std::vector<float>(vals, vals + SQ)));
+  CHK(ensure(dtm, sizeof(float) * SQ));
+  CHK(ensure(diy, sizeof(int) * SQ));
+  CHK(ensure(dm, sizeof(float) * SQ));
+  CHK(ensure(dix, sizeof(int) * SQ));
+  CHK(upload(dj1, p1));
+  CHK(upload(dj2, p2));
+  CHK(upload(dn, nodes));
+  CHK(upload(dd, depths));
+  launch_dt(c->stream, dp<DtJob>(dj1), (int)p1.size(), lds1, dp<int4>(dn), dp<int>(dd),
+            dp<float>(dv), dp<float>(dtm), dp<int>(diy));
+  launch_dt(c->stream, dp<DtJob>(dj2), (int)p2.size(), lds2, dp<int4>(dn), dp<int>(dd),
+            dp<float>(dtm), dp<float>(dm), dp<int>(dix));
+  HIPCHK(hipGetLastError());
+  std::vector<int> tIx(SQ), tIy(SQ);
+  HIPCHK(hipMemcpyAsync(M, dm.p, sizeof(float) * SQ, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(tIx.data(), dix.p, sizeof(int) * SQ, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(tIy.data(), diy.p, sizeof(int) * SQ, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // Ix = tmpIx, Iy = tmpIy[tmpIx*dims0 + y] (CT/dt.cpp:120-131)
+  long long k = 0;
+  for (int x = 0; x < d1; x++)
+    for (int y = 0; y < d0; y++, k++) {
+      Ix[k] = tIx[k];
+      Iy[k] = tIy[(long long)tIx[k] * d0 + y];
+    }
+  return GEYE_OK;
+}
+
+int geye_dev_alloc(geye_ctx *c, size_t bytes, void **ptr) {
+  if (!c || !ptr) return fail(GEYE_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  hipError_t e = hipMalloc(ptr, std::max<size_t>(bytes, 1));
+  if (e != hipSuccess) return fail(GEYE_ERR_NOMEM, hipGetErrorString(e));
+  return GEYE_OK;
+}
+
+int geye_dev_free(geye_ctx *c, void *ptr) {
+  if (!c) return fail(GEYE_ERR_ARG, "null ctx");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipFree(ptr));
+  return GEYE_OK;
+}
+
+int geye_memcpy_h2d(geye_ctx *c, void *dst, const void *src, size_t bytes) {
+  if (!c || !dst || !src) return fail(GEYE_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return GEYE_OK;
+}
+
+int geye_synchronize(geye_ctx *c) {
+  if (!c) return fail(GEYE_ERR_ARG, "null ctx");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  flush_stats(c);
+  return GEYE_OK;
+}
+
+int geye_profile_enable(geye_ctx *c, int on) {
+  if (!c) return fail(GEYE_ERR_ARG, "null ctx");
+  std::lock_guard<std::mutex> lk(c->mu);
+  flush_stats(c);
+  c->profile = on != 0;
+  c->stats.clear();
+  return GEYE_OK;
+}
+
+int geye_kernel_stats(geye_ctx *c, char *names, int names_cap, double *ms, long long *count,
+                      int cap, int *n) {
+  if (!c || !n) return fail(GEYE_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  flush_stats(c);
+  std::string all;
+  int i = 0;
+  for (auto &kv : c->stats) {
+    if (i < cap) {
+      if (ms) ms[i] = kv.second.ms;
+      if (count) count[i] = kv.second.count;
+    }
+    all += kv.first;
+    all += '\n';
+    i++;
+  }
+  *n = i;
+  if (names && names_cap > 0) {
+    std::strncpy(names, all.c_str(), names_cap - 1);
+    names[names_cap - 1] = 0;
+  }
+  return GEYE_OK;
+}
+
+}  // extern "C"

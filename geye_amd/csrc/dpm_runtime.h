@@ -1,0 +1,28 @@
+// dpm_runtime.h -- internal interfaces between the plan, the pipeline and the C ABI.
+#pragma once
+
+#include <string>
+
+#include "dpm_internal.h"
+
+namespace geye {
+
+int compute_max_scale(const Model &m, int W, int H);
+void level_geometry(int max_scale, int sbin, int interval, int W, int H, std::vector<Level> &lv,
+                    std::vector<int> &img_h, std::vector<int> &img_w,
+                    std::vector<int> &img_src, std::vector<float> &img_scale);
+int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_override = -1);
+
+void resize_tables(int sheight, int dheight, std::vector<int2> &tab, std::vector<int> &si,
+                   std::vector<float> &alpha);
+// DT jobs for one standalone map (dims0 = n0, dims1 = n1): pass 1 over columns of a
+// negation-free source, pass 2 over rows.
+int dt_single_jobs(int n0, int n1, float ax, float bx, float ay, float by,
+                   std::vector<DtJob> &p1, std::vector<DtJob> &p2, std::vector<int4> &nodes,
+                   std::vector<int> &depths, size_t &lds1, size_t &lds2);
+
+// host NMS with the reference's exact semantics (CT/nms.cpp:70-217)
+int nms_host(const float *boxes, int n, int GL, float overlap, float *out);
+void quicksort_host(float *ary, int *order, int first, int last);
+
+}  // namespace geye

@@ -1,0 +1,144 @@
+/*
+ * geye_dpm.h -- C ABI of the MI355X (gfx950) DPM detection path.
+ *
+ * The entry points replace the reference's detection hot path (CT = CAR_TRACKING/ of the
+ * reference repository): every function returns a status code (GEYE_OK = 0) instead of
+ * calling exit(), and geye_last_error() returns the message of the calling thread's last
+ * failure.  Arrays crossing the boundary are plain host pointers in the reference's
+ * layouts (column-major planes, FLOAT = float); device-resident variants take pointers
+ * obtained from geye_dev_alloc().  No torch or HIP types appear in the signatures.
+ *
+ * Threading: one geye_ctx per device; a context serialises the calls made on it (a mutex),
+ * so two threads may share one.  Concurrent detection on one device is done by the serving
+ * loop (geye_serve_*), which owns its own streams.
+ */
+#ifndef GEYE_DPM_H
+#define GEYE_DPM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  GEYE_OK = 0,
+  GEYE_ERR_ARG = -1,       /* invalid argument (sizes, null pointers, model shape) */
+  GEYE_ERR_HIP = -2,       /* a HIP runtime call failed */
+  GEYE_ERR_NOMEM = -3,     /* device or host allocation failed, or the budget is exceeded */
+  GEYE_ERR_IO = -4,        /* model file missing or malformed */
+  GEYE_ERR_UNSUPPORTED = -5,
+  GEYE_ERR_BUSY = -6       /* serving: no slot free for the request */
+};
+
+enum { GEYE_FP32 = 0, GEYE_BF16 = 1, GEYE_FP16 = 2 };
+
+typedef struct geye_ctx geye_ctx;
+typedef struct geye_model geye_model;
+
+/* Flat model description; field meaning follows Model_info / Rootfilters / Partfilters
+ * (CT/MODEL_info.h:75-142), 0-based indices.  Filters are concatenated in the reference
+ * layout [31][w][h]. */
+typedef struct {
+  int numcomponent, sbin, interval, max_Y, max_X, padx, pady;
+  int maxpart;
+  const int *ridx, *oidx;
+  const float *offw;
+  const int *rsize;        /* [numcomponent][2] (Y, X) */
+  const int *numpart;
+  const int *pidx, *didx;  /* [numcomponent][maxpart] */
+  const int *psize;        /* [numcomponent][maxpart][2] */
+  int DefL;
+  const float *def;        /* [DefL][4] = (ax, bx, ay, by) */
+  const int *anchor;       /* [DefL][2] = (x, y) */
+  int NoR;
+  const int *root_size;    /* [NoR][3] */
+  const float *rootfilter;
+  const long long *root_off;
+  int NoP;
+  const int *part_size;
+  const float *partfilter;
+  const long long *part_off;
+} geye_model_desc;
+
+/* ---- context / model (replaces init_cuda CT/GPU_init.cpp:50, clean_cuda :310,
+ *      load_model CT/load_model.cpp:541, free_model :567) ---------------------------- */
+const char *geye_last_error(void);
+int geye_init(int device, geye_ctx **ctx);
+void geye_shutdown(geye_ctx *ctx);
+int geye_device_count(int *n);
+int geye_model_load_csv(geye_ctx *ctx, const char *comp, const char *root, const char *part,
+                        geye_model **model);
+int geye_model_create(geye_ctx *ctx, const geye_model_desc *desc, geye_model **model);
+void geye_model_free(geye_model *model);
+/* Reads back the loaded model's scalar fields and the first filter value (loader checks). */
+int geye_model_info(const geye_model *model, int *numcomponent, int *sbin, int *interval,
+                    int *max_Y, int *max_X, int *NoR, int *NoP);
+
+/* ---- level plan (replaces ini_scales CT/featurepyramid.cpp.new2:174, ini_featsize :247) */
+int geye_level_plan(const geye_model *model, int width, int height, int *max_scale, int *LEN,
+                    float *scales, int *feat_h, int *feat_w);
+
+/* ---- whole-frame detection -------------------------------------------------------------
+ * geye_get_boxes: replaces detect() + get_boxes() (CT/detect.cpp:151-226,
+ *   CT/get_boxes.cpp:583-1797): every candidate above thresh, rows of GL floats
+ *   [root box][part boxes][component][score][scale], in the reference's order.
+ * geye_detect: replaces car_detection()'s detect + nms (CT/detect.cpp:232-276,
+ *   CT/nms.cpp:94-217): the NMS survivors in original candidate order.
+ * Output buffers are malloc'd; free with geye_free.  precision: GEYE_FP32 / BF16 / FP16
+ * (the MFMA forms change only the filter convolution). */
+int geye_get_boxes(geye_ctx *ctx, const geye_model *model, const unsigned char *bgr, int width,
+                   int height, int widthStep, float thresh, int precision, float **boxes,
+                   int *num, int *GL);
+int geye_detect(geye_ctx *ctx, const geye_model *model, const unsigned char *bgr, int width,
+                int height, int widthStep, float thresh, float overlap, int precision,
+                float **boxes, int *num, int *GL);
+/* Device-resident frame (d_bgr from geye_dev_alloc + geye_memcpy_h2d). */
+int geye_detect_dev(geye_ctx *ctx, const geye_model *model, const void *d_bgr, int width,
+                    int height, int widthStep, float thresh, float overlap, int precision,
+                    float **boxes, int *num, int *GL);
+/* Root+bias-parts score maps of the last geye_get_boxes/geye_detect* call on ctx,
+ * concatenated per (valid level, component) in the reference's order, column-major.
+ * Writes at most cap floats; returns the total count through *total. */
+int geye_last_scores(geye_ctx *ctx, float *out, long long cap, long long *total);
+void geye_free(void *p);
+
+/* ---- stage entry points (host arrays, reference layouts) --------------------------------
+ * geye_resize        replaces resize()          CT/resize.cc:135
+ * geye_calc_feature  replaces calc_feature()    CT/featurepyramid.cpp.new2:263
+ * geye_pyramid       replaces calc_f_pyramid()  CT/featurepyramid.cpp.new2:952 (features of
+ *                    all levels concatenated [level][31][F1][F0])
+ * geye_fconv         replaces fconvsMT()        CT/get_boxes_func.h:28, CT/fconvsMT.cpp:310
+ *                    (one feature map against n filters of one size)
+ * geye_dt            replaces dt()              CT/dt.cpp:101
+ * geye_nms           replaces nms()             CT/nms.cpp:94 (runs on the device) */
+int geye_resize(geye_ctx *ctx, const float *src, const int *sdims, float scale, float *dst,
+                int *odims);
+int geye_calc_feature(geye_ctx *ctx, const float *img, int height, int width, int sbin,
+                      float *feat, int *F0, int *F1);
+int geye_pyramid(geye_ctx *ctx, const unsigned char *bgr, int width, int height, int widthStep,
+                 int sbin, int interval, int max_scale, float *feat, long long cap,
+                 int *feat_h, int *feat_w, float *scales);
+int geye_fconv(geye_ctx *ctx, const float *feat, int F0, int F1, const float *filters, int n,
+               int h, int w, int precision, float *out);
+int geye_dt(geye_ctx *ctx, const float *vals, float ax, float bx, float ay, float by,
+            const int *dims, float *M, int *Ix, int *Iy);
+int geye_nms(geye_ctx *ctx, const float *boxes, int num, int GL, float overlap, float *out,
+             int *out_num);
+
+/* ---- device memory and timing (bench / serving plumbing) ------------------------------- */
+int geye_dev_alloc(geye_ctx *ctx, size_t bytes, void **ptr);
+int geye_dev_free(geye_ctx *ctx, void *ptr);
+int geye_memcpy_h2d(geye_ctx *ctx, void *dst, const void *src, size_t bytes);
+int geye_synchronize(geye_ctx *ctx);
+/* Per-kernel timing with HIP events on the stream the kernels run on.  While enabled,
+ * every launch is bracketed by events; geye_kernel_stats returns, per kernel class, the
+ * total milliseconds and launch count since the last reset. */
+int geye_profile_enable(geye_ctx *ctx, int on);
+int geye_kernel_stats(geye_ctx *ctx, char *names, int names_cap, double *ms, long long *count,
+                      int cap, int *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

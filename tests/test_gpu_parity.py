@@ -1,0 +1,171 @@
+"""GPU parity of every stage of the FP32 path against the CPU oracle (bit-exact).
+
+The contract (FP32 plain form): resize, HOG features, convolution, DT values and argmax
+offsets, scores and candidate rows are bit-identical to the oracle on the same inputs,
+and the post-NMS boxes are identical.  Inputs are seeded synthetic frames and models.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from geye_amd import synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    if a.shape != b.shape:
+        return False
+    if a.dtype == np.float32:
+        return np.array_equal(a.view(np.int32), b.view(np.int32))
+    return np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("W,H,scale", [(67, 45, 0.933033), (160, 120, 0.5), (97, 131, 0.707107),
+                                       (320, 240, 0.535887), (64, 48, 1.0)])
+def test_resize_matches_oracle(ctx, W, H, scale):
+    say("resize", W, H, scale)
+    img = O.ipl_to_float(synth.make_frame(W, H, seed=W * 7 + H))
+    g = ctx.resize(img, scale)
+    o = O.resize(img, scale)
+    assert bits_equal(g, o)
+
+
+def test_resize_matches_reference_vectors(ctx):
+    from conftest import GOLDEN
+
+    z = np.load(os.path.join(GOLDEN, "resize_ref.npz"), allow_pickle=False)
+    for k, s in enumerate(z["scales"]):
+        say("resize golden", k, float(s))
+        assert bits_equal(ctx.resize(z["img"], float(s)), z[f"out{k}"])
+
+
+@pytest.mark.parametrize("W,H,sbin", [(64, 48, 8), (64, 48, 4), (133, 97, 8), (133, 97, 4),
+                                      (320, 240, 4), (203, 161, 6), (37, 29, 8)])
+def test_calc_feature_matches_oracle(ctx, W, H, sbin):
+    say("calc_feature", W, H, sbin)
+    img = O.ipl_to_float(synth.make_frame(W, H, seed=3 + W))
+    g = ctx.calc_feature(img, sbin)
+    o = O.calc_feature(img, sbin)
+    assert g.shape == o.shape
+    assert bits_equal(g, o)
+
+
+def test_pyramid_matches_oracle(ctx):
+    say("pyramid 320x240")
+    frame = synth.make_frame(320, 240, seed=9)
+    gf, gh, gw, gs = ctx.calc_f_pyramid(frame, 8, 10, 20)
+    of, oh, ow, os_ = O.pyramid(frame, 8, 10, 20)
+    assert np.array_equal(gh, oh) and np.array_equal(gw, ow) and bits_equal(gs, os_)
+    for l, (a, b) in enumerate(zip(gf, of)):
+        assert bits_equal(a, b), f"level {l}"
+
+
+@pytest.mark.parametrize("h,w,n", [(6, 6, 8), (6, 11, 2), (7, 10, 2), (8, 9, 2), (9, 6, 3),
+                                   (5, 6, 1), (6, 13, 1), (14, 5, 4), (16, 16, 1)])
+def test_conv_matches_oracle(ctx, h, w, n):
+    say("conv", h, w, n)
+    rng = np.random.default_rng(h * 100 + w)
+    feat = rng.random((31, 83, 71), dtype=np.float32) * 0.3
+    filt = [rng.standard_normal(31 * w * h).astype(np.float32) * 0.05 for _ in range(n)]
+    g = ctx.fconvsMT(feat, filt, h, w)
+    for k in range(n):
+        assert bits_equal(g[k], O.conv(feat, filt[k], h, w)), f"filter {k}"
+
+
+@pytest.mark.parametrize("d0,d1,defs", [
+    (23, 17, (0.01, 0.0, 0.01, 0.0)),
+    (41, 64, (0.023062, 0.010890, 0.015328, -0.009441)),
+    (263, 40, (0.01, 0.0, 0.01, 0.0)),
+    (30, 473, (0.01, 0.0, 0.01, 0.0)),
+    (1, 9, (0.1, 0.0, 0.1, 0.0)),
+    (9, 1, (0.1, 0.0, 0.1, 0.0)),
+    (130, 70, (0.05, 0.02, 0.03, -0.01)),
+])
+def test_dt_matches_oracle(ctx, d0, d1, defs):
+    say("dt", d0, d1, defs)
+    rng = np.random.default_rng(d0 * 1000 + d1)
+    v = rng.standard_normal((d1, d0)).astype(np.float32)
+    gM, gIx, gIy = ctx.dt(v, *defs)
+    oM, oIx, oIy = O.dt(v, *defs)
+    assert bits_equal(gM, oM)
+    assert np.array_equal(gIx, oIx)
+    assert np.array_equal(gIy, oIy)
+
+
+def test_dt_matches_reference_vectors(ctx):
+    from conftest import GOLDEN
+
+    z = np.load(os.path.join(GOLDEN, "dt_ref.npz"), allow_pickle=False)
+    for k in range(int(z["n"])):
+        say("dt golden", k)
+        d = [float(x) for x in z[f"def{k}"]]
+        M, Ix, Iy = ctx.dt(z[f"in{k}"], *d)
+        assert bits_equal(M, z[f"M{k}"]) and np.array_equal(Ix, z[f"Ix{k}"]) and np.array_equal(Iy, z[f"Iy{k}"])
+
+
+def _rows_equal(g, o):
+    assert g.shape == o.shape, (g.shape, o.shape)
+    if g.size:
+        bad = np.nonzero(~np.all(g.view(np.int32) == o.view(np.int32), axis=1))[0]
+        assert bad.size == 0, f"{bad.size} rows differ, first {bad[:5]}: {g[bad[0]]} vs {o[bad[0]]}"
+
+
+def test_get_boxes_tiny_model(ctx):
+    say("get_boxes tiny 200x150")
+    m = synth.tiny_model()
+    frame = synth.make_frame(200, 150, seed=21)
+    gm = ctx.model(m)
+    g = gm.get_boxes(frame, thresh=-0.5)
+    o = O.detect(m, frame, -0.5)
+    _rows_equal(g, o)
+
+
+def test_get_boxes_real_car(ctx, car_model):
+    say("get_boxes real car 640x480")
+    frame = synth.make_frame(640, 480, seed=1)
+    gm = ctx.model(car_model)
+    for thresh in (-0.5, -1.2):
+        g = gm.get_boxes(frame, thresh=thresh)
+        o = O.detect(car_model, frame, thresh)
+        _rows_equal(g, o)
+
+
+def test_scores_synthetic_car(ctx):
+    say("scores synthetic car 480x360")
+    m = synth.synthetic_car()
+    frame = synth.make_frame(480, 360, seed=4)
+    gm = ctx.model(m)
+    g = gm.get_boxes(frame, thresh=-0.5)
+    o = O.detect(m, frame, -0.5)
+    _rows_equal(g, o)
+    maps = O.score_maps(m, frame)
+    flat = np.concatenate([mp.ravel() for _, _, mp in maps])
+    assert bits_equal(gm.last_scores(), flat)
+
+
+def test_car_detection_matches_oracle(ctx, car_model):
+    say("car_detection real car 640x480")
+    frame = synth.make_frame(640, 480, seed=1)
+    gm = ctx.model(car_model)
+    g = gm.car_detection(frame, thresh=-1.2, overlap=0.5)
+    o = O.nms(O.detect(car_model, frame, -1.2), 0.5)
+    _rows_equal(g, o)
+
+
+def test_car_detection_synthetic(ctx):
+    say("car_detection synthetic car 320x240")
+    m = synth.synthetic_car()
+    frame = synth.make_frame(320, 240, seed=8)
+    gm = ctx.model(m)
+    g = gm.car_detection(frame, thresh=-0.5, overlap=0.5)
+    o = O.nms(O.detect(m, frame, -0.5), 0.5)
+    _rows_equal(g, o)
