@@ -46,8 +46,9 @@ struct ConvTile {
   int F0, F1;
   int H, W;                    // output map size
   int x0, y0;
-  int filt_base;               // group index into the packed filter buffer
+  long long filt_off;          // float offset of the group in the packed filter buffer
   int out_slot;                // first output map slot of the group
+  int pad_;
 };
 
 // One DT job: a block of lines (columns or rows) of one map.
@@ -148,6 +149,11 @@ struct Plan {
   DeviceBuffer d_frame, d_img, d_tmp, d_hist, d_norm, d_feat, d_featb, d_conv, d_tmpM,
       d_tmpIy, d_M, d_tmpIx, d_score, d_flags, d_blocksum, d_cand, d_count, d_rows;
   long long cand_cap = 0;
+  // device NMS
+  DeviceBuffer n_box, n_keys, n_keys2, n_sbox, n_alive0, n_alive1, n_status, n_fsup, n_picks,
+      n_counters, n_mask, n_keep, n_sel, n_sort, n_flags, n_keep_orig, n_kept, n_kept_flat,
+      n_total;
+  long long nms_fallbacks = 0;
 };
 
 struct Model {
@@ -187,6 +193,7 @@ struct geye_ctx {
   std::vector<float> last_scores;
   std::vector<geye::DeviceBuffer> scratch_bufs;
   geye::Plan *last_plan = nullptr;
+  long long nms_fallbacks = 0;
 };
 
 struct geye_model {

@@ -307,7 +307,7 @@ __global__ __launch_bounds__(256) void k_conv_f32(const ConvTile *__restrict__ t
   const int AX = TX + fw - 1, AY = TY + fh - 1;
   const long long FSQ = (long long)t.F0 * t.F1;
   const float *A0 = feat_arena + t.feat_off;
-  const float *B0 = filt + (long long)t.filt_base * 31 * fw * fh * NF;
+  const float *B0 = filt + t.filt_off;
   const int nb = fw * fh * NF;
 
   auto stage = [&](int f, int buf) {

@@ -1,0 +1,37 @@
+// dpm_nms.h -- device NMS launchers (dpm_nms.hip).
+#pragma once
+
+#include "dpm_internal.h"
+
+namespace geye {
+
+constexpr int kNmsWindow = 1024;
+
+struct NmsBox {
+  float y1, x1, y2, x2, area, score;
+  int idx, pad;
+};
+
+size_t nms_sort_temp_bytes(int n);
+void launch_nms_prep(hipStream_t s, const ScoreMap *maps, int nmaps, const float *score,
+                     const int *cand, int n, int padx, int pady, const int *rsize, NmsBox *box,
+                     unsigned long long *keys);
+void launch_nms_prep_rows(hipStream_t s, const float *rows, int n, int GL, NmsBox *box,
+                          unsigned long long *keys);
+int launch_nms_sort(hipStream_t s, void *temp, size_t temp_bytes, unsigned long long *keys_in,
+                    unsigned long long *keys_out, int n);
+void launch_nms_gather(hipStream_t s, const unsigned long long *keys, const NmsBox *box, int n,
+                       NmsBox *sbox, int *alive, int *status, int *first_sup);
+void launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const int *nalive,
+                      int nalive_host, float overlap, unsigned long long *mask, int *status,
+                      int *first_sup, int *picks, int *counters, int *keep);
+void launch_nms_regather(hipStream_t s, const int *alive, const int *nalive, const int *sel,
+                         int nsel, int *alive_out);
+void launch_nms_check(hipStream_t s, const NmsBox *sbox, int n, const int *status,
+                      const int *first_sup, const int *picks, int npick, int *flags);
+void launch_nms_keep(hipStream_t s, const NmsBox *sbox, const int *picks, int npick,
+                     const int *flags, int *keep_orig);
+void launch_fill(hipStream_t s, int *p, long long n, int v);
+void launch_take(hipStream_t s, const int *src, const int *idx, int n, int *dst);
+
+}  // namespace geye

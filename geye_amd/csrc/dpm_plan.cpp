@@ -96,6 +96,7 @@ void resize_tables(int sheight, int dheight, std::vector<int2> &tab, std::vector
                          std::vector<float> &alpha) {
   const float scale = (float)dheight / (float)sheight;
   const float invscale = (float)sheight / (float)dheight;
+  const size_t first = si.size();
   for (int dy = 0; dy < dheight; dy++) {
     const int start = (int)si.size();
     const float fsy1 = (float)dy * invscale;
@@ -110,7 +111,7 @@ void resize_tables(int sheight, int dheight, std::vector<int2> &tab, std::vector
     tab.push_back(int2{start, (int)si.size() - start});
   }
   // the reference's table never indexes outside the source; keep that an invariant
-  for (size_t k = 0; k < si.size(); k++)
+  for (size_t k = first; k < si.size(); k++)
     if (si[k] < 0 || si[k] >= sheight) si[k] = si[k] < 0 ? 0 : sheight - 1;
 }
 
@@ -345,7 +346,7 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
             t.W = Ww;
             t.x0 = x0;
             t.y0 = y0;
-            t.filt_base = (int)(fc.off / (31LL * fc.w * fc.h * fc.nf)) + g;
+            t.filt_off = fc.off + (long long)g * 31 * fc.w * fc.h * fc.nf;
             t.out_slot = slot0;
             P.conv_tiles.push_back(t);
           }

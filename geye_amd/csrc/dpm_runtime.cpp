@@ -10,10 +10,12 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <memory>
 #include <string>
 
 #include "dpm_kernels.h"
+#include "dpm_nms.h"
 #include "dpm_runtime.h"
 
 using namespace geye;
@@ -64,10 +66,16 @@ void release(DeviceBuffer &b) {
   b.bytes = 0;
 }
 
+// Host-to-device copies are issued on the stream the kernels run on and waited for: the
+// context's stream does not synchronise with the null stream, and a pageable-memory
+// hipMemcpy may return before its DMA has landed.
 template <typename T>
-int upload(DeviceBuffer &b, const std::vector<T> &v) {
+int upload(DeviceBuffer &b, const std::vector<T> &v, hipStream_t s) {
   CHK(ensure(b, sizeof(T) * v.size()));
-  if (!v.empty()) HIPCHK(hipMemcpy(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  if (!v.empty()) {
+    HIPCHK(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
   return GEYE_OK;
 }
 
@@ -220,8 +228,8 @@ int model_from_desc(geye_ctx *ctx, const geye_model_desc *d, Model &m) {
       m.classes.push_back(fc);
     }
   }
-  CHK(upload(m.d_filters, packed));
-  CHK(upload(m.d_rsize, m.rsize));
+  CHK(upload(m.d_filters, packed, ctx->stream));
+  CHK(upload(m.d_rsize, m.rsize, ctx->stream));
   return GEYE_OK;
 }
 
@@ -355,25 +363,25 @@ int load_csv(geye_ctx *ctx, const char *comp, const char *root, const char *part
 // ---------------------------------------------------------------------------------------
 // Plans: built once per frame size and kept with the model.
 // ---------------------------------------------------------------------------------------
-int plan_upload(Plan &P) {
+int plan_upload(Plan &P, hipStream_t st) {
   std::vector<ResizeOp> ops;
   for (auto &R : P.rounds) {
     ops.insert(ops.end(), R.rows.begin(), R.rows.end());
     ops.insert(ops.end(), R.cols.begin(), R.cols.end());
   }
-  CHK(upload(P.d_resize_ops, ops));
-  CHK(upload(P.d_resize_tab, P.resize_tab));
-  CHK(upload(P.d_resize_si, P.resize_si));
-  CHK(upload(P.d_resize_alpha, P.resize_alpha));
-  CHK(upload(P.d_hog, P.hog));
-  CHK(upload(P.d_conv_tiles, P.conv_tiles));
-  CHK(upload(P.d_map_off, P.map_off));
-  CHK(upload(P.d_dt1, P.dt_pass1));
-  CHK(upload(P.d_dt2, P.dt_pass2));
-  CHK(upload(P.d_nodes, P.dt_nodes));
-  CHK(upload(P.d_depths, P.dt_depths));
-  CHK(upload(P.d_smaps, P.smaps));
-  CHK(upload(P.d_sparts, P.sparts));
+  CHK(upload(P.d_resize_ops, ops, st));
+  CHK(upload(P.d_resize_tab, P.resize_tab, st));
+  CHK(upload(P.d_resize_si, P.resize_si, st));
+  CHK(upload(P.d_resize_alpha, P.resize_alpha, st));
+  CHK(upload(P.d_hog, P.hog, st));
+  CHK(upload(P.d_conv_tiles, P.conv_tiles, st));
+  CHK(upload(P.d_map_off, P.map_off, st));
+  CHK(upload(P.d_dt1, P.dt_pass1, st));
+  CHK(upload(P.d_dt2, P.dt_pass2, st));
+  CHK(upload(P.d_nodes, P.dt_nodes, st));
+  CHK(upload(P.d_depths, P.dt_depths, st));
+  CHK(upload(P.d_smaps, P.smaps, st));
+  CHK(upload(P.d_sparts, P.sparts, st));
   long long tmp = 0;
   for (auto &R : P.rounds) tmp = std::max(tmp, R.tmp_total);
   CHK(ensure(P.d_img, sizeof(float) * P.img_total));
@@ -400,7 +408,11 @@ void plan_release(Plan &P) {
                           &P.d_nodes, &P.d_depths, &P.d_smaps, &P.d_sparts, &P.d_frame,
                           &P.d_img, &P.d_tmp, &P.d_hist, &P.d_norm, &P.d_feat, &P.d_featb,
                           &P.d_conv, &P.d_tmpM, &P.d_tmpIy, &P.d_M, &P.d_tmpIx, &P.d_score,
-                          &P.d_flags, &P.d_blocksum, &P.d_cand, &P.d_count, &P.d_rows})
+                          &P.d_flags, &P.d_blocksum, &P.d_cand, &P.d_count, &P.d_rows,
+                          &P.n_box, &P.n_keys, &P.n_keys2, &P.n_sbox, &P.n_alive0, &P.n_alive1,
+                          &P.n_status, &P.n_fsup, &P.n_picks, &P.n_counters, &P.n_mask,
+                          &P.n_keep, &P.n_sel, &P.n_sort, &P.n_flags, &P.n_keep_orig,
+                          &P.n_kept, &P.n_kept_flat, &P.n_total})
     release(*b);
 }
 
@@ -415,7 +427,7 @@ int get_plan(Model &m, int W, int H, Plan **out) {
   std::string err;
   int rc = build_plan(m, W, H, *P, err);
   if (rc != GEYE_OK) return fail(rc, err);
-  rc = plan_upload(*P);
+  rc = plan_upload(*P, m.ctx->stream);
   if (rc != GEYE_OK) {
     plan_release(*P);
     return rc;
@@ -517,20 +529,152 @@ int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int 
   return GEYE_OK;
 }
 
-int run_rows(geye_ctx *c, Model &m, Plan &P, int ncand, std::vector<float> &rows) {
+int run_rows(geye_ctx *c, Model &m, Plan &P, const int *d_list, int ncand,
+             std::vector<float> &rows) {
   rows.assign((size_t)ncand * P.GL, 0.0f);
   if (ncand == 0) return GEYE_OK;
   CHK(ensure(P.d_rows, sizeof(float) * (size_t)ncand * P.GL));
   {
     KTimer t(c, "rows");
     launch_rows(c->stream, dp<ScoreMap>(P.d_smaps), (int)P.smaps.size(), dp<ScorePart>(P.d_sparts),
-                dp<float>(P.d_score), dp<int>(P.d_cand), ncand, dp<int>(P.d_tmpIx),
+                dp<float>(P.d_score), d_list, ncand, dp<int>(P.d_tmpIx),
                 dp<int>(P.d_tmpIy), P.GL, m.padx, m.pady, dp<int>(m.d_rsize), dp<float>(P.d_rows));
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(rows.data(), P.d_rows.p, sizeof(float) * rows.size(),
                         hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return GEYE_OK;
+}
+
+// Buffers of the device NMS for n boxes (held by a plan).
+int nms_buffers(Plan &P, int n) {
+  const size_t N = (size_t)n;
+  CHK(ensure(P.n_box, sizeof(NmsBox) * N));
+  CHK(ensure(P.n_sbox, sizeof(NmsBox) * N));
+  CHK(ensure(P.n_keys, 8 * N));
+  CHK(ensure(P.n_keys2, 8 * N));
+  CHK(ensure(P.n_alive0, 4 * N));
+  CHK(ensure(P.n_alive1, 4 * N));
+  CHK(ensure(P.n_status, 4 * N));
+  CHK(ensure(P.n_fsup, 4 * N));
+  CHK(ensure(P.n_picks, 4 * N));
+  CHK(ensure(P.n_keep, 4 * N));
+  CHK(ensure(P.n_sel, 4 * N));
+  CHK(ensure(P.n_keep_orig, 4 * N));
+  CHK(ensure(P.n_kept, 4 * N));
+  CHK(ensure(P.n_kept_flat, 4 * N));
+  CHK(ensure(P.n_counters, 16));
+  CHK(ensure(P.n_flags, 16));
+  CHK(ensure(P.n_total, 16));
+  CHK(ensure(P.n_mask, sizeof(unsigned long long) * kNmsWindow * (kNmsWindow / 64)));
+  CHK(ensure(P.d_blocksum, sizeof(int) * (compact_blocks((long long)N) + 1)));
+  CHK(ensure(P.n_sort, nms_sort_temp_bytes(n)));
+  return GEYE_OK;
+}
+
+// Greedy NMS over P.n_box / P.n_keys (filled by a prep kernel, original order); leaves the
+// survivors' original indices, ascending, in P.n_kept.
+int nms_core(geye_ctx *c, Plan &P, int n, float overlap, int *nkept) {
+  hipStream_t s = c->stream;
+  const size_t N = (size_t)n;
+  *nkept = 0;
+  if (n <= 0) return GEYE_OK;
+  {
+    KTimer t(c, "nms_sort");
+    if (launch_nms_sort(s, P.n_sort.p, P.n_sort.bytes, dp<unsigned long long>(P.n_keys),
+                        dp<unsigned long long>(P.n_keys2), n) != 0)
+      return fail(GEYE_ERR_HIP, "rocprim radix sort failed");
+  }
+  launch_nms_gather(s, dp<unsigned long long>(P.n_keys2), dp<NmsBox>(P.n_box), n,
+                    dp<NmsBox>(P.n_sbox), dp<int>(P.n_alive0), dp<int>(P.n_status),
+                    dp<int>(P.n_fsup));
+  launch_fill(s, dp<int>(P.n_counters), 4, 0);
+  const int nn = n;
+  HIPCHK(hipMemcpyAsync(P.n_total.p, &nn, sizeof(int), hipMemcpyHostToDevice, s));
+  int *alive = dp<int>(P.n_alive0), *alive_next = dp<int>(P.n_alive1);
+  int na = n;
+  int *d_na = dp<int>(P.n_total);
+  int *d_sel_total = dp<int>(P.n_total) + 1;
+  while (na > 0) {
+    KTimer t(c, "nms_round");
+    launch_nms_round(s, dp<NmsBox>(P.n_sbox), alive, d_na, na, overlap,
+                     dp<unsigned long long>(P.n_mask), dp<int>(P.n_status), dp<int>(P.n_fsup),
+                     dp<int>(P.n_picks), dp<int>(P.n_counters), dp<int>(P.n_keep));
+    const int W = std::min(na, kNmsWindow);
+    const int rest = na - W;
+    if (rest <= 0) break;
+    launch_compact(s, dp<int>(P.n_keep), rest, dp<int>(P.d_blocksum), d_sel_total,
+                   dp<int>(P.n_sel));
+    int nsel = 0;
+    HIPCHK(hipMemcpyAsync(&nsel, d_sel_total, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    launch_nms_regather(s, alive, d_na, dp<int>(P.n_sel), nsel, alive_next);
+    HIPCHK(hipMemcpyAsync(d_na, d_sel_total, sizeof(int), hipMemcpyDeviceToDevice, s));
+    std::swap(alive, alive_next);
+    na = nsel;
+  }
+  int npick = 0;
+  HIPCHK(hipMemcpyAsync(&npick, P.n_counters.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  launch_fill(s, dp<int>(P.n_flags), 4, 0);
+  launch_nms_check(s, dp<NmsBox>(P.n_sbox), n, dp<int>(P.n_status), dp<int>(P.n_fsup),
+                   dp<int>(P.n_picks), npick, dp<int>(P.n_flags));
+  int flags[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(flags, P.n_flags.p, sizeof(flags), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  int nk = 0;
+  if (flags[0] != 0) {
+    // equal scores decided an outcome: replay the reference's exact quicksort order on the host
+    P.nms_fallbacks++;
+    c->nms_fallbacks++;
+    std::vector<NmsBox> hb(N);
+    HIPCHK(hipMemcpyAsync(hb.data(), P.n_box.p, sizeof(NmsBox) * N, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::vector<float> rows(N * 7), out(N * 7);
+    for (size_t i = 0; i < N; i++) {
+      float *r = &rows[i * 7];
+      r[0] = hb[i].y1; r[1] = hb[i].x1; r[2] = hb[i].y2; r[3] = hb[i].x2;
+      r[4] = (float)i; r[5] = hb[i].score; r[6] = 0.0f;
+    }
+    nk = nms_host(rows.data(), n, 7, overlap, out.data());
+    std::vector<int> kept(nk);
+    for (int k = 0; k < nk; k++) kept[k] = (int)out[(size_t)k * 7 + 4];
+    if (nk > 0) {
+      HIPCHK(hipMemcpyAsync(P.n_kept.p, kept.data(), sizeof(int) * nk, hipMemcpyHostToDevice, s));
+      HIPCHK(hipStreamSynchronize(s));
+    }
+  } else {
+    launch_fill(s, dp<int>(P.n_keep_orig), (long long)N, 0);
+    launch_nms_keep(s, dp<NmsBox>(P.n_sbox), dp<int>(P.n_picks), npick, dp<int>(P.n_flags),
+                    dp<int>(P.n_keep_orig));
+    launch_compact(s, dp<int>(P.n_keep_orig), (long long)N, dp<int>(P.d_blocksum),
+                   d_sel_total, dp<int>(P.n_kept));
+    HIPCHK(hipMemcpyAsync(&nk, d_sel_total, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  HIPCHK(hipGetLastError());
+  *nkept = nk;
+  return GEYE_OK;
+}
+
+// Device NMS over the ncand candidates of the last scoring pass; leaves the survivors'
+// flat score indices (original order) in P.n_kept_flat.
+int run_nms(geye_ctx *c, Model &m, Plan &P, int n, float overlap, int *nkept) {
+  *nkept = 0;
+  if (n <= 0) return GEYE_OK;
+  CHK(nms_buffers(P, n));
+  {
+    KTimer t(c, "nms_prep");
+    launch_nms_prep(c->stream, dp<ScoreMap>(P.d_smaps), (int)P.smaps.size(),
+                    dp<float>(P.d_score), dp<int>(P.d_cand), n, m.padx, m.pady,
+                    dp<int>(m.d_rsize), dp<NmsBox>(P.n_box), dp<unsigned long long>(P.n_keys));
+  }
+  int nk = 0;
+  CHK(nms_core(c, P, n, overlap, &nk));
+  launch_take(c->stream, dp<int>(P.d_cand), dp<int>(P.n_kept), nk, dp<int>(P.n_kept_flat));
+  HIPCHK(hipGetLastError());
+  *nkept = nk;
   return GEYE_OK;
 }
 
@@ -555,6 +699,7 @@ int detect_common(geye_ctx *c, const geye_model *model, const unsigned char *bgr
   if (!c || !model || !boxes || !num || !GL) return fail(GEYE_ERR_ARG, "null argument");
   std::lock_guard<std::mutex> lk(c->mu);
   HIPCHK(hipSetDevice(c->device));
+  (void)hipGetLastError();
   Model &m = const_cast<Model &>(model->m);
   Plan *P = nullptr;
   CHK(get_plan(m, W, H, &P));
@@ -564,18 +709,17 @@ int detect_common(geye_ctx *c, const geye_model *model, const unsigned char *bgr
   int ncand = 0;
   CHK(run_scores(c, m, *P, thresh, precision, &ncand));
   std::vector<float> rows;
-  CHK(run_rows(c, m, *P, ncand, rows));
   c->last_plan = P;
   if (do_nms) {
-    std::vector<float> out(rows.size());
-    int k = nms_host(rows.data(), ncand, P->GL, overlap, out.data());
-    out.resize((size_t)k * P->GL);
-    *boxes = to_malloc(out);
-    *num = k;
+    int nk = 0;
+    CHK(run_nms(c, m, *P, ncand, overlap, &nk));
+    CHK(run_rows(c, m, *P, dp<int>(P->n_kept_flat), nk, rows));
+    *num = nk;
   } else {
-    *boxes = to_malloc(rows);
+    CHK(run_rows(c, m, *P, dp<int>(P->d_cand), ncand, rows));
     *num = ncand;
   }
+  *boxes = to_malloc(rows);
   *GL = P->GL;
   flush_stats(c);
   return *boxes ? GEYE_OK : fail(GEYE_ERR_NOMEM, "malloc");
@@ -583,7 +727,7 @@ int detect_common(geye_ctx *c, const geye_model *model, const unsigned char *bgr
 
 // single-level plan pieces used by the stage entry points
 struct Scratch {
-  std::vector<DeviceBuffer> bufs;
+  std::deque<DeviceBuffer> bufs;  // references stay valid as buffers are added
   ~Scratch() {
     for (auto &b : bufs) release(b);
   }
@@ -739,6 +883,7 @@ int geye_model_create(geye_ctx *c, const geye_model_desc *desc, geye_model **mod
   if (!c || !desc || !model) return fail(GEYE_ERR_ARG, "null argument");
   std::lock_guard<std::mutex> lk(c->mu);
   HIPCHK(hipSetDevice(c->device));
+  (void)hipGetLastError();
   geye_model *gm = new geye_model();
   int rc = model_from_desc(c, desc, gm->m);
   if (rc != GEYE_OK) {
@@ -754,6 +899,7 @@ int geye_model_load_csv(geye_ctx *c, const char *comp, const char *root, const c
   if (!c || !comp || !root || !part || !model) return fail(GEYE_ERR_ARG, "null argument");
   std::lock_guard<std::mutex> lk(c->mu);
   HIPCHK(hipSetDevice(c->device));
+  (void)hipGetLastError();
   geye_model *gm = new geye_model();
   int rc = load_csv(c, comp, root, part, gm->m);
   if (rc != GEYE_OK) {
@@ -842,7 +988,8 @@ int geye_last_scores(geye_ctx *c, float *out, long long cap, long long *total) {
   *total = P->score_total;
   if (out && cap > 0) {
     long long n = std::min(cap, P->score_total);
-    HIPCHK(hipMemcpy(out, P->d_score.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(out, P->d_score.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
   }
   return GEYE_OK;
 }
@@ -851,10 +998,43 @@ void geye_free(void *p) { std::free(p); }
 
 int geye_nms(geye_ctx *c, const float *boxes, int num, int GL, float overlap, float *out,
              int *out_num) {
-  (void)c;
-  if (!out_num || (num > 0 && (!boxes || !out)) || GL < 7) return fail(GEYE_ERR_ARG, "bad argument");
-  *out_num = nms_host(boxes, num, GL, overlap, out);
-  return GEYE_OK;
+  if (!c || !out_num || (num > 0 && (!boxes || !out)) || GL < 7)
+    return fail(GEYE_ERR_ARG, "bad argument");
+  *out_num = 0;
+  if (num <= 0) return GEYE_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(hipSetDevice(c->device));
+  (void)hipGetLastError();
+  Plan B;
+  int rc = nms_buffers(B, num);
+  DeviceBuffer rows;
+  if (rc == GEYE_OK) rc = ensure(rows, sizeof(float) * (size_t)num * GL);
+  if (rc == GEYE_OK) {
+    hipError_t e = hipMemcpyAsync(rows.p, boxes, sizeof(float) * (size_t)num * GL,
+                                  hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) rc = fail(GEYE_ERR_HIP, hipGetErrorString(e));
+  }
+  int nk = 0;
+  if (rc == GEYE_OK) {
+    launch_nms_prep_rows(c->stream, dp<float>(rows), num, GL, dp<NmsBox>(B.n_box),
+                         dp<unsigned long long>(B.n_keys));
+    rc = nms_core(c, B, num, overlap, &nk);
+  }
+  std::vector<int> kept(nk);
+  if (rc == GEYE_OK && nk > 0) {
+    hipError_t e = hipMemcpyAsync(kept.data(), B.n_kept.p, sizeof(int) * nk,
+                                  hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = fail(GEYE_ERR_HIP, hipGetErrorString(e));
+  }
+  if (rc == GEYE_OK) {
+    for (int k = 0; k < nk; k++)
+      std::memcpy(out + (size_t)k * GL, boxes + (size_t)kept[k] * GL, sizeof(float) * GL);
+    *out_num = nk;
+  }
+  release(rows);
+  plan_release(B);
+  return rc;
 }
 
 // ---- stage entry points ----------------------------------------------------------------
@@ -876,6 +1056,7 @@ int geye_resize(geye_ctx *c, const float *src, const int *sdims, float scale, fl
   if (oh < 1 || ow < 1) return fail(GEYE_ERR_ARG, "resize: empty output");
   std::lock_guard<std::mutex> lk(c->mu);
   HIPCHK(hipSetDevice(c->device));
+  (void)hipGetLastError();
   // column-major [c][x][y] -> row-major [c][y][x]
   std::vector<float> in((size_t)3 * h * w);
   for (int cc = 0; cc < 3; cc++)
@@ -891,13 +1072,15 @@ int geye_resize(geye_ctx *c, const float *src, const int *sdims, float scale, fl
   Scratch S;
   DeviceBuffer &dops = S.get(), &dtab = S.get(), &dsi = S.get(), &dal = S.get(),
                &dimg = S.get(), &dtmp = S.get();
-  CHK(upload(dops, std::vector<ResizeOp>(ops, ops + 2)));
-  CHK(upload(dtab, tab));
-  CHK(upload(dsi, si));
-  CHK(upload(dal, al));
+  CHK(upload(dops, std::vector<ResizeOp>(ops, ops + 2), c->stream));
+  CHK(upload(dtab, tab, c->stream));
+  CHK(upload(dsi, si, c->stream));
+  CHK(upload(dal, al, c->stream));
   CHK(ensure(dimg, sizeof(float) * (3LL * h * w + 3LL * oh * ow)));
   CHK(ensure(dtmp, sizeof(float) * 3LL * oh * w));
-  HIPCHK(hipMemcpy(dimg.p, in.data(), sizeof(float) * in.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpyAsync(dimg.p, in.data(), sizeof(float) * in.size(), hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   launch_resize_rows(c->stream, dp<ResizeOp>(dops), 1, (int)((3LL * oh * w + 255) / 256),
                      dp<int2>(dtab), dp<int>(dsi), dp<float>(dal), dp<float>(dimg), dp<float>(dtmp));
   launch_resize_cols(c->stream, dp<ResizeOp>(dops) + 1, 1, (int)((3LL * oh * ow + 255) / 256),
@@ -925,6 +1108,7 @@ int geye_calc_feature(geye_ctx *c, const float *img, int height, int width, int 
   if (!feat || *F0 == 0 || *F1 == 0) return GEYE_OK;
   std::lock_guard<std::mutex> lk(c->mu);
   HIPCHK(hipSetDevice(c->device));
+  (void)hipGetLastError();
   const int h = height, w = width;
   std::vector<float> in((size_t)3 * h * w);
   for (int cc = 0; cc < 3; cc++)
@@ -939,8 +1123,8 @@ int geye_calc_feature(geye_ctx *c, const float *img, int height, int width, int 
   L.blocks_begin = 0;
   Scratch S;
   DeviceBuffer &dl = S.get(), &dimg = S.get(), &dh = S.get(), &dn = S.get(), &df = S.get();
-  CHK(upload(dl, std::vector<HogLevel>{L}));
-  CHK(upload(dimg, in));
+  CHK(upload(dl, std::vector<HogLevel>{L}, c->stream));
+  CHK(upload(dimg, in, c->stream));
   CHK(ensure(dh, sizeof(float) * 18LL * b0 * b1));
   CHK(ensure(dn, sizeof(float) * (size_t)b0 * b1));
   CHK(ensure(df, sizeof(float) * 31LL * *F0 * *F1));
@@ -961,6 +1145,7 @@ int geye_pyramid(geye_ctx *c, const unsigned char *bgr, int width, int height, i
   if (!c || !bgr || !feat_h || !feat_w || !scales) return fail(GEYE_ERR_ARG, "null argument");
   std::lock_guard<std::mutex> lk(c->mu);
   HIPCHK(hipSetDevice(c->device));
+  (void)hipGetLastError();
   Model m;
   m.sbin = sbin;
   m.interval = interval;
@@ -969,7 +1154,7 @@ int geye_pyramid(geye_ctx *c, const unsigned char *bgr, int width, int height, i
   std::string err;
   int rc = build_plan(m, width, height, P, err, max_scale);
   if (rc != GEYE_OK) return fail(rc, err);
-  rc = plan_upload(P);
+  rc = plan_upload(P, c->stream);
   if (rc == GEYE_OK) {
     const unsigned char *d_bgr = nullptr;
     rc = frame_in(c, P, bgr, width, height, widthStep, &d_bgr);
@@ -1003,6 +1188,7 @@ int geye_fconv(geye_ctx *c, const float *feat, int F0, int F1, const float *filt
   if (precision != GEYE_FP32) return fail(GEYE_ERR_UNSUPPORTED, "precision not available");
   std::lock_guard<std::mutex> lk(c->mu);
   HIPCHK(hipSetDevice(c->device));
+  (void)hipGetLastError();
   const int nf = pick_nf(n);
   const int ng = (n + nf - 1) / nf;
   std::vector<float> packed((size_t)ng * 31 * w * h * nf, 0.0f);
@@ -1021,14 +1207,14 @@ int geye_fconv(geye_ctx *c, const float *feat, int F0, int F1, const float *filt
   for (int g = 0; g < ng; g++) {
     for (int k = 0; k < nf; k++) moff.push_back((long long)(g * nf + k) * H * W);
     for (int x0 = 0; x0 < W; x0 += 16)
-      for (int y0 = 0; y0 < H; y0 += 64) tiles.push_back(ConvTile{0, F0, F1, H, W, x0, y0, g, g * nf});
+      for (int y0 = 0; y0 < H; y0 += 64) tiles.push_back(ConvTile{0, F0, F1, H, W, x0, y0, (long long)g * 31 * w * h * nf, g * nf, 0});
   }
   Scratch S;
   DeviceBuffer &dfe = S.get(), &dfi = S.get(), &dt = S.get(), &dm = S.get(), &dout = S.get();
-  CHK(upload(dfe, std::vector<float>(feat, feat + 31LL * F0 * F1)));
-  CHK(upload(dfi, packed));
-  CHK(upload(dt, tiles));
-  CHK(upload(dm, moff));
+  CHK(upload(dfe, std::vector<float>(feat, feat + 31LL * F0 * F1), c->stream));
+  CHK(upload(dfi, packed, c->stream));
+  CHK(upload(dt, tiles, c->stream));
+  CHK(upload(dm, moff, c->stream));
   CHK(ensure(dout, sizeof(float) * (size_t)ng * nf * H * W));
   launch_conv_f32(c->stream, nf, dp<ConvTile>(dt), (int)tiles.size(), h, w, dp<float>(dfe),
                   dp<float>(dfi), dp<long long>(dm), dp<float>(dout));
@@ -1046,6 +1232,7 @@ int geye_dt(geye_ctx *c, const float *vals, float ax, float bx, float ay, float 
   if (d0 < 1 || d1 < 1) return fail(GEYE_ERR_ARG, "dt: empty map");
   std::lock_guard<std::mutex> lk(c->mu);
   HIPCHK(hipSetDevice(c->device));
+  (void)hipGetLastError();
   std::vector<DtJob> p1, p2;
   std::vector<int4> nodes;
   std::vector<int> depths;
@@ -1056,15 +1243,15 @@ int geye_dt(geye_ctx *c, const float *vals, float ax, float bx, float ay, float 
   Scratch S;
   DeviceBuffer &dv = S.get(), &dtm = S.get(), &diy = S.get(), &dm = S.get(), &dix = S.get(),
                &dj1 = S.get(), &dj2 = S.get(), &dn = S.get(), &dd = S.get();
-  CHK(upload(dv, std::vector<float>(vals, vals + SQ)));
+  CHK(upload(dv, std::vector<float>(vals, vals + SQ), c->stream));
   CHK(ensure(dtm, sizeof(float) * SQ));
   CHK(ensure(diy, sizeof(int) * SQ));
   CHK(ensure(dm, sizeof(float) * SQ));
   CHK(ensure(dix, sizeof(int) * SQ));
-  CHK(upload(dj1, p1));
-  CHK(upload(dj2, p2));
-  CHK(upload(dn, nodes));
-  CHK(upload(dd, depths));
+  CHK(upload(dj1, p1, c->stream));
+  CHK(upload(dj2, p2, c->stream));
+  CHK(upload(dn, nodes, c->stream));
+  CHK(upload(dd, depths, c->stream));
   launch_dt(c->stream, dp<DtJob>(dj1), (int)p1.size(), lds1, dp<int4>(dn), dp<int>(dd),
             dp<float>(dv), dp<float>(dtm), dp<int>(diy));
   launch_dt(c->stream, dp<DtJob>(dj2), (int)p2.size(), lds2, dp<int4>(dn), dp<int>(dd),
@@ -1103,7 +1290,8 @@ int geye_dev_free(geye_ctx *c, void *ptr) {
 int geye_memcpy_h2d(geye_ctx *c, void *dst, const void *src, size_t bytes) {
   if (!c || !dst || !src) return fail(GEYE_ERR_ARG, "null argument");
   HIPCHK(hipSetDevice(c->device));
-  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return GEYE_OK;
 }
 

@@ -169,3 +169,30 @@ def test_car_detection_synthetic(ctx):
     g = gm.car_detection(frame, thresh=-0.5, overlap=0.5)
     o = O.nms(O.detect(m, frame, -0.5), 0.5)
     _rows_equal(g, o)
+
+
+def _random_boxes(n, seed, ties=False, GL=11):
+    rng = np.random.default_rng(seed)
+    y1 = rng.integers(0, 400, n).astype(np.float32)
+    x1 = rng.integers(0, 600, n).astype(np.float32)
+    h = rng.integers(20, 120, n).astype(np.float32)
+    w = rng.integers(20, 160, n).astype(np.float32)
+    sc = rng.standard_normal(n).astype(np.float32)
+    if ties:
+        sc = np.round(sc * 4) / 4  # many equal scores
+    rows = np.zeros((n, GL), np.float32)
+    rows[:, 0], rows[:, 1], rows[:, 2], rows[:, 3] = y1, x1, y1 + h - 1, x1 + w - 1
+    rows[:, GL - 3] = rng.integers(0, 6, n)
+    rows[:, GL - 2] = sc
+    rows[:, GL - 1] = 8.0
+    return rows
+
+
+@pytest.mark.parametrize("n,ties", [(1, False), (2, False), (50, False), (3000, False),
+                                    (3000, True), (20000, False)])
+def test_nms_matches_oracle(ctx, n, ties):
+    say("nms", n, ties)
+    rows = _random_boxes(n, seed=n + ties, ties=ties)
+    g = ctx.nms(rows, 0.5)
+    o = O.nms(rows, 0.5)
+    _rows_equal(g, o)
