@@ -1,0 +1,155 @@
+"""Benchmark of the DPM detection path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): one 1920x1080 seeded
+rectangle-noise frame, the synthetic car-shaped model (6 components, roots {6x11, 6x11,
+7x10, 7x10, 8x9, 8x9}, 8 parts of 6x6 each, 31 HOG channels read), threshold -0.5, NMS
+overlap 0.5.  A step is one whole detection of that frame: resize + HOG pyramid, root and
+part convolution, DT, scoring, candidate compaction, NMS and the surviving rows copied
+back.  The frame is resident in HBM before the timed region starts.
+
+Prints ONE JSON line (rank 0, N = 1): frames/s of the measured form, p50/p99 latency, the
+roofline of the dominant kernel (algorithmic FLOPs per launch / mean launch time from HIP
+events on the stream the kernel runs on) and the CPU oracle timed on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MI355X_PEAK = {"fp32": 157.3e12, "bf16": 2.5e15, "fp16": 2.5e15}
+HBM_PEAK = 8.0e12
+W, H = 1920, 1080
+
+
+def conv_flops(plan: dict, model, level_valid) -> dict:
+    """Algorithmic convolution FLOPs per frame, 2*sum (F0-h+1)(F1-w+1)*h*w*31, per shape."""
+    interval = model.interval
+    out = {}
+    for l in range(interval, plan["LEN"]):
+        if not level_valid[l]:
+            continue
+        L = l - interval
+        for (h, w, _) in model.root_size.tolist():
+            key = f"{h}x{w}"
+            out[key] = out.get(key, 0) + 2 * (int(plan["feat_h"][l]) - h + 1) * (int(plan["feat_w"][l]) - w + 1) * h * w * 31
+        for (h, w, _) in model.part_size.tolist():
+            key = f"{h}x{w}"
+            out[key] = out.get(key, 0) + 2 * (int(plan["feat_h"][L]) - h + 1) * (int(plan["feat_w"][L]) - w + 1) * h * w * 31
+    return out
+
+
+def cpu_baseline(model, frame, threads: int) -> dict:
+    """The oracle (CPU restatement of the reference path) on one frame of the workload."""
+    from oracle import oracle as O
+
+    t0 = time.perf_counter()
+    boxes = O.detect(model, frame, -0.5, 0, threads)
+    t1 = time.perf_counter()
+    kept = O.nms(boxes, 0.5)
+    t2 = time.perf_counter()
+    return {"value": 1.0 / (t2 - t0), "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"1 frame of the workload ({W}x{H}, synthetic car): detect {t1 - t0:.2f} s "
+                      f"({threads} threads: thread per level HOG, thread per filter conv) + "
+                      f"NMS {t2 - t1:.2f} s ({len(boxes)} candidates -> {len(kept)} boxes)",
+            "n_candidates": int(len(boxes)), "n_boxes": int(len(kept))}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is "
+                                   "described in DESIGN.md and not run", "n_gpus": args.gpus}))
+        return 2
+
+    import geye_amd
+    from geye_amd import synth
+
+    model = synth.synthetic_car()
+    frame = synth.make_frame(W, H, seed=2)
+    ctx = geye_amd.Context(0)
+    gm = ctx.model(model)
+    d_frame = ctx.dev_upload(frame)
+    plan = gm.level_plan(W, H)
+    level_valid = [(l >= model.interval and plan["feat_h"][l] >= model.max_Y
+                    and plan["feat_w"][l] >= model.max_X) for l in range(plan["LEN"])]
+
+    for _ in range(args.warmup):
+        boxes = gm.detect_dev(d_frame, W, H, -0.5, 0.5, args.precision)
+    ctx.synchronize()
+    ctx.profile(True)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s = time.perf_counter()
+        boxes = gm.detect_dev(d_frame, W, H, -0.5, 0.5, args.precision)
+        lat.append((time.perf_counter() - s) * 1e3)
+    ctx.synchronize()
+    total = time.perf_counter() - t0
+    stats = ctx.kernel_stats()
+    ctx.profile(False)
+
+    fl = conv_flops(plan, model, level_valid)
+    # dominant kernel: the convolution class with the most time
+    conv = {k[len("conv_f32_"):]: v for k, v in stats.items() if k.startswith("conv_f32_")}
+    dom = max(conv, key=lambda k: conv[k][0]) if conv else None
+    roof = None
+    if dom:
+        ms, cnt = conv[dom]
+        mean_s = ms / cnt / 1e3
+        achieved = fl[dom] / mean_s
+        peak = MI355X_PEAK[args.precision]
+        roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
+                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                "kernel": f"conv_{args.precision}_{dom}", "flops_per_launch": fl[dom],
+                "mean_launch_ms": ms / cnt}
+
+    lat = np.array(lat)
+    line = {
+        "metric": "frames/s + p50/p99 ms, BF16-MFMA batched vs FP32 plain, 1080p car DPM, 1 GPU",
+        "value": args.steps / total,
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": float(total / args.steps * 1e3),
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp32": "f32", "bf16": "bf16", "fp16": "f16"}[args.precision],
+        "data": "synthetic",
+        "config": {"workload": f"{W}x{H} rectangle-noise frame, synthetic car DPM (6 comp x 8 "
+                               f"parts 6x6), thresh -0.5, NMS 0.5, {args.precision} conv",
+                   "form": "plain" if args.precision == "fp32" else "mfma",
+                   "levels": int(plan["LEN"]), "boxes": int(len(boxes))},
+        "roofline": roof,
+        "kernel_ms_per_frame": {k: v[0] / max(1, args.steps) for k, v in sorted(stats.items())},
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        line["cpu_baseline"] = cpu_baseline(model, frame, threads)
+    print(json.dumps(line), flush=True)
+    ctx.dev_free(d_frame)
+    gm.free()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

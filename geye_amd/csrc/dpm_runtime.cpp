@@ -91,7 +91,7 @@ struct KTimer {
   geye_ctx *c;
   std::string name;
   hipEvent_t a = nullptr, b = nullptr;
-  KTimer(geye_ctx *ctx, const char *n) : c(ctx), name(n) {
+  KTimer(geye_ctx *ctx, std::string n) : c(ctx), name(std::move(n)) {
     if (!c->profile) return;
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
@@ -491,7 +491,7 @@ int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int 
   if (precision != GEYE_FP32) return fail(GEYE_ERR_UNSUPPORTED, "precision not available");
   for (auto &cc : P.conv_classes) {
     if (cc.tile_count == 0) continue;
-    KTimer t(c, "conv_f32");
+    KTimer t(c, "conv_f32_" + std::to_string(cc.h) + "x" + std::to_string(cc.w));
     launch_conv_f32(s, cc.nf, dp<ConvTile>(P.d_conv_tiles) + cc.tile_begin, cc.tile_count, cc.h,
                     cc.w, dp<float>(P.d_feat), dp<float>(m.d_filters), dp<long long>(P.d_map_off),
                     dp<float>(P.d_conv));
