@@ -48,7 +48,18 @@ struct ConvTile {
   int x0, y0;
   long long filt_off;          // float offset of the group in the packed filter buffer
   int out_slot;                // first output map slot of the group
-  int pad_;
+  int nvalid;                  // real filters in the group (the rest are zero padding)
+};
+
+// One MFMA convolution tile: 4 output columns x 64 rows of one level for up to 16*NT
+// filters of one class.
+struct MfmaTile {
+  long long featb_off;         // 16-bit channel-last feature arena offset ([F1][F0][32])
+  int F0, F1;
+  int x0, y0;
+  long long filt_off;          // 16-bit element offset of the group's packed filters
+  int out_slot;                // slot of the group's first member
+  int nvalid;                  // real filters in the group
 };
 
 // One DT job: a block of lines (columns or rows) of one map.
@@ -125,6 +136,11 @@ struct Plan {
     int h, w, nf, tile_begin, tile_count, filt_off;
   };
   std::vector<ConvClass> conv_classes;
+  std::vector<MfmaTile> mfma_tiles;
+  struct MfmaClass {
+    int h, w, nt, tile_begin, tile_count;
+  };
+  std::vector<MfmaClass> mfma_classes;
   std::vector<long long> map_off;      // output map slot -> offset in conv arena
   std::vector<int2> map_dims;          // output map slot -> (H, W)
   long long conv_total = 0;
@@ -144,7 +160,7 @@ struct Plan {
   int GL = 0;
   // device copies of the tables
   DeviceBuffer d_resize_ops, d_resize_tab, d_resize_si, d_resize_alpha, d_hog, d_conv_tiles,
-      d_map_off, d_dt1, d_dt2, d_nodes, d_depths, d_smaps, d_sparts;
+      d_mfma_tiles, d_map_dims, d_map_off, d_dt1, d_dt2, d_nodes, d_depths, d_smaps, d_sparts;
   // device arenas
   DeviceBuffer d_frame, d_img, d_tmp, d_hist, d_norm, d_feat, d_featb, d_conv, d_tmpM,
       d_tmpIy, d_M, d_tmpIx, d_score, d_flags, d_blocksum, d_cand, d_count, d_rows;
@@ -164,14 +180,22 @@ struct Model {
   std::vector<float> offw, def;
   std::vector<std::vector<float>> rootf, partf;   // reference layout [31][w][h]
   // packed filters per shape class (device): [group][31][w][h][NF]
+  // A class is a set of filters of one kind convolved together: (h, w) is the bounding
+  // size; a member smaller than it is zero-padded at its far edges, which leaves every
+  // product sum (and the FP32 path's exact accumulation) unchanged.
   struct FilterClass {
-    int h, w, nf;
+    int h, w;
     bool root;
-    std::vector<int> members;        // filter ids (padded with -1 to a multiple of nf)
-    long long off = 0;               // float offset in d_filters
+    std::vector<int> members;        // filter ids
+    std::vector<int2> msize;         // true (h, w) per member
+    int nf = 1;                      // FP32 kernel: filters per group
+    long long off = 0;               // FP32 packed offset (floats)
+    int nt = 1;                      // MFMA kernel: 16-filter tiles per group
+    long long hoff = 0;              // MFMA packed offset (16-bit elements)
+    int nslots = 0;                  // output slots per level (members + padding)
   };
   std::vector<FilterClass> classes;
-  DeviceBuffer d_filters, d_rsize;
+  DeviceBuffer d_filters, d_rsize, d_filt_bf16, d_filt_f16;
   std::unordered_map<long long, Plan *> plans;   // key = (W << 20) | H
 };
 

@@ -15,10 +15,10 @@ void launch_resize_cols(hipStream_t s, const ResizeOp *ops, int nops, int blocks
 void launch_hog_hist(hipStream_t s, int T, const HogLevel *lv, int nlv, int blocks,
                      const float *img, float *hist, float *norm);
 void launch_hog_feat(hipStream_t s, const HogLevel *lv, int nlv, int blocks, const float *hist,
-                     const float *norm, float *feat, void *featb);
+                     const float *norm, float *feat, void *featb, int hkind);
 void launch_conv_f32(hipStream_t s, int nf, const ConvTile *tiles, int ntiles, int fh, int fw,
                      const float *feat, const float *filt, const long long *map_off,
-                     float *out);
+                     const int2 *map_dims, float *out);
 void launch_dt(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, const int4 *nodes,
                const int *depths, const float *src, float *dst, int *ptr);
 void launch_score(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,

@@ -12,7 +12,7 @@
 //   boxes       CT/get_boxes.cpp:185-258, 1497-1546
 #include "dpm_kernels.h"
 
-#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 
 namespace geye {
 
@@ -232,11 +232,21 @@ __device__ inline float norm_factor(const float *p, int b0) {
   return (float)(1.0 / sqrt((double)s + 0.0001));
 }
 
+__device__ inline unsigned short half_bits(float f, int kind) {
+  if (kind == 1) {
+    unsigned int u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite values)
+    return (unsigned short)(u >> 16);
+  }
+  return __half_as_ushort(__float2half_rn(f));
+}
+
 __global__ __launch_bounds__(256) void k_hog_feat(const HogLevel *__restrict__ lv, int nlv,
                                                   const float *__restrict__ hist_arena,
                                                   const float *__restrict__ norm_arena,
                                                   float *__restrict__ feat_arena,
-                                                  __hip_bfloat16 *__restrict__ featb_arena) {
+                                                  unsigned short *__restrict__ featb_arena,
+                                                  int hkind) {
   int li = 0;
   while (li + 1 < nlv && lv[li + 1].blocks_begin <= (int)blockIdx.x) li++;
   const HogLevel L = lv[li];
@@ -253,7 +263,7 @@ __global__ __launch_bounds__(256) void k_hog_feat(const HogLevel *__restrict__ l
   const float n4 = norm_factor(norm + (long long)x * b0 + y, b0);
   const float *src = hist_arena + L.hist_off + (long long)(x + 1) * b0 + (y + 1);
   float *dst = feat_arena + L.feat_off + e;
-  __hip_bfloat16 *db = featb_arena ? featb_arena + L.featb_off + e * 32 : nullptr;
+  unsigned short *db = featb_arena ? featb_arena + L.featb_off + e * 32 : nullptr;
   float t1 = 0.0f, t2 = 0.0f, t3 = 0.0f, t4 = 0.0f;
   float hs[18];
 #pragma unroll
@@ -263,7 +273,7 @@ __global__ __launch_bounds__(256) void k_hog_feat(const HogLevel *__restrict__ l
     float h1 = min_2(hv * n1), h2 = min_2(hv * n2), h3 = min_2(hv * n3), h4 = min_2(hv * n4);
     float f = 0.5f * (h1 + h2 + h3 + h4);
     dst[o * OSQ] = f;
-    if (db) db[o] = __float2bfloat16(f);
+    if (db) db[o] = half_bits(f, hkind);
     t1 += h1; t2 += h2; t3 += h3; t4 += h4;
   }
 #pragma unroll
@@ -272,16 +282,16 @@ __global__ __launch_bounds__(256) void k_hog_feat(const HogLevel *__restrict__ l
     float h1 = min_2(sum * n1), h2 = min_2(sum * n2), h3 = min_2(sum * n3), h4 = min_2(sum * n4);
     float f = 0.5f * (h1 + h2 + h3 + h4);
     dst[(18 + o) * OSQ] = f;
-    if (db) db[18 + o] = __float2bfloat16(f);
+    if (db) db[18 + o] = half_bits(f, hkind);
   }
   float tx[4] = {(float)(0.2357 * (double)t1), (float)(0.2357 * (double)t2),
                  (float)(0.2357 * (double)t3), (float)(0.2357 * (double)t4)};
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     dst[(27 + q) * OSQ] = tx[q];
-    if (db) db[27 + q] = __float2bfloat16(tx[q]);
+    if (db) db[27 + q] = half_bits(tx[q], hkind);
   }
-  if (db) db[31] = __float2bfloat16(0.0f);
+  if (db) db[31] = 0;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -296,6 +306,7 @@ __global__ __launch_bounds__(256) void k_conv_f32(const ConvTile *__restrict__ t
                                                   int fw, const float *__restrict__ feat_arena,
                                                   const float *__restrict__ filt,
                                                   const long long *__restrict__ map_off,
+                                                  const int2 *__restrict__ map_dims,
                                                   float *__restrict__ out_arena) {
   constexpr int TX = 16, TY = 64, PY = 4;
   constexpr int AXM = TX + 15, AYP = TY + 16;  // fw, fh <= 16
@@ -359,15 +370,16 @@ __global__ __launch_bounds__(256) void k_conv_f32(const ConvTile *__restrict__ t
     __syncthreads();
   }
   const int x = t.x0 + tx;
-  if (x < t.W) {
 #pragma unroll
-    for (int n = 0; n < NF; n++) {
-      float *o = out_arena + map_off[t.out_slot + n] + (long long)x * t.H;
+  for (int n = 0; n < NF; n++) {
+    if (n >= t.nvalid) continue;
+    const int2 hw = map_dims[t.out_slot + n];  // the member's own output size
+    if (x >= hw.y) continue;
+    float *o = out_arena + map_off[t.out_slot + n] + (long long)x * hw.x;
 #pragma unroll
-      for (int i = 0; i < PY; i++) {
-        int y = t.y0 + ty * PY + i;
-        if (y < t.H) o[y] = 0.0f + acc[i][n];
-      }
+    for (int i = 0; i < PY; i++) {
+      int y = t.y0 + ty * PY + i;
+      if (y < hw.x) o[y] = 0.0f + acc[i][n];
     }
   }
 }
@@ -682,30 +694,30 @@ void launch_hog_hist(hipStream_t s, int T, const HogLevel *lv, int nlv, int bloc
 }
 
 void launch_hog_feat(hipStream_t s, const HogLevel *lv, int nlv, int blocks, const float *hist,
-                     const float *norm, float *feat, void *featb) {
+                     const float *norm, float *feat, void *featb, int hkind) {
   hipLaunchKernelGGL(k_hog_feat, dim3(blocks), dim3(256), 0, s, lv, nlv, hist, norm, feat,
-                     (__hip_bfloat16 *)featb);
+                     (unsigned short *)featb, hkind);
 }
 
 void launch_conv_f32(hipStream_t s, int nf, const ConvTile *tiles, int ntiles, int fh, int fw,
                      const float *feat, const float *filt, const long long *map_off,
-                     float *out) {
+                     const int2 *map_dims, float *out) {
   switch (nf) {
     case 8:
       hipLaunchKernelGGL(k_conv_f32<8>, dim3(ntiles), dim3(256), 0, s, tiles, fh, fw, feat, filt,
-                         map_off, out);
+                         map_off, map_dims, out);
       break;
     case 4:
       hipLaunchKernelGGL(k_conv_f32<4>, dim3(ntiles), dim3(256), 0, s, tiles, fh, fw, feat, filt,
-                         map_off, out);
+                         map_off, map_dims, out);
       break;
     case 2:
       hipLaunchKernelGGL(k_conv_f32<2>, dim3(ntiles), dim3(256), 0, s, tiles, fh, fw, feat, filt,
-                         map_off, out);
+                         map_off, map_dims, out);
       break;
     default:
       hipLaunchKernelGGL(k_conv_f32<1>, dim3(ntiles), dim3(256), 0, s, tiles, fh, fw, feat, filt,
-                         map_off, out);
+                         map_off, map_dims, out);
       break;
   }
 }

@@ -1,0 +1,14 @@
+// dpm_mfma.h -- MFMA convolution launcher (dpm_mfma.hip).
+#pragma once
+
+#include "dpm_internal.h"
+
+namespace geye {
+
+size_t mfma_lds_bytes(int fh, int fw, int nt);
+// kind: 1 = bf16, 2 = fp16.  Returns 0 on success, -1 when the shape does not fit.
+int launch_conv_mfma(hipStream_t s, int kind, int nt, const MfmaTile *tiles, int ntiles, int fh,
+                     int fw, const void *feat, const void *filt, const long long *map_off,
+                     const int2 *map_dims, float *out);
+
+}  // namespace geye

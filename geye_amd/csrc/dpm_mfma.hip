@@ -1,0 +1,148 @@
+// dpm_mfma.hip -- BF16 / FP16 filter convolution as an implicit GEMM on gfx950 MFMA.
+//
+// C[n][x][y] = sum_{xp, yp, c} F[x+xp][y+yp][c] * W_n[c][xp][yp]   (CT/GPU_function.cu:302-358)
+// with M = output positions, N = filters, K = taps x 32 channels (31 HOG + a zero pad).
+// Features are channel-last 16-bit ([x][y][32], 64 bytes per cell), so one
+// v_mfma_f32_16x16x32_{bf16,f16} consumes exactly one tap for 16 positions x 16 filters,
+// with FP32 accumulation.
+//
+// Workgroup (4 waves) = 4 output columns x 64 rows x up to 48 filters.  The feature tile
+// the workgroup needs ((4 + fw - 1) x (64 + fh - 1) cells) is staged into LDS once and read
+// by every tap; the packed filters stream through LDS one tap column (xp) at a time,
+// double-buffered.  Each wave owns one column: 4 M-tiles x NT N-tiles of accumulators.
+// Operand layout (per lane l of the wave): A row l%16, channels 8*(l/16)..+8; B filter
+// l%16, the same channels; accumulator rows 4*(l/16)..+4 of column l%16.
+#include "dpm_mfma.h"
+
+namespace geye {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(8))) _Float16 f16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
+
+template <int KIND>
+__device__ inline f32x4 mfma16(u32x4 a, u32x4 b, f32x4 c) {
+  if constexpr (KIND == 1)
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a),
+                                                  __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+}
+
+template <int KIND, int NT>
+__global__ __launch_bounds__(256) void k_conv_mfma(const MfmaTile *__restrict__ tiles, int fh,
+                                                   int fw, const u32x4 *__restrict__ feat,
+                                                   const u32x4 *__restrict__ filt,
+                                                   const long long *__restrict__ map_off,
+                                                   const int2 *__restrict__ map_dims,
+                                                   float *__restrict__ out) {
+  extern __shared__ u32x4 lds[];
+  const MfmaTile t = tiles[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int AX = 4 + fw - 1, AY = 64 + fh - 1;
+  u32x4 *As = lds;                               // [AX][AY][4 chunks]
+  u32x4 *Bs = lds + AX * AY * 4;                 // [2][fh][NT][4][16]
+  const int bstage = fh * NT * 64;               // 16-byte chunks per tap column
+  const u32x4 *F = feat + t.featb_off / 8;       // featb_off counts 16-bit elements
+  const u32x4 *Wg = filt + t.filt_off / 8;
+
+  // stage the feature tile (zero outside the level)
+  for (int i = tid; i < AX * AY * 4; i += 256) {
+    const int cell = i >> 2, part = i & 3;
+    const int ax = cell / AY, ay = cell - ax * AY;
+    const int gx = t.x0 + ax, gy = t.y0 + ay;
+    u32x4 v = {0u, 0u, 0u, 0u};
+    if (gx < t.F1 && gy < t.F0) v = F[((long long)gx * t.F0 + gy) * 4 + part];
+    As[i] = v;
+  }
+  for (int i = tid; i < bstage; i += 256) Bs[i] = Wg[i];
+  __syncthreads();
+
+  f32x4 acc[4][NT];
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) acc[mt][nt] = (f32x4){0.f, 0.f, 0.f, 0.f};
+
+  const int r = lane & 15, chunk = lane >> 4;
+  for (int xp = 0; xp < fw; xp++) {
+    const int buf = xp & 1;
+    if (xp + 1 < fw) {
+      const u32x4 *src = Wg + (long long)(xp + 1) * bstage;
+      u32x4 *dst = Bs + (buf ^ 1) * bstage;
+      for (int i = tid; i < bstage; i += 256) dst[i] = src[i];
+    }
+    const u32x4 *Bb = Bs + buf * bstage;
+    const u32x4 *Ab = As + ((wave + xp) * AY) * 4 + chunk;
+    for (int yp = 0; yp < fh; yp++) {
+      u32x4 b[NT];
+#pragma unroll
+      for (int nt = 0; nt < NT; nt++) b[nt] = Bb[((yp * NT + nt) * 4 + chunk) * 16 + r];
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
+        const u32x4 a = Ab[(mt * 16 + r + yp) * 4];
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) acc[mt][nt] = mfma16<KIND>(a, b[nt], acc[mt][nt]);
+      }
+    }
+    __syncthreads();
+  }
+
+  const int x = t.x0 + wave;
+#pragma unroll
+  for (int nt = 0; nt < NT; nt++) {
+    const int n = nt * 16 + r;
+    if (n >= t.nvalid) continue;
+    const int slot = t.out_slot + n;
+    const int2 hw = map_dims[slot];
+    if (x >= hw.y) continue;
+    float *o = out + map_off[slot] + (long long)x * hw.x;
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int y = t.y0 + mt * 16 + chunk * 4 + i;
+        if (y < hw.x) o[y] = acc[mt][nt][i];
+      }
+  }
+}
+
+size_t mfma_lds_bytes(int fh, int fw, int nt) {
+  return (size_t)16 * ((size_t)(4 + fw - 1) * (64 + fh - 1) * 4 + 2 * (size_t)fh * nt * 64);
+}
+
+template <int KIND, int NT>
+static int launch_one(hipStream_t s, const MfmaTile *tiles, int ntiles, int fh, int fw,
+                      const void *feat, const void *filt, const long long *map_off,
+                      const int2 *map_dims, float *out) {
+  const size_t lds = mfma_lds_bytes(fh, fw, NT);
+  static bool attr_set = false;
+  if (!attr_set) {
+    if (hipFuncSetAttribute((const void *)k_conv_mfma<KIND, NT>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
+      return -1;
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_conv_mfma<KIND, NT>), dim3(ntiles), dim3(256), lds, s, tiles, fh, fw,
+                     (const u32x4 *)feat, (const u32x4 *)filt, map_off, map_dims, out);
+  return 0;
+}
+
+int launch_conv_mfma(hipStream_t s, int kind, int nt, const MfmaTile *tiles, int ntiles, int fh,
+                     int fw, const void *feat, const void *filt, const long long *map_off,
+                     const int2 *map_dims, float *out) {
+  if (ntiles <= 0) return 0;
+  if (mfma_lds_bytes(fh, fw, nt) > 160 * 1024) return -1;
+  if (kind == 1) {
+    if (nt == 3) return launch_one<1, 3>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+    if (nt == 2) return launch_one<1, 2>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+    return launch_one<1, 1>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+  }
+  if (nt == 3) return launch_one<2, 3>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+  if (nt == 2) return launch_one<2, 2>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+  return launch_one<2, 1>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+}
+
+}  // namespace geye

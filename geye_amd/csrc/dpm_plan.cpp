@@ -303,57 +303,99 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
   P.hog8_n = (int)h8.size();
   P.hog8_blocks = b8;
   P.hogf_n = (int)hf.size();
-  // ---- convolution tiles and output maps
+  // ---- convolution: output slots per (level, class), FP32 and MFMA tiles
   P.conv_tiles.clear();
   P.conv_classes.clear();
+  P.mfma_tiles.clear();
+  P.mfma_classes.clear();
   P.map_off.clear();
   P.map_dims.clear();
   P.root_slot.assign((size_t)LEN * m.NoR, -1);
   P.part_slot.assign((size_t)LEN * m.NoP, -1);
+  std::vector<long long> featb_off(LEN, -1);
+  for (const HogLevel &h : hf)
+    for (int l = 0; l < LEN; l++)
+      if (P.feat_off[l] == h.feat_off) featb_off[l] = h.featb_off;
   long long co = 0;
   for (size_t ci = 0; ci < m.classes.size(); ci++) {
     const Model::FilterClass &fc = m.classes[ci];
+    const int nmem = (int)fc.members.size();
     Plan::ConvClass cc{fc.h, fc.w, fc.nf, (int)P.conv_tiles.size(), 0, (int)ci};
-    const int ngroups = (int)fc.members.size() / fc.nf;
+    Plan::MfmaClass mc{fc.h, fc.w, fc.nt, (int)P.mfma_tiles.size(), 0};
     for (int l = interval; l < LEN; l++) {
       if (!P.lv[l].valid) continue;
       const int fl = fc.root ? l : l - interval;
       const Level &FL = P.lv[fl];
-      const int Hh = FL.F0 - fc.h + 1, Ww = FL.F1 - fc.w + 1;
-      if (Hh < 1 || Ww < 1) {
-        err = "filter larger than a valid feature level";
-        return GEYE_ERR_ARG;
-      }
-      for (int g = 0; g < ngroups; g++) {
-        const int slot0 = (int)P.map_off.size();
-        for (int n = 0; n < fc.nf; n++) {
-          const int fid = fc.members[g * fc.nf + n];
+      const int base = (int)P.map_off.size();
+      for (int k = 0; k < fc.nslots; k++) {
+        if (k < nmem) {
+          const int Hh = FL.F0 - fc.msize[k].x + 1, Ww = FL.F1 - fc.msize[k].y + 1;
+          if (Hh < 1 || Ww < 1) {
+            err = "filter larger than a valid feature level";
+            return GEYE_ERR_ARG;
+          }
           P.map_off.push_back(co);
           P.map_dims.push_back(int2{Hh, Ww});
           co += (long long)Hh * Ww;
-          if (fid >= 0) {
-            if (fc.root) P.root_slot[(size_t)fl * m.NoR + fid] = slot0 + n;
-            else P.part_slot[(size_t)fl * m.NoP + fid] = slot0 + n;
-          }
+          const int fid = fc.members[k];
+          if (fc.root) P.root_slot[(size_t)fl * m.NoR + fid] = base + k;
+          else P.part_slot[(size_t)fl * m.NoP + fid] = base + k;
+        } else {
+          P.map_off.push_back(0);
+          P.map_dims.push_back(int2{0, 0});
         }
-        for (int x0 = 0; x0 < Ww; x0 += 16)
-          for (int y0 = 0; y0 < Hh; y0 += 64) {
+      }
+      auto group_extent = [&](int first, int count, int &Hm, int &Wm) {
+        Hm = Wm = 0;
+        for (int k = first; k < std::min(nmem, first + count); k++) {
+          Hm = std::max(Hm, P.map_dims[base + k].x);
+          Wm = std::max(Wm, P.map_dims[base + k].y);
+        }
+      };
+      const int g32 = (nmem + fc.nf - 1) / fc.nf;
+      for (int g = 0; g < g32; g++) {
+        int Hm, Wm;
+        group_extent(g * fc.nf, fc.nf, Hm, Wm);
+        for (int x0 = 0; x0 < Wm; x0 += 16)
+          for (int y0 = 0; y0 < Hm; y0 += 64) {
             ConvTile t;
             t.feat_off = P.feat_off[fl];
             t.F0 = FL.F0;
             t.F1 = FL.F1;
-            t.H = Hh;
-            t.W = Ww;
+            t.H = Hm;
+            t.W = Wm;
             t.x0 = x0;
             t.y0 = y0;
             t.filt_off = fc.off + (long long)g * 31 * fc.w * fc.h * fc.nf;
-            t.out_slot = slot0;
+            t.out_slot = base + g * fc.nf;
+            t.nvalid = std::min(fc.nf, nmem - g * fc.nf);
             P.conv_tiles.push_back(t);
+          }
+      }
+      const int gsz = 16 * fc.nt;
+      const int g16 = (nmem + gsz - 1) / gsz;
+      for (int g = 0; g < g16; g++) {
+        int Hm, Wm;
+        group_extent(g * gsz, gsz, Hm, Wm);
+        for (int x0 = 0; x0 < Wm; x0 += 4)
+          for (int y0 = 0; y0 < Hm; y0 += 64) {
+            MfmaTile t;
+            t.featb_off = featb_off[fl];
+            t.F0 = FL.F0;
+            t.F1 = FL.F1;
+            t.x0 = x0;
+            t.y0 = y0;
+            t.filt_off = fc.hoff + (long long)g * fc.w * fc.h * fc.nt * 512;
+            t.out_slot = base + g * gsz;
+            t.nvalid = std::min(gsz, nmem - g * gsz);
+            P.mfma_tiles.push_back(t);
           }
       }
     }
     cc.tile_count = (int)P.conv_tiles.size() - cc.tile_begin;
+    mc.tile_count = (int)P.mfma_tiles.size() - mc.tile_begin;
     P.conv_classes.push_back(cc);
+    P.mfma_classes.push_back(mc);
   }
   P.conv_total = co;
   // ---- DT jobs and score maps

@@ -15,6 +15,7 @@
 #include <string>
 
 #include "dpm_kernels.h"
+#include "dpm_mfma.h"
 #include "dpm_nms.h"
 #include "dpm_runtime.h"
 
@@ -128,6 +129,21 @@ int pick_nf(int count) {
   return 1;
 }
 
+// float -> bf16 with round-to-nearest-even (finite inputs)
+unsigned short to_bf16(float f) {
+  unsigned int u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+unsigned short to_f16(float f) {
+  _Float16 h = (_Float16)f;
+  unsigned short u;
+  std::memcpy(&u, &h, 2);
+  return u;
+}
+
 int model_from_desc(geye_ctx *ctx, const geye_model_desc *d, Model &m) {
   if (!d || d->numcomponent <= 0 || d->NoR <= 0 || d->sbin <= 0 || d->interval <= 0)
     return fail(GEYE_ERR_ARG, "model: empty or malformed description");
@@ -193,41 +209,74 @@ int model_from_desc(geye_ctx *ctx, const geye_model_desc *d, Model &m) {
   for (int i = 0; i < C; i++)
     if (m.rsize[2 * i] != m.root_size[3 * i] || m.rsize[2 * i + 1] != m.root_size[3 * i + 1])
       return fail(GEYE_ERR_ARG, "model: rsize disagrees with the root filter size");
-  // shape classes and packed filters [group][31][w][h][NF]
+  // filter classes: all roots together at their bounding size, parts grouped by shape
   m.classes.clear();
-  std::vector<float> packed;
-  for (int pass = 0; pass < 2; pass++) {
-    const bool root = pass == 0;
-    const int nfil = root ? m.NoR : m.NoP;
-    const std::vector<int> &sz = root ? m.root_size : m.part_size;
-    std::vector<bool> done(nfil, false);
-    for (int i = 0; i < nfil; i++) {
+  {
+    Model::FilterClass rc;
+    rc.root = true;
+    rc.h = rc.w = 0;
+    for (int r = 0; r < m.NoR; r++) {
+      rc.members.push_back(r);
+      rc.msize.push_back(int2{m.root_size[3 * r], m.root_size[3 * r + 1]});
+      rc.h = std::max(rc.h, m.root_size[3 * r]);
+      rc.w = std::max(rc.w, m.root_size[3 * r + 1]);
+    }
+    m.classes.push_back(rc);
+  }
+  {
+    std::vector<bool> done(m.NoP, false);
+    for (int i = 0; i < m.NoP; i++) {
       if (done[i]) continue;
-      Model::FilterClass fc;
-      fc.h = sz[3 * i];
-      fc.w = sz[3 * i + 1];
-      fc.root = root;
-      for (int j = i; j < nfil; j++)
-        if (!done[j] && sz[3 * j] == fc.h && sz[3 * j + 1] == fc.w) {
-          fc.members.push_back(j);
+      Model::FilterClass pc;
+      pc.root = false;
+      pc.h = m.part_size[3 * i];
+      pc.w = m.part_size[3 * i + 1];
+      for (int j = i; j < m.NoP; j++)
+        if (!done[j] && m.part_size[3 * j] == pc.h && m.part_size[3 * j + 1] == pc.w) {
+          pc.members.push_back(j);
+          pc.msize.push_back(int2{pc.h, pc.w});
           done[j] = true;
         }
-      fc.nf = pick_nf((int)fc.members.size());
-      while (fc.members.size() % fc.nf) fc.members.push_back(-1);
-      fc.off = (long long)packed.size();
-      const int ng = (int)fc.members.size() / fc.nf;
-      for (int g = 0; g < ng; g++)
-        for (int f = 0; f < 31; f++)
-          for (int xp = 0; xp < fc.w; xp++)
-            for (int yp = 0; yp < fc.h; yp++)
-              for (int n = 0; n < fc.nf; n++) {
-                const int fid = fc.members[g * fc.nf + n];
-                const std::vector<float> &src = root ? m.rootf[std::max(fid, 0)] : m.partf[std::max(fid, 0)];
-                packed.push_back(fid >= 0 ? src[(size_t)f * fc.h * fc.w + (size_t)xp * fc.h + yp] : 0.0f);
-              }
-      m.classes.push_back(fc);
+      m.classes.push_back(pc);
     }
   }
+  std::vector<float> packed;
+  std::vector<unsigned short> pbf, pf16;
+  for (Model::FilterClass &fc : m.classes) {
+    const int nmem = (int)fc.members.size();
+    fc.nf = pick_nf(nmem);
+    fc.nt = std::min(3, (nmem + 15) / 16);
+    const int g32 = (nmem + fc.nf - 1) / fc.nf, g16 = (nmem + 16 * fc.nt - 1) / (16 * fc.nt);
+    fc.nslots = std::max(g32 * fc.nf, g16 * 16 * fc.nt);
+    auto weight = [&](int member, int c, int xp, int yp) -> float {
+      if (member >= nmem || c >= 31) return 0.0f;
+      const int fid = fc.members[member];
+      const int mh = fc.msize[member].x, mw = fc.msize[member].y;
+      if (xp >= mw || yp >= mh) return 0.0f;
+      const std::vector<float> &src = fc.root ? m.rootf[fid] : m.partf[fid];
+      return src[(size_t)c * mh * mw + (size_t)xp * mh + yp];
+    };
+    fc.off = (long long)packed.size();
+    for (int g = 0; g < g32; g++)
+      for (int f = 0; f < 31; f++)
+        for (int xp = 0; xp < fc.w; xp++)
+          for (int yp = 0; yp < fc.h; yp++)
+            for (int n = 0; n < fc.nf; n++) packed.push_back(weight(g * fc.nf + n, f, xp, yp));
+    fc.hoff = (long long)pbf.size();
+    for (int g = 0; g < g16; g++)
+      for (int xp = 0; xp < fc.w; xp++)
+        for (int yp = 0; yp < fc.h; yp++)
+          for (int t = 0; t < fc.nt; t++)
+            for (int ch = 0; ch < 4; ch++)
+              for (int n = 0; n < 16; n++)
+                for (int j = 0; j < 8; j++) {
+                  const float v = weight(g * 16 * fc.nt + t * 16 + n, ch * 8 + j, xp, yp);
+                  pbf.push_back(to_bf16(v));
+                  pf16.push_back(to_f16(v));
+                }
+  }
+  CHK(upload(m.d_filt_bf16, pbf, ctx->stream));
+  CHK(upload(m.d_filt_f16, pf16, ctx->stream));
   CHK(upload(m.d_filters, packed, ctx->stream));
   CHK(upload(m.d_rsize, m.rsize, ctx->stream));
   return GEYE_OK;
@@ -375,6 +424,8 @@ int plan_upload(Plan &P, hipStream_t st) {
   CHK(upload(P.d_resize_alpha, P.resize_alpha, st));
   CHK(upload(P.d_hog, P.hog, st));
   CHK(upload(P.d_conv_tiles, P.conv_tiles, st));
+  CHK(upload(P.d_mfma_tiles, P.mfma_tiles, st));
+  CHK(upload(P.d_map_dims, P.map_dims, st));
   CHK(upload(P.d_map_off, P.map_off, st));
   CHK(upload(P.d_dt1, P.dt_pass1, st));
   CHK(upload(P.d_dt2, P.dt_pass2, st));
@@ -404,7 +455,8 @@ int plan_upload(Plan &P, hipStream_t st) {
 
 void plan_release(Plan &P) {
   for (DeviceBuffer *b : {&P.d_resize_ops, &P.d_resize_tab, &P.d_resize_si, &P.d_resize_alpha,
-                          &P.d_hog, &P.d_conv_tiles, &P.d_map_off, &P.d_dt1, &P.d_dt2,
+                          &P.d_hog, &P.d_conv_tiles, &P.d_mfma_tiles, &P.d_map_dims,
+                          &P.d_map_off, &P.d_dt1, &P.d_dt2,
                           &P.d_nodes, &P.d_depths, &P.d_smaps, &P.d_sparts, &P.d_frame,
                           &P.d_img, &P.d_tmp, &P.d_hist, &P.d_norm, &P.d_feat, &P.d_featb,
                           &P.d_conv, &P.d_tmpM, &P.d_tmpIy, &P.d_M, &P.d_tmpIx, &P.d_score,
@@ -440,7 +492,7 @@ int get_plan(Model &m, int W, int H, Plan **out) {
 // ---------------------------------------------------------------------------------------
 // Pipeline stages
 // ---------------------------------------------------------------------------------------
-int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, bool want_bf16) {
+int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, int hkind) {
   hipStream_t s = c->stream;
   {
     KTimer t(c, "ingest");
@@ -475,12 +527,12 @@ int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, bool w
     launch_hog_hist(s, 8, hog + P.hog16_n, P.hog8_n, P.hog8_blocks, dp<float>(P.d_img),
                     dp<float>(P.d_hist), dp<float>(P.d_norm));
   }
-  if (want_bf16) CHK(ensure(P.d_featb, 2 * P.featb_total));
+  if (hkind) CHK(ensure(P.d_featb, 2 * P.featb_total));
   if (P.hogf_n > 0) {
     KTimer t(c, "hog_feat");
     launch_hog_feat(s, hog + P.hog16_n + P.hog8_n, P.hogf_n, P.hog_feat_blocks,
                     dp<float>(P.d_hist), dp<float>(P.d_norm), dp<float>(P.d_feat),
-                    want_bf16 ? P.d_featb.p : nullptr);
+                    hkind ? P.d_featb.p : nullptr, hkind);
   }
   HIPCHK(hipGetLastError());
   return GEYE_OK;
@@ -488,13 +540,28 @@ int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, bool w
 
 int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int *ncand) {
   hipStream_t s = c->stream;
-  if (precision != GEYE_FP32) return fail(GEYE_ERR_UNSUPPORTED, "precision not available");
-  for (auto &cc : P.conv_classes) {
-    if (cc.tile_count == 0) continue;
-    KTimer t(c, "conv_f32_" + std::to_string(cc.h) + "x" + std::to_string(cc.w));
-    launch_conv_f32(s, cc.nf, dp<ConvTile>(P.d_conv_tiles) + cc.tile_begin, cc.tile_count, cc.h,
-                    cc.w, dp<float>(P.d_feat), dp<float>(m.d_filters), dp<long long>(P.d_map_off),
-                    dp<float>(P.d_conv));
+  if (precision == GEYE_FP32) {
+    for (auto &cc : P.conv_classes) {
+      if (cc.tile_count == 0) continue;
+      KTimer t(c, "conv_f32_" + std::to_string(cc.h) + "x" + std::to_string(cc.w));
+      launch_conv_f32(s, cc.nf, dp<ConvTile>(P.d_conv_tiles) + cc.tile_begin, cc.tile_count,
+                      cc.h, cc.w, dp<float>(P.d_feat), dp<float>(m.d_filters),
+                      dp<long long>(P.d_map_off), dp<int2>(P.d_map_dims), dp<float>(P.d_conv));
+    }
+  } else if (precision == GEYE_BF16 || precision == GEYE_FP16) {
+    const void *filt = precision == GEYE_BF16 ? m.d_filt_bf16.p : m.d_filt_f16.p;
+    const char *tag = precision == GEYE_BF16 ? "conv_bf16_" : "conv_fp16_";
+    for (auto &mc : P.mfma_classes) {
+      if (mc.tile_count == 0) continue;
+      KTimer t(c, tag + std::to_string(mc.h) + "x" + std::to_string(mc.w));
+      if (launch_conv_mfma(s, precision, mc.nt, dp<MfmaTile>(P.d_mfma_tiles) + mc.tile_begin,
+                           mc.tile_count, mc.h, mc.w, P.d_featb.p, filt,
+                           dp<long long>(P.d_map_off), dp<int2>(P.d_map_dims),
+                           dp<float>(P.d_conv)) != 0)
+        return fail(GEYE_ERR_UNSUPPORTED, "filter class too large for the MFMA kernel's LDS");
+    }
+  } else {
+    return fail(GEYE_ERR_ARG, "unknown precision");
   }
   if (!P.dt_pass1.empty()) {
     KTimer t(c, "dt_cols");
@@ -705,7 +772,7 @@ int detect_common(geye_ctx *c, const geye_model *model, const unsigned char *bgr
   CHK(get_plan(m, W, H, &P));
   const unsigned char *d_bgr = bgr;
   if (!dev) CHK(frame_in(c, *P, bgr, W, H, ws, &d_bgr));
-  CHK(run_pyramid(c, *P, d_bgr, ws, precision != GEYE_FP32));
+  CHK(run_pyramid(c, *P, d_bgr, ws, precision == GEYE_FP32 ? 0 : precision));
   int ncand = 0;
   CHK(run_scores(c, m, *P, thresh, precision, &ncand));
   std::vector<float> rows;
@@ -923,6 +990,8 @@ void geye_model_free(geye_model *gm) {
   }
   release(m.d_filters);
   release(m.d_rsize);
+  release(m.d_filt_bf16);
+  release(m.d_filt_f16);
   delete gm;
 }
 
@@ -1131,7 +1200,7 @@ int geye_calc_feature(geye_ctx *c, const float *img, int height, int width, int 
   launch_hog_hist(c->stream, L.T, dp<HogLevel>(dl), 1, L.tiles_x * L.tiles_y, dp<float>(dimg),
                   dp<float>(dh), dp<float>(dn));
   launch_hog_feat(c->stream, dp<HogLevel>(dl), 1, (int)((1LL * *F0 * *F1 + 255) / 256),
-                  dp<float>(dh), dp<float>(dn), dp<float>(df), nullptr);
+                  dp<float>(dh), dp<float>(dn), dp<float>(df), nullptr, 0);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(feat, df.p, sizeof(float) * 31LL * *F0 * *F1, hipMemcpyDeviceToHost,
                         c->stream));
@@ -1158,7 +1227,7 @@ int geye_pyramid(geye_ctx *c, const unsigned char *bgr, int width, int height, i
   if (rc == GEYE_OK) {
     const unsigned char *d_bgr = nullptr;
     rc = frame_in(c, P, bgr, width, height, widthStep, &d_bgr);
-    if (rc == GEYE_OK) rc = run_pyramid(c, P, d_bgr, widthStep, false);
+    if (rc == GEYE_OK) rc = run_pyramid(c, P, d_bgr, widthStep, 0);
   }
   if (rc == GEYE_OK) {
     for (int l = 0; l < P.LEN; l++) {
@@ -1185,39 +1254,87 @@ int geye_fconv(geye_ctx *c, const float *feat, int F0, int F1, const float *filt
   if (h < 1 || w < 1 || h > 16 || w > 16) return fail(GEYE_ERR_UNSUPPORTED, "filter size outside 1..16");
   const int H = F0 - h + 1, W = F1 - w + 1;
   if (H < 1 || W < 1) return fail(GEYE_ERR_ARG, "filter larger than the feature map");
-  if (precision != GEYE_FP32) return fail(GEYE_ERR_UNSUPPORTED, "precision not available");
+  if (precision != GEYE_FP32 && precision != GEYE_BF16 && precision != GEYE_FP16)
+    return fail(GEYE_ERR_ARG, "unknown precision");
   std::lock_guard<std::mutex> lk(c->mu);
   HIPCHK(hipSetDevice(c->device));
   (void)hipGetLastError();
-  const int nf = pick_nf(n);
-  const int ng = (n + nf - 1) / nf;
-  std::vector<float> packed((size_t)ng * 31 * w * h * nf, 0.0f);
-  for (int g = 0; g < ng; g++)
-    for (int f = 0; f < 31; f++)
+  auto wt = [&](int fid, int ch, int xp, int yp) -> float {
+    if (fid >= n || ch >= 31) return 0.0f;
+    return filters[(size_t)fid * 31 * h * w + (size_t)ch * h * w + (size_t)xp * h + yp];
+  };
+  Scratch S;
+  DeviceBuffer &dfe = S.get(), &dfi = S.get(), &dt = S.get(), &dm = S.get(), &dd = S.get(),
+               &dout = S.get();
+  std::vector<long long> moff;
+  std::vector<int2> mdims;
+  if (precision == GEYE_FP32) {
+    const int nf = pick_nf(n);
+    const int ng = (n + nf - 1) / nf;
+    std::vector<float> packed;
+    for (int g = 0; g < ng; g++)
+      for (int f = 0; f < 31; f++)
+        for (int xp = 0; xp < w; xp++)
+          for (int yp = 0; yp < h; yp++)
+            for (int k = 0; k < nf; k++) packed.push_back(wt(g * nf + k, f, xp, yp));
+    std::vector<ConvTile> tiles;
+    for (int k = 0; k < ng * nf; k++) {
+      moff.push_back((long long)std::min(k, n - 1) * H * W);
+      mdims.push_back(k < n ? int2{H, W} : int2{0, 0});
+    }
+    for (int g = 0; g < ng; g++)
+      for (int x0 = 0; x0 < W; x0 += 16)
+        for (int y0 = 0; y0 < H; y0 += 64)
+          tiles.push_back(ConvTile{0, F0, F1, H, W, x0, y0, (long long)g * 31 * w * h * nf,
+                                   g * nf, std::min(nf, n - g * nf)});
+    CHK(upload(dfe, std::vector<float>(feat, feat + 31LL * F0 * F1), c->stream));
+    CHK(upload(dfi, packed, c->stream));
+    CHK(upload(dt, tiles, c->stream));
+    CHK(upload(dm, moff, c->stream));
+    CHK(upload(dd, mdims, c->stream));
+    CHK(ensure(dout, sizeof(float) * (size_t)n * H * W));
+    launch_conv_f32(c->stream, nf, dp<ConvTile>(dt), (int)tiles.size(), h, w, dp<float>(dfe),
+                    dp<float>(dfi), dp<long long>(dm), dp<int2>(dd), dp<float>(dout));
+  } else {
+    const bool bf = precision == GEYE_BF16;
+    const int nt = std::min(3, (n + 15) / 16), gsz = 16 * nt, ng = (n + gsz - 1) / gsz;
+    std::vector<unsigned short> packed, fb((size_t)F0 * F1 * 32, 0);
+    for (int g = 0; g < ng; g++)
       for (int xp = 0; xp < w; xp++)
         for (int yp = 0; yp < h; yp++)
-          for (int k = 0; k < nf; k++) {
-            int fid = g * nf + k;
-            if (fid < n)
-              packed[((((size_t)g * 31 + f) * w + xp) * h + yp) * nf + k] =
-                  filters[(size_t)fid * 31 * h * w + (size_t)f * h * w + (size_t)xp * h + yp];
-          }
-  std::vector<ConvTile> tiles;
-  std::vector<long long> moff;
-  for (int g = 0; g < ng; g++) {
-    for (int k = 0; k < nf; k++) moff.push_back((long long)(g * nf + k) * H * W);
-    for (int x0 = 0; x0 < W; x0 += 16)
-      for (int y0 = 0; y0 < H; y0 += 64) tiles.push_back(ConvTile{0, F0, F1, H, W, x0, y0, (long long)g * 31 * w * h * nf, g * nf, 0});
+          for (int t = 0; t < nt; t++)
+            for (int ch = 0; ch < 4; ch++)
+              for (int k = 0; k < 16; k++)
+                for (int j = 0; j < 8; j++) {
+                  const float v = wt(g * gsz + t * 16 + k, ch * 8 + j, xp, yp);
+                  packed.push_back(bf ? to_bf16(v) : to_f16(v));
+                }
+    for (int x = 0; x < F1; x++)
+      for (int y = 0; y < F0; y++)
+        for (int f = 0; f < 31; f++) {
+          const float v = feat[(size_t)f * F0 * F1 + (size_t)x * F0 + y];
+          fb[((size_t)x * F0 + y) * 32 + f] = bf ? to_bf16(v) : to_f16(v);
+        }
+    std::vector<MfmaTile> tiles;
+    for (int k = 0; k < ng * gsz; k++) {
+      moff.push_back((long long)std::min(k, n - 1) * H * W);
+      mdims.push_back(k < n ? int2{H, W} : int2{0, 0});
+    }
+    for (int g = 0; g < ng; g++)
+      for (int x0 = 0; x0 < W; x0 += 4)
+        for (int y0 = 0; y0 < H; y0 += 64)
+          tiles.push_back(MfmaTile{0, F0, F1, x0, y0, (long long)g * w * h * nt * 512, g * gsz,
+                                   std::min(gsz, n - g * gsz)});
+    CHK(upload(dfe, fb, c->stream));
+    CHK(upload(dfi, packed, c->stream));
+    CHK(upload(dt, tiles, c->stream));
+    CHK(upload(dm, moff, c->stream));
+    CHK(upload(dd, mdims, c->stream));
+    CHK(ensure(dout, sizeof(float) * (size_t)n * H * W));
+    if (launch_conv_mfma(c->stream, precision, nt, dp<MfmaTile>(dt), (int)tiles.size(), h, w,
+                         dfe.p, dfi.p, dp<long long>(dm), dp<int2>(dd), dp<float>(dout)) != 0)
+      return fail(GEYE_ERR_UNSUPPORTED, "filter too large for the MFMA kernel's LDS");
   }
-  Scratch S;
-  DeviceBuffer &dfe = S.get(), &dfi = S.get(), &dt = S.get(), &dm = S.get(), &dout = S.get();
-  CHK(upload(dfe, std::vector<float>(feat, feat + 31LL * F0 * F1), c->stream));
-  CHK(upload(dfi, packed, c->stream));
-  CHK(upload(dt, tiles, c->stream));
-  CHK(upload(dm, moff, c->stream));
-  CHK(ensure(dout, sizeof(float) * (size_t)ng * nf * H * W));
-  launch_conv_f32(c->stream, nf, dp<ConvTile>(dt), (int)tiles.size(), h, w, dp<float>(dfe),
-                  dp<float>(dfi), dp<long long>(dm), dp<float>(dout));
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, dout.p, sizeof(float) * (size_t)n * H * W, hipMemcpyDeviceToHost,
                         c->stream));

@@ -196,3 +196,38 @@ def test_nms_matches_oracle(ctx, n, ties):
     g = ctx.nms(rows, 0.5)
     o = O.nms(rows, 0.5)
     _rows_equal(g, o)
+
+
+@pytest.mark.parametrize("prec,ulp", [("bf16", 2.0 ** -8), ("fp16", 2.0 ** -10)])
+@pytest.mark.parametrize("h,w,n", [(6, 6, 48), (6, 6, 8), (8, 11, 6), (6, 13, 2), (14, 5, 20),
+                                   (9, 6, 1)])
+def test_conv_mfma_error_bound(ctx, prec, ulp, h, w, n):
+    """MFMA convolution vs the FP64 oracle: every output within ulp * sum |a*b|, the bound
+    that rounding both operands to the 16-bit type (and accumulating in FP32) allows."""
+    say("conv mfma", prec, h, w, n)
+    rng = np.random.default_rng(h * 31 + w + n)
+    feat = (rng.random((31, 77, 69), dtype=np.float32) * 0.4).astype(np.float32)
+    filt = [(rng.standard_normal(31 * w * h) * 0.05).astype(np.float32) for _ in range(n)]
+    g = ctx.fconvsMT(feat, filt, h, w, precision=prec)
+    for k in range(n):
+        exact = O.conv(feat, filt[k], h, w, f64=True)
+        mag = O.conv(np.abs(feat), np.abs(filt[k]), h, w, f64=True)
+        err = np.abs(g[k].astype(np.float64) - exact)
+        assert np.all(err <= ulp * mag * 1.01 + 1e-6), f"filter {k}: max err {err.max()}"
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_detect_mfma_scores_vs_fp64(ctx, prec):
+    """Whole pipeline in the MFMA form: score maps against the FP64 oracle (FP32 features,
+    FP64 convolution/DT/part sums).  Tolerance 0.02 absolute (scores span about -3..8)."""
+    say("detect", prec, "320x240")
+    m = synth.synthetic_car()
+    frame = synth.make_frame(320, 240, seed=8)
+    gm = ctx.model(m)
+    gm.get_boxes(frame, thresh=-0.5, precision=prec)
+    g = gm.last_scores()
+    maps = O.score_maps(m, frame, precision=1)
+    ref = np.concatenate([mp.ravel() for _, _, mp in maps])
+    err = np.abs(g.astype(np.float64) - ref)
+    say(prec, "score max abs err", float(err.max()), "mean", float(err.mean()))
+    assert err.max() < 0.02
