@@ -66,6 +66,9 @@ _SIGS = [
     ("geye_model_free", None, [C.c_void_p]),
     ("geye_model_info", C.c_int, [C.c_void_p, _ip, _ip, _ip, _ip, _ip, _ip, _ip]),
     ("geye_level_plan", C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _fp, _ip, _ip]),
+    ("geye_level_plan_dims", C.c_int, [_ip, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip,
+                                       _fp, _ip, _ip]),
+    ("geye_nms_host", C.c_int, [_fp, C.c_int, C.c_int, C.c_float, _fp, _ip]),
     ("geye_get_boxes", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                  C.c_float, C.c_int, C.POINTER(_fp), _ip, _ip]),
     ("geye_detect", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
@@ -158,6 +161,33 @@ def desc_from_model(model):
         d.partfilter = _p(arr(np.concatenate(m.partfilters), np.float32), _fp)
         d.part_off = _p(arr(np.cumsum([0] + [len(f) for f in m.partfilters[:-1]]), np.int64), _llp)
     return d, keep
+
+
+def level_plan_dims(rsize, sbin: int, interval: int, width: int, height: int) -> dict:
+    """Host-only level plan (no device work)."""
+    rs = np.ascontiguousarray(rsize, dtype=np.int32)
+    nc = rs.size // 2
+    ms, LEN = C.c_int(), C.c_int()
+    _check(lib().geye_level_plan_dims(_p(rs, _ip), nc, sbin, interval, width, height,
+                                      C.byref(ms), C.byref(LEN), None, None, None))
+    n = LEN.value
+    sc = np.zeros(n, np.float32)
+    fh, fw = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    _check(lib().geye_level_plan_dims(_p(rs, _ip), nc, sbin, interval, width, height,
+                                      C.byref(ms), C.byref(LEN), _p(sc, _fp), _p(fh, _ip),
+                                      _p(fw, _ip)))
+    return dict(max_scale=ms.value, LEN=n, scales=sc, feat_h=fh, feat_w=fw)
+
+
+def nms_host(boxes: np.ndarray, overlap: float) -> np.ndarray:
+    """The exact host NMS the device path falls back to on score ties."""
+    boxes = np.ascontiguousarray(boxes, dtype=np.float32)
+    n, GL = boxes.shape
+    out = np.zeros_like(boxes)
+    k = C.c_int()
+    _check(lib().geye_nms_host(_p(boxes, _fp), n, GL, C.c_float(overlap), _p(out, _fp),
+                               C.byref(k)))
+    return out[:k.value].copy()
 
 
 class Context:

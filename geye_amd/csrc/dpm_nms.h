@@ -6,29 +6,41 @@
 namespace geye {
 
 constexpr int kNmsWindow = 1024;
+constexpr int kNmsMaxMaps = 1024;
+constexpr int kNoSup = 0x7fffffff;
+constexpr int kNmsScatterSplit = 8;  // workgroups per pick in the scatter pass
 
 struct NmsBox {
   float y1, x1, y2, x2, area, score;
   int idx, pad;
 };
 
+// score-map geometry for the scatter form of the suppression pass
+struct NmsMaps {
+  const ScoreMap *maps;
+  int nmaps;
+  const int *rsize;
+  int padx, pady;
+  int *rank_of_flat;
+};
+
 size_t nms_sort_temp_bytes(int n);
 void launch_nms_prep(hipStream_t s, const ScoreMap *maps, int nmaps, const float *score,
                      const int *cand, int n, int padx, int pady, const int *rsize, NmsBox *box,
-                     unsigned long long *keys);
+                     unsigned int *keys, int *vals);
 void launch_nms_prep_rows(hipStream_t s, const float *rows, int n, int GL, NmsBox *box,
-                          unsigned long long *keys);
-int launch_nms_sort(hipStream_t s, void *temp, size_t temp_bytes, unsigned long long *keys_in,
-                    unsigned long long *keys_out, int n);
-void launch_nms_gather(hipStream_t s, const unsigned long long *keys, const NmsBox *box, int n,
-                       NmsBox *sbox, int *alive, int *status, int *first_sup);
+                          unsigned int *keys, int *vals);
+int launch_nms_sort(hipStream_t s, void *temp, size_t temp_bytes, unsigned int *keys_in,
+                    unsigned int *keys_out, int *vals_in, int *vals_out, int n);
+void launch_nms_gather(hipStream_t s, const int *ord, const NmsBox *box, int n, NmsBox *sbox,
+                       int *alive, int *status, int *sup, const int *cand, int *rank_of_flat);
 void launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const int *nalive,
                       int nalive_host, float overlap, unsigned long long *mask, int *status,
-                      int *first_sup, int *picks, int *counters, int *keep);
+                      int *sup, int *picks, int *counters, int *keep, const NmsMaps *maps);
 void launch_nms_regather(hipStream_t s, const int *alive, const int *nalive, const int *sel,
                          int nsel, int *alive_out);
-void launch_nms_check(hipStream_t s, const NmsBox *sbox, int n, const int *status,
-                      const int *first_sup, const int *picks, int npick, int *flags);
+void launch_nms_check(hipStream_t s, const NmsBox *sbox, int n, const int *status, const int *sup,
+                      const int *picks, int npick, int *flags);
 void launch_nms_keep(hipStream_t s, const NmsBox *sbox, const int *picks, int npick,
                      const int *flags, int *keep_orig);
 void launch_fill(hipStream_t s, int *p, long long n, int v);

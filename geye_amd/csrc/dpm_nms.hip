@@ -28,6 +28,8 @@
 
 namespace geye {
 
+long long *g_nms_phase_clock = nullptr;  // optional phase timestamps of the greedy kernel
+
 __device__ inline unsigned int order_desc(float f) {
   unsigned int u = __float_as_uint(f);
   u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);  // ascending float order
@@ -39,7 +41,8 @@ __global__ __launch_bounds__(256) void k_nms_prep(const ScoreMap *__restrict__ m
                                                   const int *__restrict__ cand, int n, int padx,
                                                   int pady, const int *__restrict__ rsize,
                                                   NmsBox *__restrict__ box,
-                                                  unsigned long long *__restrict__ keys) {
+                                                  unsigned int *__restrict__ keys,
+                                                  int *__restrict__ vals) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const long long e = cand[i];
@@ -62,13 +65,15 @@ __global__ __launch_bounds__(256) void k_nms_prep(const ScoreMap *__restrict__ m
   b.idx = i;
   b.pad = 0;
   box[i] = b;
-  keys[i] = ((unsigned long long)order_desc(b.score) << 32) | (unsigned int)i;
+  keys[i] = order_desc(b.score);
+  vals[i] = i;
 }
 
 // prep from caller rows [y1, x1, y2, x2, ..., score, scale] (GL floats per row)
 __global__ __launch_bounds__(256) void k_nms_prep_rows(const float *__restrict__ rows, int n,
                                                        int GL, NmsBox *__restrict__ box,
-                                                       unsigned long long *__restrict__ keys) {
+                                                       unsigned int *__restrict__ keys,
+                                                       int *__restrict__ vals) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float *r = rows + (size_t)i * GL;
@@ -82,21 +87,27 @@ __global__ __launch_bounds__(256) void k_nms_prep_rows(const float *__restrict__
   b.idx = i;
   b.pad = 0;
   box[i] = b;
-  keys[i] = ((unsigned long long)order_desc(b.score) << 32) | (unsigned int)i;
+  keys[i] = order_desc(b.score);
+  vals[i] = i;
 }
 
-__global__ __launch_bounds__(256) void k_nms_gather(const unsigned long long *__restrict__ keys,
+// sorted order: sbox[r] = box[ord[r]]; optional rank lookup by flat score index
+__global__ __launch_bounds__(256) void k_nms_gather(const int *__restrict__ ord,
                                                     const NmsBox *__restrict__ box, int n,
                                                     NmsBox *__restrict__ sbox,
                                                     int *__restrict__ alive,
                                                     int *__restrict__ status,
-                                                    int *__restrict__ first_sup) {
+                                                    int *__restrict__ sup,
+                                                    const int *__restrict__ cand,
+                                                    int *__restrict__ rank_of_flat) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
-  sbox[r] = box[(unsigned int)(keys[r] & 0xffffffffu)];
+  const int i = ord[r];
+  sbox[r] = box[i];
   alive[r] = r;
   status[r] = 0;
-  first_sup[r] = -1;
+  sup[r] = kNoSup;
+  if (rank_of_flat) rank_of_flat[cand[i]] = r;
 }
 
 // overlap test of CT/nms.cpp:145-175 (symmetric in a and b)
@@ -113,7 +124,8 @@ __device__ inline bool suppresses(const NmsBox &a, const NmsBox &b, float overla
          (a.y1 > b.y1 && a.x1 > b.x1 && a.y2 < b.y2 && a.x2 < b.x2);
 }
 
-// row i of the window bitmask: bit j (j > i) set when window box i suppresses box j
+// row i of the window bitmask, stored transposed ([word][row]): bit j (j > i) of row i set
+// when window box i suppresses box j
 __global__ __launch_bounds__(64) void k_nms_winmask(const NmsBox *__restrict__ sbox,
                                                     const int *__restrict__ alive,
                                                     const int *__restrict__ nalive,
@@ -128,61 +140,151 @@ __global__ __launch_bounds__(64) void k_nms_winmask(const NmsBox *__restrict__ s
     bool s = false;
     if (j > i && j < W) s = suppresses(a, sbox[alive[j]], overlap);
     unsigned long long bits = __ballot(s);
-    if (threadIdx.x == 0) mask[(size_t)i * (kNmsWindow / 64) + w] = bits;
+    if (threadIdx.x == 0) mask[(size_t)w * kNmsWindow + i] = bits;  // [word][row]
   }
 }
 
-// sequential greedy over the window, one wave; lane w owns word w of the removed set
-__global__ __launch_bounds__(64) void k_nms_wingreedy(const int *__restrict__ alive,
-                                                      const int *__restrict__ nalive,
-                                                      const unsigned long long *__restrict__ mask,
-                                                      int *__restrict__ status,
-                                                      int *__restrict__ first_sup,
-                                                      int *__restrict__ picks,
-                                                      int *__restrict__ counters) {
-  const int W = min(*nalive, kNmsWindow);
-  const int lane = threadIdx.x;
+// Greedy over the window.  The workgroup stages the window's bitmask ([word][row], all in
+// LDS); then one wave runs the sequential decision with scalar state: block b of 64 rows is
+// decided with word b alone (row j's word b read by readlane, folded in only when row j is
+// still alive), and the rows picked in block b are OR-ed into the later words by their
+// owner lanes.  Finally every thread takes one window box that was not picked and finds its
+// first suppressor: the earliest pick whose row covers it.
+__global__ __launch_bounds__(256) void k_nms_wingreedy(const int *__restrict__ alive,
+                                                        const int *__restrict__ nalive,
+                                                        const unsigned long long *__restrict__ mask,
+                                                        int *__restrict__ status,
+                                                        int *__restrict__ sup,
+                                                        int *__restrict__ picks,
+                                                        int *__restrict__ counters,
+                                                        long long *__restrict__ phase_clock) {
   constexpr int NW = kNmsWindow / 64;
-  unsigned long long removed = 0;
-  int npick = counters[0];
-  const int round_begin = npick;
-  for (int i = 0; i < W; i++) {
-    const unsigned long long word = __shfl(removed, i >> 6);
-    if ((word >> (i & 63)) & 1ull) continue;
-    const int ri = alive[i];
-    if (lane == 0) {
-      status[ri] = 1;
-      picks[npick] = ri;
+  extern __shared__ unsigned long long smask[];  // [NW][kNmsWindow]
+  if (phase_clock && threadIdx.x == 0) phase_clock[0] = wall_clock64();
+  int *salive = (int *)(smask + NW * kNmsWindow);
+  __shared__ int s_first[kNmsWindow];  // window index of the pick that first removed a box
+  const int W = min(*nalive, kNmsWindow);
+  const int nblk = (W + 63) / 64;
+  // staging: this is the only workgroup on the device, so its loads are issued in wide,
+  // unrolled batches to pay the memory latency once rather than per element
+  {
+    typedef __attribute__((ext_vector_type(4))) unsigned int u4;
+    const u4 *src = (const u4 *)mask;
+    u4 *dst = (u4 *)smask;
+    const int n16 = nblk * kNmsWindow / 2;  // 16-byte chunks
+    constexpr int U = (kNmsWindow * (kNmsWindow / 64) / 2) / 256;
+    u4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int k = threadIdx.x + u * 256;
+      if (k < n16) v[u] = src[k];
     }
-    npick++;
-    if (lane < NW) {
-      const unsigned long long m = mask[(size_t)i * NW + lane];
-      unsigned long long fresh = m & ~removed;
-      removed |= m;
-      while (fresh) {
-        const int b = __ffsll((long long)fresh) - 1;
-        fresh &= fresh - 1;
-        const int rj = alive[lane * 64 + b];
-        status[rj] = -1;
-        first_sup[rj] = ri;
-      }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int k = threadIdx.x + u * 256;
+      if (k < n16) dst[k] = v[u];
+    }
+    int a[kNmsWindow / 256];
+#pragma unroll
+    for (int u = 0; u < kNmsWindow / 256; u++) {
+      const int i = threadIdx.x + u * 256;
+      if (i < W) a[u] = alive[i];
+    }
+#pragma unroll
+    for (int u = 0; u < kNmsWindow / 256; u++) {
+      const int i = threadIdx.x + u * 256;
+      if (i < W) salive[i] = a[u];
     }
   }
-  if (lane == 0) {
-    counters[0] = npick;        // picks so far
-    counters[1] = round_begin;  // first pick of this round
+  for (int i = threadIdx.x; i < kNmsWindow; i += blockDim.x) s_first[i] = -1;
+  __syncthreads();
+  if (phase_clock && threadIdx.x == 0) phase_clock[1] = wall_clock64();
+  const int round_begin = counters[0];
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    unsigned long long removed = 0;  // lane w < nblk: word w of the removed set
+    int np = 0;
+    for (int b = 0; b < nblk; b++) {
+      const int rows = min(64, W - 64 * b);
+      // lane j holds row (64b + j)'s word b: the block's diagonal
+      const unsigned long long diag = lane < rows ? smask[(size_t)b * kNmsWindow + 64 * b + lane] : 0ull;
+      const unsigned long long rem0 =
+          ((unsigned long long)(unsigned int)__builtin_amdgcn_readlane((int)(unsigned int)(removed >> 32), b) << 32) |
+          (unsigned long long)(unsigned int)__builtin_amdgcn_readlane((int)(unsigned int)removed, b);
+      const unsigned long long rowsmask = rows == 64 ? ~0ull : ((1ull << rows) - 1ull);
+      // alive rows of the block, decided in order: the lowest alive row is a pick and
+      // removes the rows its mask covers
+      unsigned long long cand = ~rem0 & rowsmask;
+      unsigned long long rem = rem0, picked = 0;
+      while (cand) {
+        const int i = __builtin_ctzll(cand);
+        const unsigned int hi = (unsigned int)__builtin_amdgcn_readlane((int)(unsigned int)(diag >> 32), i);
+        const unsigned int lo = (unsigned int)__builtin_amdgcn_readlane((int)(unsigned int)diag, i);
+        const unsigned long long m = ((unsigned long long)hi << 32) | (unsigned long long)lo;
+        picked |= 1ull << i;
+        const unsigned long long fresh = m & cand;  // first removed by row i
+        if (lane < rows && ((fresh >> lane) & 1ull)) s_first[64 * b + lane] = 64 * b + i;
+        cand &= ~(m | (1ull << i));
+        rem |= m;
+      }
+      if (lane == b) removed = rem;
+      // owners of the later words fold in the picked rows, in order
+      if (lane > b && lane < nblk) {
+        const unsigned long long *row = smask + (size_t)lane * kNmsWindow + 64 * b;
+        unsigned long long pk = picked;
+        while (pk) {
+          const int i = __builtin_ctzll(pk);
+          pk &= pk - 1;
+          const unsigned long long m = row[i];
+          unsigned long long fresh = m & ~removed;
+          removed |= m;
+          while (fresh) {
+            const int bit = __builtin_ctzll(fresh);
+            fresh &= fresh - 1;
+            s_first[64 * lane + bit] = 64 * b + i;
+          }
+        }
+      }
+      if (lane < rows && ((picked >> lane) & 1ull)) {
+        const unsigned long long below = lane ? (picked & ((1ull << lane) - 1ull)) : 0ull;
+        const int slot = np + __popcll(below);
+        const int ri = salive[64 * b + lane];
+        status[ri] = 1;
+        picks[round_begin + slot] = ri;
+      }
+      np += __popcll(picked);
+    }
+    if (lane == 0) {
+      counters[0] = round_begin + np;  // picks so far
+      counters[1] = round_begin;       // first pick of this round
+    }
+  }
+  __syncthreads();
+  if (phase_clock && threadIdx.x == 0) phase_clock[2] = wall_clock64();
+  // first suppressors of the window boxes removed in this round
+  for (int j = threadIdx.x; j < W; j += blockDim.x) {
+    const int f = s_first[j];
+    if (f >= 0) sup[salive[j]] = salive[f];
+  }
+  if (phase_clock) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      phase_clock[3] = wall_clock64();
+      for (int k = 0; k < 3; k++) phase_clock[4 + k] += phase_clock[k + 1] - phase_clock[k];
+      phase_clock[7] += 1;
+      phase_clock[8] += W;
+    }
   }
 }
 
-// every unchecked box after the window against this round's picks, in pick order
+// every unchecked box after the window against this round's picks, in pick order (generic
+// path for caller-supplied boxes)
 __global__ __launch_bounds__(256) void k_nms_suppress(const NmsBox *__restrict__ sbox,
                                                       const int *__restrict__ alive,
                                                       const int *__restrict__ nalive,
                                                       const int *__restrict__ picks,
                                                       const int *__restrict__ counters,
-                                                      float overlap, int *__restrict__ status,
-                                                      int *__restrict__ first_sup,
-                                                      int *__restrict__ keep) {
+                                                      float overlap, int *__restrict__ sup) {
   __shared__ NmsBox sp[kNmsWindow];
   __shared__ int sr[kNmsWindow];
   const int na = *nalive;
@@ -198,15 +300,123 @@ __global__ __launch_bounds__(256) void k_nms_suppress(const NmsBox *__restrict__
   if (t >= na) return;
   const int r = alive[t];
   const NmsBox b = sbox[r];
-  int st = 0;
   for (int k = 0; k < np; k++)
     if (suppresses(sp[k], b, overlap)) {
-      status[r] = -1;
-      first_sup[r] = sr[k];
-      st = -1;
+      sup[r] = sr[k];
       break;
     }
-  keep[t - W] = st == 0 ? 1 : 0;
+}
+
+// Map path: each pick scatters its suppression over the score-map cells whose root boxes
+// can intersect it (a conservative cell rectangle per map).  The workgroup lays the cells
+// of all maps end to end and every thread walks its share with a running map index, so the
+// rank loads are independent.  The candidate's box is rebuilt from its cell with the prep
+// kernel's arithmetic and the exact overlap test decides; the first suppressor is the
+// minimum pick rank.
+__global__ __launch_bounds__(256) void k_nms_scatter(const NmsBox *__restrict__ sbox,
+                                                     const int *__restrict__ picks,
+                                                     const int *__restrict__ counters,
+                                                     const ScoreMap *__restrict__ maps,
+                                                     int nmaps, const int *__restrict__ rsize,
+                                                     int padx, int pady,
+                                                     const int *__restrict__ rank_of_flat,
+                                                     float overlap, int *__restrict__ sup) {
+  __shared__ int s_xa[kNmsMaxMaps], s_ya[kNmsMaxMaps], s_h[kNmsMaxMaps], s_R0[kNmsMaxMaps];
+  __shared__ long long s_off[kNmsMaxMaps];
+  __shared__ float s_s[kNmsMaxMaps], s_fh[kNmsMaxMaps], s_fw[kNmsMaxMaps];
+  __shared__ int s_beg[kNmsMaxMaps + 1];
+  __shared__ int s_part[256];
+  const int p0 = counters[1], p1 = counters[0];
+  if ((int)blockIdx.x >= p1 - p0) return;
+  const int rp = picks[p0 + blockIdx.x];
+  const NmsBox P = sbox[rp];
+  const int tid = threadIdx.x;
+  // per-map rectangles; thread t owns maps [t*per, (t+1)*per)
+  const int per = (nmaps + 255) / 256;
+  int mycnt = 0;
+  for (int k = tid * per; k < min(nmaps, (tid + 1) * per); k++) {
+    const ScoreMap m = maps[k];
+    const float s = m.scale;
+    const int rh = rsize[m.comp * 2], rw = rsize[m.comp * 2 + 1];
+    const int xa = max((int)floorf(P.x1 / s) - rw + padx - 2, 0);
+    const int xb = min((int)ceilf((P.x2 + 1.0f) / s) + padx + 1, m.R1 - 1);
+    const int ya = max((int)floorf(P.y1 / s) - rh + pady - 2, 0);
+    const int yb = min((int)ceilf((P.y2 + 1.0f) / s) + pady + 1, m.R0 - 1);
+    const int h = yb - ya + 1, w = xb - xa + 1;
+    s_xa[k] = xa;
+    s_ya[k] = ya;
+    s_h[k] = h > 0 ? h : 1;
+    s_R0[k] = m.R0;
+    s_off[k] = m.score_off;
+    s_s[k] = s;
+    s_fh[k] = (float)rh * s;
+    s_fw[k] = (float)rw * s;
+    s_beg[k] = (h > 0 && w > 0) ? w * h : 0;  // count for now
+    mycnt += s_beg[k];
+  }
+  s_part[tid] = mycnt;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {  // inclusive scan of per-thread counts
+    const int v = tid >= o ? s_part[tid - o] : 0;
+    __syncthreads();
+    s_part[tid] += v;
+    __syncthreads();
+  }
+  int run = s_part[tid] - mycnt;
+  for (int k = tid * per; k < min(nmaps, (tid + 1) * per); k++) {
+    const int c = s_beg[k];
+    s_beg[k] = run;
+    run += c;
+  }
+  if (tid == 255) s_beg[nmaps] = s_part[255];
+  __syncthreads();
+  const int total = s_beg[nmaps];
+  const int stride = 256 * gridDim.y;
+  constexpr int B = 4;  // cells per batch: their rank loads are in flight together
+  int k = 0;
+  for (int base = blockIdx.y * 256 + tid; base < total; base += B * stride) {
+    int kk[B], xs[B], ys[B], rr[B];
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      const int idx = base + u * stride;
+      rr[u] = -1;
+      if (idx < total) {
+        while (s_beg[k + 1] <= idx) k++;
+        const int off = idx - s_beg[k];
+        const int h = s_h[k];
+        const int xo = off / h;
+        kk[u] = k;
+        xs[u] = s_xa[k] + xo;
+        ys[u] = s_ya[k] + (off - xo * h);
+        rr[u] = rank_of_flat[s_off[k] + (long long)xs[u] * s_R0[k] + ys[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      const int r = rr[u];
+      if (r <= rp) continue;  // not a candidate (-1), or ranked before the pick
+      const int km = kk[u];
+      const float s = s_s[km];
+      NmsBox b;
+      b.y1 = ((float)ys[u] - (float)pady + 1.0f) * s;
+      b.x1 = ((float)xs[u] - (float)padx + 1.0f) * s;
+      b.y2 = (b.y1 + s_fh[km]) - 1.0f;  // == (float)((double)(y1 + rh*s) - 1.0)
+      b.x2 = (b.x1 + s_fw[km]) - 1.0f;
+      b.area = (float)(((double)(b.x2 - b.x1) + 1.0) * ((double)(b.y2 - b.y1) + 1.0));
+      if (suppresses(P, b, overlap)) atomicMin(&sup[r], rp);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_nms_aliveflags(const int *__restrict__ alive,
+                                                        const int *__restrict__ nalive,
+                                                        const int *__restrict__ sup,
+                                                        int *__restrict__ keep) {
+  const int na = *nalive;
+  const int W = min(na, kNmsWindow);
+  const int t = W + blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= na) return;
+  keep[t - W] = sup[alive[t]] == kNoSup ? 1 : 0;
 }
 
 __global__ __launch_bounds__(256) void k_nms_regather(const int *__restrict__ alive,
@@ -221,17 +431,21 @@ __global__ __launch_bounds__(256) void k_nms_regather(const int *__restrict__ al
 // finalisation: conflicts, last-pick emission, survivors in original order
 __global__ __launch_bounds__(256) void k_nms_check(const NmsBox *__restrict__ sbox, int n,
                                                    const int *__restrict__ status,
-                                                   const int *__restrict__ first_sup,
+                                                   const int *__restrict__ sup,
                                                    const int *__restrict__ picks, int npick,
                                                    int *__restrict__ flags) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n || npick == 0) return;
   const int last = picks[npick - 1];
-  if (status[r] == -1) {
-    const int f = first_sup[r];
-    if (sbox[f].score == sbox[r].score) atomicOr(&flags[0], 1);  // tie decided the outcome
+  if (status[r] != 1) {
+    const int f = sup[r];
+    if (f == kNoSup) {
+      atomicOr(&flags[0], 4);  // undecided box: cannot happen; forces the exact host path
+      return;
+    }
+    if (sbox[f].score == sbox[r].score) atomicOr(&flags[0], 1);  // a tie decided the outcome
     if (f == last) atomicOr(&flags[1], 1);                      // last pick suppressed something
-  } else if (status[r] == 1 && r != last && sbox[r].score == sbox[last].score) {
+  } else if (r != last && sbox[r].score == sbox[last].score) {
     atomicOr(&flags[0], 2);  // the identity of the last pick depends on the tie order
   }
 }
@@ -260,50 +474,67 @@ __global__ __launch_bounds__(256) void k_take(const int *__restrict__ src,
 
 size_t nms_sort_temp_bytes(int n) {
   size_t bytes = 0;
-  unsigned long long *k = nullptr;
-  (void)rocprim::radix_sort_keys(nullptr, bytes, k, k, (unsigned int)n, 0, 64, 0);
+  unsigned int *k = nullptr;
+  int *v = nullptr;
+  (void)rocprim::radix_sort_pairs(nullptr, bytes, k, k, v, v, (unsigned int)n, 0, 32, 0);
   return bytes;
 }
 
 void launch_nms_prep(hipStream_t s, const ScoreMap *maps, int nmaps, const float *score,
                      const int *cand, int n, int padx, int pady, const int *rsize, NmsBox *box,
-                     unsigned long long *keys) {
+                     unsigned int *keys, int *vals) {
   hipLaunchKernelGGL(k_nms_prep, dim3((n + 255) / 256), dim3(256), 0, s, maps, nmaps, score, cand,
-                     n, padx, pady, rsize, box, keys);
+                     n, padx, pady, rsize, box, keys, vals);
 }
 
 void launch_nms_prep_rows(hipStream_t s, const float *rows, int n, int GL, NmsBox *box,
-                          unsigned long long *keys) {
+                          unsigned int *keys, int *vals) {
   hipLaunchKernelGGL(k_nms_prep_rows, dim3((n + 255) / 256), dim3(256), 0, s, rows, n, GL, box,
-                     keys);
+                     keys, vals);
 }
 
-int launch_nms_sort(hipStream_t s, void *temp, size_t temp_bytes, unsigned long long *keys_in,
-                    unsigned long long *keys_out, int n) {
+int launch_nms_sort(hipStream_t s, void *temp, size_t temp_bytes, unsigned int *keys_in,
+                    unsigned int *keys_out, int *vals_in, int *vals_out, int n) {
   size_t bytes = temp_bytes;
-  return rocprim::radix_sort_keys(temp, bytes, keys_in, keys_out, (unsigned int)n, 0, 64, s) ==
-                 hipSuccess
+  return rocprim::radix_sort_pairs(temp, bytes, keys_in, keys_out, vals_in, vals_out,
+                                   (unsigned int)n, 0, 32, s) == hipSuccess
              ? 0
              : -1;
 }
 
-void launch_nms_gather(hipStream_t s, const unsigned long long *keys, const NmsBox *box, int n,
-                       NmsBox *sbox, int *alive, int *status, int *first_sup) {
-  hipLaunchKernelGGL(k_nms_gather, dim3((n + 255) / 256), dim3(256), 0, s, keys, box, n, sbox,
-                     alive, status, first_sup);
+void launch_nms_gather(hipStream_t s, const int *ord, const NmsBox *box, int n, NmsBox *sbox,
+                       int *alive, int *status, int *sup, const int *cand, int *rank_of_flat) {
+  hipLaunchKernelGGL(k_nms_gather, dim3((n + 255) / 256), dim3(256), 0, s, ord, box, n, sbox,
+                     alive, status, sup, cand, rank_of_flat);
 }
 
 void launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const int *nalive,
                       int nalive_host, float overlap, unsigned long long *mask, int *status,
-                      int *first_sup, int *picks, int *counters, int *keep) {
+                      int *sup, int *picks, int *counters, int *keep, const NmsMaps *maps) {
   const int W = nalive_host < kNmsWindow ? nalive_host : kNmsWindow;
   hipLaunchKernelGGL(k_nms_winmask, dim3(W), dim3(64), 0, s, sbox, alive, nalive, overlap, mask);
-  hipLaunchKernelGGL(k_nms_wingreedy, dim3(1), dim3(64), 0, s, alive, nalive, mask, status,
-                     first_sup, picks, counters);
+  static bool attr = false;
+  const size_t glds = sizeof(unsigned long long) * kNmsWindow * (kNmsWindow / 64) + 4 * kNmsWindow;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void *)k_nms_wingreedy,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)glds);
+    attr = true;
+  }
+  hipLaunchKernelGGL(k_nms_wingreedy, dim3(1), dim3(256), glds, s, alive, nalive, mask, status,
+                     sup, picks, counters, g_nms_phase_clock);
   const int rest = nalive_host - W;
-  if (rest > 0)
+  if (rest <= 0) return;
+  if (maps) {
+    hipLaunchKernelGGL(k_nms_scatter, dim3(W, kNmsScatterSplit), dim3(256), 0, s, sbox, picks,
+                       counters, maps->maps,
+                       maps->nmaps, maps->rsize, maps->padx, maps->pady, maps->rank_of_flat,
+                       overlap, sup);
+  } else {
     hipLaunchKernelGGL(k_nms_suppress, dim3((rest + 255) / 256), dim3(256), 0, s, sbox, alive,
-                       nalive, picks, counters, overlap, status, first_sup, keep);
+                       nalive, picks, counters, overlap, sup);
+  }
+  hipLaunchKernelGGL(k_nms_aliveflags, dim3((rest + 255) / 256), dim3(256), 0, s, alive, nalive,
+                     sup, keep);
 }
 
 void launch_nms_regather(hipStream_t s, const int *alive, const int *nalive, const int *sel,
@@ -313,10 +544,10 @@ void launch_nms_regather(hipStream_t s, const int *alive, const int *nalive, con
                        sel, nsel, alive_out);
 }
 
-void launch_nms_check(hipStream_t s, const NmsBox *sbox, int n, const int *status,
-                      const int *first_sup, const int *picks, int npick, int *flags) {
-  hipLaunchKernelGGL(k_nms_check, dim3((n + 255) / 256), dim3(256), 0, s, sbox, n, status,
-                     first_sup, picks, npick, flags);
+void launch_nms_check(hipStream_t s, const NmsBox *sbox, int n, const int *status, const int *sup,
+                      const int *picks, int npick, int *flags) {
+  hipLaunchKernelGGL(k_nms_check, dim3((n + 255) / 256), dim3(256), 0, s, sbox, n, status, sup,
+                     picks, npick, flags);
 }
 
 void launch_nms_keep(hipStream_t s, const NmsBox *sbox, const int *picks, int npick,

@@ -464,7 +464,8 @@ void plan_release(Plan &P) {
                           &P.n_box, &P.n_keys, &P.n_keys2, &P.n_sbox, &P.n_alive0, &P.n_alive1,
                           &P.n_status, &P.n_fsup, &P.n_picks, &P.n_counters, &P.n_mask,
                           &P.n_keep, &P.n_sel, &P.n_sort, &P.n_flags, &P.n_keep_orig,
-                          &P.n_kept, &P.n_kept_flat, &P.n_total})
+                          &P.n_kept, &P.n_kept_flat, &P.n_total, &P.n_vals, &P.n_ord,
+                          &P.n_rank})
     release(*b);
 }
 
@@ -619,8 +620,10 @@ int nms_buffers(Plan &P, int n) {
   const size_t N = (size_t)n;
   CHK(ensure(P.n_box, sizeof(NmsBox) * N));
   CHK(ensure(P.n_sbox, sizeof(NmsBox) * N));
-  CHK(ensure(P.n_keys, 8 * N));
-  CHK(ensure(P.n_keys2, 8 * N));
+  CHK(ensure(P.n_keys, 4 * N));
+  CHK(ensure(P.n_keys2, 4 * N));
+  CHK(ensure(P.n_vals, 4 * N));
+  CHK(ensure(P.n_ord, 4 * N));
   CHK(ensure(P.n_alive0, 4 * N));
   CHK(ensure(P.n_alive1, 4 * N));
   CHK(ensure(P.n_status, 4 * N));
@@ -640,34 +643,36 @@ int nms_buffers(Plan &P, int n) {
   return GEYE_OK;
 }
 
-// Greedy NMS over P.n_box / P.n_keys (filled by a prep kernel, original order); leaves the
-// survivors' original indices, ascending, in P.n_kept.
-int nms_core(geye_ctx *c, Plan &P, int n, float overlap, int *nkept) {
+// Greedy NMS over P.n_box / P.n_keys / P.n_vals (filled by a prep kernel, original order);
+// leaves the survivors' original indices, ascending, in P.n_kept.  maps (optional) enables
+// the scatter form of the suppression pass; cand/rank_of_flat go with it.
+int nms_core(geye_ctx *c, Plan &P, int n, float overlap, const NmsMaps *maps, const int *cand,
+             int *nkept) {
   hipStream_t s = c->stream;
   const size_t N = (size_t)n;
   *nkept = 0;
   if (n <= 0) return GEYE_OK;
   {
     KTimer t(c, "nms_sort");
-    if (launch_nms_sort(s, P.n_sort.p, P.n_sort.bytes, dp<unsigned long long>(P.n_keys),
-                        dp<unsigned long long>(P.n_keys2), n) != 0)
+    if (launch_nms_sort(s, P.n_sort.p, P.n_sort.bytes, dp<unsigned int>(P.n_keys),
+                        dp<unsigned int>(P.n_keys2), dp<int>(P.n_vals), dp<int>(P.n_ord), n) != 0)
       return fail(GEYE_ERR_HIP, "rocprim radix sort failed");
   }
-  launch_nms_gather(s, dp<unsigned long long>(P.n_keys2), dp<NmsBox>(P.n_box), n,
-                    dp<NmsBox>(P.n_sbox), dp<int>(P.n_alive0), dp<int>(P.n_status),
-                    dp<int>(P.n_fsup));
+  if (maps) launch_fill(s, maps->rank_of_flat, P.score_total, -1);
+  launch_nms_gather(s, dp<int>(P.n_ord), dp<NmsBox>(P.n_box), n, dp<NmsBox>(P.n_sbox),
+                    dp<int>(P.n_alive0), dp<int>(P.n_status), dp<int>(P.n_fsup), cand,
+                    maps ? maps->rank_of_flat : nullptr);
   launch_fill(s, dp<int>(P.n_counters), 4, 0);
-  const int nn = n;
-  HIPCHK(hipMemcpyAsync(P.n_total.p, &nn, sizeof(int), hipMemcpyHostToDevice, s));
-  int *alive = dp<int>(P.n_alive0), *alive_next = dp<int>(P.n_alive1);
-  int na = n;
   int *d_na = dp<int>(P.n_total);
   int *d_sel_total = dp<int>(P.n_total) + 1;
+  launch_fill(s, d_na, 1, n);
+  int *alive = dp<int>(P.n_alive0), *alive_next = dp<int>(P.n_alive1);
+  int na = n;
   while (na > 0) {
     KTimer t(c, "nms_round");
     launch_nms_round(s, dp<NmsBox>(P.n_sbox), alive, d_na, na, overlap,
                      dp<unsigned long long>(P.n_mask), dp<int>(P.n_status), dp<int>(P.n_fsup),
-                     dp<int>(P.n_picks), dp<int>(P.n_counters), dp<int>(P.n_keep));
+                     dp<int>(P.n_picks), dp<int>(P.n_counters), dp<int>(P.n_keep), maps);
     const int W = std::min(na, kNmsWindow);
     const int rest = na - W;
     if (rest <= 0) break;
@@ -695,6 +700,7 @@ int nms_core(geye_ctx *c, Plan &P, int n, float overlap, int *nkept) {
     // equal scores decided an outcome: replay the reference's exact quicksort order on the host
     P.nms_fallbacks++;
     c->nms_fallbacks++;
+    c->stats["nms_host_fallback"].count++;
     std::vector<NmsBox> hb(N);
     HIPCHK(hipMemcpyAsync(hb.data(), P.n_box.p, sizeof(NmsBox) * N, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -735,10 +741,18 @@ int run_nms(geye_ctx *c, Model &m, Plan &P, int n, float overlap, int *nkept) {
     KTimer t(c, "nms_prep");
     launch_nms_prep(c->stream, dp<ScoreMap>(P.d_smaps), (int)P.smaps.size(),
                     dp<float>(P.d_score), dp<int>(P.d_cand), n, m.padx, m.pady,
-                    dp<int>(m.d_rsize), dp<NmsBox>(P.n_box), dp<unsigned long long>(P.n_keys));
+                    dp<int>(m.d_rsize), dp<NmsBox>(P.n_box), dp<unsigned int>(P.n_keys),
+                    dp<int>(P.n_vals));
+  }
+  NmsMaps mp{};
+  const bool use_maps = (int)P.smaps.size() <= kNmsMaxMaps;
+  if (use_maps) {
+    CHK(ensure(P.n_rank, sizeof(int) * (size_t)std::max<long long>(P.score_total, 1)));
+    mp = NmsMaps{dp<ScoreMap>(P.d_smaps), (int)P.smaps.size(), dp<int>(m.d_rsize), m.padx,
+                 m.pady, dp<int>(P.n_rank)};
   }
   int nk = 0;
-  CHK(nms_core(c, P, n, overlap, &nk));
+  CHK(nms_core(c, P, n, overlap, use_maps ? &mp : nullptr, dp<int>(P.d_cand), &nk));
   launch_take(c->stream, dp<int>(P.d_cand), dp<int>(P.n_kept), nk, dp<int>(P.n_kept_flat));
   HIPCHK(hipGetLastError());
   *nkept = nk;
@@ -1028,6 +1042,37 @@ int geye_level_plan(const geye_model *gm, int width, int height, int *max_scale,
   return GEYE_OK;
 }
 
+int geye_level_plan_dims(const int *rsize, int numcomponent, int sbin, int interval, int width,
+                         int height, int *max_scale, int *LEN, float *scales, int *feat_h,
+                         int *feat_w) {
+  if (!rsize || numcomponent <= 0 || sbin <= 0 || interval <= 0 || !max_scale || !LEN)
+    return fail(GEYE_ERR_ARG, "bad argument");
+  Model m;
+  m.numcomponent = numcomponent;
+  m.sbin = sbin;
+  m.interval = interval;
+  m.rsize.assign(rsize, rsize + 2 * numcomponent);
+  const int ms = compute_max_scale(m, width, height);
+  std::vector<Level> lv;
+  std::vector<int> ih, iw, is;
+  std::vector<float> isc;
+  level_geometry(ms, sbin, interval, width, height, lv, ih, iw, is, isc);
+  *max_scale = ms;
+  *LEN = (int)lv.size();
+  for (size_t l = 0; l < lv.size(); l++) {
+    if (scales) scales[l] = lv[l].scale;
+    if (feat_h) feat_h[l] = lv[l].F0;
+    if (feat_w) feat_w[l] = lv[l].F1;
+  }
+  return GEYE_OK;
+}
+
+int geye_nms_host(const float *boxes, int num, int GL, float overlap, float *out, int *out_num) {
+  if (!out_num || (num > 0 && (!boxes || !out)) || GL < 7) return fail(GEYE_ERR_ARG, "bad argument");
+  *out_num = nms_host(boxes, num, GL, overlap, out);
+  return GEYE_OK;
+}
+
 int geye_get_boxes(geye_ctx *c, const geye_model *model, const unsigned char *bgr, int width,
                    int height, int widthStep, float thresh, int precision, float **boxes,
                    int *num, int *GL) {
@@ -1086,8 +1131,8 @@ int geye_nms(geye_ctx *c, const float *boxes, int num, int GL, float overlap, fl
   int nk = 0;
   if (rc == GEYE_OK) {
     launch_nms_prep_rows(c->stream, dp<float>(rows), num, GL, dp<NmsBox>(B.n_box),
-                         dp<unsigned long long>(B.n_keys));
-    rc = nms_core(c, B, num, overlap, &nk);
+                         dp<unsigned int>(B.n_keys), dp<int>(B.n_vals));
+    rc = nms_core(c, B, num, overlap, nullptr, nullptr, &nk);
   }
   std::vector<int> kept(nk);
   if (rc == GEYE_OK && nk > 0) {

@@ -79,6 +79,17 @@ int geye_model_info(const geye_model *model, int *numcomponent, int *sbin, int *
 int geye_level_plan(const geye_model *model, int width, int height, int *max_scale, int *LEN,
                     float *scales, int *feat_h, int *feat_w);
 
+/* Host-only level plan from the root sizes (no device work): same outputs as
+ * geye_level_plan (ini_scales CT/featurepyramid.cpp.new2:174-239, calc_f_pyramid sizes
+ * CT/featurepyramid.cpp.org:570-630).  Arrays may be null; capacity >= LEN. */
+int geye_level_plan_dims(const int *rsize, int numcomponent, int sbin, int interval, int width,
+                         int height, int *max_scale, int *LEN, float *scales, int *feat_h,
+                         int *feat_w);
+
+/* Host-only NMS with the reference's exact quicksort order (CT/nms.cpp:70-217); the device
+ * path falls back to it when equal scores could change the outcome. */
+int geye_nms_host(const float *boxes, int num, int GL, float overlap, float *out, int *out_num);
+
 /* ---- whole-frame detection -------------------------------------------------------------
  * geye_get_boxes: replaces detect() + get_boxes() (CT/detect.cpp:151-226,
  *   CT/get_boxes.cpp:583-1797): every candidate above thresh, rows of GL floats

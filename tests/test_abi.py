@@ -1,0 +1,78 @@
+"""CPU tests of the C ABI library (no GPU needed).
+
+The library must load, export every entry point include/geye_dpm.h declares, and its
+host-only parts (the level plan, the host NMS the device path falls back to, argument
+checks) must agree with the oracle.  No compute call here touches a device.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import geye_amd
+from geye_amd import synth
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "geye_dpm.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(geye_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = geye_amd.lib()
+    syms = header_symbols()
+    assert len(syms) >= 28
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    # the Python mirror binds the same set
+    assert sorted(geye_amd.exported_symbols()) == syms
+
+
+def test_error_path_without_device_work():
+    L = geye_amd.lib()
+    rc = L.geye_level_plan_dims(None, 0, 8, 10, 640, 480, None, None, None, None, None)
+    assert rc == -1
+    assert b"" != L.geye_last_error()
+
+
+@pytest.mark.parametrize("W,H", [(640, 480), (1280, 720), (1920, 1080), (3840, 2160), (200, 150),
+                                 (90, 400)])
+def test_level_plan_matches_oracle(car_model, W, H):
+    for m in (car_model, synth.synthetic_person(), synth.tiny_model()):
+        a = geye_amd.level_plan_dims(m.rsize, m.sbin, m.interval, W, H)
+        b = O.level_plan(m, W, H)
+        assert (a["max_scale"], a["LEN"]) == (b["max_scale"], b["LEN"])
+        assert np.array_equal(a["scales"].view(np.int32), b["scales"].view(np.int32))
+        assert np.array_equal(a["feat_h"], b["feat_h"]) and np.array_equal(a["feat_w"], b["feat_w"])
+
+
+def _random_rows(rng, n, GL=11, tie=False):
+    x1 = rng.integers(0, 600, n).astype(np.float32)
+    y1 = rng.integers(0, 400, n).astype(np.float32)
+    w = rng.integers(8, 120, n).astype(np.float32)
+    h = rng.integers(8, 120, n).astype(np.float32)
+    r = np.zeros((n, GL), np.float32)
+    r[:, 0], r[:, 1], r[:, 2], r[:, 3] = x1, y1, x1 + w, y1 + h
+    r[:, 4:GL - 2] = rng.random((n, GL - 6))
+    s = rng.standard_normal(n).astype(np.float32)
+    r[:, GL - 2] = np.round(s * 2) / 2 if tie else s
+    r[:, GL - 1] = rng.random(n).astype(np.float32)
+    return r
+
+
+@pytest.mark.parametrize("n,tie,overlap", [(0, False, 0.5), (1, False, 0.5), (2, False, 0.5),
+                                           (50, False, 0.5), (400, False, 0.3),
+                                           (400, True, 0.5), (3000, True, 0.5)])
+def test_host_nms_matches_oracle(n, tie, overlap):
+    rng = np.random.default_rng(n + int(tie))
+    rows = _random_rows(rng, n, tie=tie)
+    got = geye_amd.nms_host(rows, overlap) if n else np.zeros((0, 11), np.float32)
+    exp = O.nms(rows, overlap) if n else np.zeros((0, 11), np.float32)
+    assert got.shape == exp.shape
+    assert np.array_equal(got.view(np.int32), exp.view(np.int32))

@@ -1,0 +1,355 @@
+"""CPU tests: pin the oracle before trusting it.
+
+Each stage of the oracle (oracle/dpm_oracle.cpp) is checked against, in order of strength:
+the reference's own code compiled from the reference tree into oracle/_ref (resize.cc,
+dt.cpp; skipped where that tree is absent), golden vectors that the same build wrote into
+tests/golden/, and closed forms / brute force that a plausible transcription mistake (a
+dropped term, a swapped axis, an off-by-one) would break.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from geye_amd import synth
+from oracle import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.int32)
+
+
+# ---------------------------------------------------------------------------------------
+# resize (CT/resize.cc)
+# ---------------------------------------------------------------------------------------
+def test_resize_golden_vectors():
+    z = np.load(os.path.join(GOLDEN, "resize_ref.npz"), allow_pickle=False)
+    for k, s in enumerate(z["scales"]):
+        assert np.array_equal(bits(O.resize(z["img"], float(s))), bits(z[f"out{k}"])), float(s)
+
+
+@pytest.mark.skipif(O.ref_lib() is None, reason="oracle/_ref (reference tree) not built")
+@pytest.mark.parametrize("W,H,s", [(97, 131, 0.707107), (160, 120, 0.93303), (33, 29, 0.5),
+                                   (211, 157, 0.535887), (64, 48, 0.870551)])
+def test_resize_matches_reference_build(W, H, s):
+    img = O.ipl_to_float(synth.make_frame(W, H, seed=W + H))
+    assert np.array_equal(bits(O.resize(img, s)), bits(O.resize(img, s, use_ref=True)))
+
+
+def test_resize_identity_and_halving():
+    img = O.ipl_to_float(synth.make_frame(40, 30, seed=2))
+    assert np.array_equal(O.resize(img, 1.0), img)
+    half = O.resize(img, 0.5)
+    # 2:1 area average of integer pixels is exact in float
+    exp = 0.25 * (img[:, 0::2, 0::2] + img[:, 1::2, 0::2] + img[:, 0::2, 1::2] + img[:, 1::2, 1::2])
+    assert half.shape == (3, 20, 15)
+    np.testing.assert_allclose(half, exp, rtol=0, atol=1e-4)
+
+
+def test_resize_constant_image():
+    img = np.full((3, 53, 37), 117.0, np.float32)
+    out = O.resize(img, 0.6)
+    np.testing.assert_allclose(out, 117.0, rtol=2e-6)
+
+
+def test_resize_table_weights_sum_to_one():
+    for sh, dh in [(100, 93), (131, 93), (47, 23), (480, 448)]:
+        si, di, al = O.resize_table(sh, dh, 1)
+        sums = np.zeros(dh)
+        np.add.at(sums, di, al.astype(np.float64))
+        np.testing.assert_allclose(sums, 1.0, atol=5e-5)  # float32 weights of the reference
+        assert si.min() >= 0 and si.max() < sh
+
+
+# ---------------------------------------------------------------------------------------
+# HOG features (CT/featurepyramid.cpp.new2:263-501)
+# ---------------------------------------------------------------------------------------
+def test_calc_feature_constant_image_is_zero():
+    img = np.full((3, 48, 40), 90.0, np.float32)
+    f = O.calc_feature(img, 8)
+    assert f.shape == (31, 4, 3)
+    assert np.all(f == 0.0)
+
+
+def test_calc_feature_vertical_edge_single_orientation():
+    # intensity steps along x only: dx > 0, dy = 0 -> orientation 0 (cos 1, sin 0)
+    img = np.zeros((3, 64, 64), np.float32)
+    img[:, 32:, :] = 100.0
+    f = O.calc_feature(img, 8)
+    contrast = f[:18]
+    active = np.nonzero(contrast.reshape(18, -1).max(axis=1) > 0)[0]
+    assert list(active) == [0]
+    # insensitive bin 0 collects orientations 0 and 9
+    assert f[18].max() > 0 and np.all(f[19:27] == 0)
+
+
+def test_calc_feature_horizontal_edge_orientation_4():
+    # steps along y only: dx = 0, dy > 0 -> best |Hsin|: o = 4 (0.9848) ties o = 5 (0.9848);
+    # the strict > keeps the first, so bin 4
+    img = np.zeros((3, 64, 64), np.float32)
+    img[:, :, 32:] = 50.0
+    f = O.calc_feature(img, 8)
+    active = np.nonzero(f[:18].reshape(18, -1).max(axis=1) > 0)[0]
+    assert list(active) == [4]
+
+
+def test_calc_feature_bounds_and_texture():
+    img = O.ipl_to_float(synth.make_frame(96, 80, seed=6))
+    f = O.calc_feature(img, 8)
+    # each h is truncated at 0.2: sensitive/insensitive features <= 0.5 * 4 * 0.2
+    assert f[:27].max() <= 0.4 + 1e-7 and f.min() >= 0
+    # texture features are 0.2357 * sum of 18 truncated values <= 0.2357 * 18 * 0.2
+    assert f[27:].max() <= 0.2357 * 18 * 0.2 + 1e-6
+
+
+def test_calc_feature_matches_float64_brute_force():
+    """Independent numpy restatement in float64 on a tiny image: the float32 oracle agrees
+    to rounding (this pins indexing: axes, cell assignment, 2x2 normaliser windows)."""
+    img = O.ipl_to_float(synth.make_frame(40, 32, seed=12)).astype(np.float64)
+    sbin = 8
+    _, w, h = img.shape
+    b0, b1 = int(math.floor(h / sbin + 0.5)), int(math.floor(w / sbin + 0.5))
+    hist = np.zeros((18, b1, b0))
+    hc = np.array([1.0000, 0.9397, 0.7660, 0.5000, 0.1736, -0.1736, -0.5000, -0.7660, -0.9397])
+    hs = np.array([0.0000, 0.3420, 0.6428, 0.8660, 0.9848, 0.9848, 0.8660, 0.6428, 0.3420])
+    for x in range(1, b1 * sbin - 1):
+        for y in range(1, b0 * sbin - 1):
+            xc, yc = min(x, w - 2), min(y, h - 2)
+            best = None
+            for c in range(3):
+                dy = img[c, xc, yc + 1] - img[c, xc, yc - 1]
+                dx = img[c, xc + 1, yc] - img[c, xc - 1, yc]
+                v = dx * dx + dy * dy
+                if best is None or v > best[0]:
+                    best = (v, dx, dy)
+            v, dx, dy = best
+            dots = hc * dx + hs * dy
+            o = int(np.argmax(np.abs(dots)))
+            o = o if dots[o] >= 0 else o + 9
+            if abs(dots).max() == 0:
+                o = 0
+            xp, yp = (x + 0.5) / sbin - 0.5, (y + 0.5) / sbin - 0.5
+            ix, iy = math.floor(xp), math.floor(yp)
+            vx0, vy0 = xp - ix, yp - iy
+            v = math.sqrt(v)
+            for cx, wx in ((ix, 1 - vx0), (ix + 1, vx0)):
+                for cy, wy in ((iy, 1 - vy0), (iy + 1, vy0)):
+                    if 0 <= cx < b1 and 0 <= cy < b0:
+                        hist[o, cx, cy] += wx * wy * v
+    norm = ((hist[:9] + hist[9:]) ** 2).sum(axis=0)
+    F0, F1 = b0 - 2, b1 - 2
+    exp = np.zeros((31, F1, F0))
+    for x in range(F1):
+        for y in range(F0):
+            ns = []
+            for ox, oy in ((1, 1), (1, 0), (0, 1), (0, 0)):
+                s = norm[x + ox:x + ox + 2, y + oy:y + oy + 2].sum()
+                ns.append(1.0 / math.sqrt(s + 1e-4))
+            hh = hist[:, x + 1, y + 1]
+            t = np.minimum(np.outer(hh, ns), 0.2)
+            exp[:18, x, y] = 0.5 * t.sum(axis=1)
+            t9 = np.minimum(np.outer(hh[:9] + hh[9:], ns), 0.2)
+            exp[18:27, x, y] = 0.5 * t9.sum(axis=1)
+            exp[27:, x, y] = 0.2357 * t.sum(axis=0)
+    got = O.calc_feature(img.astype(np.float32), sbin)
+    np.testing.assert_allclose(got, exp, rtol=0, atol=2e-5)
+
+
+# ---------------------------------------------------------------------------------------
+# convolution (CT/GPU_function.cu:302-358)
+# ---------------------------------------------------------------------------------------
+def _conv_brute(feat, filt, h, w):
+    _, F1, F0 = feat.shape
+    B = filt.reshape(31, w, h).astype(np.float64)
+    A = feat[:31].astype(np.float64)
+    out = np.zeros((F1 - w + 1, F0 - h + 1))
+    for xp in range(w):
+        for yp in range(h):
+            out += np.einsum("fxy,f->xy", A[:, xp:xp + F1 - w + 1, yp:yp + F0 - h + 1], B[:, xp, yp])
+    return out
+
+
+@pytest.mark.parametrize("h,w", [(6, 6), (6, 13), (9, 6), (3, 2)])
+def test_conv_matches_brute_force(h, w):
+    rng = np.random.default_rng(h + 10 * w)
+    feat = rng.random((31, 30, 25), dtype=np.float32)
+    filt = rng.standard_normal(31 * h * w).astype(np.float32)
+    exp = _conv_brute(feat, filt, h, w)
+    np.testing.assert_allclose(O.conv(feat, filt, h, w), exp, rtol=1e-5, atol=1e-4)
+    np.testing.assert_allclose(O.conv(feat, filt, h, w, f64=True), exp, rtol=1e-12, atol=1e-10)
+
+
+def test_conv_one_hot_filter_picks_channel_and_offset():
+    rng = np.random.default_rng(5)
+    feat = rng.random((31, 20, 17), dtype=np.float32)
+    h, w, f, xp, yp = 4, 5, 12, 3, 1
+    filt = np.zeros((31, w, h), np.float32)
+    filt[f, xp, yp] = 1.0
+    out = O.conv(feat, filt.ravel(), h, w)
+    assert np.array_equal(out, feat[f, xp:xp + 20 - w + 1, yp:yp + 17 - h + 1])
+
+
+# ---------------------------------------------------------------------------------------
+# distance transform (CT/dt.cpp)
+# ---------------------------------------------------------------------------------------
+def test_dt_golden_vectors():
+    z = np.load(os.path.join(GOLDEN, "dt_ref.npz"), allow_pickle=False)
+    for k in range(int(z["n"])):
+        M, Ix, Iy = O.dt(z[f"in{k}"], *[float(x) for x in z[f"def{k}"]])
+        assert np.array_equal(bits(M), bits(z[f"M{k}"])), k
+        assert np.array_equal(Ix, z[f"Ix{k}"]) and np.array_equal(Iy, z[f"Iy{k}"]), k
+
+
+@pytest.mark.skipif(O.ref_lib() is None, reason="oracle/_ref (reference tree) not built")
+@pytest.mark.parametrize("d0,d1,defs,seed", [(31, 27, (0.01, 0.0, 0.01, 0.0), 1),
+                                            (70, 9, (0.03, 0.02, 0.05, -0.01), 2),
+                                            (1, 40, (0.2, 0.1, 0.2, 0.1), 3),
+                                            (50, 50, (0.0, 0.0, 0.0, 0.0), 4)])
+def test_dt_matches_reference_build(d0, d1, defs, seed):
+    rng = np.random.default_rng(seed)
+    v = np.round(rng.standard_normal((d1, d0)) * 3).astype(np.float32)  # many exact ties
+    M, Ix, Iy = O.dt(v, *defs)
+    Mr, Ixr, Iyr = O.dt(v, *defs, use_ref=True)
+    assert np.array_equal(bits(M), bits(Mr)) and np.array_equal(Ix, Ixr) and np.array_equal(Iy, Iyr)
+
+
+def test_dt_matches_brute_force_minimum():
+    """For convex costs the divide-and-conquer minimum equals the O(n^2) minimum."""
+    rng = np.random.default_rng(9)
+    v = rng.standard_normal((13, 17)).astype(np.float64)
+    ax, bx, ay, by = 0.05, 0.01, 0.07, -0.02
+    M, Ix, Iy = O.dt(v, ax, bx, ay, by, f64=True)
+    d1, d0 = v.shape
+    for x in range(d1):
+        for y in range(d0):
+            best = min(v[xs, ys] + ax * (x - xs) ** 2 + bx * (x - xs) + ay * (y - ys) ** 2 + by * (y - ys)
+                       for xs in range(d1) for ys in range(d0))
+            assert abs(M[x, y] - best) < 1e-9
+            xs, ys = Ix[x, y], Iy[x, y]
+            cost = v[xs, ys] + ax * (x - xs) ** 2 + bx * (x - xs) + ay * (y - ys) ** 2 + by * (y - ys)
+            assert abs(cost - best) < 1e-9
+
+
+def test_dt_large_coefficients_is_identity():
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal((8, 11)).astype(np.float32)
+    M, Ix, Iy = O.dt(v, 100.0, 0.0, 100.0, 0.0)
+    xs, ys = np.meshgrid(np.arange(8), np.arange(11), indexing="ij")
+    assert np.array_equal(Ix, xs) and np.array_equal(Iy, ys) and np.array_equal(M, v)
+
+
+def test_dt_zero_coefficients_first_global_minimum():
+    v = np.array([[3, 1, 2], [1, 5, 1]], np.float32)  # dims1 = 2 columns of dims0 = 3
+    M, Ix, Iy = O.dt(v, 0.0, 0.0, 0.0, 0.0)
+    assert np.all(M == 1.0)
+
+
+# ---------------------------------------------------------------------------------------
+# NMS (CT/nms.cpp) -- the reference file needs OpenCV headers and is not built here
+# ---------------------------------------------------------------------------------------
+def _rows(boxes, scores, GL=7):
+    r = np.zeros((len(boxes), GL), np.float32)
+    r[:, :4] = np.array(boxes, np.float32)
+    r[:, GL - 2] = scores
+    return r
+
+
+def test_nms_single_candidate_yields_nothing():
+    assert O.nms(_rows([[0, 0, 10, 10]], [1.0]), 0.5).shape[0] == 0
+
+
+def test_nms_last_survivor_is_dropped():
+    # two disjoint boxes: the loop stops with one unchecked box left, which is never emitted
+    out = O.nms(_rows([[0, 0, 9, 9], [100, 100, 109, 109]], [1.0, 2.0]), 0.5)
+    assert out.shape[0] == 1 and out[0, 5] == 2.0
+
+
+def test_nms_min_area_overlap_and_original_order():
+    rows = _rows([[0, 0, 99, 99],      # big, score 1
+                  [10, 10, 19, 19],    # inside the big one: inter/min area = 1 > 0.5
+                  [300, 300, 309, 309],
+                  [500, 0, 509, 9],
+                  [200, 0, 209, 9]], [1.0, 3.0, 2.0, 0.5, 0.7])
+    out = O.nms(rows, 0.5)
+    # order by score: 1 (picks, suppresses 0), 2, 4, 3 -> 4 picks with 1 left -> 3 dropped
+    assert [int(round(v)) for v in out[:, 5] * 10] == [30, 20, 7]
+
+
+def _quicksort_py(a, order, first, last):
+    i, j = first, last
+    key = a[(first + last) // 2]
+    while True:
+        while a[i] > key:
+            i += 1
+        while a[j] < key:
+            j -= 1
+        if i >= j:
+            break
+        a[i], a[j] = a[j], a[i]
+        order[i], order[j] = order[j], order[i]
+        i += 1
+        j -= 1
+    if first < i - 1:
+        _quicksort_py(a, order, first, i - 1)
+    if last > j + 1:
+        _quicksort_py(a, order, j + 1, last)
+
+
+def test_quicksort_tie_order_matches_independent_port():
+    rng = np.random.default_rng(21)
+    for n in (1, 2, 7, 50, 300):
+        s = np.round(rng.standard_normal(n) * 2).astype(np.float32)  # many ties
+        a, o = list(s), list(range(n))
+        _quicksort_py(a, o, 0, n - 1)
+        sa, so = O.quicksort(s)
+        assert np.array_equal(sa, np.array(a, np.float32)) and list(so) == o
+
+
+# ---------------------------------------------------------------------------------------
+# level plan and model loader
+# ---------------------------------------------------------------------------------------
+def test_level_plan_known_configs(car_model):
+    p = O.level_plan(car_model, 640, 480)
+    assert (p["max_scale"], p["LEN"]) == (32, 42)
+    car = synth.synthetic_car()
+    assert O.level_plan(car, 1920, 1080)["LEN"] == 46
+    assert O.level_plan(car, 1280, 720)["LEN"] == 46
+    assert O.level_plan(synth.synthetic_person(), 1280, 720)["LEN"] == 39
+    # feature size formula F = floor(h / sbin + 0.5) - 2 on the level image
+    assert p["feat_h"][10] == int(math.floor(480 / 8 + 0.5)) - 2
+    assert p["feat_w"][0] == int(math.floor(640 / 4 + 0.5)) - 2
+
+
+def test_car_model_known_values(car_model):
+    m = car_model
+    assert (m.numcomponent, m.sbin, m.interval, m.max_Y, m.max_X) == (2, 8, 10, 6, 13)
+    assert m.offw[0] == np.float32(-4.058444)
+    assert m.rsize[0].tolist() == [6, 13] and m.root_size[0].tolist() == [6, 13, 31]
+    np.testing.assert_array_equal(m.deff[0], np.float32([0.023062, 0.010890, 0.015328, -0.009441]))
+    assert m.anchor[0].tolist() == [1, 4] and m.NoP == 12
+
+
+@pytest.mark.skipif(not os.path.isdir(O.REF_DIR), reason="reference tree absent")
+def test_csv_loader_matches_fixture(car_model):
+    m = O.load_csv_model(os.path.join(O.REF_DIR, "car_comp.csv"),
+                         os.path.join(O.REF_DIR, "car_root.csv"),
+                         os.path.join(O.REF_DIR, "car_part.csv"))
+    for a, b in zip(m.partfilters, car_model.partfilters):
+        assert np.array_equal(a, b)
+
+
+def test_detect_tiny_model_candidate_invariants():
+    m = synth.tiny_model()
+    frame = synth.make_frame(160, 120, seed=4)
+    rows = O.detect(m, frame, -0.5)
+    assert rows.shape[1] == (m.numpart[0] + 1) * 4 + 3
+    assert np.all(rows[:, -2] > -0.5)                     # threshold is strict
+    assert np.all(rows[:, 2] >= rows[:, 0]) and np.all(rows[:, 3] >= rows[:, 1])
+    # FP64 scoring agrees with FP32 to rounding on the shared candidates
+    maps32 = O.score_maps(m, frame, 0)
+    maps64 = O.score_maps(m, frame, 1)
+    for (_, _, a), (_, _, b) in zip(maps32, maps64):
+        np.testing.assert_allclose(a, b, atol=2e-4)
