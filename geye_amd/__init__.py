@@ -76,6 +76,8 @@ _SIGS = [
     ("geye_detect_dev", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                   C.c_float, C.c_float, C.c_int, C.POINTER(_fp), _ip, _ip]),
     ("geye_last_scores", C.c_int, [C.c_void_p, _fp, C.c_longlong, _llp]),
+    ("geye_accumulated_score", C.c_int, [C.c_void_p, C.c_float, _fp]),
+    ("geye_accumulated_score_dev", C.c_int, [C.c_void_p, C.c_float, C.c_void_p]),
     ("geye_free", None, [C.c_void_p]),
     ("geye_resize", C.c_int, [C.c_void_p, _fp, _ip, C.c_float, _fp, _ip]),
     ("geye_calc_feature", C.c_int, [C.c_void_p, _fp, C.c_int, C.c_int, C.c_int, _fp, _ip, _ip]),
@@ -398,6 +400,18 @@ class Model:
                    precision: str = "fp32") -> np.ndarray:
         return self._rows(lib().geye_detect_dev, self.ctx.h, self.h, d_frame, W, H, 3 * W,
                           C.c_float(thresh), C.c_float(overlap), PRECISIONS[precision])
+
+    def accumulated_score(self, W: int, H: int, thresh: float = -0.5,
+                          a_score: np.ndarray | None = None) -> np.ndarray:
+        """A_SCORE of the last detection on this context (get_boxes' side output).
+
+        a_score: the caller's map as [W][H] float32 (column-major image), initialised to
+        -100 like ini_ac_score (CT/detect.cpp:136-145) when not given; updated in place."""
+        if a_score is None:
+            a_score = np.full((W, H), -100.0, np.float32)
+        assert a_score.dtype == np.float32 and a_score.shape == (W, H) and a_score.flags.c_contiguous
+        _check(lib().geye_accumulated_score(self.ctx.h, C.c_float(thresh), _p(a_score, _fp)))
+        return a_score
 
     def last_scores(self) -> np.ndarray:
         total = C.c_longlong()

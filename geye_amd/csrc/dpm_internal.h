@@ -82,6 +82,8 @@ struct ScoreMap {              // one (valid level, component) root score map
   int part_first;              // index into ScorePart[]
   float off;                   // offw[ridx[comp]]
   float scale;                 // sbin / scales[level]
+  int RX, RY;                  // accumulated-score shift of this component at this level
+  int tab_off;                 // level's (Xn per image column, Yn per image row) table
 };
 
 struct ScorePart {
@@ -158,12 +160,14 @@ struct Plan {
   std::vector<ScorePart> sparts;
   long long score_total = 0;
   int GL = 0;
+  std::vector<int> ascore_tab;         // per valid level: Xn[W] then Yn[H]
   // device copies of the tables
   DeviceBuffer d_resize_ops, d_resize_tab, d_resize_si, d_resize_alpha, d_hog, d_conv_tiles,
-      d_mfma_tiles, d_map_dims, d_map_off, d_dt1, d_dt2, d_nodes, d_depths, d_smaps, d_sparts;
+      d_mfma_tiles, d_map_dims, d_map_off, d_dt1, d_dt2, d_nodes, d_depths, d_smaps, d_sparts,
+      d_ascore_tab;
   // device arenas
   DeviceBuffer d_frame, d_img, d_tmp, d_hist, d_norm, d_feat, d_featb, d_conv, d_tmpM,
-      d_tmpIy, d_M, d_tmpIx, d_score, d_flags, d_blocksum, d_cand, d_count, d_rows;
+      d_tmpIy, d_M, d_tmpIx, d_score, d_flags, d_blocksum, d_cand, d_count, d_rows, d_ascore;
   long long cand_cap = 0;
   // device NMS
   DeviceBuffer n_box, n_keys, n_keys2, n_sbox, n_alive0, n_alive1, n_status, n_fsup, n_picks,

@@ -406,11 +406,20 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
   P.dt_len_tab.clear();
   P.smaps.clear();
   P.sparts.clear();
+  P.ascore_tab.clear();
   long long dto = 0, so = 0;
   const int MP = m.maxpart;
   for (int l = interval; l < LEN; l++) {
     if (!P.lv[l].valid) continue;
     const int L = l - interval;
+    // accumulated-score geometry (calc_a_score, CT/get_boxes.cpp:265-300): image column ii
+    // reads score column Xn = (int)((float)ii / scale + padx), row jj reads Yn likewise;
+    // the same float operations as the reference, evaluated once per level here.
+    const float lscale = (float)m.sbin / P.lv[l].scale;
+    const int tab_off = (int)P.ascore_tab.size();
+    for (int ii = 0; ii < W; ii++) P.ascore_tab.push_back((int)((float)ii / lscale + (float)m.padx));
+    for (int jj = 0; jj < H; jj++) P.ascore_tab.push_back((int)((float)jj / lscale + (float)m.pady));
+    const int block_pad = (int)((double)lscale / 2.0);
     for (int jj = 0; jj < m.numcomponent; jj++) {
       if (jj >= m.NoR) {
         err = "component without a root filter";
@@ -427,7 +436,10 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
       sm.npart = m.numpart[jj];
       sm.part_first = (int)P.sparts.size();
       sm.off = m.offw[m.ridx[jj]];
-      sm.scale = (float)m.sbin / P.lv[l].scale;
+      sm.scale = lscale;
+      sm.RY = (int)((double)((float)m.rsize[jj * 2] * lscale) / 2.0 - 1.0 + (double)block_pad);
+      sm.RX = (int)((double)((float)m.rsize[jj * 2 + 1] * lscale) / 2.0 - 1.0 + (double)block_pad);
+      sm.tab_off = tab_off;
       so += (long long)sm.R0 * sm.R1;
       for (int kk = 0; kk < m.numpart[jj]; kk++) {
         const int pid = m.pidx[jj * MP + kk], did = m.didx[jj * MP + kk];

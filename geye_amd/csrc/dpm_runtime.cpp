@@ -14,6 +14,7 @@
 #include <memory>
 #include <string>
 
+#include "dpm_ascore.h"
 #include "dpm_kernels.h"
 #include "dpm_mfma.h"
 #include "dpm_nms.h"
@@ -433,6 +434,7 @@ int plan_upload(Plan &P, hipStream_t st) {
   CHK(upload(P.d_depths, P.dt_depths, st));
   CHK(upload(P.d_smaps, P.smaps, st));
   CHK(upload(P.d_sparts, P.sparts, st));
+  CHK(upload(P.d_ascore_tab, P.ascore_tab, st));
   long long tmp = 0;
   for (auto &R : P.rounds) tmp = std::max(tmp, R.tmp_total);
   CHK(ensure(P.d_img, sizeof(float) * P.img_total));
@@ -465,7 +467,7 @@ void plan_release(Plan &P) {
                           &P.n_status, &P.n_fsup, &P.n_picks, &P.n_counters, &P.n_mask,
                           &P.n_keep, &P.n_sel, &P.n_sort, &P.n_flags, &P.n_keep_orig,
                           &P.n_kept, &P.n_kept_flat, &P.n_total, &P.n_vals, &P.n_ord,
-                          &P.n_rank})
+                          &P.n_rank, &P.d_ascore_tab, &P.d_ascore})
     release(*b);
 }
 
@@ -1105,6 +1107,54 @@ int geye_last_scores(geye_ctx *c, float *out, long long cap, long long *total) {
     HIPCHK(hipMemcpyAsync(out, P->d_score.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   }
+  return GEYE_OK;
+}
+
+namespace {
+// A_SCORE of the last detection into device memory d_ac (IHEI x IWID column-major, read and
+// overwritten).
+int ascore_run(geye_ctx *c, Plan &P, float thresh, float *d_ac) {
+  CHK(ensure(P.d_ascore, sizeof(float) * ((size_t)P.W * P.H + 64)));
+  unsigned int *maxbits = (unsigned int *)(dp<float>(P.d_ascore) + (size_t)P.W * P.H);
+  launch_fill(c->stream, (int *)maxbits, 1, 0);
+  {
+    KTimer t(c, "ascore");
+    launch_ascore(c->stream, dp<ScoreMap>(P.d_smaps), (int)P.smaps.size(), dp<int>(P.d_ascore_tab),
+                  dp<float>(P.d_score), P.W, P.H, thresh, d_ac, maxbits);
+  }
+  HIPCHK(hipGetLastError());
+  return GEYE_OK;
+}
+}  // namespace
+
+int geye_accumulated_score(geye_ctx *c, float thresh, float *a_score) {
+  if (!c || !a_score) return fail(GEYE_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(hipSetDevice(c->device));
+  (void)hipGetLastError();
+  Plan *P = c->last_plan;
+  if (!P) return fail(GEYE_ERR_ARG, "no detection has run on this context");
+  const size_t n = (size_t)P->W * P->H;
+  CHK(ensure(P->d_ascore, sizeof(float) * (n + 64)));
+  float *d_ac = dp<float>(P->d_ascore);
+  HIPCHK(hipMemcpyAsync(d_ac, a_score, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+  CHK(ascore_run(c, *P, thresh, d_ac));
+  HIPCHK(hipMemcpyAsync(a_score, d_ac, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  flush_stats(c);
+  return GEYE_OK;
+}
+
+int geye_accumulated_score_dev(geye_ctx *c, float thresh, void *d_a_score) {
+  if (!c || !d_a_score) return fail(GEYE_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(hipSetDevice(c->device));
+  (void)hipGetLastError();
+  Plan *P = c->last_plan;
+  if (!P) return fail(GEYE_ERR_ARG, "no detection has run on this context");
+  CHK(ascore_run(c, *P, thresh, (float *)d_a_score));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  flush_stats(c);
   return GEYE_OK;
 }
 

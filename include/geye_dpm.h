@@ -112,6 +112,13 @@ int geye_detect_dev(geye_ctx *ctx, const geye_model *model, const void *d_bgr, i
  * concatenated per (valid level, component) in the reference's order, column-major.
  * Writes at most cap floats; returns the total count through *total. */
 int geye_last_scores(geye_ctx *ctx, float *out, long long cap, long long *total);
+/* Accumulated-score map of the last geye_get_boxes/geye_detect* call on ctx: the A_SCORE
+ * side output of get_boxes (calc_a_score CT/get_boxes.cpp:265-300, normalisation
+ * :1765-1787).  a_score is the caller's height x width map, column-major, as ini_ac_score
+ * (CT/detect.cpp:136-145) allocates it; it is read and updated in place.  thresh is the
+ * detection threshold.  The _dev form takes device memory (geye_dev_alloc). */
+int geye_accumulated_score(geye_ctx *ctx, float thresh, float *a_score);
+int geye_accumulated_score_dev(geye_ctx *ctx, float thresh, void *d_a_score);
 void geye_free(void *p);
 
 /* ---- stage entry points (host arrays, reference layouts) --------------------------------

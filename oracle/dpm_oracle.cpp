@@ -779,6 +779,48 @@ std::vector<float> boxes_from_scores(const ora_model_desc *m, const ScoreMaps &S
 }
 
 // ---------------------------------------------------------------------------------------
+// accumulated score (CT/get_boxes.cpp:265-300, :1765-1787)
+// ---------------------------------------------------------------------------------------
+void calc_a_score(float *ac_score, const float *score, const int *ssize, const int *rsize,
+                  int IHEI, int IWID, int padx_n, int pady_n, float scale) {
+  const int block_pad = (int)(scale / 2.0);
+  const int RY = (int)((float)rsize[0] * scale / 2.0 - 1.0 + block_pad);
+  const int RX = (int)((float)rsize[1] * scale / 2.0 - 1.0 + block_pad);
+  for (int ii = 0; ii < IWID; ii++) {
+    const int Xn = (int)((float)ii / scale + padx_n);
+    for (int jj = 0; jj < IHEI; jj++) {
+      const int Yn = (int)((float)jj / scale + pady_n);
+      if (Yn < ssize[0] && Xn < ssize[1]) {
+        const float sc = score[Yn + Xn * ssize[0]];
+        const int Im_Y = jj + RY, Im_X = ii + RX;
+        if (Im_Y < IHEI && Im_X < IWID) {
+          float *PP = ac_score + Im_Y + Im_X * IHEI;
+          if (sc > *PP) *PP = sc;
+        }
+      }
+    }
+  }
+}
+
+// AS_OFF = |thresh| as a float (the reference's abs() on a FLOAT).
+void ascore_finish(float *A, long long n, float thresh) {
+  const float AS_OFF = std::fabs(thresh);
+  float max_ac = 0.0f;
+  for (long long ii = 0; ii < n; ii++) {
+    if (A[ii] < thresh) {
+      A[ii] = 0.0f;
+    } else {
+      A[ii] += AS_OFF;
+      if (A[ii] > max_ac) max_ac = A[ii];
+    }
+  }
+  if (max_ac > 0.0f) {
+    const float ac_ratio = (float)(1.0 / max_ac);
+    for (long long ii = 0; ii < n; ii++) A[ii] *= ac_ratio;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
 // CSV model (CT/load_model.cpp:57-558)
 // ---------------------------------------------------------------------------------------
 struct OwnedModel {
@@ -1007,6 +1049,37 @@ long long ora_detect_maps(const ora_model_desc *m, const unsigned char *bgr, int
   *nmaps = count;
   return total;
 }
+
+int ora_detect_ascore(const ora_model_desc *m, const unsigned char *bgr, int width, int height,
+                      int widthStep, float thresh, int threads, float *ac) {
+  int ms = compute_max_scale(m->rsize, m->numcomponent, m->sbin, m->interval, width, height);
+  Pyramid P = build_pyramid(bgr, width, height, widthStep, m->sbin, m->interval, ms, threads);
+  ScoreMaps S;
+  std::vector<std::vector<float>> sc;
+  compute_scores<float>(m, P, threads, S, sc);
+  const int NoC = m->numcomponent;
+  for (int level = m->interval; level < P.plan.LEN; level++) {
+    if (!S.valid[level]) continue;
+    const int L = level - m->interval;
+    const float scale = (float)m->sbin / S.plan.scales[level];
+    for (int jj = 0; jj < NoC; jj++) {
+      const int ssize[2] = {S.rdims[((size_t)L * NoC + jj) * 2],
+                            S.rdims[((size_t)L * NoC + jj) * 2 + 1]};
+      calc_a_score(ac, sc[(size_t)L * NoC + jj].data(), ssize, m->rsize + 2 * jj, height, width,
+                   m->padx, m->pady, scale);
+    }
+  }
+  ascore_finish(ac, (long long)width * height, thresh);
+  return 0;
+}
+
+void ora_calc_a_score(float *ac, int IHEI, int IWID, const float *score, int R0, int R1,
+                      const int *rsize, int padx, int pady, float scale) {
+  const int ssize[2] = {R0, R1};
+  calc_a_score(ac, score, ssize, rsize, IHEI, IWID, padx, pady, scale);
+}
+
+void ora_ascore_finish(float *ac, long long n, float thresh) { ascore_finish(ac, n, thresh); }
 
 void ora_free(void *p) { std::free(p); }
 

@@ -16,6 +16,7 @@
  *   ora_add_part         CT/dt.cpp:38-55
  *   ora_detect           CT/get_boxes.cpp:583-1797, CT/detect.cpp:151-226
  *   ora_nms              CT/nms.cpp:70-217
+ *   ora_detect_ascore    CT/get_boxes.cpp:265-300 (calc_a_score), :1765-1787 (normalisation)
  *   ora_load_csv         CT/load_model.cpp:57-558
  */
 #ifndef GEYE_DPM_ORACLE_H
@@ -117,6 +118,15 @@ long long ora_pyramid(const unsigned char *bgr, int width, int height, int width
                       int sbin, int interval, int max_scale, int threads, float **feat,
                       int *feat_h, int *feat_w, float *scales);
 
+/* Accumulated-score map (A_SCORE side output of get_boxes) of one detection: ac is the
+ * caller's height x width map (column-major), read and updated in place; FP32 scores. */
+int ora_detect_ascore(const ora_model_desc *m, const unsigned char *bgr, int width, int height,
+                      int widthStep, float thresh, int threads, float *ac);
+/* One calc_a_score call (CT/get_boxes.cpp:265-300) and the final normalisation
+ * (CT/get_boxes.cpp:1765-1787), exposed for hand-built checks. */
+void ora_calc_a_score(float *ac, int IHEI, int IWID, const float *score, int R0, int R1,
+                      const int *rsize, int padx, int pady, float scale);
+void ora_ascore_finish(float *ac, long long n, float thresh);
 void ora_free(void *p);
 
 /* CSV model loader (CT/load_model.cpp:57-558).  Returns an opaque handle; the desc view

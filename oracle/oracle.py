@@ -85,6 +85,11 @@ def lib():
                                   C.c_int, C.c_int, C.POINTER(_fp), _ip, _ip, _fp]
         L.ora_pyramid.restype = C.c_longlong
         L.ora_free.argtypes = [C.c_void_p]
+        L.ora_detect_ascore.argtypes = [C.POINTER(ModelDesc), C.c_void_p, C.c_int, C.c_int,
+                                        C.c_int, C.c_float, C.c_int, _fp]
+        L.ora_calc_a_score.argtypes = [_fp, C.c_int, C.c_int, _fp, C.c_int, C.c_int, _ip,
+                                       C.c_int, C.c_int, C.c_float]
+        L.ora_ascore_finish.argtypes = [_fp, C.c_longlong, C.c_float]
         L.ora_load_csv.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(ModelDesc)]
         L.ora_load_csv.restype = C.c_void_p
         L.ora_model_free.argtypes = [C.c_void_p]
@@ -315,6 +320,39 @@ def score_maps(model, frame: np.ndarray, precision: int = 0):
         out.append((int(lv), int(jj), buf[off:off + n].reshape(int(r1), int(r0))))
         off += n
     return out
+
+
+def ascore(model, frame: np.ndarray, thresh: float = -0.5, ac: np.ndarray | None = None,
+           threads: int = 8) -> np.ndarray:
+    """A_SCORE of one FP32 detection; ac is [W][H] float32 (column-major image), -100 if None."""
+    keep = _DescKeeper(model)
+    H, W, _ = frame.shape
+    frame = np.ascontiguousarray(frame)
+    if ac is None:
+        ac = np.full((W, H), -100.0, np.float32)
+    ac = np.ascontiguousarray(ac, dtype=np.float32)
+    lib().ora_detect_ascore(C.byref(keep.desc), frame.ctypes.data, W, H, 3 * W, C.c_float(thresh),
+                            threads, _p(ac, _fp))
+    return ac
+
+
+def calc_a_score(ac: np.ndarray, score: np.ndarray, rsize, scale: float, padx: int = 0,
+                 pady: int = 0) -> np.ndarray:
+    """One calc_a_score call: ac [IWID][IHEI], score [R1][R0] (both column-major)."""
+    ac = np.ascontiguousarray(ac, dtype=np.float32)
+    score = np.ascontiguousarray(score, dtype=np.float32)
+    IWID, IHEI = ac.shape
+    R1, R0 = score.shape
+    rs = np.ascontiguousarray(rsize, dtype=np.int32)
+    lib().ora_calc_a_score(_p(ac, _fp), IHEI, IWID, _p(score, _fp), R0, R1, _p(rs, _ip), padx, pady,
+                           C.c_float(scale))
+    return ac
+
+
+def ascore_finish(ac: np.ndarray, thresh: float) -> np.ndarray:
+    ac = np.ascontiguousarray(ac, dtype=np.float32)
+    lib().ora_ascore_finish(_p(ac, _fp), ac.size, C.c_float(thresh))
+    return ac
 
 
 def pyramid(frame: np.ndarray, sbin: int, interval: int, max_scale: int, threads: int = 1):

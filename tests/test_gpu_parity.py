@@ -231,3 +231,24 @@ def test_detect_mfma_scores_vs_fp64(ctx, prec):
     err = np.abs(g.astype(np.float64) - ref)
     say(prec, "score max abs err", float(err.max()), "mean", float(err.mean()))
     assert err.max() < 0.02
+
+
+@pytest.mark.parametrize("which,W,H,thresh", [("car", 640, 480, -0.5), ("car", 640, 480, -1.2),
+                                              ("synthetic", 480, 360, -0.5)])
+def test_accumulated_score_matches_oracle(ctx, car_model, which, W, H, thresh):
+    """A_SCORE side output of get_boxes: bit-exact against the oracle's calc_a_score loop,
+    from the -100 initial map and from a caller's prior map (updated in place)."""
+    say("accumulated score", which, W, H, thresh)
+    m = car_model if which == "car" else synth.synthetic_car()
+    frame = synth.make_frame(W, H, seed=5)
+    gm = ctx.model(m)
+    gm.get_boxes(frame, thresh=thresh)
+    g = gm.accumulated_score(W, H, thresh)
+    o = O.ascore(m, frame, thresh)
+    assert bits_equal(g, o)
+    if thresh < -1.0 or which == "synthetic":   # these frames have scores above thresh
+        assert o.max() == 1.0
+    prior = np.random.default_rng(1).uniform(-2, 3, (W, H)).astype(np.float32)
+    gm.get_boxes(frame, thresh=thresh)
+    g2 = gm.accumulated_score(W, H, thresh, prior.copy())
+    assert bits_equal(g2, O.ascore(m, frame, thresh, prior.copy()))

@@ -353,3 +353,45 @@ def test_detect_tiny_model_candidate_invariants():
     maps64 = O.score_maps(m, frame, 1)
     for (_, _, a), (_, _, b) in zip(maps32, maps64):
         np.testing.assert_allclose(a, b, atol=2e-4)
+
+
+# ---------------------------------------------------------------------------------------
+# accumulated score (CT/get_boxes.cpp:265-300, :1765-1787)
+# ---------------------------------------------------------------------------------------
+def test_calc_a_score_hand_built_shift_and_block():
+    # IHEI 40, IWID 30, scale 4, rsize (2, 3): block_pad = 2, RY = (int)(2*4/2 - 1 + 2) = 5,
+    # RX = (int)(3*4/2 - 1 + 2) = 7.  Score cell (Yn=3, Xn=2) = 5 covers image rows 12..15,
+    # columns 8..11, which land at rows 17..20, columns 15..18.
+    ac = np.full((30, 40), -100.0, np.float32)
+    score = np.full((8, 10), -10.0, np.float32)     # [R1=8][R0=10]
+    score[2, 3] = 5.0
+    O.calc_a_score(ac, score, [2, 3], 4.0)
+    exp = np.full((30, 40), -100.0, np.float32)
+    exp[7:, 5:] = -10.0
+    exp[15:19, 17:21] = 5.0
+    assert np.array_equal(ac, exp)
+    O.ascore_finish(ac, -0.5)
+    fin = np.zeros_like(exp)
+    fin[15:19, 17:21] = 1.0                          # (5 + 0.5) * (1 / 5.5)
+    np.testing.assert_array_equal(ac, fin)
+
+
+def test_calc_a_score_map_bounds_and_max():
+    # a score map smaller than the image: only columns/rows with Xn < R1, Yn < R0 contribute,
+    # and a second map only raises values (strict max)
+    ac = np.full((20, 16), -100.0, np.float32)
+    s1 = np.zeros((2, 3), np.float32) + 1.0          # R1 = 2, R0 = 3
+    O.calc_a_score(ac, s1, [1, 1], 2.0)              # block_pad 1: RY = RX = (int)(1 - 1 + 1) = 1
+    assert np.all(ac[1:5, 1:7] == 1.0) and np.all(ac[5:] == -100.0) and np.all(ac[:, 7:] == -100.0)
+    s2 = np.full((2, 3), 0.5, np.float32)
+    O.calc_a_score(ac, s2, [1, 1], 2.0)
+    assert np.all(ac[1:5, 1:7] == 1.0)
+
+
+def test_ascore_finish_threshold_and_all_below():
+    ac = np.array([[-3.0, -0.5, 0.0, 1.5]], np.float32)
+    O.ascore_finish(ac, -0.5)
+    np.testing.assert_array_equal(ac, np.float32([[0.0, 0.0, 0.25, 1.0]]))
+    z = np.full((3, 3), -100.0, np.float32)
+    O.ascore_finish(z, -0.5)
+    assert np.all(z == 0.0)
