@@ -51,6 +51,7 @@ class ModelDesc(C.Structure):
         ("DefL", C.c_int), ("deff", _fp), ("anchor", _ip),
         ("NoR", C.c_int), ("root_size", _ip), ("rootfilter", _fp), ("root_off", _llp),
         ("NoP", C.c_int), ("part_size", _ip), ("partfilter", _fp), ("part_off", _llp),
+        ("lsq", _fp),
     ]
 
 
@@ -78,6 +79,12 @@ _SIGS = [
     ("geye_last_scores", C.c_int, [C.c_void_p, _fp, C.c_longlong, _llp]),
     ("geye_accumulated_score", C.c_int, [C.c_void_p, C.c_float, _fp]),
     ("geye_accumulated_score_dev", C.c_int, [C.c_void_p, C.c_float, C.c_void_p]),
+    ("geye_new_rects", C.c_int, [C.c_void_p, C.c_void_p, _fp, C.c_int, C.c_int, C.c_int,
+                                 C.c_int, C.c_float, _ip, _ip, _ip, _fp, _fp]),
+    ("geye_get_boxes_features", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                          _fp, C.c_longlong, C.c_float, C.POINTER(_fp), _ip,
+                                          _ip]),
+    ("geye_add_part", C.c_int, [C.c_void_p, _fp, _fp, _ip, _ip, C.c_int, C.c_int]),
     ("geye_free", None, [C.c_void_p]),
     ("geye_resize", C.c_int, [C.c_void_p, _fp, _ip, C.c_float, _fp, _ip]),
     ("geye_calc_feature", C.c_int, [C.c_void_p, _fp, C.c_int, C.c_int, C.c_int, _fp, _ip, _ip]),
@@ -162,7 +169,16 @@ def desc_from_model(model):
         d.part_size = _p(arr(m.part_size, np.int32), _ip)
         d.partfilter = _p(arr(np.concatenate(m.partfilters), np.float32), _fp)
         d.part_off = _p(arr(np.cumsum([0] + [len(f) for f in m.partfilters[:-1]]), np.int64), _llp)
+    if getattr(m, "lsq", None) is not None:
+        d.lsq = _p(arr(lsq_flat(m), np.float32), _fp)
     return d, keep
+
+
+def lsq_flat(m) -> np.ndarray:
+    """Least-squares coefficients as the C ABI takes them: per component x1, y1, x2, y2."""
+    if isinstance(m.lsq, (list, tuple)):
+        return np.concatenate([np.asarray(a, np.float32).ravel() for a in m.lsq])
+    return np.asarray(m.lsq, np.float32).ravel()
 
 
 def level_plan_dims(rsize, sbin: int, interval: int, width: int, height: int) -> dict:
@@ -300,6 +316,16 @@ class Context:
                              C.c_float(by), _p(dims, _ip), _p(M, _fp), _p(Ix, _ip), _p(Iy, _ip)))
         return M, Ix, Iy
 
+    def add_part_calculation(self, score: np.ndarray, M: np.ndarray, ax: int, ay: int) -> np.ndarray:
+        """score [R1][R0] -= M[(ay-1+2y) + (ax-1+2x)*P0] (M is [P1][P0]); returns a copy."""
+        s = np.ascontiguousarray(score, dtype=np.float32).copy()
+        Mv = np.ascontiguousarray(M, dtype=np.float32)
+        rs = np.array([s.shape[1], s.shape[0]], np.int32)
+        ps = np.array([Mv.shape[1], Mv.shape[0]], np.int32)
+        _check(lib().geye_add_part(self.h, _p(s, _fp), _p(Mv, _fp), _p(rs, _ip), _p(ps, _ip),
+                                   ax, ay))
+        return s
+
     def nms(self, boxes: np.ndarray, overlap: float) -> np.ndarray:
         boxes = np.ascontiguousarray(boxes, dtype=np.float32)
         n, GL = boxes.shape
@@ -412,6 +438,27 @@ class Model:
         assert a_score.dtype == np.float32 and a_score.shape == (W, H) and a_score.flags.c_contiguous
         _check(lib().geye_accumulated_score(self.ctx.h, C.c_float(thresh), _p(a_score, _fp)))
         return a_score
+
+    def new_rects(self, rows: np.ndarray, W: int, H: int, ratio: float = 1.0) -> dict:
+        """get_new_rects (CT/tracking.cpp:90-166) of NMS survivor rows."""
+        rows = np.ascontiguousarray(rows, dtype=np.float32)
+        n, GL = rows.shape if rows.ndim == 2 else (0, 0)
+        out = dict(point=np.zeros((n, 4), np.int32), OR_point=np.zeros((n, 4), np.int32),
+                   type=np.zeros(n, np.int32), scale=np.zeros(n, np.float32),
+                   score=np.zeros(n, np.float32))
+        if n:
+            _check(lib().geye_new_rects(self.ctx.h, self.h, _p(rows, _fp), n, GL, W, H,
+                                        C.c_float(ratio), _p(out["point"], _ip),
+                                        _p(out["OR_point"], _ip), _p(out["type"], _ip),
+                                        _p(out["scale"], _fp), _p(out["score"], _fp)))
+        return out
+
+    def get_boxes_features(self, feats: list, W: int, H: int, thresh: float = -0.5,
+                           max_scale: int = 0) -> np.ndarray:
+        """get_boxes on host features (one [31][F1][F0] array per pyramid level)."""
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(f, np.float32).ravel() for f in feats]))
+        return self._rows(lib().geye_get_boxes_features, self.ctx.h, self.h, W, H, max_scale,
+                          _p(flat, _fp), flat.size, C.c_float(thresh))
 
     def last_scores(self) -> np.ndarray:
         total = C.c_longlong()

@@ -18,7 +18,7 @@ OUT = os.path.join(HERE, "libgeye_amd.so")
 BUILD = os.path.join(HERE, "_build")
 
 SOURCES = ["dpm_kernels.hip", "dpm_plan.cpp", "dpm_runtime.cpp"]
-EXTRA_SOURCES = ["dpm_mfma.hip", "dpm_nms.hip", "dpm_ascore.hip", "dpm_serve.cpp"]
+EXTRA_SOURCES = ["dpm_mfma.hip", "dpm_nms.hip", "dpm_ascore.hip", "dpm_rects.hip", "dpm_serve.cpp"]
 
 
 def hipcc() -> str:
@@ -56,9 +56,39 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp, OUT)
+    build_compat()
     if verbose:
         print("built", OUT)
     return OUT
+
+
+COMPAT_OUT = os.path.join(HERE, "libgeye_dpm_compat.so")
+ROOT = os.path.dirname(HERE)
+COMPAT_DEMO = os.path.join(ROOT, "tests", "compat_detect.cpp")
+COMPAT_DEMO_OUT = os.path.join(ROOT, "tests", "_bin", "compat_detect")
+
+
+def build_compat() -> None:
+    """The reference-named C++ layer (host code over the C ABI) and its test driver."""
+    gxx = shutil.which("g++") or "g++"
+    inc = os.path.join(ROOT, "include")
+    cmd = [gxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-I", inc,
+           os.path.join(CSRC, "dpm_compat.cpp"), "-L", HERE, "-lgeye_amd",
+           "-Wl,-rpath,$ORIGIN", "-o", COMPAT_OUT + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("compat library build failed")
+    os.replace(COMPAT_OUT + ".tmp", COMPAT_OUT)
+    if os.path.exists(COMPAT_DEMO):
+        os.makedirs(os.path.dirname(COMPAT_DEMO_OUT), exist_ok=True)
+        cmd = [gxx, "-O2", "-std=c++17", "-I", inc, COMPAT_DEMO, "-L", HERE,
+               "-lgeye_dpm_compat", "-lgeye_amd", "-Wl,-rpath," + HERE,
+               "-Wl,-rpath,$ORIGIN/../../geye_amd", "-o", COMPAT_DEMO_OUT]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("compat test driver build failed")
 
 
 if __name__ == "__main__":

@@ -199,7 +199,10 @@ struct Model {
     int nslots = 0;                  // output slots per level (members + padding)
   };
   std::vector<FilterClass> classes;
-  DeviceBuffer d_filters, d_rsize, d_filt_bf16, d_filt_f16;
+  // least-squares box coefficients (x1, y1, x2, y2 per component), may be empty
+  std::vector<float> lsq;
+  std::vector<int> lsq_idx;                       // [offsets C][lengths C]
+  DeviceBuffer d_filters, d_rsize, d_filt_bf16, d_filt_f16, d_lsq, d_lsq_idx;
   std::unordered_map<long long, Plan *> plans;   // key = (W << 20) | H
 };
 

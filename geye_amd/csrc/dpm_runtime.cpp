@@ -18,6 +18,7 @@
 #include "dpm_kernels.h"
 #include "dpm_mfma.h"
 #include "dpm_nms.h"
+#include "dpm_rects.h"
 #include "dpm_runtime.h"
 
 using namespace geye;
@@ -280,6 +281,19 @@ int model_from_desc(geye_ctx *ctx, const geye_model_desc *d, Model &m) {
   CHK(upload(m.d_filt_f16, pf16, ctx->stream));
   CHK(upload(m.d_filters, packed, ctx->stream));
   CHK(upload(m.d_rsize, m.rsize, ctx->stream));
+  m.lsq.clear();
+  m.lsq_idx.clear();
+  if (d->lsq) {
+    m.lsq_idx.assign(2 * C, 0);
+    for (int i = 0; i < C; i++) {
+      const int ml = 1 + 2 * (1 + m.numpart[i]);
+      m.lsq_idx[i] = (int)m.lsq.size();
+      m.lsq_idx[C + i] = ml;
+      m.lsq.insert(m.lsq.end(), d->lsq + m.lsq.size(), d->lsq + m.lsq.size() + 4 * ml);
+    }
+    CHK(upload(m.d_lsq, m.lsq, ctx->stream));
+    CHK(upload(m.d_lsq_idx, m.lsq_idx, ctx->stream));
+  }
   return GEYE_OK;
 }
 
@@ -354,6 +368,10 @@ int load_csv(geye_ctx *ctx, const char *comp, const char *root, const char *part
     anchor[2 * q] = (int)next(c);
     anchor[2 * q + 1] = (int)next(c);
   }
+  // least-squares box coefficients: per component x1, y1, x2, y2 of 1 + 2(1 + np) values
+  std::vector<float> lsq;
+  for (int i = 0; i < C && !bad; i++)
+    for (int q = 0; q < 4 * (1 + 2 * (1 + numpart[i])); q++) lsq.push_back(next(c));
   if (bad) return fail(GEYE_ERR_IO, "component file truncated");
   std::vector<int> pidx(C * MP, 0), didx(C * MP, 0), psize(C * MP * 2, 0);
   for (int i = 0; i < C; i++)
@@ -407,6 +425,7 @@ int load_csv(geye_ctx *ctx, const char *comp, const char *root, const char *part
   d.part_size = pss.data();
   d.partfilter = pw.data();
   d.part_off = po.data();
+  d.lsq = lsq.data();
   return model_from_desc(ctx, &d, m);
 }
 
@@ -471,8 +490,10 @@ void plan_release(Plan &P) {
     release(*b);
 }
 
-int get_plan(Model &m, int W, int H, Plan **out) {
-  const long long key = ((long long)W << 20) | (long long)H;
+// max_scale > 0 pins the pyramid depth (ini_scales computes it once per model from the
+// first frame, CT/featurepyramid.cpp.new2:179-226); otherwise it follows this frame.
+int get_plan(Model &m, int W, int H, Plan **out, int max_scale = -1) {
+  const long long key = ((long long)std::max(max_scale, 0) << 40) | ((long long)W << 20) | (long long)H;
   auto it = m.plans.find(key);
   if (it != m.plans.end()) {
     *out = it->second;
@@ -480,7 +501,7 @@ int get_plan(Model &m, int W, int H, Plan **out) {
   }
   std::unique_ptr<Plan> P(new Plan());
   std::string err;
-  int rc = build_plan(m, W, H, *P, err);
+  int rc = build_plan(m, W, H, *P, err, max_scale > 0 ? max_scale : -1);
   if (rc != GEYE_OK) return fail(rc, err);
   rc = plan_upload(*P, m.ctx->stream);
   if (rc != GEYE_OK) {
@@ -1008,6 +1029,8 @@ void geye_model_free(geye_model *gm) {
   release(m.d_rsize);
   release(m.d_filt_bf16);
   release(m.d_filt_f16);
+  release(m.d_lsq);
+  release(m.d_lsq_idx);
   delete gm;
 }
 
@@ -1159,6 +1182,113 @@ int geye_accumulated_score_dev(geye_ctx *c, float thresh, void *d_a_score) {
 }
 
 void geye_free(void *p) { std::free(p); }
+
+int geye_new_rects(geye_ctx *c, const geye_model *model, const float *boxes, int num, int GL,
+                   int width, int height, float ratio, int *point, int *or_point, int *type,
+                   float *scale, float *score) {
+  if (!c || !model || num < 0 || (num > 0 && (!boxes || !point || !or_point || !type || !scale ||
+                                              !score)))
+    return fail(GEYE_ERR_ARG, "null argument");
+  if (num == 0) return GEYE_OK;
+  const Model &m = model->m;
+  if (m.lsq.empty()) return fail(GEYE_ERR_ARG, "model has no least-squares box coefficients");
+  const int ML = 1 + 2 * (1 + m.numpart[0]);
+  if (GL != (m.numpart[0] + 1) * 4 + 3) return fail(GEYE_ERR_ARG, "GL does not match the model");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(hipSetDevice(c->device));
+  (void)hipGetLastError();
+  Scratch S;
+  DeviceBuffer &drows = S.get(), &dint = S.get(), &dflt = S.get();
+  const size_t n = (size_t)num;
+  CHK(ensure(drows, sizeof(float) * n * GL));
+  CHK(ensure(dint, sizeof(int) * (n * 9 + 1)));
+  CHK(ensure(dflt, sizeof(float) * n * 2));
+  int *d_point = dp<int>(dint), *d_or = d_point + 4 * n, *d_type = d_or + 4 * n, *d_bad = d_type + n;
+  float *d_scale = dp<float>(dflt), *d_score = d_scale + n;
+  HIPCHK(hipMemcpyAsync(drows.p, boxes, sizeof(float) * n * GL, hipMemcpyHostToDevice, c->stream));
+  launch_fill(c->stream, d_bad, 1, 0);
+  {
+    KTimer t(c, "new_rects");
+    launch_new_rects(c->stream, dp<float>(drows), num, GL, ML, (const float *)m.d_lsq.p,
+                     (const int *)m.d_lsq_idx.p, (const int *)m.d_lsq_idx.p + m.numcomponent,
+                     m.numcomponent, width, height, ratio, d_point, d_or, d_type, d_scale,
+                     d_score, d_bad);
+  }
+  HIPCHK(hipGetLastError());
+  int bad = 0;
+  HIPCHK(hipMemcpyAsync(point, d_point, sizeof(int) * 4 * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(or_point, d_or, sizeof(int) * 4 * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(type, d_type, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(scale, d_scale, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(score, d_score, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  flush_stats(c);
+  if (bad) return fail(GEYE_ERR_ARG, "a row's component index is outside the model");
+  return GEYE_OK;
+}
+
+int geye_add_part(geye_ctx *c, float *score, const float *M, const int *rootsize,
+                  const int *partsize, int ax, int ay) {
+  if (!c || !score || !M || !rootsize || !partsize) return fail(GEYE_ERR_ARG, "null argument");
+  const int R0 = rootsize[0], R1 = rootsize[1], P0 = partsize[0], P1 = partsize[1];
+  if (R0 <= 0 || R1 <= 0 || P0 <= 0 || P1 <= 0) return fail(GEYE_ERR_ARG, "empty map");
+  if (ay < 1 || ax < 1 || ay - 1 + 2 * (R0 - 1) >= P0 || ax - 1 + 2 * (R1 - 1) >= P1)
+    return fail(GEYE_ERR_ARG, "anchor reads outside the part map");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(hipSetDevice(c->device));
+  (void)hipGetLastError();
+  Scratch S;
+  DeviceBuffer &ds = S.get(), &dm = S.get();
+  const size_t nr = (size_t)R0 * R1, np = (size_t)P0 * P1;
+  CHK(ensure(ds, sizeof(float) * nr));
+  CHK(ensure(dm, sizeof(float) * np));
+  HIPCHK(hipMemcpyAsync(ds.p, score, sizeof(float) * nr, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dm.p, M, sizeof(float) * np, hipMemcpyHostToDevice, c->stream));
+  {
+    KTimer t(c, "add_part");
+    launch_add_part(c->stream, dp<float>(ds), dp<float>(dm), R0, R1, P0, ax, ay);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(score, ds.p, sizeof(float) * nr, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  flush_stats(c);
+  return GEYE_OK;
+}
+
+int geye_get_boxes_features(geye_ctx *c, const geye_model *model, int width, int height,
+                            int max_scale, const float *feat, long long nfeat, float thresh,
+                            float **boxes, int *num, int *GL) {
+  if (!c || !model || !feat || !boxes || !num || !GL) return fail(GEYE_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(hipSetDevice(c->device));
+  (void)hipGetLastError();
+  Model &m = const_cast<Model &>(model->m);
+  Plan *P = nullptr;
+  CHK(get_plan(m, width, height, &P, max_scale));
+  long long total = 0;
+  auto lvl_n = [&](int l) { return 31LL * std::max(0, P->lv[l].F0) * std::max(0, P->lv[l].F1); };
+  for (int l = 0; l < P->LEN; l++) total += lvl_n(l);
+  if (nfeat != total) return fail(GEYE_ERR_ARG, "feature count does not match the level plan");
+  long long off = 0;
+  for (int l = 0; l < P->LEN; l++) {
+    const long long n = lvl_n(l);
+    if (n > 0 && P->feat_off[l] >= 0)
+      HIPCHK(hipMemcpyAsync(dp<float>(P->d_feat) + P->feat_off[l], feat + off, sizeof(float) * n,
+                            hipMemcpyHostToDevice, c->stream));
+    off += n;
+  }
+  int ncand = 0;
+  CHK(run_scores(c, m, *P, thresh, GEYE_FP32, &ncand));
+  std::vector<float> rows;
+  c->last_plan = P;
+  CHK(run_rows(c, m, *P, dp<int>(P->d_cand), ncand, rows));
+  *num = ncand;
+  *boxes = to_malloc(rows);
+  *GL = P->GL;
+  flush_stats(c);
+  return *boxes ? GEYE_OK : fail(GEYE_ERR_NOMEM, "malloc");
+}
 
 int geye_nms(geye_ctx *c, const float *boxes, int num, int GL, float overlap, float *out,
              int *out_num) {

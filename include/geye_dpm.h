@@ -59,6 +59,10 @@ typedef struct {
   const int *part_size;
   const float *partfilter;
   const long long *part_off;
+  /* Least-squares box coefficients (Model_info x1/y1/x2/y2, CT/load_model.cpp:207-229):
+   * per component c, four arrays (x1, y1, x2, y2) of 1 + 2*(1 + numpart[c]) floats,
+   * components concatenated.  May be null (then geye_new_rects is unavailable). */
+  const float *lsq;
 } geye_model_desc;
 
 /* ---- context / model (replaces init_cuda CT/GPU_init.cpp:50, clean_cuda :310,
@@ -121,6 +125,21 @@ int geye_accumulated_score(geye_ctx *ctx, float thresh, float *a_score);
 int geye_accumulated_score_dev(geye_ctx *ctx, float thresh, void *d_a_score);
 void geye_free(void *p);
 
+/* get_new_rects (CT/tracking.cpp:90-166): the least-squares refinement of NMS survivors
+ * (rows as geye_detect returns them) into integer boxes clamped to the width x height
+ * image, plus the same boxes divided by ratio (OR_point).  Outputs hold num*4 (point,
+ * or_point) and num (type, scale, score) entries.  Computed on the device. */
+int geye_new_rects(geye_ctx *ctx, const geye_model *model, const float *boxes, int num, int GL,
+                   int width, int height, float ratio, int *point, int *or_point, int *type,
+                   float *scale, float *score);
+/* get_boxes() (CT/get_boxes.cpp:583-1797) on caller-supplied features: all levels of a
+ * width x height frame's pyramid concatenated as calc_f_pyramid returns them
+ * ([level][31][F1][F0]).  max_scale > 0 fixes the pyramid depth (Model_info.max_scale as
+ * ini_scales left it); <= 0 derives it from the frame size.  FP32; uploads the features. */
+int geye_get_boxes_features(geye_ctx *ctx, const geye_model *model, int width, int height,
+                            int max_scale, const float *feat, long long nfeat, float thresh,
+                            float **boxes, int *num, int *GL);
+
 /* ---- stage entry points (host arrays, reference layouts) --------------------------------
  * geye_resize        replaces resize()          CT/resize.cc:135
  * geye_calc_feature  replaces calc_feature()    CT/featurepyramid.cpp.new2:263
@@ -129,6 +148,7 @@ void geye_free(void *p);
  * geye_fconv         replaces fconvsMT()        CT/get_boxes_func.h:28, CT/fconvsMT.cpp:310
  *                    (one feature map against n filters of one size)
  * geye_dt            replaces dt()              CT/dt.cpp:101
+ * geye_add_part      replaces add_part_calculation()  CT/dt.cpp:38
  * geye_nms           replaces nms()             CT/nms.cpp:94 (runs on the device) */
 int geye_resize(geye_ctx *ctx, const float *src, const int *sdims, float scale, float *dst,
                 int *odims);
@@ -141,6 +161,11 @@ int geye_fconv(geye_ctx *ctx, const float *feat, int F0, int F1, const float *fi
                int h, int w, int precision, float *out);
 int geye_dt(geye_ctx *ctx, const float *vals, float ax, float bx, float ay, float by,
             const int *dims, float *M, int *Ix, int *Iy);
+/* add_part_calculation (CT/dt.cpp:38-55): score[y + x*rootsize[0]] -=
+ * M[(ay-1+2y) + (ax-1+2x)*partsize[0]] over the root map, in place; rejects anchors that
+ * would read outside M (the reference does not check). */
+int geye_add_part(geye_ctx *ctx, float *score, const float *M, const int *rootsize,
+                  const int *partsize, int ax, int ay);
 int geye_nms(geye_ctx *ctx, const float *boxes, int num, int GL, float overlap, float *out,
              int *out_num);
 

@@ -1081,6 +1081,49 @@ void ora_calc_a_score(float *ac, int IHEI, int IWID, const float *score, int R0,
 
 void ora_ascore_finish(float *ac, long long n, float thresh) { ascore_finish(ac, n, thresh); }
 
+void ora_new_rects(const ora_model_desc *m, const float *boxes, int num, int GL, int width,
+                   int height, float ratio, int *point, int *or_point, int *type, float *scale,
+                   float *score) {
+  const int np0 = m->numpart[0];
+  const int LL = GL - 3;
+  const int ML = 1 + 2 * (1 + np0);
+  std::vector<long long> loff(m->numcomponent + 1, 0);
+  for (int c = 0; c < m->numcomponent; c++)
+    loff[c + 1] = loff[c] + 4LL * (1 + 2 * (1 + m->numpart[c]));
+  std::vector<float> Avec(ML);
+  for (int ii = 0; ii < num; ii++) {
+    const float *P = boxes + (long long)GL * ii;
+    const int CNUM = (int)P[GL - 3];
+    int k = 0;
+    Avec[k++] = P[3] - P[1];
+    for (int kk = 0; kk < LL; kk += 4) {
+      Avec[k++] = P[kk + 1];
+      Avec[k++] = P[kk];
+    }
+    const int MLc = 1 + 2 * (1 + m->numpart[CNUM]);
+    const float *x1 = m->lsq + loff[CNUM], *y1 = x1 + MLc, *x2 = y1 + MLc, *y2 = x2 + MLc;
+    float XP1 = 0, XP2 = 0, YP1 = 0, YP2 = 0;
+    for (int kk = 0; kk < ML; kk++) {
+      YP1 += Avec[kk] * y1[kk];
+      YP2 += Avec[kk] * y2[kk];
+      XP1 += Avec[kk] * x1[kk];
+      XP2 += Avec[kk] * x2[kk];
+    }
+    int PP[4];
+    PP[0] = XP1 > 0 ? (int)XP1 : 0;
+    PP[1] = YP1 > 0 ? (int)YP1 : 0;
+    PP[2] = XP2 < width ? (int)XP2 : width;
+    PP[3] = YP2 < height ? (int)YP2 : height;
+    for (int q = 0; q < 4; q++) {
+      point[ii * 4 + q] = PP[q];
+      or_point[ii * 4 + q] = (int)((float)PP[q] / ratio);
+    }
+    scale[ii] = P[GL - 1];
+    score[ii] = P[GL - 2];
+    type[ii] = CNUM;
+  }
+}
+
 void ora_free(void *p) { std::free(p); }
 
 void *ora_load_csv(const char *comp, const char *root, const char *part, ora_model_desc *desc) {
@@ -1187,6 +1230,7 @@ void *ora_load_csv(const char *comp, const char *root, const char *part, ora_mod
   desc->part_size = M->part_size.data();
   desc->partfilter = M->partfilter.data();
   desc->part_off = M->part_off.data();
+  desc->lsq = M->lsq.data();
   return M;
 }
 

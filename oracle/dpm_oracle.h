@@ -17,6 +17,7 @@
  *   ora_detect           CT/get_boxes.cpp:583-1797, CT/detect.cpp:151-226
  *   ora_nms              CT/nms.cpp:70-217
  *   ora_detect_ascore    CT/get_boxes.cpp:265-300 (calc_a_score), :1765-1787 (normalisation)
+ *   ora_new_rects        CT/tracking.cpp:90-166
  *   ora_load_csv         CT/load_model.cpp:57-558
  */
 #ifndef GEYE_DPM_ORACLE_H
@@ -49,6 +50,7 @@ typedef struct {
   const int *part_size;      /* [NoP][3] */
   const float *partfilter;
   const long long *part_off; /* [NoP] */
+  const float *lsq;          /* per component: x1, y1, x2, y2 of 1+2(1+numpart) floats */
 } ora_model_desc;
 
 /* BGR interleaved uint8 -> planar float, column-major [c][x][y], plane 0 = B. */
@@ -127,6 +129,10 @@ int ora_detect_ascore(const ora_model_desc *m, const unsigned char *bgr, int wid
 void ora_calc_a_score(float *ac, int IHEI, int IWID, const float *score, int R0, int R1,
                       const int *rsize, int padx, int pady, float scale);
 void ora_ascore_finish(float *ac, long long n, float thresh);
+/* get_new_rects (CT/tracking.cpp:90-166) over num NMS survivors. */
+void ora_new_rects(const ora_model_desc *m, const float *boxes, int num, int GL, int width,
+                   int height, float ratio, int *point, int *or_point, int *type, float *scale,
+                   float *score);
 void ora_free(void *p);
 
 /* CSV model loader (CT/load_model.cpp:57-558).  Returns an opaque handle; the desc view

@@ -35,6 +35,7 @@ class ModelDesc(C.Structure):
         ("DefL", C.c_int), ("deff", _fp), ("anchor", _ip),
         ("NoR", C.c_int), ("root_size", _ip), ("rootfilter", _fp), ("root_off", _llp),
         ("NoP", C.c_int), ("part_size", _ip), ("partfilter", _fp), ("part_off", _llp),
+        ("lsq", _fp),
     ]
 
 
@@ -90,6 +91,8 @@ def lib():
         L.ora_calc_a_score.argtypes = [_fp, C.c_int, C.c_int, _fp, C.c_int, C.c_int, _ip,
                                        C.c_int, C.c_int, C.c_float]
         L.ora_ascore_finish.argtypes = [_fp, C.c_longlong, C.c_float]
+        L.ora_new_rects.argtypes = [C.POINTER(ModelDesc), _fp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_float, _ip, _ip, _ip, _fp, _fp]
         L.ora_load_csv.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(ModelDesc)]
         L.ora_load_csv.restype = C.c_void_p
         L.ora_model_free.argtypes = [C.c_void_p]
@@ -153,6 +156,9 @@ class _DescKeeper:
             d.part_size = _p(keep(m.part_size, np.int32), _ip)
             d.partfilter = _p(keep(np.concatenate(m.partfilters), np.float32), _fp)
             d.part_off = _p(keep(poff, np.int64), _llp)
+        if getattr(m, "lsq", None) is not None:
+            d.lsq = _p(keep(np.concatenate([np.asarray(a, np.float32).ravel() for a in m.lsq]),
+                            np.float32), _fp)
         self.desc = d
 
 
@@ -355,6 +361,57 @@ def ascore_finish(ac: np.ndarray, thresh: float) -> np.ndarray:
     return ac
 
 
+def new_rects(model, rows: np.ndarray, W: int, H: int, ratio: float = 1.0) -> dict:
+    """get_new_rects (CT/tracking.cpp:90-166) of NMS survivor rows."""
+    keep = _DescKeeper(model)
+    rows = np.ascontiguousarray(rows, dtype=np.float32)
+    n = rows.shape[0]
+    GL = rows.shape[1] if rows.ndim == 2 else 0
+    out = dict(point=np.zeros((n, 4), np.int32), OR_point=np.zeros((n, 4), np.int32),
+               type=np.zeros(n, np.int32), scale=np.zeros(n, np.float32),
+               score=np.zeros(n, np.float32))
+    if n:
+        lib().ora_new_rects(C.byref(keep.desc), _p(rows, _fp), n, GL, W, H, C.c_float(ratio),
+                            _p(out["point"], _ip), _p(out["OR_point"], _ip), _p(out["type"], _ip),
+                            _p(out["scale"], _fp), _p(out["score"], _fp))
+    return out
+
+
+def write_csv_model(model, directory: str, prefix: str = "model") -> tuple:
+    """Write a DPMModel in the reference's CSV model format (CT/load_model.cpp:57-530):
+    returns the (comp, root, part) paths.  Floats are written with 9 significant digits, so
+    they parse back to the same float32 values."""
+    import os as _os
+
+    m = model
+    f9 = lambda v: "%.9g" % float(np.float32(v))
+    comp = [m.numcomponent, m.sbin, m.interval, m.max_Y, m.max_X]
+    for c in range(m.numcomponent):
+        comp += [int(m.ridx[c]) + 1, int(m.oidx[c]) + 1, f9(m.offw[c]), int(m.rsize[c][0]),
+                 int(m.rsize[c][1]), int(m.numpart[c])]
+        for k in range(int(m.numpart[c])):
+            comp += [int(m.pidx[c][k]) + 1, int(m.didx[c][k]) + 1, int(m.psize[c][k][0]),
+                     int(m.psize[c][k][1])]
+    comp.append(int(m.deff.shape[0]))
+    for q in range(m.deff.shape[0]):
+        comp += [f9(v) for v in m.deff[q]] + [int(m.anchor[q][0]), int(m.anchor[q][1])]
+    for c in range(m.numcomponent):
+        comp += [f9(v) for v in np.asarray(m.lsq[c], np.float32).ravel()]
+    root = [m.NoR]
+    for r in range(m.NoR):
+        root += [int(v) for v in m.root_size[r]] + [f9(v) for v in m.rootfilters[r]]
+    part = [m.NoP]
+    for p in range(m.NoP):
+        part += [int(v) for v in m.part_size[p]] + [f9(v) for v in m.partfilters[p]] + [0]
+    paths = []
+    for name, vals in (("comp", comp), ("root", root), ("part", part)):
+        path = _os.path.join(directory, f"{prefix}_{name}.csv")
+        with open(path, "w") as fh:
+            fh.write(",\n".join(str(v) for v in vals) + ",\n")
+        paths.append(path)
+    return tuple(paths)
+
+
 def pyramid(frame: np.ndarray, sbin: int, interval: int, max_scale: int, threads: int = 1):
     H, W, _ = frame.shape
     frame = np.ascontiguousarray(frame)
@@ -395,6 +452,13 @@ def load_csv_model(comp: str, root: str, part: str):
         pf = arr(d.partfilter, ptot, np.float32)
         rootfilters = [rf[roff[i]:roff[i] + int(np.prod(root_size[i]))].copy() for i in range(d.NoR)]
         partfilters = [pf[poff[i]:poff[i] + int(np.prod(part_size[i]))].copy() for i in range(d.NoP)]
+        nps = arr(d.numpart, NC, np.int32)
+        mls = [1 + 2 * (1 + int(n)) for n in nps]
+        lflat = arr(d.lsq, 4 * sum(mls), np.float32)
+        lsq, o = [], 0
+        for ml in mls:
+            lsq.append(lflat[o:o + 4 * ml].reshape(4, ml).copy())
+            o += 4 * ml
         return DPMModel(
             numcomponent=NC, sbin=d.sbin, interval=d.interval, max_Y=d.max_Y, max_X=d.max_X,
             ridx=arr(d.ridx, NC, np.int32), oidx=arr(d.oidx, NC, np.int32),
@@ -405,6 +469,6 @@ def load_csv_model(comp: str, root: str, part: str):
             deff=arr(d.deff, 4 * d.DefL, np.float32).reshape(-1, 4),
             anchor=arr(d.anchor, 2 * d.DefL, np.int32).reshape(-1, 2),
             root_size=root_size, rootfilters=rootfilters, part_size=part_size,
-            partfilters=partfilters, name=os.path.basename(comp).split("_")[0])
+            partfilters=partfilters, lsq=lsq, name=os.path.basename(comp).split("_")[0])
     finally:
         lib().ora_model_free(h)

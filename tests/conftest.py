@@ -39,4 +39,5 @@ def car_model():
         offw=z["offw"], rsize=z["rsize"], numpart=z["numpart"], pidx=z["pidx"], didx=z["didx"],
         psize=z["psize"], deff=z["deff"], anchor=z["anchor"], root_size=z["root_size"],
         rootfilters=[z[f"root{i}"] for i in range(nroot)], part_size=z["part_size"],
-        partfilters=[z[f"part{i}"] for i in range(npart)], name="car")
+        partfilters=[z[f"part{i}"] for i in range(npart)],
+        lsq=[z[f"lsq{i}"] for i in range(int(z["numcomponent"]))], name="car")

@@ -33,6 +33,8 @@ def car_model():
         d[f"root{i}"] = f
     for i, f in enumerate(m.partfilters):
         d[f"part{i}"] = f
+    for i, a in enumerate(m.lsq):
+        d[f"lsq{i}"] = a
     np.savez_compressed(os.path.join(HERE, "car_model.npz"), **d)
 
 

@@ -76,3 +76,46 @@ def test_host_nms_matches_oracle(n, tie, overlap):
     exp = O.nms(rows, overlap) if n else np.zeros((0, 11), np.float32)
     assert got.shape == exp.shape
     assert np.array_equal(got.view(np.int32), exp.view(np.int32))
+
+
+# the reference's declarations (CT/detect_func.h:23-45, CT/get_boxes_func.h:28-35,
+# CT/featurepyramid.cpp.new2:114,135, CT/load_model.cpp:42-45, CT/for_use_GPU.h:128-130),
+# as the demangler prints them with FLOAT = float
+REFERENCE_SIGNATURES = [
+    "ini_scales(Model_info*, _IplImage*, int, int)",
+    "ini_featsize(Model_info*)",
+    "calc_f_pyramid(_IplImage*, Model_info*, int*, float*)",
+    "get_boxes(float**, float*, int*, MODEL*, int*, float*, float)",
+    "free_features(float**, Model_info*)",
+    "nms(float*, float, int*, MODEL*)",
+    "get_new_rects(_IplImage*, MODEL*, float*, int*)",
+    "fconvsMT(float*, float*, float**, int*, int, int, int*, int**, int*)",
+    "dt(float*, float, float, float, float, int*, int*, int*)",
+    "add_part_calculation(float*, float*, int*, int*, int, int)",
+    "resize(float*, int*, int*, float)",
+    "calc_feature(float*, int*, int*, int)",
+    "load_model(float)",
+    "free_model(MODEL*)",
+    "init_cuda",
+    "clean_cuda",
+]
+
+
+def test_compat_library_exports_reference_signatures():
+    import subprocess
+
+    so = os.path.join(ROOT, "geye_amd", "libgeye_dpm_compat.so")
+    out = subprocess.run(["nm", "-D", "--defined-only", "-C", so], capture_output=True, text=True,
+                         check=True).stdout
+    names = {line.split(" ", 2)[2] for line in out.splitlines() if line.count(" ") >= 2}
+    missing = [s for s in REFERENCE_SIGNATURES if s not in names]
+    assert not missing, missing
+
+
+def test_compat_header_compiles_the_reference_call_sequence():
+    import subprocess
+
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "compat_detect.cpp")], capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr

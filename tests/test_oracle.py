@@ -395,3 +395,64 @@ def test_ascore_finish_threshold_and_all_below():
     z = np.full((3, 3), -100.0, np.float32)
     O.ascore_finish(z, -0.5)
     assert np.all(z == 0.0)
+
+
+# ---------------------------------------------------------------------------------------
+# get_new_rects (CT/tracking.cpp:90-166) and the CSV model format
+# ---------------------------------------------------------------------------------------
+def test_new_rects_hand_built_identity_regressor():
+    """The synthetic model's coefficients pick x1 = rX1, y1 = rY1, x2 = rX1 + (rX2 - rX1),
+    y2 = rY1 + (h / w)(rX2 - rX1); clamps at 0 and at the image size; OR_point = PP / ratio."""
+    m = synth.synthetic_car()
+    GL = (int(m.numpart[0]) + 1) * 4 + 3
+    rows = np.zeros((3, GL), np.float32)
+    rows[0, :4] = [10.5, 20.25, 60.0, 120.75]          # rY1, rX1, rY2, rX2 ; component 0 (6x11)
+    rows[1, :4] = [-8.0, -3.0, 500.0, 900.0]           # clamps (component 2, 7x10)
+    rows[2, :4] = [100.0, 200.0, 150.0, 230.0]         # component 5 (8x9)
+    rows[:, GL - 3] = [0, 2, 5]
+    rows[:, GL - 2] = [1.5, -0.25, 3.0]
+    rows[:, GL - 1] = [4.0, 8.0, 16.0]
+    out = O.new_rects(m, rows, 640, 480, ratio=0.5)
+    w0 = np.float32(120.75) - np.float32(20.25)
+    y2_0 = np.float32(10.5) + np.float32(np.float32(6.0 / 11.0) * w0)
+    y2_2 = np.float32(100.0) + np.float32(np.float32(8.0 / 9.0) * np.float32(30.0))
+    exp = np.array([[20, 10, 120, int(y2_0)], [0, 0, 640, 480], [200, 100, 230, int(y2_2)]])
+    assert np.array_equal(out["point"], exp)
+    assert np.array_equal(out["OR_point"], (exp / np.float32(0.5)).astype(np.int32))
+    assert list(out["type"]) == [0, 2, 5]
+    assert np.array_equal(out["score"], rows[:, GL - 2]) and np.array_equal(out["scale"], rows[:, GL - 1])
+
+
+def test_new_rects_real_car_coefficients(car_model):
+    """Real coefficients (CT/car_comp.csv:77-84): sequential float sums, checked against a
+    float64 dot product to rounding, and against truncation toward zero."""
+    m = car_model
+    GL = (int(m.numpart[0]) + 1) * 4 + 3
+    rng = np.random.default_rng(4)
+    rows = rng.uniform(20, 400, (40, GL)).astype(np.float32)
+    rows[:, GL - 3] = rng.integers(0, 2, 40)
+    out = O.new_rects(m, rows, 640, 480)
+    for i in range(40):
+        P = rows[i]
+        c = int(P[GL - 3])
+        avec = [P[3] - P[1]] + [v for k in range(0, GL - 3, 4) for v in (P[k + 1], P[k])]
+        x1, y1, x2, y2 = (np.dot(np.float64(avec), np.float64(m.lsq[c][q])) for q in range(4))
+        got = out["point"][i]
+        for g, v, lo, hi in ((got[0], x1, 0, None), (got[1], y1, 0, None), (got[2], x2, None, 640),
+                             (got[3], y2, None, 480)):
+            if lo is not None and v <= 0:
+                assert g == 0
+            elif hi is not None and v >= hi:
+                assert g == hi
+            else:
+                assert abs(g - v) <= 1.0 + 1e-3 * abs(v)
+
+
+def test_csv_writer_round_trips_through_the_loader(tmp_path):
+    m = synth.synthetic_person()
+    m2 = O.load_csv_model(*O.write_csv_model(m, str(tmp_path)))
+    for a, b in zip(m.rootfilters + m.partfilters, m2.rootfilters + m2.partfilters):
+        assert np.array_equal(a, b)
+    assert all(np.array_equal(np.asarray(a), b) for a, b in zip(m.lsq, m2.lsq))
+    assert np.array_equal(m.deff, m2.deff) and np.array_equal(m.anchor, m2.anchor)
+    assert np.array_equal(m.pidx, m2.pidx) and np.array_equal(m.psize, m2.psize)
