@@ -51,16 +51,17 @@ __global__ __launch_bounds__(256) void k_resize_rows(const ResizeOp *__restrict_
                                                      const float *__restrict__ src,
                                                      float *__restrict__ dst) {
   const ResizeOp op = ops[find_op(ops, nops, blockIdx.x)];
-  long long e = (long long)(blockIdx.x - op.blocks_begin) * blockDim.x + threadIdx.x;
-  long long per_c = (long long)op.out_h * op.in_w;
-  if (e >= 3 * per_c) return;
-  int c = (int)(e / per_c);
-  long long r = e - c * per_c;
-  int dy = (int)(r / op.in_w), x = (int)(r - (long long)dy * op.in_w);
-  const float *in = src + op.in_off + (long long)c * op.in_h * op.in_w + x;
-  int2 t = tab[op.tab_off + dy];
+  // 32-bit index arithmetic: an op covers at most 3 x 2^24 elements
+  const unsigned e = (blockIdx.x - op.blocks_begin) * blockDim.x + threadIdx.x;
+  const unsigned per_c = (unsigned)op.out_h * (unsigned)op.in_w;
+  if (e >= 3u * per_c) return;
+  const unsigned c = e / per_c;
+  const unsigned r = e - c * per_c;
+  const unsigned dy = r / (unsigned)op.in_w, x = r - dy * (unsigned)op.in_w;
+  const float *in = src + op.in_off + (c * (unsigned)op.in_h) * (unsigned)op.in_w + x;
+  const int2 t = tab[op.tab_off + dy];
   float acc = 0.0f;
-  for (int k = 0; k < t.y; k++) acc += alpha[t.x + k] * in[(long long)si[t.x + k] * op.in_w];
+  for (int k = 0; k < t.y; k++) acc += alpha[t.x + k] * in[(unsigned)si[t.x + k] * (unsigned)op.in_w];
   dst[op.out_off + e] = acc;
 }
 
@@ -71,14 +72,14 @@ __global__ __launch_bounds__(256) void k_resize_cols(const ResizeOp *__restrict_
                                                      const float *__restrict__ src,
                                                      float *__restrict__ dst) {
   const ResizeOp op = ops[find_op(ops, nops, blockIdx.x)];
-  long long e = (long long)(blockIdx.x - op.blocks_begin) * blockDim.x + threadIdx.x;
-  long long per_c = (long long)op.out_h * op.out_w;
-  if (e >= 3 * per_c) return;
-  int c = (int)(e / per_c);
-  long long r = e - c * per_c;
-  int y = (int)(r / op.out_w), dx = (int)(r - (long long)y * op.out_w);
-  const float *in = src + op.in_off + ((long long)c * op.in_h + y) * op.in_w;
-  int2 t = tab[op.tab_off + dx];
+  const unsigned e = (blockIdx.x - op.blocks_begin) * blockDim.x + threadIdx.x;
+  const unsigned per_c = (unsigned)op.out_h * (unsigned)op.out_w;
+  if (e >= 3u * per_c) return;
+  const unsigned c = e / per_c;
+  const unsigned r = e - c * per_c;
+  const unsigned y = r / (unsigned)op.out_w, dx = r - y * (unsigned)op.out_w;
+  const float *in = src + op.in_off + (c * (unsigned)op.in_h + y) * (unsigned)op.in_w;
+  const int2 t = tab[op.tab_off + dx];
   float acc = 0.0f;
   for (int k = 0; k < t.y; k++) acc += alpha[t.x + k] * in[si[t.x + k]];
   dst[op.out_off + e] = acc;
@@ -137,9 +138,13 @@ __global__ __launch_bounds__(T *T) void k_hog_hist(const HogLevel *__restrict__ 
   const int ya = max(1, first_pix(cy0 - 1, sbin));
   const int yb = min(vis0 - 2, first_pix(cy0 + T, sbin) - 1);
   constexpr int RMAX = (T + 1) * (T == 16 ? 4 : 8);
-  constexpr int RS = RMAX + 1;
-  __shared__ float s_v[RMAX * RS];
-  __shared__ int s_o[RMAX * RS];
+  // Pixel (rx, ry) of the tile lives at ry * RMAX + rx + ry / sbin.  Lanes of one wave read
+  // pixels sbin apart in both axes; the per-cell-row skew turns the row step into
+  // sbin * RMAX + 1 words, which spreads the lanes over the LDS banks.
+  constexpr int RS = RMAX;
+  constexpr int PIX = RMAX * RS + RMAX;
+  __shared__ float s_v[PIX];
+  __shared__ unsigned char s_o[PIX];
   __shared__ float s_acc[18 * T * T];
   __shared__ int s_ix[RMAX], s_iy[RMAX];
   __shared__ float s_vx0[RMAX], s_vx1[RMAX], s_vy0[RMAX], s_vy1[RMAX];
@@ -182,8 +187,9 @@ __global__ __launch_bounds__(T *T) void k_hog_hist(const HogLevel *__restrict__ 
           best_o = o + 9;
         }
       }
-      s_v[ry * RS + rx] = sqrtf(v);
-      s_o[ry * RS + rx] = best_o;
+      const int q = ry * RS + rx + ry / sbin;
+      s_v[q] = sqrtf(v);
+      s_o[q] = (unsigned char)best_o;
     }
   }
   __syncthreads();
@@ -196,13 +202,28 @@ __global__ __launch_bounds__(T *T) void k_hog_hist(const HogLevel *__restrict__ 
     const int py0 = max(ya, first_pix(cy - 1, sbin)) - ya;
     const int py1 = min(yb, first_pix(cy + 1, sbin) - 1) - ya;
     float *acc = s_acc + tid;
+    // the cell's pixel rows (at most 2 * sbin): weights and LDS row offsets in registers
+    constexpr int JM = 2 * (T == 16 ? 4 : 8);
+    const int npy = py1 - py0 + 1;
+    float wyr[JM];
+    int roff[JM];
+#pragma unroll
+    for (int j = 0; j < JM; j++) {
+      const int ry = max(0, min(py0 + j, py1));
+      wyr[j] = (s_iy[ry] == cy) ? s_vy1[ry] : s_vy0[ry];
+      roff[j] = ry * RS + ry / sbin;
+    }
+    // votes in the reference's pixel order: x outer, y inner
     for (int rx = px0; rx <= px1; rx++) {
       const float wx = (s_ix[rx] == cx) ? s_vx1[rx] : s_vx0[rx];
-      for (int ry = py0; ry <= py1; ry++) {
-        const float wy = (s_iy[ry] == cy) ? s_vy1[ry] : s_vy0[ry];
-        const float v = s_v[ry * RS + rx];
-        const int o = s_o[ry * RS + rx];
-        acc[o * T * T] += wx * wy * v;
+#pragma unroll
+      for (int j = 0; j < JM; j++) {
+        if (j < npy) {
+          const int q = roff[j] + rx;
+          const float v = s_v[q];
+          const int o = s_o[q];
+          acc[o * T * T] += wx * wyr[j] * v;
+        }
       }
     }
     const long long BSQ = (long long)L.b0 * L.b1;
@@ -256,42 +277,62 @@ __global__ __launch_bounds__(256) void k_hog_feat(const HogLevel *__restrict__ l
   const int x = (int)(e / L.F0), y = (int)(e - (long long)x * L.F0);
   const int b0 = L.b0;
   const long long BSQ = (long long)b0 * L.b1;
-  const float *norm = norm_arena + L.norm_off;
-  const float n1 = norm_factor(norm + (long long)(x + 1) * b0 + y + 1, b0);
-  const float n2 = norm_factor(norm + (long long)(x + 1) * b0 + y, b0);
-  const float n3 = norm_factor(norm + (long long)x * b0 + y + 1, b0);
-  const float n4 = norm_factor(norm + (long long)x * b0 + y, b0);
+  // the 3x3 norm neighbourhood (x..x+2, y..y+2) once; each normaliser sums its 2x2 window
+  // in the reference's order p[0] + p[1] + p[b0] + p[b0 + 1]
+  const float *nb = norm_arena + L.norm_off + (long long)x * b0 + y;
+  float nv[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) nv[i][j] = nb[(long long)i * b0 + j];
+  auto nf = [&](int i, int j) {
+    const float sum = nv[i][j] + nv[i][j + 1] + nv[i + 1][j] + nv[i + 1][j + 1];
+    return (float)(1.0 / sqrt((double)sum + 0.0001));
+  };
+  const float n1 = nf(1, 1), n2 = nf(1, 0), n3 = nf(0, 1), n4 = nf(0, 0);
   const float *src = hist_arena + L.hist_off + (long long)(x + 1) * b0 + (y + 1);
   float *dst = feat_arena + L.feat_off + e;
-  unsigned short *db = featb_arena ? featb_arena + L.featb_off + e * 32 : nullptr;
+  float fv[31];
   float t1 = 0.0f, t2 = 0.0f, t3 = 0.0f, t4 = 0.0f;
   float hs[18];
 #pragma unroll
+  for (int o = 0; o < 18; o++) hs[o] = src[o * BSQ];
+#pragma unroll
   for (int o = 0; o < 18; o++) {
-    float hv = src[o * BSQ];
-    hs[o] = hv;
+    const float hv = hs[o];
     float h1 = min_2(hv * n1), h2 = min_2(hv * n2), h3 = min_2(hv * n3), h4 = min_2(hv * n4);
-    float f = 0.5f * (h1 + h2 + h3 + h4);
-    dst[o * OSQ] = f;
-    if (db) db[o] = half_bits(f, hkind);
+    fv[o] = 0.5f * (h1 + h2 + h3 + h4);
     t1 += h1; t2 += h2; t3 += h3; t4 += h4;
   }
 #pragma unroll
   for (int o = 0; o < 9; o++) {
     float sum = hs[o] + hs[o + 9];
     float h1 = min_2(sum * n1), h2 = min_2(sum * n2), h3 = min_2(sum * n3), h4 = min_2(sum * n4);
-    float f = 0.5f * (h1 + h2 + h3 + h4);
-    dst[(18 + o) * OSQ] = f;
-    if (db) db[18 + o] = half_bits(f, hkind);
+    fv[18 + o] = 0.5f * (h1 + h2 + h3 + h4);
   }
-  float tx[4] = {(float)(0.2357 * (double)t1), (float)(0.2357 * (double)t2),
-                 (float)(0.2357 * (double)t3), (float)(0.2357 * (double)t4)};
+  fv[27] = (float)(0.2357 * (double)t1);
+  fv[28] = (float)(0.2357 * (double)t2);
+  fv[29] = (float)(0.2357 * (double)t3);
+  fv[30] = (float)(0.2357 * (double)t4);
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
-    dst[(27 + q) * OSQ] = tx[q];
-    if (db) db[27 + q] = half_bits(tx[q], hkind);
+  for (int o = 0; o < 31; o++) dst[o * OSQ] = fv[o];
+  if (featb_arena) {
+    // 32 16-bit channels (the last one 0) as four 16-byte stores; featb_off is a multiple
+    // of 32 elements, so every cell starts on a 64-byte boundary
+    uint4 *db = (uint4 *)(featb_arena + L.featb_off + e * 32);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      unsigned int w[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int c = q * 8 + r * 2;
+        const unsigned int lo = half_bits(fv[c], hkind);
+        const unsigned int hi = c + 1 < 31 ? half_bits(fv[c + 1], hkind) : 0u;
+        w[r] = lo | (hi << 16);
+      }
+      db[q] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
   }
-  if (db) db[31] = 0;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -399,6 +440,35 @@ __device__ inline float dt_cost(float src, float a, float b, int t) {
   return src + a * ft * ft + b * ft;
 }
 
+// Scans p in [p0, p1] in ascending order after (best, bi) was set from p0 - 1, keeping the
+// first minimum (the reference's strict "best > cost" switch).  Loads go out U at a time
+// so the LDS latency overlaps the compare chain.
+template <int U>
+__device__ inline void dt_scan(const float *ls, float a, float b, int d, int p0, int p1,
+                               float &best, int &bi) {
+  int p = p0;
+  for (; p + U - 1 <= p1; p += U) {
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) v[u] = ls[p + u];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const float c = dt_cost(v[u], a, b, d - (p + u));
+      if (best > c) {
+        best = c;
+        bi = p + u;
+      }
+    }
+  }
+  for (; p <= p1; p++) {
+    const float c = dt_cost(ls[p], a, b, d - p);
+    if (best > c) {
+      best = c;
+      bi = p;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_dt(const DtJob *__restrict__ jobs,
                                             const int4 *__restrict__ nodes,
                                             const int *__restrict__ depths,
@@ -412,6 +482,11 @@ __global__ __launch_bounds__(256) void k_dt(const DtJob *__restrict__ jobs,
   const int line = tid / G, g = tid - line * G;
   float *s_src = smem;                                  // [NLn][n+1]
   int *s_ptr = (int *)(smem + NLn * (n + 1));           // [NLn][n]
+  int2 *s_node = (int2 *)(s_ptr + NLn * n + (NLn * n & 1));  // [n] (d | d1 << 16, d2)
+  for (int i = tid; i < n; i += 256) {
+    const int4 nd = nodes[J.node_off + i];
+    s_node[i] = make_int2(nd.x | ((nd.y + 1) << 16), nd.z - 1);
+  }
   const float *src = src_arena + J.src_off + (long long)J.line0 * J.line_stride;
   // stage the lines
   if (J.elem_stride == 1) {
@@ -431,31 +506,25 @@ __global__ __launch_bounds__(256) void k_dt(const DtJob *__restrict__ jobs,
   const bool active_line = line < J.nlines;
   const float *ls = s_src + line * (n + 1);
   int *lp = s_ptr + line * n;
-  float *dst = dst_arena + J.dst_off + (long long)(J.line0 + line) * J.line_stride;
-  int *pdst = ptr_arena + J.dst_off + (long long)(J.line0 + line) * J.line_stride;
   const float a = J.a, b = J.b;
+  // divide and conquer, one tree depth at a time: a node (d1, d2) takes d = (d1 + d2) / 2
+  // and searches the sources between the argmins of its bounding ancestors d1 - 1 and
+  // d2 + 1 (CT/dt.cpp:61-85).  The nodes of each depth come from the LDS copy of the
+  // breadth-first node table of length n.
   for (int k = 0; k < J.depth_count; k++) {
     const int nb = depths[J.depth_off + k], ne = depths[J.depth_off + k + 1];
     const int nk = ne - nb;
     if (nk >= G) {
       for (int i = nb + g; i < ne; i += G) {
         if (!active_line) break;
-        const int4 nd = nodes[J.node_off + i];
-        const int d = nd.x;
-        const int s1 = nd.y < 0 ? 0 : lp[nd.y];
-        const int s2 = nd.z >= n ? n - 1 : lp[nd.z];
+        const int2 nd = s_node[i];
+        const int d = nd.x & 0xffff, d1 = nd.x >> 16, d2 = nd.y;
+        const int s1 = d1 == 0 ? 0 : lp[d1 - 1];
+        const int s2 = d2 == n - 1 ? n - 1 : lp[d2 + 1];
         int s = s1;
         float best = dt_cost(ls[s1], a, b, d - s1);
-        for (int p = s1 + 1; p <= s2; p++) {
-          float c = dt_cost(ls[p], a, b, d - p);
-          if (best > c) {
-            best = c;
-            s = p;
-          }
-        }
+        dt_scan<4>(ls, a, b, d, s1 + 1, s2, best, s);
         lp[d] = s;
-        dst[(long long)d * J.elem_stride] = best;
-        pdst[(long long)d * J.elem_stride] = s;
       }
     } else {
       int S = 1;
@@ -466,10 +535,11 @@ __global__ __launch_bounds__(256) void k_dt(const DtJob *__restrict__ jobs,
       int bi = 0x7fffffff;
       int d = 0;
       if (act) {
-        const int4 nd = nodes[J.node_off + nb + ni];
-        d = nd.x;
-        const int s1 = nd.y < 0 ? 0 : lp[nd.y];
-        const int s2 = nd.z >= n ? n - 1 : lp[nd.z];
+        const int2 nd = s_node[nb + ni];
+        d = nd.x & 0xffff;
+        const int d1 = nd.x >> 16, d2 = nd.y;
+        const int s1 = d1 == 0 ? 0 : lp[d1 - 1];
+        const int s2 = d2 == n - 1 ? n - 1 : lp[d2 + 1];
         const int len = s2 - s1 + 1;
         const int chunk = (len + S - 1) / S;
         const int p0 = s1 + part * chunk;
@@ -477,13 +547,7 @@ __global__ __launch_bounds__(256) void k_dt(const DtJob *__restrict__ jobs,
         if (p0 <= p1) {
           best = dt_cost(ls[p0], a, b, d - p0);
           bi = p0;
-          for (int p = p0 + 1; p <= p1; p++) {
-            float c = dt_cost(ls[p], a, b, d - p);
-            if (best > c) {
-              best = c;
-              bi = p;
-            }
-          }
+          dt_scan<4>(ls, a, b, d, p0 + 1, p1, best, bi);
         }
       }
       for (int m = 1; m < S; m <<= 1) {
@@ -494,13 +558,30 @@ __global__ __launch_bounds__(256) void k_dt(const DtJob *__restrict__ jobs,
           bi = oi;
         }
       }
-      if (act && part == 0) {
-        lp[d] = bi;
-        dst[(long long)d * J.elem_stride] = best;
-        pdst[(long long)d * J.elem_stride] = bi;
-      }
+      if (act && part == 0) lp[d] = bi;
     }
     __syncthreads();
+  }
+  // write M and the argmin back with the staging pattern (coalesced along the memory
+  // order); M is the same cost expression at the chosen argmin, hence the same bits
+  float *dst = dst_arena + J.dst_off + (long long)J.line0 * J.line_stride;
+  int *pdst = ptr_arena + J.dst_off + (long long)J.line0 * J.line_stride;
+  if (J.elem_stride == 1) {
+    for (int i = tid; i < J.nlines * n; i += 256) {
+      const int l = i / n, p = i - l * n;
+      const int sp = s_ptr[l * n + p];
+      const long long o = (long long)l * J.line_stride + p;
+      dst[o] = dt_cost(s_src[l * (n + 1) + sp], a, b, p - sp);
+      pdst[o] = sp;
+    }
+  } else {
+    for (int i = tid; i < J.nlines * n; i += 256) {
+      const int p = i / J.nlines, l = i - p * J.nlines;
+      const int sp = s_ptr[l * n + p];
+      const long long o = (long long)l * J.line_stride + (long long)p * J.elem_stride;
+      dst[o] = dt_cost(s_src[l * (n + 1) + sp], a, b, p - sp);
+      pdst[o] = sp;
+    }
   }
 }
 

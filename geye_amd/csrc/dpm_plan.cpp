@@ -147,7 +147,13 @@ static int dt_depth_count(const Plan &P, int n) {
   return k;
 }
 
-static size_t dt_lds_bytes(int n, int G) { return (size_t)(256 / G) * ((size_t)(n + 1) * 4 + (size_t)n * 4); }
+// per line: source (n + 1 floats, padded) and argmins (n ints); per block: the node table
+// (n int2, 8-byte aligned)
+static size_t dt_lds_bytes(int n, int G) {
+  const size_t NL = (size_t)(256 / G);
+  const size_t words = NL * (size_t)(n + 1) + NL * (size_t)n;
+  return (words + (words & 1)) * 4 + (size_t)n * 8;
+}
 
 static int dt_pick_G(int n) {
   for (int G = 1; G <= 64; G *= 2)
