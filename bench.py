@@ -5,11 +5,16 @@ rectangle-noise frame, the synthetic car-shaped model (6 components, roots {6x11
 7x10, 7x10, 8x9, 8x9}, 8 parts of 6x6 each, 31 HOG channels read), threshold -0.5, NMS
 overlap 0.5.  A step is one whole detection of that frame: resize + HOG pyramid, root and
 part convolution, DT, scoring, candidate compaction, NMS and the surviving rows copied
-back.  The frame is resident in HBM before the timed region starts.
+back.  Frames are resident in HBM before the timed region starts.
 
-Prints ONE JSON line (rank 0, N = 1): frames/s of the measured form, p50/p99 latency, the
-roofline of the dominant kernel (algorithmic FLOPs per launch / mean launch time from HIP
-events on the stream the kernel runs on) and the CPU oracle timed on this host's cores.
+Headline (`value`): the batched BF16-MFMA serving form -- the serving loop with several
+frames on the device at once (closed loop, `slots` requests in flight), the slot count
+being the fastest one whose p99 latency stays under 10 ms.  Beside it: the FP32 plain
+form (one frame at a time, one stream), the roofline of the dominant kernel (algorithmic
+bytes or FLOPs per launch / mean launch time from HIP events on the stream the kernel runs
+on), BF16 vs FP32 detection agreement, the two-model batch (16 x 1280x720, car + person),
+the four-camera Poisson serving mix, and the CPU oracle timed on this host's cores.
+Prints ONE JSON line.
 """
 from __future__ import annotations
 
@@ -62,13 +67,117 @@ def cpu_baseline(model, frame, threads: int) -> dict:
             "n_candidates": int(len(boxes)), "n_boxes": int(len(kept))}
 
 
+def iou_agreement(ref: np.ndarray, got: np.ndarray, thr: float = 0.99) -> dict:
+    """Fraction of boxes of each set matched by a box of the other at IoU >= thr (root boxes)."""
+    if len(ref) == 0 or len(got) == 0:
+        same = float(len(ref) == len(got))
+        return {"ref_matched": same, "got_matched": same, "n_ref": int(len(ref)), "n_got": int(len(got))}
+    a, b = ref[:, :4].astype(np.float64), got[:, :4].astype(np.float64)
+    y1 = np.maximum(a[:, None, 0], b[None, :, 0]); x1 = np.maximum(a[:, None, 1], b[None, :, 1])
+    y2 = np.minimum(a[:, None, 2], b[None, :, 2]); x2 = np.minimum(a[:, None, 3], b[None, :, 3])
+    inter = np.clip(y2 - y1 + 1, 0, None) * np.clip(x2 - x1 + 1, 0, None)
+    aa = (a[:, 2] - a[:, 0] + 1) * (a[:, 3] - a[:, 1] + 1)
+    ab = (b[:, 2] - b[:, 0] + 1) * (b[:, 3] - b[:, 1] + 1)
+    m = inter / (aa[:, None] + ab[None, :] - inter)
+    return {"ref_matched": float((m.max(axis=1) >= thr).mean()),
+            "got_matched": float((m.max(axis=0) >= thr).mean()),
+            "n_ref": int(len(ref)), "n_got": int(len(got)), "iou": thr}
+
+
+def serve_closed_loop(geye_amd, gm, d_frame, precision, slots, steps, warmup):
+    """`steps` detections of one resident frame with `slots` in flight; (fps, p50, p99)."""
+    with geye_amd.Server(slots=slots, precision=precision, queue_cap=4 * slots + 4) as srv:
+        for _ in range(warmup):  # a batch of `slots` reaches every worker
+            for t in [srv.submit_dev(gm, d_frame, W, H) for _ in range(slots)]:
+                srv.wait(t)
+        srv.reset_stats()
+        inflight, lat = [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            inflight.append(srv.submit_dev(gm, d_frame, W, H))
+            if len(inflight) >= slots:
+                lat.append(srv.wait(inflight.pop(0))[1])
+        for t in inflight:
+            lat.append(srv.wait(t)[1])
+        total = time.perf_counter() - t0
+    lat = np.array(lat)
+    return {"slots": slots, "fps": steps / total, "ms_per_step": total / steps * 1e3,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
+
+
+def two_model_batch(geye_amd, ctx, synth, slots):
+    """16 seeded 1280x720 frames, each detected with the car and the person model (BF16)."""
+    car, person = ctx.model(synth.synthetic_car()), ctx.model(synth.synthetic_person())
+    frames = [ctx.dev_upload(synth.make_frame(1280, 720, seed=300 + i)) for i in range(16)]
+    with geye_amd.Server(slots=slots, precision="bf16", queue_cap=64) as srv:
+        for _ in range(2):
+            for t in [srv.submit_dev(m, frames[0], 1280, 720) for m in (car, person)
+                      for _ in range(slots)]:
+                srv.wait(t)
+        t0 = time.perf_counter()
+        tickets = [srv.submit_dev(m, f, 1280, 720) for f in frames for m in (car, person)]
+        lat = [srv.wait(t)[1] for t in tickets]
+        total = time.perf_counter() - t0
+    for f in frames:
+        ctx.dev_free(f)
+    car.free()
+    person.free()
+    return {"workload": "16 x 1280x720 frames x (synthetic car + synthetic person), bf16",
+            "slots": slots, "frames_per_s": 16 / total, "batch_ms": total * 1e3,
+            "max_request_ms": float(max(lat))}
+
+
+def serving_mix(geye_amd, ctx, synth, slots, seconds: float = 2.0):
+    """Four cameras (640x480, 1280x720, 1920x1080, 3840x2160), Poisson arrivals at 30 fps
+    each, every frame detected with the car and the person model (BF16, host frames)."""
+    sizes = [(640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
+    car, person = ctx.model(synth.synthetic_car()), ctx.model(synth.synthetic_person())
+    frames = {sz: synth.make_frame(sz[0], sz[1], seed=500 + i) for i, sz in enumerate(sizes)}
+    rng = np.random.default_rng(11)
+    events = []
+    for sz in sizes:
+        t = 0.0
+        while True:
+            t += rng.exponential(1.0 / 30.0)
+            if t >= seconds:
+                break
+            events.append((t, sz))
+    events.sort()
+    with geye_amd.Server(slots=slots, precision="bf16", queue_cap=1024) as srv:
+        for _ in range(2):  # working memory of every (model, size) resident before the clock
+            for t in [srv.submit(m, frames[sz]) for sz in sizes for m in (car, person)
+                      for _ in range(slots)]:
+                srv.wait(t)
+        srv.reset_stats()
+        pending = []
+        t0 = time.perf_counter()
+        for t, sz in events:
+            dt = t - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            pending.append((srv.submit(car, frames[sz]), srv.submit(person, frames[sz])))
+        lat = [max(srv.wait(a)[1], srv.wait(b)[1]) for a, b in pending]
+        total = time.perf_counter() - t0
+        st = srv.stats()
+    car.free()
+    person.free()
+    lat = np.array(lat)
+    return {"workload": "4 Poisson 30 fps cameras (640x480, 1280x720, 1920x1080, 3840x2160), "
+                        "car + person per frame, bf16, frames from host memory",
+            "slots": slots, "frames": int(len(events)), "offered_fps": len(events) / seconds,
+            "frames_per_s": len(events) / total, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "peak_working_bytes": st["peak_bytes"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "fp16"])
+    ap.add_argument("--slots", type=int, default=0, help="serving slots (0 = sweep 1..4)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true", help="skip the two-model and mix legs")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -88,14 +197,14 @@ def main() -> int:
     level_valid = [(l >= model.interval and plan["feat_h"][l] >= model.max_Y
                     and plan["feat_w"][l] >= model.max_X) for l in range(plan["LEN"])]
 
-    def measure(precision: str) -> dict:
+    def single_stream(precision: str, steps: int) -> dict:
         for _ in range(args.warmup):
             boxes = gm.detect_dev(d_frame, W, H, -0.5, 0.5, precision)
         ctx.synchronize()
         ctx.profile(True)
         lat = []
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             s = time.perf_counter()
             boxes = gm.detect_dev(d_frame, W, H, -0.5, 0.5, precision)
             lat.append((time.perf_counter() - s) * 1e3)
@@ -104,32 +213,62 @@ def main() -> int:
         stats = ctx.kernel_stats()
         ctx.profile(False)
         lat = np.array(lat)
-        return {"fps": args.steps / total, "ms_per_step": total / args.steps * 1e3,
+        return {"fps": steps / total, "ms_per_step": total / steps * 1e3,
                 "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                "stats": stats, "boxes": boxes}
+                "stats": stats, "boxes": boxes, "steps": steps}
 
-    head = measure(args.precision)
-    plain = measure("fp32") if args.precision != "fp32" else head
+    # one frame at a time: the kernel breakdown and the rooflines come from this run
+    single = single_stream(args.precision, 20)
+    plain = single_stream("fp32", 20) if args.precision != "fp32" else single
+
+    # the batched serving form: the headline
+    sweep = []
+    for slots in ([args.slots] if args.slots else [1, 2, 3, 4]):
+        sweep.append(serve_closed_loop(geye_amd, gm, d_frame, args.precision, slots,
+                                       args.steps, args.warmup))
+    ok = [r for r in sweep if r["p99_ms"] < 10.0] or sweep
+    head = max(ok, key=lambda r: r["fps"])
+
+    # dominant kernel: the one with the most time per frame in the single-stream run
+    st = single["stats"]
+    per_frame = {k: v[0] / single["steps"] for k, v in st.items()}
+    kgroups = {"dt": ["dt_cols", "dt_rows"]}
+    dt_ms = sum(st[k][0] for k in kgroups["dt"] if k in st)
+    dt_launches = sum(st[k][1] for k in kgroups["dt"] if k in st)
+    part_cells = 0
+    for l in range(model.interval, plan["LEN"]):
+        if level_valid[l]:
+            L = l - model.interval
+            for (h, w, _) in model.part_size.tolist():
+                part_cells += (int(plan["feat_h"][L]) - h + 1) * (int(plan["feat_w"][L]) - w + 1)
+    roof = None
+    if dt_launches:
+        launches_per_frame = dt_launches / single["steps"]
+        bytes_per_launch = 16 * part_cells / launches_per_frame
+        mean_s = dt_ms / dt_launches / 1e3
+        roof = {"bound": "hbm", "achieved": bytes_per_launch / mean_s / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": bytes_per_launch / mean_s / HBM_PEAK, "traffic": None,
+                "kernel": "k_dt (dt_cols + dt_rows launches)",
+                "algorithmic_bytes_per_launch": int(bytes_per_launch),
+                "per_unit": "16 B per part score cell (read score, write M, Ix, Iy)",
+                "mean_launch_ms": mean_s * 1e3}
 
     fl = conv_flops(plan, model, level_valid)
     tag = {"fp32": "conv_f32_", "bf16": "conv_bf16_", "fp16": "conv_fp16_"}[args.precision]
-    conv = {k[len(tag):]: v for k, v in head["stats"].items() if k.startswith(tag)}
+    conv = {k[len(tag):]: v for k, v in st.items() if k.startswith(tag)}
     dom = max(conv, key=lambda k: conv[k][0]) if conv else None
-    roof = None
+    conv_roof = None
     if dom:
         ms, cnt = conv[dom]
         mean_s = ms / cnt / 1e3
-        # the dominant class holds every filter of its shape; its algorithmic FLOPs are the
-        # sum over those filters at their true sizes
         flops = fl.get(dom, 0)
         if dom not in fl:  # the merged root class: all roots
             flops = sum(v for k, v in fl.items() if k != "6x6")
         achieved = flops / mean_s
         peak = MI355X_PEAK[args.precision]
-        roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
-                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                "kernel": f"{tag}{dom}", "flops_per_launch": flops,
-                "mean_launch_ms": ms / cnt}
+        conv_roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "kernel": f"{tag}{dom}",
+                     "flops_per_launch": flops, "mean_launch_ms": ms / cnt}
 
     line = {
         "metric": "frames/s + p50/p99 ms, BF16-MFMA batched vs FP32 plain, 1080p car DPM, 1 GPU",
@@ -148,14 +287,22 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"{W}x{H} rectangle-noise frame, synthetic car DPM (6 comp x 8 "
                                f"parts 6x6), thresh -0.5, NMS 0.5, {args.precision} conv",
-                   "form": "plain" if args.precision == "fp32" else "mfma",
-                   "levels": int(plan["LEN"]), "boxes": int(len(head["boxes"]))},
+                   "form": "serving, closed loop" if args.precision != "fp32" else "serving, fp32",
+                   "slots": head["slots"], "levels": int(plan["LEN"]),
+                   "boxes": int(len(single["boxes"]))},
         "roofline": roof,
+        "conv_roofline": conv_roof,
         "fp32_plain": {"fps": plain["fps"], "p50_ms": plain["p50_ms"], "p99_ms": plain["p99_ms"],
-                       "boxes": int(len(plain["boxes"]))},
+                       "boxes": int(len(plain["boxes"])), "form": "one frame at a time, one stream"},
         "speedup_vs_fp32_plain": head["fps"] / plain["fps"],
-        "kernel_ms_per_frame": {k: v[0] / max(1, args.steps) for k, v in sorted(head["stats"].items())},
+        "single_stream": {"fps": single["fps"], "p50_ms": single["p50_ms"], "p99_ms": single["p99_ms"]},
+        "slot_sweep": sweep,
+        "detection_agreement_vs_fp32": iou_agreement(plain["boxes"], single["boxes"]),
+        "kernel_ms_per_frame": {k: round(v, 4) for k, v in sorted(per_frame.items())},
     }
+    if not args.no_extra:
+        line["two_model_batch"] = two_model_batch(geye_amd, ctx, synth, head["slots"])
+        line["serving_mix"] = serving_mix(geye_amd, ctx, synth, max(head["slots"], 2))
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(model, frame, threads)

@@ -85,6 +85,16 @@ _SIGS = [
                                           _fp, C.c_longlong, C.c_float, C.POINTER(_fp), _ip,
                                           _ip]),
     ("geye_add_part", C.c_int, [C.c_void_p, _fp, _fp, _ip, _ip, C.c_int, C.c_int]),
+    ("geye_serve_create", C.c_int, [C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("geye_serve_submit", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                    _llp]),
+    ("geye_serve_submit_dev", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                        C.c_int, _llp]),
+    ("geye_serve_wait", C.c_int, [C.c_void_p, C.c_longlong, C.POINTER(_fp), _ip, _ip, _dp]),
+    ("geye_serve_stats", C.c_int, [C.c_void_p, _llp, _dp, _dp, _dp, _llp, _llp]),
+    ("geye_serve_reset_stats", C.c_int, [C.c_void_p]),
+    ("geye_plan_bytes", C.c_int, [C.c_void_p, C.c_int, C.c_int, _llp]),
+    ("geye_serve_destroy", None, [C.c_void_p]),
     ("geye_free", None, [C.c_void_p]),
     ("geye_resize", C.c_int, [C.c_void_p, _fp, _ip, C.c_float, _fp, _ip]),
     ("geye_calc_feature", C.c_int, [C.c_void_p, _fp, C.c_int, C.c_int, C.c_int, _fp, _ip, _ip]),
@@ -206,6 +216,78 @@ def nms_host(boxes: np.ndarray, overlap: float) -> np.ndarray:
     _check(lib().geye_nms_host(_p(boxes, _fp), n, GL, C.c_float(overlap), _p(out, _fp),
                                C.byref(k)))
     return out[:k.value].copy()
+
+
+class ServeConfig(C.Structure):
+    """geye_serve_config (include/geye_dpm.h)."""
+
+    _fields_ = [("slots", C.c_int), ("hbm_budget", C.c_longlong), ("queue_cap", C.c_int),
+                ("precision", C.c_int), ("thresh", C.c_float), ("overlap", C.c_float)]
+
+
+class Server:
+    """Serving loop over several camera streams (geye_serve_*)."""
+
+    def __init__(self, device: int = 0, slots: int = 4, hbm_budget: int = 0, queue_cap: int = 256,
+                 precision: str = "bf16", thresh: float = -0.5, overlap: float = 0.5):
+        cfg = ServeConfig(slots, hbm_budget, queue_cap, PRECISIONS[precision], thresh, overlap)
+        h = C.c_void_p()
+        rc = lib().geye_serve_create(device, C.byref(cfg), C.byref(h))
+        if rc != 0:
+            raise GeyeError(f"geye_serve_create failed ({rc}): {lib().geye_last_error().decode()}")
+        self.h = h
+
+    def submit(self, model, frame: np.ndarray) -> int:
+        H, W, _ = frame.shape
+        frame = np.ascontiguousarray(frame, dtype=np.uint8)
+        t = C.c_longlong()
+        rc = lib().geye_serve_submit(self.h, model.h, frame.ctypes.data, W, H, 3 * W, C.byref(t))
+        if rc == -6:
+            raise BlockingIOError("serving queue full")
+        _check(rc)
+        return t.value
+
+    def submit_dev(self, model, d_frame, W: int, H: int) -> int:
+        """Device-resident frame (Context.dev_upload); keep it alive until waited for."""
+        t = C.c_longlong()
+        rc = lib().geye_serve_submit_dev(self.h, model.h, d_frame, W, H, 3 * W, C.byref(t))
+        if rc == -6:
+            raise BlockingIOError("serving queue full")
+        _check(rc)
+        return t.value
+
+    def wait(self, ticket: int):
+        bp = _fp()
+        n, GL = C.c_int(), C.c_int()
+        lat = C.c_double()
+        _check(lib().geye_serve_wait(self.h, ticket, C.byref(bp), C.byref(n), C.byref(GL),
+                                     C.byref(lat)))
+        rows = np.ctypeslib.as_array(bp, shape=(max(n.value, 1) * GL.value,))[: n.value * GL.value]
+        rows = rows.copy().reshape(n.value, GL.value)
+        lib().geye_free(bp)
+        return rows, lat.value
+
+    def stats(self) -> dict:
+        c, ev, pk = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        p50, p99, fps = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().geye_serve_stats(self.h, C.byref(c), C.byref(p50), C.byref(p99), C.byref(fps),
+                                      C.byref(pk), C.byref(ev)))
+        return dict(completed=c.value, p50_ms=p50.value, p99_ms=p99.value, fps=fps.value,
+                    peak_bytes=pk.value, evictions=ev.value)
+
+    def reset_stats(self):
+        _check(lib().geye_serve_reset_stats(self.h))
+
+    def close(self):
+        if self.h:
+            lib().geye_serve_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 class Context:
@@ -459,6 +541,12 @@ class Model:
         flat = np.ascontiguousarray(np.concatenate([np.asarray(f, np.float32).ravel() for f in feats]))
         return self._rows(lib().geye_get_boxes_features, self.ctx.h, self.h, W, H, max_scale,
                           _p(flat, _fp), flat.size, C.c_float(thresh))
+
+    def plan_bytes(self, W: int, H: int) -> int:
+        """Working memory of one W x H frame, as a server's HBM budget counts it."""
+        b = C.c_longlong()
+        _check(lib().geye_plan_bytes(self.h, W, H, C.byref(b)))
+        return b.value
 
     def last_scores(self) -> np.ndarray:
         total = C.c_longlong()

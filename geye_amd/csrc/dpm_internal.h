@@ -203,7 +203,6 @@ struct Model {
   std::vector<float> lsq;
   std::vector<int> lsq_idx;                       // [offsets C][lengths C]
   DeviceBuffer d_filters, d_rsize, d_filt_bf16, d_filt_f16, d_lsq, d_lsq_idx;
-  std::unordered_map<long long, Plan *> plans;   // key = (W << 20) | H
 };
 
 struct KernelStat {

@@ -11,6 +11,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <map>
+#include <tuple>
 #include <memory>
 #include <string>
 
@@ -490,28 +492,96 @@ void plan_release(Plan &P) {
     release(*b);
 }
 
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// Plan registry.  A plan (tables + every per-frame arena) belongs to one (context, model,
+// frame geometry): contexts that run concurrently (serving slots) never share arenas,
+// while the model's filters are shared read-only.
+// ---------------------------------------------------------------------------------------
+namespace geye {
+namespace {
+std::mutex g_plan_mu;
+std::map<std::tuple<const geye_ctx *, const Model *, long long>, Plan *> g_plans;
+}  // namespace
+
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+
+long long plan_key(int W, int H, int max_scale) {
+  return ((long long)std::max(max_scale, 0) << 40) | ((long long)W << 20) | (long long)H;
+}
+
 // max_scale > 0 pins the pyramid depth (ini_scales computes it once per model from the
 // first frame, CT/featurepyramid.cpp.new2:179-226); otherwise it follows this frame.
-int get_plan(Model &m, int W, int H, Plan **out, int max_scale = -1) {
-  const long long key = ((long long)std::max(max_scale, 0) << 40) | ((long long)W << 20) | (long long)H;
-  auto it = m.plans.find(key);
-  if (it != m.plans.end()) {
-    *out = it->second;
-    return GEYE_OK;
+int get_plan(geye_ctx *c, Model &m, int W, int H, Plan **out, int max_scale) {
+  const auto key = std::make_tuple((const geye_ctx *)c, (const Model *)&m, plan_key(W, H, max_scale));
+  {
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto it = g_plans.find(key);
+    if (it != g_plans.end()) {
+      *out = it->second;
+      return GEYE_OK;
+    }
   }
   std::unique_ptr<Plan> P(new Plan());
   std::string err;
   int rc = build_plan(m, W, H, *P, err, max_scale > 0 ? max_scale : -1);
   if (rc != GEYE_OK) return fail(rc, err);
-  rc = plan_upload(*P, m.ctx->stream);
+  rc = plan_upload(*P, c->stream);
   if (rc != GEYE_OK) {
     plan_release(*P);
     return rc;
   }
   *out = P.release();
-  m.plans[key] = *out;
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  g_plans[key] = *out;
   return GEYE_OK;
 }
+
+// Device bytes a plan for this geometry holds once the frame has run, with the NMS buffers
+// at their largest (every score cell a candidate).  Host-side: builds the tables only.
+int plan_bytes(const Model &m, int W, int H, long long *bytes) {
+  Plan P;
+  std::string err;
+  int rc = build_plan(const_cast<Model &>(m), W, H, P, err);
+  if (rc != GEYE_OK) return fail(rc, err);
+  long long tmp = 0;
+  for (auto &R : P.rounds) tmp = std::max(tmp, R.tmp_total);
+  const long long N = std::max<long long>(P.score_total, 1);
+  long long b = 4 * (P.img_total + tmp + P.hist_total + P.norm_total + P.feat_total + P.conv_total +
+                     4 * P.dt_total + 3 * P.score_total) +
+                2 * P.featb_total + (long long)P.W * P.H * (3 + 4) +
+                N * (2 * (long long)sizeof(NmsBox) + 16 * 4) + (long long)P.GL * 4 * N;
+  *bytes = b;
+  return GEYE_OK;
+}
+
+// Releases the plans matching (c, m, key); a null c, m or key < 0 matches any.
+void release_plans(const geye_ctx *c, const Model *m, long long key) {
+  std::vector<Plan *> gone;
+  {
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    for (auto it = g_plans.begin(); it != g_plans.end();) {
+      const bool hit = (!c || std::get<0>(it->first) == c) && (!m || std::get<1>(it->first) == m) &&
+                       (key < 0 || std::get<2>(it->first) == key);
+      if (hit) {
+        geye_ctx *owner = const_cast<geye_ctx *>(std::get<0>(it->first));
+        if (owner->last_plan == it->second) owner->last_plan = nullptr;
+        gone.push_back(it->second);
+        it = g_plans.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
+  for (Plan *P : gone) {
+    plan_release(*P);
+    delete P;
+  }
+}
+}  // namespace geye
+
+namespace {
 
 // ---------------------------------------------------------------------------------------
 // Pipeline stages
@@ -806,7 +876,7 @@ int detect_common(geye_ctx *c, const geye_model *model, const unsigned char *bgr
   (void)hipGetLastError();
   Model &m = const_cast<Model &>(model->m);
   Plan *P = nullptr;
-  CHK(get_plan(m, W, H, &P));
+  CHK(get_plan(c, m, W, H, &P, -1));
   const unsigned char *d_bgr = bgr;
   if (!dev) CHK(frame_in(c, *P, bgr, W, H, ws, &d_bgr));
   CHK(run_pyramid(c, *P, d_bgr, ws, precision == GEYE_FP32 ? 0 : precision));
@@ -978,6 +1048,7 @@ void geye_shutdown(geye_ctx *c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   flush_stats(c);
+  release_plans(c, nullptr, -1);
   for (auto &b : c->scratch_bufs) release(b);
   (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1017,14 +1088,7 @@ int geye_model_load_csv(geye_ctx *c, const char *comp, const char *root, const c
 void geye_model_free(geye_model *gm) {
   if (!gm) return;
   Model &m = gm->m;
-  if (m.ctx && m.ctx->last_plan) {
-    for (auto &kv : m.plans)
-      if (kv.second == m.ctx->last_plan) m.ctx->last_plan = nullptr;
-  }
-  for (auto &kv : m.plans) {
-    plan_release(*kv.second);
-    delete kv.second;
-  }
+  release_plans(nullptr, &m, -1);
   release(m.d_filters);
   release(m.d_rsize);
   release(m.d_filt_bf16);
@@ -1265,7 +1329,7 @@ int geye_get_boxes_features(geye_ctx *c, const geye_model *model, int width, int
   (void)hipGetLastError();
   Model &m = const_cast<Model &>(model->m);
   Plan *P = nullptr;
-  CHK(get_plan(m, width, height, &P, max_scale));
+  CHK(get_plan(c, m, width, height, &P, max_scale));
   long long total = 0;
   auto lvl_n = [&](int l) { return 31LL * std::max(0, P->lv[l].F0) * std::max(0, P->lv[l].F1); };
   for (int l = 0; l < P->LEN; l++) total += lvl_n(l);

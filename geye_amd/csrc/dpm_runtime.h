@@ -21,6 +21,15 @@ int dt_single_jobs(int n0, int n1, float ax, float bx, float ay, float by,
                    std::vector<DtJob> &p1, std::vector<DtJob> &p2, std::vector<int4> &nodes,
                    std::vector<int> &depths, size_t &lds1, size_t &lds2);
 
+// sets the calling thread's geye_last_error() message; returns code
+int set_error(int code, const std::string &msg);
+
+// plan registry (dpm_runtime.cpp)
+long long plan_key(int W, int H, int max_scale);
+int get_plan(geye_ctx *c, Model &m, int W, int H, Plan **out, int max_scale);
+int plan_bytes(const Model &m, int W, int H, long long *bytes);
+void release_plans(const geye_ctx *c, const Model *m, long long key);
+
 // host NMS with the reference's exact semantics (CT/nms.cpp:70-217)
 int nms_host(const float *boxes, int n, int GL, float overlap, float *out);
 void quicksort_host(float *ary, int *order, int first, int last);

@@ -1,0 +1,354 @@
+// dpm_serve.cpp -- the serving loop: several camera streams, frames admitted by HBM budget,
+// several frames on the device at once.
+//
+// The reference runs one frame at a time from main() (CT/main.cpp:240-300); this server is
+// the device-side equivalent for many cameras.  A server owns `slots` workers; each worker
+// has its own device context (its own HIP stream and its own per-frame arenas), so frames
+// of different workers overlap on the GPU while every model's filters stay resident once.
+// Requests queue in arrival order.  A worker takes the oldest request whose working memory
+// (the plan of its model and frame size on that worker) is already resident or fits in the
+// HBM budget, evicting plans of idle workers (least recently used first) when needed; a
+// request that cannot fit waits until memory frees up.  Each frame runs the same kernels
+// as geye_detect, so a result does not depend on what else is in flight.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "dpm_internal.h"
+#include "dpm_runtime.h"
+
+using namespace geye;
+
+namespace {
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+  long long ticket = 0;
+  const geye_model *model = nullptr;
+  std::vector<unsigned char> frame;        // host copy, or empty for a device frame
+  const void *dev = nullptr;               // device-resident frame (caller-owned)
+  int W = 0, H = 0, ws = 0;
+  double t_submit = 0;
+};
+
+struct Result {
+  int rc = GEYE_OK;
+  std::string err;
+  float *rows = nullptr;
+  int num = 0, GL = 0, slot = -1;
+  double t_submit = 0, t_start = 0, t_done = 0;
+};
+
+struct Resident {             // a plan held by one worker
+  int slot;
+  const Model *model;
+  long long key;
+  long long bytes;
+  double last_use;
+};
+
+}  // namespace
+
+struct geye_server {
+  geye_serve_config cfg{};
+  int device = 0;
+  std::vector<geye_ctx *> ctx;
+  std::vector<unsigned char *> pinned;        // per worker: frame staging (pinned host memory)
+  std::vector<size_t> pinned_bytes;
+  std::vector<std::thread> workers;
+  std::vector<bool> busy;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request> queue;
+  std::unordered_map<long long, Result> done;
+  std::vector<Resident> resident;
+  std::map<std::pair<const Model *, long long>, long long> bytes_cache;
+  long long held = 0, peak = 0, next_ticket = 1, completed = 0, evictions = 0;
+  std::vector<double> latencies;
+  double t_first = -1, t_last = 0;
+  bool stop = false;
+};
+
+namespace {
+
+// Memory for (model, frame) on worker s: 0 when resident there, else its estimate.
+long long need_bytes(geye_server *S, int s, const Model *m, int W, int H, long long key,
+                     bool *resident_here) {
+  *resident_here = false;
+  for (auto &r : S->resident)
+    if (r.slot == s && r.model == m && r.key == key) {
+      *resident_here = true;
+      return 0;
+    }
+  auto ck = std::make_pair(m, key);
+  auto it = S->bytes_cache.find(ck);
+  if (it != S->bytes_cache.end()) return it->second;
+  long long b = 0;
+  if (plan_bytes(*m, W, H, &b) != GEYE_OK) b = 0;  // the run reports the error
+  S->bytes_cache[ck] = b;
+  return b;
+}
+
+// Frees plans of idle workers (oldest use first) until `need` more bytes fit.  Called with
+// S->mu held; the evicted workers are idle, so none of their kernels is running.
+bool make_room(geye_server *S, long long need) {
+  if (S->cfg.hbm_budget <= 0) return true;
+  while (S->held + need > S->cfg.hbm_budget) {
+    int victim = -1;
+    for (int i = 0; i < (int)S->resident.size(); i++) {
+      const Resident &r = S->resident[i];
+      if (S->busy[r.slot]) continue;
+      if (victim < 0 || r.last_use < S->resident[victim].last_use) victim = i;
+    }
+    if (victim < 0) return false;
+    const Resident r = S->resident[victim];
+    release_plans(S->ctx[r.slot], r.model, r.key);
+    S->held -= r.bytes;
+    S->resident.erase(S->resident.begin() + victim);
+    S->evictions++;
+  }
+  return true;
+}
+
+void worker(geye_server *S, int s) {
+  std::unique_lock<std::mutex> lk(S->mu);
+  for (;;) {
+    // the oldest request, once this worker can hold its working memory (arrival order is
+    // kept: the head waits for memory rather than being overtaken)
+    auto pick = S->queue.end();
+    long long need = 0;
+    bool here = false;
+    while (!S->stop) {
+      pick = S->queue.end();
+      if (!S->queue.empty()) {
+        auto it = S->queue.begin();
+        const Model *m = &it->model->m;
+        const long long key = plan_key(it->W, it->H, -1);
+        need = need_bytes(S, s, m, it->W, it->H, key, &here);
+        if (!here && S->cfg.hbm_budget > 0 && need > S->cfg.hbm_budget) {
+          // can never fit: answer it now instead of blocking the queue
+          Result res;
+          res.rc = GEYE_ERR_NOMEM;
+          res.err = "frame working memory exceeds the server's HBM budget";
+          res.t_submit = it->t_submit;
+          res.t_start = res.t_done = now_ms();
+          const long long t = it->ticket;
+          S->queue.erase(it);
+          S->done[t] = std::move(res);
+          S->cv_done.notify_all();
+          continue;
+        }
+        if (here || make_room(S, need)) pick = it;
+      }
+      if (pick != S->queue.end()) break;
+      S->cv_work.wait(lk);
+    }
+    if (S->stop) return;
+    Request req = std::move(*pick);
+    S->queue.erase(pick);
+    S->busy[s] = true;
+    const long long key = plan_key(req.W, req.H, -1);
+    if (!here) {
+      S->resident.push_back(Resident{s, &req.model->m, key, need, now_ms()});
+      S->held += need;
+      S->peak = std::max(S->peak, S->held);
+    }
+    lk.unlock();
+
+    Result res;
+    res.t_submit = req.t_submit;
+    res.t_start = now_ms();
+    res.slot = s;
+    const size_t fb = req.frame.size();
+    if (req.dev) {
+      res.rc = geye_detect_dev(S->ctx[s], req.model, req.dev, req.W, req.H, req.ws, S->cfg.thresh,
+                               S->cfg.overlap, S->cfg.precision, &res.rows, &res.num, &res.GL);
+    } else {
+    if (S->pinned_bytes[s] < fb) {
+      if (S->pinned[s]) (void)hipHostFree(S->pinned[s]);
+      S->pinned[s] = nullptr;
+      S->pinned_bytes[s] = 0;
+      if (hipHostMalloc((void **)&S->pinned[s], fb, hipHostMallocDefault) == hipSuccess)
+        S->pinned_bytes[s] = fb;
+    }
+    const unsigned char *src = req.frame.data();
+    if (S->pinned[s]) {
+      std::memcpy(S->pinned[s], req.frame.data(), fb);
+      src = S->pinned[s];
+    }
+    res.rc = geye_detect(S->ctx[s], req.model, src, req.W, req.H, 3 * req.W, S->cfg.thresh,
+                         S->cfg.overlap, S->cfg.precision, &res.rows, &res.num, &res.GL);
+    }
+    if (res.rc != GEYE_OK) res.err = geye_last_error();
+    res.t_done = now_ms();
+
+    lk.lock();
+    for (auto &r : S->resident)
+      if (r.slot == s && r.model == &req.model->m && r.key == key) r.last_use = res.t_done;
+    S->busy[s] = false;
+    S->completed++;
+    S->latencies.push_back(res.t_done - res.t_submit);
+    S->t_last = std::max(S->t_last, res.t_done);
+    S->done[req.ticket] = std::move(res);
+    S->cv_done.notify_all();
+    S->cv_work.notify_all();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int geye_serve_create(int device, const geye_serve_config *cfg, geye_server **out) {
+  if (!cfg || !out) return GEYE_ERR_ARG;
+  if (cfg->slots < 1 || cfg->slots > 16 || cfg->queue_cap < 1 ||
+      (cfg->precision != GEYE_FP32 && cfg->precision != GEYE_BF16 && cfg->precision != GEYE_FP16))
+    return GEYE_ERR_ARG;
+  geye_server *S = new geye_server();
+  S->cfg = *cfg;
+  S->device = device;
+  for (int s = 0; s < cfg->slots; s++) {
+    geye_ctx *c = nullptr;
+    const int rc = geye_init(device, &c);
+    if (rc != GEYE_OK) {
+      for (geye_ctx *x : S->ctx) geye_shutdown(x);
+      delete S;
+      return rc;
+    }
+    S->ctx.push_back(c);
+  }
+  S->pinned.assign(cfg->slots, nullptr);
+  S->pinned_bytes.assign(cfg->slots, 0);
+  S->busy.assign(cfg->slots, false);
+  for (int s = 0; s < cfg->slots; s++) S->workers.emplace_back(worker, S, s);
+  *out = S;
+  return GEYE_OK;
+}
+
+int geye_serve_submit(geye_server *S, const geye_model *model, const unsigned char *bgr,
+                      int width, int height, int widthStep, long long *ticket) {
+  if (!S || !model || !bgr || !ticket || width < 8 || height < 8 || widthStep < 3 * width)
+    return GEYE_ERR_ARG;
+  Request r;
+  r.model = model;
+  r.W = width;
+  r.H = height;
+  r.frame.resize((size_t)3 * width * height);
+  for (int y = 0; y < height; y++)
+    std::memcpy(&r.frame[(size_t)y * 3 * width], bgr + (size_t)y * widthStep, (size_t)3 * width);
+  std::lock_guard<std::mutex> lk(S->mu);
+  if ((int)S->queue.size() >= S->cfg.queue_cap) return GEYE_ERR_BUSY;
+  r.t_submit = now_ms();
+  if (S->t_first < 0) S->t_first = r.t_submit;
+  r.ticket = S->next_ticket++;
+  *ticket = r.ticket;
+  S->queue.push_back(std::move(r));
+  S->cv_work.notify_all();
+  return GEYE_OK;
+}
+
+int geye_serve_submit_dev(geye_server *S, const geye_model *model, const void *d_bgr, int width,
+                          int height, int widthStep, long long *ticket) {
+  if (!S || !model || !d_bgr || !ticket || width < 8 || height < 8 || widthStep < 3 * width)
+    return GEYE_ERR_ARG;
+  Request r;
+  r.model = model;
+  r.dev = d_bgr;
+  r.W = width;
+  r.H = height;
+  r.ws = widthStep;
+  std::lock_guard<std::mutex> lk(S->mu);
+  if ((int)S->queue.size() >= S->cfg.queue_cap) return GEYE_ERR_BUSY;
+  r.t_submit = now_ms();
+  if (S->t_first < 0) S->t_first = r.t_submit;
+  r.ticket = S->next_ticket++;
+  *ticket = r.ticket;
+  S->queue.push_back(std::move(r));
+  S->cv_work.notify_all();
+  return GEYE_OK;
+}
+
+int geye_serve_wait(geye_server *S, long long ticket, float **boxes, int *num, int *GL,
+                    double *latency_ms) {
+  if (!S || !boxes || !num || !GL) return GEYE_ERR_ARG;
+  std::unique_lock<std::mutex> lk(S->mu);
+  if (ticket <= 0 || ticket >= S->next_ticket) return GEYE_ERR_ARG;
+  S->cv_done.wait(lk, [&] { return S->done.count(ticket) > 0; });
+  Result r = std::move(S->done[ticket]);
+  S->done.erase(ticket);
+  lk.unlock();
+  if (latency_ms) *latency_ms = r.t_done - r.t_submit;
+  if (r.rc != GEYE_OK) return set_error(r.rc, r.err);
+  *boxes = r.rows;
+  *num = r.num;
+  *GL = r.GL;
+  return GEYE_OK;
+}
+
+int geye_serve_stats(geye_server *S, long long *completed, double *p50_ms, double *p99_ms,
+                     double *fps, long long *peak_bytes, long long *evictions) {
+  if (!S) return GEYE_ERR_ARG;
+  std::lock_guard<std::mutex> lk(S->mu);
+  std::vector<double> l = S->latencies;
+  std::sort(l.begin(), l.end());
+  auto pct = [&](double q) {
+    if (l.empty()) return 0.0;
+    const size_t i = (size_t)std::min<double>(l.size() - 1, q * (l.size() - 1) + 0.5);
+    return l[i];
+  };
+  if (completed) *completed = S->completed;
+  if (p50_ms) *p50_ms = pct(0.50);
+  if (p99_ms) *p99_ms = pct(0.99);
+  if (fps) *fps = (S->completed > 0 && S->t_last > S->t_first)
+                      ? 1000.0 * S->completed / (S->t_last - S->t_first) : 0.0;
+  if (peak_bytes) *peak_bytes = S->peak;
+  if (evictions) *evictions = S->evictions;
+  return GEYE_OK;
+}
+
+int geye_plan_bytes(const geye_model *model, int width, int height, long long *bytes) {
+  if (!model || !bytes || width < 8 || height < 8) return GEYE_ERR_ARG;
+  return plan_bytes(model->m, width, height, bytes);
+}
+
+int geye_serve_reset_stats(geye_server *S) {
+  if (!S) return GEYE_ERR_ARG;
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->latencies.clear();
+  S->completed = 0;
+  S->t_first = -1;
+  S->t_last = 0;
+  S->peak = S->held;
+  S->evictions = 0;
+  return GEYE_OK;
+}
+
+void geye_serve_destroy(geye_server *S) {
+  if (!S) return;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+    S->cv_work.notify_all();
+  }
+  for (auto &t : S->workers) t.join();
+  for (auto &kv : S->done) std::free(kv.second.rows);
+  for (size_t s = 0; s < S->ctx.size(); s++) {
+    geye_shutdown(S->ctx[s]);
+    if (S->pinned[s]) (void)hipHostFree(S->pinned[s]);
+  }
+  delete S;
+}
+
+}  // extern "C"

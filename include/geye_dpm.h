@@ -169,6 +169,44 @@ int geye_add_part(geye_ctx *ctx, float *score, const float *M, const int *rootsi
 int geye_nms(geye_ctx *ctx, const float *boxes, int num, int GL, float overlap, float *out,
              int *out_num);
 
+/* ---- serving: several camera streams on one device ---------------------------------------
+ * The reference detects one frame at a time (CT/main.cpp:240-300).  A server runs `slots`
+ * workers, each with its own stream and per-frame working memory, so several frames are on
+ * the device at once; models (geye_model, created on any context of the same device) stay
+ * resident and shared.  Requests are taken in arrival order as soon as a worker is free and
+ * the frame's working memory is resident on it or fits in hbm_budget (bytes; 0 = no limit),
+ * evicting idle workers' least recently used plans.  geye_serve_submit copies the frame and
+ * returns at once (GEYE_ERR_BUSY when queue_cap requests are waiting); geye_serve_wait blocks
+ * for one ticket and returns its NMS survivors like geye_detect (free with geye_free) and
+ * its latency from submission.  Results do not depend on the other frames in flight. */
+typedef struct geye_server geye_server;
+typedef struct {
+  int slots;               /* 1..16 frames on the device at once */
+  long long hbm_budget;    /* bytes of per-frame working memory held, 0 = unlimited */
+  int queue_cap;           /* requests waiting before submit returns GEYE_ERR_BUSY */
+  int precision;           /* GEYE_FP32 / GEYE_BF16 / GEYE_FP16 */
+  float thresh, overlap;
+} geye_serve_config;
+
+int geye_serve_create(int device, const geye_serve_config *cfg, geye_server **srv);
+int geye_serve_submit(geye_server *srv, const geye_model *model, const unsigned char *bgr,
+                      int width, int height, int widthStep, long long *ticket);
+/* Device-resident frame (geye_dev_alloc on the same device); it must stay valid and unchanged
+ * until its ticket has been waited for. */
+int geye_serve_submit_dev(geye_server *srv, const geye_model *model, const void *d_bgr,
+                          int width, int height, int widthStep, long long *ticket);
+int geye_serve_wait(geye_server *srv, long long ticket, float **boxes, int *num, int *GL,
+                    double *latency_ms);
+/* Completed requests, latency percentiles (ms, submission to result), throughput since the
+ * first submission, peak bytes of working memory held, and plan evictions. */
+int geye_serve_stats(geye_server *srv, long long *completed, double *p50_ms, double *p99_ms,
+                     double *fps, long long *peak_bytes, long long *evictions);
+int geye_serve_reset_stats(geye_server *srv);
+/* Device bytes of one frame's working memory for this model and frame size, as the server
+ * accounts it against hbm_budget (host-only; no device work). */
+int geye_plan_bytes(const geye_model *model, int width, int height, long long *bytes);
+void geye_serve_destroy(geye_server *srv);
+
 /* ---- device memory and timing (bench / serving plumbing) ------------------------------- */
 int geye_dev_alloc(geye_ctx *ctx, size_t bytes, void **ptr);
 int geye_dev_free(geye_ctx *ctx, void *ptr);
