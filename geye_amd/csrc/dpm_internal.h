@@ -28,8 +28,19 @@ struct ResizeOp {
   int blocks_begin;            // first flat block of this op in the launch
 };
 
+// Gradient grid of one pyramid image: per pixel (x, y), 1 <= x <= w-2, 1 <= y <= h-2, the
+// strongest channel's gradient magnitude and snapped orientation, column-major int2
+// {float bits of v, o} at grid_off + x*h + y.  Tiles of 64 (x) x 16 (y) pixels.
+struct GradImg {
+  long long img_off, grid_off;
+  int h, w;
+  int tiles_x, tiles_y;
+  int blocks_begin;
+};
+
 struct HogLevel {
   long long img_off;           // image arena offset ([3][h][w] row-major)
+  long long grid_off;          // gradient grid of the image (GradImg)
   long long hist_off;          // hist arena offset ([18][b1][b0])
   long long norm_off;          // norm arena offset ([b1][b0])
   long long feat_off;          // feature arena offset ([31][F1][F0])
@@ -128,6 +139,9 @@ struct Plan {
   std::vector<float> resize_alpha;
   // HOG
   std::vector<HogLevel> hog;          // [T16 levels][T8 levels][feature levels]
+  std::vector<GradImg> grad;          // one per pyramid image
+  int grad_blocks = 0;
+  long long grid_total = 0;           // int2 entries
   int hog16_n = 0, hog8_n = 0, hogf_n = 0;
   int hog_hist_blocks = 0, hog8_blocks = 0, hog_feat_blocks = 0;
   long long hist_total = 0, norm_total = 0, feat_total = 0, featb_total = 0;
@@ -164,7 +178,7 @@ struct Plan {
   // device copies of the tables
   DeviceBuffer d_resize_ops, d_resize_tab, d_resize_si, d_resize_alpha, d_hog, d_conv_tiles,
       d_mfma_tiles, d_map_dims, d_map_off, d_dt1, d_dt2, d_nodes, d_depths, d_smaps, d_sparts,
-      d_ascore_tab;
+      d_ascore_tab, d_grad, d_grid;
   // device arenas
   DeviceBuffer d_frame, d_img, d_tmp, d_hist, d_norm, d_feat, d_featb, d_conv, d_tmpM,
       d_tmpIy, d_M, d_tmpIx, d_score, d_flags, d_blocksum, d_cand, d_count, d_rows, d_ascore;

@@ -12,8 +12,12 @@ void launch_resize_rows(hipStream_t s, const ResizeOp *ops, int nops, int blocks
 void launch_resize_cols(hipStream_t s, const ResizeOp *ops, int nops, int blocks,
                         const int2 *tab, const int *si, const float *alpha, const float *src,
                         float *dst);
-void launch_hog_hist(hipStream_t s, int T, const HogLevel *lv, int nlv, int blocks,
-                     const float *img, float *hist, float *norm);
+// HOG histogram in two launches: gradients per pyramid image, then votes per level
+// (HogLevel.T must be 16: 16 x 16 cell tiles)
+void launch_hog_grad(hipStream_t s, const GradImg *imgs, int nimg, int blocks, const float *img,
+                     int2 *grid);
+void launch_hog_votes(hipStream_t s, const HogLevel *lv, int nlv, int blocks, const int2 *grid,
+                      float *hist, float *norm);
 void launch_hog_feat(hipStream_t s, const HogLevel *lv, int nlv, int blocks, const float *hist,
                      const float *norm, float *feat, void *featb, int hkind);
 void launch_conv_f32(hipStream_t s, int nf, const ConvTile *tiles, int ntiles, int fh, int fw,

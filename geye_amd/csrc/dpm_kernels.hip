@@ -118,126 +118,142 @@ __device__ inline void interp_coord(int x, int sbin, int *ix, float *v0, float *
   *v1 = (float)(1.0 - (double)a);
 }
 
-template <int T>
-__global__ __launch_bounds__(T *T) void k_hog_hist(const HogLevel *__restrict__ lv, int nlv,
-                                                   const float *__restrict__ img_arena,
+// Gradient of the strongest channel at pixel (x, y) of a [3][h][w] tile staged in LDS
+// (CT/featurepyramid.cpp.new2:325-364): central differences per channel, the first channel
+// with the largest squared magnitude (strict >), the orientation of the largest |dot| with
+// the 4-decimal direction table (first maximum; the sign picks o or o + 9), v = sqrt.
+__device__ inline int2 hog_grad_at(const float *t0, const float *t1, const float *t2, int i,
+                                   int row) {
+  float dy = t0[i + row] - t0[i - row];
+  float dx = t0[i + 1] - t0[i - 1];
+  float v = dx * dx + dy * dy;
+  const float dy2 = t1[i + row] - t1[i - row];
+  const float dx2 = t1[i + 1] - t1[i - 1];
+  const float v2 = dx2 * dx2 + dy2 * dy2;
+  const float dy3 = t2[i + row] - t2[i - row];
+  const float dx3 = t2[i + 1] - t2[i - 1];
+  const float v3 = dx3 * dx3 + dy3 * dy3;
+  if (v2 > v) { v = v2; dx = dx2; dy = dy2; }
+  if (v3 > v) { v = v3; dx = dx3; dy = dy3; }
+  float best_dot = 0.0f;
+  int best_o = 0;
+#pragma unroll
+  for (int o = 0; o < 9; o++) {
+    const float dot = c_hcos[o] * dx + c_hsin[o] * dy;
+    if (dot > best_dot) {
+      best_dot = dot;
+      best_o = o;
+    } else if (-dot > best_dot) {
+      best_dot = -dot;
+      best_o = o + 9;
+    }
+  }
+  return make_int2(__float_as_int(sqrtf(v)), best_o);
+}
+
+// One block = 64 (x) x 16 (y) pixels of one image: the 66 x 18 x 3 neighbourhood is read
+// row-major (coalesced) into LDS, and the results go out column-major, 16 consecutive y per
+// column.  Pixels outside [1, w-2] x [1, h-2] are not written: readers clamp to that range,
+// exactly as the reference clamps the pixel it differentiates.
+__global__ __launch_bounds__(256) void k_hog_grad(const GradImg *__restrict__ imgs, int nimg,
+                                                  const float *__restrict__ img_arena,
+                                                  int2 *__restrict__ grid_arena) {
+  constexpr int TXP = 64, TYP = 16, SX = TXP + 2, SY = TYP + 2;
+  __shared__ float s_t[3][SY * SX];
+  int ii = 0;
+  while (ii + 1 < nimg && imgs[ii + 1].blocks_begin <= (int)blockIdx.x) ii++;
+  const GradImg G = imgs[ii];
+  const int tile = blockIdx.x - G.blocks_begin;
+  const int tx = tile / G.tiles_y, ty = tile - tx * G.tiles_y;
+  const int x0 = tx * TXP, y0 = ty * TYP;
+  const int h = G.h, w = G.w;
+  const float *img = img_arena + G.img_off;
+  for (int i = threadIdx.x; i < SY * SX; i += 256) {
+    const int r = i / SX, q = i - r * SX;
+    const int yy = min(max(y0 - 1 + r, 0), h - 1), xx = min(max(x0 - 1 + q, 0), w - 1);
+    const long long o = (long long)yy * w + xx;
+    s_t[0][i] = img[o];
+    s_t[1][i] = img[o + (long long)h * w];
+    s_t[2][i] = img[o + 2LL * h * w];
+  }
+  __syncthreads();
+  const int ly = threadIdx.x & 15;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int lx = (threadIdx.x >> 4) + 16 * k;
+    const int x = x0 + lx, y = y0 + ly;
+    if (x < 1 || y < 1 || x > w - 2 || y > h - 2) continue;
+    const int i = (ly + 1) * SX + (lx + 1);
+    grid_arena[G.grid_off + (long long)x * h + y] = hog_grad_at(s_t[0], s_t[1], s_t[2], i, SX);
+  }
+}
+
+// Orientation histogram + block energy, one thread per cell of a 16 x 16 cell tile.  The
+// thread gathers its cell's bilinear votes from the gradient grid in the reference's pixel
+// order, x outer and y inner (CT/featurepyramid.cpp.new2:323-378), so every bin sees the same
+// sequence of float additions as the reference's scatter loop.  Then hist[o][x][y] and
+// norm[x][y] = sum_{o<9} (h_o + h_{o+9})^2 (CT/featurepyramid.cpp.new2:391-404).
+constexpr int kHogT = 16;
+__global__ __launch_bounds__(256) void k_hog_votes(const HogLevel *__restrict__ lv, int nlv,
+                                                   const int2 *__restrict__ grid_arena,
                                                    float *__restrict__ hist_arena,
                                                    float *__restrict__ norm_arena) {
-  // find the level of this block
+  constexpr int T = kHogT, JM = 16;  // 2 * sbin <= 16 rows and columns per cell
+  __shared__ float s_acc[18 * T * T];
   int li = 0;
   while (li + 1 < nlv && lv[li + 1].blocks_begin <= (int)blockIdx.x) li++;
   const HogLevel L = lv[li];
   const int tile = blockIdx.x - L.blocks_begin;
   const int tcx = tile / L.tiles_y, tcy = tile - tcx * L.tiles_y;
-  const int cx0 = tcx * T, cy0 = tcy * T;
-  const int sbin = L.sbin;
-  const int vis0 = L.b0 * sbin, vis1 = L.b1 * sbin;
-  // pixel support of the tile, clipped to the reference's loop range [1, vis-2]
-  const int xa = max(1, first_pix(cx0 - 1, sbin));
-  const int xb = min(vis1 - 2, first_pix(cx0 + T, sbin) - 1);
-  const int ya = max(1, first_pix(cy0 - 1, sbin));
-  const int yb = min(vis0 - 2, first_pix(cy0 + T, sbin) - 1);
-  constexpr int RMAX = (T + 1) * (T == 16 ? 4 : 8);
-  // Pixel (rx, ry) of the tile lives at ry * RMAX + rx + ry / sbin.  Lanes of one wave read
-  // pixels sbin apart in both axes; the per-cell-row skew turns the row step into
-  // sbin * RMAX + 1 words, which spreads the lanes over the LDS banks.
-  constexpr int RS = RMAX;
-  constexpr int PIX = RMAX * RS + RMAX;
-  __shared__ float s_v[PIX];
-  __shared__ unsigned char s_o[PIX];
-  __shared__ float s_acc[18 * T * T];
-  __shared__ int s_ix[RMAX], s_iy[RMAX];
-  __shared__ float s_vx0[RMAX], s_vx1[RMAX], s_vy0[RMAX], s_vy1[RMAX];
   const int tid = threadIdx.x;
-  const int nx = xb - xa + 1, ny = yb - ya + 1;
   for (int i = tid; i < 18 * T * T; i += T * T) s_acc[i] = 0.0f;
-  if (nx > 0 && ny > 0) {
-    for (int i = tid; i < nx; i += T * T) interp_coord(xa + i, sbin, &s_ix[i], &s_vx0[i], &s_vx1[i]);
-    for (int i = tid; i < ny; i += T * T) interp_coord(ya + i, sbin, &s_iy[i], &s_vy0[i], &s_vy1[i]);
-    const int h = L.h, w = L.w;
-    const long long SQ = (long long)h * w;
-    const float *img = img_arena + L.img_off;
-    for (int i = tid; i < nx * ny; i += T * T) {
-      int ry = i / nx, rx = i - ry * nx;
-      int x = min(xa + rx, w - 2), y = min(ya + ry, h - 2);
-      const float *s = img + (long long)y * w + x;
-      float dy = s[w] - s[-w];
-      float dx = s[1] - s[-1];
-      float v = dx * dx + dy * dy;
-      s += SQ;
-      float dy2 = s[w] - s[-w];
-      float dx2 = s[1] - s[-1];
-      float v2 = dx2 * dx2 + dy2 * dy2;
-      s += SQ;
-      float dy3 = s[w] - s[-w];
-      float dx3 = s[1] - s[-1];
-      float v3 = dx3 * dx3 + dy3 * dy3;
-      if (v2 > v) { v = v2; dx = dx2; dy = dy2; }
-      if (v3 > v) { v = v3; dx = dx3; dy = dy3; }
-      float best_dot = 0.0f;
-      int best_o = 0;
-#pragma unroll
-      for (int o = 0; o < 9; o++) {
-        float dot = c_hcos[o] * dx + c_hsin[o] * dy;
-        if (dot > best_dot) {
-          best_dot = dot;
-          best_o = o;
-        } else if (-dot > best_dot) {
-          best_dot = -dot;
-          best_o = o + 9;
-        }
-      }
-      const int q = ry * RS + rx + ry / sbin;
-      s_v[q] = sqrtf(v);
-      s_o[q] = (unsigned char)best_o;
-    }
-  }
   __syncthreads();
-  const int ci = tid / T, cj = tid - ci * T;  // cell (cx0 + ci, cy0 + cj)
-  const int cx = cx0 + ci, cy = cy0 + cj;
-  if (cx < L.b1 && cy < L.b0) {
-    // pixels voting into this cell: ixp in {cx-1, cx}
-    const int px0 = max(xa, first_pix(cx - 1, sbin)) - xa;
-    const int px1 = min(xb, first_pix(cx + 1, sbin) - 1) - xa;
-    const int py0 = max(ya, first_pix(cy - 1, sbin)) - ya;
-    const int py1 = min(yb, first_pix(cy + 1, sbin) - 1) - ya;
-    float *acc = s_acc + tid;
-    // the cell's pixel rows (at most 2 * sbin): weights and LDS row offsets in registers
-    constexpr int JM = 2 * (T == 16 ? 4 : 8);
-    const int npy = py1 - py0 + 1;
-    float wyr[JM];
-    int roff[JM];
+  const int sbin = L.sbin, h = L.h, w = L.w;
+  const int cx = tcx * T + tid / T, cy = tcy * T + (tid & (T - 1));
+  if (cx >= L.b1 || cy >= L.b0) return;
+  const int vis0 = L.b0 * sbin, vis1 = L.b1 * sbin;
+  // pixels voting into this cell: interpolation cell in {c - 1, c}, within [1, vis - 2]
+  const int x0 = max(1, first_pix(cx - 1, sbin)), x1 = min(vis1 - 2, first_pix(cx + 1, sbin) - 1);
+  const int y0 = max(1, first_pix(cy - 1, sbin)), y1 = min(vis0 - 2, first_pix(cy + 1, sbin) - 1);
+  const int ny = y1 - y0 + 1;
+  float wy[JM];
+  int yc[JM];
 #pragma unroll
-    for (int j = 0; j < JM; j++) {
-      const int ry = max(0, min(py0 + j, py1));
-      wyr[j] = (s_iy[ry] == cy) ? s_vy1[ry] : s_vy0[ry];
-      roff[j] = ry * RS + ry / sbin;
-    }
-    // votes in the reference's pixel order: x outer, y inner
-    for (int rx = px0; rx <= px1; rx++) {
-      const float wx = (s_ix[rx] == cx) ? s_vx1[rx] : s_vx0[rx];
-#pragma unroll
-      for (int j = 0; j < JM; j++) {
-        if (j < npy) {
-          const int q = roff[j] + rx;
-          const float v = s_v[q];
-          const int o = s_o[q];
-          acc[o * T * T] += wx * wyr[j] * v;
-        }
-      }
-    }
-    const long long BSQ = (long long)L.b0 * L.b1;
-    float *hist = hist_arena + L.hist_off + (long long)cx * L.b0 + cy;
-    float nrm = 0.0f;
-#pragma unroll
-    for (int o = 0; o < 18; o++) hist[o * BSQ] = acc[o * T * T];
-#pragma unroll
-    for (int o = 0; o < 9; o++) {
-      float t = acc[o * T * T] + acc[(o + 9) * T * T];
-      nrm += t * t;
-    }
-    norm_arena[L.norm_off + (long long)cx * L.b0 + cy] = nrm;
+  for (int j = 0; j < JM; j++) {
+    const int y = min(y0 + j, y1);
+    int iy;
+    float v0, v1;
+    interp_coord(y, sbin, &iy, &v0, &v1);
+    wy[j] = (iy == cy) ? v1 : v0;
+    yc[j] = min(y, h - 2);
   }
+  float *acc = s_acc + tid;
+  const int2 *grid = grid_arena + L.grid_off;
+  for (int x = x0; x <= x1; x++) {
+    int ix;
+    float v0, v1;
+    interp_coord(x, sbin, &ix, &v0, &v1);
+    const float wx = (ix == cx) ? v1 : v0;
+    const int2 *col = grid + (long long)min(x, w - 2) * h;
+    int2 g[JM];
+#pragma unroll
+    for (int j = 0; j < JM; j++)
+      if (j < ny) g[j] = col[yc[j]];
+#pragma unroll
+    for (int j = 0; j < JM; j++)
+      if (j < ny) acc[g[j].y * T * T] += wx * wy[j] * __int_as_float(g[j].x);
+  }
+  const long long BSQ = (long long)L.b0 * L.b1;
+  float *hist = hist_arena + L.hist_off + (long long)cx * L.b0 + cy;
+  float nrm = 0.0f;
+#pragma unroll
+  for (int o = 0; o < 18; o++) hist[o * BSQ] = acc[o * T * T];
+#pragma unroll
+  for (int o = 0; o < 9; o++) {
+    const float t = acc[o * T * T] + acc[(o + 9) * T * T];
+    nrm += t * t;
+  }
+  norm_arena[L.norm_off + (long long)cx * L.b0 + cy] = nrm;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -766,12 +782,17 @@ void launch_resize_cols(hipStream_t s, const ResizeOp *ops, int nops, int blocks
                      src, dst);
 }
 
-void launch_hog_hist(hipStream_t s, int T, const HogLevel *lv, int nlv, int blocks,
-                     const float *img, float *hist, float *norm) {
-  if (T == 16)
-    hipLaunchKernelGGL(k_hog_hist<16>, dim3(blocks), dim3(256), 0, s, lv, nlv, img, hist, norm);
-  else
-    hipLaunchKernelGGL(k_hog_hist<8>, dim3(blocks), dim3(64), 0, s, lv, nlv, img, hist, norm);
+void launch_hog_grad(hipStream_t s, const GradImg *imgs, int nimg, int blocks, const float *img,
+                     int2 *grid) {
+  if (blocks > 0)
+    hipLaunchKernelGGL(k_hog_grad, dim3(blocks), dim3(256), 0, s, imgs, nimg, img, grid);
+}
+
+void launch_hog_votes(hipStream_t s, const HogLevel *lv, int nlv, int blocks, const int2 *grid,
+                      float *hist, float *norm) {
+  if (blocks > 0)
+    hipLaunchKernelGGL(k_hog_votes, dim3(blocks), dim3(kHogT * kHogT), 0, s, lv, nlv, grid, hist,
+                       norm);
 }
 
 void launch_hog_feat(hipStream_t s, const HogLevel *lv, int nlv, int blocks, const float *hist,

@@ -258,7 +258,30 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
     }
     R.tmp_total = toff;
   }
-  // ---- HOG levels (two hist launches: T = 16 for sbin <= 4, T = 8 otherwise)
+  // ---- gradient grids (one per pyramid image, shared by its sbin 4 and sbin 8 levels)
+  P.grad.clear();
+  std::vector<long long> grid_of(P.img_h.size(), 0);
+  {
+    long long go = 0;
+    int gb = 0;
+    for (size_t i = 0; i < P.img_h.size(); i++) {
+      GradImg g{};
+      g.img_off = P.img_off[i];
+      g.grid_off = go;
+      g.h = P.img_h[i];
+      g.w = P.img_w[i];
+      g.tiles_x = (g.w + 63) / 64;
+      g.tiles_y = (g.h + 15) / 16;
+      g.blocks_begin = gb;
+      gb += g.tiles_x * g.tiles_y;
+      grid_of[i] = go;
+      go += (long long)g.h * g.w;
+      P.grad.push_back(g);
+    }
+    P.grad_blocks = gb;
+    P.grid_total = go;
+  }
+  // ---- HOG levels (one vote launch over 16 x 16 cell tiles, then one feature launch)
   P.hog.clear();
   P.feat_off.assign(LEN, -1);
   long long ho = 0, no = 0, fo = 0, fbo = 0;
@@ -269,6 +292,7 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
     if (L.F0 <= 0 || L.F1 <= 0) continue;
     HogLevel h{};
     h.img_off = P.img_off[L.img_index];
+    h.grid_off = grid_of[L.img_index];
     h.h = L.img_h;
     h.w = L.img_w;
     h.sbin = L.sbin;
@@ -285,7 +309,7 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
     no += (long long)h.b0 * h.b1;
     fo += 31LL * L.F0 * L.F1;
     fbo += 32LL * L.F0 * L.F1;
-    h.T = (L.sbin <= 4) ? 16 : 8;
+    h.T = 16;
     h.tiles_x = (h.b1 + h.T - 1) / h.T;
     h.tiles_y = (h.b0 + h.T - 1) / h.T;
     HogLevel hh = h;

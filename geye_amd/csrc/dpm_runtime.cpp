@@ -456,6 +456,8 @@ int plan_upload(Plan &P, hipStream_t st) {
   CHK(upload(P.d_smaps, P.smaps, st));
   CHK(upload(P.d_sparts, P.sparts, st));
   CHK(upload(P.d_ascore_tab, P.ascore_tab, st));
+  CHK(upload(P.d_grad, P.grad, st));
+  CHK(ensure(P.d_grid, sizeof(int2) * std::max<long long>(P.grid_total, 1)));
   long long tmp = 0;
   for (auto &R : P.rounds) tmp = std::max(tmp, R.tmp_total);
   CHK(ensure(P.d_img, sizeof(float) * P.img_total));
@@ -488,7 +490,7 @@ void plan_release(Plan &P) {
                           &P.n_status, &P.n_fsup, &P.n_picks, &P.n_counters, &P.n_mask,
                           &P.n_keep, &P.n_sel, &P.n_sort, &P.n_flags, &P.n_keep_orig,
                           &P.n_kept, &P.n_kept_flat, &P.n_total, &P.n_vals, &P.n_ord,
-                          &P.n_rank, &P.d_ascore_tab, &P.d_ascore})
+                          &P.n_rank, &P.d_ascore_tab, &P.d_ascore, &P.d_grad, &P.d_grid})
     release(*b);
 }
 
@@ -550,7 +552,7 @@ int plan_bytes(const Model &m, int W, int H, long long *bytes) {
   const long long N = std::max<long long>(P.score_total, 1);
   long long b = 4 * (P.img_total + tmp + P.hist_total + P.norm_total + P.feat_total + P.conv_total +
                      4 * P.dt_total + 3 * P.score_total) +
-                2 * P.featb_total + (long long)P.W * P.H * (3 + 4) +
+                2 * P.featb_total + 8 * P.grid_total + (long long)P.W * P.H * (3 + 4) +
                 N * (2 * (long long)sizeof(NmsBox) + 16 * 4) + (long long)P.GL * 4 * N;
   *bytes = b;
   return GEYE_OK;
@@ -611,15 +613,15 @@ int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, int hk
     op_base += nr + nc;
   }
   const HogLevel *hog = dp<HogLevel>(P.d_hog);
-  if (P.hog16_n > 0) {
-    KTimer t(c, "hog_hist");
-    launch_hog_hist(s, 16, hog, P.hog16_n, P.hog_hist_blocks, dp<float>(P.d_img),
-                    dp<float>(P.d_hist), dp<float>(P.d_norm));
+  {
+    KTimer t(c, "hog_grad");
+    launch_hog_grad(s, dp<GradImg>(P.d_grad), (int)P.grad.size(), P.grad_blocks,
+                    dp<float>(P.d_img), dp<int2>(P.d_grid));
   }
-  if (P.hog8_n > 0) {
+  if (P.hog16_n + P.hog8_n > 0) {
     KTimer t(c, "hog_hist");
-    launch_hog_hist(s, 8, hog + P.hog16_n, P.hog8_n, P.hog8_blocks, dp<float>(P.d_img),
-                    dp<float>(P.d_hist), dp<float>(P.d_norm));
+    launch_hog_votes(s, hog, P.hog16_n + P.hog8_n, P.hog_hist_blocks + P.hog8_blocks,
+                     dp<int2>(P.d_grid), dp<float>(P.d_hist), dp<float>(P.d_norm));
   }
   if (hkind) CHK(ensure(P.d_featb, 2 * P.featb_total));
   if (P.hogf_n > 0) {
@@ -1475,19 +1477,30 @@ int geye_calc_feature(geye_ctx *c, const float *img, int height, int width, int 
   HogLevel L{};
   L.img_off = 0;
   L.h = h; L.w = w; L.sbin = sbin; L.b0 = b0; L.b1 = b1; L.F0 = *F0; L.F1 = *F1;
-  L.T = sbin <= 4 ? 16 : 8;
+  L.T = 16;
   L.tiles_x = (b1 + L.T - 1) / L.T;
   L.tiles_y = (b0 + L.T - 1) / L.T;
   L.blocks_begin = 0;
+  L.grid_off = 0;
+  GradImg G{};
+  G.h = h;
+  G.w = w;
+  G.tiles_x = (w + 63) / 64;
+  G.tiles_y = (h + 15) / 16;
   Scratch S;
-  DeviceBuffer &dl = S.get(), &dimg = S.get(), &dh = S.get(), &dn = S.get(), &df = S.get();
+  DeviceBuffer &dl = S.get(), &dimg = S.get(), &dh = S.get(), &dn = S.get(), &df = S.get(),
+               &dg = S.get(), &dgrid = S.get();
   CHK(upload(dl, std::vector<HogLevel>{L}, c->stream));
+  CHK(upload(dg, std::vector<GradImg>{G}, c->stream));
   CHK(upload(dimg, in, c->stream));
+  CHK(ensure(dgrid, sizeof(int2) * (size_t)h * w));
   CHK(ensure(dh, sizeof(float) * 18LL * b0 * b1));
   CHK(ensure(dn, sizeof(float) * (size_t)b0 * b1));
   CHK(ensure(df, sizeof(float) * 31LL * *F0 * *F1));
-  launch_hog_hist(c->stream, L.T, dp<HogLevel>(dl), 1, L.tiles_x * L.tiles_y, dp<float>(dimg),
-                  dp<float>(dh), dp<float>(dn));
+  launch_hog_grad(c->stream, dp<GradImg>(dg), 1, G.tiles_x * G.tiles_y, dp<float>(dimg),
+                  dp<int2>(dgrid));
+  launch_hog_votes(c->stream, dp<HogLevel>(dl), 1, L.tiles_x * L.tiles_y, dp<int2>(dgrid),
+                   dp<float>(dh), dp<float>(dn));
   launch_hog_feat(c->stream, dp<HogLevel>(dl), 1, (int)((1LL * *F0 * *F1 + 255) / 256),
                   dp<float>(dh), dp<float>(dn), dp<float>(df), nullptr, 0);
   HIPCHK(hipGetLastError());
