@@ -172,7 +172,7 @@ def serving_mix(geye_amd, ctx, synth, slots, seconds: float = 2.0):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "fp16"])
     ap.add_argument("--slots", type=int, default=0, help="serving slots (0 = sweep 1..4)")

@@ -85,6 +85,7 @@ _SIGS = [
                                           _fp, C.c_longlong, C.c_float, C.POINTER(_fp), _ip,
                                           _ip]),
     ("geye_add_part", C.c_int, [C.c_void_p, _fp, _fp, _ip, _ip, C.c_int, C.c_int]),
+    ("geye_graph_enable", C.c_int, [C.c_void_p, C.c_int]),
     ("geye_serve_create", C.c_int, [C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
     ("geye_serve_submit", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                     _llp]),
@@ -436,6 +437,10 @@ class Context:
         _check(lib().geye_synchronize(self.h))
 
     # ---- device memory
+    def graphs(self, on: bool) -> None:
+        """Replay the per-frame front as a HIP graph (default) or launch kernels one by one."""
+        _check(lib().geye_graph_enable(self.h, int(bool(on))))
+
     def dev_upload(self, arr: np.ndarray) -> C.c_void_p:
         arr = np.ascontiguousarray(arr)
         p = C.c_void_p()

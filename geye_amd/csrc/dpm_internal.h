@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -188,6 +189,7 @@ struct Plan {
       n_counters, n_mask, n_keep, n_sel, n_sort, n_flags, n_keep_orig, n_kept, n_kept_flat,
       n_total, n_vals, n_ord, n_rank;
   long long nms_fallbacks = 0;
+  std::map<long long, hipGraphExec_t> graphs;  // front of the frame, per (precision, ws, thresh)
 };
 
 struct Model {
@@ -237,6 +239,7 @@ struct geye_ctx {
   std::vector<float> last_scores;
   std::vector<geye::DeviceBuffer> scratch_bufs;
   geye::Plan *last_plan = nullptr;
+  bool graphs = true;                 // replay the per-frame front as a HIP graph
   long long nms_fallbacks = 0;
 };
 

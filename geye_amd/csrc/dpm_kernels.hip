@@ -826,6 +826,10 @@ void launch_conv_f32(hipStream_t s, int nf, const ConvTile *tiles, int ntiles, i
 
 void launch_dt(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, const int4 *nodes,
                const int *depths, const float *src, float *dst, int *ptr) {
+  static const bool attr = hipFuncSetAttribute((const void *)k_dt,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               160 * 1024) == hipSuccess;
+  if (!attr && lds > 64 * 1024) return;  // the launch would fail; the caller checks the error
   hipLaunchKernelGGL(k_dt, dim3(njobs), dim3(256), lds, s, jobs, nodes, depths, src, dst, ptr);
 }
 

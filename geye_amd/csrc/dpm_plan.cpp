@@ -155,10 +155,12 @@ static size_t dt_lds_bytes(int n, int G) {
   return (words + (words & 1)) * 4 + (size_t)n * 8;
 }
 
+// lanes per line: the fewest whose block fits 48 KB of LDS (several blocks per CU), else a
+// whole wave per line with up to 160 KB
 static int dt_pick_G(int n) {
   for (int G = 1; G <= 64; G *= 2)
     if (dt_lds_bytes(n, G) <= 48 * 1024) return G;
-  return -1;
+  return dt_lds_bytes(n, 64) <= 160 * 1024 ? 64 : -1;
 }
 
 static void add_dt_jobs(Plan &P, std::vector<DtJob> &jobs, long long src_off, long long dst_off,

@@ -479,6 +479,8 @@ int plan_upload(Plan &P, hipStream_t st) {
 }
 
 void plan_release(Plan &P) {
+  for (auto &kv : P.graphs) (void)hipGraphExecDestroy(kv.second);
+  P.graphs.clear();
   for (DeviceBuffer *b : {&P.d_resize_ops, &P.d_resize_tab, &P.d_resize_si, &P.d_resize_alpha,
                           &P.d_hog, &P.d_conv_tiles, &P.d_mfma_tiles, &P.d_map_dims,
                           &P.d_map_off, &P.d_dt1, &P.d_dt2,
@@ -634,6 +636,10 @@ int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, int hk
   return GEYE_OK;
 }
 
+int read_count(geye_ctx *c, Plan &P, int *ncand);
+
+// Launches the convolution, DT, scoring and candidate compaction; with ncand non-null also
+// waits for the candidate count.
 int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int *ncand) {
   hipStream_t s = c->stream;
   if (precision == GEYE_FP32) {
@@ -669,7 +675,6 @@ int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int 
     launch_dt(s, dp<DtJob>(P.d_dt2), (int)P.dt_pass2.size(), P.dt_lds2, dp<int4>(P.d_nodes),
               dp<int>(P.d_depths), dp<float>(P.d_tmpM), dp<float>(P.d_M), dp<int>(P.d_tmpIx));
   }
-  *ncand = 0;
   if (P.score_total > 0) {
     {
       KTimer t(c, "score");
@@ -682,13 +687,19 @@ int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int 
       launch_compact(s, dp<int>(P.d_flags), P.score_total, dp<int>(P.d_blocksum),
                      dp<int>(P.d_count), dp<int>(P.d_cand));
     }
-    HIPCHK(hipGetLastError());
-    int n = 0;
-    HIPCHK(hipMemcpyAsync(&n, P.d_count.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    *ncand = n;
   }
   HIPCHK(hipGetLastError());
+  return ncand ? read_count(c, P, ncand) : GEYE_OK;
+}
+
+// number of candidates the last scoring pass left in P.d_count (waits for it)
+int read_count(geye_ctx *c, Plan &P, int *ncand) {
+  *ncand = 0;
+  if (P.score_total <= 0) return GEYE_OK;
+  int n = 0;
+  HIPCHK(hipMemcpyAsync(&n, P.d_count.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *ncand = n;
   return GEYE_OK;
 }
 
@@ -854,10 +865,67 @@ int run_nms(geye_ctx *c, Model &m, Plan &P, int n, float overlap, int *nkept) {
   return GEYE_OK;
 }
 
+// The plan's frame buffer; graphs captured against an older buffer are dropped when it grows.
+int ensure_frame(Plan &P, size_t bytes) {
+  const void *old = P.d_frame.p;
+  CHK(ensure(P.d_frame, bytes));
+  if (P.d_frame.p != old) {
+    for (auto &kv : P.graphs) (void)hipGraphExecDestroy(kv.second);
+    P.graphs.clear();
+  }
+  return GEYE_OK;
+}
+
+// The per-frame work before the first host wait -- ingest, resize rounds, HOG, filter
+// convolution, DT, scoring and candidate compaction, about 30 launches -- runs as one HIP
+// graph per (plan, precision, threshold, widthStep): captured on the first frame after a
+// warm-up run (which makes every allocation and one-time attribute call), then replayed.
+// The graph reads the frame from the plan's own frame buffer, so a caller's device frame is
+// copied there first.  With profiling on, or graphs off, the launches run one by one.
+int run_front(geye_ctx *c, Model &m, Plan &P, const unsigned char *d_bgr, int ws, float thresh,
+              int precision, int *ncand) {
+  const int hkind = precision == GEYE_FP32 ? 0 : precision;
+  if (c->profile || !c->graphs) {
+    CHK(run_pyramid(c, P, d_bgr, ws, hkind));
+    return run_scores(c, m, P, thresh, precision, ncand);
+  }
+  unsigned int tb;
+  std::memcpy(&tb, &thresh, 4);
+  const long long key = ((long long)precision << 48) ^ ((long long)ws << 32) ^ (long long)tb;
+  CHK(ensure_frame(P, (size_t)ws * P.H));
+  if (d_bgr != P.d_frame.p)
+    HIPCHK(hipMemcpyAsync(P.d_frame.p, d_bgr, (size_t)ws * P.H, hipMemcpyDeviceToDevice, c->stream));
+  const unsigned char *fr = (const unsigned char *)P.d_frame.p;
+  auto it = P.graphs.find(key);
+  if (it == P.graphs.end()) {
+    // warm-up: allocations and attributes happen outside the capture
+    CHK(run_pyramid(c, P, fr, ws, hkind));
+    CHK(run_scores(c, m, P, thresh, precision, nullptr));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = run_pyramid(c, P, fr, ws, hkind);
+    if (rc == GEYE_OK) rc = run_scores(c, m, P, thresh, precision, nullptr);
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc != GEYE_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(GEYE_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return fail(GEYE_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+    it = P.graphs.emplace(key, ex).first;
+  }
+  HIPCHK(hipGraphLaunch(it->second, c->stream));
+  return read_count(c, P, ncand);
+}
+
 int frame_in(geye_ctx *c, Plan &P, const unsigned char *bgr, int W, int H, int ws,
              const unsigned char **d_bgr) {
   if (!bgr || ws < 3 * W) return fail(GEYE_ERR_ARG, "frame: null pointer or widthStep < 3*width");
-  CHK(ensure(P.d_frame, (size_t)ws * H));
+  CHK(ensure_frame(P, (size_t)ws * H));
   HIPCHK(hipMemcpyAsync(P.d_frame.p, bgr, (size_t)ws * H, hipMemcpyHostToDevice, c->stream));
   *d_bgr = (const unsigned char *)P.d_frame.p;
   return GEYE_OK;
@@ -881,9 +949,8 @@ int detect_common(geye_ctx *c, const geye_model *model, const unsigned char *bgr
   CHK(get_plan(c, m, W, H, &P, -1));
   const unsigned char *d_bgr = bgr;
   if (!dev) CHK(frame_in(c, *P, bgr, W, H, ws, &d_bgr));
-  CHK(run_pyramid(c, *P, d_bgr, ws, precision == GEYE_FP32 ? 0 : precision));
   int ncand = 0;
-  CHK(run_scores(c, m, *P, thresh, precision, &ncand));
+  CHK(run_front(c, m, *P, d_bgr, ws, thresh, precision, &ncand));
   std::vector<float> rows;
   c->last_plan = P;
   if (do_nms) {
@@ -1719,6 +1786,13 @@ int geye_synchronize(geye_ctx *c) {
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
   flush_stats(c);
+  return GEYE_OK;
+}
+
+int geye_graph_enable(geye_ctx *c, int on) {
+  if (!c) return fail(GEYE_ERR_ARG, "null ctx");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->graphs = on != 0;
   return GEYE_OK;
 }
 

@@ -216,6 +216,10 @@ int geye_synchronize(geye_ctx *ctx);
  * every launch is bracketed by events; geye_kernel_stats returns, per kernel class, the
  * total milliseconds and launch count since the last reset. */
 int geye_profile_enable(geye_ctx *ctx, int on);
+/* The per-frame work up to the candidate count (ingest .. compaction) is captured once per
+ * plan as a HIP graph and replayed (default on); off runs every launch individually.
+ * Profiling always runs the launches individually. */
+int geye_graph_enable(geye_ctx *ctx, int on);
 int geye_kernel_stats(geye_ctx *ctx, char *names, int names_cap, double *ms, long long *count,
                       int cap, int *n);
 

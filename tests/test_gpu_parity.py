@@ -252,3 +252,32 @@ def test_accumulated_score_matches_oracle(ctx, car_model, which, W, H, thresh):
     gm.get_boxes(frame, thresh=thresh)
     g2 = gm.accumulated_score(W, H, thresh, prior.copy())
     assert bits_equal(g2, O.ascore(m, frame, thresh, prior.copy()))
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_graph_replays_match_eager(ctx, prec):
+    """The per-frame front replayed as a HIP graph equals the launch-by-launch run on every
+    replay, with unrelated eager work and host synchronisation between the replays, for host
+    and device-resident frames."""
+    say("graph replays", prec)
+    m = ctx.model(synth.synthetic_car())
+    other = ctx.model(synth.tiny_model())
+    f1 = synth.make_frame(480, 360, seed=31)
+    f2 = synth.make_frame(200, 150, seed=32)
+    ctx.graphs(False)
+    ref = m.car_detection(f1, thresh=-0.5, overlap=0.5, precision=prec)
+    ref_scores = m.last_scores().copy()
+    ctx.graphs(True)
+    d = ctx.dev_upload(f1)
+    try:
+        for k in range(4):
+            got = (m.car_detection(f1, thresh=-0.5, overlap=0.5, precision=prec) if k % 2 == 0
+                   else m.detect_dev(d, 480, 360, -0.5, 0.5, prec))
+            assert bits_equal(got, ref), k
+            assert bits_equal(m.last_scores(), ref_scores), k
+            ctx.graphs(False)
+            other.car_detection(f2, thresh=-0.5, overlap=0.5, precision=prec)  # eager in between
+            ctx.synchronize()
+            ctx.graphs(True)
+    finally:
+        ctx.dev_free(d)
