@@ -219,6 +219,7 @@ def main() -> int:
 
     # one frame at a time: the kernel breakdown and the rooflines come from this run
     single = single_stream(args.precision, 20)
+    tuning = ctx.mfma_tuning(args.precision) if args.precision != "fp32" else None
     plain = single_stream("fp32", 20) if args.precision != "fp32" else single
 
     # the batched serving form: the headline
@@ -297,6 +298,7 @@ def main() -> int:
         "speedup_vs_fp32_plain": head["fps"] / plain["fps"],
         "single_stream": {"fps": single["fps"], "p50_ms": single["p50_ms"], "p99_ms": single["p99_ms"]},
         "slot_sweep": sweep,
+        "mfma_autotune": tuning,
         "detection_agreement_vs_fp32": iou_agreement(plain["boxes"], single["boxes"]),
         "kernel_ms_per_frame": {k: round(v, 4) for k, v in sorted(per_frame.items())},
     }

@@ -86,6 +86,8 @@ _SIGS = [
                                           _ip]),
     ("geye_add_part", C.c_int, [C.c_void_p, _fp, _fp, _ip, _ip, C.c_int, C.c_int]),
     ("geye_graph_enable", C.c_int, [C.c_void_p, C.c_int]),
+    ("geye_mfma_tuning", C.c_int, [C.c_void_p, C.c_int, _ip, C.c_char_p, C.c_int]),
+    ("geye_mfma_schedule", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("geye_serve_create", C.c_int, [C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
     ("geye_serve_submit", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                     _llp]),
@@ -437,6 +439,21 @@ class Context:
         _check(lib().geye_synchronize(self.h))
 
     # ---- device memory
+    def mfma_tuning(self, precision: str = "bf16") -> dict:
+        """Autotuned MFMA schedule of the last plan: the choice and each schedule's ms."""
+        ch = C.c_int()
+        buf = C.create_string_buffer(4096)
+        _check(lib().geye_mfma_tuning(self.h, PRECISIONS[precision], C.byref(ch), buf, 4096))
+        times = {}
+        for item in buf.value.decode().split(";"):
+            if "=" in item:
+                k, v = item.split("=")
+                times[k] = float(v)
+        return {"choice": ch.value, "ms": times}
+
+    def mfma_schedule(self, precision: str, choice: int) -> None:
+        _check(lib().geye_mfma_schedule(self.h, PRECISIONS[precision], choice))
+
     def graphs(self, on: bool) -> None:
         """Replay the per-frame front as a HIP graph (default) or launch kernels one by one."""
         _check(lib().geye_graph_enable(self.h, int(bool(on))))

@@ -72,6 +72,8 @@ struct MfmaTile {
   long long filt_off;          // 16-bit element offset of the group's packed filters
   int out_slot;                // slot of the group's first member
   int nvalid;                  // real filters in the group
+  int fh, fw, nt;              // filter class shape and 16-filter tiles (work-queue kernel)
+  int cost;                    // relative work, for ordering the queue
 };
 
 // One DT job: a block of lines (columns or rows) of one map.
@@ -158,6 +160,14 @@ struct Plan {
     int h, w, nt, tile_begin, tile_count;
   };
   std::vector<MfmaClass> mfma_classes;
+  // work-queue form: every class's tiles in one list, largest first; one list per tile
+  // height (16 * MT rows, MT = 2 or 4)
+  std::vector<MfmaTile> q_tiles[2];
+  size_t q_lds[2] = {0, 0};
+  // autotuned MFMA schedule per precision (index 1 = bf16, 2 = fp16): -1 = not tuned yet,
+  // 0 = one launch per class, else 1 + (MT index) * 8 + blocks per CU - 1
+  int mfma_choice[3] = {-1, -1, -1};
+  std::vector<std::pair<std::string, float>> mfma_tuning[3];
   std::vector<long long> map_off;      // output map slot -> offset in conv arena
   std::vector<int2> map_dims;          // output map slot -> (H, W)
   long long conv_total = 0;
@@ -179,7 +189,7 @@ struct Plan {
   // device copies of the tables
   DeviceBuffer d_resize_ops, d_resize_tab, d_resize_si, d_resize_alpha, d_hog, d_conv_tiles,
       d_mfma_tiles, d_map_dims, d_map_off, d_dt1, d_dt2, d_nodes, d_depths, d_smaps, d_sparts,
-      d_ascore_tab, d_grad, d_grid;
+      d_ascore_tab, d_grad, d_grid, d_qtiles0, d_qtiles1, d_qcounter;
   // device arenas
   DeviceBuffer d_frame, d_img, d_tmp, d_hist, d_norm, d_feat, d_featb, d_conv, d_tmpM,
       d_tmpIy, d_M, d_tmpIx, d_score, d_flags, d_blocksum, d_cand, d_count, d_rows, d_ascore;
@@ -240,6 +250,7 @@ struct geye_ctx {
   std::vector<geye::DeviceBuffer> scratch_bufs;
   geye::Plan *last_plan = nullptr;
   bool graphs = true;                 // replay the per-frame front as a HIP graph
+  int num_cu = 256;
   long long nms_fallbacks = 0;
 };
 

@@ -109,6 +109,129 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const MfmaTile *__restrict__ 
   }
 }
 
+constexpr int kQueueMaxLds = 160 * 1024 - 1024;  // dynamic LDS cap of the queue kernel
+
+// Work-queue form.  A persistent workgroup takes tile indices from *counter until the list
+// is exhausted (every workgroup reaches that exit).  The tile carries its filter class's
+// shape (fh, fw) and its number of 16-filter tiles nt (1..3), so one launch covers every
+// level of every class; per output the taps are accumulated in the same order as the
+// per-class kernel, so both give the same bits.  Waves own one column of 16*MT rows each.
+template <int KIND, int MT>
+__global__ __launch_bounds__(256) void k_conv_mfma_q(const MfmaTile *__restrict__ tiles,
+                                                     int ntiles, int *__restrict__ counter,
+                                                     const u32x4 *__restrict__ feat,
+                                                     const u32x4 *__restrict__ filt,
+                                                     const long long *__restrict__ map_off,
+                                                     const int2 *__restrict__ map_dims,
+                                                     float *__restrict__ out) {
+  extern __shared__ u32x4 lds[];
+  __shared__ int s_tile;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r = lane & 15, chunk = lane >> 4;
+  for (;;) {
+    if (tid == 0) s_tile = atomicAdd(counter, 1);
+    __syncthreads();
+    const int ti = s_tile;
+    __syncthreads();
+    if (ti >= ntiles) break;
+    const MfmaTile t = tiles[ti];
+    const int fh = t.fh, fw = t.fw, nt = t.nt;
+    const int AX = 4 + fw - 1, AY = 16 * MT + fh - 1;
+    u32x4 *As = lds;                               // [AX][AY][4 chunks]
+    u32x4 *Bs = lds + AX * AY * 4;                 // [2][fh][nt][4][16]
+    const int bstage = fh * nt * 64;
+    const u32x4 *F = feat + t.featb_off / 8;
+    const u32x4 *Wg = filt + t.filt_off / 8;
+    for (int i = tid; i < AX * AY * 4; i += 256) {
+      const int cell = i >> 2, part = i & 3;
+      const int ax = cell / AY, ay = cell - ax * AY;
+      const int gx = t.x0 + ax, gy = t.y0 + ay;
+      u32x4 v = {0u, 0u, 0u, 0u};
+      if (gx < t.F1 && gy < t.F0) v = F[((long long)gx * t.F0 + gy) * 4 + part];
+      As[i] = v;
+    }
+    for (int i = tid; i < bstage; i += 256) Bs[i] = Wg[i];
+    __syncthreads();
+    f32x4 acc[MT][3];
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+      for (int n = 0; n < 3; n++) acc[mt][n] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (int xp = 0; xp < fw; xp++) {
+      const int buf = xp & 1;
+      if (xp + 1 < fw) {
+        const u32x4 *src = Wg + (long long)(xp + 1) * bstage;
+        u32x4 *dst = Bs + (buf ^ 1) * bstage;
+        for (int i = tid; i < bstage; i += 256) dst[i] = src[i];
+      }
+      const u32x4 *Bb = Bs + buf * bstage;
+      const u32x4 *Ab = As + ((wave + xp) * AY) * 4 + chunk;
+      for (int yp = 0; yp < fh; yp++) {
+        u32x4 b0 = Bb[((yp * nt + 0) * 4 + chunk) * 16 + r], b1 = b0, b2 = b0;
+        if (nt > 1) b1 = Bb[((yp * nt + 1) * 4 + chunk) * 16 + r];
+        if (nt > 2) b2 = Bb[((yp * nt + 2) * 4 + chunk) * 16 + r];
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++) {
+          const u32x4 a = Ab[(mt * 16 + r + yp) * 4];
+          acc[mt][0] = mfma16<KIND>(a, b0, acc[mt][0]);
+          if (nt > 1) acc[mt][1] = mfma16<KIND>(a, b1, acc[mt][1]);
+          if (nt > 2) acc[mt][2] = mfma16<KIND>(a, b2, acc[mt][2]);
+        }
+      }
+      __syncthreads();
+    }
+    const int x = t.x0 + wave;
+#pragma unroll
+    for (int n = 0; n < 3; n++) {
+      if (n >= nt) break;
+      const int f = n * 16 + r;
+      if (f >= t.nvalid) continue;
+      const int slot = t.out_slot + f;
+      const int2 hw = map_dims[slot];
+      if (x >= hw.y) continue;
+      float *o = out + map_off[slot] + (long long)x * hw.x;
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int y = t.y0 + mt * 16 + chunk * 4 + i;
+          if (y < hw.x) o[y] = acc[mt][n][i];
+        }
+    }
+  }
+}
+
+size_t mfma_q_lds_bytes(int fh, int fw, int nt, int MT) {
+  return (size_t)16 * ((size_t)(4 + fw - 1) * (16 * MT + fh - 1) * 4 + 2 * (size_t)fh * nt * 64);
+}
+
+template <int KIND, int MT>
+static int launch_q(hipStream_t s, int blocks, size_t lds, const MfmaTile *tiles, int ntiles,
+                    int *counter, const void *feat, const void *filt, const long long *map_off,
+                    const int2 *map_dims, float *out) {
+  // the kernel's static LDS (the tile index) comes out of the same 160 KB
+  static const bool attr = hipFuncSetAttribute((const void *)k_conv_mfma_q<KIND, MT>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               kQueueMaxLds) == hipSuccess;
+  if (!attr) return -1;
+  hipLaunchKernelGGL((k_conv_mfma_q<KIND, MT>), dim3(blocks), dim3(256), lds, s, tiles, ntiles,
+                     counter, (const u32x4 *)feat, (const u32x4 *)filt, map_off, map_dims, out);
+  return 0;
+}
+
+int launch_conv_mfma_queue(hipStream_t s, int kind, int MT, int blocks, size_t lds,
+                           const MfmaTile *tiles, int ntiles, int *counter, const void *feat,
+                           const void *filt, const long long *map_off, const int2 *map_dims,
+                           float *out) {
+  if (ntiles <= 0) return 0;
+  if (lds > (size_t)kQueueMaxLds || blocks <= 0) return -1;
+  if (kind == 1)
+    return MT == 2 ? launch_q<1, 2>(s, blocks, lds, tiles, ntiles, counter, feat, filt, map_off, map_dims, out)
+                   : launch_q<1, 4>(s, blocks, lds, tiles, ntiles, counter, feat, filt, map_off, map_dims, out);
+  return MT == 2 ? launch_q<2, 2>(s, blocks, lds, tiles, ntiles, counter, feat, filt, map_off, map_dims, out)
+                 : launch_q<2, 4>(s, blocks, lds, tiles, ntiles, counter, feat, filt, map_off, map_dims, out);
+}
+
 size_t mfma_lds_bytes(int fh, int fw, int nt) {
   return (size_t)16 * ((size_t)(4 + fw - 1) * (64 + fh - 1) * 4 + 2 * (size_t)fh * nt * 64);
 }

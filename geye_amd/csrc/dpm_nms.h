@@ -22,6 +22,7 @@ struct NmsMaps {
   const int *rsize;
   int padx, pady;
   int *rank_of_flat;
+  const int *cand;   // candidate -> flat score index (retires decided cells between rounds)
 };
 
 size_t nms_sort_temp_bytes(int n);

@@ -408,6 +408,23 @@ __global__ __launch_bounds__(256) void k_nms_scatter(const NmsBox *__restrict__ 
   }
 }
 
+// End of a round: every box of the window is decided, and so is every later box some pick
+// suppressed.  Their score cells leave rank_of_flat (-1), so the next rounds' scatters
+// reject them with the lookup they already make.
+__global__ __launch_bounds__(256) void k_nms_retire(const int *__restrict__ alive,
+                                                    const int *__restrict__ nalive,
+                                                    const int *__restrict__ sup,
+                                                    const NmsBox *__restrict__ sbox,
+                                                    const int *__restrict__ cand,
+                                                    int *__restrict__ rank_of_flat) {
+  const int na = *nalive;
+  const int W = min(na, kNmsWindow);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= na) return;
+  const int r = alive[t];
+  if (t < W || sup[r] != kNoSup) rank_of_flat[cand[sbox[r].idx]] = -1;
+}
+
 __global__ __launch_bounds__(256) void k_nms_aliveflags(const int *__restrict__ alive,
                                                         const int *__restrict__ nalive,
                                                         const int *__restrict__ sup,
@@ -535,6 +552,9 @@ void launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const
   }
   hipLaunchKernelGGL(k_nms_aliveflags, dim3((rest + 255) / 256), dim3(256), 0, s, alive, nalive,
                      sup, keep);
+  if (maps && maps->cand)
+    hipLaunchKernelGGL(k_nms_retire, dim3((nalive_host + 255) / 256), dim3(256), 0, s, alive,
+                       nalive, sup, sbox, maps->cand, maps->rank_of_flat);
 }
 
 void launch_nms_regather(hipStream_t s, const int *alive, const int *nalive, const int *sel,

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "dpm_mfma.h"
 #include "dpm_runtime.h"
 
 namespace geye {
@@ -340,6 +341,9 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
   P.conv_classes.clear();
   P.mfma_tiles.clear();
   P.mfma_classes.clear();
+  P.q_tiles[0].clear();
+  P.q_tiles[1].clear();
+  P.q_lds[0] = P.q_lds[1] = 0;
   P.map_off.clear();
   P.map_dims.clear();
   P.root_slot.assign((size_t)LEN * m.NoR, -1);
@@ -420,8 +424,34 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
             t.filt_off = fc.hoff + (long long)g * fc.w * fc.h * fc.nt * 512;
             t.out_slot = base + g * gsz;
             t.nvalid = std::min(gsz, nmem - g * gsz);
+            t.fh = fc.h;
+            t.fw = fc.w;
+            t.nt = fc.nt;
+            t.cost = 0;
             P.mfma_tiles.push_back(t);
           }
+        // work-queue tiles: 4 columns x 32 or 64 rows
+        for (int mi = 0; mi < 2; mi++) {
+          const int rows = mi == 0 ? 32 : 64;
+          for (int x0 = 0; x0 < Wm; x0 += 4)
+            for (int y0 = 0; y0 < Hm; y0 += rows) {
+              MfmaTile t;
+              t.featb_off = featb_off[fl];
+              t.F0 = FL.F0;
+              t.F1 = FL.F1;
+              t.x0 = x0;
+              t.y0 = y0;
+              t.filt_off = fc.hoff + (long long)g * fc.w * fc.h * fc.nt * 512;
+              t.out_slot = base + g * gsz;
+              t.nvalid = std::min(gsz, nmem - g * gsz);
+              t.fh = fc.h;
+              t.fw = fc.w;
+              t.nt = fc.nt;
+              t.cost = fc.h * fc.w * fc.nt * std::min(rows, Hm - y0);
+              P.q_tiles[mi].push_back(t);
+            }
+          P.q_lds[mi] = std::max(P.q_lds[mi], mfma_q_lds_bytes(fc.h, fc.w, fc.nt, mi == 0 ? 2 : 4));
+        }
       }
     }
     cc.tile_count = (int)P.conv_tiles.size() - cc.tile_begin;
@@ -430,6 +460,9 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
     P.mfma_classes.push_back(mc);
   }
   P.conv_total = co;
+  for (int mi = 0; mi < 2; mi++)
+    std::stable_sort(P.q_tiles[mi].begin(), P.q_tiles[mi].end(),
+                     [](const MfmaTile &a, const MfmaTile &b) { return a.cost > b.cost; });
   // ---- DT jobs and score maps
   P.dt_pass1.clear();
   P.dt_pass2.clear();

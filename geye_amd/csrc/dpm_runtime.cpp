@@ -97,13 +97,13 @@ struct KTimer {
   std::string name;
   hipEvent_t a = nullptr, b = nullptr;
   KTimer(geye_ctx *ctx, std::string n) : c(ctx), name(std::move(n)) {
-    if (!c->profile) return;
+    if (!c->profile || name.empty()) return;
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
     (void)hipEventRecord(a, c->stream);
   }
   ~KTimer() {
-    if (!c->profile) return;
+    if (!c->profile || name.empty()) return;
     (void)hipEventRecord(b, c->stream);
     c->pending.push_back({name, {a, b}});
   }
@@ -457,6 +457,9 @@ int plan_upload(Plan &P, hipStream_t st) {
   CHK(upload(P.d_sparts, P.sparts, st));
   CHK(upload(P.d_ascore_tab, P.ascore_tab, st));
   CHK(upload(P.d_grad, P.grad, st));
+  CHK(upload(P.d_qtiles0, P.q_tiles[0], st));
+  CHK(upload(P.d_qtiles1, P.q_tiles[1], st));
+  CHK(ensure(P.d_qcounter, 256));
   CHK(ensure(P.d_grid, sizeof(int2) * std::max<long long>(P.grid_total, 1)));
   long long tmp = 0;
   for (auto &R : P.rounds) tmp = std::max(tmp, R.tmp_total);
@@ -492,7 +495,8 @@ void plan_release(Plan &P) {
                           &P.n_status, &P.n_fsup, &P.n_picks, &P.n_counters, &P.n_mask,
                           &P.n_keep, &P.n_sel, &P.n_sort, &P.n_flags, &P.n_keep_orig,
                           &P.n_kept, &P.n_kept_flat, &P.n_total, &P.n_vals, &P.n_ord,
-                          &P.n_rank, &P.d_ascore_tab, &P.d_ascore, &P.d_grad, &P.d_grid})
+                          &P.n_rank, &P.d_ascore_tab, &P.d_ascore, &P.d_grad, &P.d_grid,
+                          &P.d_qtiles0, &P.d_qtiles1, &P.d_qcounter})
     release(*b);
 }
 
@@ -638,6 +642,83 @@ int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, int hk
 
 int read_count(geye_ctx *c, Plan &P, int *ncand);
 
+// MFMA convolution schedule `choice`: 0 = one launch per filter class (4 x 64 tiles), else
+// the work-queue launch over every class with 16 * MT row tiles and `bpc` workgroups per CU.
+int run_mfma(geye_ctx *c, Model &m, Plan &P, int precision, int choice, bool timed) {
+  hipStream_t s = c->stream;
+  const void *filt = precision == GEYE_BF16 ? m.d_filt_bf16.p : m.d_filt_f16.p;
+  const std::string tag = precision == GEYE_BF16 ? "conv_bf16_" : "conv_fp16_";
+  if (choice == 0) {
+    for (auto &mc : P.mfma_classes) {
+      if (mc.tile_count == 0) continue;
+      KTimer t(c, timed ? tag + std::to_string(mc.h) + "x" + std::to_string(mc.w) : std::string());
+      if (launch_conv_mfma(s, precision, mc.nt, dp<MfmaTile>(P.d_mfma_tiles) + mc.tile_begin,
+                           mc.tile_count, mc.h, mc.w, P.d_featb.p, filt,
+                           dp<long long>(P.d_map_off), dp<int2>(P.d_map_dims),
+                           dp<float>(P.d_conv)) != 0)
+        return fail(GEYE_ERR_UNSUPPORTED, "filter class too large for the MFMA kernel's LDS");
+    }
+    return GEYE_OK;
+  }
+  const int mi = (choice - 1) / 8, bpc = (choice - 1) % 8 + 1;
+  const int MT = mi == 0 ? 2 : 4;
+  const DeviceBuffer &qt = mi == 0 ? P.d_qtiles0 : P.d_qtiles1;
+  launch_fill(s, dp<int>(P.d_qcounter), 1, 0);
+  KTimer t(c, timed ? tag + "queue" : std::string());
+  if (launch_conv_mfma_queue(s, precision, MT, c->num_cu * bpc, P.q_lds[mi], (const MfmaTile *)qt.p,
+                             (int)P.q_tiles[mi].size(), dp<int>(P.d_qcounter), P.d_featb.p, filt,
+                             dp<long long>(P.d_map_off), dp<int2>(P.d_map_dims),
+                             dp<float>(P.d_conv)) != 0)
+    return fail(GEYE_ERR_UNSUPPORTED, "filter classes too large for the work-queue kernel's LDS");
+  return GEYE_OK;
+}
+
+// Autotuner: times every admissible schedule on this plan's own features (the pyramid of
+// the frame being processed is already in HBM) and keeps the fastest.  All schedules give
+// the same bits, so the choice only changes speed.  Runs once per plan and precision,
+// before any graph capture.
+int tune_mfma(geye_ctx *c, Model &m, Plan &P, int precision) {
+  std::vector<int> cands{0};
+  for (int mi = 0; mi < 2; mi++)
+    for (int bpc = 1; bpc <= 4; bpc++)
+      if (!P.q_tiles[mi].empty() && P.q_lds[mi] * bpc <= 159 * 1024) cands.push_back(1 + mi * 8 + bpc - 1);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const bool prof = c->profile;
+  c->profile = false;
+  float best = 1e30f;
+  int pick = 0;
+  P.mfma_tuning[precision].clear();
+  for (int cand : cands) {
+    int rc = run_mfma(c, m, P, precision, cand, false);  // warm-up (attributes, caches)
+    float ms = 0;
+    if (rc == GEYE_OK && hipEventRecord(e0, c->stream) == hipSuccess) {
+      for (int k = 0; k < 3 && rc == GEYE_OK; k++) rc = run_mfma(c, m, P, precision, cand, false);
+      (void)hipEventRecord(e1, c->stream);
+      (void)hipEventSynchronize(e1);
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      ms /= 3;
+    }
+    if (rc != GEYE_OK) continue;
+    const int mi = (cand - 1) / 8, bpc = (cand - 1) % 8 + 1;
+    P.mfma_tuning[precision].push_back(
+        {cand == 0 ? std::string("per-class 4x64") :
+                     "queue 4x" + std::to_string(mi == 0 ? 32 : 64) + " bpc" + std::to_string(bpc),
+         ms});
+    if (ms < best) {
+      best = ms;
+      pick = cand;
+    }
+  }
+  c->profile = prof;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipGetLastError();
+  P.mfma_choice[precision] = pick;
+  return GEYE_OK;
+}
+
 // Launches the convolution, DT, scoring and candidate compaction; with ncand non-null also
 // waits for the candidate count.
 int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int *ncand) {
@@ -651,17 +732,8 @@ int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int 
                       dp<long long>(P.d_map_off), dp<int2>(P.d_map_dims), dp<float>(P.d_conv));
     }
   } else if (precision == GEYE_BF16 || precision == GEYE_FP16) {
-    const void *filt = precision == GEYE_BF16 ? m.d_filt_bf16.p : m.d_filt_f16.p;
-    const char *tag = precision == GEYE_BF16 ? "conv_bf16_" : "conv_fp16_";
-    for (auto &mc : P.mfma_classes) {
-      if (mc.tile_count == 0) continue;
-      KTimer t(c, tag + std::to_string(mc.h) + "x" + std::to_string(mc.w));
-      if (launch_conv_mfma(s, precision, mc.nt, dp<MfmaTile>(P.d_mfma_tiles) + mc.tile_begin,
-                           mc.tile_count, mc.h, mc.w, P.d_featb.p, filt,
-                           dp<long long>(P.d_map_off), dp<int2>(P.d_map_dims),
-                           dp<float>(P.d_conv)) != 0)
-        return fail(GEYE_ERR_UNSUPPORTED, "filter class too large for the MFMA kernel's LDS");
-    }
+    if (P.mfma_choice[precision] < 0) CHK(tune_mfma(c, m, P, precision));
+    CHK(run_mfma(c, m, P, precision, P.mfma_choice[precision], true));
   } else {
     return fail(GEYE_ERR_ARG, "unknown precision");
   }
@@ -855,7 +927,7 @@ int run_nms(geye_ctx *c, Model &m, Plan &P, int n, float overlap, int *nkept) {
   if (use_maps) {
     CHK(ensure(P.n_rank, sizeof(int) * (size_t)std::max<long long>(P.score_total, 1)));
     mp = NmsMaps{dp<ScoreMap>(P.d_smaps), (int)P.smaps.size(), dp<int>(m.d_rsize), m.padx,
-                 m.pady, dp<int>(P.n_rank)};
+                 m.pady, dp<int>(P.n_rank), dp<int>(P.d_cand)};
   }
   int nk = 0;
   CHK(nms_core(c, P, n, overlap, use_maps ? &mp : nullptr, dp<int>(P.d_cand), &nk));
@@ -1103,6 +1175,9 @@ int geye_init(int device, geye_ctx **ctx) {
   HIPCHK(hipSetDevice(device));
   geye_ctx *c = new geye_ctx();
   c->device = device;
+  if (hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      c->num_cu <= 0)
+    c->num_cu = 256;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete c;
@@ -1786,6 +1861,41 @@ int geye_synchronize(geye_ctx *c) {
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
   flush_stats(c);
+  return GEYE_OK;
+}
+
+int geye_mfma_tuning(geye_ctx *c, int precision, int *choice, char *text, int cap) {
+  if (!c || (precision != GEYE_BF16 && precision != GEYE_FP16)) return fail(GEYE_ERR_ARG, "bad argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  Plan *P = c->last_plan;
+  if (!P) return fail(GEYE_ERR_ARG, "no detection has run on this context");
+  if (choice) *choice = P->mfma_choice[precision];
+  std::string t;
+  for (auto &kv : P->mfma_tuning[precision]) {
+    char b[96];
+    std::snprintf(b, sizeof b, "%s=%.4f;", kv.first.c_str(), kv.second);
+    t += b;
+  }
+  if (text && cap > 0) {
+    std::strncpy(text, t.c_str(), (size_t)cap - 1);
+    text[cap - 1] = 0;
+  }
+  return GEYE_OK;
+}
+
+int geye_mfma_schedule(geye_ctx *c, int precision, int choice) {
+  if (!c || (precision != GEYE_BF16 && precision != GEYE_FP16)) return fail(GEYE_ERR_ARG, "bad argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  Plan *P = c->last_plan;
+  if (!P) return fail(GEYE_ERR_ARG, "no detection has run on this context");
+  if (choice != 0) {
+    const int mi = (choice - 1) / 8, bpc = (choice - 1) % 8 + 1;
+    if (choice < 0 || mi > 1 || P->q_tiles[mi].empty() || P->q_lds[mi] * bpc > 159 * 1024)
+      return fail(GEYE_ERR_ARG, "no such MFMA schedule for this plan");
+  }
+  P->mfma_choice[precision] = choice;
+  for (auto &kv : P->graphs) (void)hipGraphExecDestroy(kv.second);
+  P->graphs.clear();
   return GEYE_OK;
 }
 

@@ -220,6 +220,14 @@ int geye_profile_enable(geye_ctx *ctx, int on);
  * plan as a HIP graph and replayed (default on); off runs every launch individually.
  * Profiling always runs the launches individually. */
 int geye_graph_enable(geye_ctx *ctx, int on);
+/* MFMA convolution schedule of the last plan run on ctx.  On its first BF16/FP16 frame a
+ * plan times every admissible schedule on its own features and keeps the fastest: 0 = one
+ * launch per filter class (4 x 64 tiles), 1 + 8*m + (b - 1) = one persistent work-queue
+ * launch over every level and filter class with 4 x (32, 64)[m] tiles and b workgroups per
+ * CU.  geye_mfma_tuning reports the choice and "name=ms;" per schedule tried;
+ * geye_mfma_schedule overrides it (every schedule gives the same bits). */
+int geye_mfma_tuning(geye_ctx *ctx, int precision, int *choice, char *text, int cap);
+int geye_mfma_schedule(geye_ctx *ctx, int precision, int choice);
 int geye_kernel_stats(geye_ctx *ctx, char *names, int names_cap, double *ms, long long *count,
                       int cap, int *n);
 

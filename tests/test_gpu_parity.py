@@ -281,3 +281,28 @@ def test_graph_replays_match_eager(ctx, prec):
             ctx.graphs(True)
     finally:
         ctx.dev_free(d)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_every_mfma_schedule_gives_the_same_scores(ctx, prec):
+    """The autotuner's candidates (one launch per class; the persistent work-queue launch at
+    two tile heights and several grid sizes) produce identical score maps."""
+    say("mfma schedules", prec)
+    m = ctx.model(synth.synthetic_car())
+    frame = synth.make_frame(480, 360, seed=12)
+    ctx.graphs(False)
+    try:
+        m.get_boxes(frame, thresh=-0.5, precision=prec)
+        tuned = ctx.mfma_tuning(prec)
+        say(tuned)
+        assert tuned["choice"] >= 0 and len(tuned["ms"]) >= 2
+        ref = m.last_scores().copy()
+        for choice in [0, 1, 2, 9, 10]:
+            try:
+                ctx.mfma_schedule(prec, choice)
+            except Exception:
+                continue  # not admissible for this plan's LDS
+            m.get_boxes(frame, thresh=-0.5, precision=prec)
+            assert bits_equal(m.last_scores(), ref), choice
+    finally:
+        ctx.graphs(True)
