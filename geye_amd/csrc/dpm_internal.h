@@ -39,6 +39,16 @@ struct GradImg {
   int blocks_begin;
 };
 
+// One fused resize of a whole image: out = cols(rows(in)), 16 x 64 output tiles.  rtab /
+// ctab index the (start, count) tap lists of the row and column passes.
+struct FusedResize {
+  long long in_off, out_off;
+  int in_h, in_w, out_h, out_w;
+  int rtab_off, ctab_off;
+  int tiles_x, tiles_y, blocks_begin;
+};
+constexpr int kResizeTY = 16, kResizeTX = 64, kResizeSpan = 136;  // span: source columns
+
 struct HogLevel {
   long long img_off;           // image arena offset ([3][h][w] row-major)
   long long grid_off;          // gradient grid of the image (GradImg)
@@ -137,6 +147,8 @@ struct Plan {
     long long tmp_total = 0;
   };
   std::vector<Round> rounds;
+  std::vector<std::vector<FusedResize>> fused;  // per round
+  std::vector<int> fused_blocks;
   std::vector<int2> resize_tab;      // (start, count) per destination index, all ops
   std::vector<int> resize_si;
   std::vector<float> resize_alpha;
@@ -189,7 +201,7 @@ struct Plan {
   // device copies of the tables
   DeviceBuffer d_resize_ops, d_resize_tab, d_resize_si, d_resize_alpha, d_hog, d_conv_tiles,
       d_mfma_tiles, d_map_dims, d_map_off, d_dt1, d_dt2, d_nodes, d_depths, d_smaps, d_sparts,
-      d_ascore_tab, d_grad, d_grid, d_qtiles0, d_qtiles1, d_qcounter;
+      d_ascore_tab, d_grad, d_grid, d_qtiles0, d_qtiles1, d_qcounter, d_fused;
   // device arenas
   DeviceBuffer d_frame, d_img, d_tmp, d_hist, d_norm, d_feat, d_featb, d_conv, d_tmpM,
       d_tmpIy, d_M, d_tmpIx, d_score, d_flags, d_blocksum, d_cand, d_count, d_rows, d_ascore;

@@ -14,6 +14,9 @@ void launch_resize_cols(hipStream_t s, const ResizeOp *ops, int nops, int blocks
                         float *dst);
 // HOG histogram in two launches: gradients per pyramid image, then votes per level
 // (HogLevel.T must be 16: 16 x 16 cell tiles)
+void launch_resize_fused(hipStream_t s, const FusedResize *ops, int nops, int blocks,
+                         const int2 *tab, const int *si, const float *alpha, const float *src,
+                         float *dst);
 void launch_hog_grad(hipStream_t s, const GradImg *imgs, int nimg, int blocks, const float *img,
                      int2 *grid);
 void launch_hog_votes(hipStream_t s, const HogLevel *lv, int nlv, int blocks, const int2 *grid,

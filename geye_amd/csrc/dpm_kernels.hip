@@ -85,6 +85,85 @@ __global__ __launch_bounds__(256) void k_resize_cols(const ResizeOp *__restrict_
   dst[op.out_off + e] = acc;
 }
 
+// Both passes of one resize, per 16 x 64 output tile (CT/resize.cc:135-160): the row pass
+// for the tile's 16 output rows over the source columns its 64 outputs tap goes to LDS, the
+// column pass reads it there.  Each element is the same ordered tap sum (from 0, table
+// order) as the two separate passes, so the image is bit-identical.
+__global__ __launch_bounds__(256) void k_resize_fused(const FusedResize *__restrict__ ops,
+                                                      int nops, const int2 *__restrict__ tab,
+                                                      const int *__restrict__ si,
+                                                      const float *__restrict__ alpha,
+                                                      const float *__restrict__ src,
+                                                      float *__restrict__ dst) {
+  constexpr int MAXTAP = 4;  // scales >= 0.5: at most 4 source samples per output
+  __shared__ float s_t[kResizeTY][kResizeSpan];
+  __shared__ int s_rn[kResizeTY], s_cn[kResizeTX];
+  __shared__ int s_rsi[kResizeTY][MAXTAP], s_csi[kResizeTX][MAXTAP];
+  __shared__ float s_ral[kResizeTY][MAXTAP], s_cal[kResizeTX][MAXTAP];
+  int k = 0;
+  while (k + 1 < nops && ops[k + 1].blocks_begin <= (int)blockIdx.x) k++;
+  const FusedResize op = ops[k];
+  const int tile = blockIdx.x - op.blocks_begin;
+  const int tyi = tile / op.tiles_x, txi = tile - tyi * op.tiles_x;
+  const int dy0 = tyi * kResizeTY, dx0 = txi * kResizeTX;
+  const int ny = min(kResizeTY, op.out_h - dy0), nx = min(kResizeTX, op.out_w - dx0);
+  const int tid = threadIdx.x;
+  // the tile's tap lists (row pass for its rows, column pass for its columns)
+  if (tid < kResizeTY + kResizeTX) {
+    const bool row = tid < kResizeTY;
+    const int j = row ? tid : tid - kResizeTY;
+    const bool live = row ? j < ny : j < nx;
+    const int2 t = live ? tab[(row ? op.rtab_off + dy0 : op.ctab_off + dx0) + j] : make_int2(0, 0);
+    const int n = min(t.y, MAXTAP);
+    int *osi = row ? s_rsi[j] : s_csi[j];
+    float *oal = row ? s_ral[j] : s_cal[j];
+    for (int q = 0; q < n; q++) {
+      osi[q] = si[t.x + q];
+      oal[q] = alpha[t.x + q];
+    }
+    (row ? s_rn : s_cn)[j] = n;
+  }
+  __syncthreads();
+  const int c0 = s_csi[0][0], span = s_csi[nx - 1][s_cn[nx - 1] - 1] - c0 + 1;
+  for (int c = 0; c < 3; c++) {
+    const float *in = src + op.in_off + (long long)c * op.in_h * op.in_w + c0;
+    for (int i = tid; i < ny * span; i += 256) {
+      const int r = i / span, x = i - r * span;
+      const int n = s_rn[r];
+      float v[MAXTAP];
+#pragma unroll
+      for (int q = 0; q < MAXTAP; q++)
+        if (q < n) v[q] = in[(long long)s_rsi[r][q] * op.in_w + x];
+      float acc = 0.0f;
+#pragma unroll
+      for (int q = 0; q < MAXTAP; q++)
+        if (q < n) acc += s_ral[r][q] * v[q];
+      s_t[r][x] = acc;
+    }
+    __syncthreads();
+    float *out = dst + op.out_off + (long long)c * op.out_h * op.out_w;
+    for (int i = tid; i < ny * kResizeTX; i += 256) {
+      const int r = i / kResizeTX, x = i - r * kResizeTX;
+      if (x >= nx) continue;
+      const int n = s_cn[x];
+      float acc = 0.0f;
+#pragma unroll
+      for (int q = 0; q < MAXTAP; q++)
+        if (q < n) acc += s_cal[x][q] * s_t[r][s_csi[x][q] - c0];
+      out[(long long)(dy0 + r) * op.out_w + dx0 + x] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+void launch_resize_fused(hipStream_t s, const FusedResize *ops, int nops, int blocks,
+                         const int2 *tab, const int *si, const float *alpha, const float *src,
+                         float *dst) {
+  if (blocks > 0)
+    hipLaunchKernelGGL(k_resize_fused, dim3(blocks), dim3(256), 0, s, ops, nops, tab, si, alpha,
+                       src, dst);
+}
+
 // ---------------------------------------------------------------------------------------
 // HOG orientation histogram + block energy.
 // A workgroup owns a TxT tile of cells.  Stage 1 computes, once per pixel of the tile's

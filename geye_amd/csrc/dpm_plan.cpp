@@ -261,6 +261,50 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
     }
     R.tmp_total = toff;
   }
+  // fused form: the same tables, one launch per round
+  P.fused.assign(P.rounds.size(), {});
+  P.fused_blocks.assign(P.rounds.size(), 0);
+  for (size_t r = 0; r < P.rounds.size(); r++) {
+    const Plan::Round &R = P.rounds[r];
+    for (size_t k = 0; k < R.rows.size(); k++) {
+      const ResizeOp &ro = R.rows[k], &co = R.cols[k];
+      FusedResize f{};
+      f.in_off = ro.in_off;
+      f.out_off = co.out_off;
+      f.in_h = ro.in_h;
+      f.in_w = ro.in_w;
+      f.out_h = co.out_h;
+      f.out_w = co.out_w;
+      f.rtab_off = ro.tab_off;
+      f.ctab_off = co.tab_off;
+      f.tiles_x = (f.out_w + kResizeTX - 1) / kResizeTX;
+      f.tiles_y = (f.out_h + kResizeTY - 1) / kResizeTY;
+      f.blocks_begin = P.fused_blocks[r];
+      P.fused_blocks[r] += f.tiles_x * f.tiles_y;
+      // every tile's source-column span must fit the kernel's LDS rows
+      for (int tx = 0; tx < f.tiles_x; tx++) {
+        const int d0 = tx * kResizeTX, d1 = std::min(f.out_w, d0 + kResizeTX) - 1;
+        const int2 a = P.resize_tab[f.ctab_off + d0], b = P.resize_tab[f.ctab_off + d1];
+        const int c0 = P.resize_si[a.x], c1 = P.resize_si[b.x + b.y - 1];
+        if (c1 - c0 + 1 > kResizeSpan) {
+          err = "resize scale below the fused kernel's range";
+          return GEYE_ERR_UNSUPPORTED;
+        }
+      }
+      // and every output must use at most 4 source samples per pass
+      for (int d = 0; d < f.out_w; d++)
+        if (P.resize_tab[f.ctab_off + d].y > 4) {
+          err = "resize scale below the fused kernel's range";
+          return GEYE_ERR_UNSUPPORTED;
+        }
+      for (int d = 0; d < f.out_h; d++)
+        if (P.resize_tab[f.rtab_off + d].y > 4) {
+          err = "resize scale below the fused kernel's range";
+          return GEYE_ERR_UNSUPPORTED;
+        }
+      P.fused[r].push_back(f);
+    }
+  }
   // ---- gradient grids (one per pyramid image, shared by its sbin 4 and sbin 8 levels)
   P.grad.clear();
   std::vector<long long> grid_of(P.img_h.size(), 0);

@@ -457,6 +457,11 @@ int plan_upload(Plan &P, hipStream_t st) {
   CHK(upload(P.d_sparts, P.sparts, st));
   CHK(upload(P.d_ascore_tab, P.ascore_tab, st));
   CHK(upload(P.d_grad, P.grad, st));
+  {
+    std::vector<FusedResize> fl;
+    for (auto &v : P.fused) fl.insert(fl.end(), v.begin(), v.end());
+    CHK(upload(P.d_fused, fl, st));
+  }
   CHK(upload(P.d_qtiles0, P.q_tiles[0], st));
   CHK(upload(P.d_qtiles1, P.q_tiles[1], st));
   CHK(ensure(P.d_qcounter, 256));
@@ -464,7 +469,7 @@ int plan_upload(Plan &P, hipStream_t st) {
   long long tmp = 0;
   for (auto &R : P.rounds) tmp = std::max(tmp, R.tmp_total);
   CHK(ensure(P.d_img, sizeof(float) * P.img_total));
-  CHK(ensure(P.d_tmp, sizeof(float) * tmp));
+  (void)tmp;  // the fused resize keeps the row pass in LDS: no intermediate image
   CHK(ensure(P.d_hist, sizeof(float) * P.hist_total));
   CHK(ensure(P.d_norm, sizeof(float) * P.norm_total));
   CHK(ensure(P.d_feat, sizeof(float) * P.feat_total));
@@ -496,7 +501,7 @@ void plan_release(Plan &P) {
                           &P.n_keep, &P.n_sel, &P.n_sort, &P.n_flags, &P.n_keep_orig,
                           &P.n_kept, &P.n_kept_flat, &P.n_total, &P.n_vals, &P.n_ord,
                           &P.n_rank, &P.d_ascore_tab, &P.d_ascore, &P.d_grad, &P.d_grid,
-                          &P.d_qtiles0, &P.d_qtiles1, &P.d_qcounter})
+                          &P.d_qtiles0, &P.d_qtiles1, &P.d_qcounter, &P.d_fused})
     release(*b);
 }
 
@@ -556,7 +561,8 @@ int plan_bytes(const Model &m, int W, int H, long long *bytes) {
   long long tmp = 0;
   for (auto &R : P.rounds) tmp = std::max(tmp, R.tmp_total);
   const long long N = std::max<long long>(P.score_total, 1);
-  long long b = 4 * (P.img_total + tmp + P.hist_total + P.norm_total + P.feat_total + P.conv_total +
+  (void)tmp;
+  long long b = 4 * (P.img_total + P.hist_total + P.norm_total + P.feat_total + P.conv_total +
                      4 * P.dt_total + 3 * P.score_total) +
                 2 * P.featb_total + 8 * P.grid_total + (long long)P.W * P.H * (3 + 4) +
                 N * (2 * (long long)sizeof(NmsBox) + 16 * 4) + (long long)P.GL * 4 * N;
@@ -600,23 +606,13 @@ int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, int hk
     KTimer t(c, "ingest");
     launch_ingest(s, d_bgr, P.W, P.H, ws, dp<float>(P.d_img) + P.img_off[0]);
   }
-  int op_base = 0;
-  for (auto &R : P.rounds) {
-    const ResizeOp *ops = dp<ResizeOp>(P.d_resize_ops) + op_base;
-    const int nr = (int)R.rows.size(), nc = (int)R.cols.size();
-    {
-      KTimer t(c, "resize_rows");
-      launch_resize_rows(s, ops, nr, R.rows_blocks, dp<int2>(P.d_resize_tab),
-                         dp<int>(P.d_resize_si), dp<float>(P.d_resize_alpha), dp<float>(P.d_img),
-                         dp<float>(P.d_tmp));
-    }
-    {
-      KTimer t(c, "resize_cols");
-      launch_resize_cols(s, ops + nr, nc, R.cols_blocks, dp<int2>(P.d_resize_tab),
-                         dp<int>(P.d_resize_si), dp<float>(P.d_resize_alpha), dp<float>(P.d_tmp),
-                         dp<float>(P.d_img));
-    }
-    op_base += nr + nc;
+  int fbase = 0;
+  for (size_t r = 0; r < P.fused.size(); r++) {
+    KTimer t(c, "resize");
+    launch_resize_fused(s, dp<FusedResize>(P.d_fused) + fbase, (int)P.fused[r].size(),
+                        P.fused_blocks[r], dp<int2>(P.d_resize_tab), dp<int>(P.d_resize_si),
+                        dp<float>(P.d_resize_alpha), dp<float>(P.d_img), dp<float>(P.d_img));
+    fbase += (int)P.fused[r].size();
   }
   const HogLevel *hog = dp<HogLevel>(P.d_hog);
   {
