@@ -38,8 +38,10 @@ void launch_nms_gather(hipStream_t s, const int *ord, const NmsBox *box, int n, 
 void launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const int *nalive,
                       int nalive_host, float overlap, unsigned long long *mask, int *status,
                       int *sup, int *picks, int *counters, int *keep, const NmsMaps *maps);
+// nalive_host in launch_nms_round and bound here are host upper bounds of the device counts
+// (*nalive, *nsel): the kernels read the true counts and do nothing past them.
 void launch_nms_regather(hipStream_t s, const int *alive, const int *nalive, const int *sel,
-                         int nsel, int *alive_out);
+                         const int *nsel, int bound, int *alive_out);
 void launch_nms_check(hipStream_t s, const NmsBox *sbox, int n, const int *status, const int *sup,
                       const int *picks, int npick, int *flags);
 void launch_nms_keep(hipStream_t s, const NmsBox *sbox, const int *picks, int npick,

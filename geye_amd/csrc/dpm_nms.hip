@@ -425,24 +425,28 @@ __global__ __launch_bounds__(256) void k_nms_retire(const int *__restrict__ aliv
   if (t < W || sup[r] != kNoSup) rank_of_flat[cand[sbox[r].idx]] = -1;
 }
 
+// keep[j] for the j-th box after the window; the launch covers a host bound on that count
+// and writes 0 past the true count, so the compaction over the bound counts exactly.
 __global__ __launch_bounds__(256) void k_nms_aliveflags(const int *__restrict__ alive,
                                                         const int *__restrict__ nalive,
                                                         const int *__restrict__ sup,
-                                                        int *__restrict__ keep) {
+                                                        int *__restrict__ keep, int bound) {
   const int na = *nalive;
   const int W = min(na, kNmsWindow);
-  const int t = W + blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= na) return;
-  keep[t - W] = sup[alive[t]] == kNoSup ? 1 : 0;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= bound) return;
+  const int t = W + j;
+  keep[j] = (t < na && sup[alive[t]] == kNoSup) ? 1 : 0;
 }
 
 __global__ __launch_bounds__(256) void k_nms_regather(const int *__restrict__ alive,
                                                       const int *__restrict__ nalive,
-                                                      const int *__restrict__ sel, int nsel,
+                                                      const int *__restrict__ sel,
+                                                      const int *__restrict__ nsel,
                                                       int *__restrict__ alive_out) {
   const int W = min(*nalive, kNmsWindow);
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < nsel) alive_out[k] = alive[W + sel[k]];
+  if (k < *nsel) alive_out[k] = alive[W + sel[k]];
 }
 
 // finalisation: conflicts, last-pick emission, survivors in original order
@@ -551,16 +555,16 @@ void launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const
                        nalive, picks, counters, overlap, sup);
   }
   hipLaunchKernelGGL(k_nms_aliveflags, dim3((rest + 255) / 256), dim3(256), 0, s, alive, nalive,
-                     sup, keep);
+                     sup, keep, rest);
   if (maps && maps->cand)
     hipLaunchKernelGGL(k_nms_retire, dim3((nalive_host + 255) / 256), dim3(256), 0, s, alive,
                        nalive, sup, sbox, maps->cand, maps->rank_of_flat);
 }
 
 void launch_nms_regather(hipStream_t s, const int *alive, const int *nalive, const int *sel,
-                         int nsel, int *alive_out) {
-  if (nsel > 0)
-    hipLaunchKernelGGL(k_nms_regather, dim3((nsel + 255) / 256), dim3(256), 0, s, alive, nalive,
+                         const int *nsel, int bound, int *alive_out) {
+  if (bound > 0)
+    hipLaunchKernelGGL(k_nms_regather, dim3((bound + 255) / 256), dim3(256), 0, s, alive, nalive,
                        sel, nsel, alive_out);
 }
 

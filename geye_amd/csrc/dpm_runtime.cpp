@@ -841,24 +841,36 @@ int nms_core(geye_ctx *c, Plan &P, int n, float overlap, const NmsMaps *maps, co
   int *d_sel_total = dp<int>(P.n_total) + 1;
   launch_fill(s, d_na, 1, n);
   int *alive = dp<int>(P.n_alive0), *alive_next = dp<int>(P.n_alive1);
-  int na = n;
-  while (na > 0) {
+  // Rounds run one ahead of the host: after round k's compaction, round k+1 is enqueued with
+  // a grid bound of round k's boxes after its window (its alive count can only be smaller)
+  // before the host waits for round k's survivor count, so the GPU never idles on the
+  // read-back.  The kernels read the true counts from device memory and do nothing past
+  // them; the compaction over a bound counts exactly because the flags past the true count
+  // are written as 0.
+  auto round = [&](int bound) {
     KTimer t(c, "nms_round");
-    launch_nms_round(s, dp<NmsBox>(P.n_sbox), alive, d_na, na, overlap,
+    launch_nms_round(s, dp<NmsBox>(P.n_sbox), alive, d_na, bound, overlap,
                      dp<unsigned long long>(P.n_mask), dp<int>(P.n_status), dp<int>(P.n_fsup),
                      dp<int>(P.n_picks), dp<int>(P.n_counters), dp<int>(P.n_keep), maps);
-    const int W = std::min(na, kNmsWindow);
-    const int rest = na - W;
-    if (rest <= 0) break;
+  };
+  int na = n;  // exact alive count of the round last enqueued
+  int slot = 0;
+  round(na);
+  for (;;) {
+    const int rest = na - std::min(na, kNmsWindow);  // boxes after that round's window
+    if (rest <= 0) break;                           // it held every alive box
     launch_compact(s, dp<int>(P.n_keep), rest, dp<int>(P.d_blocksum), d_sel_total,
                    dp<int>(P.n_sel));
-    int nsel = 0;
-    HIPCHK(hipMemcpyAsync(&nsel, d_sel_total, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    launch_nms_regather(s, alive, d_na, dp<int>(P.n_sel), nsel, alive_next);
+    launch_nms_regather(s, alive, d_na, dp<int>(P.n_sel), d_sel_total, rest, alive_next);
     HIPCHK(hipMemcpyAsync(d_na, d_sel_total, sizeof(int), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->h_count + slot, d_sel_total, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(c->ev_count[slot], s));
     std::swap(alive, alive_next);
-    na = nsel;
+    round(rest);                                    // next round, bounded by rest
+    HIPCHK(hipEventSynchronize(c->ev_count[slot]));
+    na = c->h_count[slot];                          // its exact alive count
+    slot ^= 1;
+    if (na == 0) break;
   }
   int npick = 0;
   HIPCHK(hipMemcpyAsync(&npick, P.n_counters.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1174,6 +1186,13 @@ int geye_init(int device, geye_ctx **ctx) {
   if (hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
       c->num_cu <= 0)
     c->num_cu = 256;
+  if (hipHostMalloc((void **)&c->h_count, 64, hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_count[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_count[1], hipEventDisableTiming) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return fail(GEYE_ERR_HIP, "pinned count buffer / events");
+  }
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete c;
@@ -1190,6 +1209,9 @@ void geye_shutdown(geye_ctx *c) {
   flush_stats(c);
   release_plans(c, nullptr, -1);
   for (auto &b : c->scratch_bufs) release(b);
+  if (c->h_count) (void)hipHostFree(c->h_count);
+  for (hipEvent_t e : c->ev_count)
+    if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
