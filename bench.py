@@ -169,6 +169,50 @@ def serving_mix(geye_amd, ctx, synth, slots, seconds: float = 2.0):
             "p99_ms": float(np.percentile(lat, 99)), "peak_working_bytes": st["peak_bytes"]}
 
 
+def probe(precision: str) -> int:
+    """Short run for `rocprofv3 --pmc`: two detections of the workload, then a plain copy of
+    256 MiB between device buffers (a kernel of known traffic for the counter scale)."""
+    import torch
+
+    import geye_amd
+    from geye_amd import synth
+
+    ctx = geye_amd.Context(0)
+    gm = ctx.model(synth.synthetic_car())
+    d_frame = ctx.dev_upload(synth.make_frame(W, H, seed=2))
+    for _ in range(2):
+        gm.detect_dev(d_frame, W, H, -0.5, 0.5, precision)
+    ctx.synchronize()
+    a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print("probe done: 2 frames +", a.numel() * 4, "byte copy", flush=True)
+    ctx.dev_free(d_frame)
+    gm.free()
+    ctx.close()
+    return 0
+
+
+def dt_traffic() -> dict | None:
+    """HBM bytes per DT launch from the committed counter summary, if present."""
+    path = os.path.join(ROOT, "profiles", "k_dt_traffic_1080p_synthetic_car.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            try:
+                vals[k.strip()] = float(v.split()[0])
+            except ValueError:
+                pass
+    if "bytes_per_launch" not in vals:
+        return None
+    return {"bytes_per_launch": vals["bytes_per_launch"],
+            "source": "profiles/k_dt_traffic_1080p_synthetic_car.txt"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,7 +223,12 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the two-model and mix legs")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--probe", action="store_true",
+                    help="counter run: 2 frames of the workload, then a 256 MiB device copy "
+                         "whose counters scale FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
+    if args.probe:
+        return probe(args.precision)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is "
                                    "described in DESIGN.md and not run", "n_gpus": args.gpus}))
@@ -247,8 +296,11 @@ def main() -> int:
         launches_per_frame = dt_launches / single["steps"]
         bytes_per_launch = 16 * part_cells / launches_per_frame
         mean_s = dt_ms / dt_launches / 1e3
+        tr = dt_traffic()
         roof = {"bound": "hbm", "achieved": bytes_per_launch / mean_s / 1e9, "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": bytes_per_launch / mean_s / HBM_PEAK, "traffic": None,
+                "unit": "GB/s", "frac": bytes_per_launch / mean_s / HBM_PEAK,
+                "traffic": tr["bytes_per_launch"] if tr else None,
+                "traffic_source": tr["source"] if tr else None,
                 "kernel": "k_dt (dt_cols + dt_rows launches)",
                 "algorithmic_bytes_per_launch": int(bytes_per_launch),
                 "per_unit": "16 B per part score cell (read score, write M, Ix, Iy)",

@@ -564,6 +564,18 @@ __device__ inline void dt_scan(const float *ls, float a, float b, int d, int p0,
   }
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin (block b on XCD b % 8), each XCD with its
+// own L2.  This hands every XCD runs of C consecutive jobs -- neighbouring jobs share cache
+// lines at their edges, so those lines meet in one L2 -- while successive runs still rotate
+// over the XCDs, which keeps the heavy and light parts of the job list spread across them.
+// Blocks past the last whole group of 8 * C keep their own index.
+__device__ inline int xcd_local_block(int b, int nb) {
+  constexpr int X = 8, C = 16, GRP = X * C;
+  if (b >= (nb / GRP) * GRP) return b;
+  const int g = b / GRP, l = b - g * GRP;
+  return g * GRP + (l % X) * C + l / X;
+}
+
 __global__ __launch_bounds__(256) void k_dt(const DtJob *__restrict__ jobs,
                                             const int4 *__restrict__ nodes,
                                             const int *__restrict__ depths,
@@ -571,7 +583,7 @@ __global__ __launch_bounds__(256) void k_dt(const DtJob *__restrict__ jobs,
                                             float *__restrict__ dst_arena,
                                             int *__restrict__ ptr_arena) {
   extern __shared__ float smem[];
-  const DtJob J = jobs[blockIdx.x];
+  const DtJob J = jobs[xcd_local_block(blockIdx.x, gridDim.x)];
   const int n = J.n, G = J.G, NLn = 256 / G;
   const int tid = threadIdx.x;
   const int line = tid / G, g = tid - line * G;

@@ -638,6 +638,9 @@ int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, int hk
 
 int read_count(geye_ctx *c, Plan &P, int *ncand);
 
+std::mutex g_tune_mu;
+std::map<std::tuple<const Model *, long long, int>, int> g_tuned;  // autotuned MFMA schedules
+
 // MFMA convolution schedule `choice`: 0 = one launch per filter class (4 x 64 tiles), else
 // the work-queue launch over every class with 16 * MT row tiles and `bpc` workgroups per CU.
 int run_mfma(geye_ctx *c, Model &m, Plan &P, int precision, int choice, bool timed) {
@@ -728,7 +731,20 @@ int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int 
                       dp<long long>(P.d_map_off), dp<int2>(P.d_map_dims), dp<float>(P.d_conv));
     }
   } else if (precision == GEYE_BF16 || precision == GEYE_FP16) {
-    if (P.mfma_choice[precision] < 0) CHK(tune_mfma(c, m, P, precision));
+    if (P.mfma_choice[precision] < 0) {
+      // a schedule tuned for the same model and frame geometry on another context is reused
+      const auto key = std::make_tuple((const Model *)&m, plan_key(P.W, P.H, P.max_scale), precision);
+      {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) P.mfma_choice[precision] = it->second;
+      }
+      if (P.mfma_choice[precision] < 0) {
+        CHK(tune_mfma(c, m, P, precision));
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[key] = P.mfma_choice[precision];
+      }
+    }
     CHK(run_mfma(c, m, P, precision, P.mfma_choice[precision], true));
   } else {
     return fail(GEYE_ERR_ARG, "unknown precision");
@@ -1251,6 +1267,11 @@ void geye_model_free(geye_model *gm) {
   if (!gm) return;
   Model &m = gm->m;
   release_plans(nullptr, &m, -1);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (auto it = g_tuned.begin(); it != g_tuned.end();)
+      it = std::get<0>(it->first) == &m ? g_tuned.erase(it) : std::next(it);
+  }
   release(m.d_filters);
   release(m.d_rsize);
   release(m.d_filt_bf16);
