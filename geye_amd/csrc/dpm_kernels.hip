@@ -16,6 +16,19 @@
 
 namespace geye {
 
+// Workgroups are dealt to the 8 XCDs round-robin (block b on XCD b % 8), each XCD with its
+// own L2.  This hands every XCD runs of C consecutive jobs -- neighbouring jobs share cache
+// lines at their edges, so those lines meet in one L2 -- while successive runs still rotate
+// over the XCDs, which keeps the heavy and light parts of the job list spread across them.
+// Blocks past the last whole group of 8 * C keep their own index.
+__device__ inline int xcd_local_block(int b, int nb) {
+  constexpr int X = 8, C = 16, GRP = X * C;
+  if (b >= (nb / GRP) * GRP) return b;
+  const int g = b / GRP, l = b - g * GRP;
+  return g * GRP + (l % X) * C + l / X;
+}
+
+
 // ---------------------------------------------------------------------------------------
 // Ingest: BGR interleaved uint8 -> planar float [3][H][W] (row-major).  Plane 0 is B, as in
 // Ipl_to_FLOAT (CT/featurepyramid.cpp.new2:816-850); the conversion is exact.
@@ -100,10 +113,11 @@ __global__ __launch_bounds__(256) void k_resize_fused(const FusedResize *__restr
   __shared__ int s_rn[kResizeTY], s_cn[kResizeTX];
   __shared__ int s_rsi[kResizeTY][MAXTAP], s_csi[kResizeTX][MAXTAP];
   __shared__ float s_ral[kResizeTY][MAXTAP], s_cal[kResizeTX][MAXTAP];
+  const int bx = xcd_local_block(blockIdx.x, gridDim.x);
   int k = 0;
-  while (k + 1 < nops && ops[k + 1].blocks_begin <= (int)blockIdx.x) k++;
+  while (k + 1 < nops && ops[k + 1].blocks_begin <= bx) k++;
   const FusedResize op = ops[k];
-  const int tile = blockIdx.x - op.blocks_begin;
+  const int tile = bx - op.blocks_begin;
   const int tyi = tile / op.tiles_x, txi = tile - tyi * op.tiles_x;
   const int dy0 = tyi * kResizeTY, dx0 = txi * kResizeTX;
   const int ny = min(kResizeTY, op.out_h - dy0), nx = min(kResizeTX, op.out_w - dx0);
@@ -239,10 +253,11 @@ __global__ __launch_bounds__(256) void k_hog_grad(const GradImg *__restrict__ im
                                                   int2 *__restrict__ grid_arena) {
   constexpr int TXP = 64, TYP = 16, SX = TXP + 2, SY = TYP + 2;
   __shared__ float s_t[3][SY * SX];
+  const int bx = xcd_local_block(blockIdx.x, gridDim.x);
   int ii = 0;
-  while (ii + 1 < nimg && imgs[ii + 1].blocks_begin <= (int)blockIdx.x) ii++;
+  while (ii + 1 < nimg && imgs[ii + 1].blocks_begin <= bx) ii++;
   const GradImg G = imgs[ii];
-  const int tile = blockIdx.x - G.blocks_begin;
+  const int tile = bx - G.blocks_begin;
   const int tx = tile / G.tiles_y, ty = tile - tx * G.tiles_y;
   const int x0 = tx * TXP, y0 = ty * TYP;
   const int h = G.h, w = G.w;
@@ -279,10 +294,11 @@ __global__ __launch_bounds__(256) void k_hog_votes(const HogLevel *__restrict__ 
                                                    float *__restrict__ norm_arena) {
   constexpr int T = kHogT, JM = 16;  // 2 * sbin <= 16 rows and columns per cell
   __shared__ float s_acc[18 * T * T];
+  const int bx = xcd_local_block(blockIdx.x, gridDim.x);
   int li = 0;
-  while (li + 1 < nlv && lv[li + 1].blocks_begin <= (int)blockIdx.x) li++;
+  while (li + 1 < nlv && lv[li + 1].blocks_begin <= bx) li++;
   const HogLevel L = lv[li];
-  const int tile = blockIdx.x - L.blocks_begin;
+  const int tile = bx - L.blocks_begin;
   const int tcx = tile / L.tiles_y, tcy = tile - tcx * L.tiles_y;
   const int tid = threadIdx.x;
   for (int i = tid; i < 18 * T * T; i += T * T) s_acc[i] = 0.0f;
@@ -564,17 +580,6 @@ __device__ inline void dt_scan(const float *ls, float a, float b, int d, int p0,
   }
 }
 
-// Workgroups are dealt to the 8 XCDs round-robin (block b on XCD b % 8), each XCD with its
-// own L2.  This hands every XCD runs of C consecutive jobs -- neighbouring jobs share cache
-// lines at their edges, so those lines meet in one L2 -- while successive runs still rotate
-// over the XCDs, which keeps the heavy and light parts of the job list spread across them.
-// Blocks past the last whole group of 8 * C keep their own index.
-__device__ inline int xcd_local_block(int b, int nb) {
-  constexpr int X = 8, C = 16, GRP = X * C;
-  if (b >= (nb / GRP) * GRP) return b;
-  const int g = b / GRP, l = b - g * GRP;
-  return g * GRP + (l % X) * C + l / X;
-}
 
 __global__ __launch_bounds__(256) void k_dt(const DtJob *__restrict__ jobs,
                                             const int4 *__restrict__ nodes,

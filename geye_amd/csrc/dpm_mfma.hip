@@ -39,7 +39,14 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const MfmaTile *__restrict__ 
                                                    const int2 *__restrict__ map_dims,
                                                    float *__restrict__ out) {
   extern __shared__ u32x4 lds[];
-  const MfmaTile t = tiles[blockIdx.x];
+  // runs of 16 consecutive tiles (neighbouring columns of one level) per XCD
+  constexpr int X = 8, C = 16, GRP = X * C;
+  int bx = blockIdx.x;
+  if (bx < ((int)gridDim.x / GRP) * GRP) {
+    const int g = bx / GRP, l = bx - g * GRP;
+    bx = g * GRP + (l % X) * C + l / X;
+  }
+  const MfmaTile t = tiles[bx];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int AX = 4 + fw - 1, AY = 64 + fh - 1;
   u32x4 *As = lds;                               // [AX][AY][4 chunks]
