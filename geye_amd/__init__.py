@@ -107,6 +107,8 @@ _SIGS = [
                              C.c_int, _fp]),
     ("geye_dt", C.c_int, [C.c_void_p, _fp, C.c_float, C.c_float, C.c_float, C.c_float, _ip, _fp,
                           _ip, _ip]),
+    ("geye_dt_window", C.c_int, [C.c_void_p, _fp, C.c_float, C.c_float, C.c_float, C.c_float, _ip,
+                                 _fp, _ip, _ip]),
     ("geye_nms", C.c_int, [C.c_void_p, _fp, C.c_int, C.c_int, C.c_float, _fp, _ip]),
     ("geye_dev_alloc", C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     ("geye_dev_free", C.c_int, [C.c_void_p, C.c_void_p]),
@@ -389,16 +391,18 @@ class Context:
                                 PRECISIONS[precision], _p(out, _fp)))
         return out
 
-    def dt(self, vals: np.ndarray, ax, bx, ay, by):
-        """vals [dims1][dims0] -> (M, Ix, Iy) in the same layout."""
+    def dt(self, vals: np.ndarray, ax, bx, ay, by, window: bool = False):
+        """vals [dims1][dims0] -> (M, Ix, Iy) in the same layout.  window=True runs the
+        16-bit forms' windowed minimum (geye_dt_window) instead of the reference's order."""
         d1, d0 = vals.shape
         v = np.ascontiguousarray(vals, dtype=np.float32)
         dims = np.array([d0, d1], np.int32)
         M = np.zeros_like(v)
         Ix = np.zeros((d1, d0), np.int32)
         Iy = np.zeros((d1, d0), np.int32)
-        _check(lib().geye_dt(self.h, _p(v, _fp), C.c_float(ax), C.c_float(bx), C.c_float(ay),
-                             C.c_float(by), _p(dims, _ip), _p(M, _fp), _p(Ix, _ip), _p(Iy, _ip)))
+        fn = lib().geye_dt_window if window else lib().geye_dt
+        _check(fn(self.h, _p(v, _fp), C.c_float(ax), C.c_float(bx), C.c_float(ay),
+                  C.c_float(by), _p(dims, _ip), _p(M, _fp), _p(Ix, _ip), _p(Iy, _ip)))
         return M, Ix, Iy
 
     def add_part_calculation(self, score: np.ndarray, M: np.ndarray, ax: int, ay: int) -> np.ndarray:

@@ -86,6 +86,25 @@ struct MfmaTile {
   int cost;                    // relative work, for ordering the queue
 };
 
+// Halo on each side of an LDS line in the windowed DT: the widest window half-width.
+constexpr int kDtHalo = 24;
+// dynamic LDS limit of the windowed DT kernel (160 KB less its static reduction buffer)
+constexpr int kDtWindowMaxLds = 160 * 1024 - 256;
+
+// LDS words per key line of the windowed DT (the line and both halos), padded so that the
+// row pass's lanes, lines fastest, fall on distinct banks of the 32 that serve each half
+// wave: with NL <= 32 lines the stride is 32 / NL modulo 32, otherwise odd.
+__host__ __device__ inline int dt_key_stride(int n, int NL) {
+  int w = n + 2 * kDtHalo;
+  if (NL <= 32) {
+    const int r = 32 / NL;
+    w += ((r - w) % 32 + 32) % 32;
+  } else {
+    w |= 1;
+  }
+  return w;
+}
+
 // One DT job: a block of lines (columns or rows) of one map.
 struct DtJob {
   long long src_off, dst_off;  // map base offsets (part score arena / DT arenas)
@@ -187,11 +206,13 @@ struct Plan {
   std::vector<long long> part_slot;    // [L][NoP] -> slot (or -1)
   // DT
   std::vector<DtJob> dt_pass1, dt_pass2;
+  std::vector<DtJob> dt_pass1w, dt_pass2w;  // the windowed DT's jobs (lines per job fit LDS)
   std::vector<int4> dt_nodes;          // (d, lo, hi, 0) per node, per length
   std::vector<int> dt_depths;          // node index where each depth starts (+ end)
   std::unordered_map<int, std::pair<int, int>> dt_len_tab;  // n -> (node_off, depth_off)
   long long dt_total = 0;              // cells per DT arena
   size_t dt_lds1 = 0, dt_lds2 = 0;     // dynamic LDS of the two DT launches
+  size_t dt_lds1w = 0, dt_lds2w = 0;   // the same for the windowed DT of the 16-bit forms
   // scoring
   std::vector<ScoreMap> smaps;
   std::vector<ScorePart> sparts;
@@ -200,7 +221,7 @@ struct Plan {
   std::vector<int> ascore_tab;         // per valid level: Xn[W] then Yn[H]
   // device copies of the tables
   DeviceBuffer d_resize_ops, d_resize_tab, d_resize_si, d_resize_alpha, d_hog, d_conv_tiles,
-      d_mfma_tiles, d_map_dims, d_map_off, d_dt1, d_dt2, d_nodes, d_depths, d_smaps, d_sparts,
+      d_mfma_tiles, d_map_dims, d_map_off, d_dt1, d_dt2, d_dt1w, d_dt2w, d_nodes, d_depths, d_smaps, d_sparts,
       d_ascore_tab, d_grad, d_grid, d_qtiles0, d_qtiles1, d_qcounter, d_fused;
   // device arenas
   DeviceBuffer d_frame, d_img, d_tmp, d_hist, d_norm, d_feat, d_featb, d_conv, d_tmpM,

@@ -28,6 +28,9 @@ void launch_conv_f32(hipStream_t s, int nf, const ConvTile *tiles, int ntiles, i
                      const int2 *map_dims, float *out);
 void launch_dt(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, const int4 *nodes,
                const int *depths, const float *src, float *dst, int *ptr);
+// the 16-bit forms' DT: windowed minimum, same outputs up to near-ties of the cost
+void launch_dt_window(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, const int4 *nodes,
+                      const int *depths, const float *src, float *dst, int *ptr);
 void launch_score(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,
                   const float *conv, const float *M, float thresh, float *score, int *flags,
                   long long total);

@@ -581,48 +581,65 @@ __device__ inline void dt_scan(const float *ls, float a, float b, int d, int p0,
 }
 
 
-__global__ __launch_bounds__(256) void k_dt(const DtJob *__restrict__ jobs,
-                                            const int4 *__restrict__ nodes,
-                                            const int *__restrict__ depths,
-                                            const float *__restrict__ src_arena,
-                                            float *__restrict__ dst_arena,
-                                            int *__restrict__ ptr_arena) {
-  extern __shared__ float smem[];
-  const DtJob J = jobs[xcd_local_block(blockIdx.x, gridDim.x)];
-  const int n = J.n, G = J.G, NLn = 256 / G;
+// Stages the job's lines into LDS, line l at s_src[l * LS + H0 + p], negated when the job
+// says so.  MINMAX also returns this thread's minimum and maximum of the staged values.
+template <bool MINMAX>
+__device__ inline void dt_stage(const DtJob &J, const float *__restrict__ src_arena, float *s_src,
+                                int LS, int H0, float &vmin, float &vmax) {
+  const int n = J.n, tid = threadIdx.x;
+  const float *src = src_arena + J.src_off + (long long)J.line0 * J.line_stride;
+  // element i of the job is (line l, position p), p fastest for lines contiguous in memory
+  // and l fastest otherwise; (l, p) advance by 256 elements without a division
+  // U loads are issued before the first is used, so their latencies overlap
+  constexpr int U = 8;
+  const bool cols = J.elem_stride == 1;
+  const int inner = cols ? n : J.nlines;
+  const int total = J.nlines * n;
+  const int sq = 256 / inner, so = 256 - sq * inner;
+  int q = tid / inner, o = tid - q * inner;
+  for (int i0 = tid; i0 < total; i0 += 256 * U) {
+    float v[U];
+    int at[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int l = cols ? q : o, p = cols ? o : q;
+      const bool ok = i0 + u * 256 < total;
+      at[u] = ok ? l * LS + H0 + p : -1;
+      v[u] = ok ? src[(long long)l * J.line_stride + (long long)p * J.elem_stride] : 0.0f;
+      q += sq;
+      o += so;
+      if (o >= inner) {
+        o -= inner;
+        q++;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (at[u] < 0) continue;
+      const float x = J.neg ? v[u] * -1.0f : v[u];
+      s_src[at[u]] = x;
+      if (MINMAX) {
+        vmin = fminf(vmin, x);
+        vmax = fmaxf(vmax, x);
+      }
+    }
+  }
+}
+
+// divide and conquer, one tree depth at a time: a node (d1, d2) takes d = (d1 + d2) / 2
+// and searches the sources between the argmins of its bounding ancestors d1 - 1 and
+// d2 + 1 (CT/dt.cpp:61-85).  The nodes of each depth come from the LDS copy of the
+// breadth-first node table of length n.
+__device__ inline void dt_divide_conquer(const DtJob &J, const int *__restrict__ depths,
+                                         const float *s_src, int LS, int H0, int *s_ptr,
+                                         const int2 *s_node) {
+  const int n = J.n, G = J.G;
   const int tid = threadIdx.x;
   const int line = tid / G, g = tid - line * G;
-  float *s_src = smem;                                  // [NLn][n+1]
-  int *s_ptr = (int *)(smem + NLn * (n + 1));           // [NLn][n]
-  int2 *s_node = (int2 *)(s_ptr + NLn * n + (NLn * n & 1));  // [n] (d | d1 << 16, d2)
-  for (int i = tid; i < n; i += 256) {
-    const int4 nd = nodes[J.node_off + i];
-    s_node[i] = make_int2(nd.x | ((nd.y + 1) << 16), nd.z - 1);
-  }
-  const float *src = src_arena + J.src_off + (long long)J.line0 * J.line_stride;
-  // stage the lines
-  if (J.elem_stride == 1) {
-    for (int i = tid; i < J.nlines * n; i += 256) {
-      int l = i / n, p = i - l * n;
-      float v = src[(long long)l * J.line_stride + p];
-      s_src[l * (n + 1) + p] = J.neg ? v * -1.0f : v;
-    }
-  } else {
-    for (int i = tid; i < J.nlines * n; i += 256) {
-      int p = i / J.nlines, l = i - p * J.nlines;
-      float v = src[(long long)l * J.line_stride + (long long)p * J.elem_stride];
-      s_src[l * (n + 1) + p] = J.neg ? v * -1.0f : v;
-    }
-  }
-  __syncthreads();
   const bool active_line = line < J.nlines;
-  const float *ls = s_src + line * (n + 1);
+  const float *ls = s_src + line * LS + H0;
   int *lp = s_ptr + line * n;
   const float a = J.a, b = J.b;
-  // divide and conquer, one tree depth at a time: a node (d1, d2) takes d = (d1 + d2) / 2
-  // and searches the sources between the argmins of its bounding ancestors d1 - 1 and
-  // d2 + 1 (CT/dt.cpp:61-85).  The nodes of each depth come from the LDS copy of the
-  // breadth-first node table of length n.
   for (int k = 0; k < J.depth_count; k++) {
     const int nb = depths[J.depth_off + k], ne = depths[J.depth_off + k + 1];
     const int nk = ne - nb;
@@ -674,8 +691,24 @@ __global__ __launch_bounds__(256) void k_dt(const DtJob *__restrict__ jobs,
     }
     __syncthreads();
   }
-  // write M and the argmin back with the staging pattern (coalesced along the memory
-  // order); M is the same cost expression at the chosen argmin, hence the same bits
+}
+
+__device__ inline void dt_load_nodes(const DtJob &J, const int4 *__restrict__ nodes,
+                                     int2 *s_node) {
+  for (int i = threadIdx.x; i < J.n; i += 256) {
+    const int4 nd = nodes[J.node_off + i];
+    s_node[i] = make_int2(nd.x | ((nd.y + 1) << 16), nd.z - 1);
+  }
+}
+
+// writes M and the argmin back with the staging pattern (coalesced along the memory
+// order); M is the reference's cost expression at the chosen argmin, hence the same bits
+// as the sequential scan whenever the argmin is the same
+__device__ inline void dt_write(const DtJob &J, const float *s_src, int LS, int H0,
+                                const int *s_ptr, float *__restrict__ dst_arena,
+                                int *__restrict__ ptr_arena) {
+  const int n = J.n, tid = threadIdx.x;
+  const float a = J.a, b = J.b;
   float *dst = dst_arena + J.dst_off + (long long)J.line0 * J.line_stride;
   int *pdst = ptr_arena + J.dst_off + (long long)J.line0 * J.line_stride;
   if (J.elem_stride == 1) {
@@ -683,7 +716,7 @@ __global__ __launch_bounds__(256) void k_dt(const DtJob *__restrict__ jobs,
       const int l = i / n, p = i - l * n;
       const int sp = s_ptr[l * n + p];
       const long long o = (long long)l * J.line_stride + p;
-      dst[o] = dt_cost(s_src[l * (n + 1) + sp], a, b, p - sp);
+      dst[o] = dt_cost(s_src[l * LS + H0 + sp], a, b, p - sp);
       pdst[o] = sp;
     }
   } else {
@@ -691,10 +724,196 @@ __global__ __launch_bounds__(256) void k_dt(const DtJob *__restrict__ jobs,
       const int p = i / J.nlines, l = i - p * J.nlines;
       const int sp = s_ptr[l * n + p];
       const long long o = (long long)l * J.line_stride + (long long)p * J.elem_stride;
-      dst[o] = dt_cost(s_src[l * (n + 1) + sp], a, b, p - sp);
+      dst[o] = dt_cost(s_src[l * LS + H0 + sp], a, b, p - sp);
       pdst[o] = sp;
     }
   }
+}
+
+// The reference's DT: a group of G lanes per line, LDS lines of n + 1 floats.
+__global__ __launch_bounds__(256) void k_dt(const DtJob *__restrict__ jobs,
+                                            const int4 *__restrict__ nodes,
+                                            const int *__restrict__ depths,
+                                            const float *__restrict__ src_arena,
+                                            float *__restrict__ dst_arena,
+                                            int *__restrict__ ptr_arena) {
+  extern __shared__ float smem[];
+  const DtJob J = jobs[xcd_local_block(blockIdx.x, gridDim.x)];
+  const int n = J.n, NLn = 256 / J.G;
+  float *s_src = smem;                                  // [NLn][n+1]
+  int *s_ptr = (int *)(smem + NLn * (n + 1));           // [NLn][n]
+  int2 *s_node = (int2 *)(s_ptr + NLn * n + (NLn * n & 1));  // [n] (d | d1 << 16, d2)
+  dt_load_nodes(J, nodes, s_node);
+  float unused0 = 0.0f, unused1 = 0.0f;
+  dt_stage<false>(J, src_arena, s_src, n + 1, 0, unused0, unused1);
+  __syncthreads();
+  dt_divide_conquer(J, depths, s_src, n + 1, 0, s_ptr, s_node);
+  dt_write(J, s_src, n + 1, 0, s_ptr, dst_arena, ptr_arena);
+}
+
+// ---------------------------------------------------------------------------------------
+// Distance transform of the 16-bit forms: the same minimum, found by a bounded window.
+// For output d the argmin s satisfies f(s) + q(d - s) <= f(d) with q(t) = a*t*t + b*t,
+// so q(d - s) <= R, the job's value range, and for a > 0 every argmin lies in
+// t = d - s in [t-, t+], the roots of a*t*t + b*t = R.  When that fits a window of
+// half-width T (8, 12, 16 or 24), each output takes the minimum over the 2T + 1 sources of
+// its window; otherwise, and for a <= 0, the job runs the divide and conquer above.
+//
+// The window search runs on 32-bit fixed-point keys: (f - fmin) and (q - qmin) are scaled
+// so that their sum stays below 2^30, shifted up 6 bits, and the window position j is
+// kept in the 6 low bits of the q term.  One integer add per candidate then gives a key
+// whose unsigned minimum is the minimal cost and its position; costs within one fixed-point
+// step (the job's cost span / 2^24) count as tied, so a near-tie need not pick the
+// reference's first index.  This form is only used where detections are matched by
+// overlap, not bit for bit.  M is the reference's cost expression at the chosen argmin,
+// evaluated from the float source.
+// ---------------------------------------------------------------------------------------
+template <int T, bool ROWS>
+__device__ inline void dt_window(const DtJob &J, const float *s_f, const unsigned *s_key,
+                                 int LSk, const unsigned long long *s_q,
+                                 float *__restrict__ dst_arena, int *__restrict__ ptr_arena) {
+  constexpr int NP = T + 1;  // 2T + 1 positions as T + 1 pairs (the last one padded)
+  const int n = J.n, nl = J.nlines;
+  const float a = J.a, b = J.b;
+  // q terms of positions (2i, 2i + 1) as one 64-bit word: one 64-bit add forms two keys,
+  // since no key reaches 2^32 there is no carry between the halves
+  unsigned long long q[NP];
+#pragma unroll
+  for (int i = 0; i < NP; i++) q[i] = s_q[i];
+  float *dst = dst_arena + J.dst_off + (long long)J.line0 * J.line_stride;
+  int *pdst = ptr_arena + J.dst_off + (long long)J.line0 * J.line_stride;
+  const int tid = threadIdx.x;
+  // ROWS (lines interleaved in memory): lanes take consecutive lines of one position, so the
+  // stores are contiguous; otherwise consecutive positions of one line
+  const int NL = 256 / J.G;
+  int l, d, dstep, lstep = 0;
+  if (ROWS) {
+    l = tid & (NL - 1);
+    d = tid / NL;
+    dstep = 256 / NL;
+  } else {
+    l = tid / n;
+    d = tid - l * n;
+    lstep = 256 / n;
+    dstep = 256 - lstep * n;
+  }
+  while (ROWS ? d < n : l < nl) {
+    if (!ROWS || l < nl) {
+      const unsigned *w = s_key + l * LSk + kDtHalo - T + d;
+      unsigned best = 0xffffffffu;
+#pragma unroll
+      for (int i = 0; i < NP; i++) {
+        const unsigned hi = 2 * i + 1 < 2 * T + 1 ? w[2 * i + 1] : 0u;
+        const unsigned long long k = (((unsigned long long)hi << 32) | w[2 * i]) + q[i];
+        best = min(best, min((unsigned)k, (unsigned)(k >> 32)));
+      }
+      const int s = d - T + (int)(best & 63u);
+      const float M = dt_cost(s_f[l * n + s], a, b, d - s);
+      const long long o = (long long)l * J.line_stride + (long long)d * J.elem_stride;
+      dst[o] = M;
+      pdst[o] = s;
+    }
+    d += dstep;
+    if (!ROWS) {
+      l += lstep;
+      if (d >= n) {
+        d -= n;
+        l++;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_dt_window(const DtJob *__restrict__ jobs,
+                                                   const int4 *__restrict__ nodes,
+                                                   const int *__restrict__ depths,
+                                                   const float *__restrict__ src_arena,
+                                                   float *__restrict__ dst_arena,
+                                                   int *__restrict__ ptr_arena) {
+  extern __shared__ float smem[];
+  __shared__ float s_red[2][4];
+  __shared__ unsigned long long s_q[kDtHalo + 1];
+  const DtJob J = jobs[xcd_local_block(blockIdx.x, gridDim.x)];
+  const int n = J.n, NLn = 256 / J.G, LSk = dt_key_stride(n, NLn);
+  const int tid = threadIdx.x;
+  float *s_f = smem;                                            // [NLn][n] source
+  unsigned *s_key = (unsigned *)(smem + NLn * n);               // [NLn][LSk] keys
+  int2 *s_node = (int2 *)(s_key + NLn * LSk + ((NLn * (n + LSk)) & 1));  // [n]
+  float vmin = __builtin_inff(), vmax = -__builtin_inff();
+  dt_stage<true>(J, src_arena, s_f, n, 0, vmin, vmax);
+  for (int m = 1; m < 64; m <<= 1) {
+    vmin = fminf(vmin, __shfl_xor(vmin, m));
+    vmax = fmaxf(vmax, __shfl_xor(vmax, m));
+  }
+  if ((tid & 63) == 0) {
+    s_red[0][tid >> 6] = vmin;
+    s_red[1][tid >> 6] = vmax;
+  }
+  __syncthreads();
+  vmin = fminf(fminf(s_red[0][0], s_red[0][1]), fminf(s_red[0][2], s_red[0][3]));
+  vmax = fmaxf(fmaxf(s_red[1][0], s_red[1][1]), fmaxf(s_red[1][2], s_red[1][3]));
+  // half-width of the argmin band, with a margin for the rounding of the costs
+  int need = 1 << 20;
+  const float a = J.a, b = J.b;
+  const float R = (vmax - vmin) * 1.00001f + 1e-4f;
+  if (a > 0.0f && R < 1e30f) {
+    const float disc = sqrtf(b * b + 4.0f * a * R);
+    const float tp = (-b + disc) / (2.0f * a), tm = (-b - disc) / (2.0f * a);
+    const float h = fmaxf(tp, -tm);
+    if (h < 64.0f) need = (int)ceilf(h) + 1;
+  }
+  if (need > kDtHalo) {
+    // the reference's divide and conquer on the float lines; argmins go to the key area
+    int *s_ptr = (int *)s_key;
+    dt_load_nodes(J, nodes, s_node);
+    __syncthreads();
+    dt_divide_conquer(J, depths, s_f, n, 0, s_ptr, s_node);
+    dt_write(J, s_f, n, 0, s_ptr, dst_arena, ptr_arena);
+    return;
+  }
+  const int T = need <= 8 ? 8 : need <= 12 ? 12 : need <= 16 ? 16 : kDtHalo;
+  // fixed-point scale: (f - fmin) + (q - qmin) over the window stays below 2^24 steps
+  const float qmin = fminf(0.0f, -b * b / (4.0f * a));  // the parabola's vertex bounds q below
+  const float qmax = a * (float)T * (float)T + fabsf(b) * (float)T;
+  const float span = (vmax - vmin) + (qmax - qmin);
+  const float scale = span > 0.0f ? 16777216.0f / span : 1.0f;
+  // the q terms of the 2T + 1 window positions, one lane each, paired for the 64-bit adds:
+  // position j is source s = d - T + j, t = T - j; 6 low bits hold j
+  if (tid < 2 * T + 2) {
+    unsigned v = 0xffffffffu;  // pad position of the last pair (source 0): never the minimum
+    if (tid <= 2 * T) {
+      const float t = (float)(T - tid);
+      const float qv = fmaxf((a * t + b) * t - qmin, 0.0f);
+      v = ((unsigned)(qv * scale + 0.5f) << 6) | (unsigned)tid;
+    }
+    ((unsigned *)s_q)[tid] = v;
+  }
+  {
+    const int sl = 256 / LSk, sh = 256 - sl * LSk;
+    int l = tid / LSk, h = tid - l * LSk;
+    for (int i = tid; i < J.nlines * LSk; i += 256) {
+      const int p = h - kDtHalo;
+      unsigned k = 0x80000000u;  // halo: above every window cost, no carry out of 32 bits
+      if (p >= 0 && p < n) k = (unsigned)((s_f[l * n + p] - vmin) * scale + 0.5f) << 6;
+      s_key[i] = k;
+      l += sl;
+      h += sh;
+      if (h >= LSk) {
+        h -= LSk;
+        l++;
+      }
+    }
+  }
+  __syncthreads();
+  const bool rows = J.elem_stride != 1;
+#define GEYE_DT_WINDOW(TT)                                                              \
+  (rows ? dt_window<TT, true>(J, s_f, s_key, LSk, s_q, dst_arena, ptr_arena)            \
+        : dt_window<TT, false>(J, s_f, s_key, LSk, s_q, dst_arena, ptr_arena))
+  if (T == 8) GEYE_DT_WINDOW(8);
+  else if (T == 12) GEYE_DT_WINDOW(12);
+  else if (T == 16) GEYE_DT_WINDOW(16);
+  else GEYE_DT_WINDOW(kDtHalo);
+#undef GEYE_DT_WINDOW
 }
 
 // ---------------------------------------------------------------------------------------
@@ -927,6 +1146,17 @@ void launch_dt(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, const in
                                                160 * 1024) == hipSuccess;
   if (!attr && lds > 64 * 1024) return;  // the launch would fail; the caller checks the error
   hipLaunchKernelGGL(k_dt, dim3(njobs), dim3(256), lds, s, jobs, nodes, depths, src, dst, ptr);
+}
+
+void launch_dt_window(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, const int4 *nodes,
+                      const int *depths, const float *src, float *dst, int *ptr) {
+  // the kernel's own 32 bytes of static LDS count against the same 160 KB
+  static const bool attr = hipFuncSetAttribute((const void *)k_dt_window,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               kDtWindowMaxLds) == hipSuccess;
+  if (!attr && lds > 64 * 1024) return;  // the launch would fail; the caller checks the error
+  hipLaunchKernelGGL(k_dt_window, dim3(njobs), dim3(256), lds, s, jobs, nodes, depths, src, dst,
+                     ptr);
 }
 
 void launch_score(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,

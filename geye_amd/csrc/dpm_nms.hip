@@ -493,11 +493,18 @@ __global__ __launch_bounds__(256) void k_take(const int *__restrict__ src,
   if (k < n) dst[k] = src[idx[k]];
 }
 
+// rocPRIM picks a merge sort below about 1 M items (about 20 dependent launches for a
+// 1080p candidate set); onesweep is four digit passes over the same keys, so the merge
+// path is switched off.
+using NmsSortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                                 rocprim::default_config, 0>;
+
 size_t nms_sort_temp_bytes(int n) {
   size_t bytes = 0;
   unsigned int *k = nullptr;
   int *v = nullptr;
-  (void)rocprim::radix_sort_pairs(nullptr, bytes, k, k, v, v, (unsigned int)n, 0, 32, 0);
+  (void)rocprim::radix_sort_pairs<NmsSortConfig>(nullptr, bytes, k, k, v, v, (unsigned int)n, 0,
+                                                 32, 0);
   return bytes;
 }
 
@@ -517,8 +524,8 @@ void launch_nms_prep_rows(hipStream_t s, const float *rows, int n, int GL, NmsBo
 int launch_nms_sort(hipStream_t s, void *temp, size_t temp_bytes, unsigned int *keys_in,
                     unsigned int *keys_out, int *vals_in, int *vals_out, int n) {
   size_t bytes = temp_bytes;
-  return rocprim::radix_sort_pairs(temp, bytes, keys_in, keys_out, vals_in, vals_out,
-                                   (unsigned int)n, 0, 32, s) == hipSuccess
+  return rocprim::radix_sort_pairs<NmsSortConfig>(temp, bytes, keys_in, keys_out, vals_in,
+                                                  vals_out, (unsigned int)n, 0, 32, s) == hipSuccess
              ? 0
              : -1;
 }

@@ -156,20 +156,34 @@ static size_t dt_lds_bytes(int n, int G) {
   return (words + (words & 1)) * 4 + (size_t)n * 8;
 }
 
+// the windowed DT: float lines (n) and key lines (n plus halos, padded); node table
+static size_t dt_lds_bytes_window(int n, int G) {
+  const int NL = 256 / G;
+  const size_t words = (size_t)NL * (size_t)(n + dt_key_stride(n, NL));
+  return (words + (words & 1)) * 4 + (size_t)n * 8;
+}
+
 // lanes per line: the fewest whose block fits 48 KB of LDS (several blocks per CU), else a
 // whole wave per line with up to 160 KB
 static int dt_pick_G(int n) {
   for (int G = 1; G <= 64; G *= 2)
     if (dt_lds_bytes(n, G) <= 48 * 1024) return G;
-  return dt_lds_bytes(n, 64) <= 160 * 1024 ? 64 : -1;
+  return dt_lds_bytes_window(n, 64) <= (size_t)kDtWindowMaxLds ? 64 : -1;
+}
+
+// the same for the windowed kernel, whose lines carry halos
+static int dt_pick_G_window(int n) {
+  for (int G = 1; G <= 64; G *= 2)
+    if (dt_lds_bytes_window(n, G) <= 48 * 1024) return G;
+  return dt_lds_bytes_window(n, 64) <= (size_t)kDtWindowMaxLds ? 64 : -1;
 }
 
 static void add_dt_jobs(Plan &P, std::vector<DtJob> &jobs, long long src_off, long long dst_off,
                         int n, int nlines, int line_stride, int elem_stride, int neg, float a,
-                        float b) {
+                        float b, bool window = false) {
   dt_nodes_for(P, n);
   const auto tabs = P.dt_len_tab[n];
-  const int G = dt_pick_G(n);
+  const int G = window ? dt_pick_G_window(n) : dt_pick_G(n);
   const int NL = 256 / G;
   const int dc = dt_depth_count(P, n);
   for (int l0 = 0; l0 < nlines; l0 += NL) {
@@ -510,6 +524,8 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
   // ---- DT jobs and score maps
   P.dt_pass1.clear();
   P.dt_pass2.clear();
+  P.dt_pass1w.clear();
+  P.dt_pass2w.clear();
   P.dt_nodes.clear();
   P.dt_depths.clear();
   P.dt_len_tab.clear();
@@ -576,8 +592,11 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
         }
         // pass 1: columns (along y, n = P0) with (ay, by); source negated (inverse_Q)
         add_dt_jobs(P, P.dt_pass1, P.map_off[pslot], dto, P0, P1, P0, 1, 1, df[2], df[3]);
+        add_dt_jobs(P, P.dt_pass1w, P.map_off[pslot], dto, P0, P1, P0, 1, 1, df[2], df[3],
+                    true);
         // pass 2: rows (along x, n = P1) with (ax, bx)
         add_dt_jobs(P, P.dt_pass2, dto, dto, P1, P0, 1, P0, 0, df[0], df[1]);
+        add_dt_jobs(P, P.dt_pass2w, dto, dto, P1, P0, 1, P0, 0, df[0], df[1], true);
         P.sparts.push_back(q);
         dto += (long long)P0 * P1;
       }
@@ -588,8 +607,12 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
   P.score_total = so;
   P.dt_lds1 = 0;
   P.dt_lds2 = 0;
+  P.dt_lds1w = 0;
+  P.dt_lds2w = 0;
   for (auto &j : P.dt_pass1) P.dt_lds1 = std::max(P.dt_lds1, dt_lds_bytes(j.n, j.G));
   for (auto &j : P.dt_pass2) P.dt_lds2 = std::max(P.dt_lds2, dt_lds_bytes(j.n, j.G));
+  for (auto &j : P.dt_pass1w) P.dt_lds1w = std::max(P.dt_lds1w, dt_lds_bytes_window(j.n, j.G));
+  for (auto &j : P.dt_pass2w) P.dt_lds2w = std::max(P.dt_lds2w, dt_lds_bytes_window(j.n, j.G));
   P.GL = m.numcomponent > 0 ? (m.numpart[0] + 1) * 4 + 3 : 0;
   for (int jj = 1; jj < m.numcomponent; jj++)
     if (m.numpart[jj] != m.numpart[0]) {
@@ -601,17 +624,17 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
 
 int dt_single_jobs(int n0, int n1, float ax, float bx, float ay, float by,
                    std::vector<DtJob> &p1, std::vector<DtJob> &p2, std::vector<int4> &nodes,
-                   std::vector<int> &depths, size_t &lds1, size_t &lds2) {
+                   std::vector<int> &depths, size_t &lds1, size_t &lds2, bool window) {
   if (dt_pick_G(n0) < 0 || dt_pick_G(n1) < 0) return GEYE_ERR_UNSUPPORTED;
   Plan P;
-  add_dt_jobs(P, P.dt_pass1, 0, 0, n0, n1, n0, 1, 0, ay, by);
-  add_dt_jobs(P, P.dt_pass2, 0, 0, n1, n0, 1, n0, 0, ax, bx);
+  add_dt_jobs(P, P.dt_pass1, 0, 0, n0, n1, n0, 1, 0, ay, by, window);
+  add_dt_jobs(P, P.dt_pass2, 0, 0, n1, n0, 1, n0, 0, ax, bx, window);
   p1 = P.dt_pass1;
   p2 = P.dt_pass2;
   nodes = P.dt_nodes;
   depths = P.dt_depths;
-  lds1 = dt_lds_bytes(n0, dt_pick_G(n0));
-  lds2 = dt_lds_bytes(n1, dt_pick_G(n1));
+  lds1 = window ? dt_lds_bytes_window(n0, dt_pick_G_window(n0)) : dt_lds_bytes(n0, dt_pick_G(n0));
+  lds2 = window ? dt_lds_bytes_window(n1, dt_pick_G_window(n1)) : dt_lds_bytes(n1, dt_pick_G(n1));
   return GEYE_OK;
 }
 

@@ -451,6 +451,8 @@ int plan_upload(Plan &P, hipStream_t st) {
   CHK(upload(P.d_map_off, P.map_off, st));
   CHK(upload(P.d_dt1, P.dt_pass1, st));
   CHK(upload(P.d_dt2, P.dt_pass2, st));
+  CHK(upload(P.d_dt1w, P.dt_pass1w, st));
+  CHK(upload(P.d_dt2w, P.dt_pass2w, st));
   CHK(upload(P.d_nodes, P.dt_nodes, st));
   CHK(upload(P.d_depths, P.dt_depths, st));
   CHK(upload(P.d_smaps, P.smaps, st));
@@ -491,7 +493,7 @@ void plan_release(Plan &P) {
   P.graphs.clear();
   for (DeviceBuffer *b : {&P.d_resize_ops, &P.d_resize_tab, &P.d_resize_si, &P.d_resize_alpha,
                           &P.d_hog, &P.d_conv_tiles, &P.d_mfma_tiles, &P.d_map_dims,
-                          &P.d_map_off, &P.d_dt1, &P.d_dt2,
+                          &P.d_map_off, &P.d_dt1, &P.d_dt2, &P.d_dt1w, &P.d_dt2w,
                           &P.d_nodes, &P.d_depths, &P.d_smaps, &P.d_sparts, &P.d_frame,
                           &P.d_img, &P.d_tmp, &P.d_hist, &P.d_norm, &P.d_feat, &P.d_featb,
                           &P.d_conv, &P.d_tmpM, &P.d_tmpIy, &P.d_M, &P.d_tmpIx, &P.d_score,
@@ -749,15 +751,23 @@ int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int 
   } else {
     return fail(GEYE_ERR_ARG, "unknown precision");
   }
-  if (!P.dt_pass1.empty()) {
+  // FP32 runs the reference's divide and conquer (bit-exact argmax); the 16-bit forms,
+  // whose detections are matched by overlap, take the windowed minimum
+  const bool exact_dt = precision == GEYE_FP32;
+  auto *dt_launch = exact_dt ? launch_dt : launch_dt_window;
+  const auto &jobs1 = exact_dt ? P.dt_pass1 : P.dt_pass1w;
+  const auto &jobs2 = exact_dt ? P.dt_pass2 : P.dt_pass2w;
+  if (!jobs1.empty()) {
     KTimer t(c, "dt_cols");
-    launch_dt(s, dp<DtJob>(P.d_dt1), (int)P.dt_pass1.size(), P.dt_lds1, dp<int4>(P.d_nodes),
-              dp<int>(P.d_depths), dp<float>(P.d_conv), dp<float>(P.d_tmpM), dp<int>(P.d_tmpIy));
+    dt_launch(s, dp<DtJob>(exact_dt ? P.d_dt1 : P.d_dt1w), (int)jobs1.size(),
+              exact_dt ? P.dt_lds1 : P.dt_lds1w, dp<int4>(P.d_nodes), dp<int>(P.d_depths),
+              dp<float>(P.d_conv), dp<float>(P.d_tmpM), dp<int>(P.d_tmpIy));
   }
-  if (!P.dt_pass2.empty()) {
+  if (!jobs2.empty()) {
     KTimer t(c, "dt_rows");
-    launch_dt(s, dp<DtJob>(P.d_dt2), (int)P.dt_pass2.size(), P.dt_lds2, dp<int4>(P.d_nodes),
-              dp<int>(P.d_depths), dp<float>(P.d_tmpM), dp<float>(P.d_M), dp<int>(P.d_tmpIx));
+    dt_launch(s, dp<DtJob>(exact_dt ? P.d_dt2 : P.d_dt2w), (int)jobs2.size(),
+              exact_dt ? P.dt_lds2 : P.dt_lds2w, dp<int4>(P.d_nodes), dp<int>(P.d_depths),
+              dp<float>(P.d_tmpM), dp<float>(P.d_M), dp<int>(P.d_tmpIx));
   }
   if (P.score_total > 0) {
     {
@@ -1825,8 +1835,8 @@ int geye_fconv(geye_ctx *c, const float *feat, int F0, int F1, const float *filt
   return GEYE_OK;
 }
 
-int geye_dt(geye_ctx *c, const float *vals, float ax, float bx, float ay, float by,
-            const int *dims, float *M, int *Ix, int *Iy) {
+static int dt_standalone(geye_ctx *c, const float *vals, float ax, float bx, float ay, float by,
+                         const int *dims, float *M, int *Ix, int *Iy, bool window) {
   if (!c || !vals || !dims || !M || !Ix || !Iy) return fail(GEYE_ERR_ARG, "null argument");
   const int d0 = dims[0], d1 = dims[1];
   if (d0 < 1 || d1 < 1) return fail(GEYE_ERR_ARG, "dt: empty map");
@@ -1837,7 +1847,8 @@ int geye_dt(geye_ctx *c, const float *vals, float ax, float bx, float ay, float 
   std::vector<int4> nodes;
   std::vector<int> depths;
   size_t lds1 = 0, lds2 = 0;
-  if (dt_single_jobs(d0, d1, ax, bx, ay, by, p1, p2, nodes, depths, lds1, lds2) != GEYE_OK)
+  if (dt_single_jobs(d0, d1, ax, bx, ay, by, p1, p2, nodes, depths, lds1, lds2, window) !=
+      GEYE_OK)
     return fail(GEYE_ERR_UNSUPPORTED, "dt: map dimension too large");
   const long long SQ = (long long)d0 * d1;
   Scratch S;
@@ -1852,9 +1863,10 @@ int geye_dt(geye_ctx *c, const float *vals, float ax, float bx, float ay, float 
   CHK(upload(dj2, p2, c->stream));
   CHK(upload(dn, nodes, c->stream));
   CHK(upload(dd, depths, c->stream));
-  launch_dt(c->stream, dp<DtJob>(dj1), (int)p1.size(), lds1, dp<int4>(dn), dp<int>(dd),
+  auto *dt_launch = window ? launch_dt_window : launch_dt;
+  dt_launch(c->stream, dp<DtJob>(dj1), (int)p1.size(), lds1, dp<int4>(dn), dp<int>(dd),
             dp<float>(dv), dp<float>(dtm), dp<int>(diy));
-  launch_dt(c->stream, dp<DtJob>(dj2), (int)p2.size(), lds2, dp<int4>(dn), dp<int>(dd),
+  dt_launch(c->stream, dp<DtJob>(dj2), (int)p2.size(), lds2, dp<int4>(dn), dp<int>(dd),
             dp<float>(dtm), dp<float>(dm), dp<int>(dix));
   HIPCHK(hipGetLastError());
   std::vector<int> tIx(SQ), tIy(SQ);
@@ -1870,6 +1882,16 @@ int geye_dt(geye_ctx *c, const float *vals, float ax, float bx, float ay, float 
       Iy[k] = tIy[(long long)tIx[k] * d0 + y];
     }
   return GEYE_OK;
+}
+
+int geye_dt(geye_ctx *c, const float *vals, float ax, float bx, float ay, float by,
+            const int *dims, float *M, int *Ix, int *Iy) {
+  return dt_standalone(c, vals, ax, bx, ay, by, dims, M, Ix, Iy, false);
+}
+
+int geye_dt_window(geye_ctx *c, const float *vals, float ax, float bx, float ay, float by,
+                   const int *dims, float *M, int *Ix, int *Iy) {
+  return dt_standalone(c, vals, ax, bx, ay, by, dims, M, Ix, Iy, true);
 }
 
 int geye_dev_alloc(geye_ctx *c, size_t bytes, void **ptr) {

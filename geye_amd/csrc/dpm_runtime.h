@@ -16,10 +16,10 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
 void resize_tables(int sheight, int dheight, std::vector<int2> &tab, std::vector<int> &si,
                    std::vector<float> &alpha);
 // DT jobs for one standalone map (dims0 = n0, dims1 = n1): pass 1 over columns of a
-// negation-free source, pass 2 over rows.
+// negation-free source, pass 2 over rows; lds1/lds2 sized for the windowed kernel if window.
 int dt_single_jobs(int n0, int n1, float ax, float bx, float ay, float by,
                    std::vector<DtJob> &p1, std::vector<DtJob> &p2, std::vector<int4> &nodes,
-                   std::vector<int> &depths, size_t &lds1, size_t &lds2);
+                   std::vector<int> &depths, size_t &lds1, size_t &lds2, bool window = false);
 
 // sets the calling thread's geye_last_error() message; returns code
 int set_error(int code, const std::string &msg);

@@ -161,6 +161,15 @@ int geye_fconv(geye_ctx *ctx, const float *feat, int F0, int F1, const float *fi
                int h, int w, int precision, float *out);
 int geye_dt(geye_ctx *ctx, const float *vals, float ax, float bx, float ay, float by,
             const int *dims, float *M, int *Ix, int *Iy);
+/* The DT of the 16-bit forms (GEYE_BF16, GEYE_FP16), same arguments as geye_dt
+ * (CT/dt.cpp:101).  The minimum over each line is searched in a window bounded by the
+ * line's value range, which contains every argmin when ax, ay > 0; otherwise, or when the
+ * window would pass 24, it is geye_dt's divide and conquer.  M is the reference's cost
+ * expression at the chosen argmin; costs closer than 2^-24 of the line block's cost span
+ * are treated as tied, so at such near-ties the argmin may differ from dt()'s first
+ * minimum. */
+int geye_dt_window(geye_ctx *ctx, const float *vals, float ax, float bx, float ay, float by,
+                   const int *dims, float *M, int *Ix, int *Iy);
 /* add_part_calculation (CT/dt.cpp:38-55): score[y + x*rootsize[0]] -=
  * M[(ay-1+2y) + (ax-1+2x)*partsize[0]] over the root map, in place; rejects anchors that
  * would read outside M (the reference does not check). */

@@ -112,6 +112,57 @@ def test_dt_matches_reference_vectors(ctx):
         assert bits_equal(M, z[f"M{k}"]) and np.array_equal(Ix, z[f"Ix{k}"]) and np.array_equal(Iy, z[f"Iy{k}"])
 
 
+def _dt_cost(src, a, b, t):
+    """The reference's cost expression ((src + (a*t)*t) + b*t) in float32 (CT/dt.cpp:71-78)."""
+    a, b = np.float32(a), np.float32(b)
+    t = t.astype(np.float32)
+    return ((src + (a * t) * t) + b * t).astype(np.float32)
+
+
+@pytest.mark.parametrize("d0,d1,defs,scale,exact", [
+    (23, 17, (0.01, 0.0, 0.01, 0.0), 0.15, False),
+    (41, 64, (0.023062, 0.010890, 0.015328, -0.009441), 0.3, False),
+    (263, 40, (0.01, 0.0, 0.01, 0.0), 0.1, False),
+    (30, 473, (0.05, 0.02, 0.03, -0.01), 0.5, False),
+    (1, 9, (0.1, 0.0, 0.1, 0.0), 0.2, False),
+    (9, 1, (0.1, 0.0, 0.1, 0.0), 0.2, False),
+    (130, 70, (0.2, 0.1, 0.3, -0.2), 0.4, False),
+    (5, 6, (0.01, 0.0, 0.01, 0.0), 0.2, False),
+    (64, 64, (0.01, 0.0, 0.01, 0.0), 100.0, True),   # argmin band wider than 24
+    (50, 40, (0.0, 0.0, 0.0, 0.0), 1.0, True),       # a = 0: no band
+    (37, 29, (-0.01, 0.0, -0.02, 0.0), 1.0, True),   # a < 0: no band
+])
+def test_dt_window_minimum(ctx, d0, d1, defs, scale, exact):
+    """DT of the 16-bit forms (geye_dt_window) against the oracle's dt():
+    - M is the reference's cost at the returned (Ix, Iy), bit for bit;
+    - M is the minimum: within 2^-14 * (1 + range) of the FP64 oracle's minimum;
+    - argmins equal dt()'s except at near-ties (>= 98% of cells on random data);
+    - where no window can hold the argmin band (range too wide, a <= 0) it is dt()'s divide
+      and conquer, bit for bit."""
+    say("dt window", d0, d1, defs, scale)
+    rng = np.random.default_rng(d0 * 7919 + d1)
+    v = (rng.standard_normal((d1, d0)) * scale).astype(np.float32)
+    M, Ix, Iy = ctx.dt(v, *defs, window=True)
+    oM, oIx, oIy = O.dt(v, *defs)
+    if exact:
+        assert bits_equal(M, oM) and np.array_equal(Ix, oIx) and np.array_equal(Iy, oIy)
+        return
+    ax, bx, ay, by = defs
+    assert Ix.min() >= 0 and Ix.max() < d1 and Iy.min() >= 0 and Iy.max() < d0
+    xs = np.arange(d1)[:, None]
+    ys = np.arange(d0)[None, :]
+    inner = _dt_cost(v[Ix, Iy], ay, by, ys - Iy)
+    assert bits_equal(M, _dt_cost(inner, ax, bx, xs - Ix))
+    M64, _, _ = O.dt(v, *defs, f64=True)
+    tol = 2.0 ** -14 * (1.0 + float(v.max() - v.min()))
+    err = np.abs(M.astype(np.float64) - M64)
+    say("max |M - M64|", float(err.max()), "tol", tol)
+    assert err.max() <= tol
+    agree = np.mean((Ix == oIx) & (Iy == oIy))
+    say("argmin agreement", float(agree))
+    assert agree >= 0.98
+
+
 def _rows_equal(g, o):
     assert g.shape == o.shape, (g.shape, o.shape)
     if g.size:
