@@ -55,14 +55,22 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const MfmaTile *__restrict__ 
   const u32x4 *F = feat + t.featb_off / 8;       // featb_off counts 16-bit elements
   const u32x4 *Wg = filt + t.filt_off / 8;
 
-  // stage the feature tile (zero outside the level)
-  for (int i = tid; i < AX * AY * 4; i += 256) {
-    const int cell = i >> 2, part = i & 3;
-    const int ax = cell / AY, ay = cell - ax * AY;
-    const int gx = t.x0 + ax, gy = t.y0 + ay;
-    u32x4 v = {0u, 0u, 0u, 0u};
-    if (gx < t.F1 && gy < t.F0) v = F[((long long)gx * t.F0 + gy) * 4 + part];
-    As[i] = v;
+  // stage the feature tile (zero outside the level), 4 loads in flight per thread
+  for (int i0 = tid; i0 < AX * AY * 4; i0 += 4 * 256) {
+    u32x4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int i = i0 + u * 256;
+      const int cell = i >> 2, part = i & 3;
+      const int ax = cell / AY, ay = cell - ax * AY;
+      const int gx = t.x0 + ax, gy = t.y0 + ay;
+      v[u] = (u32x4){0u, 0u, 0u, 0u};
+      if (i < AX * AY * 4 && gx < t.F1 && gy < t.F0)
+        v[u] = F[((long long)gx * t.F0 + gy) * 4 + part];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (i0 + u * 256 < AX * AY * 4) As[i0 + u * 256] = v[u];
   }
   for (int i = tid; i < bstage; i += 256) Bs[i] = Wg[i];
   __syncthreads();
@@ -74,24 +82,62 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const MfmaTile *__restrict__ 
     for (int nt = 0; nt < NT; nt++) acc[mt][nt] = (f32x4){0.f, 0.f, 0.f, 0.f};
 
   const int r = lane & 15, chunk = lane >> 4;
+  // the next tap column's filters are loaded into registers before this column's MFMAs and
+  // written to LDS after them, so their latency hides behind the arithmetic
+  constexpr int PRE = 8;
+  const bool in_regs = bstage <= PRE * 256;
   for (int xp = 0; xp < fw; xp++) {
     const int buf = xp & 1;
-    if (xp + 1 < fw) {
-      const u32x4 *src = Wg + (long long)(xp + 1) * bstage;
-      u32x4 *dst = Bs + (buf ^ 1) * bstage;
-      for (int i = tid; i < bstage; i += 256) dst[i] = src[i];
+    u32x4 pre[PRE];
+    const bool next = xp + 1 < fw;
+    const u32x4 *src = Wg + (long long)(xp + 1) * bstage;
+    u32x4 *dst = Bs + (buf ^ 1) * bstage;
+    if (next) {
+      if (in_regs) {
+#pragma unroll
+        for (int c = 0; c < PRE; c++) {
+          const int i = tid + c * 256;
+          if (i < bstage) pre[c] = src[i];
+        }
+      } else {
+        for (int i = tid; i < bstage; i += 256) dst[i] = src[i];
+      }
     }
     const u32x4 *Bb = Bs + buf * bstage;
     const u32x4 *Ab = As + ((wave + xp) * AY) * 4 + chunk;
-    for (int yp = 0; yp < fh; yp++) {
-      u32x4 b[NT];
+    // two register sets: tap yp + 1's fragments are read while tap yp's MFMAs run (the
+    // scheduling barriers keep the compiler from sinking the reads next to their use)
+    u32x4 b0[NT], a0[4], b1[NT], a1[4];
+    auto load = [&](u32x4 *b, u32x4 *a, int yp) {
 #pragma unroll
       for (int nt = 0; nt < NT; nt++) b[nt] = Bb[((yp * NT + nt) * 4 + chunk) * 16 + r];
 #pragma unroll
-      for (int mt = 0; mt < 4; mt++) {
-        const u32x4 a = Ab[(mt * 16 + r + yp) * 4];
+      for (int mt = 0; mt < 4; mt++) a[mt] = Ab[(mt * 16 + r + yp) * 4];
+    };
+    auto mma = [&](const u32x4 *b, const u32x4 *a) {
 #pragma unroll
-        for (int nt = 0; nt < NT; nt++) acc[mt][nt] = mfma16<KIND>(a, b[nt], acc[mt][nt]);
+      for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) acc[mt][nt] = mfma16<KIND>(a[mt], b[nt], acc[mt][nt]);
+    };
+    // the reads are unconditional (the last ones repeat tap fh - 1) so that the wait
+    // counts before each set's MFMAs are fixed
+    load(b0, a0, 0);
+    for (int yp = 0; yp < fh; yp += 2) {
+      load(b1, a1, min(yp + 1, fh - 1));
+      __builtin_amdgcn_sched_barrier(0);
+      mma(b0, a0);
+      __builtin_amdgcn_sched_barrier(0);
+      load(b0, a0, min(yp + 2, fh - 1));
+      __builtin_amdgcn_sched_barrier(0);
+      if (yp + 1 < fh) mma(b1, a1);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (next && in_regs) {
+#pragma unroll
+      for (int c = 0; c < PRE; c++) {
+        const int i = tid + c * 256;
+        if (i < bstage) dst[i] = pre[c];
       }
     }
     __syncthreads();
