@@ -171,6 +171,8 @@ struct Plan {
   std::vector<int2> resize_tab;      // (start, count) per destination index, all ops
   std::vector<int> resize_si;
   std::vector<float> resize_alpha;
+  std::vector<int4> resize_si4;       // the same taps padded to 4 per destination index
+  std::vector<float4> resize_al4;
   // HOG
   std::vector<HogLevel> hog;          // [T16 levels][T8 levels][feature levels]
   std::vector<GradImg> grad;          // one per pyramid image
@@ -221,7 +223,8 @@ struct Plan {
   std::vector<int> ascore_tab;         // per valid level: Xn[W] then Yn[H]
   // device copies of the tables
   DeviceBuffer d_resize_ops, d_resize_tab, d_resize_si, d_resize_alpha, d_hog, d_conv_tiles,
-      d_mfma_tiles, d_map_dims, d_map_off, d_dt1, d_dt2, d_dt1w, d_dt2w, d_nodes, d_depths, d_smaps, d_sparts,
+      d_mfma_tiles, d_map_dims, d_map_off, d_dt1, d_dt2, d_dt1w, d_dt2w, d_resize_si4,
+      d_resize_al4, d_nodes, d_depths, d_smaps, d_sparts,
       d_ascore_tab, d_grad, d_grid, d_qtiles0, d_qtiles1, d_qcounter, d_fused;
   // device arenas
   DeviceBuffer d_frame, d_img, d_tmp, d_hist, d_norm, d_feat, d_featb, d_conv, d_tmpM,

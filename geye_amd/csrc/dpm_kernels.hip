@@ -104,15 +104,16 @@ __global__ __launch_bounds__(256) void k_resize_cols(const ResizeOp *__restrict_
 // order) as the two separate passes, so the image is bit-identical.
 __global__ __launch_bounds__(256) void k_resize_fused(const FusedResize *__restrict__ ops,
                                                       int nops, const int2 *__restrict__ tab,
-                                                      const int *__restrict__ si,
-                                                      const float *__restrict__ alpha,
+                                                      const int4 *__restrict__ si4,
+                                                      const float4 *__restrict__ al4,
                                                       const float *__restrict__ src,
                                                       float *__restrict__ dst) {
   constexpr int MAXTAP = 4;  // scales >= 0.5: at most 4 source samples per output
-  __shared__ float s_t[kResizeTY][kResizeSpan];
+  __shared__ float s_t[3][kResizeTY][kResizeSpan];
   __shared__ int s_rn[kResizeTY], s_cn[kResizeTX];
-  __shared__ int s_rsi[kResizeTY][MAXTAP], s_csi[kResizeTX][MAXTAP];
-  __shared__ float s_ral[kResizeTY][MAXTAP], s_cal[kResizeTX][MAXTAP];
+  // column taps stored tap-major so that a wave's lanes (consecutive x) hit distinct banks
+  __shared__ int s_rsi[kResizeTY][MAXTAP], s_csi[MAXTAP][kResizeTX];
+  __shared__ float s_ral[kResizeTY][MAXTAP], s_cal[MAXTAP][kResizeTX];
   const int bx = xcd_local_block(blockIdx.x, gridDim.x);
   int k = 0;
   while (k + 1 < nops && ops[k + 1].blocks_begin <= bx) k++;
@@ -127,54 +128,83 @@ __global__ __launch_bounds__(256) void k_resize_fused(const FusedResize *__restr
     const bool row = tid < kResizeTY;
     const int j = row ? tid : tid - kResizeTY;
     const bool live = row ? j < ny : j < nx;
-    const int2 t = live ? tab[(row ? op.rtab_off + dy0 : op.ctab_off + dx0) + j] : make_int2(0, 0);
-    const int n = min(t.y, MAXTAP);
-    int *osi = row ? s_rsi[j] : s_csi[j];
-    float *oal = row ? s_ral[j] : s_cal[j];
-    for (int q = 0; q < n; q++) {
-      osi[q] = si[t.x + q];
-      oal[q] = alpha[t.x + q];
+    const int k = (row ? op.rtab_off + dy0 : op.ctab_off + dx0) + j;
+    const int n = live ? min(tab[k].y, MAXTAP) : 0;
+    const int4 sv = live ? si4[k] : make_int4(0, 0, 0, 0);
+    const float4 av = live ? al4[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int svq[MAXTAP] = {sv.x, sv.y, sv.z, sv.w};
+    const float avq[MAXTAP] = {av.x, av.y, av.z, av.w};
+#pragma unroll
+    for (int q = 0; q < MAXTAP; q++) {
+      if (row) {
+        s_rsi[j][q] = svq[q];
+        s_ral[j][q] = avq[q];
+      } else {
+        s_csi[q][j] = svq[q];
+        s_cal[q][j] = avq[q];
+      }
     }
     (row ? s_rn : s_cn)[j] = n;
   }
   __syncthreads();
-  const int c0 = s_csi[0][0], span = s_csi[nx - 1][s_cn[nx - 1] - 1] - c0 + 1;
-  for (int c = 0; c < 3; c++) {
-    const float *in = src + op.in_off + (long long)c * op.in_h * op.in_w + c0;
-    for (int i = tid; i < ny * span; i += 256) {
-      const int r = i / span, x = i - r * span;
-      const int n = s_rn[r];
-      float v[MAXTAP];
+  const int c0 = s_csi[0][0], span = s_csi[s_cn[nx - 1] - 1][nx - 1] - c0 + 1;
+  // row pass, the three planes of an element together: up to 12 loads in flight
+  const long long plane = (long long)op.in_h * op.in_w;
+  const float *in = src + op.in_off + c0;
+  constexpr int U = 2;  // elements per thread per step
+  for (int i0 = tid; i0 < ny * span; i0 += U * 256) {
+    float v[U][3][MAXTAP];
+    int rr[U], xx[U], nn[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = i0 + u * 256;
+      rr[u] = i / span;
+      xx[u] = i - rr[u] * span;
+      nn[u] = i < ny * span ? s_rn[rr[u]] : 0;
 #pragma unroll
       for (int q = 0; q < MAXTAP; q++)
-        if (q < n) v[q] = in[(long long)s_rsi[r][q] * op.in_w + x];
+        if (q < nn[u]) {
+          const float *p = in + (long long)s_rsi[rr[u]][q] * op.in_w + xx[u];
+#pragma unroll
+          for (int c = 0; c < 3; c++) v[u][c][q] = p[c * plane];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (nn[u] == 0) continue;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int q = 0; q < MAXTAP; q++)
+          if (q < nn[u]) acc += s_ral[rr[u]][q] * v[u][c][q];
+        s_t[c][rr[u]][xx[u]] = acc;
+      }
+    }
+  }
+  __syncthreads();
+  const long long oplane = (long long)op.out_h * op.out_w;
+  float *out = dst + op.out_off + (long long)dy0 * op.out_w + dx0;
+  for (int i = tid; i < ny * kResizeTX; i += 256) {
+    const int r = i / kResizeTX, x = i - r * kResizeTX;
+    if (x >= nx) continue;
+    const int n = s_cn[x];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
       float acc = 0.0f;
 #pragma unroll
       for (int q = 0; q < MAXTAP; q++)
-        if (q < n) acc += s_ral[r][q] * v[q];
-      s_t[r][x] = acc;
+        if (q < n) acc += s_cal[q][x] * s_t[c][r][s_csi[q][x] - c0];
+      out[c * oplane + (long long)r * op.out_w + x] = acc;
     }
-    __syncthreads();
-    float *out = dst + op.out_off + (long long)c * op.out_h * op.out_w;
-    for (int i = tid; i < ny * kResizeTX; i += 256) {
-      const int r = i / kResizeTX, x = i - r * kResizeTX;
-      if (x >= nx) continue;
-      const int n = s_cn[x];
-      float acc = 0.0f;
-#pragma unroll
-      for (int q = 0; q < MAXTAP; q++)
-        if (q < n) acc += s_cal[x][q] * s_t[r][s_csi[x][q] - c0];
-      out[(long long)(dy0 + r) * op.out_w + dx0 + x] = acc;
-    }
-    __syncthreads();
   }
 }
 
 void launch_resize_fused(hipStream_t s, const FusedResize *ops, int nops, int blocks,
-                         const int2 *tab, const int *si, const float *alpha, const float *src,
+                         const int2 *tab, const int4 *si4, const float4 *al4, const float *src,
                          float *dst) {
   if (blocks > 0)
-    hipLaunchKernelGGL(k_resize_fused, dim3(blocks), dim3(256), 0, s, ops, nops, tab, si, alpha,
+    hipLaunchKernelGGL(k_resize_fused, dim3(blocks), dim3(256), 0, s, ops, nops, tab, si4, al4,
                        src, dst);
 }
 

@@ -444,6 +444,20 @@ int plan_upload(Plan &P, hipStream_t st) {
   CHK(upload(P.d_resize_tab, P.resize_tab, st));
   CHK(upload(P.d_resize_si, P.resize_si, st));
   CHK(upload(P.d_resize_alpha, P.resize_alpha, st));
+  // per destination index, its (at most 4) taps in one 16-byte load each
+  P.resize_si4.assign(P.resize_tab.size(), int4{0, 0, 0, 0});
+  P.resize_al4.assign(P.resize_tab.size(), float4{0.f, 0.f, 0.f, 0.f});
+  for (size_t k = 0; k < P.resize_tab.size(); k++) {
+    const int2 t = P.resize_tab[k];
+    int *si4 = &P.resize_si4[k].x;
+    float *al4 = &P.resize_al4[k].x;
+    for (int q = 0; q < std::min(t.y, 4); q++) {
+      si4[q] = P.resize_si[t.x + q];
+      al4[q] = P.resize_alpha[t.x + q];
+    }
+  }
+  CHK(upload(P.d_resize_si4, P.resize_si4, st));
+  CHK(upload(P.d_resize_al4, P.resize_al4, st));
   CHK(upload(P.d_hog, P.hog, st));
   CHK(upload(P.d_conv_tiles, P.conv_tiles, st));
   CHK(upload(P.d_mfma_tiles, P.mfma_tiles, st));
@@ -492,6 +506,7 @@ void plan_release(Plan &P) {
   for (auto &kv : P.graphs) (void)hipGraphExecDestroy(kv.second);
   P.graphs.clear();
   for (DeviceBuffer *b : {&P.d_resize_ops, &P.d_resize_tab, &P.d_resize_si, &P.d_resize_alpha,
+                          &P.d_resize_si4, &P.d_resize_al4,
                           &P.d_hog, &P.d_conv_tiles, &P.d_mfma_tiles, &P.d_map_dims,
                           &P.d_map_off, &P.d_dt1, &P.d_dt2, &P.d_dt1w, &P.d_dt2w,
                           &P.d_nodes, &P.d_depths, &P.d_smaps, &P.d_sparts, &P.d_frame,
@@ -612,8 +627,8 @@ int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, int hk
   for (size_t r = 0; r < P.fused.size(); r++) {
     KTimer t(c, "resize");
     launch_resize_fused(s, dp<FusedResize>(P.d_fused) + fbase, (int)P.fused[r].size(),
-                        P.fused_blocks[r], dp<int2>(P.d_resize_tab), dp<int>(P.d_resize_si),
-                        dp<float>(P.d_resize_alpha), dp<float>(P.d_img), dp<float>(P.d_img));
+                        P.fused_blocks[r], dp<int2>(P.d_resize_tab), dp<int4>(P.d_resize_si4),
+                        dp<float4>(P.d_resize_al4), dp<float>(P.d_img), dp<float>(P.d_img));
     fbase += (int)P.fused[r].size();
   }
   const HogLevel *hog = dp<HogLevel>(P.d_hog);
