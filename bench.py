@@ -196,7 +196,7 @@ def probe(precision: str) -> int:
 
 def dt_traffic() -> dict | None:
     """HBM bytes per DT launch from the committed counter summary, if present."""
-    path = os.path.join(ROOT, "profiles", "k_dt_traffic_1080p_synthetic_car.txt")
+    path = os.path.join(ROOT, "profiles", "k_dt_window_traffic_1080p_synthetic_car.txt")
     if not os.path.exists(path):
         return None
     vals = {}
@@ -210,7 +210,7 @@ def dt_traffic() -> dict | None:
     if "bytes_per_launch" not in vals:
         return None
     return {"bytes_per_launch": vals["bytes_per_launch"],
-            "source": "profiles/k_dt_traffic_1080p_synthetic_car.txt"}
+            "source": "profiles/k_dt_window_traffic_1080p_synthetic_car.txt"}
 
 
 def main() -> int:
@@ -301,7 +301,8 @@ def main() -> int:
                 "unit": "GB/s", "frac": bytes_per_launch / mean_s / HBM_PEAK,
                 "traffic": tr["bytes_per_launch"] if tr else None,
                 "traffic_source": tr["source"] if tr else None,
-                "kernel": "k_dt (dt_cols + dt_rows launches)",
+                "kernel": ("k_dt" if args.precision == "fp32" else "k_dt_window")
+                          + " (dt_cols + dt_rows launches)",
                 "algorithmic_bytes_per_launch": int(bytes_per_launch),
                 "per_unit": "16 B per part score cell (read score, write M, Ix, Iy)",
                 "mean_launch_ms": mean_s * 1e3}
