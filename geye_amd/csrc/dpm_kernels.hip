@@ -1206,6 +1206,10 @@ void launch_compact(hipStream_t s, const int *flags, long long n, int *blocksum,
   hipLaunchKernelGGL(k_flag_scatter, dim3(nb), dim3(256), 0, s, flags, n, blocksum, out);
 }
 
+void launch_scan_blocks(hipStream_t s, int *blocksum, int nb, int *total) {
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, blocksum, nb, total);
+}
+
 void launch_rows(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,
                  const float *score, const int *cand, int ncand, const int *tmpIx,
                  const int *tmpIy, int GL, int padx, int pady, const int *rsize, float *rows) {

@@ -35,17 +35,24 @@ int launch_nms_sort(hipStream_t s, void *temp, size_t temp_bytes, unsigned int *
                     unsigned int *keys_out, int *vals_in, int *vals_out, int n);
 void launch_nms_gather(hipStream_t s, const int *ord, const NmsBox *box, int n, NmsBox *sbox,
                        int *alive, int *status, int *sup, const int *cand, int *rank_of_flat);
+// One round: window bitmask, greedy, suppression of the later boxes, then the fused
+// retire-and-count pass that leaves per-block counts of the surviving later boxes in
+// blocksum.  nalive_host is a host upper bound of *nalive: the kernels read the true count
+// and do nothing past it.
+// maps selects the score-map scatter for the suppression of the later boxes; without it
+// every later box is tested against every pick.
 void launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const int *nalive,
                       int nalive_host, float overlap, unsigned long long *mask, int *status,
-                      int *sup, int *picks, int *counters, int *keep, const NmsMaps *maps);
-// nalive_host in launch_nms_round and bound here are host upper bounds of the device counts
-// (*nalive, *nsel): the kernels read the true counts and do nothing past them.
-void launch_nms_regather(hipStream_t s, const int *alive, const int *nalive, const int *sel,
-                         const int *nsel, int bound, int *alive_out);
+                      int *sup, int *picks, int *counters, int *blocksum, const NmsMaps *maps);
+// The round's surviving later boxes, in order, to alive_out, their count to *nalive_out;
+// bound >= *nalive (positions of the round).
+void launch_nms_compact(hipStream_t s, const int *alive, const int *nalive, const int *sup,
+                        int *blocksum, int bound, int *nalive_out, int *alive_out);
+// the pick count is read from counters[0] on the device; bound >= that count
 void launch_nms_check(hipStream_t s, const NmsBox *sbox, int n, const int *status, const int *sup,
-                      const int *picks, int npick, int *flags);
-void launch_nms_keep(hipStream_t s, const NmsBox *sbox, const int *picks, int npick,
-                     const int *flags, int *keep_orig);
+                      const int *picks, const int *counters, int *flags);
+void launch_nms_keep(hipStream_t s, const NmsBox *sbox, const int *picks, const int *counters,
+                     int bound, const int *flags, int *keep_orig);
 void launch_fill(hipStream_t s, int *p, long long n, int v);
 void launch_take(hipStream_t s, const int *src, const int *idx, int n, int *dst);
 

@@ -24,6 +24,7 @@
 // that check fails the caller re-runs the exact host quicksort (a reported fallback).
 #include <rocprim/device/device_radix_sort.hpp>
 
+#include "dpm_kernels.h"
 #include "dpm_nms.h"
 
 namespace geye {
@@ -408,54 +409,92 @@ __global__ __launch_bounds__(256) void k_nms_scatter(const NmsBox *__restrict__ 
   }
 }
 
-// End of a round: every box of the window is decided, and so is every later box some pick
-// suppressed.  Their score cells leave rank_of_flat (-1), so the next rounds' scatters
-// reject them with the lookup they already make.
-__global__ __launch_bounds__(256) void k_nms_retire(const int *__restrict__ alive,
-                                                    const int *__restrict__ nalive,
-                                                    const int *__restrict__ sup,
-                                                    const NmsBox *__restrict__ sbox,
-                                                    const int *__restrict__ cand,
-                                                    int *__restrict__ rank_of_flat) {
-  const int na = *nalive;
-  const int W = min(na, kNmsWindow);
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= na) return;
-  const int r = alive[t];
-  if (t < W || sup[r] != kNoSup) rank_of_flat[cand[sbox[r].idx]] = -1;
+
+constexpr int kNmsCompactBlock = 1024;  // positions per workgroup of the per-round counts
+
+// End of a round, fused: every alive position t < na is either decided (a window box, or a
+// later box some pick suppressed), whose score cell then leaves rank_of_flat (-1, when the
+// map path is on), or still alive.  Each workgroup counts the alive later boxes of its
+// kNmsCompactBlock positions (4 consecutive per thread) into blocksum; positions past the
+// true count add nothing, so a grid over a host bound counts exactly.
+
+__device__ inline bool nms_stays(int t, int na, int W, const int *alive, const int *sup) {
+  return t >= W && t < na && sup[alive[t]] == kNoSup;
 }
 
-// keep[j] for the j-th box after the window; the launch covers a host bound on that count
-// and writes 0 past the true count, so the compaction over the bound counts exactly.
-__global__ __launch_bounds__(256) void k_nms_aliveflags(const int *__restrict__ alive,
-                                                        const int *__restrict__ nalive,
-                                                        const int *__restrict__ sup,
-                                                        int *__restrict__ keep, int bound) {
+__global__ __launch_bounds__(256) void k_nms_post(const int *__restrict__ alive,
+                                                  const int *__restrict__ nalive,
+                                                  const int *__restrict__ sup,
+                                                  const NmsBox *__restrict__ sbox,
+                                                  const int *__restrict__ cand,
+                                                  int *__restrict__ rank_of_flat,
+                                                  int *__restrict__ blocksum) {
+  __shared__ int s_w[4];
   const int na = *nalive;
   const int W = min(na, kNmsWindow);
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= bound) return;
-  const int t = W + j;
-  keep[j] = (t < na && sup[alive[t]] == kNoSup) ? 1 : 0;
+  const int t0 = blockIdx.x * kNmsCompactBlock + threadIdx.x * 4;
+  int c = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int t = t0 + k;
+    if (t >= na) break;
+    const int r = alive[t];
+    if (t >= W && sup[r] == kNoSup)
+      c++;
+    else if (rank_of_flat)
+      rank_of_flat[cand[sbox[r].idx]] = -1;
+  }
+  for (int m = 1; m < 64; m <<= 1) c += __shfl_xor(c, m);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) blocksum[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
 }
 
-__global__ __launch_bounds__(256) void k_nms_regather(const int *__restrict__ alive,
-                                                      const int *__restrict__ nalive,
-                                                      const int *__restrict__ sel,
-                                                      const int *__restrict__ nsel,
-                                                      int *__restrict__ alive_out) {
-  const int W = min(*nalive, kNmsWindow);
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < *nsel) alive_out[k] = alive[W + sel[k]];
+// The alive later boxes, in order, to alive_out: blocksum holds each workgroup's exclusive
+// offset (scanned), positions as in k_nms_post.
+__global__ __launch_bounds__(256) void k_nms_compact(const int *__restrict__ alive,
+                                                     const int *__restrict__ nalive,
+                                                     const int *__restrict__ sup,
+                                                     const int *__restrict__ blocksum,
+                                                     int *__restrict__ alive_out) {
+  __shared__ int s_w[4];
+  const int na = *nalive;
+  const int W = min(na, kNmsWindow);
+  const int t0 = blockIdx.x * kNmsCompactBlock + threadIdx.x * 4;
+  bool f[4];
+  int c = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    f[k] = nms_stays(t0 + k, na, W, alive, sup);
+    c += f[k] ? 1 : 0;
+  }
+  // inclusive scan of c over the wave, then the preceding waves' totals
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int inc = c;
+  for (int m = 1; m < 64; m <<= 1) {
+    const int v = __shfl_up(inc, m);
+    if (lane >= m) inc += v;
+  }
+  if (lane == 63) s_w[wave] = inc;
+  __syncthreads();
+  int pos = blocksum[blockIdx.x] + inc - c;
+  for (int w = 0; w < wave; w++) pos += s_w[w];
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (f[k]) alive_out[pos++] = alive[t0 + k];
 }
+
+
 
 // finalisation: conflicts, last-pick emission, survivors in original order
 __global__ __launch_bounds__(256) void k_nms_check(const NmsBox *__restrict__ sbox, int n,
                                                    const int *__restrict__ status,
                                                    const int *__restrict__ sup,
-                                                   const int *__restrict__ picks, int npick,
+                                                   const int *__restrict__ picks,
+                                                   const int *__restrict__ counters,
                                                    int *__restrict__ flags) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int npick = counters[0];
   if (r >= n || npick == 0) return;
   const int last = picks[npick - 1];
   if (status[r] != 1) {
@@ -472,10 +511,12 @@ __global__ __launch_bounds__(256) void k_nms_check(const NmsBox *__restrict__ sb
 }
 
 __global__ __launch_bounds__(256) void k_nms_keep(const NmsBox *__restrict__ sbox,
-                                                  const int *__restrict__ picks, int npick,
+                                                  const int *__restrict__ picks,
+                                                  const int *__restrict__ counters,
                                                   const int *__restrict__ flags,
                                                   int *__restrict__ keep_orig) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int npick = counters[0];
   if (k >= npick) return;
   if (k == npick - 1 && flags[1] == 0) return;
   keep_orig[sbox[picks[k]].idx] = 1;
@@ -538,7 +579,8 @@ void launch_nms_gather(hipStream_t s, const int *ord, const NmsBox *box, int n, 
 
 void launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const int *nalive,
                       int nalive_host, float overlap, unsigned long long *mask, int *status,
-                      int *sup, int *picks, int *counters, int *keep, const NmsMaps *maps) {
+                      int *sup, int *picks, int *counters, int *blocksum,
+                      const NmsMaps *maps) {
   const int W = nalive_host < kNmsWindow ? nalive_host : kNmsWindow;
   hipLaunchKernelGGL(k_nms_winmask, dim3(W), dim3(64), 0, s, sbox, alive, nalive, overlap, mask);
   static bool attr = false;
@@ -552,6 +594,7 @@ void launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const
                      sup, picks, counters, g_nms_phase_clock);
   const int rest = nalive_host - W;
   if (rest <= 0) return;
+  const int nb = (nalive_host + kNmsCompactBlock - 1) / kNmsCompactBlock;
   if (maps) {
     hipLaunchKernelGGL(k_nms_scatter, dim3(W, kNmsScatterSplit), dim3(256), 0, s, sbox, picks,
                        counters, maps->maps,
@@ -561,31 +604,31 @@ void launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const
     hipLaunchKernelGGL(k_nms_suppress, dim3((rest + 255) / 256), dim3(256), 0, s, sbox, alive,
                        nalive, picks, counters, overlap, sup);
   }
-  hipLaunchKernelGGL(k_nms_aliveflags, dim3((rest + 255) / 256), dim3(256), 0, s, alive, nalive,
-                     sup, keep, rest);
-  if (maps && maps->cand)
-    hipLaunchKernelGGL(k_nms_retire, dim3((nalive_host + 255) / 256), dim3(256), 0, s, alive,
-                       nalive, sup, sbox, maps->cand, maps->rank_of_flat);
+  const bool retire = maps && maps->cand;
+  hipLaunchKernelGGL(k_nms_post, dim3(nb), dim3(256), 0, s, alive, nalive, sup, sbox,
+                     retire ? maps->cand : nullptr, retire ? maps->rank_of_flat : nullptr,
+                     blocksum);
 }
 
-void launch_nms_regather(hipStream_t s, const int *alive, const int *nalive, const int *sel,
-                         const int *nsel, int bound, int *alive_out) {
-  if (bound > 0)
-    hipLaunchKernelGGL(k_nms_regather, dim3((bound + 255) / 256), dim3(256), 0, s, alive, nalive,
-                       sel, nsel, alive_out);
+void launch_nms_compact(hipStream_t s, const int *alive, const int *nalive, const int *sup,
+                        int *blocksum, int bound, int *nalive_out, int *alive_out) {
+  const int nb = (bound + kNmsCompactBlock - 1) / kNmsCompactBlock;
+  launch_scan_blocks(s, blocksum, nb, nalive_out);
+  hipLaunchKernelGGL(k_nms_compact, dim3(nb), dim3(256), 0, s, alive, nalive, sup, blocksum,
+                     alive_out);
 }
 
 void launch_nms_check(hipStream_t s, const NmsBox *sbox, int n, const int *status, const int *sup,
-                      const int *picks, int npick, int *flags) {
+                      const int *picks, const int *counters, int *flags) {
   hipLaunchKernelGGL(k_nms_check, dim3((n + 255) / 256), dim3(256), 0, s, sbox, n, status, sup,
-                     picks, npick, flags);
+                     picks, counters, flags);
 }
 
-void launch_nms_keep(hipStream_t s, const NmsBox *sbox, const int *picks, int npick,
-                     const int *flags, int *keep_orig) {
-  if (npick > 0)
-    hipLaunchKernelGGL(k_nms_keep, dim3((npick + 255) / 256), dim3(256), 0, s, sbox, picks, npick,
-                       flags, keep_orig);
+void launch_nms_keep(hipStream_t s, const NmsBox *sbox, const int *picks, const int *counters,
+                     int bound, const int *flags, int *keep_orig) {
+  if (bound > 0)
+    hipLaunchKernelGGL(k_nms_keep, dim3((bound + 255) / 256), dim3(256), 0, s, sbox, picks,
+                       counters, flags, keep_orig);
 }
 
 void launch_fill(hipStream_t s, int *p, long long n, int v) {

@@ -515,7 +515,7 @@ void plan_release(Plan &P) {
                           &P.d_flags, &P.d_blocksum, &P.d_cand, &P.d_count, &P.d_rows,
                           &P.n_box, &P.n_keys, &P.n_keys2, &P.n_sbox, &P.n_alive0, &P.n_alive1,
                           &P.n_status, &P.n_fsup, &P.n_picks, &P.n_counters, &P.n_mask,
-                          &P.n_keep, &P.n_sel, &P.n_sort, &P.n_flags, &P.n_keep_orig,
+                          &P.n_sort, &P.n_flags, &P.n_keep_orig,
                           &P.n_kept, &P.n_kept_flat, &P.n_total, &P.n_vals, &P.n_ord,
                           &P.n_rank, &P.d_ascore_tab, &P.d_ascore, &P.d_grad, &P.d_grid,
                           &P.d_qtiles0, &P.d_qtiles1, &P.d_qcounter, &P.d_fused})
@@ -844,8 +844,6 @@ int nms_buffers(Plan &P, int n) {
   CHK(ensure(P.n_status, 4 * N));
   CHK(ensure(P.n_fsup, 4 * N));
   CHK(ensure(P.n_picks, 4 * N));
-  CHK(ensure(P.n_keep, 4 * N));
-  CHK(ensure(P.n_sel, 4 * N));
   CHK(ensure(P.n_keep_orig, 4 * N));
   CHK(ensure(P.n_kept, 4 * N));
   CHK(ensure(P.n_kept_flat, 4 * N));
@@ -878,21 +876,22 @@ int nms_core(geye_ctx *c, Plan &P, int n, float overlap, const NmsMaps *maps, co
                     dp<int>(P.n_alive0), dp<int>(P.n_status), dp<int>(P.n_fsup), cand,
                     maps ? maps->rank_of_flat : nullptr);
   launch_fill(s, dp<int>(P.n_counters), 4, 0);
-  int *d_na = dp<int>(P.n_total);
-  int *d_sel_total = dp<int>(P.n_total) + 1;
-  launch_fill(s, d_na, 1, n);
+  // alive counts, double-buffered: round k reads na_buf[cur], its compaction writes the other
+  int *na_buf[2] = {dp<int>(P.n_total), dp<int>(P.n_total) + 2};
+  int cur = 0;
+  launch_fill(s, na_buf[0], 1, n);
   int *alive = dp<int>(P.n_alive0), *alive_next = dp<int>(P.n_alive1);
+  int *blocksum = dp<int>(P.d_blocksum);
+  int *d_sel_total = dp<int>(P.n_total) + 1;  // count of the final survivor compaction
   // Rounds run one ahead of the host: after round k's compaction, round k+1 is enqueued with
-  // a grid bound of round k's boxes after its window (its alive count can only be smaller)
-  // before the host waits for round k's survivor count, so the GPU never idles on the
-  // read-back.  The kernels read the true counts from device memory and do nothing past
-  // them; the compaction over a bound counts exactly because the flags past the true count
-  // are written as 0.
+  // a bound of round k's boxes after its window (its alive count can only be smaller) before
+  // the host waits for round k's survivor count, so the GPU never idles on the read-back.
+  // The kernels read the true counts from device memory and do nothing past them.
   auto round = [&](int bound) {
     KTimer t(c, "nms_round");
-    launch_nms_round(s, dp<NmsBox>(P.n_sbox), alive, d_na, bound, overlap,
+    launch_nms_round(s, dp<NmsBox>(P.n_sbox), alive, na_buf[cur], bound, overlap,
                      dp<unsigned long long>(P.n_mask), dp<int>(P.n_status), dp<int>(P.n_fsup),
-                     dp<int>(P.n_picks), dp<int>(P.n_counters), dp<int>(P.n_keep), maps);
+                     dp<int>(P.n_picks), dp<int>(P.n_counters), blocksum, maps);
   };
   int na = n;  // exact alive count of the round last enqueued
   int slot = 0;
@@ -900,27 +899,34 @@ int nms_core(geye_ctx *c, Plan &P, int n, float overlap, const NmsMaps *maps, co
   for (;;) {
     const int rest = na - std::min(na, kNmsWindow);  // boxes after that round's window
     if (rest <= 0) break;                           // it held every alive box
-    launch_compact(s, dp<int>(P.n_keep), rest, dp<int>(P.d_blocksum), d_sel_total,
-                   dp<int>(P.n_sel));
-    launch_nms_regather(s, alive, d_na, dp<int>(P.n_sel), d_sel_total, rest, alive_next);
-    HIPCHK(hipMemcpyAsync(d_na, d_sel_total, sizeof(int), hipMemcpyDeviceToDevice, s));
-    HIPCHK(hipMemcpyAsync(c->h_count + slot, d_sel_total, sizeof(int), hipMemcpyDeviceToHost, s));
+    launch_nms_compact(s, alive, na_buf[cur], dp<int>(P.n_fsup), blocksum, na,
+                       na_buf[cur ^ 1], alive_next);
+    HIPCHK(hipMemcpyAsync(c->h_count + slot, na_buf[cur ^ 1], sizeof(int), hipMemcpyDeviceToHost,
+                          s));
     HIPCHK(hipEventRecord(c->ev_count[slot], s));
     std::swap(alive, alive_next);
+    cur ^= 1;
     round(rest);                                    // next round, bounded by rest
     HIPCHK(hipEventSynchronize(c->ev_count[slot]));
     na = c->h_count[slot];                          // its exact alive count
     slot ^= 1;
     if (na == 0) break;
   }
-  int npick = 0;
-  HIPCHK(hipMemcpyAsync(&npick, P.n_counters.p, sizeof(int), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
+  // Finalisation with one read-back: the tie check and the survivor compaction are both
+  // enqueued (the pick count stays on the device), then flags and count come back together.
+  // A tie flag discards the device result for the host replay below.
   launch_fill(s, dp<int>(P.n_flags), 4, 0);
   launch_nms_check(s, dp<NmsBox>(P.n_sbox), n, dp<int>(P.n_status), dp<int>(P.n_fsup),
-                   dp<int>(P.n_picks), npick, dp<int>(P.n_flags));
+                   dp<int>(P.n_picks), dp<int>(P.n_counters), dp<int>(P.n_flags));
+  launch_fill(s, dp<int>(P.n_keep_orig), (long long)N, 0);
+  launch_nms_keep(s, dp<NmsBox>(P.n_sbox), dp<int>(P.n_picks), dp<int>(P.n_counters), n,
+                  dp<int>(P.n_flags), dp<int>(P.n_keep_orig));
+  launch_compact(s, dp<int>(P.n_keep_orig), (long long)N, dp<int>(P.d_blocksum), d_sel_total,
+                 dp<int>(P.n_kept));
   int flags[2] = {0, 0};
+  int nk_dev = 0;
   HIPCHK(hipMemcpyAsync(flags, P.n_flags.p, sizeof(flags), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&nk_dev, d_sel_total, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   int nk = 0;
   if (flags[0] != 0) {
@@ -945,13 +951,7 @@ int nms_core(geye_ctx *c, Plan &P, int n, float overlap, const NmsMaps *maps, co
       HIPCHK(hipStreamSynchronize(s));
     }
   } else {
-    launch_fill(s, dp<int>(P.n_keep_orig), (long long)N, 0);
-    launch_nms_keep(s, dp<NmsBox>(P.n_sbox), dp<int>(P.n_picks), npick, dp<int>(P.n_flags),
-                    dp<int>(P.n_keep_orig));
-    launch_compact(s, dp<int>(P.n_keep_orig), (long long)N, dp<int>(P.d_blocksum),
-                   d_sel_total, dp<int>(P.n_kept));
-    HIPCHK(hipMemcpyAsync(&nk, d_sel_total, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    nk = nk_dev;
   }
   HIPCHK(hipGetLastError());
   *nkept = nk;

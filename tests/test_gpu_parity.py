@@ -222,6 +222,19 @@ def test_car_detection_synthetic(ctx):
     _rows_equal(g, o)
 
 
+@pytest.mark.parametrize("overlap", [0.3, 0.5, 0.7, 0.95])
+def test_car_detection_overlaps(ctx, overlap):
+    """Device NMS in the detect path (score-map scatter suppression) against the oracle at
+    several overlaps, from loose to nearly disjoint-only."""
+    say("car_detection synthetic car 480x360 overlap", overlap)
+    m = synth.synthetic_car()
+    frame = synth.make_frame(480, 360, seed=13)
+    gm = ctx.model(m)
+    g = gm.car_detection(frame, thresh=-0.5, overlap=overlap)
+    o = O.nms(O.detect(m, frame, -0.5), overlap)
+    _rows_equal(g, o)
+
+
 def _random_boxes(n, seed, ties=False, GL=11):
     rng = np.random.default_rng(seed)
     y1 = rng.integers(0, 400, n).astype(np.float32)
