@@ -318,6 +318,40 @@ __global__ __launch_bounds__(256) void k_hog_grad(const GradImg *__restrict__ im
 // sequence of float additions as the reference's scatter loop.  Then hist[o][x][y] and
 // norm[x][y] = sum_{o<9} (h_o + h_{o+9})^2 (CT/featurepyramid.cpp.new2:391-404).
 constexpr int kHogT = 16;
+// One cell's votes in the reference's order (x outer, y inner).  Its rows are loaded for
+// all NJ positions of a pixel column at once (rows clamped into the image), so the loads
+// overlap; only the accumulation is limited to the cell's ny rows.
+template <int NJ>
+__device__ inline void hog_vote_cell(const int2 *__restrict__ grid, int h, int w, int sbin,
+                                     int cx, int cy, int x0, int x1, int y0, int y1, int ny,
+                                     float *acc) {
+  constexpr int TT = kHogT * kHogT;
+  float wy[NJ];
+  int yc[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; j++) {
+    const int y = min(y0 + j, y1);
+    int iy;
+    float v0, v1;
+    interp_coord(y, sbin, &iy, &v0, &v1);
+    wy[j] = (iy == cy) ? v1 : v0;
+    yc[j] = max(0, min(y, h - 2));
+  }
+  for (int x = x0; x <= x1; x++) {
+    int ix;
+    float v0, v1;
+    interp_coord(x, sbin, &ix, &v0, &v1);
+    const float wx = (ix == cx) ? v1 : v0;
+    const int2 *col = grid + (long long)min(x, w - 2) * h;
+    int2 g[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; j++) g[j] = col[yc[j]];
+#pragma unroll
+    for (int j = 0; j < NJ; j++)
+      if (j < ny) acc[g[j].y * TT] += wx * wy[j] * __int_as_float(g[j].x);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_hog_votes(const HogLevel *__restrict__ lv, int nlv,
                                                    const int2 *__restrict__ grid_arena,
                                                    float *__restrict__ hist_arena,
@@ -341,33 +375,12 @@ __global__ __launch_bounds__(256) void k_hog_votes(const HogLevel *__restrict__ 
   const int x0 = max(1, first_pix(cx - 1, sbin)), x1 = min(vis1 - 2, first_pix(cx + 1, sbin) - 1);
   const int y0 = max(1, first_pix(cy - 1, sbin)), y1 = min(vis0 - 2, first_pix(cy + 1, sbin) - 1);
   const int ny = y1 - y0 + 1;
-  float wy[JM];
-  int yc[JM];
-#pragma unroll
-  for (int j = 0; j < JM; j++) {
-    const int y = min(y0 + j, y1);
-    int iy;
-    float v0, v1;
-    interp_coord(y, sbin, &iy, &v0, &v1);
-    wy[j] = (iy == cy) ? v1 : v0;
-    yc[j] = min(y, h - 2);
-  }
   float *acc = s_acc + tid;
   const int2 *grid = grid_arena + L.grid_off;
-  for (int x = x0; x <= x1; x++) {
-    int ix;
-    float v0, v1;
-    interp_coord(x, sbin, &ix, &v0, &v1);
-    const float wx = (ix == cx) ? v1 : v0;
-    const int2 *col = grid + (long long)min(x, w - 2) * h;
-    int2 g[JM];
-#pragma unroll
-    for (int j = 0; j < JM; j++)
-      if (j < ny) g[j] = col[yc[j]];
-#pragma unroll
-    for (int j = 0; j < JM; j++)
-      if (j < ny) acc[g[j].y * T * T] += wx * wy[j] * __int_as_float(g[j].x);
-  }
+  if (2 * sbin <= JM / 2)
+    hog_vote_cell<JM / 2>(grid, h, w, sbin, cx, cy, x0, x1, y0, y1, ny, acc);
+  else
+    hog_vote_cell<JM>(grid, h, w, sbin, cx, cy, x0, x1, y0, y1, ny, acc);
   const long long BSQ = (long long)L.b0 * L.b1;
   float *hist = hist_arena + L.hist_off + (long long)cx * L.b0 + cy;
   float nrm = 0.0f;

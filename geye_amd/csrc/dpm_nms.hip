@@ -418,10 +418,6 @@ constexpr int kNmsCompactBlock = 1024;  // positions per workgroup of the per-ro
 // kNmsCompactBlock positions (4 consecutive per thread) into blocksum; positions past the
 // true count add nothing, so a grid over a host bound counts exactly.
 
-__device__ inline bool nms_stays(int t, int na, int W, const int *alive, const int *sup) {
-  return t >= W && t < na && sup[alive[t]] == kNoSup;
-}
-
 __global__ __launch_bounds__(256) void k_nms_post(const int *__restrict__ alive,
                                                   const int *__restrict__ nalive,
                                                   const int *__restrict__ sup,
@@ -433,16 +429,30 @@ __global__ __launch_bounds__(256) void k_nms_post(const int *__restrict__ alive,
   const int na = *nalive;
   const int W = min(na, kNmsWindow);
   const int t0 = blockIdx.x * kNmsCompactBlock + threadIdx.x * 4;
+  // the four positions' loads go out together at each step of the chain
+  int r[4], su[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) r[k] = t0 + k < na ? alive[t0 + k] : -1;
+#pragma unroll
+  for (int k = 0; k < 4; k++) su[k] = r[k] >= 0 ? sup[r[k]] : kNoSup;
   int c = 0;
+  bool ret[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    const int t = t0 + k;
-    if (t >= na) break;
-    const int r = alive[t];
-    if (t >= W && sup[r] == kNoSup)
-      c++;
-    else if (rank_of_flat)
-      rank_of_flat[cand[sbox[r].idx]] = -1;
+    const bool stay = r[k] >= 0 && t0 + k >= W && su[k] == kNoSup;
+    c += stay ? 1 : 0;
+    ret[k] = r[k] >= 0 && !stay;
+  }
+  if (rank_of_flat) {
+    int ci[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) ci[k] = ret[k] ? sbox[r[k]].idx : -1;
+    int fl[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) fl[k] = ci[k] >= 0 ? cand[ci[k]] : -1;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (fl[k] >= 0) rank_of_flat[fl[k]] = -1;
   }
   for (int m = 1; m < 64; m <<= 1) c += __shfl_xor(c, m);
   if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
@@ -461,11 +471,16 @@ __global__ __launch_bounds__(256) void k_nms_compact(const int *__restrict__ ali
   const int na = *nalive;
   const int W = min(na, kNmsWindow);
   const int t0 = blockIdx.x * kNmsCompactBlock + threadIdx.x * 4;
+  int r[4], su[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) r[k] = t0 + k >= W && t0 + k < na ? alive[t0 + k] : -1;
+#pragma unroll
+  for (int k = 0; k < 4; k++) su[k] = r[k] >= 0 ? sup[r[k]] : 0;
   bool f[4];
   int c = 0;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    f[k] = nms_stays(t0 + k, na, W, alive, sup);
+    f[k] = r[k] >= 0 && su[k] == kNoSup;
     c += f[k] ? 1 : 0;
   }
   // inclusive scan of c over the wave, then the preceding waves' totals
@@ -481,7 +496,7 @@ __global__ __launch_bounds__(256) void k_nms_compact(const int *__restrict__ ali
   for (int w = 0; w < wave; w++) pos += s_w[w];
 #pragma unroll
   for (int k = 0; k < 4; k++)
-    if (f[k]) alive_out[pos++] = alive[t0 + k];
+    if (f[k]) alive_out[pos++] = r[k];
 }
 
 
