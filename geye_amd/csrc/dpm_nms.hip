@@ -372,16 +372,31 @@ __global__ __launch_bounds__(256) void k_nms_scatter(const NmsBox *__restrict__ 
   if (tid == 255) s_beg[nmaps] = s_part[255];
   __syncthreads();
   const int total = s_beg[nmaps];
-  const int stride = 256 * gridDim.y;
+  // each wave takes one contiguous chunk of the cells laid end to end (lanes consecutive,
+  // so the rank loads stay coalesced) and crosses only the maps that chunk spans; the
+  // chunk's first map comes from a binary search of the offsets
   constexpr int B = 4;  // cells per batch: their rank loads are in flight together
+  const int lane = tid & 63;
+  const int nw = 4 * gridDim.y, wv = blockIdx.y * 4 + (tid >> 6);
+  const int chunk = (total + nw * 64 * B - 1) / (nw * 64 * B) * (64 * B);
+  const int c0 = wv * chunk, c1 = min(total, c0 + chunk);
   int k = 0;
-  for (int base = blockIdx.y * 256 + tid; base < total; base += B * stride) {
+  {
+    int lo = 0, hi = nmaps - 1;  // last map with s_beg[k] <= c0
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_beg[mid] <= c0) lo = mid; else hi = mid - 1;
+    }
+    k = lo;
+  }
+  const int stride = 64;
+  for (int base = c0 + lane; base < c1; base += B * stride) {
     int kk[B], xs[B], ys[B], rr[B];
 #pragma unroll
     for (int u = 0; u < B; u++) {
       const int idx = base + u * stride;
       rr[u] = -1;
-      if (idx < total) {
+      if (idx < c1) {
         while (s_beg[k + 1] <= idx) k++;
         const int off = idx - s_beg[k];
         const int h = s_h[k];
