@@ -51,6 +51,70 @@ def conv_flops(plan: dict, model, level_valid) -> dict:
     return out
 
 
+def pyramid_images(W: int, H: int, interval: int, LEN: int) -> list:
+    """(source px, image px) of every resized pyramid image: levels below interval come from
+    the frame at 1/sc^l, levels from 2*interval on halve the image interval levels up
+    (the reference's resize rounds, (int)(s*scale + 0.5))."""
+    sc = 2.0 ** (1.0 / interval)
+    img = {}
+    out = []
+    for l in range(interval):
+        s = 1.0 / sc ** l
+        img[l] = (int(H * s + 0.5), int(W * s + 0.5))
+        out.append((W * H, img[l][0] * img[l][1]))
+    for l in range(interval, min(2 * interval, LEN)):
+        img[l] = img[l - interval]  # the same images, HOG at sbin instead of sbin / 2
+    for l in range(2 * interval, LEN):
+        ph, pw = img[l - interval]
+        img[l] = (int(ph * 0.5 + 0.5), int(pw * 0.5 + 0.5))
+        out.append((ph * pw, img[l][0] * img[l][1]))
+    return out
+
+
+def stage_rooflines(plan, model, level_valid, W, H, per_frame: dict, precision: str,
+                    conv_flops_total: float) -> dict:
+    """Per-stage algorithmic work, ideal time at the MI355X peak and the fraction reached
+    (ideal / measured), with the frame fraction (sum of ideals / sum of stage times)."""
+    interval, LEN = model.interval, int(plan["LEN"])
+    imgs = pyramid_images(W, H, interval, LEN)
+    feat_cells = sum(int(plan["feat_h"][l]) * int(plan["feat_w"][l]) for l in range(LEN))
+    elt = 4 if precision == "fp32" else 2
+    root_cells = part_cells = 0
+    for l in range(interval, LEN):
+        if level_valid[l]:
+            for (h, w, _) in model.root_size.tolist():
+                root_cells += (int(plan["feat_h"][l]) - h + 1) * (int(plan["feat_w"][l]) - w + 1)
+            for (h, w, _) in model.part_size.tolist():
+                L = l - interval
+                part_cells += (int(plan["feat_h"][L]) - h + 1) * (int(plan["feat_w"][L]) - w + 1)
+    nparts = int(model.numpart[0])
+    peak = MI355X_PEAK[precision]
+    conv_tag = {"fp32": "conv_f32_", "bf16": "conv_bf16_", "fp16": "conv_fp16_"}[precision]
+    stages = {
+        "resize": ("hbm", 12 * sum(a + b for a, b in imgs), ["resize"]),
+        "hog": ("hbm", 12 * sum(b for _, b in imgs) + 32 * elt * feat_cells,
+                ["hog_grad", "hog_hist", "hog_feat"]),
+        "conv": ("mfma" if precision != "fp32" else "fp32", conv_flops_total,
+                 [k for k in per_frame if k.startswith(conv_tag)]),
+        "dt": ("hbm", 16 * part_cells, ["dt_cols", "dt_rows"]),
+        "score": ("hbm", 4 * root_cells * (1 + nparts), ["score", "compact"]),
+    }
+    out, ideal_sum, meas_sum = {}, 0.0, 0.0
+    for name, (bound, work, keys) in stages.items():
+        ms = sum(per_frame.get(k, 0.0) for k in keys)
+        ideal = work / (HBM_PEAK if bound == "hbm" else peak) * 1e3
+        out[name] = {"bound": bound, "work": int(work), "unit": "B" if bound == "hbm" else "FLOP",
+                     "ms": round(ms, 4), "ideal_ms": round(ideal, 4),
+                     "frac": round(ideal / ms, 4) if ms > 0 else None}
+        ideal_sum += ideal
+        meas_sum += ms
+    nms_ms = sum(v for k, v in per_frame.items() if k.startswith("nms"))
+    out["nms"] = {"bound": "latency", "ms": round(nms_ms, 4)}
+    out["frame"] = {"ideal_ms": round(ideal_sum, 4), "stage_ms": round(meas_sum + nms_ms, 4),
+                    "frac": round(ideal_sum / (meas_sum + nms_ms), 4)}
+    return out
+
+
 def cpu_baseline(model, frame, threads: int) -> dict:
     """The oracle (CPU restatement of the reference path) on one frame of the workload."""
     from oracle import oracle as O
@@ -354,6 +418,8 @@ def main() -> int:
         "mfma_autotune": tuning,
         "detection_agreement_vs_fp32": iou_agreement(plain["boxes"], single["boxes"]),
         "kernel_ms_per_frame": {k: round(v, 4) for k, v in sorted(per_frame.items())},
+        "stage_roofline": stage_rooflines(plan, model, level_valid, W, H, per_frame,
+                                          args.precision, float(sum(fl.values()))),
     }
     if not args.no_extra:
         line["two_model_batch"] = two_model_batch(geye_amd, ctx, synth, head["slots"])

@@ -398,7 +398,8 @@ __global__ __launch_bounds__(256) void k_hog_votes(const HogLevel *__restrict__ 
 // HOG features: 18 contrast-sensitive, 9 insensitive, 4 texture (CT/featurepyramid.cpp.new2:
 // 407-492).  The four normalisers are 1/sqrt((double)sum + 1e-4) rounded to float and the
 // texture features 0.2357*t in double, as the reference's promotions compute them.
-// Writes [31][F1][F0] float and, when featb_arena is non-null, [F1][F0][32] bf16.
+// Writes [31][F1][F0] float when feat_arena is non-null and, when featb_arena is non-null,
+// [F1][F0][32] 16-bit (the 16-bit forms read only the latter).
 // ---------------------------------------------------------------------------------------
 __device__ inline float min_2(float x) { return (x < 0.2f) ? x : 0.2f; }
 
@@ -445,7 +446,6 @@ __global__ __launch_bounds__(256) void k_hog_feat(const HogLevel *__restrict__ l
   };
   const float n1 = nf(1, 1), n2 = nf(1, 0), n3 = nf(0, 1), n4 = nf(0, 0);
   const float *src = hist_arena + L.hist_off + (long long)(x + 1) * b0 + (y + 1);
-  float *dst = feat_arena + L.feat_off + e;
   float fv[31];
   float t1 = 0.0f, t2 = 0.0f, t3 = 0.0f, t4 = 0.0f;
   float hs[18];
@@ -468,8 +468,11 @@ __global__ __launch_bounds__(256) void k_hog_feat(const HogLevel *__restrict__ l
   fv[28] = (float)(0.2357 * (double)t2);
   fv[29] = (float)(0.2357 * (double)t3);
   fv[30] = (float)(0.2357 * (double)t4);
+  if (feat_arena) {
+    float *dst = feat_arena + L.feat_off + e;
 #pragma unroll
-  for (int o = 0; o < 31; o++) dst[o * OSQ] = fv[o];
+    for (int o = 0; o < 31; o++) dst[o * OSQ] = fv[o];
+  }
   if (featb_arena) {
     // 32 16-bit channels (the last one 0) as four 16-byte stores; featb_off is a multiple
     // of 32 elements, so every cell starts on a 64-byte boundary

@@ -645,9 +645,12 @@ int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, int hk
   if (hkind) CHK(ensure(P.d_featb, 2 * P.featb_total));
   if (P.hogf_n > 0) {
     KTimer t(c, "hog_feat");
+    // the 16-bit forms convolve the channel-last copy only, so the FP32 features are not
+    // written for them
     launch_hog_feat(s, hog + P.hog16_n + P.hog8_n, P.hogf_n, P.hog_feat_blocks,
-                    dp<float>(P.d_hist), dp<float>(P.d_norm), dp<float>(P.d_feat),
-                    hkind ? P.d_featb.p : nullptr, hkind);
+                    dp<float>(P.d_hist), dp<float>(P.d_norm),
+                    hkind ? nullptr : dp<float>(P.d_feat), hkind ? P.d_featb.p : nullptr,
+                    hkind);
   }
   HIPCHK(hipGetLastError());
   return GEYE_OK;
