@@ -114,6 +114,11 @@ struct DtJob {
   int G;                       // lanes per line
   int neg;                     // negate the source on load
   float a, b;
+  // outputs the windowed DT writes (the 16-bit forms): 0 every output; 1 (columns pass) M
+  // everywhere, the argmin on rows y == ay mod 2; 2 (rows pass) on rows y == ay - 1 mod 2
+  // M at columns x == ax - 1 mod 2, on rows y == ay mod 2 the argmin at x == ax mod 2 --
+  // exactly the cells the scoring and the part boxes read
+  int sel, ax, ay;
 };
 
 struct ScoreMap {              // one (valid level, component) root score map

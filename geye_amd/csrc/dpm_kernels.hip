@@ -832,20 +832,30 @@ __device__ inline void dt_window(const DtJob &J, const float *s_f, const unsigne
   // ROWS (lines interleaved in memory): lanes take consecutive lines of one position, so the
   // stores are contiguous; otherwise consecutive positions of one line
   const int NL = 256 / J.G;
+  // output selection (DtJob::sel): rows pass, each line keeps every other position, and
+  // writes M (on rows y == ay - 1 mod 2) or the argmin (on rows y == ay mod 2)
+  const bool sel_rows = ROWS && J.sel == 2, sel_cols = !ROWS && J.sel == 1;
   int l, d, dstep, lstep = 0;
+  int par = 0, kind = 0;  // rows pass: first position and output kind of this lane's line
   if (ROWS) {
     l = tid & (NL - 1);
     d = tid / NL;
     dstep = 256 / NL;
+    if (sel_rows) {
+      kind = (J.line0 + l - J.ay + 1) & 1;  // 0: M, 1: argmin
+      par = (J.ax - 1 + kind) & 1;
+    }
   } else {
     l = tid / n;
     d = tid - l * n;
     lstep = 256 / n;
     dstep = 256 - lstep * n;
   }
-  while (ROWS ? d < n : l < nl) {
-    if (!ROWS || l < nl) {
-      const unsigned *w = s_key + l * LSk + kDtHalo - T + d;
+  const int dlim = sel_rows ? (n + 1) / 2 : n;
+  while (ROWS ? d < dlim : l < nl) {
+    const int dd = sel_rows ? 2 * d + par : d;
+    if (!ROWS || (l < nl && dd < n)) {
+      const unsigned *w = s_key + l * LSk + kDtHalo - T + dd;
       unsigned best = 0xffffffffu;
 #pragma unroll
       for (int i = 0; i < NP; i++) {
@@ -853,11 +863,10 @@ __device__ inline void dt_window(const DtJob &J, const float *s_f, const unsigne
         const unsigned long long k = (((unsigned long long)hi << 32) | w[2 * i]) + q[i];
         best = min(best, min((unsigned)k, (unsigned)(k >> 32)));
       }
-      const int s = d - T + (int)(best & 63u);
-      const float M = dt_cost(s_f[l * n + s], a, b, d - s);
-      const long long o = (long long)l * J.line_stride + (long long)d * J.elem_stride;
-      dst[o] = M;
-      pdst[o] = s;
+      const int s = dd - T + (int)(best & 63u);
+      const long long o = (long long)l * J.line_stride + (long long)dd * J.elem_stride;
+      if (!sel_rows || kind == 0) dst[o] = dt_cost(s_f[l * n + s], a, b, dd - s);
+      if ((!sel_rows || kind == 1) && (!sel_cols || ((dd - J.ay) & 1) == 0)) pdst[o] = s;
     }
     d += dstep;
     if (!ROWS) {

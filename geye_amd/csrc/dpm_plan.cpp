@@ -180,7 +180,7 @@ static int dt_pick_G_window(int n) {
 
 static void add_dt_jobs(Plan &P, std::vector<DtJob> &jobs, long long src_off, long long dst_off,
                         int n, int nlines, int line_stride, int elem_stride, int neg, float a,
-                        float b, bool window = false) {
+                        float b, bool window = false, int sel = 0, int ax = 0, int ay = 0) {
   dt_nodes_for(P, n);
   const auto tabs = P.dt_len_tab[n];
   const int G = window ? dt_pick_G_window(n) : dt_pick_G(n);
@@ -202,6 +202,9 @@ static void add_dt_jobs(Plan &P, std::vector<DtJob> &jobs, long long src_off, lo
     J.neg = neg;
     J.a = a;
     J.b = b;
+    J.sel = sel;
+    J.ax = ax;
+    J.ay = ay;
     jobs.push_back(J);
   }
 }
@@ -593,10 +596,11 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
         // pass 1: columns (along y, n = P0) with (ay, by); source negated (inverse_Q)
         add_dt_jobs(P, P.dt_pass1, P.map_off[pslot], dto, P0, P1, P0, 1, 1, df[2], df[3]);
         add_dt_jobs(P, P.dt_pass1w, P.map_off[pslot], dto, P0, P1, P0, 1, 1, df[2], df[3],
-                    true);
+                    true, 1, q.ax, q.ay);
         // pass 2: rows (along x, n = P1) with (ax, bx)
         add_dt_jobs(P, P.dt_pass2, dto, dto, P1, P0, 1, P0, 0, df[0], df[1]);
-        add_dt_jobs(P, P.dt_pass2w, dto, dto, P1, P0, 1, P0, 0, df[0], df[1], true);
+        add_dt_jobs(P, P.dt_pass2w, dto, dto, P1, P0, 1, P0, 0, df[0], df[1], true, 2, q.ax,
+                    q.ay);
         P.sparts.push_back(q);
         dto += (long long)P0 * P1;
       }

@@ -297,6 +297,34 @@ def test_detect_mfma_scores_vs_fp64(ctx, prec):
     assert err.max() < 0.02
 
 
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_mfma_part_boxes_match_fp32(ctx, prec):
+    """The 16-bit forms' part boxes come from their own DT argmins: candidate rows matched
+    to the FP32 rows by root box, component and scale carry the same part boxes except
+    where a near-tie of the 16-bit scores moved an argmin (>= 97% of the part boxes)."""
+    say("part boxes", prec, "640x480 synthetic car")
+    frame = synth.make_frame(640, 480, seed=5)
+    gm = ctx.model(synth.synthetic_car())
+    g16 = gm.get_boxes(frame, thresh=-0.5, precision=prec)
+    g32 = gm.get_boxes(frame, thresh=-0.5, precision="fp32")
+    GL = g32.shape[1]
+    npart = (GL - 7) // 4
+    key = lambda r: (tuple(r[0:4].tolist()), float(r[GL - 3]), float(r[GL - 1]))
+    ref = {key(r): r for r in g32}
+    same = total = 0
+    for r in g16:
+        o = ref.get(key(r))
+        if o is None:
+            continue
+        pb = r[4:4 + 4 * npart].reshape(npart, 4)
+        po = o[4:4 + 4 * npart].reshape(npart, 4)
+        same += int(np.sum(np.all(pb == po, axis=1)))
+        total += npart
+    say(prec, "matched part boxes", same, "of", total)
+    assert total > 1000
+    assert same >= 0.97 * total
+
+
 @pytest.mark.parametrize("which,W,H,thresh", [("car", 640, 480, -0.5), ("car", 640, 480, -1.2),
                                               ("synthetic", 480, 360, -0.5)])
 def test_accumulated_score_matches_oracle(ctx, car_model, which, W, H, thresh):
