@@ -14,12 +14,19 @@
 // l%16, the same channels; accumulator rows 4*(l/16)..+4 of column l%16.
 #include "dpm_mfma.h"
 
+#include <algorithm>
+
 namespace geye {
 
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(8))) _Float16 f16x8;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
+
+// 4 consecutive output rows, stored as one 16-byte write at 4-byte alignment
+struct __attribute__((packed, aligned(4))) RowQuad {
+  float v0, v1, v2, v3;
+};
 
 template <int KIND>
 __device__ inline f32x4 mfma16(u32x4 a, u32x4 b, f32x4 c) {
@@ -31,118 +38,141 @@ __device__ inline f32x4 mfma16(u32x4 a, u32x4 b, f32x4 c) {
                                                   __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
 }
 
+// Feature tile in LDS: a ring of AX = 4 + fw - 1 column slots, feature column gx in slot
+// gx mod AX, each slot AY = 64 + fh - 1 cells of 4 16-byte chunks.  The chunk index of cell c
+// (c = slot * AY + row) is XOR-swizzled by (c >> 1) & 3, so the 8 consecutive cells that 8
+// lanes read in one LDS cycle fall on 8 distinct 16-byte bank groups.
+__device__ inline int conv_cell_chunk(int cell, int part) { return cell * 4 + (part ^ ((cell >> 1) & 3)); }
+
+// Global chunk of feature column gx, chunk rem (= row * 4 + part) of the tile's rows; the
+// load is unconditional from a clamped in-level address and `valid` says whether it is real.
+__device__ inline u32x4 conv_feat_load(const u32x4 *__restrict__ F, const MfmaTile &t, int gx,
+                                       int rem, bool &valid) {
+  const int gy = t.y0 + (rem >> 2);
+  valid = gx < t.F1 && gy < t.F0;
+  const int cx = min(gx, t.F1 - 1), cy = min(gy, t.F0 - 1);
+  return F[(unsigned)((cx * t.F0 + cy) * 4 + (rem & 3))];   // a level holds < 2^31 chunks
+}
+
+// Registers per thread: PN chunks of the next tile's 4 new feature columns, PRE chunks of the
+// next tap column's filters.
+template <int NT> struct ConvRegs {
+  static constexpr int PN = 5;
+  static constexpr int PRE = NT == 3 ? 5 : 4;
+};
+
+// Stages all AX feature columns of tile t into the ring (8 loads in flight per thread; a
+// column holds at least 256 chunks, so one step of 256 crosses at most one column boundary).
+__device__ inline void conv_stage_tile(u32x4 *As, const u32x4 *__restrict__ feat,
+                                       const MfmaTile &t, int AX, int AY, int tid) {
+  const int CS = AY * 4, NA = AX * CS;
+  const u32x4 *F = feat + t.featb_off / 8;     // featb_off counts 16-bit elements
+  int c = 0, rem = tid, slot = t.x0 % AX;
+  for (int i0 = 0; i0 < NA; i0 += 8 * 256) {
+    u32x4 v[8];
+    bool ok[8];
+    int cc[8], rr[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      cc[u] = slot;
+      rr[u] = rem;
+      v[u] = conv_feat_load(F, t, t.x0 + c, rem, ok[u]);
+      rem += 256;
+      if (rem >= CS) {
+        rem -= CS;
+        c++;
+        slot = slot + 1 == AX ? 0 : slot + 1;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (i0 + u * 256 + tid < NA)
+        As[conv_cell_chunk(cc[u] * AY + (rr[u] >> 2), rr[u] & 3)] =
+            ok[u] ? v[u] : (u32x4){0u, 0u, 0u, 0u};
+  }
+}
+
+// Loads the 4 feature columns tile tn adds to its predecessor (tn's relative columns
+// AX-4..AX-1) into registers.  live == false still issues the loads, all from one cell, so
+// that the code around them stays free of branches and the wait counts stay exact.
+template <int PN>
+__device__ inline void conv_fetch_new(u32x4 (&nf)[PN], unsigned &vmask,
+                                      const u32x4 *__restrict__ feat, const MfmaTile &tn,
+                                      bool live, int AX, int CS, int tid) {
+  const u32x4 *F = feat + tn.featb_off / 8;
+  int c = 0, rem = tid;
+  vmask = 0;
+#pragma unroll
+  for (int k = 0; k < PN; k++) {
+    bool v;
+    nf[k] = conv_feat_load(F, tn, live ? tn.x0 + AX - 4 + c : tn.x0, live ? rem : 0, v);
+    vmask |= (unsigned)(v && live) << k;
+    rem += 256;
+    if (rem >= CS) { rem -= CS; c++; }
+  }
+}
+
+// Writes those columns into the slots of the current tile's relative columns 0..3 (sx is the
+// slot of its relative column 0): feature column x0 + AX + k lives in slot (x0 + k) mod AX.
+template <int PN>
+__device__ inline void conv_put_new(u32x4 *As, const u32x4 (&nf)[PN], unsigned vmask, int sx,
+                                    int AX, int AY, int tid) {
+  const int CS = AY * 4;
+  int c = 0, rem = tid;
+#pragma unroll
+  for (int k = 0; k < PN; k++) {
+    if (k * 256 + tid < 4 * CS) {
+      int s2 = sx + c;
+      if (s2 >= AX) s2 -= AX;
+      As[conv_cell_chunk(s2 * AY + (rem >> 2), rem & 3)] =
+          ((vmask >> k) & 1) ? nf[k] : (u32x4){0u, 0u, 0u, 0u};
+    }
+    rem += 256;
+    if (rem >= CS) { rem -= CS; c++; }
+  }
+}
+
+// One tap column for one wave: fh taps x 4 M-tiles x NT N-tiles.  Bcol is the column's
+// filters in LDS ([fh][NT][4 chunks][16 filters]); cell0 the ring cell of the wave's row r.
+// Two register sets: tap yp + 1's fragments are read while tap yp's MFMAs run (the
+// scheduling barriers keep the compiler from sinking the reads next to their use); the reads
+// are unconditional (the last ones repeat tap fh - 1) so the wait counts are fixed.
 template <int KIND, int NT>
-__global__ __launch_bounds__(256) void k_conv_mfma(const MfmaTile *__restrict__ tiles, int fh,
-                                                   int fw, const u32x4 *__restrict__ feat,
-                                                   const u32x4 *__restrict__ filt,
-                                                   const long long *__restrict__ map_off,
-                                                   const int2 *__restrict__ map_dims,
-                                                   float *__restrict__ out) {
-  extern __shared__ u32x4 lds[];
-  // runs of 16 consecutive tiles (neighbouring columns of one level) per XCD
-  constexpr int X = 8, C = 16, GRP = X * C;
-  int bx = blockIdx.x;
-  if (bx < ((int)gridDim.x / GRP) * GRP) {
-    const int g = bx / GRP, l = bx - g * GRP;
-    bx = g * GRP + (l % X) * C + l / X;
+__device__ inline void conv_column(f32x4 (&acc)[4][NT], const u32x4 *Bcol, const u32x4 *As,
+                                   int cell0, int fh, int r, int chunk) {
+  u32x4 b0[NT], a0[4], b1[NT], a1[4];
+  auto load = [&](u32x4 *b, u32x4 *a, int yp) {
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) b[nt] = Bcol[((yp * NT + nt) * 4 + chunk) * 16 + r];
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) a[mt] = As[conv_cell_chunk(cell0 + mt * 16 + yp, chunk)];
+  };
+  auto mma = [&](const u32x4 *b, const u32x4 *a) {
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+      for (int nt = 0; nt < NT; nt++) acc[mt][nt] = mfma16<KIND>(a[mt], b[nt], acc[mt][nt]);
+  };
+  load(b0, a0, 0);
+  for (int yp = 0; yp < fh; yp += 2) {
+    load(b1, a1, min(yp + 1, fh - 1));
+    __builtin_amdgcn_sched_barrier(0);
+    mma(b0, a0);
+    __builtin_amdgcn_sched_barrier(0);
+    load(b0, a0, min(yp + 2, fh - 1));
+    __builtin_amdgcn_sched_barrier(0);
+    if (yp + 1 < fh) mma(b1, a1);
+    __builtin_amdgcn_sched_barrier(0);
   }
-  const MfmaTile t = tiles[bx];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int AX = 4 + fw - 1, AY = 64 + fh - 1;
-  u32x4 *As = lds;                               // [AX][AY][4 chunks]
-  u32x4 *Bs = lds + AX * AY * 4;                 // [2][fh][NT][4][16]
-  const int bstage = fh * NT * 64;               // 16-byte chunks per tap column
-  const u32x4 *F = feat + t.featb_off / 8;       // featb_off counts 16-bit elements
-  const u32x4 *Wg = filt + t.filt_off / 8;
+}
 
-  // stage the feature tile (zero outside the level), 4 loads in flight per thread
-  for (int i0 = tid; i0 < AX * AY * 4; i0 += 4 * 256) {
-    u32x4 v[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int i = i0 + u * 256;
-      const int cell = i >> 2, part = i & 3;
-      const int ax = cell / AY, ay = cell - ax * AY;
-      const int gx = t.x0 + ax, gy = t.y0 + ay;
-      v[u] = (u32x4){0u, 0u, 0u, 0u};
-      if (i < AX * AY * 4 && gx < t.F1 && gy < t.F0)
-        v[u] = F[((long long)gx * t.F0 + gy) * 4 + part];
-    }
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-      if (i0 + u * 256 < AX * AY * 4) As[i0 + u * 256] = v[u];
-  }
-  for (int i = tid; i < bstage; i += 256) Bs[i] = Wg[i];
-  __syncthreads();
-
-  f32x4 acc[4][NT];
-#pragma unroll
-  for (int mt = 0; mt < 4; mt++)
-#pragma unroll
-    for (int nt = 0; nt < NT; nt++) acc[mt][nt] = (f32x4){0.f, 0.f, 0.f, 0.f};
-
-  const int r = lane & 15, chunk = lane >> 4;
-  // the next tap column's filters are loaded into registers before this column's MFMAs and
-  // written to LDS after them, so their latency hides behind the arithmetic
-  constexpr int PRE = 8;
-  const bool in_regs = bstage <= PRE * 256;
-  for (int xp = 0; xp < fw; xp++) {
-    const int buf = xp & 1;
-    u32x4 pre[PRE];
-    const bool next = xp + 1 < fw;
-    const u32x4 *src = Wg + (long long)(xp + 1) * bstage;
-    u32x4 *dst = Bs + (buf ^ 1) * bstage;
-    if (next) {
-      if (in_regs) {
-#pragma unroll
-        for (int c = 0; c < PRE; c++) {
-          const int i = tid + c * 256;
-          if (i < bstage) pre[c] = src[i];
-        }
-      } else {
-        for (int i = tid; i < bstage; i += 256) dst[i] = src[i];
-      }
-    }
-    const u32x4 *Bb = Bs + buf * bstage;
-    const u32x4 *Ab = As + ((wave + xp) * AY) * 4 + chunk;
-    // two register sets: tap yp + 1's fragments are read while tap yp's MFMAs run (the
-    // scheduling barriers keep the compiler from sinking the reads next to their use)
-    u32x4 b0[NT], a0[4], b1[NT], a1[4];
-    auto load = [&](u32x4 *b, u32x4 *a, int yp) {
-#pragma unroll
-      for (int nt = 0; nt < NT; nt++) b[nt] = Bb[((yp * NT + nt) * 4 + chunk) * 16 + r];
-#pragma unroll
-      for (int mt = 0; mt < 4; mt++) a[mt] = Ab[(mt * 16 + r + yp) * 4];
-    };
-    auto mma = [&](const u32x4 *b, const u32x4 *a) {
-#pragma unroll
-      for (int mt = 0; mt < 4; mt++)
-#pragma unroll
-        for (int nt = 0; nt < NT; nt++) acc[mt][nt] = mfma16<KIND>(a[mt], b[nt], acc[mt][nt]);
-    };
-    // the reads are unconditional (the last ones repeat tap fh - 1) so that the wait
-    // counts before each set's MFMAs are fixed
-    load(b0, a0, 0);
-    for (int yp = 0; yp < fh; yp += 2) {
-      load(b1, a1, min(yp + 1, fh - 1));
-      __builtin_amdgcn_sched_barrier(0);
-      mma(b0, a0);
-      __builtin_amdgcn_sched_barrier(0);
-      load(b0, a0, min(yp + 2, fh - 1));
-      __builtin_amdgcn_sched_barrier(0);
-      if (yp + 1 < fh) mma(b1, a1);
-      __builtin_amdgcn_sched_barrier(0);
-    }
-    if (next && in_regs) {
-#pragma unroll
-      for (int c = 0; c < PRE; c++) {
-        const int i = tid + c * 256;
-        if (i < bstage) dst[i] = pre[c];
-      }
-    }
-    __syncthreads();
-  }
-
+// Writes one wave's outputs (column x0 + wave, 64 rows, the group's filters).  A lane holds 4
+// consecutive rows of one filter's map: one 16-byte store (4-byte aligned, which gfx950
+// global stores accept) unless the map ends inside them.
+template <int NT>
+__device__ inline void conv_store(const f32x4 (&acc)[4][NT], const MfmaTile &t, int wave, int r,
+                                  int chunk, const long long *__restrict__ map_off,
+                                  const int2 *__restrict__ map_dims, float *__restrict__ out) {
   const int x = t.x0 + wave;
 #pragma unroll
   for (int nt = 0; nt < NT; nt++) {
@@ -153,12 +183,124 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const MfmaTile *__restrict__ 
     if (x >= hw.y) continue;
     float *o = out + map_off[slot] + (long long)x * hw.x;
 #pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+      const int y = t.y0 + mt * 16 + chunk * 4;
+      if (y + 3 < hw.x) {
+        *(RowQuad *)(o + y) = RowQuad{acc[mt][nt][0], acc[mt][nt][1], acc[mt][nt][2],
+                                      acc[mt][nt][3]};
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          if (y + i < hw.x) o[y + i] = acc[mt][nt][i];
+      }
+    }
+  }
+}
+
+// Per-class kernel, persistent: workgroup b of G takes the contiguous tile range
+// [b*n/G, (b+1)*n/G) of a list ordered along x within (level, group, row of tiles).  When the
+// next tile is the x + 4 neighbour, its 4 new feature columns are loaded into registers during
+// this tile's first tap column and written into the ring slots this tile has finished with
+// (relative columns 0..3, last read at tap column min(3, fw - 1)); the next tile's first
+// filter column is loaded by this tile's last tap column.  A tile that does not continue its
+// predecessor stages its whole feature tile first.  Waves own one output column each:
+// 4 M-tiles x NT N-tiles of accumulators.
+template <int KIND, int NT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv_mfma(
+    const MfmaTile *__restrict__ tiles, int ntiles, int fh, int fw,
+    const u32x4 *__restrict__ feat, const u32x4 *__restrict__ filt,
+    const long long *__restrict__ map_off, const int2 *__restrict__ map_dims,
+    float *__restrict__ out) {
+  extern __shared__ u32x4 lds[];
+  constexpr int PN = ConvRegs<NT>::PN, PRE = ConvRegs<NT>::PRE;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int AX = 4 + fw - 1, AY = 64 + fh - 1, CS = AY * 4, NA = AX * CS;
+  u32x4 *As = lds;                               // [AX slots][AY][4 chunks], swizzled
+  u32x4 *Bs = lds + NA;                          // [2][fh][NT][4][16]
+  const int bstage = fh * NT * 64;               // 16-byte chunks per tap column
+  const bool in_regs = bstage <= PRE * 256;
+  const bool ring = 4 * CS <= PN * 256;
+
+  int bx = blockIdx.x;
+  if ((int)gridDim.x == ntiles) {
+    // one tile per workgroup: runs of 16 consecutive tiles (neighbouring columns, sharing
+    // feature columns) go to one XCD (workgroup b runs on XCD b % 8)
+    constexpr int X = 8, C = 16, GRP = X * C;
+    if (bx < (ntiles / GRP) * GRP) {
+      const int g = bx / GRP, l = bx - g * GRP;
+      bx = g * GRP + (l % X) * C + l / X;
+    }
+  }
+  const int tb = (int)((long long)bx * ntiles / gridDim.x);
+  const int te = (int)((long long)(bx + 1) * ntiles / gridDim.x);
+  if (tb >= te) return;
+  const int r = lane & 15, chunk = lane >> 4;
+
+  MfmaTile t = tiles[tb];
+  bool have_feat = false, have_filt = false;
+  int bpar = 0;                                  // filter buffer of this tile's tap column 0
+  for (int ti = tb; ti < te; ti++) {
+    const bool has_next = ti + 1 < te;
+    const MfmaTile tn = tiles[has_next ? ti + 1 : ti];
+    const bool cont = has_next && ring && tn.featb_off == t.featb_off && tn.y0 == t.y0 &&
+                      tn.x0 == t.x0 + 4;
+    const u32x4 *Wg = filt + t.filt_off / 8;   // offsets count 16-bit elements
+    const int sx = t.x0 % AX;                  // slot of the tile's relative column 0
+    // the previous tile ended with a barrier after its last reads of As
+    if (!have_feat) conv_stage_tile(As, feat, t, AX, AY, tid);
+    if (!have_filt)
+      for (int i = tid; i < bstage; i += 256) Bs[bpar * bstage + i] = Wg[i];
+    if (!have_feat || !have_filt) __syncthreads();
+
+    f32x4 acc[4][NT];
+#pragma unroll
     for (int mt = 0; mt < 4; mt++)
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int y = t.y0 + mt * 16 + chunk * 4 + i;
-        if (y < hw.x) o[y] = acc[mt][nt][i];
-      }
+      for (int nt = 0; nt < NT; nt++) acc[mt][nt] = (f32x4){0.f, 0.f, 0.f, 0.f};
+
+    u32x4 nf[PN];
+    unsigned vmask = 0;
+    auto step = [&](int xp, bool first) {
+      const int buf = (bpar + xp) & 1;
+      // the next filter column (this tile's xp + 1, or the next tile's column 0) is loaded into
+      // registers before this column's MFMAs and written to LDS after them (the register copy
+      // runs even with nothing to copy -- last column of the last tile -- into the idle
+      // buffer, and alongside the plain copy, writing the same values, when a column exceeds
+      // the registers: branch-free, so the wait counts stay exact)
+      const bool more = xp + 1 < fw;
+      const u32x4 *src = more ? Wg + (long long)(xp + 1) * bstage : filt + tn.filt_off / 8;
+      u32x4 *dst = Bs + (buf ^ 1) * bstage;
+      u32x4 pre[PRE];
+#pragma unroll
+      for (int c = 0; c < PRE; c++) pre[c] = src[(unsigned)min(tid + c * 256, bstage - 1)];
+      if (!in_regs && (more || has_next))
+        for (int i = tid; i < bstage; i += 256) dst[i] = src[i];
+      // issued after the filter loads, the next tile's feature loads may stay in flight past
+      // this column's wait for the filters
+      if (first) conv_fetch_new<PN>(nf, vmask, feat, tn, cont, AX, CS, tid);
+      int slot = sx + wave + xp;
+      if (slot >= AX) slot -= AX;
+      conv_column<KIND, NT>(acc, Bs + buf * bstage, As, slot * AY + r, fh, r, chunk);
+#pragma unroll
+      for (int c = 0; c < PRE; c++) dst[(unsigned)min(tid + c * 256, bstage - 1)] = pre[c];
+      __syncthreads();
+    };
+    // tap column xp reads relative columns xp..xp+3, so from tap column 4 on (or after the
+    // last one) the slots of relative columns 0..3 take the next tile's new columns
+    step(0, true);
+    for (int xp = 1; xp < fw; xp++) {
+      if (xp == 4 && cont) conv_put_new<PN>(As, nf, vmask, sx, AX, AY, tid);
+      step(xp, false);
+    }
+    if (fw <= 4 && cont) {
+      conv_put_new<PN>(As, nf, vmask, sx, AX, AY, tid);
+      __syncthreads();
+    }
+    conv_store<NT>(acc, t, wave, r, chunk, map_off, map_dims, out);
+    have_feat = cont;
+    have_filt = has_next;
+    bpar = (bpar + fw) & 1;
+    t = tn;
   }
 }
 
@@ -289,20 +431,34 @@ size_t mfma_lds_bytes(int fh, int fw, int nt) {
   return (size_t)16 * ((size_t)(4 + fw - 1) * (64 + fh - 1) * 4 + 2 * (size_t)fh * nt * 64);
 }
 
+// Launches the persistent kernel with every workgroup resident at once (its tile ranges are
+// static).  With fewer than 4 tiles per resident workgroup a static split would leave the
+// last round uneven, so each tile gets its own workgroup and the hardware balances them.
 template <int KIND, int NT>
 static int launch_one(hipStream_t s, const MfmaTile *tiles, int ntiles, int fh, int fw,
                       const void *feat, const void *filt, const long long *map_off,
                       const int2 *map_dims, float *out) {
   const size_t lds = mfma_lds_bytes(fh, fw, NT);
   static bool attr_set = false;
+  static int ncu = 0;
   if (!attr_set) {
     if (hipFuncSetAttribute((const void *)k_conv_mfma<KIND, NT>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
       return -1;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return -1;
     attr_set = true;
   }
-  hipLaunchKernelGGL((k_conv_mfma<KIND, NT>), dim3(ntiles), dim3(256), lds, s, tiles, fh, fw,
-                     (const u32x4 *)feat, (const u32x4 *)filt, map_off, map_dims, out);
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_conv_mfma<KIND, NT>,
+                                                   256, lds) != hipSuccess || per_cu < 1)
+    return -1;
+  const int full = per_cu * ncu;
+  const int blocks = ntiles <= 4 * full ? ntiles : full;
+  hipLaunchKernelGGL((k_conv_mfma<KIND, NT>), dim3(blocks), dim3(256), lds, s, tiles, ntiles, fh,
+                     fw, (const u32x4 *)feat, (const u32x4 *)filt, map_off, map_dims, out);
   return 0;
 }
 

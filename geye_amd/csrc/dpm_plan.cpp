@@ -474,8 +474,10 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
       for (int g = 0; g < g16; g++) {
         int Hm, Wm;
         group_extent(g * gsz, gsz, Hm, Wm);
-        for (int x0 = 0; x0 < Wm; x0 += 4)
-          for (int y0 = 0; y0 < Hm; y0 += 64) {
+        // rows of tiles along x: the MFMA kernel carries the shared feature columns from
+        // one tile to the next
+        for (int y0 = 0; y0 < Hm; y0 += 64)
+          for (int x0 = 0; x0 < Wm; x0 += 4) {
             MfmaTile t;
             t.featb_off = featb_off[fl];
             t.F0 = FL.F0;

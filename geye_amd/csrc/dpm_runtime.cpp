@@ -1832,8 +1832,8 @@ int geye_fconv(geye_ctx *c, const float *feat, int F0, int F1, const float *filt
       mdims.push_back(k < n ? int2{H, W} : int2{0, 0});
     }
     for (int g = 0; g < ng; g++)
-      for (int x0 = 0; x0 < W; x0 += 4)
-        for (int y0 = 0; y0 < H; y0 += 64)
+      for (int y0 = 0; y0 < H; y0 += 64)
+        for (int x0 = 0; x0 < W; x0 += 4)
           tiles.push_back(MfmaTile{0, F0, F1, x0, y0, (long long)g * w * h * nt * 512, g * gsz,
                                    std::min(gsz, n - g * gsz)});
     CHK(upload(dfe, fb, c->stream));

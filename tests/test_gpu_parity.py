@@ -264,10 +264,12 @@ def test_nms_matches_oracle(ctx, n, ties):
 
 @pytest.mark.parametrize("prec,ulp", [("bf16", 2.0 ** -8), ("fp16", 2.0 ** -10)])
 @pytest.mark.parametrize("h,w,n", [(6, 6, 48), (6, 6, 8), (8, 11, 6), (6, 13, 2), (14, 5, 20),
-                                   (9, 6, 1)])
+                                   (9, 6, 1), (12, 12, 48), (16, 4, 20), (16, 16, 4)])
 def test_conv_mfma_error_bound(ctx, prec, ulp, h, w, n):
     """MFMA convolution vs the FP64 oracle: every output within ulp * sum |a*b|, the bound
-    that rounding both operands to the 16-bit type (and accumulating in FP32) allows."""
+    that rounding both operands to the 16-bit type (and accumulating in FP32) allows.
+    12x12x48 and 16x4x20 have tap columns too large for the kernel's filter registers (plain
+    copy path); 16x16x4 has the widest feature ring."""
     say("conv mfma", prec, h, w, n)
     rng = np.random.default_rng(h * 31 + w + n)
     feat = (rng.random((31, 77, 69), dtype=np.float32) * 0.4).astype(np.float32)
