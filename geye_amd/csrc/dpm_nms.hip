@@ -149,8 +149,11 @@ __global__ __launch_bounds__(64) void k_nms_winmask(const NmsBox *__restrict__ s
 // LDS); then one wave runs the sequential decision with scalar state: block b of 64 rows is
 // decided with word b alone (row j's word b read by readlane, folded in only when row j is
 // still alive), and the rows picked in block b are OR-ed into the later words by their
-// owner lanes.  Finally every thread takes one window box that was not picked and finds its
-// first suppressor: the earliest pick whose row covers it.
+// owner lanes (whole words: the decision needs only the removed set).  The first suppressor
+// of a removed window box (the earliest pick whose row covers it) comes from the round's
+// suppression pass, which tests every box ranked after each pick; in a round with no such
+// pass (window_sup != 0) every thread takes window boxes that were not picked and scans the
+// picks in order here.
 __global__ __launch_bounds__(256) void k_nms_wingreedy(const int *__restrict__ alive,
                                                         const int *__restrict__ nalive,
                                                         const unsigned long long *__restrict__ mask,
@@ -158,12 +161,15 @@ __global__ __launch_bounds__(256) void k_nms_wingreedy(const int *__restrict__ a
                                                         int *__restrict__ sup,
                                                         int *__restrict__ picks,
                                                         int *__restrict__ counters,
+                                                        int window_sup,
                                                         long long *__restrict__ phase_clock) {
   constexpr int NW = kNmsWindow / 64;
   extern __shared__ unsigned long long smask[];  // [NW][kNmsWindow]
   if (phase_clock && threadIdx.x == 0) phase_clock[0] = wall_clock64();
   int *salive = (int *)(smask + NW * kNmsWindow);
-  __shared__ int s_first[kNmsWindow];  // window index of the pick that first removed a box
+  __shared__ short s_pick[kNmsWindow];              // window indices of the picks, in order
+  __shared__ unsigned long long s_picked[NW];       // picked rows of each 64-row block
+  __shared__ int s_np;
   const int W = min(*nalive, kNmsWindow);
   const int nblk = (W + 63) / 64;
   // staging: this is the only workgroup on the device, so its loads are issued in wide,
@@ -197,7 +203,6 @@ __global__ __launch_bounds__(256) void k_nms_wingreedy(const int *__restrict__ a
       if (i < W) salive[i] = a[u];
     }
   }
-  for (int i = threadIdx.x; i < kNmsWindow; i += blockDim.x) s_first[i] = -1;
   __syncthreads();
   if (phase_clock && threadIdx.x == 0) phase_clock[1] = wall_clock64();
   const int round_begin = counters[0];
@@ -223,27 +228,26 @@ __global__ __launch_bounds__(256) void k_nms_wingreedy(const int *__restrict__ a
         const unsigned int lo = (unsigned int)__builtin_amdgcn_readlane((int)(unsigned int)diag, i);
         const unsigned long long m = ((unsigned long long)hi << 32) | (unsigned long long)lo;
         picked |= 1ull << i;
-        const unsigned long long fresh = m & cand;  // first removed by row i
-        if (lane < rows && ((fresh >> lane) & 1ull)) s_first[64 * b + lane] = 64 * b + i;
         cand &= ~(m | (1ull << i));
         rem |= m;
       }
       if (lane == b) removed = rem;
-      // owners of the later words fold in the picked rows, in order
+      // owners of the later words fold in the picked rows (8 row reads in flight at a time)
       if (lane > b && lane < nblk) {
         const unsigned long long *row = smask + (size_t)lane * kNmsWindow + 64 * b;
         unsigned long long pk = picked;
         while (pk) {
-          const int i = __builtin_ctzll(pk);
-          pk &= pk - 1;
-          const unsigned long long m = row[i];
-          unsigned long long fresh = m & ~removed;
-          removed |= m;
-          while (fresh) {
-            const int bit = __builtin_ctzll(fresh);
-            fresh &= fresh - 1;
-            s_first[64 * lane + bit] = 64 * b + i;
+          int ix[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            ix[u] = pk ? __builtin_ctzll(pk) : -1;
+            pk &= pk - 1;
           }
+          unsigned long long m[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) m[u] = ix[u] >= 0 ? row[ix[u]] : 0ull;
+#pragma unroll
+          for (int u = 0; u < 8; u++) removed |= m[u];
         }
       }
       if (lane < rows && ((picked >> lane) & 1ull)) {
@@ -252,20 +256,41 @@ __global__ __launch_bounds__(256) void k_nms_wingreedy(const int *__restrict__ a
         const int ri = salive[64 * b + lane];
         status[ri] = 1;
         picks[round_begin + slot] = ri;
+        s_pick[slot] = (short)(64 * b + lane);
       }
+      if (lane == 0) s_picked[b] = picked;
       np += __popcll(picked);
     }
     if (lane == 0) {
       counters[0] = round_begin + np;  // picks so far
       counters[1] = round_begin;       // first pick of this round
+      s_np = np;
     }
   }
   __syncthreads();
   if (phase_clock && threadIdx.x == 0) phase_clock[2] = wall_clock64();
-  // first suppressors of the window boxes removed in this round
-  for (int j = threadIdx.x; j < W; j += blockDim.x) {
-    const int f = s_first[j];
-    if (f >= 0) sup[salive[j]] = salive[f];
+  // first suppressors of the window boxes removed in this round, when no suppression pass
+  // follows: every window box is either a pick or covered by an earlier pick's row
+  if (window_sup) {
+    const int np = s_np;
+    for (int j = threadIdx.x; j < W; j += blockDim.x) {
+      const int w = j >> 6, bit = j & 63;
+      if ((s_picked[w] >> bit) & 1ull) continue;
+      const unsigned long long *col = smask + (size_t)w * kNmsWindow;
+      int f = -1;
+      for (int k0 = 0; k0 < np && f < 0; k0 += 8) {
+        int pp[8];
+        unsigned long long v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) pp[u] = k0 + u < np ? s_pick[k0 + u] : -1;
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = pp[u] >= 0 ? col[pp[u]] : 0ull;
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+          if (f < 0 && ((v[u] >> bit) & 1ull)) f = pp[u];
+      }
+      if (f >= 0) sup[salive[j]] = salive[f];
+    }
   }
   if (phase_clock) {
     __syncthreads();
@@ -278,8 +303,8 @@ __global__ __launch_bounds__(256) void k_nms_wingreedy(const int *__restrict__ a
   }
 }
 
-// every unchecked box after the window against this round's picks, in pick order (generic
-// path for caller-supplied boxes)
+// every unchecked box against this round's picks ranked before it, in pick order: the first
+// suppressor of a removed window box or a later box (generic path for caller-supplied boxes)
 __global__ __launch_bounds__(256) void k_nms_suppress(const NmsBox *__restrict__ sbox,
                                                       const int *__restrict__ alive,
                                                       const int *__restrict__ nalive,
@@ -289,7 +314,6 @@ __global__ __launch_bounds__(256) void k_nms_suppress(const NmsBox *__restrict__
   __shared__ NmsBox sp[kNmsWindow];
   __shared__ int sr[kNmsWindow];
   const int na = *nalive;
-  const int W = min(na, kNmsWindow);
   const int p0 = counters[1], p1 = counters[0];
   const int np = p1 - p0;
   for (int k = threadIdx.x; k < np; k += blockDim.x) {
@@ -297,11 +321,11 @@ __global__ __launch_bounds__(256) void k_nms_suppress(const NmsBox *__restrict__
     sp[k] = sbox[sr[k]];
   }
   __syncthreads();
-  const int t = W + blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= na) return;
   const int r = alive[t];
   const NmsBox b = sbox[r];
-  for (int k = 0; k < np; k++)
+  for (int k = 0; k < np && sr[k] < r; k++)
     if (suppresses(sp[k], b, overlap)) {
       sup[r] = sr[k];
       break;
@@ -620,9 +644,11 @@ void launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)glds);
     attr = true;
   }
-  hipLaunchKernelGGL(k_nms_wingreedy, dim3(1), dim3(256), glds, s, alive, nalive, mask, status,
-                     sup, picks, counters, g_nms_phase_clock);
+  // the suppression pass below (launched when the bound leaves boxes after the window) also
+  // gives the window's removed boxes their first suppressor
   const int rest = nalive_host - W;
+  hipLaunchKernelGGL(k_nms_wingreedy, dim3(1), dim3(256), glds, s, alive, nalive, mask, status,
+                     sup, picks, counters, rest <= 0 ? 1 : 0, g_nms_phase_clock);
   if (rest <= 0) return;
   const int nb = (nalive_host + kNmsCompactBlock - 1) / kNmsCompactBlock;
   if (maps) {
@@ -631,8 +657,8 @@ void launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const
                        maps->nmaps, maps->rsize, maps->padx, maps->pady, maps->rank_of_flat,
                        overlap, sup);
   } else {
-    hipLaunchKernelGGL(k_nms_suppress, dim3((rest + 255) / 256), dim3(256), 0, s, sbox, alive,
-                       nalive, picks, counters, overlap, sup);
+    hipLaunchKernelGGL(k_nms_suppress, dim3((nalive_host + 255) / 256), dim3(256), 0, s, sbox,
+                       alive, nalive, picks, counters, overlap, sup);
   }
   const bool retire = maps && maps->cand;
   hipLaunchKernelGGL(k_nms_post, dim3(nb), dim3(256), 0, s, alive, nalive, sup, sbox,
