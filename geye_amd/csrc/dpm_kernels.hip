@@ -907,7 +907,9 @@ __global__ __launch_bounds__(256) void k_dt_window(const DtJob *__restrict__ job
   __syncthreads();
   vmin = fminf(fminf(s_red[0][0], s_red[0][1]), fminf(s_red[0][2], s_red[0][3]));
   vmax = fmaxf(fmaxf(s_red[1][0], s_red[1][1]), fmaxf(s_red[1][2], s_red[1][3]));
-  // half-width of the argmin band, with a margin for the rounding of the costs
+  // half-width of the argmin band: a source at offset t = p - q can beat the centre only
+  // where a t^2 + b t < f(p) - f(q) <= R (R carries a margin for the rounding of the costs),
+  // i.e. for integer t with |t| <= h, the larger root magnitude
   int need = 1 << 20;
   const float a = J.a, b = J.b;
   const float R = (vmax - vmin) * 1.00001f + 1e-4f;
@@ -915,7 +917,7 @@ __global__ __launch_bounds__(256) void k_dt_window(const DtJob *__restrict__ job
     const float disc = sqrtf(b * b + 4.0f * a * R);
     const float tp = (-b + disc) / (2.0f * a), tm = (-b - disc) / (2.0f * a);
     const float h = fmaxf(tp, -tm);
-    if (h < 64.0f) need = (int)ceilf(h) + 1;
+    if (h < 64.0f) need = (int)floorf(h) + 1;
   }
   if (need > kDtHalo) {
     // the reference's divide and conquer on the float lines; argmins go to the key area
@@ -926,7 +928,8 @@ __global__ __launch_bounds__(256) void k_dt_window(const DtJob *__restrict__ job
     dt_write(J, s_f, n, 0, s_ptr, dst_arena, ptr_arena);
     return;
   }
-  const int T = need <= 8 ? 8 : need <= 12 ? 12 : need <= 16 ? 16 : kDtHalo;
+  const int T = need <= 8 ? 8 : need <= 10 ? 10 : need <= 12 ? 12 : need <= 14 ? 14
+              : need <= 16 ? 16 : need <= 20 ? 20 : kDtHalo;
   // fixed-point scale: (f - fmin) + (q - qmin) over the window stays below 2^24 steps
   const float qmin = fminf(0.0f, -b * b / (4.0f * a));  // the parabola's vertex bounds q below
   const float qmax = a * (float)T * (float)T + fabsf(b) * (float)T;
@@ -965,8 +968,11 @@ __global__ __launch_bounds__(256) void k_dt_window(const DtJob *__restrict__ job
   (rows ? dt_window<TT, true>(J, s_f, s_key, LSk, s_q, dst_arena, ptr_arena)            \
         : dt_window<TT, false>(J, s_f, s_key, LSk, s_q, dst_arena, ptr_arena))
   if (T == 8) GEYE_DT_WINDOW(8);
+  else if (T == 10) GEYE_DT_WINDOW(10);
   else if (T == 12) GEYE_DT_WINDOW(12);
+  else if (T == 14) GEYE_DT_WINDOW(14);
   else if (T == 16) GEYE_DT_WINDOW(16);
+  else if (T == 20) GEYE_DT_WINDOW(20);
   else GEYE_DT_WINDOW(kDtHalo);
 #undef GEYE_DT_WINDOW
 }

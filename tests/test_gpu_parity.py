@@ -128,6 +128,10 @@ def _dt_cost(src, a, b, t):
     (9, 1, (0.1, 0.0, 0.1, 0.0), 0.2, False),
     (130, 70, (0.2, 0.1, 0.3, -0.2), 0.4, False),
     (5, 6, (0.01, 0.0, 0.01, 0.0), 0.2, False),
+    (200, 90, (0.01, 0.0, 0.01, 0.0), 0.3, False),    # value ranges giving half-widths
+    (150, 33, (0.01, 0.0, 0.01, 0.0), 0.4, False),    # of about 14, 16, 20 and 24
+    (96, 80, (0.01, 0.0, 0.01, 0.0), 0.6, False),
+    (70, 75, (0.01, 0.0, 0.01, 0.0), 0.85, False),
     (64, 64, (0.01, 0.0, 0.01, 0.0), 100.0, True),   # argmin band wider than 24
     (50, 40, (0.0, 0.0, 0.0, 0.0), 1.0, True),       # a = 0: no band
     (37, 29, (-0.01, 0.0, -0.02, 0.0), 1.0, True),   # a < 0: no band
