@@ -47,7 +47,9 @@ struct FusedResize {
   int rtab_off, ctab_off;
   int tiles_x, tiles_y, blocks_begin;
 };
-constexpr int kResizeTY = 16, kResizeTX = 64, kResizeSpan = 136;  // span: source columns
+// fused resize tile: 8 x 64 outputs (13 KB of LDS, so a CU holds as many workgroups as its
+// wave slots allow; on 1080p 8 rows beat 16 and 32); span: source columns
+constexpr int kResizeTY = 8, kResizeTX = 64, kResizeSpan = 136;
 
 struct HogLevel {
   long long img_off;           // image arena offset ([3][h][w] row-major)

@@ -98,8 +98,8 @@ __global__ __launch_bounds__(256) void k_resize_cols(const ResizeOp *__restrict_
   dst[op.out_off + e] = acc;
 }
 
-// Both passes of one resize, per 16 x 64 output tile (CT/resize.cc:135-160): the row pass
-// for the tile's 16 output rows over the source columns its 64 outputs tap goes to LDS, the
+// Both passes of one resize, per kResizeTY x 64 output tile (CT/resize.cc:135-160): the row
+// pass for the tile's output rows over the source columns its 64 outputs tap goes to LDS, the
 // column pass reads it there.  Each element is the same ordered tap sum (from 0, table
 // order) as the two separate passes, so the image is bit-identical.
 __global__ __launch_bounds__(256) void k_resize_fused(const FusedResize *__restrict__ ops,
