@@ -822,10 +822,15 @@ __device__ inline void dt_window(const DtJob &J, const float *s_f, const unsigne
   const int n = J.n, nl = J.nlines;
   const float a = J.a, b = J.b;
   // q terms of positions (2i, 2i + 1) as one 64-bit word: one 64-bit add forms two keys,
-  // since no key reaches 2^32 there is no carry between the halves
+  // since no key reaches 2^32 there is no carry between the halves.  They are the same for
+  // every lane, so they live in scalar registers (read once from LDS, first lane's copy).
   unsigned long long q[NP];
 #pragma unroll
-  for (int i = 0; i < NP; i++) q[i] = s_q[i];
+  for (int i = 0; i < NP; i++) {
+    const unsigned long long v = s_q[i];
+    q[i] = ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32)) << 32) |
+           (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+  }
   float *dst = dst_arena + J.dst_off + (long long)J.line0 * J.line_stride;
   int *pdst = ptr_arena + J.dst_off + (long long)J.line0 * J.line_stride;
   const int tid = threadIdx.x;
