@@ -342,10 +342,10 @@ __device__ inline void hog_vote_cell(const int2 *__restrict__ grid, int h, int w
     float v0, v1;
     interp_coord(x, sbin, &ix, &v0, &v1);
     const float wx = (ix == cx) ? v1 : v0;
-    const int2 *col = grid + (long long)min(x, w - 2) * h;
+    const int colbase = min(x, w - 2) * h;  // a level's grid holds < 2^31 pixels
     int2 g[NJ];
 #pragma unroll
-    for (int j = 0; j < NJ; j++) g[j] = col[yc[j]];
+    for (int j = 0; j < NJ; j++) g[j] = grid[(unsigned)(colbase + yc[j])];
 #pragma unroll
     for (int j = 0; j < NJ; j++)
       if (j < ny) acc[g[j].y * TT] += wx * wy[j] * __int_as_float(g[j].x);

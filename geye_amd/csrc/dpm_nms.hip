@@ -346,10 +346,14 @@ __global__ __launch_bounds__(256) void k_nms_scatter(const NmsBox *__restrict__ 
                                                      int padx, int pady,
                                                      const int *__restrict__ rank_of_flat,
                                                      float overlap, int *__restrict__ sup) {
-  __shared__ int s_xa[kNmsMaxMaps], s_ya[kNmsMaxMaps], s_h[kNmsMaxMaps], s_R0[kNmsMaxMaps];
-  __shared__ long long s_off[kNmsMaxMaps];
-  __shared__ float s_s[kNmsMaxMaps], s_fh[kNmsMaxMaps], s_fw[kNmsMaxMaps];
-  __shared__ int s_beg[kNmsMaxMaps + 1];
+  // per-map tables in dynamic LDS sized to nmaps (40 B per map): a workgroup takes only what
+  // the plan's maps need, so more of them (and other kernels) share a CU
+  extern __shared__ long long s_dyn[];
+  long long *s_off = s_dyn;                                 // [nmaps]
+  int *s_xa = (int *)(s_off + nmaps);                       // [nmaps] each
+  int *s_ya = s_xa + nmaps, *s_h = s_ya + nmaps, *s_R0 = s_h + nmaps;
+  float *s_s = (float *)(s_R0 + nmaps), *s_fh = s_s + nmaps, *s_fw = s_fh + nmaps;
+  int *s_beg = (int *)(s_fw + nmaps);                       // [nmaps + 1]
   __shared__ int s_part[256];
   const int p0 = counters[1], p1 = counters[0];
   if ((int)blockIdx.x >= p1 - p0) return;
@@ -652,7 +656,9 @@ void launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const
   if (rest <= 0) return;
   const int nb = (nalive_host + kNmsCompactBlock - 1) / kNmsCompactBlock;
   if (maps) {
-    hipLaunchKernelGGL(k_nms_scatter, dim3(W, kNmsScatterSplit), dim3(256), 0, s, sbox, picks,
+    // s_off 8 B, seven 4-byte tables and s_beg (nmaps + 1 entries) per map
+    const size_t slds = 40 * (size_t)maps->nmaps + 4;
+    hipLaunchKernelGGL(k_nms_scatter, dim3(W, kNmsScatterSplit), dim3(256), slds, s, sbox, picks,
                        counters, maps->maps,
                        maps->nmaps, maps->rsize, maps->padx, maps->pady, maps->rank_of_flat,
                        overlap, sup);
