@@ -594,9 +594,15 @@ __global__ __launch_bounds__(256) void k_take(const int *__restrict__ src,
 
 // rocPRIM picks a merge sort below about 1 M items (about 20 dependent launches for a
 // 1080p candidate set); onesweep is four digit passes over the same keys, so the merge
-// path is switched off.
-using NmsSortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
-                                                 rocprim::default_config, 0>;
+// path is switched off.  Onesweep tiles of 256 x 16 keys ranked with the match algorithm:
+// 0.131 ms for 1.03 M keys against 0.148 with the default ranking (11-bit digits do not fit
+// the LDS at 256 threads; 512-thread tiles neither).
+using NmsSortConfig = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<256, 12>,
+                                        rocprim::kernel_config<256, 16>, 8,
+                                        rocprim::block_radix_rank_algorithm::match>,
+    0>;
 
 size_t nms_sort_temp_bytes(int n) {
   size_t bytes = 0;
