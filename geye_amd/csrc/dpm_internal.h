@@ -134,6 +134,7 @@ struct ScoreMap {              // one (valid level, component) root score map
   float scale;                 // sbin / scales[level]
   int RX, RY;                  // accumulated-score shift of this component at this level
   int tab_off;                 // level's (Xn per image column, Yn per image row) table
+  int blocks_begin;            // first 256-cell block of this map in the score launch
 };
 
 struct ScorePart {
@@ -226,6 +227,7 @@ struct Plan {
   std::vector<ScoreMap> smaps;
   std::vector<ScorePart> sparts;
   long long score_total = 0;
+  int score_blocks = 0;        // 256-cell blocks of the score launch (each within one map)
   int GL = 0;
   std::vector<int> ascore_tab;         // per valid level: Xn[W] then Yn[H]
   // device copies of the tables

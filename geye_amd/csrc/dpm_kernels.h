@@ -33,7 +33,7 @@ void launch_dt_window(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, c
                       const int *depths, const float *src, float *dst, int *ptr);
 void launch_score(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,
                   const float *conv, const float *M, float thresh, float *score, int *flags,
-                  long long total);
+                  int blocks);
 int compact_blocks(long long n);
 void launch_compact(hipStream_t s, const int *flags, long long n, int *blocksum, int *total,
                     int *out);

@@ -992,16 +992,19 @@ __global__ __launch_bounds__(256) void k_score(const ScoreMap *__restrict__ maps
                                                const float *__restrict__ conv_arena,
                                                const float *__restrict__ M_arena, float thresh,
                                                float *__restrict__ score,
-                                               int *__restrict__ flags, long long total) {
-  long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= total) return;
+                                               int *__restrict__ flags) {
+  // each workgroup covers 256 cells of one map (ScoreMap::blocks_begin), so the map and its
+  // parts are the same for every lane
+  const int bx = blockIdx.x;
   int lo = 0, hi = nmaps - 1;
   while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (maps[mid].score_off <= e) lo = mid; else hi = mid - 1;
+    const int mid = (lo + hi + 1) >> 1;
+    if (maps[mid].blocks_begin <= bx) lo = mid; else hi = mid - 1;
   }
   const ScoreMap m = maps[lo];
-  const long long i = e - m.score_off;
+  const long long i = (long long)(bx - m.blocks_begin) * 256 + threadIdx.x;
+  if (i >= (long long)m.R0 * m.R1) return;
+  const long long e = m.score_off + i;
   const int x = (int)(i / m.R0), y = (int)(i - (long long)x * m.R0);
   float s = conv_arena[m.root_off + i] + m.off;
   for (int k = 0; k < m.npart; k++) {
@@ -1227,9 +1230,10 @@ void launch_dt_window(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, c
 
 void launch_score(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,
                   const float *conv, const float *M, float thresh, float *score, int *flags,
-                  long long total) {
-  hipLaunchKernelGGL(k_score, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, maps, nmaps,
-                     parts, conv, M, thresh, score, flags, total);
+                  int blocks) {
+  if (blocks > 0)
+    hipLaunchKernelGGL(k_score, dim3(blocks), dim3(256), 0, s, maps, nmaps, parts, conv, M,
+                       thresh, score, flags);
 }
 
 int compact_blocks(long long n) { return (int)((n + kCompactBlock - 1) / kCompactBlock); }

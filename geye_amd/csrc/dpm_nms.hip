@@ -47,11 +47,15 @@ __global__ __launch_bounds__(256) void k_nms_prep(const ScoreMap *__restrict__ m
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const long long e = cand[i];
+  // candidates ascend, so a wave's cells lie in the map of its first active lane's cell or a
+  // few after it: one search per wave, then each lane steps forward
+  const int e0 = __builtin_amdgcn_readfirstlane((int)e);
   int lo = 0, hi = nmaps - 1;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
-    if (maps[mid].score_off <= e) lo = mid; else hi = mid - 1;
+    if (maps[mid].score_off <= e0) lo = mid; else hi = mid - 1;
   }
+  while (lo + 1 < nmaps && maps[lo + 1].score_off <= e) lo++;
   const ScoreMap m = maps[lo];
   const long long k = e - m.score_off;
   const int x = (int)(k / m.R0), y = (int)(k - (long long)x * m.R0);

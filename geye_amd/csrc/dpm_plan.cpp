@@ -540,6 +540,7 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
   P.sparts.clear();
   P.ascore_tab.clear();
   long long dto = 0, so = 0;
+  int sblocks = 0;
   const int MP = m.maxpart;
   for (int l = interval; l < LEN; l++) {
     if (!P.lv[l].valid) continue;
@@ -572,6 +573,8 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
       sm.RY = (int)((double)((float)m.rsize[jj * 2] * lscale) / 2.0 - 1.0 + (double)block_pad);
       sm.RX = (int)((double)((float)m.rsize[jj * 2 + 1] * lscale) / 2.0 - 1.0 + (double)block_pad);
       sm.tab_off = tab_off;
+      sm.blocks_begin = sblocks;
+      sblocks += (int)(((long long)sm.R0 * sm.R1 + 255) / 256);
       so += (long long)sm.R0 * sm.R1;
       for (int kk = 0; kk < m.numpart[jj]; kk++) {
         const int pid = m.pidx[jj * MP + kk], did = m.didx[jj * MP + kk];
@@ -613,6 +616,7 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
   }
   P.dt_total = dto;
   P.score_total = so;
+  P.score_blocks = sblocks;
   P.dt_lds1 = 0;
   P.dt_lds2 = 0;
   P.dt_lds1w = 0;

@@ -792,7 +792,7 @@ int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int 
       KTimer t(c, "score");
       launch_score(s, dp<ScoreMap>(P.d_smaps), (int)P.smaps.size(), dp<ScorePart>(P.d_sparts),
                    dp<float>(P.d_conv), dp<float>(P.d_M), thresh, dp<float>(P.d_score),
-                   dp<int>(P.d_flags), P.score_total);
+                   dp<int>(P.d_flags), P.score_blocks);
     }
     {
       KTimer t(c, "compact");
