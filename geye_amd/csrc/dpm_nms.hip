@@ -364,17 +364,26 @@ __global__ __launch_bounds__(256) void k_nms_scatter(const NmsBox *__restrict__ 
   const int rp = picks[p0 + blockIdx.x];
   const NmsBox P = sbox[rp];
   const int tid = threadIdx.x;
-  // per-map rectangles; thread t owns maps [t*per, (t+1)*per)
+  // per-map rectangles; thread t owns maps [t*per, (t+1)*per).  A cell's box B (width
+  // bw = rw*s, height bh = rh*s) can be suppressed only if its overlap with the pick, ix by iy,
+  // exceeds overlap * R with R the smaller area; iy <= min(bh, ph) then needs ix > tx =
+  // overlap * R / min(bh, ph) (and likewise iy > ty), which bounds B's corner on both sides.
+  // The bounds are relaxed by 0.1% and one cell each way; the exact test decides.
   const int per = (nmaps + 255) / 256;
+  const float pw = P.x2 - P.x1 + 1.0f, ph = P.y2 - P.y1 + 1.0f;
   int mycnt = 0;
   for (int k = tid * per; k < min(nmaps, (tid + 1) * per); k++) {
     const ScoreMap m = maps[k];
     const float s = m.scale;
     const int rh = rsize[m.comp * 2], rw = rsize[m.comp * 2 + 1];
-    const int xa = max((int)floorf(P.x1 / s) - rw + padx - 2, 0);
-    const int xb = min((int)ceilf((P.x2 + 1.0f) / s) + padx + 1, m.R1 - 1);
-    const int ya = max((int)floorf(P.y1 / s) - rh + pady - 2, 0);
-    const int yb = min((int)ceilf((P.y2 + 1.0f) / s) + pady + 1, m.R0 - 1);
+    const float bw = (float)rw * s, bh = (float)rh * s;
+    const float R = fminf(bw * bh, P.area) * 0.999f;
+    const float tx = overlap > 0.0f ? overlap * R / fminf(bh, ph) : 0.0f;
+    const float ty = overlap > 0.0f ? overlap * R / fminf(bw, pw) : 0.0f;
+    const int xa = max((int)floorf((P.x1 - bw + tx) / s) + padx - 2, 0);
+    const int xb = min((int)ceilf((P.x2 + 1.0f - tx) / s) + padx + 1, m.R1 - 1);
+    const int ya = max((int)floorf((P.y1 - bh + ty) / s) + pady - 2, 0);
+    const int yb = min((int)ceilf((P.y2 + 1.0f - ty) / s) + pady + 1, m.R0 - 1);
     const int h = yb - ya + 1, w = xb - xa + 1;
     s_xa[k] = xa;
     s_ya[k] = ya;
