@@ -88,6 +88,20 @@ struct MfmaTile {
   int cost;                    // relative work, for ordering the queue
 };
 
+// Sum of the counts of the workgroups before workgroup b (a 256-thread workgroup reads them
+// together; s_red holds 4 ints).  Stream compactions use it in place of a separate scan
+// launch: each workgroup reads at most a few thousand counts.
+__device__ inline int preceding_count(const int *__restrict__ blocksum, int b, int *s_red) {
+  int part = 0;
+  for (int j = threadIdx.x; j < b; j += 256) part += blocksum[j];
+  for (int m = 32; m > 0; m >>= 1) part += __shfl_xor(part, m);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = part;
+  __syncthreads();
+  const int r = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+  __syncthreads();
+  return r;
+}
+
 // Halo on each side of an LDS line in the windowed DT: the widest window half-width.
 constexpr int kDtHalo = 24;
 // dynamic LDS limit of the windowed DT kernel (160 KB less its static reduction buffer)

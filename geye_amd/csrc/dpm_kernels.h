@@ -37,8 +37,6 @@ void launch_score(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePar
 int compact_blocks(long long n);
 void launch_compact(hipStream_t s, const int *flags, long long n, int *blocksum, int *total,
                     int *out);
-// exclusive scan of nb block counts in place (one workgroup); the sum goes to *total
-void launch_scan_blocks(hipStream_t s, int *blocksum, int nb, int *total);
 void launch_rows(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,
                  const float *score, const int *cand, int ncand, const int *tmpIx,
                  const int *tmpIy, int GL, int padx, int pady, const int *rsize, float *rows);

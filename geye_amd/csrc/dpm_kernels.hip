@@ -1016,7 +1016,8 @@ __global__ __launch_bounds__(256) void k_score(const ScoreMap *__restrict__ maps
 }
 
 // ---------------------------------------------------------------------------------------
-// Order-preserving stream compaction of flagged indices (block counts, scan, scatter).
+// Order-preserving stream compaction of flagged indices: block counts, then a scatter pass
+// whose workgroups add up the counts before them.
 // ---------------------------------------------------------------------------------------
 constexpr int kCompactBlock = 1024;
 
@@ -1036,37 +1037,14 @@ __global__ __launch_bounds__(256) void k_flag_count(const int *__restrict__ flag
   if (threadIdx.x == 0) blocksum[blockIdx.x] = s[0];
 }
 
-// exclusive scan of nb block sums in one workgroup; writes the total at blocksum[nb]
-__global__ __launch_bounds__(1024) void k_scan_blocks(int *__restrict__ blocksum, int nb,
-                                                      int *__restrict__ total) {
-  __shared__ int s[1024];
-  __shared__ int carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int base = 0; base < nb; base += 1024) {
-    int i = base + threadIdx.x;
-    int v = i < nb ? blocksum[i] : 0;
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-      int t = (int)threadIdx.x >= o ? s[threadIdx.x - o] : 0;
-      __syncthreads();
-      s[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (i < nb) blocksum[i] = carry + s[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == 1023) carry += s[1023];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *total = carry;
-}
-
 __global__ __launch_bounds__(256) void k_flag_scatter(const int *__restrict__ flags,
                                                       long long n,
                                                       const int *__restrict__ blocksum,
+                                                      int *__restrict__ total,
                                                       int *__restrict__ out) {
   __shared__ int s[256];
+  __shared__ int s_red[4];
+  const int before = preceding_count(blocksum, blockIdx.x, s_red);
   long long base = (long long)blockIdx.x * kCompactBlock;
   // each thread owns 4 consecutive elements
   int f[4], c = 0;
@@ -1084,7 +1062,8 @@ __global__ __launch_bounds__(256) void k_flag_scatter(const int *__restrict__ fl
     s[threadIdx.x] += t;
     __syncthreads();
   }
-  int pos = blocksum[blockIdx.x] + s[threadIdx.x] - c;
+  int pos = before + s[threadIdx.x] - c;
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 255) *total = before + s[255];
 #pragma unroll
   for (int k = 0; k < 4; k++)
     if (f[k]) out[pos++] = (int)(base + threadIdx.x * 4 + k);
@@ -1238,16 +1217,13 @@ void launch_score(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePar
 
 int compact_blocks(long long n) { return (int)((n + kCompactBlock - 1) / kCompactBlock); }
 
+// flagged indices of [0, n) in order to out, their count to *total (one workgroup at least,
+// so that an empty input writes 0)
 void launch_compact(hipStream_t s, const int *flags, long long n, int *blocksum, int *total,
                     int *out) {
-  int nb = compact_blocks(n);
+  const int nb = compact_blocks(n) > 0 ? compact_blocks(n) : 1;
   hipLaunchKernelGGL(k_flag_count, dim3(nb), dim3(256), 0, s, flags, n, blocksum);
-  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, blocksum, nb, total);
-  hipLaunchKernelGGL(k_flag_scatter, dim3(nb), dim3(256), 0, s, flags, n, blocksum, out);
-}
-
-void launch_scan_blocks(hipStream_t s, int *blocksum, int nb, int *total) {
-  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, blocksum, nb, total);
+  hipLaunchKernelGGL(k_flag_scatter, dim3(nb), dim3(256), 0, s, flags, n, blocksum, total, out);
 }
 
 void launch_rows(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,

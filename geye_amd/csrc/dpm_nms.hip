@@ -516,14 +516,17 @@ __global__ __launch_bounds__(256) void k_nms_post(const int *__restrict__ alive,
   if (threadIdx.x == 0) blocksum[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
 }
 
-// The alive later boxes, in order, to alive_out: blocksum holds each workgroup's exclusive
-// offset (scanned), positions as in k_nms_post.
+// The alive later boxes, in order, to alive_out: blocksum holds each workgroup's count (from
+// k_nms_post, positions as there); a workgroup's offset is the sum of the counts before it,
+// and the last workgroup writes the total, the next round's alive count.
 __global__ __launch_bounds__(256) void k_nms_compact(const int *__restrict__ alive,
                                                      const int *__restrict__ nalive,
                                                      const int *__restrict__ sup,
                                                      const int *__restrict__ blocksum,
+                                                     int *__restrict__ nalive_out,
                                                      int *__restrict__ alive_out) {
-  __shared__ int s_w[4];
+  __shared__ int s_w[4], s_red[4];
+  const int before = preceding_count(blocksum, blockIdx.x, s_red);
   const int na = *nalive;
   const int W = min(na, kNmsWindow);
   const int t0 = blockIdx.x * kNmsCompactBlock + threadIdx.x * 4;
@@ -548,8 +551,10 @@ __global__ __launch_bounds__(256) void k_nms_compact(const int *__restrict__ ali
   }
   if (lane == 63) s_w[wave] = inc;
   __syncthreads();
-  int pos = blocksum[blockIdx.x] + inc - c;
+  int pos = before + inc - c;
   for (int w = 0; w < wave; w++) pos += s_w[w];
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+    *nalive_out = before + s_w[0] + s_w[1] + s_w[2] + s_w[3];
 #pragma unroll
   for (int k = 0; k < 4; k++)
     if (f[k]) alive_out[pos++] = r[k];
@@ -694,9 +699,8 @@ void launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const
 void launch_nms_compact(hipStream_t s, const int *alive, const int *nalive, const int *sup,
                         int *blocksum, int bound, int *nalive_out, int *alive_out) {
   const int nb = (bound + kNmsCompactBlock - 1) / kNmsCompactBlock;
-  launch_scan_blocks(s, blocksum, nb, nalive_out);
   hipLaunchKernelGGL(k_nms_compact, dim3(nb), dim3(256), 0, s, alive, nalive, sup, blocksum,
-                     alive_out);
+                     nalive_out, alive_out);
 }
 
 void launch_nms_check(hipStream_t s, const NmsBox *sbox, int n, const int *status, const int *sup,
