@@ -130,22 +130,66 @@ __device__ inline bool suppresses(const NmsBox &a, const NmsBox &b, float overla
 }
 
 // row i of the window bitmask, stored transposed ([word][row]): bit j (j > i) of row i set
-// when window box i suppresses box j
-__global__ __launch_bounds__(64) void k_nms_winmask(const NmsBox *__restrict__ sbox,
-                                                    const int *__restrict__ alive,
-                                                    const int *__restrict__ nalive,
-                                                    float overlap,
-                                                    unsigned long long *__restrict__ mask) {
+// when window box i suppresses box j.  A workgroup stages the window's boxes in LDS (the
+// alive indices, then the boxes, 4 per thread, loads in flight together) and each of its
+// waves computes one row from there.
+constexpr int kWinmaskRows = 4;  // rows per workgroup, one per wave
+
+__global__ __launch_bounds__(256) void k_nms_winmask(const NmsBox *__restrict__ sbox,
+                                                     const int *__restrict__ alive,
+                                                     const int *__restrict__ nalive,
+                                                     float overlap,
+                                                     unsigned long long *__restrict__ mask) {
+  __shared__ float s_y1[kNmsWindow], s_x1[kNmsWindow], s_y2[kNmsWindow], s_x2[kNmsWindow],
+      s_ar[kNmsWindow];
   const int W = min(*nalive, kNmsWindow);
-  const int i = blockIdx.x;
+  if ((int)blockIdx.x * kWinmaskRows >= W) return;
+  const int tid = threadIdx.x;
+  constexpr int U = kNmsWindow / 256;
+  // loads unconditional (position 0 stands in past W), so they are in flight together
+  int a[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int j = tid + u * 256;
+    a[u] = alive[j < W ? j : 0];
+  }
+  NmsBox b[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) b[u] = sbox[a[u]];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int j = tid + u * 256;
+    if (j < W) {
+      s_y1[j] = b[u].y1;
+      s_x1[j] = b[u].x1;
+      s_y2[j] = b[u].y2;
+      s_x2[j] = b[u].x2;
+      s_ar[j] = b[u].area;
+    }
+  }
+  __syncthreads();
+  const int i = blockIdx.x * kWinmaskRows + (tid >> 6), lane = tid & 63;
   if (i >= W) return;
-  const NmsBox a = sbox[alive[i]];
+  NmsBox A;
+  A.y1 = s_y1[i];
+  A.x1 = s_x1[i];
+  A.y2 = s_y2[i];
+  A.x2 = s_x2[i];
+  A.area = s_ar[i];
   for (int w = 0; w < kNmsWindow / 64; w++) {
-    const int j = w * 64 + threadIdx.x;
-    bool s = false;
-    if (j > i && j < W) s = suppresses(a, sbox[alive[j]], overlap);
-    unsigned long long bits = __ballot(s);
-    if (threadIdx.x == 0) mask[(size_t)w * kNmsWindow + i] = bits;  // [word][row]
+    const int j = w * 64 + lane;
+    bool sp = false;
+    if (j > i && j < W) {
+      NmsBox B;
+      B.y1 = s_y1[j];
+      B.x1 = s_x1[j];
+      B.y2 = s_y2[j];
+      B.x2 = s_x2[j];
+      B.area = s_ar[j];
+      sp = suppresses(A, B, overlap);
+    }
+    const unsigned long long bits = __ballot(sp);
+    if (lane == 0) mask[(size_t)w * kNmsWindow + i] = bits;  // [word][row]
   }
 }
 
@@ -664,7 +708,8 @@ void launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const
                       int *sup, int *picks, int *counters, int *blocksum,
                       const NmsMaps *maps) {
   const int W = nalive_host < kNmsWindow ? nalive_host : kNmsWindow;
-  hipLaunchKernelGGL(k_nms_winmask, dim3(W), dim3(64), 0, s, sbox, alive, nalive, overlap, mask);
+  hipLaunchKernelGGL(k_nms_winmask, dim3((W + kWinmaskRows - 1) / kWinmaskRows), dim3(256), 0, s,
+                     sbox, alive, nalive, overlap, mask);
   static bool attr = false;
   const size_t glds = sizeof(unsigned long long) * kNmsWindow * (kNmsWindow / 64) + 4 * kNmsWindow;
   if (!attr) {
