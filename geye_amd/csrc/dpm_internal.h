@@ -310,7 +310,8 @@ struct geye_ctx {
   geye::Plan *last_plan = nullptr;
   bool graphs = true;                 // replay the per-frame front as a HIP graph
   int num_cu = 256;
-  int *h_count = nullptr;             // pinned host slots for NMS survivor counts
+  int *h_count = nullptr;             // pinned host slots for NMS survivor counts (coherent,
+  int *d_hcount = nullptr;            // mapped: the compaction kernel writes them directly)
   hipEvent_t ev_count[2] = {nullptr, nullptr};
   long long nms_fallbacks = 0;
 };

@@ -46,8 +46,10 @@ void launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const
                       int *sup, int *picks, int *counters, int *blocksum, const NmsMaps *maps);
 // The round's surviving later boxes, in order, to alive_out, their count to *nalive_out;
 // bound >= *nalive (positions of the round).
+// host_total (optional): coherent pinned host memory that also receives the new alive count
 void launch_nms_compact(hipStream_t s, const int *alive, const int *nalive, const int *sup,
-                        int *blocksum, int bound, int *nalive_out, int *alive_out);
+                        int *blocksum, int bound, int *nalive_out, int *alive_out,
+                        int *host_total);
 // the pick count is read from counters[0] on the device; bound >= that count
 void launch_nms_check(hipStream_t s, const NmsBox *sbox, int n, const int *status, const int *sup,
                       const int *picks, const int *counters, int *flags);

@@ -568,7 +568,8 @@ __global__ __launch_bounds__(256) void k_nms_compact(const int *__restrict__ ali
                                                      const int *__restrict__ sup,
                                                      const int *__restrict__ blocksum,
                                                      int *__restrict__ nalive_out,
-                                                     int *__restrict__ alive_out) {
+                                                     int *__restrict__ alive_out,
+                                                     int *host_total) {
   __shared__ int s_w[4], s_red[4];
   const int before = preceding_count(blocksum, blockIdx.x, s_red);
   const int na = *nalive;
@@ -597,8 +598,11 @@ __global__ __launch_bounds__(256) void k_nms_compact(const int *__restrict__ ali
   __syncthreads();
   int pos = before + inc - c;
   for (int w = 0; w < wave; w++) pos += s_w[w];
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
-    *nalive_out = before + s_w[0] + s_w[1] + s_w[2] + s_w[3];
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    const int total = before + s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    *nalive_out = total;
+    if (host_total) *(volatile int *)host_total = total;  // coherent pinned host memory
+  }
 #pragma unroll
   for (int k = 0; k < 4; k++)
     if (f[k]) alive_out[pos++] = r[k];
@@ -742,10 +746,11 @@ void launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const
 }
 
 void launch_nms_compact(hipStream_t s, const int *alive, const int *nalive, const int *sup,
-                        int *blocksum, int bound, int *nalive_out, int *alive_out) {
+                        int *blocksum, int bound, int *nalive_out, int *alive_out,
+                        int *host_total) {
   const int nb = (bound + kNmsCompactBlock - 1) / kNmsCompactBlock;
   hipLaunchKernelGGL(k_nms_compact, dim3(nb), dim3(256), 0, s, alive, nalive, sup, blocksum,
-                     nalive_out, alive_out);
+                     nalive_out, alive_out, host_total);
 }
 
 void launch_nms_check(hipStream_t s, const NmsBox *sbox, int n, const int *status, const int *sup,

@@ -903,15 +903,13 @@ int nms_core(geye_ctx *c, Plan &P, int n, float overlap, const NmsMaps *maps, co
     const int rest = na - std::min(na, kNmsWindow);  // boxes after that round's window
     if (rest <= 0) break;                           // it held every alive box
     launch_nms_compact(s, alive, na_buf[cur], dp<int>(P.n_fsup), blocksum, na,
-                       na_buf[cur ^ 1], alive_next);
-    HIPCHK(hipMemcpyAsync(c->h_count + slot, na_buf[cur ^ 1], sizeof(int), hipMemcpyDeviceToHost,
-                          s));
+                       na_buf[cur ^ 1], alive_next, c->d_hcount + slot);
     HIPCHK(hipEventRecord(c->ev_count[slot], s));
     std::swap(alive, alive_next);
     cur ^= 1;
     round(rest);                                    // next round, bounded by rest
     HIPCHK(hipEventSynchronize(c->ev_count[slot]));
-    na = c->h_count[slot];                          // its exact alive count
+    na = ((volatile int *)c->h_count)[slot];        // its exact alive count
     slot ^= 1;
     if (na == 0) break;
   }
@@ -1230,17 +1228,24 @@ int geye_init(int device, geye_ctx **ctx) {
   if (hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
       c->num_cu <= 0)
     c->num_cu = 256;
-  if (hipHostMalloc((void **)&c->h_count, 64, hipHostMallocDefault) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_count[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_count[1], hipEventDisableTiming) != hipSuccess) {
-    (void)hipStreamDestroy(c->stream);
-    delete c;
-    return fail(GEYE_ERR_HIP, "pinned count buffer / events");
-  }
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete c;
     return fail(GEYE_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  // the NMS rounds' alive counts: written by the compaction kernel into coherent pinned
+  // memory, read by the host after the round's event
+  if (hipHostMalloc((void **)&c->h_count, 64, hipHostMallocCoherent | hipHostMallocMapped) !=
+          hipSuccess ||
+      hipHostGetDevicePointer((void **)&c->d_hcount, c->h_count, 0) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_count[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_count[1], hipEventDisableTiming) != hipSuccess) {
+    if (c->h_count) (void)hipHostFree(c->h_count);
+    for (hipEvent_t ev : c->ev_count)
+      if (ev) (void)hipEventDestroy(ev);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return fail(GEYE_ERR_HIP, "pinned count buffer / events");
   }
   *ctx = c;
   return GEYE_OK;
