@@ -133,7 +133,10 @@ struct DtJob {
   // outputs the windowed DT writes (the 16-bit forms): 0 every output; 1 (columns pass) M
   // everywhere, the argmin on rows y == ay mod 2; 2 (rows pass) on rows y == ay - 1 mod 2
   // M at columns x == ax - 1 mod 2, on rows y == ay mod 2 the argmin at x == ax mod 2 --
-  // exactly the cells the scoring and the part boxes read
+  // exactly the cells the scoring and the part boxes read.  With sel != 0 the selected
+  // argmins and the rows pass's M go to half grids, cell (y, x) at (x >> 1) * ceil(P0 / 2)
+  // + (y >> 1) (the columns pass's argmins at x * ceil(P0 / 2) + (y >> 1)), argmins as
+  // 16-bit indices
   int sel, ax, ay;
 };
 

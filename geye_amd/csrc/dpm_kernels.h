@@ -32,13 +32,14 @@ void launch_dt(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, const in
 void launch_dt_window(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, const int4 *nodes,
                       const int *depths, const float *src, float *dst, int *ptr);
 void launch_score(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,
-                  const float *conv, const float *M, float thresh, float *score, int *flags,
-                  int blocks);
+                  const float *conv, const float *M, float thresh, int half, float *score,
+                  int *flags, int blocks);
 int compact_blocks(long long n);
 void launch_compact(hipStream_t s, const int *flags, long long n, int *blocksum, int *total,
                     int *out);
 void launch_rows(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,
                  const float *score, const int *cand, int ncand, const int *tmpIx,
-                 const int *tmpIy, int GL, int padx, int pady, const int *rsize, float *rows);
+                 const int *tmpIy, int half, int GL, int padx, int pady, const int *rsize,
+                 float *rows);
 
 }  // namespace geye

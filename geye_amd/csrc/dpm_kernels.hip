@@ -833,6 +833,12 @@ __device__ inline void dt_window(const DtJob &J, const float *s_f, const unsigne
   }
   float *dst = dst_arena + J.dst_off + (long long)J.line0 * J.line_stride;
   int *pdst = ptr_arena + J.dst_off + (long long)J.line0 * J.line_stride;
+  // the selected outputs go to half grids of the map (H = ceil(P0 / 2) cells per column):
+  // the argmins as 16-bit indices in the first half of the map's int area, the rows pass's M
+  // at the start of the map's float area
+  float *dstc = dst_arena + J.dst_off;
+  unsigned short *pc = (unsigned short *)ptr_arena + 2 * J.dst_off;
+  const int H = ((ROWS ? J.elem_stride : n) + 1) >> 1;
   const int tid = threadIdx.x;
   // ROWS (lines interleaved in memory): lanes take consecutive lines of one position, so the
   // stores are contiguous; otherwise consecutive positions of one line
@@ -869,9 +875,19 @@ __device__ inline void dt_window(const DtJob &J, const float *s_f, const unsigne
         best = min(best, min((unsigned)k, (unsigned)(k >> 32)));
       }
       const int s = dd - T + (int)(best & 63u);
-      const long long o = (long long)l * J.line_stride + (long long)dd * J.elem_stride;
-      if (!sel_rows || kind == 0) dst[o] = dt_cost(s_f[l * n + s], a, b, dd - s);
-      if ((!sel_rows || kind == 1) && (!sel_cols || ((dd - J.ay) & 1) == 0)) pdst[o] = s;
+      const float v = dt_cost(s_f[l * n + s], a, b, dd - s);
+      if (sel_rows) {
+        // half grid: (y, x) at (x >> 1) * H + (y >> 1), M or the 16-bit argmin
+        const long long oc = (long long)(dd >> 1) * H + ((J.line0 + l) >> 1);
+        if (kind == 0) dstc[oc] = v;
+        else pc[oc] = (unsigned short)s;
+      } else {
+        const long long o = (long long)l * J.line_stride + (long long)dd * J.elem_stride;
+        dst[o] = v;
+        if (!sel_cols) pdst[o] = s;
+        else if (((dd - J.ay) & 1) == 0)
+          pc[(long long)(J.line0 + l) * H + (dd >> 1)] = (unsigned short)s;  // column-major half
+      }
     }
     d += dstep;
     if (!ROWS) {
@@ -991,7 +1007,7 @@ __global__ __launch_bounds__(256) void k_score(const ScoreMap *__restrict__ maps
                                                const ScorePart *__restrict__ parts,
                                                const float *__restrict__ conv_arena,
                                                const float *__restrict__ M_arena, float thresh,
-                                               float *__restrict__ score,
+                                               int half, float *__restrict__ score,
                                                int *__restrict__ flags) {
   // each workgroup covers 256 cells of one map (ScoreMap::blocks_begin), so the map and its
   // parts are the same for every lane
@@ -1009,7 +1025,11 @@ __global__ __launch_bounds__(256) void k_score(const ScoreMap *__restrict__ maps
   float s = conv_arena[m.root_off + i] + m.off;
   for (int k = 0; k < m.npart; k++) {
     const ScorePart q = parts[m.part_first + k];
-    s -= M_arena[q.M_off + (q.ay - 1 + 2 * y) + (long long)(q.ax - 1 + 2 * x) * q.P0];
+    // half grid (the windowed DT's selected outputs): cell (ay - 1 + 2y, ax - 1 + 2x) sits
+    // at (x + (ax - 1) / 2) * ceil(P0 / 2) + y + (ay - 1) / 2
+    s -= half ? M_arena[q.M_off + (long long)(x + ((q.ax - 1) >> 1)) * ((q.P0 + 1) >> 1) + y +
+                        ((q.ay - 1) >> 1)]
+              : M_arena[q.M_off + (q.ay - 1 + 2 * y) + (long long)(q.ax - 1 + 2 * x) * q.P0];
   }
   score[e] = s;
   flags[e] = s > thresh ? 1 : 0;
@@ -1080,8 +1100,8 @@ __global__ __launch_bounds__(256) void k_rows(const ScoreMap *__restrict__ maps,
                                               const float *__restrict__ score,
                                               const int *__restrict__ cand, int ncand,
                                               const int *__restrict__ tmpIx,
-                                              const int *__restrict__ tmpIy, int GL, int padx,
-                                              int pady, const int *__restrict__ rsize,
+                                              const int *__restrict__ tmpIy, int half, int GL,
+                                              int padx, int pady, const int *__restrict__ rsize,
                                               float *__restrict__ rows) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncand) return;
@@ -1105,9 +1125,17 @@ __global__ __launch_bounds__(256) void k_rows(const ScoreMap *__restrict__ maps,
   for (int k = 0; k < m.npart; k++) {
     const ScorePart q = parts[m.part_first + k];
     const int probex = (x - 1) * 2 + q.ax, probey = (y - 1) * 2 + q.ay;
-    const long long P = probey + (long long)probex * q.P0;
-    const int ix = tmpIx[q.Ix_off + P];
-    const int iy = tmpIy[q.Iy_off + (long long)ix * q.P0 + probey];
+    int ix, iy;
+    if (half) {
+      // 16-bit half grids: (y, x) at (x >> 1) * ceil(P0 / 2) + (y >> 1), the columns pass's
+      // argmins at every column and rows y == ay mod 2 (the probe's parity)
+      const long long H = (q.P0 + 1) >> 1;
+      ix = ((const unsigned short *)tmpIx)[2 * q.Ix_off + (probex >> 1) * H + (probey >> 1)];
+      iy = ((const unsigned short *)tmpIy)[2 * q.Iy_off + ix * H + (probey >> 1)];
+    } else {
+      ix = tmpIx[q.Ix_off + probey + (long long)probex * q.P0];
+      iy = tmpIy[q.Iy_off + (long long)ix * q.P0 + probey];
+    }
     const float px = (float)ix + 1.0f, py = (float)iy + 1.0f;
     float b0 = (float)((((double)py - 2.0) / 2.0 + 1.0 - (double)(float)pady) * (double)scale);
     float b1 = (float)((((double)px - 2.0) / 2.0 + 1.0 - (double)(float)padx) * (double)scale);
@@ -1208,11 +1236,11 @@ void launch_dt_window(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, c
 }
 
 void launch_score(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,
-                  const float *conv, const float *M, float thresh, float *score, int *flags,
-                  int blocks) {
+                  const float *conv, const float *M, float thresh, int half, float *score,
+                  int *flags, int blocks) {
   if (blocks > 0)
     hipLaunchKernelGGL(k_score, dim3(blocks), dim3(256), 0, s, maps, nmaps, parts, conv, M,
-                       thresh, score, flags);
+                       thresh, half, score, flags);
 }
 
 int compact_blocks(long long n) { return (int)((n + kCompactBlock - 1) / kCompactBlock); }
@@ -1228,10 +1256,11 @@ void launch_compact(hipStream_t s, const int *flags, long long n, int *blocksum,
 
 void launch_rows(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,
                  const float *score, const int *cand, int ncand, const int *tmpIx,
-                 const int *tmpIy, int GL, int padx, int pady, const int *rsize, float *rows) {
+                 const int *tmpIy, int half, int GL, int padx, int pady, const int *rsize,
+                 float *rows) {
   if (ncand <= 0) return;
   hipLaunchKernelGGL(k_rows, dim3((ncand + 255) / 256), dim3(256), 0, s, maps, nmaps, parts,
-                     score, cand, ncand, tmpIx, tmpIy, GL, padx, pady, rsize, rows);
+                     score, cand, ncand, tmpIx, tmpIy, half, GL, padx, pady, rsize, rows);
 }
 
 }  // namespace geye

@@ -791,8 +791,8 @@ int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int 
     {
       KTimer t(c, "score");
       launch_score(s, dp<ScoreMap>(P.d_smaps), (int)P.smaps.size(), dp<ScorePart>(P.d_sparts),
-                   dp<float>(P.d_conv), dp<float>(P.d_M), thresh, dp<float>(P.d_score),
-                   dp<int>(P.d_flags), P.score_blocks);
+                   dp<float>(P.d_conv), dp<float>(P.d_M), thresh, exact_dt ? 0 : 1,
+                   dp<float>(P.d_score), dp<int>(P.d_flags), P.score_blocks);
     }
     {
       KTimer t(c, "compact");
@@ -815,7 +815,8 @@ int read_count(geye_ctx *c, Plan &P, int *ncand) {
   return GEYE_OK;
 }
 
-int run_rows(geye_ctx *c, Model &m, Plan &P, const int *d_list, int ncand,
+// half: the DT outputs are the windowed DT's half grids (the 16-bit forms)
+int run_rows(geye_ctx *c, Model &m, Plan &P, const int *d_list, int ncand, bool half,
              std::vector<float> &rows) {
   rows.assign((size_t)ncand * P.GL, 0.0f);
   if (ncand == 0) return GEYE_OK;
@@ -824,7 +825,8 @@ int run_rows(geye_ctx *c, Model &m, Plan &P, const int *d_list, int ncand,
     KTimer t(c, "rows");
     launch_rows(c->stream, dp<ScoreMap>(P.d_smaps), (int)P.smaps.size(), dp<ScorePart>(P.d_sparts),
                 dp<float>(P.d_score), d_list, ncand, dp<int>(P.d_tmpIx),
-                dp<int>(P.d_tmpIy), P.GL, m.padx, m.pady, dp<int>(m.d_rsize), dp<float>(P.d_rows));
+                dp<int>(P.d_tmpIy), half ? 1 : 0, P.GL, m.padx, m.pady, dp<int>(m.d_rsize),
+                dp<float>(P.d_rows));
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(rows.data(), P.d_rows.p, sizeof(float) * rows.size(),
@@ -1078,10 +1080,10 @@ int detect_common(geye_ctx *c, const geye_model *model, const unsigned char *bgr
   if (do_nms) {
     int nk = 0;
     CHK(run_nms(c, m, *P, ncand, overlap, &nk));
-    CHK(run_rows(c, m, *P, dp<int>(P->n_kept_flat), nk, rows));
+    CHK(run_rows(c, m, *P, dp<int>(P->n_kept_flat), nk, precision != GEYE_FP32, rows));
     *num = nk;
   } else {
-    CHK(run_rows(c, m, *P, dp<int>(P->d_cand), ncand, rows));
+    CHK(run_rows(c, m, *P, dp<int>(P->d_cand), ncand, precision != GEYE_FP32, rows));
     *num = ncand;
   }
   *boxes = to_malloc(rows);
@@ -1562,7 +1564,7 @@ int geye_get_boxes_features(geye_ctx *c, const geye_model *model, int width, int
   CHK(run_scores(c, m, *P, thresh, GEYE_FP32, &ncand));
   std::vector<float> rows;
   c->last_plan = P;
-  CHK(run_rows(c, m, *P, dp<int>(P->d_cand), ncand, rows));
+  CHK(run_rows(c, m, *P, dp<int>(P->d_cand), ncand, false, rows));
   *num = ncand;
   *boxes = to_malloc(rows);
   *GL = P->GL;
