@@ -51,6 +51,9 @@ struct FusedResize {
 // wave slots allow; on 1080p 8 rows beat 16 and 32); span: source columns
 constexpr int kResizeTY = 8, kResizeTX = 64, kResizeSpan = 136;
 
+// cell rows per HOG vote workgroup (one wave, one lane per cell of a cell column)
+constexpr int kHogVoteRows = 64;
+
 struct HogLevel {
   long long img_off;           // image arena offset ([3][h][w] row-major)
   long long grid_off;          // gradient grid of the image (GradImg)
@@ -59,9 +62,9 @@ struct HogLevel {
   long long feat_off;          // feature arena offset ([31][F1][F0])
   long long featb_off;         // bf16 channel-last arena offset ([F1][F0][32])
   int h, w, sbin, b0, b1, F0, F1;
-  int tiles_x, tiles_y;        // hist tiles
+  int tiles_x, tiles_y;        // vote tiles: b1 cell columns x ceil(b0 / kHogVoteRows)
   int blocks_begin;            // first flat block (hist or feat launch)
-  int T;                       // cells per hist tile side
+  int T;                       // cell rows per vote tile (kHogVoteRows)
 };
 
 // One convolution tile: NF filters of one shape against one feature level.

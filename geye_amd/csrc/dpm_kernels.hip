@@ -312,22 +312,61 @@ __global__ __launch_bounds__(256) void k_hog_grad(const GradImg *__restrict__ im
   }
 }
 
-// Orientation histogram + block energy, one thread per cell of a 16 x 16 cell tile.  The
-// thread gathers its cell's bilinear votes from the gradient grid in the reference's pixel
-// order, x outer and y inner (CT/featurepyramid.cpp.new2:323-378), so every bin sees the same
+// Orientation histogram + block energy, one lane per cell of a column of kHogVoteRows cells
+// (one wave per workgroup).  Each cell adds its bilinear votes in the reference's pixel order,
+// x outer and y inner (CT/featurepyramid.cpp.new2:323-378), so every bin sees the same
 // sequence of float additions as the reference's scatter loop.  Then hist[o][x][y] and
 // norm[x][y] = sum_{o<9} (h_o + h_{o+9})^2 (CT/featurepyramid.cpp.new2:391-404).
-constexpr int kHogT = 16;
-// One cell's votes in the reference's order (x outer, y inner).  Its rows are loaded for
-// all NJ positions of a pixel column at once (rows clamped into the image), so the loads
-// overlap; only the accumulation is limited to the cell's ny rows.
+//
+// The cells of one column share their pixel columns, so the wave stages each pixel column of
+// its rows in LDS with contiguous loads (the grid is column-major) and the lanes read their
+// rows from there; the next column's loads are in flight while the current one votes.  A
+// lane's bins live at s_acc[o][lane] (LDS float atomics, tried in their place, ran 5x slower;
+// reading a column's bins together and forwarding repeats in registers ran slower than this
+// read-add-write per vote, for its registers).
+constexpr int kHogVoteJM = 16;                                         // 2 * sbin <= 16
+constexpr int kHogVoteLoads = (kHogVoteRows * kHogVoteJM / 2 + kHogVoteJM / 2 + 1 + 63) / 64;
+constexpr int kHogVoteStage = kHogVoteLoads * 64;                      // staged rows (>= (R + 1) sbin + 1)
+constexpr int kHogVotePad = kHogVoteStage + kHogVoteStage / 8;         // row r at r + r / 8
+
+__device__ inline int hog_stage_index(int r) { return r + (r >> 3); }
+
 template <int NJ>
-__device__ inline void hog_vote_cell(const int2 *__restrict__ grid, int h, int w, int sbin,
-                                     int cx, int cy, int x0, int x1, int y0, int y1, int ny,
-                                     float *acc) {
-  constexpr int TT = kHogT * kHogT;
+__device__ inline void hog_vote_column(const int2 *s_col, float *acc, float wx, int ry0, int ry1,
+                                       const float *wy, int ny) {
+  // two halves of NJ / 2 rows: their LDS reads go out together
+#pragma unroll
+  for (int h0 = 0; h0 < NJ; h0 += NJ / 2) {
+    int2 g[NJ / 2];
+#pragma unroll
+    for (int j = 0; j < NJ / 2; j++) g[j] = s_col[hog_stage_index(min(ry0 + h0 + j, ry1))];
+#pragma unroll
+    for (int j = 0; j < NJ / 2; j++)
+      if (h0 + j < ny) acc[g[j].y * kHogVoteRows] += wx * wy[h0 + j] * __int_as_float(g[j].x);
+  }
+}
+
+template <int NJ>
+__device__ inline void hog_votes_cells(const HogLevel &L, const int2 *__restrict__ grid, int cx,
+                                       int cy0, int2 *s_col, float *s_acc) {
+  constexpr int R = kHogVoteRows, NL = kHogVoteLoads;
+  const int lane = threadIdx.x;
+  const int sbin = L.sbin, h = L.h, w = L.w;
+  const int vis0 = L.b0 * sbin, vis1 = L.b1 * sbin;
+  const int cy = cy0 + lane;
+  // pixels voting into a cell: interpolation cell in {c - 1, c}, within [1, vis - 2]
+  const int xa = max(1, first_pix(cx - 1, sbin)), xb = min(vis1 - 2, first_pix(cx + 1, sbin) - 1);
+  const int cyl = min(cy0 + R - 1, L.b0 - 1);
+  const int Ya = max(1, first_pix(cy0 - 1, sbin));
+  const int Yb = min(vis0 - 2, first_pix(cyl + 1, sbin) - 1);
+  const int ns = min(Yb - Ya + 1, kHogVoteStage);
+  int y0 = Ya, y1 = Ya, ny = 0;
+  if (cy < L.b0) {
+    y0 = max(1, first_pix(cy - 1, sbin));
+    y1 = min(vis0 - 2, first_pix(cy + 1, sbin) - 1);
+    ny = y1 - y0 + 1;
+  }
   float wy[NJ];
-  int yc[NJ];
 #pragma unroll
   for (int j = 0; j < NJ; j++) {
     const int y = min(y0 + j, y1);
@@ -335,60 +374,69 @@ __device__ inline void hog_vote_cell(const int2 *__restrict__ grid, int h, int w
     float v0, v1;
     interp_coord(y, sbin, &iy, &v0, &v1);
     wy[j] = (iy == cy) ? v1 : v0;
-    yc[j] = max(0, min(y, h - 2));
   }
-  for (int x = x0; x <= x1; x++) {
+  // staged rows of this lane's cell (clamped into the stage)
+  const int ry0 = min(max(y0 - Ya, 0), kHogVoteStage - 1);
+  const int ry1 = min(max(y1 - Ya, 0), kHogVoteStage - 1);
+  // staged row i of a column is grid row min(Ya + min(i, ns - 1), h - 2), as the reference
+  // clamps the pixel it differentiates; every lane issues the same NL loads per column, and
+  // the next column's loads are in flight while one votes (two columns ahead ran slower)
+  if (xa > xb) return;
+  const int ylast = max(ns - 1, 0);
+  int2 v[NL];
+  auto fetch = [&](int x) {
+    const int2 *col = grid + (long long)min(x, w - 2) * h;
+#pragma unroll
+    for (int k = 0; k < NL; k++) v[k] = col[max(0, min(Ya + min(lane + 64 * k, ylast), h - 2))];
+  };
+  fetch(xa);
+  for (int x = xa; x <= xb; x++) {
+#pragma unroll
+    for (int k = 0; k < NL; k++) s_col[hog_stage_index(lane + 64 * k)] = v[k];
+    __syncthreads();
+    if (x < xb) fetch(x + 1);
     int ix;
     float v0, v1;
     interp_coord(x, sbin, &ix, &v0, &v1);
     const float wx = (ix == cx) ? v1 : v0;
-    const int colbase = min(x, w - 2) * h;  // a level's grid holds < 2^31 pixels
-    int2 g[NJ];
-#pragma unroll
-    for (int j = 0; j < NJ; j++) g[j] = grid[(unsigned)(colbase + yc[j])];
-#pragma unroll
-    for (int j = 0; j < NJ; j++)
-      if (j < ny) acc[g[j].y * TT] += wx * wy[j] * __int_as_float(g[j].x);
+    hog_vote_column<NJ>(s_col, s_acc + lane, wx, ry0, ry1, wy, ny);
+    __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(256) void k_hog_votes(const HogLevel *__restrict__ lv, int nlv,
-                                                   const int2 *__restrict__ grid_arena,
-                                                   float *__restrict__ hist_arena,
-                                                   float *__restrict__ norm_arena) {
-  constexpr int T = kHogT, JM = 16;  // 2 * sbin <= 16 rows and columns per cell
-  __shared__ float s_acc[18 * T * T];
+__global__ __launch_bounds__(64) void k_hog_votes(const HogLevel *__restrict__ lv, int nlv,
+                                                  const int2 *__restrict__ grid_arena,
+                                                  float *__restrict__ hist_arena,
+                                                  float *__restrict__ norm_arena) {
+  constexpr int R = kHogVoteRows;
+  __shared__ int2 s_col[kHogVotePad];
+  __shared__ float s_acc[18 * R];
   const int bx = xcd_local_block(blockIdx.x, gridDim.x);
   int li = 0;
   while (li + 1 < nlv && lv[li + 1].blocks_begin <= bx) li++;
   const HogLevel L = lv[li];
   const int tile = bx - L.blocks_begin;
-  const int tcx = tile / L.tiles_y, tcy = tile - tcx * L.tiles_y;
-  const int tid = threadIdx.x;
-  for (int i = tid; i < 18 * T * T; i += T * T) s_acc[i] = 0.0f;
-  __syncthreads();
-  const int sbin = L.sbin, h = L.h, w = L.w;
-  const int cx = tcx * T + tid / T, cy = tcy * T + (tid & (T - 1));
-  if (cx >= L.b1 || cy >= L.b0) return;
-  const int vis0 = L.b0 * sbin, vis1 = L.b1 * sbin;
-  // pixels voting into this cell: interpolation cell in {c - 1, c}, within [1, vis - 2]
-  const int x0 = max(1, first_pix(cx - 1, sbin)), x1 = min(vis1 - 2, first_pix(cx + 1, sbin) - 1);
-  const int y0 = max(1, first_pix(cy - 1, sbin)), y1 = min(vis0 - 2, first_pix(cy + 1, sbin) - 1);
-  const int ny = y1 - y0 + 1;
-  float *acc = s_acc + tid;
+  const int cx = tile / L.tiles_y, cy0 = (tile - cx * L.tiles_y) * R;
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int o = 0; o < 18; o++) s_acc[o * R + lane] = 0.0f;
   const int2 *grid = grid_arena + L.grid_off;
-  if (2 * sbin <= JM / 2)
-    hog_vote_cell<JM / 2>(grid, h, w, sbin, cx, cy, x0, x1, y0, y1, ny, acc);
+  if (2 * L.sbin <= kHogVoteJM / 2)
+    hog_votes_cells<kHogVoteJM / 2>(L, grid, cx, cy0, s_col, s_acc);
   else
-    hog_vote_cell<JM>(grid, h, w, sbin, cx, cy, x0, x1, y0, y1, ny, acc);
+    hog_votes_cells<kHogVoteJM>(L, grid, cx, cy0, s_col, s_acc);
+  __syncthreads();
+  const int cy = cy0 + lane;
+  if (cy >= L.b0) return;
   const long long BSQ = (long long)L.b0 * L.b1;
   float *hist = hist_arena + L.hist_off + (long long)cx * L.b0 + cy;
+  const float *acc = s_acc + lane;
   float nrm = 0.0f;
 #pragma unroll
-  for (int o = 0; o < 18; o++) hist[o * BSQ] = acc[o * T * T];
+  for (int o = 0; o < 18; o++) hist[o * BSQ] = acc[o * R];
 #pragma unroll
   for (int o = 0; o < 9; o++) {
-    const float t = acc[o * T * T] + acc[(o + 9) * T * T];
+    const float t = acc[o * R] + acc[(o + 9) * R];
     nrm += t * t;
   }
   norm_arena[L.norm_off + (long long)cx * L.b0 + cy] = nrm;
@@ -1182,8 +1230,7 @@ void launch_hog_grad(hipStream_t s, const GradImg *imgs, int nimg, int blocks, c
 void launch_hog_votes(hipStream_t s, const HogLevel *lv, int nlv, int blocks, const int2 *grid,
                       float *hist, float *norm) {
   if (blocks > 0)
-    hipLaunchKernelGGL(k_hog_votes, dim3(blocks), dim3(kHogT * kHogT), 0, s, lv, nlv, grid, hist,
-                       norm);
+    hipLaunchKernelGGL(k_hog_votes, dim3(blocks), dim3(64), 0, s, lv, nlv, grid, hist, norm);
 }
 
 void launch_hog_feat(hipStream_t s, const HogLevel *lv, int nlv, int blocks, const float *hist,

@@ -347,7 +347,7 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
     P.grad_blocks = gb;
     P.grid_total = go;
   }
-  // ---- HOG levels (one vote launch over 16 x 16 cell tiles, then one feature launch)
+  // ---- HOG levels (one vote launch over cell-column tiles, then one feature launch)
   P.hog.clear();
   P.feat_off.assign(LEN, -1);
   long long ho = 0, no = 0, fo = 0, fbo = 0;
@@ -375,12 +375,13 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
     no += (long long)h.b0 * h.b1;
     fo += 31LL * L.F0 * L.F1;
     fbo += 32LL * L.F0 * L.F1;
-    h.T = 16;
-    h.tiles_x = (h.b1 + h.T - 1) / h.T;
+    h.T = kHogVoteRows;
+    h.tiles_x = h.b1;
     h.tiles_y = (h.b0 + h.T - 1) / h.T;
     HogLevel hh = h;
-    if (h.T == 16) { hh.blocks_begin = b16; b16 += h.tiles_x * h.tiles_y; h16.push_back(hh); }
-    else { hh.blocks_begin = b8; b8 += h.tiles_x * h.tiles_y; h8.push_back(hh); }
+    hh.blocks_begin = b16;
+    b16 += h.tiles_x * h.tiles_y;
+    h16.push_back(hh);
     HogLevel hft = h;
     hft.blocks_begin = bf;
     bf += (int)((1LL * L.F0 * L.F1 + 255) / 256);

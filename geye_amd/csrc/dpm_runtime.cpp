@@ -1693,8 +1693,8 @@ int geye_calc_feature(geye_ctx *c, const float *img, int height, int width, int 
   HogLevel L{};
   L.img_off = 0;
   L.h = h; L.w = w; L.sbin = sbin; L.b0 = b0; L.b1 = b1; L.F0 = *F0; L.F1 = *F1;
-  L.T = 16;
-  L.tiles_x = (b1 + L.T - 1) / L.T;
+  L.T = kHogVoteRows;
+  L.tiles_x = b1;
   L.tiles_y = (b0 + L.T - 1) / L.T;
   L.blocks_begin = 0;
   L.grid_off = 0;
