@@ -33,8 +33,10 @@ void launch_nms_prep_rows(hipStream_t s, const float *rows, int n, int GL, NmsBo
                           unsigned int *keys, int *vals);
 int launch_nms_sort(hipStream_t s, void *temp, size_t temp_bytes, unsigned int *keys_in,
                     unsigned int *keys_out, int *vals_in, int *vals_out, int n);
+// also zeroes counters[0..3], flags[0..3] and keep[0..n), and sets *nalive0 = n
 void launch_nms_gather(hipStream_t s, const int *ord, const NmsBox *box, int n, NmsBox *sbox,
-                       int *alive, int *status, int *sup, const int *cand, int *rank_of_flat);
+                       int *alive, int *status, int *sup, const int *cand, int *rank_of_flat,
+                       int *counters, int *nalive0, int *flags, int *keep);
 // One round: window bitmask, greedy, suppression of the later boxes, then the fused
 // retire-and-count pass that leaves per-block counts of the surviving later boxes in
 // blocksum.  nalive_host is a host upper bound of *nalive: the kernels read the true count

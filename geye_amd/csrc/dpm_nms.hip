@@ -104,14 +104,26 @@ __global__ __launch_bounds__(256) void k_nms_gather(const int *__restrict__ ord,
                                                     int *__restrict__ status,
                                                     int *__restrict__ sup,
                                                     const int *__restrict__ cand,
-                                                    int *__restrict__ rank_of_flat) {
+                                                    int *__restrict__ rank_of_flat,
+                                                    int *__restrict__ counters,
+                                                    int *__restrict__ nalive0,
+                                                    int *__restrict__ flags,
+                                                    int *__restrict__ keep) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
+  // the pass's other initial state: pick counters, the first round's alive count, the tie
+  // flags and the survivor flags (written by the finalisation, after the rounds)
+  if (r < 4) {
+    counters[r] = 0;
+    flags[r] = 0;
+  }
+  if (r == 0) *nalive0 = n;
   if (r >= n) return;
   const int i = ord[r];
   sbox[r] = box[i];
   alive[r] = r;
   status[r] = 0;
   sup[r] = kNoSup;
+  keep[r] = 0;
   if (rank_of_flat) rank_of_flat[cand[i]] = r;
 }
 
@@ -702,9 +714,10 @@ int launch_nms_sort(hipStream_t s, void *temp, size_t temp_bytes, unsigned int *
 }
 
 void launch_nms_gather(hipStream_t s, const int *ord, const NmsBox *box, int n, NmsBox *sbox,
-                       int *alive, int *status, int *sup, const int *cand, int *rank_of_flat) {
+                       int *alive, int *status, int *sup, const int *cand, int *rank_of_flat,
+                       int *counters, int *nalive0, int *flags, int *keep) {
   hipLaunchKernelGGL(k_nms_gather, dim3((n + 255) / 256), dim3(256), 0, s, ord, box, n, sbox,
-                     alive, status, sup, cand, rank_of_flat);
+                     alive, status, sup, cand, rank_of_flat, counters, nalive0, flags, keep);
 }
 
 void launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const int *nalive,

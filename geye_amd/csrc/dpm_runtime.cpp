@@ -877,14 +877,13 @@ int nms_core(geye_ctx *c, Plan &P, int n, float overlap, const NmsMaps *maps, co
       return fail(GEYE_ERR_HIP, "rocprim radix sort failed");
   }
   if (maps) launch_fill(s, maps->rank_of_flat, P.score_total, -1);
-  launch_nms_gather(s, dp<int>(P.n_ord), dp<NmsBox>(P.n_box), n, dp<NmsBox>(P.n_sbox),
-                    dp<int>(P.n_alive0), dp<int>(P.n_status), dp<int>(P.n_fsup), cand,
-                    maps ? maps->rank_of_flat : nullptr);
-  launch_fill(s, dp<int>(P.n_counters), 4, 0);
   // alive counts, double-buffered: round k reads na_buf[cur], its compaction writes the other
   int *na_buf[2] = {dp<int>(P.n_total), dp<int>(P.n_total) + 2};
   int cur = 0;
-  launch_fill(s, na_buf[0], 1, n);
+  launch_nms_gather(s, dp<int>(P.n_ord), dp<NmsBox>(P.n_box), n, dp<NmsBox>(P.n_sbox),
+                    dp<int>(P.n_alive0), dp<int>(P.n_status), dp<int>(P.n_fsup), cand,
+                    maps ? maps->rank_of_flat : nullptr, dp<int>(P.n_counters), na_buf[0],
+                    dp<int>(P.n_flags), dp<int>(P.n_keep_orig));
   int *alive = dp<int>(P.n_alive0), *alive_next = dp<int>(P.n_alive1);
   int *blocksum = dp<int>(P.d_blocksum);
   int *d_sel_total = dp<int>(P.n_total) + 1;  // count of the final survivor compaction
@@ -918,10 +917,8 @@ int nms_core(geye_ctx *c, Plan &P, int n, float overlap, const NmsMaps *maps, co
   // Finalisation with one read-back: the tie check and the survivor compaction are both
   // enqueued (the pick count stays on the device), then flags and count come back together.
   // A tie flag discards the device result for the host replay below.
-  launch_fill(s, dp<int>(P.n_flags), 4, 0);
   launch_nms_check(s, dp<NmsBox>(P.n_sbox), n, dp<int>(P.n_status), dp<int>(P.n_fsup),
                    dp<int>(P.n_picks), dp<int>(P.n_counters), dp<int>(P.n_flags));
-  launch_fill(s, dp<int>(P.n_keep_orig), (long long)N, 0);
   launch_nms_keep(s, dp<NmsBox>(P.n_sbox), dp<int>(P.n_picks), dp<int>(P.n_counters), n,
                   dp<int>(P.n_flags), dp<int>(P.n_keep_orig));
   launch_compact(s, dp<int>(P.n_keep_orig), (long long)N, dp<int>(P.d_blocksum), d_sel_total,
