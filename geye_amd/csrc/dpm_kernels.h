@@ -13,7 +13,7 @@ void launch_resize_cols(hipStream_t s, const ResizeOp *ops, int nops, int blocks
                         const int2 *tab, const int *si, const float *alpha, const float *src,
                         float *dst);
 // HOG histogram in two launches: gradients per pyramid image, then votes per level
-// (HogLevel.T must be 16: 16 x 16 cell tiles)
+// (HogLevel tiles: one cell column of kHogVoteRows cells per 64-lane workgroup)
 void launch_resize_fused(hipStream_t s, const FusedResize *ops, int nops, int blocks,
                          const int2 *tab, const int4 *si4, const float4 *al4, const float *src,
                          float *dst);
