@@ -948,6 +948,40 @@ __device__ inline void dt_window(const DtJob &J, const float *s_f, const unsigne
   }
 }
 
+// The selected outputs (DtJob::sel 1 or 2) of a job decided by the divide and conquer, in the
+// windowed form's half-grid layout (the same cells and places as dt_window writes).
+__device__ inline void dt_write_selected(const DtJob &J, const float *s_src, const int *s_ptr,
+                                         float *__restrict__ dst_arena,
+                                         int *__restrict__ ptr_arena) {
+  const int n = J.n, tid = threadIdx.x;
+  const float a = J.a, b = J.b;
+  float *dstc = dst_arena + J.dst_off;
+  unsigned short *pc = (unsigned short *)ptr_arena + 2 * J.dst_off;
+  if (J.sel == 1) {  // columns pass: M everywhere, the argmin on rows y == ay mod 2
+    const int H = (n + 1) >> 1;
+    float *dst = dstc + (long long)J.line0 * J.line_stride;
+    for (int i = tid; i < J.nlines * n; i += 256) {
+      const int l = i / n, p = i - l * n;
+      const int sp = s_ptr[l * n + p];
+      dst[(long long)l * J.line_stride + p] = dt_cost(s_src[l * n + sp], a, b, p - sp);
+      if (((p - J.ay) & 1) == 0)
+        pc[(long long)(J.line0 + l) * H + (p >> 1)] = (unsigned short)sp;
+    }
+  } else {  // rows pass: M on rows y == ay - 1 at x == ax - 1, the argmin on y == ay at x == ax
+    const int H = (J.elem_stride + 1) >> 1;
+    for (int i = tid; i < J.nlines * n; i += 256) {
+      const int p = i / J.nlines, l = i - p * J.nlines;
+      const int y = J.line0 + l;
+      const int kind = (y - J.ay + 1) & 1;  // 0: M, 1: argmin
+      if (((p - (J.ax - 1 + kind)) & 1) != 0) continue;
+      const int sp = s_ptr[l * n + p];
+      const long long oc = (long long)(p >> 1) * H + (y >> 1);
+      if (kind == 0) dstc[oc] = dt_cost(s_src[l * n + sp], a, b, p - sp);
+      else pc[oc] = (unsigned short)sp;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_dt_window(const DtJob *__restrict__ jobs,
                                                    const int4 *__restrict__ nodes,
                                                    const int *__restrict__ depths,
@@ -994,7 +1028,8 @@ __global__ __launch_bounds__(256) void k_dt_window(const DtJob *__restrict__ job
     dt_load_nodes(J, nodes, s_node);
     __syncthreads();
     dt_divide_conquer(J, depths, s_f, n, 0, s_ptr, s_node);
-    dt_write(J, s_f, n, 0, s_ptr, dst_arena, ptr_arena);
+    if (J.sel == 0) dt_write(J, s_f, n, 0, s_ptr, dst_arena, ptr_arena);
+    else dt_write_selected(J, s_f, s_ptr, dst_arena, ptr_arena);
     return;
   }
   const int T = need <= 8 ? 8 : need <= 10 ? 10 : need <= 12 ? 12 : need <= 14 ? 14

@@ -331,6 +331,43 @@ def test_mfma_part_boxes_match_fp32(ctx, prec):
     assert same >= 0.97 * total
 
 
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_detect_mfma_weak_deformation(ctx, prec):
+    """Quadratic deformation costs 20x weaker (a = 5e-4): a job whose part responses span
+    more than a * 24^2 = 0.29 needs a window wider than the windowed DT's 24, so its jobs
+    take the divide and conquer, which must still write the selected outputs where the
+    scoring and the part boxes read them.  Scores against the FP64 oracle (0.02 absolute),
+    part boxes against the FP32 form's."""
+    say("weak deformation", prec, "320x240 synthetic car")
+    m = synth.synthetic_car()
+    m.deff = (m.deff * np.array([0.05, 1.0, 0.05, 1.0], dtype=np.float32)).astype(np.float32)
+    frame = synth.make_frame(320, 240, seed=8)
+    gm = ctx.model(m)
+    g16 = gm.get_boxes(frame, thresh=-0.5, precision=prec)
+    g = gm.last_scores()
+    maps = O.score_maps(m, frame, precision=1)
+    ref = np.concatenate([mp.ravel() for _, _, mp in maps])
+    err = np.abs(g.astype(np.float64) - ref)
+    say(prec, "score max abs err", float(err.max()))
+    assert err.max() < 0.02
+    g32 = gm.get_boxes(frame, thresh=-0.5, precision="fp32")
+    GL = g32.shape[1]
+    npart = (GL - 7) // 4
+    key = lambda r: (tuple(r[0:4].tolist()), float(r[GL - 3]), float(r[GL - 1]))
+    ref_rows = {key(r): r for r in g32}
+    same = total = 0
+    for r in g16:
+        o = ref_rows.get(key(r))
+        if o is None:
+            continue
+        same += int(np.sum(np.all(r[4:4 + 4 * npart].reshape(npart, 4) ==
+                                  o[4:4 + 4 * npart].reshape(npart, 4), axis=1)))
+        total += npart
+    say(prec, "matched part boxes", same, "of", total)
+    assert total > 100
+    assert same >= 0.9 * total
+
+
 @pytest.mark.parametrize("which,W,H,thresh", [("car", 640, 480, -0.5), ("car", 640, 480, -1.2),
                                               ("synthetic", 480, 360, -0.5)])
 def test_accumulated_score_matches_oracle(ctx, car_model, which, W, H, thresh):
