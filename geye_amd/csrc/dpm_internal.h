@@ -141,6 +141,9 @@ struct DtJob {
   // + (y >> 1) (the columns pass's argmins at x * ceil(P0 / 2) + (y >> 1)), argmins as
   // 16-bit indices
   int sel, ax, ay;
+  // windowed form: lines per batch of the divide and conquer it falls back to when the
+  // window would be too wide (float lines, argmins and node table within the key lines' LDS)
+  int fb_lines;
 };
 
 struct ScoreMap {              // one (valid level, component) root score map

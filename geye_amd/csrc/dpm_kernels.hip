@@ -863,8 +863,8 @@ __global__ __launch_bounds__(256) void k_dt(const DtJob *__restrict__ jobs,
 // evaluated from the float source.
 // ---------------------------------------------------------------------------------------
 template <int T, bool ROWS>
-__device__ inline void dt_window(const DtJob &J, const float *s_f, const unsigned *s_key,
-                                 int LSk, const unsigned long long *s_q,
+__device__ inline void dt_window(const DtJob &J, const float *__restrict__ src_arena,
+                                 const unsigned *s_key, int LSk, const unsigned long long *s_q,
                                  float *__restrict__ dst_arena, int *__restrict__ ptr_arena) {
   constexpr int NP = T + 1;  // 2T + 1 positions as T + 1 pairs (the last one padded)
   const int n = J.n, nl = J.nlines;
@@ -879,6 +879,10 @@ __device__ inline void dt_window(const DtJob &J, const float *s_f, const unsigne
     q[i] = ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32)) << 32) |
            (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
   }
+  // M takes the source value at the argmin from memory (the job has just read it, so it
+  // comes from L2), negated as staged
+  const float *src = src_arena + J.src_off + (long long)J.line0 * J.line_stride;
+  const float sgn = J.neg ? -1.0f : 1.0f;
   float *dst = dst_arena + J.dst_off + (long long)J.line0 * J.line_stride;
   int *pdst = ptr_arena + J.dst_off + (long long)J.line0 * J.line_stride;
   // the selected outputs go to half grids of the map (H = ceil(P0 / 2) cells per column):
@@ -923,7 +927,11 @@ __device__ inline void dt_window(const DtJob &J, const float *s_f, const unsigne
         best = min(best, min((unsigned)k, (unsigned)(k >> 32)));
       }
       const int s = dd - T + (int)(best & 63u);
-      const float v = dt_cost(s_f[l * n + s], a, b, dd - s);
+      float v = 0.0f;
+      if (!sel_rows || kind == 0) {
+        const float f = src[(long long)l * J.line_stride + (long long)s * J.elem_stride];
+        v = dt_cost(J.neg ? f * sgn : f, a, b, dd - s);
+      }
       if (sel_rows) {
         // half grid: (y, x) at (x >> 1) * H + (y >> 1), M or the 16-bit argmin
         const long long oc = (long long)(dd >> 1) * H + ((J.line0 + l) >> 1);
@@ -994,11 +1002,11 @@ __global__ __launch_bounds__(256) void k_dt_window(const DtJob *__restrict__ job
   const DtJob J = jobs[xcd_local_block(blockIdx.x, gridDim.x)];
   const int n = J.n, NLn = 256 / J.G, LSk = dt_key_stride(n, NLn);
   const int tid = threadIdx.x;
-  float *s_f = smem;                                            // [NLn][n] source
-  unsigned *s_key = (unsigned *)(smem + NLn * n);               // [NLn][LSk] keys
-  int2 *s_node = (int2 *)(s_key + NLn * LSk + ((NLn * (n + LSk)) & 1));  // [n]
+  // [NLn][LSk] key lines: the source is staged in place (between the halos) and turned into
+  // keys there, so a job needs no separate float copy and more jobs share a CU
+  unsigned *s_key = (unsigned *)smem;
   float vmin = __builtin_inff(), vmax = -__builtin_inff();
-  dt_stage<true>(J, src_arena, s_f, n, 0, vmin, vmax);
+  dt_stage<true>(J, src_arena, smem, LSk, kDtHalo, vmin, vmax);
   for (int m = 1; m < 64; m <<= 1) {
     vmin = fminf(vmin, __shfl_xor(vmin, m));
     vmax = fmaxf(vmax, __shfl_xor(vmax, m));
@@ -1023,13 +1031,25 @@ __global__ __launch_bounds__(256) void k_dt_window(const DtJob *__restrict__ job
     if (h < 64.0f) need = (int)floorf(h) + 1;
   }
   if (need > kDtHalo) {
-    // the reference's divide and conquer on the float lines; argmins go to the key area
-    int *s_ptr = (int *)s_key;
+    // the reference's divide and conquer, J.fb_lines lines at a time (float lines, argmins
+    // and the node table fit the key area); the lines are staged again from memory
+    const int fb = J.fb_lines;
+    float *s_f = smem;                                                // [fb][n]
+    int *s_ptr = (int *)(smem + fb * n);                              // [fb][n]
+    int2 *s_node = (int2 *)(s_ptr + fb * n + ((2 * fb * n) & 1));     // [n]
     dt_load_nodes(J, nodes, s_node);
-    __syncthreads();
-    dt_divide_conquer(J, depths, s_f, n, 0, s_ptr, s_node);
-    if (J.sel == 0) dt_write(J, s_f, n, 0, s_ptr, dst_arena, ptr_arena);
-    else dt_write_selected(J, s_f, s_ptr, dst_arena, ptr_arena);
+    for (int l0 = 0; l0 < J.nlines; l0 += fb) {
+      DtJob Js = J;
+      Js.line0 = J.line0 + l0;
+      Js.nlines = min(fb, J.nlines - l0);
+      float u0 = 0.0f, u1 = 0.0f;
+      dt_stage<false>(Js, src_arena, s_f, n, 0, u0, u1);
+      __syncthreads();
+      dt_divide_conquer(Js, depths, s_f, n, 0, s_ptr, s_node);
+      if (J.sel == 0) dt_write(Js, s_f, n, 0, s_ptr, dst_arena, ptr_arena);
+      else dt_write_selected(Js, s_f, s_ptr, dst_arena, ptr_arena);
+      __syncthreads();
+    }
     return;
   }
   const int T = need <= 8 ? 8 : need <= 10 ? 10 : need <= 12 ? 12 : need <= 14 ? 14
@@ -1056,7 +1076,7 @@ __global__ __launch_bounds__(256) void k_dt_window(const DtJob *__restrict__ job
     for (int i = tid; i < J.nlines * LSk; i += 256) {
       const int p = h - kDtHalo;
       unsigned k = 0x80000000u;  // halo: above every window cost, no carry out of 32 bits
-      if (p >= 0 && p < n) k = (unsigned)((s_f[l * n + p] - vmin) * scale + 0.5f) << 6;
+      if (p >= 0 && p < n) k = (unsigned)((__uint_as_float(s_key[i]) - vmin) * scale + 0.5f) << 6;
       s_key[i] = k;
       l += sl;
       h += sh;
@@ -1069,8 +1089,8 @@ __global__ __launch_bounds__(256) void k_dt_window(const DtJob *__restrict__ job
   __syncthreads();
   const bool rows = J.elem_stride != 1;
 #define GEYE_DT_WINDOW(TT)                                                              \
-  (rows ? dt_window<TT, true>(J, s_f, s_key, LSk, s_q, dst_arena, ptr_arena)            \
-        : dt_window<TT, false>(J, s_f, s_key, LSk, s_q, dst_arena, ptr_arena))
+  (rows ? dt_window<TT, true>(J, src_arena, s_key, LSk, s_q, dst_arena, ptr_arena)      \
+        : dt_window<TT, false>(J, src_arena, s_key, LSk, s_q, dst_arena, ptr_arena))
   if (T == 8) GEYE_DT_WINDOW(8);
   else if (T == 10) GEYE_DT_WINDOW(10);
   else if (T == 12) GEYE_DT_WINDOW(12);

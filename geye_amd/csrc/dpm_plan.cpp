@@ -156,11 +156,29 @@ static size_t dt_lds_bytes(int n, int G) {
   return (words + (words & 1)) * 4 + (size_t)n * 8;
 }
 
-// the windowed DT: float lines (n) and key lines (n plus halos, padded); node table
-static size_t dt_lds_bytes_window(int n, int G) {
+// the windowed DT: key lines (n plus halos, padded), the source staged into them
+static size_t dt_lds_bytes_window_keys(int n, int G) {
   const int NL = 256 / G;
-  const size_t words = (size_t)NL * (size_t)(n + dt_key_stride(n, NL));
+  return (size_t)NL * (size_t)dt_key_stride(n, NL) * 4;
+}
+
+// its divide-and-conquer fallback, fb lines at a time: float lines, argmins, node table
+static size_t dt_lds_bytes_fallback(int n, int fb) {
+  const size_t words = 2 * (size_t)fb * (size_t)n;
   return (words + (words & 1)) * 4 + (size_t)n * 8;
+}
+
+// the most fallback lines per batch within the key lines' LDS (one always fits: 16 n bytes
+// against at least 4 (n + 48) words)
+static int dt_fallback_lines(int n, int G) {
+  int fb = 256 / G;
+  while (fb > 1 && dt_lds_bytes_fallback(n, fb) > dt_lds_bytes_window_keys(n, G)) fb--;
+  return fb;
+}
+
+static size_t dt_lds_bytes_window(int n, int G) {
+  return std::max(dt_lds_bytes_window_keys(n, G),
+                  dt_lds_bytes_fallback(n, dt_fallback_lines(n, G)));
 }
 
 // lanes per line: the fewest whose block fits 48 KB of LDS (several blocks per CU), else a
@@ -168,15 +186,16 @@ static size_t dt_lds_bytes_window(int n, int G) {
 static int dt_pick_G(int n) {
   for (int G = 1; G <= 64; G *= 2)
     if (dt_lds_bytes(n, G) <= 48 * 1024) return G;
-  return dt_lds_bytes_window(n, 64) <= (size_t)kDtWindowMaxLds ? 64 : -1;
+  return dt_lds_bytes(n, 64) <= (size_t)kDtWindowMaxLds ? 64 : -1;
 }
 
-// the same for the windowed kernel, whose lines carry halos, within 24 KB: each job is a
-// load - compute - store sequence with barriers between, so several jobs per CU overlap
-// their phases (on 1080p part maps 24 KB beat 16, 20, 28, 32 and 48 KB)
+// the same for the windowed kernel, whose lines carry halos, within kDtWindowJobLds: each
+// job is a load - compute - store sequence with barriers between, so several jobs per CU
+// overlap their phases (on 1080p part maps 24 KB of key lines beat 12, 16, 32 and 40 KB)
+constexpr size_t kDtWindowJobLds = 24 * 1024;
 static int dt_pick_G_window(int n) {
   for (int G = 1; G <= 64; G *= 2)
-    if (dt_lds_bytes_window(n, G) <= 24 * 1024) return G;
+    if (dt_lds_bytes_window(n, G) <= kDtWindowJobLds) return G;
   return dt_lds_bytes_window(n, 64) <= (size_t)kDtWindowMaxLds ? 64 : -1;
 }
 
@@ -207,6 +226,7 @@ static void add_dt_jobs(Plan &P, std::vector<DtJob> &jobs, long long src_off, lo
     J.sel = sel;
     J.ax = ax;
     J.ay = ay;
+    J.fb_lines = window ? dt_fallback_lines(n, G) : 0;
     jobs.push_back(J);
   }
 }
