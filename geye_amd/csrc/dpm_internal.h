@@ -110,11 +110,13 @@ constexpr int kDtHalo = 24;
 // dynamic LDS limit of the windowed DT kernel (160 KB less its static reduction buffer)
 constexpr int kDtWindowMaxLds = 160 * 1024 - 256;
 
-// LDS words per key line of the windowed DT (the line and both halos), padded so that the
-// row pass's lanes, lines fastest, fall on distinct banks of the 32 that serve each half
-// wave: with NL <= 32 lines the stride is 32 / NL modulo 32, otherwise odd.
+// LDS words per key line of the windowed DT: the line and one halo, which it shares with the
+// next line (line l at l * stride + kDtHalo, kDtHalo more words after the last line; every
+// halo word holds the same never-minimal key).  Padded so that the row pass's lanes, lines
+// fastest, fall on distinct banks of the 32 that serve each half wave: with NL <= 32 lines
+// the stride is 32 / NL modulo 32, otherwise odd.
 __host__ __device__ inline int dt_key_stride(int n, int NL) {
-  int w = n + 2 * kDtHalo;
+  int w = n + kDtHalo;
   if (NL <= 32) {
     const int r = 32 / NL;
     w += ((r - w) % 32 + 32) % 32;

@@ -1002,8 +1002,9 @@ __global__ __launch_bounds__(256) void k_dt_window(const DtJob *__restrict__ job
   const DtJob J = jobs[xcd_local_block(blockIdx.x, gridDim.x)];
   const int n = J.n, NLn = 256 / J.G, LSk = dt_key_stride(n, NLn);
   const int tid = threadIdx.x;
-  // [NLn][LSk] key lines: the source is staged in place (between the halos) and turned into
-  // keys there, so a job needs no separate float copy and more jobs share a CU
+  // [NLn][LSk] + kDtHalo key lines with shared halos: the source is staged in place (between
+  // the halos) and turned into keys there, so a job needs no separate float copy and more
+  // jobs share a CU
   unsigned *s_key = (unsigned *)smem;
   float vmin = __builtin_inff(), vmax = -__builtin_inff();
   dt_stage<true>(J, src_arena, smem, LSk, kDtHalo, vmin, vmax);
@@ -1073,7 +1074,7 @@ __global__ __launch_bounds__(256) void k_dt_window(const DtJob *__restrict__ job
   {
     const int sl = 256 / LSk, sh = 256 - sl * LSk;
     int l = tid / LSk, h = tid - l * LSk;
-    for (int i = tid; i < J.nlines * LSk; i += 256) {
+    for (int i = tid; i < J.nlines * LSk + kDtHalo; i += 256) {
       const int p = h - kDtHalo;
       unsigned k = 0x80000000u;  // halo: above every window cost, no carry out of 32 bits
       if (p >= 0 && p < n) k = (unsigned)((__uint_as_float(s_key[i]) - vmin) * scale + 0.5f) << 6;

@@ -159,7 +159,7 @@ static size_t dt_lds_bytes(int n, int G) {
 // the windowed DT: key lines (n plus halos, padded), the source staged into them
 static size_t dt_lds_bytes_window_keys(int n, int G) {
   const int NL = 256 / G;
-  return (size_t)NL * (size_t)dt_key_stride(n, NL) * 4;
+  return ((size_t)NL * (size_t)dt_key_stride(n, NL) + kDtHalo) * 4;
 }
 
 // its divide-and-conquer fallback, fb lines at a time: float lines, argmins, node table
@@ -169,7 +169,7 @@ static size_t dt_lds_bytes_fallback(int n, int fb) {
 }
 
 // the most fallback lines per batch within the key lines' LDS (one always fits: 16 n bytes
-// against at least 4 (n + 48) words)
+// against at least 4 (n + 24) + 24 words)
 static int dt_fallback_lines(int n, int G) {
   int fb = 256 / G;
   while (fb > 1 && dt_lds_bytes_fallback(n, fb) > dt_lds_bytes_window_keys(n, G)) fb--;
