@@ -365,7 +365,7 @@ def test_detect_mfma_weak_deformation(ctx, prec):
         total += npart
     say(prec, "matched part boxes", same, "of", total)
     assert total > 100
-    assert same >= 0.9 * total
+    assert same >= 0.97 * total  # measured 99.5% (bf16) and 99.9% (fp16) at 320x240
 
 
 @pytest.mark.parametrize("which,W,H,thresh", [("car", 640, 480, -0.5), ("car", 640, 480, -1.2),
