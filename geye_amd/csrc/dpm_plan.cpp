@@ -191,7 +191,7 @@ static int dt_pick_G(int n) {
 
 // the same for the windowed kernel, whose lines carry halos, within kDtWindowJobLds: each
 // job is a load - compute - store sequence with barriers between, so several jobs per CU
-// overlap their phases (on 1080p part maps 24 KB of key lines beat 12, 16, 32 and 40 KB)
+// overlap their phases (on 1080p part maps 24 KB of key lines beat 12, 16, 20, 28, 32 and 40 KB)
 constexpr size_t kDtWindowJobLds = 24 * 1024;
 static int dt_pick_G_window(int n) {
   for (int G = 1; G <= 64; G *= 2)
