@@ -49,7 +49,10 @@ def test_resize_matches_reference_vectors(ctx):
 
 
 @pytest.mark.parametrize("W,H,sbin", [(64, 48, 8), (64, 48, 4), (133, 97, 8), (133, 97, 4),
-                                      (320, 240, 4), (203, 161, 6), (37, 29, 8)])
+                                      (320, 240, 4), (203, 161, 6), (37, 29, 8),
+                                      # 65 and 129 cell rows: vote tiles of 64 cells with one
+                                      # left over; 65 cell columns
+                                      (40, 520, 8), (36, 516, 4), (520, 40, 8)])
 def test_calc_feature_matches_oracle(ctx, W, H, sbin):
     say("calc_feature", W, H, sbin)
     img = O.ipl_to_float(synth.make_frame(W, H, seed=3 + W))
