@@ -1127,13 +1127,24 @@ __global__ __launch_bounds__(256) void k_score(const ScoreMap *__restrict__ maps
   const long long e = m.score_off + i;
   const int x = (int)(i / m.R0), y = (int)(i - (long long)x * m.R0);
   float s = conv_arena[m.root_off + i] + m.off;
-  for (int k = 0; k < m.npart; k++) {
-    const ScorePart q = parts[m.part_first + k];
-    // half grid (the windowed DT's selected outputs): cell (ay - 1 + 2y, ax - 1 + 2x) sits
-    // at (x + (ax - 1) / 2) * ceil(P0 / 2) + y + (ay - 1) / 2
-    s -= half ? M_arena[q.M_off + (long long)(x + ((q.ax - 1) >> 1)) * ((q.P0 + 1) >> 1) + y +
-                        ((q.ay - 1) >> 1)]
-              : M_arena[q.M_off + (q.ay - 1 + 2 * y) + (long long)(q.ax - 1 + 2 * x) * q.P0];
+  // up to 8 parts' values are loaded together, then subtracted in part order
+  for (int k0 = 0; k0 < m.npart; k0 += 8) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      v[u] = 0.0f;
+      if (k0 + u < m.npart) {
+        const ScorePart q = parts[m.part_first + k0 + u];
+        // half grid (the windowed DT's selected outputs): cell (ay - 1 + 2y, ax - 1 + 2x)
+        // sits at (x + (ax - 1) / 2) * ceil(P0 / 2) + y + (ay - 1) / 2
+        v[u] = half ? M_arena[q.M_off + (long long)(x + ((q.ax - 1) >> 1)) * ((q.P0 + 1) >> 1) +
+                              y + ((q.ay - 1) >> 1)]
+                    : M_arena[q.M_off + (q.ay - 1 + 2 * y) + (long long)(q.ax - 1 + 2 * x) * q.P0];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (k0 + u < m.npart) s -= v[u];
   }
   score[e] = s;
   flags[e] = s > thresh ? 1 : 0;
