@@ -1237,28 +1237,45 @@ __global__ __launch_bounds__(256) void k_rows(const ScoreMap *__restrict__ maps,
   r[1] = r1;
   r[2] = (float)((double)(r0 + (float)rsize[m.comp * 2] * scale) - 1.0);
   r[3] = (float)((double)(r1 + (float)rsize[m.comp * 2 + 1] * scale) - 1.0);
-  for (int k = 0; k < m.npart; k++) {
-    const ScorePart q = parts[m.part_first + k];
-    const int probex = (x - 1) * 2 + q.ax, probey = (y - 1) * 2 + q.ay;
-    int ix, iy;
-    if (half) {
-      // 16-bit half grids: (y, x) at (x >> 1) * ceil(P0 / 2) + (y >> 1), the columns pass's
-      // argmins at every column and rows y == ay mod 2 (the probe's parity)
-      const long long H = (q.P0 + 1) >> 1;
-      ix = ((const unsigned short *)tmpIx)[2 * q.Ix_off + (probex >> 1) * H + (probey >> 1)];
-      iy = ((const unsigned short *)tmpIy)[2 * q.Iy_off + ix * H + (probey >> 1)];
-    } else {
-      ix = tmpIx[q.Ix_off + probey + (long long)probex * q.P0];
-      iy = tmpIy[q.Iy_off + (long long)ix * q.P0 + probey];
+  // up to 8 parts at a time: their Ix loads go out together, then their Iy loads
+  for (int k0 = 0; k0 < m.npart; k0 += 8) {
+    ScorePart q[8];
+    int ix[8], iy[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      ix[u] = 0;
+      if (k0 + u >= m.npart) continue;
+      q[u] = parts[m.part_first + k0 + u];
+      const int probex = (x - 1) * 2 + q[u].ax, probey = (y - 1) * 2 + q[u].ay;
+      // 16-bit half grids: (y, x) at (x >> 1) * ceil(P0 / 2) + (y >> 1)
+      ix[u] = half ? ((const unsigned short *)tmpIx)[2 * q[u].Ix_off +
+                                                     (long long)(probex >> 1) * ((q[u].P0 + 1) >> 1) +
+                                                     (probey >> 1)]
+                   : tmpIx[q[u].Ix_off + probey + (long long)probex * q[u].P0];
     }
-    const float px = (float)ix + 1.0f, py = (float)iy + 1.0f;
-    float b0 = (float)((((double)py - 2.0) / 2.0 + 1.0 - (double)(float)pady) * (double)scale);
-    float b1 = (float)((((double)px - 2.0) / 2.0 + 1.0 - (double)(float)padx) * (double)scale);
-    float *o = r + 4 + 4 * k;
-    o[0] = b0;
-    o[1] = b1;
-    o[2] = (float)(((double)b0 + (double)((float)q.ps0 * scale) / 2.0) - 1.0);
-    o[3] = (float)(((double)b1 + (double)((float)q.ps1 * scale) / 2.0) - 1.0);
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      iy[u] = 0;
+      if (k0 + u >= m.npart) continue;
+      const int probey = (y - 1) * 2 + q[u].ay;
+      // the columns pass's argmins: every column, rows y == ay mod 2 (the probe's parity)
+      iy[u] = half ? ((const unsigned short *)tmpIy)[2 * q[u].Iy_off +
+                                                     (long long)ix[u] * ((q[u].P0 + 1) >> 1) +
+                                                     (probey >> 1)]
+                   : tmpIy[q[u].Iy_off + (long long)ix[u] * q[u].P0 + probey];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      if (k0 + u >= m.npart) continue;
+      const float px = (float)ix[u] + 1.0f, py = (float)iy[u] + 1.0f;
+      float b0 = (float)((((double)py - 2.0) / 2.0 + 1.0 - (double)(float)pady) * (double)scale);
+      float b1 = (float)((((double)px - 2.0) / 2.0 + 1.0 - (double)(float)padx) * (double)scale);
+      float *o = r + 4 + 4 * (k0 + u);
+      o[0] = b0;
+      o[1] = b1;
+      o[2] = (float)(((double)b0 + (double)((float)q[u].ps0 * scale) / 2.0) - 1.0);
+      o[3] = (float)(((double)b1 + (double)((float)q[u].ps1 * scale) / 2.0) - 1.0);
+    }
   }
   r[4 + 4 * m.npart] = (float)m.comp;
   r[5 + 4 * m.npart] = score[e];
