@@ -105,6 +105,10 @@ __device__ inline int preceding_count(const int *__restrict__ blocksum, int b, i
   return r;
 }
 
+// Slots of a context's mapped host counts: 0 and 1 alternate between NMS rounds; the
+// candidate count of the last scoring pass; the survivor count of the last NMS.
+constexpr int kHostCandidates = 2, kHostSurvivors = 3;
+
 // Halo on each side of an LDS line in the windowed DT: the widest window half-width.
 constexpr int kDtHalo = 24;
 // dynamic LDS limit of the windowed DT kernel (160 KB less its static reduction buffer)
@@ -321,8 +325,8 @@ struct geye_ctx {
   geye::Plan *last_plan = nullptr;
   bool graphs = true;                 // replay the per-frame front as a HIP graph
   int num_cu = 256;
-  int *h_count = nullptr;             // pinned host slots for NMS survivor counts (coherent,
-  int *d_hcount = nullptr;            // mapped: the compaction kernel writes them directly)
+  int *h_count = nullptr;             // pinned host slots for counts (coherent, mapped: the
+  int *d_hcount = nullptr;            // compaction kernels write them directly; see kHost*)
   hipEvent_t ev_count[2] = {nullptr, nullptr};
   long long nms_fallbacks = 0;
 };

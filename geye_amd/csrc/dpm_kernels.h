@@ -35,8 +35,9 @@ void launch_score(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePar
                   const float *conv, const float *M, float thresh, int half, float *score,
                   int *flags, int blocks);
 int compact_blocks(long long n);
+// host_total (optional): mapped host memory that also receives the count
 void launch_compact(hipStream_t s, const int *flags, long long n, int *blocksum, int *total,
-                    int *out);
+                    int *out, int *host_total = nullptr);
 void launch_rows(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,
                  const float *score, const int *cand, int ncand, const int *tmpIx,
                  const int *tmpIy, int half, int GL, int padx, int pady, const int *rsize,

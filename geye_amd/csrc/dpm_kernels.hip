@@ -1176,6 +1176,7 @@ __global__ __launch_bounds__(256) void k_flag_scatter(const int *__restrict__ fl
                                                       long long n,
                                                       const int *__restrict__ blocksum,
                                                       int *__restrict__ total,
+                                                      int *__restrict__ host_total,
                                                       int *__restrict__ out) {
   __shared__ int s[256];
   __shared__ int s_red[4];
@@ -1198,7 +1199,10 @@ __global__ __launch_bounds__(256) void k_flag_scatter(const int *__restrict__ fl
     __syncthreads();
   }
   int pos = before + s[threadIdx.x] - c;
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 255) *total = before + s[255];
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 255) {
+    *total = before + s[255];
+    if (host_total) *(volatile int *)host_total = before + s[255];  // mapped host memory
+  }
 #pragma unroll
   for (int k = 0; k < 4; k++)
     if (f[k]) out[pos++] = (int)(base + threadIdx.x * 4 + k);
@@ -1379,10 +1383,11 @@ int compact_blocks(long long n) { return (int)((n + kCompactBlock - 1) / kCompac
 // flagged indices of [0, n) in order to out, their count to *total (one workgroup at least,
 // so that an empty input writes 0)
 void launch_compact(hipStream_t s, const int *flags, long long n, int *blocksum, int *total,
-                    int *out) {
+                    int *out, int *host_total) {
   const int nb = compact_blocks(n) > 0 ? compact_blocks(n) : 1;
   hipLaunchKernelGGL(k_flag_count, dim3(nb), dim3(256), 0, s, flags, n, blocksum);
-  hipLaunchKernelGGL(k_flag_scatter, dim3(nb), dim3(256), 0, s, flags, n, blocksum, total, out);
+  hipLaunchKernelGGL(k_flag_scatter, dim3(nb), dim3(256), 0, s, flags, n, blocksum, total,
+                     host_total, out);
 }
 
 void launch_rows(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,

@@ -797,7 +797,7 @@ int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int 
     {
       KTimer t(c, "compact");
       launch_compact(s, dp<int>(P.d_flags), P.score_total, dp<int>(P.d_blocksum),
-                     dp<int>(P.d_count), dp<int>(P.d_cand));
+                     dp<int>(P.d_count), dp<int>(P.d_cand), c->d_hcount + kHostCandidates);
     }
   }
   HIPCHK(hipGetLastError());
@@ -808,10 +808,9 @@ int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int 
 int read_count(geye_ctx *c, Plan &P, int *ncand) {
   *ncand = 0;
   if (P.score_total <= 0) return GEYE_OK;
-  int n = 0;
-  HIPCHK(hipMemcpyAsync(&n, P.d_count.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  // the compaction's last workgroup wrote the count into the context's mapped host slot
   HIPCHK(hipStreamSynchronize(c->stream));
-  *ncand = n;
+  *ncand = ((volatile int *)c->h_count)[kHostCandidates];
   return GEYE_OK;
 }
 
@@ -922,12 +921,11 @@ int nms_core(geye_ctx *c, Plan &P, int n, float overlap, const NmsMaps *maps, co
   launch_nms_keep(s, dp<NmsBox>(P.n_sbox), dp<int>(P.n_picks), dp<int>(P.n_counters), n,
                   dp<int>(P.n_flags), dp<int>(P.n_keep_orig));
   launch_compact(s, dp<int>(P.n_keep_orig), (long long)N, dp<int>(P.d_blocksum), d_sel_total,
-                 dp<int>(P.n_kept));
+                 dp<int>(P.n_kept), c->d_hcount + kHostSurvivors);
   int flags[2] = {0, 0};
-  int nk_dev = 0;
   HIPCHK(hipMemcpyAsync(flags, P.n_flags.p, sizeof(flags), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(&nk_dev, d_sel_total, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  const int nk_dev = ((volatile int *)c->h_count)[kHostSurvivors];
   int nk = 0;
   if (flags[0] != 0) {
     // equal scores decided an outcome: replay the reference's exact quicksort order on the host
