@@ -106,8 +106,9 @@ __device__ inline int preceding_count(const int *__restrict__ blocksum, int b, i
 }
 
 // Slots of a context's mapped host counts: 0 and 1 alternate between NMS rounds; the
-// candidate count of the last scoring pass; the survivor count of the last NMS.
-constexpr int kHostCandidates = 2, kHostSurvivors = 3;
+// candidate count of the last scoring pass; the survivor count and the tie flag of the last
+// NMS.
+constexpr int kHostCandidates = 2, kHostSurvivors = 3, kHostTie = 4;
 
 // Halo on each side of an LDS line in the windowed DT: the widest window half-width.
 constexpr int kDtHalo = 24;

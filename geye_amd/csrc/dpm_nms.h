@@ -56,7 +56,8 @@ void launch_nms_compact(hipStream_t s, const int *alive, const int *nalive, cons
 void launch_nms_check(hipStream_t s, const NmsBox *sbox, int n, const int *status, const int *sup,
                       const int *picks, const int *counters, int *flags);
 void launch_nms_keep(hipStream_t s, const NmsBox *sbox, const int *picks, const int *counters,
-                     int bound, const int *flags, int *keep_orig);
+                     int bound, const int *flags, int *keep_orig,
+                     int *host_tie);
 void launch_fill(hipStream_t s, int *p, long long n, int v);
 void launch_take(hipStream_t s, const int *src, const int *idx, int n, int *dst);
 

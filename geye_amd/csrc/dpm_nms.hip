@@ -650,8 +650,11 @@ __global__ __launch_bounds__(256) void k_nms_keep(const NmsBox *__restrict__ sbo
                                                   const int *__restrict__ picks,
                                                   const int *__restrict__ counters,
                                                   const int *__restrict__ flags,
-                                                  int *__restrict__ keep_orig) {
+                                                  int *__restrict__ keep_orig,
+                                                  int *__restrict__ host_tie) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  // the check has finished: its tie flag goes to the mapped host slot
+  if (k == 0 && host_tie) *(volatile int *)host_tie = flags[0];
   const int npick = counters[0];
   if (k >= npick) return;
   if (k == npick - 1 && flags[1] == 0) return;
@@ -773,10 +776,10 @@ void launch_nms_check(hipStream_t s, const NmsBox *sbox, int n, const int *statu
 }
 
 void launch_nms_keep(hipStream_t s, const NmsBox *sbox, const int *picks, const int *counters,
-                     int bound, const int *flags, int *keep_orig) {
+                     int bound, const int *flags, int *keep_orig, int *host_tie) {
   if (bound > 0)
     hipLaunchKernelGGL(k_nms_keep, dim3((bound + 255) / 256), dim3(256), 0, s, sbox, picks,
-                       counters, flags, keep_orig);
+                       counters, flags, keep_orig, host_tie);
 }
 
 void launch_fill(hipStream_t s, int *p, long long n, int v) {

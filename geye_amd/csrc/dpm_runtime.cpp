@@ -913,21 +913,21 @@ int nms_core(geye_ctx *c, Plan &P, int n, float overlap, const NmsMaps *maps, co
     slot ^= 1;
     if (na == 0) break;
   }
-  // Finalisation with one read-back: the tie check and the survivor compaction are both
-  // enqueued (the pick count stays on the device), then flags and count come back together.
+  // Finalisation with one wait: the tie check and the survivor compaction are both enqueued
+  // (the pick count stays on the device); the tie flag and the count arrive in the context's
+  // mapped host slots.
   // A tie flag discards the device result for the host replay below.
   launch_nms_check(s, dp<NmsBox>(P.n_sbox), n, dp<int>(P.n_status), dp<int>(P.n_fsup),
                    dp<int>(P.n_picks), dp<int>(P.n_counters), dp<int>(P.n_flags));
   launch_nms_keep(s, dp<NmsBox>(P.n_sbox), dp<int>(P.n_picks), dp<int>(P.n_counters), n,
-                  dp<int>(P.n_flags), dp<int>(P.n_keep_orig));
+                  dp<int>(P.n_flags), dp<int>(P.n_keep_orig), c->d_hcount + kHostTie);
   launch_compact(s, dp<int>(P.n_keep_orig), (long long)N, dp<int>(P.d_blocksum), d_sel_total,
                  dp<int>(P.n_kept), c->d_hcount + kHostSurvivors);
-  int flags[2] = {0, 0};
-  HIPCHK(hipMemcpyAsync(flags, P.n_flags.p, sizeof(flags), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  const int tie = ((volatile int *)c->h_count)[kHostTie];
   const int nk_dev = ((volatile int *)c->h_count)[kHostSurvivors];
   int nk = 0;
-  if (flags[0] != 0) {
+  if (tie != 0) {
     // equal scores decided an outcome: replay the reference's exact quicksort order on the host
     P.nms_fallbacks++;
     c->nms_fallbacks++;
