@@ -271,7 +271,7 @@ struct Plan {
   long long cand_cap = 0;
   // device NMS
   DeviceBuffer n_box, n_keys, n_keys2, n_sbox, n_alive0, n_alive1, n_status, n_fsup, n_picks,
-      n_counters, n_mask, n_sort, n_flags, n_keep_orig, n_kept, n_kept_flat,
+      n_counters, n_mask, n_sort, n_flags, n_keep_orig, n_kept,
       n_total, n_vals, n_ord, n_rank;
   long long nms_fallbacks = 0;
   std::map<long long, hipGraphExec_t> graphs;  // front of the frame, per (precision, ws, thresh)

@@ -39,8 +39,8 @@ int compact_blocks(long long n);
 void launch_compact(hipStream_t s, const int *flags, long long n, int *blocksum, int *total,
                     int *out, int *host_total = nullptr);
 void launch_rows(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,
-                 const float *score, const int *cand, int ncand, const int *tmpIx,
-                 const int *tmpIy, int half, int GL, int padx, int pady, const int *rsize,
-                 float *rows);
+                 const float *score, const int *cand, const int *sel, int ncand,
+                 const int *tmpIx, const int *tmpIy, int half, int GL, int padx, int pady,
+                 const int *rsize, float *rows);
 
 }  // namespace geye

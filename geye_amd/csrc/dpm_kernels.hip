@@ -1217,14 +1217,16 @@ __global__ __launch_bounds__(256) void k_flag_scatter(const int *__restrict__ fl
 __global__ __launch_bounds__(256) void k_rows(const ScoreMap *__restrict__ maps, int nmaps,
                                               const ScorePart *__restrict__ parts,
                                               const float *__restrict__ score,
-                                              const int *__restrict__ cand, int ncand,
+                                              const int *__restrict__ cand,
+                                              const int *__restrict__ sel, int ncand,
                                               const int *__restrict__ tmpIx,
                                               const int *__restrict__ tmpIy, int half, int GL,
                                               int padx, int pady, const int *__restrict__ rsize,
                                               float *__restrict__ rows) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncand) return;
-  const long long e = cand[c];
+  // row c is candidate sel[c] (the NMS survivors, in their original order) or candidate c
+  const long long e = cand[sel ? sel[c] : c];
   int lo = 0, hi = nmaps - 1;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
@@ -1391,12 +1393,12 @@ void launch_compact(hipStream_t s, const int *flags, long long n, int *blocksum,
 }
 
 void launch_rows(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,
-                 const float *score, const int *cand, int ncand, const int *tmpIx,
-                 const int *tmpIy, int half, int GL, int padx, int pady, const int *rsize,
-                 float *rows) {
+                 const float *score, const int *cand, const int *sel, int ncand,
+                 const int *tmpIx, const int *tmpIy, int half, int GL, int padx, int pady,
+                 const int *rsize, float *rows) {
   if (ncand <= 0) return;
   hipLaunchKernelGGL(k_rows, dim3((ncand + 255) / 256), dim3(256), 0, s, maps, nmaps, parts,
-                     score, cand, ncand, tmpIx, tmpIy, half, GL, padx, pady, rsize, rows);
+                     score, cand, sel, ncand, tmpIx, tmpIy, half, GL, padx, pady, rsize, rows);
 }
 
 }  // namespace geye

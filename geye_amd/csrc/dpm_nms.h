@@ -59,6 +59,5 @@ void launch_nms_keep(hipStream_t s, const NmsBox *sbox, const int *picks, const 
                      int bound, const int *flags, int *keep_orig,
                      int *host_tie);
 void launch_fill(hipStream_t s, int *p, long long n, int v);
-void launch_take(hipStream_t s, const int *src, const int *idx, int n, int *dst);
 
 }  // namespace geye

@@ -666,13 +666,6 @@ __global__ __launch_bounds__(256) void k_fill(int *__restrict__ p, long long n, 
   if (i < n) p[i] = v;
 }
 
-__global__ __launch_bounds__(256) void k_take(const int *__restrict__ src,
-                                              const int *__restrict__ idx, int n,
-                                              int *__restrict__ dst) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) dst[k] = src[idx[k]];
-}
-
 // rocPRIM picks a merge sort below about 1 M items (about 20 dependent launches for a
 // 1080p candidate set); onesweep is four digit passes over the same keys, so the merge
 // path is switched off.  Onesweep tiles of 256 x 16 keys ranked with the match algorithm:
@@ -785,11 +778,6 @@ void launch_nms_keep(hipStream_t s, const NmsBox *sbox, const int *picks, const 
 void launch_fill(hipStream_t s, int *p, long long n, int v) {
   if (n > 0)
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
-}
-
-void launch_take(hipStream_t s, const int *src, const int *idx, int n, int *dst) {
-  if (n > 0)
-    hipLaunchKernelGGL(k_take, dim3((n + 255) / 256), dim3(256), 0, s, src, idx, n, dst);
 }
 
 }  // namespace geye

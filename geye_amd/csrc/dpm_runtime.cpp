@@ -516,7 +516,7 @@ void plan_release(Plan &P) {
                           &P.n_box, &P.n_keys, &P.n_keys2, &P.n_sbox, &P.n_alive0, &P.n_alive1,
                           &P.n_status, &P.n_fsup, &P.n_picks, &P.n_counters, &P.n_mask,
                           &P.n_sort, &P.n_flags, &P.n_keep_orig,
-                          &P.n_kept, &P.n_kept_flat, &P.n_total, &P.n_vals, &P.n_ord,
+                          &P.n_kept, &P.n_total, &P.n_vals, &P.n_ord,
                           &P.n_rank, &P.d_ascore_tab, &P.d_ascore, &P.d_grad, &P.d_grid,
                           &P.d_qtiles0, &P.d_qtiles1, &P.d_qcounter, &P.d_fused})
     release(*b);
@@ -814,8 +814,10 @@ int read_count(geye_ctx *c, Plan &P, int *ncand) {
   return GEYE_OK;
 }
 
-// half: the DT outputs are the windowed DT's half grids (the 16-bit forms)
-int run_rows(geye_ctx *c, Model &m, Plan &P, const int *d_list, int ncand, bool half,
+// Rows of the candidates of the last scoring pass: all of them, or (sel non-null) the
+// candidate positions sel[0..ncand).  half: the DT outputs are the windowed DT's half grids
+// (the 16-bit forms).
+int run_rows(geye_ctx *c, Model &m, Plan &P, const int *sel, int ncand, bool half,
              std::vector<float> &rows) {
   rows.assign((size_t)ncand * P.GL, 0.0f);
   if (ncand == 0) return GEYE_OK;
@@ -823,7 +825,7 @@ int run_rows(geye_ctx *c, Model &m, Plan &P, const int *d_list, int ncand, bool 
   {
     KTimer t(c, "rows");
     launch_rows(c->stream, dp<ScoreMap>(P.d_smaps), (int)P.smaps.size(), dp<ScorePart>(P.d_sparts),
-                dp<float>(P.d_score), d_list, ncand, dp<int>(P.d_tmpIx),
+                dp<float>(P.d_score), dp<int>(P.d_cand), sel, ncand, dp<int>(P.d_tmpIx),
                 dp<int>(P.d_tmpIy), half ? 1 : 0, P.GL, m.padx, m.pady, dp<int>(m.d_rsize),
                 dp<float>(P.d_rows));
   }
@@ -850,7 +852,6 @@ int nms_buffers(Plan &P, int n) {
   CHK(ensure(P.n_picks, 4 * N));
   CHK(ensure(P.n_keep_orig, 4 * N));
   CHK(ensure(P.n_kept, 4 * N));
-  CHK(ensure(P.n_kept_flat, 4 * N));
   CHK(ensure(P.n_counters, 16));
   CHK(ensure(P.n_flags, 16));
   CHK(ensure(P.n_total, 16));
@@ -957,7 +958,7 @@ int nms_core(geye_ctx *c, Plan &P, int n, float overlap, const NmsMaps *maps, co
 }
 
 // Device NMS over the ncand candidates of the last scoring pass; leaves the survivors'
-// flat score indices (original order) in P.n_kept_flat.
+// candidate positions (original order) in P.n_kept.
 int run_nms(geye_ctx *c, Model &m, Plan &P, int n, float overlap, int *nkept) {
   *nkept = 0;
   if (n <= 0) return GEYE_OK;
@@ -978,7 +979,6 @@ int run_nms(geye_ctx *c, Model &m, Plan &P, int n, float overlap, int *nkept) {
   }
   int nk = 0;
   CHK(nms_core(c, P, n, overlap, use_maps ? &mp : nullptr, dp<int>(P.d_cand), &nk));
-  launch_take(c->stream, dp<int>(P.d_cand), dp<int>(P.n_kept), nk, dp<int>(P.n_kept_flat));
   HIPCHK(hipGetLastError());
   *nkept = nk;
   return GEYE_OK;
@@ -1075,10 +1075,10 @@ int detect_common(geye_ctx *c, const geye_model *model, const unsigned char *bgr
   if (do_nms) {
     int nk = 0;
     CHK(run_nms(c, m, *P, ncand, overlap, &nk));
-    CHK(run_rows(c, m, *P, dp<int>(P->n_kept_flat), nk, precision != GEYE_FP32, rows));
+    CHK(run_rows(c, m, *P, dp<int>(P->n_kept), nk, precision != GEYE_FP32, rows));
     *num = nk;
   } else {
-    CHK(run_rows(c, m, *P, dp<int>(P->d_cand), ncand, precision != GEYE_FP32, rows));
+    CHK(run_rows(c, m, *P, nullptr, ncand, precision != GEYE_FP32, rows));
     *num = ncand;
   }
   *boxes = to_malloc(rows);
@@ -1559,7 +1559,7 @@ int geye_get_boxes_features(geye_ctx *c, const geye_model *model, int width, int
   CHK(run_scores(c, m, *P, thresh, GEYE_FP32, &ncand));
   std::vector<float> rows;
   c->last_plan = P;
-  CHK(run_rows(c, m, *P, dp<int>(P->d_cand), ncand, false, rows));
+  CHK(run_rows(c, m, *P, nullptr, ncand, false, rows));
   *num = ncand;
   *boxes = to_malloc(rows);
   *GL = P->GL;
