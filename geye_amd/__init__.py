@@ -116,6 +116,7 @@ _SIGS = [
     ("geye_synchronize", C.c_int, [C.c_void_p]),
     ("geye_profile_enable", C.c_int, [C.c_void_p, C.c_int]),
     ("geye_kernel_stats", C.c_int, [C.c_void_p, C.c_char_p, C.c_int, _dp, _llp, C.c_int, _ip]),
+    ("geye_nms_fallbacks", C.c_int, [C.c_void_p, _llp]),
 ]
 
 _lib = None
@@ -438,6 +439,12 @@ class Context:
                                        C.byref(n)))
         keys = names.value.decode().split("\n")[: n.value]
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(keys)}
+
+    def nms_fallbacks(self) -> int:
+        """NMS calls on this context that were replayed on the host (score ties)."""
+        n = C.c_longlong()
+        _check(lib().geye_nms_fallbacks(self.h, C.byref(n)))
+        return int(n.value)
 
     def synchronize(self):
         _check(lib().geye_synchronize(self.h))

@@ -26,11 +26,13 @@ void launch_hog_feat(hipStream_t s, const HogLevel *lv, int nlv, int blocks, con
 void launch_conv_f32(hipStream_t s, int nf, const ConvTile *tiles, int ntiles, int fh, int fw,
                      const float *feat, const float *filt, const long long *map_off,
                      const int2 *map_dims, float *out);
-void launch_dt(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, const int4 *nodes,
-               const int *depths, const float *src, float *dst, int *ptr);
-// the 16-bit forms' DT: windowed minimum, same outputs up to near-ties of the cost
-void launch_dt_window(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, const int4 *nodes,
-                      const int *depths, const float *src, float *dst, int *ptr);
+// The DT launchers return -1 (nothing launched) when the dynamic LDS the jobs need could
+// not be enabled for the kernel.
+int launch_dt(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, const int4 *nodes,
+              const int *depths, const float *src, float *dst, int *ptr);
+// the 16-bit forms' DT: windowed minimum, same outputs as launch_dt
+int launch_dt_window(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, const int4 *nodes,
+                     const int *depths, const float *src, float *dst, int *ptr);
 void launch_score(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,
                   const float *conv, const float *M, float thresh, int half, float *score,
                   int *flags, int blocks);

@@ -1352,24 +1352,26 @@ void launch_conv_f32(hipStream_t s, int nf, const ConvTile *tiles, int ntiles, i
   }
 }
 
-void launch_dt(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, const int4 *nodes,
-               const int *depths, const float *src, float *dst, int *ptr) {
+int launch_dt(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, const int4 *nodes,
+              const int *depths, const float *src, float *dst, int *ptr) {
   static const bool attr = hipFuncSetAttribute((const void *)k_dt,
                                                hipFuncAttributeMaxDynamicSharedMemorySize,
                                                160 * 1024) == hipSuccess;
-  if (!attr && lds > 64 * 1024) return;  // the launch would fail; the caller checks the error
+  if (!attr && lds > 64 * 1024) return -1;
   hipLaunchKernelGGL(k_dt, dim3(njobs), dim3(256), lds, s, jobs, nodes, depths, src, dst, ptr);
+  return 0;
 }
 
-void launch_dt_window(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, const int4 *nodes,
-                      const int *depths, const float *src, float *dst, int *ptr) {
-  // the kernel's own 32 bytes of static LDS count against the same 160 KB
+int launch_dt_window(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, const int4 *nodes,
+                     const int *depths, const float *src, float *dst, int *ptr) {
+  // the kernel's own static LDS counts against the same 160 KB
   static const bool attr = hipFuncSetAttribute((const void *)k_dt_window,
                                                hipFuncAttributeMaxDynamicSharedMemorySize,
                                                kDtWindowMaxLds) == hipSuccess;
-  if (!attr && lds > 64 * 1024) return;  // the launch would fail; the caller checks the error
+  if (!attr && lds > 64 * 1024) return -1;
   hipLaunchKernelGGL(k_dt_window, dim3(njobs), dim3(256), lds, s, jobs, nodes, depths, src, dst,
                      ptr);
+  return 0;
 }
 
 void launch_score(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,

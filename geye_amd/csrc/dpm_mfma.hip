@@ -439,18 +439,18 @@ static int launch_one(hipStream_t s, const MfmaTile *tiles, int ntiles, int fh, 
                       const void *feat, const void *filt, const long long *map_off,
                       const int2 *map_dims, float *out) {
   const size_t lds = mfma_lds_bytes(fh, fw, NT);
-  static bool attr_set = false;
-  static int ncu = 0;
-  if (!attr_set) {
+  // once per process (function-local statics are initialised once, also when serving
+  // workers launch concurrently): the LDS attribute and the CU count, 0 on failure
+  static const int ncu = [] {
+    int dev = 0, n = 0;
     if (hipFuncSetAttribute((const void *)k_conv_mfma<KIND, NT>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
-      return -1;
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-      return -1;
-    attr_set = true;
-  }
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return 0;
+    return n;
+  }();
+  if (ncu <= 0) return -1;
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_conv_mfma<KIND, NT>,
                                                    256, lds) != hipSuccess || per_cu < 1)

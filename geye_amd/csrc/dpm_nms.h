@@ -42,8 +42,9 @@ void launch_nms_gather(hipStream_t s, const int *ord, const NmsBox *box, int n, 
 // blocksum.  nalive_host is a host upper bound of *nalive: the kernels read the true count
 // and do nothing past it.
 // maps selects the score-map scatter for the suppression of the later boxes; without it
-// every later box is tested against every pick.
-void launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const int *nalive,
+// every later box is tested against every pick.  Returns -1 (nothing launched) when the
+// greedy kernel's LDS attribute could not be set.
+int launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const int *nalive,
                       int nalive_host, float overlap, unsigned long long *mask, int *status,
                       int *sup, int *picks, int *counters, int *blocksum, const NmsMaps *maps);
 // The round's surviving later boxes, in order, to alive_out, their count to *nalive_out;

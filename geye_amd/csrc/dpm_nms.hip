@@ -29,8 +29,6 @@
 
 namespace geye {
 
-long long *g_nms_phase_clock = nullptr;  // optional phase timestamps of the greedy kernel
-
 __device__ inline unsigned int order_desc(float f) {
   unsigned int u = __float_as_uint(f);
   u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);  // ascending float order
@@ -221,11 +219,9 @@ __global__ __launch_bounds__(256) void k_nms_wingreedy(const int *__restrict__ a
                                                         int *__restrict__ sup,
                                                         int *__restrict__ picks,
                                                         int *__restrict__ counters,
-                                                        int window_sup,
-                                                        long long *__restrict__ phase_clock) {
+                                                        int window_sup) {
   constexpr int NW = kNmsWindow / 64;
   extern __shared__ unsigned long long smask[];  // [NW][kNmsWindow]
-  if (phase_clock && threadIdx.x == 0) phase_clock[0] = wall_clock64();
   int *salive = (int *)(smask + NW * kNmsWindow);
   __shared__ short s_pick[kNmsWindow];              // window indices of the picks, in order
   __shared__ unsigned long long s_picked[NW];       // picked rows of each 64-row block
@@ -264,7 +260,6 @@ __global__ __launch_bounds__(256) void k_nms_wingreedy(const int *__restrict__ a
     }
   }
   __syncthreads();
-  if (phase_clock && threadIdx.x == 0) phase_clock[1] = wall_clock64();
   const int round_begin = counters[0];
   if (threadIdx.x < 64) {
     const int lane = threadIdx.x;
@@ -328,7 +323,6 @@ __global__ __launch_bounds__(256) void k_nms_wingreedy(const int *__restrict__ a
     }
   }
   __syncthreads();
-  if (phase_clock && threadIdx.x == 0) phase_clock[2] = wall_clock64();
   // first suppressors of the window boxes removed in this round, when no suppression pass
   // follows: every window box is either a pick or covered by an earlier pick's row
   if (window_sup) {
@@ -350,15 +344,6 @@ __global__ __launch_bounds__(256) void k_nms_wingreedy(const int *__restrict__ a
           if (f < 0 && ((v[u] >> bit) & 1ull)) f = pp[u];
       }
       if (f >= 0) sup[salive[j]] = salive[f];
-    }
-  }
-  if (phase_clock) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      phase_clock[3] = wall_clock64();
-      for (int k = 0; k < 3; k++) phase_clock[4 + k] += phase_clock[k + 1] - phase_clock[k];
-      phase_clock[7] += 1;
-      phase_clock[8] += W;
     }
   }
 }
@@ -716,26 +701,26 @@ void launch_nms_gather(hipStream_t s, const int *ord, const NmsBox *box, int n, 
                      alive, status, sup, cand, rank_of_flat, counters, nalive0, flags, keep);
 }
 
-void launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const int *nalive,
-                      int nalive_host, float overlap, unsigned long long *mask, int *status,
+int launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const int *nalive,
+                     int nalive_host, float overlap, unsigned long long *mask, int *status,
                       int *sup, int *picks, int *counters, int *blocksum,
                       const NmsMaps *maps) {
   const int W = nalive_host < kNmsWindow ? nalive_host : kNmsWindow;
+  const size_t glds = sizeof(unsigned long long) * kNmsWindow * (kNmsWindow / 64) + 4 * kNmsWindow;
+  // set once per process (a function-local static is initialised once, also when serving
+  // workers launch concurrently)
+  static const bool attr = hipFuncSetAttribute((const void *)k_nms_wingreedy,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)glds) == hipSuccess;
+  if (!attr) return -1;
   hipLaunchKernelGGL(k_nms_winmask, dim3((W + kWinmaskRows - 1) / kWinmaskRows), dim3(256), 0, s,
                      sbox, alive, nalive, overlap, mask);
-  static bool attr = false;
-  const size_t glds = sizeof(unsigned long long) * kNmsWindow * (kNmsWindow / 64) + 4 * kNmsWindow;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void *)k_nms_wingreedy,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)glds);
-    attr = true;
-  }
   // the suppression pass below (launched when the bound leaves boxes after the window) also
   // gives the window's removed boxes their first suppressor
   const int rest = nalive_host - W;
   hipLaunchKernelGGL(k_nms_wingreedy, dim3(1), dim3(256), glds, s, alive, nalive, mask, status,
-                     sup, picks, counters, rest <= 0 ? 1 : 0, g_nms_phase_clock);
-  if (rest <= 0) return;
+                     sup, picks, counters, rest <= 0 ? 1 : 0);
+  if (rest <= 0) return 0;
   const int nb = (nalive_host + kNmsCompactBlock - 1) / kNmsCompactBlock;
   if (maps) {
     // s_off 8 B, seven 4-byte tables and s_beg (nmaps + 1 entries) per map
@@ -752,6 +737,7 @@ void launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const
   hipLaunchKernelGGL(k_nms_post, dim3(nb), dim3(256), 0, s, alive, nalive, sup, sbox,
                      retire ? maps->cand : nullptr, retire ? maps->rank_of_flat : nullptr,
                      blocksum);
+  return 0;
 }
 
 void launch_nms_compact(hipStream_t s, const int *alive, const int *nalive, const int *sup,

@@ -777,15 +777,17 @@ int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int 
   const auto &jobs2 = exact_dt ? P.dt_pass2 : P.dt_pass2w;
   if (!jobs1.empty()) {
     KTimer t(c, "dt_cols");
-    dt_launch(s, dp<DtJob>(exact_dt ? P.d_dt1 : P.d_dt1w), (int)jobs1.size(),
-              exact_dt ? P.dt_lds1 : P.dt_lds1w, dp<int4>(P.d_nodes), dp<int>(P.d_depths),
-              dp<float>(P.d_conv), dp<float>(P.d_tmpM), dp<int>(P.d_tmpIy));
+    if (dt_launch(s, dp<DtJob>(exact_dt ? P.d_dt1 : P.d_dt1w), (int)jobs1.size(),
+                  exact_dt ? P.dt_lds1 : P.dt_lds1w, dp<int4>(P.d_nodes), dp<int>(P.d_depths),
+                  dp<float>(P.d_conv), dp<float>(P.d_tmpM), dp<int>(P.d_tmpIy)) != 0)
+      return fail(GEYE_ERR_HIP, "DT kernel: dynamic LDS attribute not set");
   }
   if (!jobs2.empty()) {
     KTimer t(c, "dt_rows");
-    dt_launch(s, dp<DtJob>(exact_dt ? P.d_dt2 : P.d_dt2w), (int)jobs2.size(),
-              exact_dt ? P.dt_lds2 : P.dt_lds2w, dp<int4>(P.d_nodes), dp<int>(P.d_depths),
-              dp<float>(P.d_tmpM), dp<float>(P.d_M), dp<int>(P.d_tmpIx));
+    if (dt_launch(s, dp<DtJob>(exact_dt ? P.d_dt2 : P.d_dt2w), (int)jobs2.size(),
+                  exact_dt ? P.dt_lds2 : P.dt_lds2w, dp<int4>(P.d_nodes), dp<int>(P.d_depths),
+                  dp<float>(P.d_tmpM), dp<float>(P.d_M), dp<int>(P.d_tmpIx)) != 0)
+      return fail(GEYE_ERR_HIP, "DT kernel: dynamic LDS attribute not set");
   }
   if (P.score_total > 0) {
     {
@@ -893,13 +895,16 @@ int nms_core(geye_ctx *c, Plan &P, int n, float overlap, const NmsMaps *maps, co
   // The kernels read the true counts from device memory and do nothing past them.
   auto round = [&](int bound) {
     KTimer t(c, "nms_round");
-    launch_nms_round(s, dp<NmsBox>(P.n_sbox), alive, na_buf[cur], bound, overlap,
-                     dp<unsigned long long>(P.n_mask), dp<int>(P.n_status), dp<int>(P.n_fsup),
-                     dp<int>(P.n_picks), dp<int>(P.n_counters), blocksum, maps);
+    return launch_nms_round(s, dp<NmsBox>(P.n_sbox), alive, na_buf[cur], bound, overlap,
+                            dp<unsigned long long>(P.n_mask), dp<int>(P.n_status),
+                            dp<int>(P.n_fsup), dp<int>(P.n_picks), dp<int>(P.n_counters),
+                            blocksum, maps) == 0
+               ? GEYE_OK
+               : fail(GEYE_ERR_HIP, "NMS greedy kernel: dynamic LDS attribute not set");
   };
   int na = n;  // exact alive count of the round last enqueued
   int slot = 0;
-  round(na);
+  CHK(round(na));
   for (;;) {
     const int rest = na - std::min(na, kNmsWindow);  // boxes after that round's window
     if (rest <= 0) break;                           // it held every alive box
@@ -908,7 +913,7 @@ int nms_core(geye_ctx *c, Plan &P, int n, float overlap, const NmsMaps *maps, co
     HIPCHK(hipEventRecord(c->ev_count[slot], s));
     std::swap(alive, alive_next);
     cur ^= 1;
-    round(rest);                                    // next round, bounded by rest
+    CHK(round(rest));                               // next round, bounded by rest
     HIPCHK(hipEventSynchronize(c->ev_count[slot]));
     na = ((volatile int *)c->h_count)[slot];        // its exact alive count
     slot ^= 1;
@@ -1884,10 +1889,11 @@ static int dt_standalone(geye_ctx *c, const float *vals, float ax, float bx, flo
   CHK(upload(dn, nodes, c->stream));
   CHK(upload(dd, depths, c->stream));
   auto *dt_launch = window ? launch_dt_window : launch_dt;
-  dt_launch(c->stream, dp<DtJob>(dj1), (int)p1.size(), lds1, dp<int4>(dn), dp<int>(dd),
-            dp<float>(dv), dp<float>(dtm), dp<int>(diy));
-  dt_launch(c->stream, dp<DtJob>(dj2), (int)p2.size(), lds2, dp<int4>(dn), dp<int>(dd),
-            dp<float>(dtm), dp<float>(dm), dp<int>(dix));
+  if (dt_launch(c->stream, dp<DtJob>(dj1), (int)p1.size(), lds1, dp<int4>(dn), dp<int>(dd),
+                dp<float>(dv), dp<float>(dtm), dp<int>(diy)) != 0 ||
+      dt_launch(c->stream, dp<DtJob>(dj2), (int)p2.size(), lds2, dp<int4>(dn), dp<int>(dd),
+                dp<float>(dtm), dp<float>(dm), dp<int>(dix)) != 0)
+    return fail(GEYE_ERR_HIP, "DT kernel: dynamic LDS attribute not set");
   HIPCHK(hipGetLastError());
   std::vector<int> tIx(SQ), tIy(SQ);
   HIPCHK(hipMemcpyAsync(M, dm.p, sizeof(float) * SQ, hipMemcpyDeviceToHost, c->stream));
@@ -1993,6 +1999,13 @@ int geye_profile_enable(geye_ctx *c, int on) {
   flush_stats(c);
   c->profile = on != 0;
   c->stats.clear();
+  return GEYE_OK;
+}
+
+int geye_nms_fallbacks(geye_ctx *c, long long *count) {
+  if (!c || !count) return fail(GEYE_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  *count = c->nms_fallbacks;
   return GEYE_OK;
 }
 

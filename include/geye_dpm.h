@@ -239,6 +239,11 @@ int geye_mfma_tuning(geye_ctx *ctx, int precision, int *choice, char *text, int 
 int geye_mfma_schedule(geye_ctx *ctx, int precision, int choice);
 int geye_kernel_stats(geye_ctx *ctx, char *names, int names_cap, double *ms, long long *count,
                       int cap, int *n);
+/* NMS runs on the device; where equal scores could change its outcome (the reference's
+ * quicksort, CT/nms.cpp:70-84, orders ties in its own way) that call is replayed on the host
+ * with the reference's exact order.  Reports how many NMS calls on ctx took that host path
+ * since the context was created. */
+int geye_nms_fallbacks(geye_ctx *ctx, long long *count);
 
 #ifdef __cplusplus
 }

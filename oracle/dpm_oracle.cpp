@@ -1001,11 +1001,53 @@ long long ora_detect(const ora_model_desc *m, const unsigned char *bgr, int widt
   return n;
 }
 
+long long ora_rows_one_map(const ora_model_desc *m, int jj, float scale, const float *score,
+                           int R0, int R1, const int *Ix, const int *Iy, const int *pdims,
+                           float thresh, float *out, long long cap, int *GL) {
+  // a one-level plan: the level `interval` is the only valid one, with L = 0 and
+  // sbin / scales[level] == scale; component jj's maps are filled, the others stay empty
+  const int NoC = m->numcomponent, MP = m->maxpart, NoP = m->NoP, np = m->numpart[jj];
+  ScoreMaps S;
+  S.maxpart = MP;
+  S.plan.LEN = m->interval + 1;
+  S.plan.scales.assign(S.plan.LEN, 1.0f);
+  S.plan.scales[m->interval] = (float)m->sbin / scale;
+  S.valid.assign(S.plan.LEN, 0);
+  S.valid[m->interval] = 1;
+  S.rdims.assign((size_t)NoC * 2, 0);
+  S.rdims[jj * 2] = R0;
+  S.rdims[jj * 2 + 1] = R1;
+  S.pdims.assign((size_t)NoP * 2, 0);
+  S.Ix.assign((size_t)NoC * MP, {});
+  S.Iy.assign((size_t)NoC * MP, {});
+  long long off = 0;
+  for (int kk = 0; kk < np; kk++) {
+    const int pid = m->pidx[jj * MP + kk];
+    const int P0 = pdims[2 * kk], P1 = pdims[2 * kk + 1];
+    S.pdims[pid * 2] = P0;
+    S.pdims[pid * 2 + 1] = P1;
+    S.Ix[(size_t)jj * MP + kk].assign(Ix + off, Ix + off + (long long)P0 * P1);
+    S.Iy[(size_t)jj * MP + kk].assign(Iy + off, Iy + off + (long long)P0 * P1);
+    off += (long long)P0 * P1;
+  }
+  std::vector<std::vector<float>> sc((size_t)NoC);
+  sc[jj].assign(score, score + (long long)R0 * R1);
+  // only component jj has candidates; boxes_from_scores checks every component's GL
+  ora_model_desc one = *m;
+  std::vector<int> npart(m->numpart, m->numpart + NoC);
+  for (int c = 0; c < NoC; c++) npart[c] = np;
+  one.numpart = npart.data();
+  std::vector<float> rows = boxes_from_scores<float>(&one, S, sc, thresh, GL);
+  const long long n = (long long)rows.size() / *GL;
+  if (out) std::memcpy(out, rows.data(), sizeof(float) * (size_t)(std::min(n, cap) * *GL));
+  return n;
+}
+
 long long ora_detect_maps(const ora_model_desc *m, const unsigned char *bgr, int width,
-                          int height, int widthStep, int precision, void **maps, int **map_dims,
-                          int *nmaps) {
+                          int height, int widthStep, int precision, int threads, void **maps,
+                          int **map_dims, int *nmaps) {
   int ms = compute_max_scale(m->rsize, m->numcomponent, m->sbin, m->interval, width, height);
-  Pyramid P = build_pyramid(bgr, width, height, widthStep, m->sbin, m->interval, ms, 1);
+  Pyramid P = build_pyramid(bgr, width, height, widthStep, m->sbin, m->interval, ms, threads);
   ScoreMaps S;
   const int NoC = m->numcomponent, NL = P.plan.LEN - m->interval;
   long long total = 0;
@@ -1036,11 +1078,11 @@ long long ora_detect_maps(const ora_model_desc *m, const unsigned char *bgr, int
   };
   if (precision == 1) {
     std::vector<std::vector<double>> sc;
-    compute_scores<double>(m, P, 1, S, sc);
+    compute_scores<double>(m, P, threads, S, sc);
     emit(sc, double(0));
   } else {
     std::vector<std::vector<float>> sc;
-    compute_scores<float>(m, P, 1, S, sc);
+    compute_scores<float>(m, P, threads, S, sc);
     emit(sc, float(0));
   }
   int *d = (int *)std::malloc(sizeof(int) * std::max<size_t>(dims.size(), 1));

@@ -109,10 +109,19 @@ long long ora_detect(const ora_model_desc *m, const unsigned char *bgr, int widt
                      int widthStep, float thresh, int precision, int threads, float **boxes,
                      int *GL);
 
+/* Candidate rows of one component's score map at one level (CT/get_boxes.cpp:185-258,
+ * 1497-1546): get_gmpc's column-major scan over the R0 x R1 map, rootbox and partbox per
+ * candidate, with the part argmin maps Ix/Iy (numpart[jj] maps of P0 x P1 each, column-major,
+ * concatenated) as the DT left them.  scale = sbin / scales[level].  Exposed for hand-built
+ * checks of the box geometry.  Writes at most cap rows of GL floats; returns the row count. */
+long long ora_rows_one_map(const ora_model_desc *m, int jj, float scale, const float *score,
+                           int R0, int R1, const int *Ix, const int *Iy, const int *pdims,
+                           float thresh, float *out, long long cap, int *GL);
+
 /* Raw score maps of the last ora_detect_maps call: root+bias-parts per (level, component)
  * concatenated in level-major order, in double for precision 1 and float otherwise. */
 long long ora_detect_maps(const ora_model_desc *m, const unsigned char *bgr, int width,
-                          int height, int widthStep, int precision, void **maps,
+                          int height, int widthStep, int precision, int threads, void **maps,
                           int **map_dims, int *nmaps);
 
 /* The HOG pyramid only: concatenated features [level][31][F1][F0]. */

@@ -79,9 +79,13 @@ def lib():
                                  C.c_float, C.c_int, C.c_int, C.POINTER(_fp), _ip]
         L.ora_detect.restype = C.c_longlong
         L.ora_detect_maps.argtypes = [C.POINTER(ModelDesc), C.c_void_p, C.c_int, C.c_int,
-                                      C.c_int, C.c_int, C.POINTER(C.c_void_p),
+                                      C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p),
                                       C.POINTER(_ip), _ip]
         L.ora_detect_maps.restype = C.c_longlong
+        L.ora_rows_one_map.argtypes = [C.POINTER(ModelDesc), C.c_int, C.c_float, _fp, C.c_int,
+                                       C.c_int, _ip, _ip, _ip, C.c_float, _fp, C.c_longlong,
+                                       _ip]
+        L.ora_rows_one_map.restype = C.c_longlong
         L.ora_pyramid.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                   C.c_int, C.c_int, C.POINTER(_fp), _ip, _ip, _fp]
         L.ora_pyramid.restype = C.c_longlong
@@ -272,6 +276,22 @@ def dt_passes(vals: np.ndarray, ax, bx, ay, by):
     return tmpM, tIy, M, tIx
 
 
+def add_part(score: np.ndarray, M: np.ndarray, ax: int, ay: int, use_ref: bool = False):
+    """add_part_calculation (CT/dt.cpp:38-55) on a copy of score [R1][R0] with the part's DT
+    map M [P1][P0] (both column-major); returns the updated copy."""
+    R1, R0 = score.shape
+    P1, P0 = M.shape
+    if ay - 1 + 2 * (R0 - 1) >= P0 or ax - 1 + 2 * (R1 - 1) >= P1 or ax < 1 or ay < 1:
+        raise ValueError("anchor reads outside the part map")
+    out = np.ascontiguousarray(score, dtype=np.float32).copy()
+    Mc = np.ascontiguousarray(M, dtype=np.float32)
+    rs = np.array([R0, R1], np.int32)
+    ps = np.array([P0, P1], np.int32)
+    fn = ref_lib().ref_add_part if use_ref else lib().ora_add_part_f32
+    fn(_p(out, _fp), _p(Mc, _fp), _p(rs, _ip), _p(ps, _ip), ax, ay)
+    return out
+
+
 def nms(boxes: np.ndarray, overlap: float) -> np.ndarray:
     boxes = np.ascontiguousarray(boxes, dtype=np.float32)
     n, GL = boxes.shape
@@ -304,7 +324,28 @@ def detect(model, frame: np.ndarray, thresh: float = -0.5, precision: int = 0,
     return out.reshape(n, GL.value)
 
 
-def score_maps(model, frame: np.ndarray, precision: int = 0):
+def rows_one_map(model, jj: int, scale: float, score: np.ndarray, Ix: list, Iy: list,
+                 thresh: float) -> np.ndarray:
+    """Candidate rows of component jj's score map score[R1][R0] (column-major) at one level
+    of the given scale, with the part argmin maps Ix[k], Iy[k] ([P1][P0] each)."""
+    keep = _DescKeeper(model)
+    score = np.ascontiguousarray(score, dtype=np.float32)
+    R1, R0 = score.shape
+    pd = np.array([[a.shape[1], a.shape[0]] for a in Ix], np.int32).ravel()
+    ix = np.ascontiguousarray(np.concatenate([np.asarray(a, np.int32).ravel() for a in Ix]))
+    iy = np.ascontiguousarray(np.concatenate([np.asarray(a, np.int32).ravel() for a in Iy]))
+    GL = C.c_int()
+    n = lib().ora_rows_one_map(C.byref(keep.desc), jj, C.c_float(scale), _p(score, _fp), R0, R1,
+                               _p(ix, _ip), _p(iy, _ip), _p(pd, _ip), C.c_float(thresh), None, 0,
+                               C.byref(GL))
+    out = np.zeros((max(n, 1), GL.value), np.float32)
+    lib().ora_rows_one_map(C.byref(keep.desc), jj, C.c_float(scale), _p(score, _fp), R0, R1,
+                           _p(ix, _ip), _p(iy, _ip), _p(pd, _ip), C.c_float(thresh),
+                           _p(out, _fp), n, C.byref(GL))
+    return out[:n]
+
+
+def score_maps(model, frame: np.ndarray, precision: int = 0, threads: int = 1):
     """List of (level, comp, map[R1][R0]) root+bias-parts score maps."""
     keep = _DescKeeper(model)
     H, W, _ = frame.shape
@@ -313,7 +354,7 @@ def score_maps(model, frame: np.ndarray, precision: int = 0):
     dp = _ip()
     nm = C.c_int()
     total = lib().ora_detect_maps(C.byref(keep.desc), frame.ctypes.data, W, H, 3 * W, precision,
-                                  C.byref(mp), C.byref(dp), C.byref(nm))
+                                  threads, C.byref(mp), C.byref(dp), C.byref(nm))
     dt_ = np.float64 if precision == 1 else np.float32
     buf = np.ctypeslib.as_array(C.cast(mp, C.POINTER(C.c_double if precision == 1 else C.c_float)),
                                 shape=(max(total, 1),))[:total].astype(dt_)

@@ -6,6 +6,8 @@
   compiled from the reference tree into oracle/_ref.
 * dt_ref.npz -- input/output vectors of the reference's own dt() (CT/dt.cpp), same build,
   on random maps, plateaus, exact ties and large-coefficient maps.
+* add_part_ref.npz -- input/output vectors of the reference's own add_part_calculation()
+  (CT/dt.cpp:38-55), same build, at several anchors and map sizes.
 """
 import os
 import sys
@@ -72,6 +74,24 @@ def dt_vectors():
     np.savez_compressed(os.path.join(HERE, "dt_ref.npz"), **out)
 
 
+def add_part_vectors():
+    rng = np.random.default_rng(77)
+    # (R0, R1, P0, P1, ax, ay): anchors at the map's start, inside, and at the last cell the
+    # root map can reach (ay - 1 + 2 (R0 - 1) = P0 - 1)
+    cases = [(5, 4, 12, 10, 1, 1), (11, 17, 30, 40, 3, 2), (6, 9, 16, 23, 6, 5),
+             (1, 1, 3, 3, 2, 3), (13, 7, 30, 16, 3, 5)]
+    out = {}
+    for k, (R0, R1, P0, P1, ax, ay) in enumerate(cases):
+        S = rng.standard_normal((R1, R0)).astype(np.float32)
+        M = rng.standard_normal((P1, P0)).astype(np.float32)
+        out[f"score{k}"] = S
+        out[f"M{k}"] = M
+        out[f"anchor{k}"] = np.array([ax, ay], np.int32)
+        out[f"out{k}"] = O.add_part(S, M, ax, ay, use_ref=True)
+    out["n"] = np.array(len(cases))
+    np.savez_compressed(os.path.join(HERE, "add_part_ref.npz"), **out)
+
+
 if __name__ == "__main__":
     O.build()
     if O.ref_lib() is None:
@@ -79,4 +99,5 @@ if __name__ == "__main__":
     car_model()
     resize_vectors()
     dt_vectors()
+    add_part_vectors()
     print("fixtures written")

@@ -19,6 +19,20 @@ def say(*a):
     print(*a, flush=True)
 
 
+@pytest.fixture(autouse=True)
+def nms_path(request, ctx):
+    """Every test here runs its NMS on the device: the host replay that score ties trigger
+    (geye_nms_fallbacks) must not have run, except in the test built to create ties, whose
+    one NMS call must have taken it."""
+    before = ctx.nms_fallbacks()
+    yield
+    ran = ctx.nms_fallbacks() - before
+    if request.node.get_closest_marker("nms_ties"):
+        assert ran == 1, f"host NMS replays: {ran}"
+    else:
+        assert ran == 0, f"host NMS replays: {ran}"
+
+
 def bits_equal(a, b):
     a = np.ascontiguousarray(a)
     b = np.ascontiguousarray(b)
@@ -260,8 +274,12 @@ def _random_boxes(n, seed, ties=False, GL=11):
 
 
 @pytest.mark.parametrize("n,ties", [(1, False), (2, False), (50, False), (3000, False),
-                                    (3000, True), (20000, False)])
+                                    pytest.param(3000, True, marks=pytest.mark.nms_ties),
+                                    (20000, False)])
 def test_nms_matches_oracle(ctx, n, ties):
+    """Device NMS against the oracle.  The ties case (scores rounded to quarters) is the test
+    of the host replay: equal scores decide outcomes there, so the call must take the host
+    path with the reference's quicksort order (checked by the nms_path fixture)."""
     say("nms", n, ties)
     rows = _random_boxes(n, seed=n + ties, ties=ties)
     g = ctx.nms(rows, 0.5)

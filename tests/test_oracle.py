@@ -22,6 +22,13 @@ def bits(a):
     return np.ascontiguousarray(a, dtype=np.float32).view(np.int32)
 
 
+def need_ref():
+    """Skips the calling test where oracle/_ref (built from the reference tree) is absent.
+    Checked inside the test body, so collecting the tests never loads that library."""
+    if O.ref_lib() is None:
+        pytest.skip("oracle/_ref (reference tree) not built")
+
+
 # ---------------------------------------------------------------------------------------
 # resize (CT/resize.cc)
 # ---------------------------------------------------------------------------------------
@@ -31,10 +38,10 @@ def test_resize_golden_vectors():
         assert np.array_equal(bits(O.resize(z["img"], float(s))), bits(z[f"out{k}"])), float(s)
 
 
-@pytest.mark.skipif(O.ref_lib() is None, reason="oracle/_ref (reference tree) not built")
 @pytest.mark.parametrize("W,H,s", [(97, 131, 0.707107), (160, 120, 0.93303), (33, 29, 0.5),
                                    (211, 157, 0.535887), (64, 48, 0.870551)])
 def test_resize_matches_reference_build(W, H, s):
+    need_ref()
     img = O.ipl_to_float(synth.make_frame(W, H, seed=W + H))
     assert np.array_equal(bits(O.resize(img, s)), bits(O.resize(img, s, use_ref=True)))
 
@@ -203,17 +210,48 @@ def test_dt_golden_vectors():
         assert np.array_equal(Ix, z[f"Ix{k}"]) and np.array_equal(Iy, z[f"Iy{k}"]), k
 
 
-@pytest.mark.skipif(O.ref_lib() is None, reason="oracle/_ref (reference tree) not built")
 @pytest.mark.parametrize("d0,d1,defs,seed", [(31, 27, (0.01, 0.0, 0.01, 0.0), 1),
                                             (70, 9, (0.03, 0.02, 0.05, -0.01), 2),
                                             (1, 40, (0.2, 0.1, 0.2, 0.1), 3),
                                             (50, 50, (0.0, 0.0, 0.0, 0.0), 4)])
 def test_dt_matches_reference_build(d0, d1, defs, seed):
+    need_ref()
     rng = np.random.default_rng(seed)
     v = np.round(rng.standard_normal((d1, d0)) * 3).astype(np.float32)  # many exact ties
     M, Ix, Iy = O.dt(v, *defs)
     Mr, Ixr, Iyr = O.dt(v, *defs, use_ref=True)
     assert np.array_equal(bits(M), bits(Mr)) and np.array_equal(Ix, Ixr) and np.array_equal(Iy, Iyr)
+
+
+def test_add_part_golden_vectors():
+    z = np.load(os.path.join(GOLDEN, "add_part_ref.npz"), allow_pickle=False)
+    for k in range(int(z["n"])):
+        ax, ay = (int(v) for v in z[f"anchor{k}"])
+        got = O.add_part(z[f"score{k}"], z[f"M{k}"], ax, ay)
+        assert np.array_equal(bits(got), bits(z[f"out{k}"])), k
+
+
+@pytest.mark.parametrize("R0,R1,P0,P1,ax,ay,seed", [(7, 5, 20, 14, 2, 3, 1),
+                                                   (1, 9, 2, 20, 2, 1, 2),
+                                                   (12, 12, 26, 27, 3, 3, 3)])
+def test_add_part_matches_reference_build(R0, R1, P0, P1, ax, ay, seed):
+    need_ref()
+    rng = np.random.default_rng(seed)
+    S = rng.standard_normal((R1, R0)).astype(np.float32)
+    M = rng.standard_normal((P1, P0)).astype(np.float32)
+    assert np.array_equal(bits(O.add_part(S, M, ax, ay)),
+                          bits(O.add_part(S, M, ax, ay, use_ref=True)))
+
+
+def test_add_part_hand_built():
+    """Root map 2 x 3 (R0 x R1), part map 6 x 8 with M[i] = i: root cell (y, x) loses
+    M[(ay - 1 + 2y) + (ax - 1 + 2x) * 6]; values worked out by hand for ax = 2, ay = 1."""
+    S = np.zeros((3, 2), np.float32)          # [R1][R0]
+    M = np.arange(48, dtype=np.float32).reshape(8, 6)
+    got = O.add_part(S, M, 2, 1)
+    # (y, x) = (0, 0) -> M[0 + 1*6] = 6; (1, 0) -> M[2 + 6] = 8; (0, 1) -> M[0 + 3*6] = 18;
+    # (1, 1) -> 20; (0, 2) -> M[0 + 5*6] = 30; (1, 2) -> 32
+    assert got.tolist() == [[-6.0, -8.0], [-18.0, -20.0], [-30.0, -32.0]]
 
 
 def test_dt_matches_brute_force_minimum():
@@ -353,6 +391,84 @@ def test_detect_tiny_model_candidate_invariants():
     maps64 = O.score_maps(m, frame, 1)
     for (_, _, a), (_, _, b) in zip(maps32, maps64):
         np.testing.assert_allclose(a, b, atol=2e-4)
+
+
+def _box_model(padx=1, pady=2):
+    """One component, root 6 x 11, two 6 x 6 parts with anchors (x, y) = (2, 1) and (0, 3),
+    i.e. (ax, ay) = (3, 2) and (1, 4) after the loader's + 1 (CT/get_boxes.cpp:1262-1263)."""
+    m = synth.tiny_model(roots=((6, 11),), nparts=2, psize=6)
+    m.anchor = np.array([[2, 1], [0, 3]], np.int32)
+    m.padx, m.pady = padx, pady
+    return m
+
+
+def test_box_geometry_hand_built():
+    """get_gmpc + rootbox + partbox (CT/get_boxes.cpp:185-258) on a hand-built 5 x 4 score
+    map with two cells above the threshold and hand-filled argmin maps.  Every expected number
+    below was worked out by hand from those formulas (scale 8, padx 1, pady 2, rsize (6, 11),
+    psize (6, 6)):
+      candidates in column-major scan order: A = (y 4, x 1) before B = (y 2, x 3);
+      root A: Y1 (4-2+1)*8 = 24, X1 (1-1+1)*8 = 8, Y2 24+48-1 = 71, X2 8+88-1 = 95;
+      root B: 8, 24, 55, 111;
+      the part probe is (2(y-1) + ay, 2(x-1) + ax), one cell before add_part's
+      (2y + ay - 1, 2x + ax - 1) on each axis; P0 = 14:
+        part 0 (ax 3, ay 2): A -> 8 + 3*14 = 50, B -> 4 + 7*14 = 102
+        part 1 (ax 1, ay 4): A -> 10 + 1*14 = 24, B -> 6 + 5*14 = 76
+      argmins placed there (Ix, Iy): 0A (5, 9), 0B (8, 3), 1A (0, 12), 1B (6, 7); every other
+      cell holds (11, 11), add_part's cells included, so reading one of them shows;
+      part box with px = Ix + 1, py = Iy + 1: Y1 ((py-2)/2 + 1 - 2)*8, X1 ((px-2)/2 + 1 - 1)*8,
+      Y2 = Y1 + 24 - 1, X2 = X1 + 24 - 1:
+        0A (24, 16, 47, 39); 1A (36, -4, 59, 19); 0B (0, 28, 23, 51); 1B (16, 20, 39, 43)."""
+    m = _box_model()
+    score = np.full((4, 5), -1.0, np.float32)      # [R1][R0]
+    score[1, 4] = 0.5                               # (y 4, x 1)
+    score[3, 2] = 0.75                              # (y 2, x 3)
+    Ix = [np.full((12, 14), 11, np.int32) for _ in range(2)]   # [P1][P0]
+    Iy = [np.full((12, 14), 11, np.int32) for _ in range(2)]
+    for k, flat, ix, iy in [(0, 50, 5, 9), (0, 102, 8, 3), (1, 24, 0, 12), (1, 76, 6, 7)]:
+        Ix[k].ravel()[flat] = ix
+        Iy[k].ravel()[flat] = iy
+    rows = O.rows_one_map(m, 0, 8.0, score, Ix, Iy, thresh=0.0)
+    assert rows.tolist() == [
+        [24, 8, 71, 95, 24, 16, 47, 39, 36, -4, 59, 19, 0, 0.5, 8],
+        [8, 24, 55, 111, 0, 28, 23, 51, 16, 20, 39, 43, 0, 0.75, 8]]
+
+
+def test_box_geometry_threshold_is_strict_and_scale_halves():
+    """A cell equal to the threshold is not a candidate (score > thresh); at scale 16 (scales
+    0.5) with no padding, root (y 0, x 2): Y1 (0+1)*16 = 16, X1 (2+1)*16 = 48, Y2 16+96-1 = 111,
+    X2 48+176-1 = 223; part 0 probe (2(0-1)+2, 2(2-1)+3) = (0, 5) -> 0 + 5*14 = 70 with
+    (Ix, Iy) = (3, 1): py 2, px 4 -> Y1 ((2-2)/2+1)*16 = 16, X1 ((4-2)/2+1)*16 = 32,
+    Y2 16+48-1 = 63, X2 32+48-1 = 79; part 1 probe (2(0-1)+4, 2(2-1)+1) = (2, 3) -> 44 with
+    (0, 0): py 1, px 1 -> Y1 X1 ((1-2)/2+1)*16 = 8, Y2 X2 8+47 = 55."""
+    m = _box_model(padx=0, pady=0)
+    score = np.full((4, 5), -2.0, np.float32)
+    score[2, 0] = 1.25
+    score[0, 3] = 0.25                                          # == thresh: not a candidate
+    Ix = [np.full((12, 14), 11, np.int32) for _ in range(2)]
+    Iy = [np.full((12, 14), 11, np.int32) for _ in range(2)]
+    Ix[0].ravel()[70], Iy[0].ravel()[70] = 3, 1
+    Ix[1].ravel()[44], Iy[1].ravel()[44] = 0, 0
+    rows = O.rows_one_map(m, 0, 16.0, score, Ix, Iy, thresh=0.25)
+    assert rows.tolist() == [[16, 48, 111, 223, 16, 32, 63, 79, 8, 8, 55, 55, 0, 1.25, 16]]
+
+
+def test_ipl_to_float_plane_order_hand_built():
+    """Ipl_to_FLOAT (CT/featurepyramid.cpp.new2:816-850): plane 0 takes byte 0 of each pixel
+    (B in OpenCV's BGR order), plane 1 byte 1, plane 2 byte 2; each plane is column-major
+    [x][y]; rows are widthStep bytes apart (here 3*2 + 2 padding bytes that must be skipped)."""
+    import ctypes as C
+
+    W, H, ws = 2, 2, 8
+    buf = np.array([1, 2, 3, 4, 5, 6, 99, 99,        # row y = 0: pixels x = 0, 1; padding
+                    7, 8, 9, 10, 11, 12, 99, 99], np.uint8)
+    out = np.zeros((3, W, H), np.float32)
+    O.lib().ora_ipl_to_float(buf.ctypes.data, W, H, ws, out.ctypes.data_as(C.POINTER(C.c_float)))
+    assert out[0].tolist() == [[1, 7], [4, 10]]     # B: [x][y]
+    assert out[1].tolist() == [[2, 8], [5, 11]]     # G
+    assert out[2].tolist() == [[3, 9], [6, 12]]     # R
+    frame = buf.reshape(2, 8)[:, :6].reshape(2, 2, 3).copy()
+    assert np.array_equal(O.ipl_to_float(frame), out)
 
 
 # ---------------------------------------------------------------------------------------
