@@ -117,6 +117,7 @@ _SIGS = [
     ("geye_profile_enable", C.c_int, [C.c_void_p, C.c_int]),
     ("geye_kernel_stats", C.c_int, [C.c_void_p, C.c_char_p, C.c_int, _dp, _llp, C.c_int, _ip]),
     ("geye_nms_fallbacks", C.c_int, [C.c_void_p, _llp]),
+    ("geye_dt_form", C.c_int, [C.c_void_p, C.c_int]),
 ]
 
 _lib = None
@@ -439,6 +440,11 @@ class Context:
                                        C.byref(n)))
         keys = names.value.decode().split("\n")[: n.value]
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(keys)}
+
+    def dt_form(self, form: str = "auto") -> None:
+        """DT form of whole-frame detection: "auto" (divide and conquer for fp32, windowed
+        for the 16-bit forms), "divide" or "window" for every precision."""
+        _check(lib().geye_dt_form(self.h, {"auto": 0, "divide": 1, "window": 2}[form]))
 
     def nms_fallbacks(self) -> int:
         """NMS calls on this context that were replayed on the host (score ties)."""

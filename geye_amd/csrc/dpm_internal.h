@@ -112,17 +112,18 @@ __device__ inline int preceding_count(const int *__restrict__ blocksum, int b, i
 // NMS.
 constexpr int kHostCandidates = 2, kHostSurvivors = 3, kHostTie = 4;
 
-// Halo on each side of an LDS line in the windowed DT: the widest window half-width.
+// Halo on each side of an LDS line in the windowed DT (at least its widest window
+// half-width, 16).
 constexpr int kDtHalo = 24;
 // dynamic LDS limit of the windowed DT kernel (160 KB less its static reduction buffer)
 constexpr int kDtWindowMaxLds = 160 * 1024 - 256;
 
-// LDS words per key line of the windowed DT: the line and one halo, which it shares with the
+// LDS words per line of the windowed DT: the line and one halo, which it shares with the
 // next line (line l at l * stride + kDtHalo, kDtHalo more words after the last line; every
-// halo word holds the same never-minimal key).  Padded so that the row pass's lanes, lines
+// halo word holds +inf, a cost that never wins).  Padded so that the row pass's lanes, lines
 // fastest, fall on distinct banks of the 32 that serve each half wave: with NL <= 32 lines
 // the stride is 32 / NL modulo 32, otherwise odd.
-__host__ __device__ inline int dt_key_stride(int n, int NL) {
+__host__ __device__ inline int dt_line_stride(int n, int NL) {
   int w = n + kDtHalo;
   if (NL <= 32) {
     const int r = 32 / NL;
@@ -151,7 +152,8 @@ struct DtJob {
   // 16-bit indices
   int sel, ax, ay;
   // windowed form: lines per batch of the divide and conquer it falls back to when the
-  // window would be too wide (float lines, argmins and node table within the key lines' LDS)
+  // window would be too wide (float lines, argmins and node table within the window lines'
+  // LDS)
   int fb_lines;
 };
 
@@ -327,6 +329,7 @@ struct geye_ctx {
   std::vector<geye::DeviceBuffer> scratch_bufs;
   geye::Plan *last_plan = nullptr;
   bool graphs = true;                 // replay the per-frame front as a HIP graph
+  int dt_form = GEYE_DT_AUTO;         // geye_dt_form
   int num_cu = 256;
   int *h_count = nullptr;             // pinned host slots for counts (coherent, mapped: the
   int *d_hcount = nullptr;            // compaction kernels write them directly; see kHost*)

@@ -30,7 +30,7 @@ void launch_conv_f32(hipStream_t s, int nf, const ConvTile *tiles, int ntiles, i
 // not be enabled for the kernel.
 int launch_dt(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, const int4 *nodes,
               const int *depths, const float *src, float *dst, int *ptr);
-// the 16-bit forms' DT: windowed minimum, same outputs as launch_dt
+// the 16-bit forms' DT: windowed minimum, the same outputs as launch_dt
 int launch_dt_window(hipStream_t s, const DtJob *jobs, int njobs, size_t lds, const int4 *nodes,
                      const int *depths, const float *src, float *dst, int *ptr);
 void launch_score(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,

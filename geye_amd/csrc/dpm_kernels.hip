@@ -846,43 +846,44 @@ __global__ __launch_bounds__(256) void k_dt(const DtJob *__restrict__ jobs,
 }
 
 // ---------------------------------------------------------------------------------------
-// Distance transform of the 16-bit forms: the same minimum, found by a bounded window.
-// For output d the argmin s satisfies f(s) + q(d - s) <= f(d) with q(t) = a*t*t + b*t,
-// so q(d - s) <= R, the job's value range, and for a > 0 every argmin lies in
-// t = d - s in [t-, t+], the roots of a*t*t + b*t = R.  When that fits a window of
-// half-width T (8, 12, 16 or 24), each output takes the minimum over the 2T + 1 sources of
-// its window; otherwise, and for a <= 0, the job runs the divide and conquer above.
+// Distance transform of the 16-bit forms: the reference's DT, with each output's minimum
+// searched in a bounded window instead of the divide and conquer's source range.
 //
-// The window search runs on 32-bit fixed-point keys: (f - fmin) and (q - qmin) are scaled
-// so that their sum stays below 2^30, shifted up 6 bits, and the window position j is
-// kept in the 6 low bits of the q term.  One integer add per candidate then gives a key
-// whose unsigned minimum is the minimal cost and its position; costs within one fixed-point
-// step (the job's cost span / 2^24) count as tied, so a near-tie need not pick the
-// reference's first index.  This form is only used where detections are matched by
-// overlap, not bit for bit.  M is the reference's cost expression at the chosen argmin,
-// evaluated from the float source.
+// For output d let g(d) be the first source of least cost c_d(s) = ((f(s) + (a*t)*t) + b*t),
+// t = d - s, over the whole line, with every cost rounded as the reference rounds it.
+//  - Window.  c_d(d) = f(d) exactly, and a source at |t| >= T loses to it whenever
+//    a t^2 - |b| |t| exceeds the line block's value range plus the costs' rounding error, so
+//    g(d) lies within T of d for the T the job takes from that bound.
+//  - Equality with the divide and conquer (CT/dt.cpp:61-85).  In exact arithmetic the
+//    cost difference of two sources s' < s changes by -2a (d' - d)(s - s') <= -2a from d to
+//    d' > d.  With eps a bound of the rounding error of every cost in the windows, the
+//    rounded first argmins g therefore cannot decrease along the line when a >= 2 eps
+//    (each decrease would need a rounded difference to move against an exact change of at
+//    least 2a, by more than 4 eps).  The divide and conquer scans node d over
+//    [s1, s2] = [g(d1 - 1), g(d2 + 1)], which holds g(d) when g is monotone, and keeps the
+//    first least cost there, which is g(d) (the first global minimum; earlier sources cost
+//    strictly more).  So its argmin at every node is g(d), and M is the same expression at
+//    the same source: the outputs are bit-identical to dt()'s.
+// The kernel checks both conditions per job from the staged values; a job that fails one
+// (a <= 0, a window wider than 16, or a too small against eps) runs the divide and conquer.
+// Each output then takes the first least cost over the 2T + 1 sources of its window in
+// ascending order, with the reference's cost expression and strict comparison.
 // ---------------------------------------------------------------------------------------
 template <int T, bool ROWS>
-__device__ inline void dt_window(const DtJob &J, const float *__restrict__ src_arena,
-                                 const unsigned *s_key, int LSk, const unsigned long long *s_q,
+__device__ inline void dt_window(const DtJob &J, const float *s_f, int LSf,
                                  float *__restrict__ dst_arena, int *__restrict__ ptr_arena) {
-  constexpr int NP = T + 1;  // 2T + 1 positions as T + 1 pairs (the last one padded)
+  constexpr int NW = 2 * T + 1;
   const int n = J.n, nl = J.nlines;
   const float a = J.a, b = J.b;
-  // q terms of positions (2i, 2i + 1) as one 64-bit word: one 64-bit add forms two keys,
-  // since no key reaches 2^32 there is no carry between the halves.  They are the same for
-  // every lane, so they live in scalar registers (read once from LDS, first lane's copy).
-  unsigned long long q[NP];
+  // the cost terms (a*t)*t and b*t of window position j (t = T - j) are the same for every
+  // lane, so they live in scalar registers
+  float Q[NW], B[NW];
 #pragma unroll
-  for (int i = 0; i < NP; i++) {
-    const unsigned long long v = s_q[i];
-    q[i] = ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32)) << 32) |
-           (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+  for (int j = 0; j < NW; j++) {
+    const float t = (float)(T - j);
+    Q[j] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((a * t) * t)));
+    B[j] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(b * t)));
   }
-  // M takes the source value at the argmin from memory (the job has just read it, so it
-  // comes from L2), negated as staged
-  const float *src = src_arena + J.src_off + (long long)J.line0 * J.line_stride;
-  const float sgn = J.neg ? -1.0f : 1.0f;
   float *dst = dst_arena + J.dst_off + (long long)J.line0 * J.line_stride;
   int *pdst = ptr_arena + J.dst_off + (long long)J.line0 * J.line_stride;
   // the selected outputs go to half grids of the map (H = ceil(P0 / 2) cells per column):
@@ -918,28 +919,27 @@ __device__ inline void dt_window(const DtJob &J, const float *__restrict__ src_a
   while (ROWS ? d < dlim : l < nl) {
     const int dd = sel_rows ? 2 * d + par : d;
     if (!ROWS || (l < nl && dd < n)) {
-      const unsigned *w = s_key + l * LSk + kDtHalo - T + dd;
-      unsigned best = 0xffffffffu;
+      // sources dd - T .. dd + T; outside the line the halo holds +inf, which never wins
+      const float *w = s_f + l * LSf + kDtHalo - T + dd;
+      float best = __builtin_inff();
+      int bj = T;
 #pragma unroll
-      for (int i = 0; i < NP; i++) {
-        const unsigned hi = 2 * i + 1 < 2 * T + 1 ? w[2 * i + 1] : 0u;
-        const unsigned long long k = (((unsigned long long)hi << 32) | w[2 * i]) + q[i];
-        best = min(best, min((unsigned)k, (unsigned)(k >> 32)));
+      for (int j = 0; j < NW; j++) {
+        const float c = (w[j] + Q[j]) + B[j];
+        if (best > c) {
+          best = c;
+          bj = j;
+        }
       }
-      const int s = dd - T + (int)(best & 63u);
-      float v = 0.0f;
-      if (!sel_rows || kind == 0) {
-        const float f = src[(long long)l * J.line_stride + (long long)s * J.elem_stride];
-        v = dt_cost(J.neg ? f * sgn : f, a, b, dd - s);
-      }
+      const int s = dd - T + bj;
       if (sel_rows) {
         // half grid: (y, x) at (x >> 1) * H + (y >> 1), M or the 16-bit argmin
         const long long oc = (long long)(dd >> 1) * H + ((J.line0 + l) >> 1);
-        if (kind == 0) dstc[oc] = v;
+        if (kind == 0) dstc[oc] = best;
         else pc[oc] = (unsigned short)s;
       } else {
         const long long o = (long long)l * J.line_stride + (long long)dd * J.elem_stride;
-        dst[o] = v;
+        dst[o] = best;
         if (!sel_cols) pdst[o] = s;
         else if (((dd - J.ay) & 1) == 0)
           pc[(long long)(J.line0 + l) * H + (dd >> 1)] = (unsigned short)s;  // column-major half
@@ -998,16 +998,24 @@ __global__ __launch_bounds__(256) void k_dt_window(const DtJob *__restrict__ job
                                                    int *__restrict__ ptr_arena) {
   extern __shared__ float smem[];
   __shared__ float s_red[2][4];
-  __shared__ unsigned long long s_q[kDtHalo + 1];
   const DtJob J = jobs[xcd_local_block(blockIdx.x, gridDim.x)];
-  const int n = J.n, NLn = 256 / J.G, LSk = dt_key_stride(n, NLn);
+  const int n = J.n, NLn = 256 / J.G, LSf = dt_line_stride(n, NLn);
   const int tid = threadIdx.x;
-  // [NLn][LSk] + kDtHalo key lines with shared halos: the source is staged in place (between
-  // the halos) and turned into keys there, so a job needs no separate float copy and more
-  // jobs share a CU
-  unsigned *s_key = (unsigned *)smem;
+  // [NLn][LSf] + kDtHalo float lines that share their halos (line l at l * LSf + kDtHalo):
+  // the halo words hold +inf, the staged source the rest
+  {
+    const int gap = LSf - n;
+    for (int i = tid; i < kDtHalo + J.nlines * gap; i += 256) {
+      int at = i;
+      if (i >= kDtHalo) {
+        const int k = i - kDtHalo, ln = k / gap;
+        at = ln * LSf + kDtHalo + n + (k - ln * gap);
+      }
+      smem[at] = __builtin_inff();
+    }
+  }
   float vmin = __builtin_inff(), vmax = -__builtin_inff();
-  dt_stage<true>(J, src_arena, smem, LSk, kDtHalo, vmin, vmax);
+  dt_stage<true>(J, src_arena, smem, LSf, kDtHalo, vmin, vmax);
   for (int m = 1; m < 64; m <<= 1) {
     vmin = fminf(vmin, __shfl_xor(vmin, m));
     vmax = fmaxf(vmax, __shfl_xor(vmax, m));
@@ -1019,21 +1027,30 @@ __global__ __launch_bounds__(256) void k_dt_window(const DtJob *__restrict__ job
   __syncthreads();
   vmin = fminf(fminf(s_red[0][0], s_red[0][1]), fminf(s_red[0][2], s_red[0][3]));
   vmax = fmaxf(fmaxf(s_red[1][0], s_red[1][1]), fmaxf(s_red[1][2], s_red[1][3]));
-  // half-width of the argmin band: a source at offset t = p - q can beat the centre only
-  // where a t^2 + b t < f(p) - f(q) <= R (R carries a margin for the rounding of the costs),
-  // i.e. for integer t with |t| <= h, the larger root magnitude
-  int need = 1 << 20;
+  // Window half-width.  A cost (f + (a*t)*t) + b*t is within u (2|f| + 4 a t^2 + 2|b t|)
+  // (1 + 1e-3) of the exact f + a t^2 + b t (u = 2^-24), so a source at offset t loses to
+  // the centre (cost f(d), exact) once a (1 - 2^-20) t^2 - |b| (1 + 2^-20) |t| exceeds the
+  // value range plus 2^-22 max|f|: beyond the positive root x of that quadratic.
   const float a = J.a, b = J.b;
-  const float R = (vmax - vmin) * 1.00001f + 1e-4f;
-  if (a > 0.0f && R < 1e30f) {
-    const float disc = sqrtf(b * b + 4.0f * a * R);
-    const float tp = (-b + disc) / (2.0f * a), tm = (-b - disc) / (2.0f * a);
-    const float h = fmaxf(tp, -tm);
-    if (h < 64.0f) need = (int)floorf(h) + 1;
+  const float fmag = fmaxf(fabsf(vmin), fabsf(vmax));
+  int T = 0;
+  if (a > 0.0f && fmag < 1e30f) {
+    const float a1 = a * (1.0f - 9.5367431640625e-7f), b1 = fabsf(b) * (1.0f + 9.5367431640625e-7f);
+    const float R = (vmax - vmin) + fmag * 2.384185791015625e-7f + 1e-30f;
+    const float x = (b1 + sqrtf(b1 * b1 + 4.0f * a1 * R)) / (2.0f * a1);
+    if (x < 16.0f) {
+      const int need = (int)floorf(x * (1.0f + 1.52587890625e-5f) + 9.765625e-4f) + 1;
+      const int Tw = need <= 8 ? 8 : need <= 10 ? 10 : need <= 12 ? 12 : need <= 14 ? 14 : 16;
+      // monotone argmins: a >= 2 eps with eps = 2^-23 (2 max|f| + 4 a T^2 + 2 |b| T) (twice
+      // the rounding bound above)
+      const float ft = (float)Tw;
+      const float eps = 1.1920928955078125e-7f * (2.0f * fmag + 4.0f * a * ft * ft + 2.0f * fabsf(b) * ft);
+      if (need <= 16 && a >= 2.0f * eps) T = Tw;
+    }
   }
-  if (need > kDtHalo) {
+  if (T == 0) {
     // the reference's divide and conquer, J.fb_lines lines at a time (float lines, argmins
-    // and the node table fit the key area); the lines are staged again from memory
+    // and the node table fit the window lines' area); the lines are staged again from memory
     const int fb = J.fb_lines;
     float *s_f = smem;                                                // [fb][n]
     int *s_ptr = (int *)(smem + fb * n);                              // [fb][n]
@@ -1053,52 +1070,15 @@ __global__ __launch_bounds__(256) void k_dt_window(const DtJob *__restrict__ job
     }
     return;
   }
-  const int T = need <= 8 ? 8 : need <= 10 ? 10 : need <= 12 ? 12 : need <= 14 ? 14
-              : need <= 16 ? 16 : need <= 20 ? 20 : kDtHalo;
-  // fixed-point scale: (f - fmin) + (q - qmin) over the window stays below 2^24 steps
-  const float qmin = fminf(0.0f, -b * b / (4.0f * a));  // the parabola's vertex bounds q below
-  const float qmax = a * (float)T * (float)T + fabsf(b) * (float)T;
-  const float span = (vmax - vmin) + (qmax - qmin);
-  const float scale = span > 0.0f ? 16777216.0f / span : 1.0f;
-  // the q terms of the 2T + 1 window positions, one lane each, paired for the 64-bit adds:
-  // position j is source s = d - T + j, t = T - j; 6 low bits hold j
-  if (tid < 2 * T + 2) {
-    unsigned v = 0xffffffffu;  // pad position of the last pair (source 0): never the minimum
-    if (tid <= 2 * T) {
-      const float t = (float)(T - tid);
-      const float qv = fmaxf((a * t + b) * t - qmin, 0.0f);
-      v = ((unsigned)(qv * scale + 0.5f) << 6) | (unsigned)tid;
-    }
-    ((unsigned *)s_q)[tid] = v;
-  }
-  {
-    const int sl = 256 / LSk, sh = 256 - sl * LSk;
-    int l = tid / LSk, h = tid - l * LSk;
-    for (int i = tid; i < J.nlines * LSk + kDtHalo; i += 256) {
-      const int p = h - kDtHalo;
-      unsigned k = 0x80000000u;  // halo: above every window cost, no carry out of 32 bits
-      if (p >= 0 && p < n) k = (unsigned)((__uint_as_float(s_key[i]) - vmin) * scale + 0.5f) << 6;
-      s_key[i] = k;
-      l += sl;
-      h += sh;
-      if (h >= LSk) {
-        h -= LSk;
-        l++;
-      }
-    }
-  }
-  __syncthreads();
   const bool rows = J.elem_stride != 1;
-#define GEYE_DT_WINDOW(TT)                                                              \
-  (rows ? dt_window<TT, true>(J, src_arena, s_key, LSk, s_q, dst_arena, ptr_arena)      \
-        : dt_window<TT, false>(J, src_arena, s_key, LSk, s_q, dst_arena, ptr_arena))
+#define GEYE_DT_WINDOW(TT)                                                 \
+  (rows ? dt_window<TT, true>(J, smem, LSf, dst_arena, ptr_arena)           \
+        : dt_window<TT, false>(J, smem, LSf, dst_arena, ptr_arena))
   if (T == 8) GEYE_DT_WINDOW(8);
   else if (T == 10) GEYE_DT_WINDOW(10);
   else if (T == 12) GEYE_DT_WINDOW(12);
   else if (T == 14) GEYE_DT_WINDOW(14);
-  else if (T == 16) GEYE_DT_WINDOW(16);
-  else if (T == 20) GEYE_DT_WINDOW(20);
-  else GEYE_DT_WINDOW(kDtHalo);
+  else GEYE_DT_WINDOW(16);
 #undef GEYE_DT_WINDOW
 }
 

@@ -156,10 +156,10 @@ static size_t dt_lds_bytes(int n, int G) {
   return (words + (words & 1)) * 4 + (size_t)n * 8;
 }
 
-// the windowed DT: key lines (n plus halos, padded), the source staged into them
-static size_t dt_lds_bytes_window_keys(int n, int G) {
+// the windowed DT: float lines (n plus halos, padded), the source staged into them
+static size_t dt_lds_bytes_window_lines(int n, int G) {
   const int NL = 256 / G;
-  return ((size_t)NL * (size_t)dt_key_stride(n, NL) + kDtHalo) * 4;
+  return ((size_t)NL * (size_t)dt_line_stride(n, NL) + kDtHalo) * 4;
 }
 
 // its divide-and-conquer fallback, fb lines at a time: float lines, argmins, node table
@@ -168,16 +168,16 @@ static size_t dt_lds_bytes_fallback(int n, int fb) {
   return (words + (words & 1)) * 4 + (size_t)n * 8;
 }
 
-// the most fallback lines per batch within the key lines' LDS (one always fits: 16 n bytes
+// the most fallback lines per batch within the window lines' LDS (one always fits: 16 n bytes
 // against at least 4 (n + 24) + 24 words)
 static int dt_fallback_lines(int n, int G) {
   int fb = 256 / G;
-  while (fb > 1 && dt_lds_bytes_fallback(n, fb) > dt_lds_bytes_window_keys(n, G)) fb--;
+  while (fb > 1 && dt_lds_bytes_fallback(n, fb) > dt_lds_bytes_window_lines(n, G)) fb--;
   return fb;
 }
 
 static size_t dt_lds_bytes_window(int n, int G) {
-  return std::max(dt_lds_bytes_window_keys(n, G),
+  return std::max(dt_lds_bytes_window_lines(n, G),
                   dt_lds_bytes_fallback(n, dt_fallback_lines(n, G)));
 }
 
@@ -191,7 +191,7 @@ static int dt_pick_G(int n) {
 
 // the same for the windowed kernel, whose lines carry halos, within kDtWindowJobLds: each
 // job is a load - compute - store sequence with barriers between, so several jobs per CU
-// overlap their phases (on 1080p part maps 24 KB of key lines beat 12, 16, 20, 28, 32 and 40 KB)
+// overlap their phases (on 1080p part maps 24 KB of lines beat 12, 16, 20, 28, 32 and 40 KB)
 constexpr size_t kDtWindowJobLds = 24 * 1024;
 static int dt_pick_G_window(int n) {
   for (int G = 1; G <= 64; G *= 2)

@@ -617,6 +617,13 @@ namespace {
 // ---------------------------------------------------------------------------------------
 // Pipeline stages
 // ---------------------------------------------------------------------------------------
+// The DT form a detection runs: the divide and conquer (the reference's order of work) for
+// FP32, the windowed minimum for the 16-bit forms, unless geye_dt_form chose one for all.
+// Both give the same outputs; the windowed one writes them as half grids.
+bool dt_windowed(const geye_ctx *c, int precision) {
+  return c->dt_form == GEYE_DT_WINDOW || (c->dt_form == GEYE_DT_AUTO && precision != GEYE_FP32);
+}
+
 int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, int hkind) {
   hipStream_t s = c->stream;
   {
@@ -769,9 +776,10 @@ int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int 
   } else {
     return fail(GEYE_ERR_ARG, "unknown precision");
   }
-  // FP32 runs the reference's divide and conquer (bit-exact argmax); the 16-bit forms,
-  // whose detections are matched by overlap, take the windowed minimum
-  const bool exact_dt = precision == GEYE_FP32;
+  // FP32 runs the reference's divide and conquer; the 16-bit forms take the windowed
+  // minimum, which gives the same outputs and writes only the cells the scoring and the part
+  // boxes read
+  const bool exact_dt = !dt_windowed(c, precision);
   auto *dt_launch = exact_dt ? launch_dt : launch_dt_window;
   const auto &jobs1 = exact_dt ? P.dt_pass1 : P.dt_pass1w;
   const auto &jobs2 = exact_dt ? P.dt_pass2 : P.dt_pass2w;
@@ -1015,7 +1023,8 @@ int run_front(geye_ctx *c, Model &m, Plan &P, const unsigned char *d_bgr, int ws
   }
   unsigned int tb;
   std::memcpy(&tb, &thresh, 4);
-  const long long key = ((long long)precision << 48) ^ ((long long)ws << 32) ^ (long long)tb;
+  const long long key = ((long long)precision << 48) ^ ((long long)dt_windowed(c, precision) << 52) ^
+                        ((long long)ws << 32) ^ (long long)tb;
   CHK(ensure_frame(P, (size_t)ws * P.H));
   if (d_bgr != P.d_frame.p)
     HIPCHK(hipMemcpyAsync(P.d_frame.p, d_bgr, (size_t)ws * P.H, hipMemcpyDeviceToDevice, c->stream));
@@ -1080,10 +1089,10 @@ int detect_common(geye_ctx *c, const geye_model *model, const unsigned char *bgr
   if (do_nms) {
     int nk = 0;
     CHK(run_nms(c, m, *P, ncand, overlap, &nk));
-    CHK(run_rows(c, m, *P, dp<int>(P->n_kept), nk, precision != GEYE_FP32, rows));
+    CHK(run_rows(c, m, *P, dp<int>(P->n_kept), nk, dt_windowed(c, precision), rows));
     *num = nk;
   } else {
-    CHK(run_rows(c, m, *P, nullptr, ncand, precision != GEYE_FP32, rows));
+    CHK(run_rows(c, m, *P, nullptr, ncand, dt_windowed(c, precision), rows));
     *num = ncand;
   }
   *boxes = to_malloc(rows);
@@ -1564,7 +1573,7 @@ int geye_get_boxes_features(geye_ctx *c, const geye_model *model, int width, int
   CHK(run_scores(c, m, *P, thresh, GEYE_FP32, &ncand));
   std::vector<float> rows;
   c->last_plan = P;
-  CHK(run_rows(c, m, *P, nullptr, ncand, false, rows));
+  CHK(run_rows(c, m, *P, nullptr, ncand, dt_windowed(c, GEYE_FP32), rows));
   *num = ncand;
   *boxes = to_malloc(rows);
   *GL = P->GL;
@@ -1999,6 +2008,15 @@ int geye_profile_enable(geye_ctx *c, int on) {
   flush_stats(c);
   c->profile = on != 0;
   c->stats.clear();
+  return GEYE_OK;
+}
+
+int geye_dt_form(geye_ctx *c, int form) {
+  if (!c) return fail(GEYE_ERR_ARG, "null ctx");
+  if (form != GEYE_DT_AUTO && form != GEYE_DT_DIVIDE && form != GEYE_DT_WINDOW)
+    return fail(GEYE_ERR_ARG, "unknown DT form");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->dt_form = form;
   return GEYE_OK;
 }
 

@@ -161,13 +161,11 @@ int geye_fconv(geye_ctx *ctx, const float *feat, int F0, int F1, const float *fi
                int h, int w, int precision, float *out);
 int geye_dt(geye_ctx *ctx, const float *vals, float ax, float bx, float ay, float by,
             const int *dims, float *M, int *Ix, int *Iy);
-/* The DT of the 16-bit forms (GEYE_BF16, GEYE_FP16), same arguments as geye_dt
- * (CT/dt.cpp:101).  The minimum over each line is searched in a window bounded by the
- * line's value range, which contains every argmin when ax, ay > 0; otherwise, or when the
- * window would pass 24, it is geye_dt's divide and conquer.  M is the reference's cost
- * expression at the chosen argmin; costs closer than 2^-24 of the line block's cost span
- * are treated as tied, so at such near-ties the argmin may differ from dt()'s first
- * minimum. */
+/* The DT of the 16-bit forms (GEYE_BF16, GEYE_FP16), same arguments and the same outputs,
+ * bit for bit, as geye_dt (CT/dt.cpp:101).  Each output takes the first least cost in a window
+ * bounded by the line block's value range, which holds dt()'s argmin when ax, ay > 0 and the
+ * costs' rounding cannot reorder the argmins along a line; where either fails, or the window
+ * would pass 16, the block runs geye_dt's divide and conquer. */
 int geye_dt_window(geye_ctx *ctx, const float *vals, float ax, float bx, float ay, float by,
                    const int *dims, float *M, int *Ix, int *Iy);
 /* add_part_calculation (CT/dt.cpp:38-55): score[y + x*rootsize[0]] -=
@@ -236,6 +234,12 @@ int geye_graph_enable(geye_ctx *ctx, int on);
  * CU.  geye_mfma_tuning reports the choice and "name=ms;" per schedule tried;
  * geye_mfma_schedule overrides it (every schedule gives the same bits). */
 int geye_mfma_tuning(geye_ctx *ctx, int precision, int *choice, char *text, int cap);
+/* The DT form whole-frame detection runs on ctx: GEYE_DT_AUTO (default) takes the
+ * divide and conquer of dt() (CT/dt.cpp:61-136) for GEYE_FP32 and the windowed minimum of
+ * geye_dt_window for the 16-bit forms; GEYE_DT_DIVIDE / GEYE_DT_WINDOW take one of them for
+ * every precision.  Both give the same bits. */
+enum { GEYE_DT_AUTO = 0, GEYE_DT_DIVIDE = 1, GEYE_DT_WINDOW = 2 };
+int geye_dt_form(geye_ctx *ctx, int form);
 int geye_mfma_schedule(geye_ctx *ctx, int precision, int choice);
 int geye_kernel_stats(geye_ctx *ctx, char *names, int names_cap, double *ms, long long *count,
                       int cap, int *n);

@@ -5,8 +5,8 @@ C2: one 1920x1080 rectangle-noise frame (seed 2), the synthetic car model, FP32,
     bit-identical to the oracle (CT/get_boxes.cpp:1497-1570, CT/nms.cpp:94-217).
 C3: the same frame with the convolution in BF16 and FP16 on MFMA (FP32 accumulation): score
     maps against the oracle's FP64 scoring (max-abs and relative error printed and bounded),
-    the DT argmins (the part boxes) exact for the scores the form computed, and the post-NMS
-    detection set against the FP32 form's at IoU >= 0.99.
+    the windowed DT's rows equal to the divide and conquer's on the scores the form computed,
+    and the post-NMS detection set against the FP32 form's at IoU >= 0.99.
 C5: the four camera sizes 640x480, 1280x720, 1920x1080 and 3840x2160, each frame detected
     with the car and the person model: every result identical alone and in the serving mix.
 """
@@ -64,15 +64,22 @@ def c2():
     return model, frame, rows, kept
 
 
-def test_c2_fp32_1080p_bit_exact(ctx, c2):
+@pytest.mark.parametrize("dt_form", ["auto", "window"])
+def test_c2_fp32_1080p_bit_exact(ctx, c2, dt_form):
+    """FP32 with the reference's divide-and-conquer DT (the plain form) and with the windowed
+    DT of the 16-bit forms: both bit-identical to the oracle at 1080p."""
     model, frame, rows, kept = c2
     gm = ctx.model(model)
     before = ctx.nms_fallbacks()
-    g = gm.get_boxes(frame, thresh=-0.5, precision="fp32")
-    say("candidates", len(g))
-    assert len(g) > 500_000
-    rows_equal(g, rows)
-    k = gm.car_detection(frame, thresh=-0.5, overlap=0.5, precision="fp32")
+    ctx.dt_form(dt_form)
+    try:
+        g = gm.get_boxes(frame, thresh=-0.5, precision="fp32")
+        say("candidates", len(g))
+        assert len(g) > 500_000
+        rows_equal(g, rows)
+        k = gm.car_detection(frame, thresh=-0.5, overlap=0.5, precision="fp32")
+    finally:
+        ctx.dt_form("auto")
     say("boxes", len(k))
     rows_equal(k, kept)
     assert ctx.nms_fallbacks() == before, "the NMS was replayed on the host"
@@ -93,8 +100,15 @@ def c3_fp64(c2):
 def test_c3_precision_1080p(ctx, c2, c3_fp64, prec, max_abs, min_agree):
     model, frame, _, kept32 = c2
     gm = ctx.model(model)
-    gm.get_boxes(frame, thresh=-0.5, precision=prec)
+    rows = gm.get_boxes(frame, thresh=-0.5, precision=prec)
     g = gm.last_scores().astype(np.float64)
+    # the DT of this form's part scores is dt()'s: the divide and conquer gives the same rows
+    ctx.dt_form("divide")
+    try:
+        rows_dc = gm.get_boxes(frame, thresh=-0.5, precision=prec)
+    finally:
+        ctx.dt_form("auto")
+    rows_equal(rows, rows_dc)
     ref = c3_fp64
     assert g.shape == ref.shape
     err = np.abs(g - ref)

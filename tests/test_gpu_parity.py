@@ -129,14 +129,7 @@ def test_dt_matches_reference_vectors(ctx):
         assert bits_equal(M, z[f"M{k}"]) and np.array_equal(Ix, z[f"Ix{k}"]) and np.array_equal(Iy, z[f"Iy{k}"])
 
 
-def _dt_cost(src, a, b, t):
-    """The reference's cost expression ((src + (a*t)*t) + b*t) in float32 (CT/dt.cpp:71-78)."""
-    a, b = np.float32(a), np.float32(b)
-    t = t.astype(np.float32)
-    return ((src + (a * t) * t) + b * t).astype(np.float32)
-
-
-@pytest.mark.parametrize("d0,d1,defs,scale,exact", [
+@pytest.mark.parametrize("d0,d1,defs,scale,ties", [
     (23, 17, (0.01, 0.0, 0.01, 0.0), 0.15, False),
     (41, 64, (0.023062, 0.010890, 0.015328, -0.009441), 0.3, False),
     (263, 40, (0.01, 0.0, 0.01, 0.0), 0.1, False),
@@ -146,42 +139,44 @@ def _dt_cost(src, a, b, t):
     (130, 70, (0.2, 0.1, 0.3, -0.2), 0.4, False),
     (5, 6, (0.01, 0.0, 0.01, 0.0), 0.2, False),
     (200, 90, (0.01, 0.0, 0.01, 0.0), 0.3, False),    # value ranges giving half-widths
-    (150, 33, (0.01, 0.0, 0.01, 0.0), 0.4, False),    # of about 14, 16, 20 and 24
-    (96, 80, (0.01, 0.0, 0.01, 0.0), 0.6, False),
+    (150, 33, (0.01, 0.0, 0.01, 0.0), 0.4, False),    # of about 14, 16, 20 and 24 (the
+    (96, 80, (0.01, 0.0, 0.01, 0.0), 0.6, False),     # last two run the divide and conquer)
     (70, 75, (0.01, 0.0, 0.01, 0.0), 0.85, False),
-    (64, 64, (0.01, 0.0, 0.01, 0.0), 100.0, True),   # argmin band wider than 24
-    (50, 40, (0.0, 0.0, 0.0, 0.0), 1.0, True),       # a = 0: no band
-    (37, 29, (-0.01, 0.0, -0.02, 0.0), 1.0, True),   # a < 0: no band
+    (64, 64, (0.01, 0.0, 0.01, 0.0), 100.0, False),  # argmin band wider than any window
+    (50, 40, (0.0, 0.0, 0.0, 0.0), 1.0, False),      # a = 0: no band
+    (37, 29, (-0.01, 0.0, -0.02, 0.0), 1.0, False),  # a < 0: no band
+    (120, 90, (0.01, 0.0, 0.01, 0.0), 0.3, True),    # values on a 1/8 grid: many exact ties
+    (90, 120, (0.25, 0.5, 0.25, -0.5), 1.0, True),
+    (60, 70, (0.001, 0.0002, 0.002, -0.0001), 0.01, True),  # weak costs, tiny ranges
 ])
-def test_dt_window_minimum(ctx, d0, d1, defs, scale, exact):
-    """DT of the 16-bit forms (geye_dt_window) against the oracle's dt():
-    - M is the reference's cost at the returned (Ix, Iy), bit for bit;
-    - M is the minimum: within 2^-14 * (1 + range) of the FP64 oracle's minimum;
-    - argmins equal dt()'s except at near-ties (>= 98% of cells on random data);
-    - where no window can hold the argmin band (range too wide, a <= 0) it is dt()'s divide
-      and conquer, bit for bit."""
-    say("dt window", d0, d1, defs, scale)
+def test_dt_window_matches_oracle(ctx, d0, d1, defs, scale, ties):
+    """DT of the 16-bit forms (geye_dt_window) is dt() bit for bit: M, Ix and Iy equal the
+    oracle's on random maps whose value ranges give window half-widths from 8 to 16, on ranges
+    too wide for any window and on a <= 0 (the divide and conquer runs there), and on maps
+    full of exact ties, where the first minimum decides."""
+    say("dt window", d0, d1, defs, scale, ties)
     rng = np.random.default_rng(d0 * 7919 + d1)
-    v = (rng.standard_normal((d1, d0)) * scale).astype(np.float32)
+    v = rng.standard_normal((d1, d0)) * scale
+    if ties:
+        v = np.round(v * 8 / scale) * scale / 8
+    v = v.astype(np.float32)
     M, Ix, Iy = ctx.dt(v, *defs, window=True)
     oM, oIx, oIy = O.dt(v, *defs)
-    if exact:
-        assert bits_equal(M, oM) and np.array_equal(Ix, oIx) and np.array_equal(Iy, oIy)
-        return
-    ax, bx, ay, by = defs
-    assert Ix.min() >= 0 and Ix.max() < d1 and Iy.min() >= 0 and Iy.max() < d0
-    xs = np.arange(d1)[:, None]
-    ys = np.arange(d0)[None, :]
-    inner = _dt_cost(v[Ix, Iy], ay, by, ys - Iy)
-    assert bits_equal(M, _dt_cost(inner, ax, bx, xs - Ix))
-    M64, _, _ = O.dt(v, *defs, f64=True)
-    tol = 2.0 ** -14 * (1.0 + float(v.max() - v.min()))
-    err = np.abs(M.astype(np.float64) - M64)
-    say("max |M - M64|", float(err.max()), "tol", tol)
-    assert err.max() <= tol
-    agree = np.mean((Ix == oIx) & (Iy == oIy))
-    say("argmin agreement", float(agree))
-    assert agree >= 0.98
+    assert bits_equal(M, oM)
+    assert np.array_equal(Ix, oIx) and np.array_equal(Iy, oIy)
+
+
+def test_dt_window_matches_reference_vectors(ctx):
+    """The windowed DT on the reference's own dt() vectors (random maps, plateaus, ties,
+    all-zero maps with a = 0, large coefficients)."""
+    from conftest import GOLDEN
+
+    z = np.load(os.path.join(GOLDEN, "dt_ref.npz"), allow_pickle=False)
+    for k in range(int(z["n"])):
+        say("dt window golden", k)
+        d = [float(x) for x in z[f"def{k}"]]
+        M, Ix, Iy = ctx.dt(z[f"in{k}"], *d, window=True)
+        assert bits_equal(M, z[f"M{k}"]) and np.array_equal(Ix, z[f"Ix{k}"]) and np.array_equal(Iy, z[f"Iy{k}"])
 
 
 def _rows_equal(g, o):
@@ -326,9 +321,10 @@ def test_detect_mfma_scores_vs_fp64(ctx, prec):
 
 @pytest.mark.parametrize("prec", ["bf16", "fp16"])
 def test_mfma_part_boxes_match_fp32(ctx, prec):
-    """The 16-bit forms' part boxes come from their own DT argmins: candidate rows matched
-    to the FP32 rows by root box, component and scale carry the same part boxes except
-    where a near-tie of the 16-bit scores moved an argmin (>= 97% of the part boxes)."""
+    """The 16-bit forms' part boxes come from the DT of their own part scores (the same DT
+    as the FP32 form's, bit for bit, on those scores): candidate rows matched to the FP32 rows
+    by root box, component and scale carry the same part boxes except where the 16-bit part
+    scores moved an argmin (>= 97% of the part boxes)."""
     say("part boxes", prec, "640x480 synthetic car")
     frame = synth.make_frame(640, 480, seed=5)
     gm = ctx.model(synth.synthetic_car())
@@ -355,7 +351,7 @@ def test_mfma_part_boxes_match_fp32(ctx, prec):
 @pytest.mark.parametrize("prec", ["bf16", "fp16"])
 def test_detect_mfma_weak_deformation(ctx, prec):
     """Quadratic deformation costs 20x weaker (a = 5e-4): a job whose part responses span
-    more than a * 24^2 = 0.29 needs a window wider than the windowed DT's 24, so its jobs
+    more than a * 16^2 = 0.13 needs a window wider than the windowed DT's 16, so its jobs
     take the divide and conquer, which must still write the selected outputs where the
     scoring and the part boxes read them.  Scores against the FP64 oracle (0.02 absolute),
     part boxes against the FP32 form's."""
@@ -408,6 +404,37 @@ def test_accumulated_score_matches_oracle(ctx, car_model, which, W, H, thresh):
     gm.get_boxes(frame, thresh=thresh)
     g2 = gm.accumulated_score(W, H, thresh, prior.copy())
     assert bits_equal(g2, O.ascore(m, frame, thresh, prior.copy()))
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
+@pytest.mark.parametrize("which", ["synthetic", "car"])
+def test_dt_forms_give_the_same_detections(ctx, car_model, prec, which):
+    """The windowed DT inside the pipeline (half-grid outputs read by the scoring and the
+    part boxes) against the divide and conquer, for every precision: candidate rows, score
+    maps and NMS survivors bit for bit; in FP32 both also equal the oracle."""
+    say("dt forms", prec, which)
+    m = ctx.model(car_model if which == "car" else synth.synthetic_car())
+    frame = synth.make_frame(640, 480, seed=6)
+    thresh = -1.2 if which == "car" else -0.5   # the real car model scores lower on noise
+    out = {}
+    try:
+        for form in ("divide", "window"):
+            ctx.dt_form(form)
+            rows = m.get_boxes(frame, thresh=thresh, precision=prec)
+            scores = m.last_scores().copy()
+            kept = m.car_detection(frame, thresh=thresh, overlap=0.5, precision=prec)
+            out[form] = (rows, scores, kept)
+    finally:
+        ctx.dt_form("auto")
+    (r1, s1, k1), (r2, s2, k2) = out["divide"], out["window"]
+    say("candidates", len(r1), "boxes", len(k1))
+    assert len(r1) > 20
+    _rows_equal(r2, r1)
+    assert bits_equal(s2, s1)
+    _rows_equal(k2, k1)
+    if prec == "fp32":
+        o = O.detect(car_model if which == "car" else synth.synthetic_car(), frame, thresh)
+        _rows_equal(r2, o)
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16"])
