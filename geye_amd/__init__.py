@@ -110,6 +110,8 @@ _SIGS = [
     ("geye_dt_window", C.c_int, [C.c_void_p, _fp, C.c_float, C.c_float, C.c_float, C.c_float, _ip,
                                  _fp, _ip, _ip]),
     ("geye_nms", C.c_int, [C.c_void_p, _fp, C.c_int, C.c_int, C.c_float, _fp, _ip]),
+    ("geye_dt_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_float,
+                                  C.c_float, _ip, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("geye_dev_alloc", C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     ("geye_dev_free", C.c_int, [C.c_void_p, C.c_void_p]),
     ("geye_memcpy_h2d", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),

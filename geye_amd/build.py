@@ -18,7 +18,8 @@ OUT = os.path.join(HERE, "libgeye_amd.so")
 BUILD = os.path.join(HERE, "_build")
 
 SOURCES = ["dpm_kernels.hip", "dpm_plan.cpp", "dpm_runtime.cpp"]
-EXTRA_SOURCES = ["dpm_mfma.hip", "dpm_nms.hip", "dpm_ascore.hip", "dpm_rects.hip", "dpm_serve.cpp"]
+EXTRA_SOURCES = ["dpm_mfma.hip", "dpm_nms.hip", "dpm_ascore.hip", "dpm_rects.hip", "dpm_serve.cpp",
+                 "dpm_noalloc.hip"]
 
 
 def hipcc() -> str:

@@ -655,6 +655,14 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
   return GEYE_OK;
 }
 
+int dt_line_geometry(int n, bool window, int *G, size_t *lds) {
+  const int g = window ? dt_pick_G_window(n) : dt_pick_G(n);
+  if (g < 0) return GEYE_ERR_UNSUPPORTED;
+  *G = g;
+  *lds = window ? dt_lds_bytes_window(n, g) : dt_lds_bytes(n, g);
+  return GEYE_OK;
+}
+
 int dt_single_jobs(int n0, int n1, float ax, float bx, float ay, float by,
                    std::vector<DtJob> &p1, std::vector<DtJob> &p2, std::vector<int4> &nodes,
                    std::vector<int> &depths, size_t &lds1, size_t &lds2, bool window) {

@@ -1258,6 +1258,10 @@ int geye_init(int device, geye_ctx **ctx) {
     delete c;
     return fail(GEYE_ERR_HIP, "pinned count buffer / events");
   }
+  // the allocation-free entry point's scratch array on this device (its address is
+  // resolved here, once per device, with the device current); a failure is reported by
+  // that entry point, not here
+  (void)noalloc_setup(device);
   *ctx = c;
   return GEYE_OK;
 }

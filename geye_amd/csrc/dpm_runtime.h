@@ -21,6 +21,13 @@ int dt_single_jobs(int n0, int n1, float ax, float bx, float ay, float by,
                    std::vector<DtJob> &p1, std::vector<DtJob> &p2, std::vector<int4> &nodes,
                    std::vector<int> &depths, size_t &lds1, size_t &lds2, bool window = false);
 
+// lanes per line and dynamic LDS of a DT job over lines of length n (host only, allocates
+// nothing); GEYE_ERR_UNSUPPORTED when no job shape fits
+int dt_line_geometry(int n, bool window, int *G, size_t *lds);
+// resolves the allocation-free entry point's scratch array on the current device, once per
+// device (dpm_noalloc.hip)
+int noalloc_setup(int device);
+
 // sets the calling thread's geye_last_error() message; returns code
 int set_error(int code, const std::string &msg);
 

@@ -175,6 +175,17 @@ int geye_add_part(geye_ctx *ctx, float *score, const float *M, const int *rootsi
                   const int *partsize, int ax, int ay);
 int geye_nms(geye_ctx *ctx, const float *boxes, int num, int GL, float overlap, float *out,
              int *out_num);
+/* geye_dt on device memory, allocating nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  d_vals, d_M, d_Ix and d_Iy are device arrays of
+ * dims[0] x dims[1] cells (column-major, as geye_dt's host arrays); dims is a host array.
+ * The outputs are geye_dt's, bit for bit, and are complete when the call returns.  The
+ * working memory is a static device array of about 8.4 MB per GPU, held until the process
+ * ends, which limits each dimension to 2048 and the map to 2^20 cells (larger maps return
+ * GEYE_ERR_ARG).  Calls that overlap on one GPU share that array and race; calls on one
+ * context are serialised. */
+int geye_dt_noalloc(geye_ctx *ctx, const float *d_vals, float ax, float bx, float ay, float by,
+                    const int *dims, float *d_M, int *d_Ix, int *d_Iy);
 
 /* ---- serving: several camera streams on one device ---------------------------------------
  * The reference detects one frame at a time (CT/main.cpp:240-300).  A server runs `slots`
