@@ -76,6 +76,16 @@ _SIGS = [
                               C.c_float, C.c_float, C.c_int, C.POINTER(_fp), _ip, _ip]),
     ("geye_detect_dev", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                   C.c_float, C.c_float, C.c_int, C.POINTER(_fp), _ip, _ip]),
+    ("geye_detect_models", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                     C.c_int, C.c_int, C.c_float, C.c_float, C.c_int,
+                                     C.c_void_p, _ip, _ip]),
+    ("geye_detect_models_dev", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                         C.c_int, C.c_int, C.c_float, C.c_float, C.c_int,
+                                         C.c_void_p, _ip, _ip]),
+    ("geye_serve_submit_models", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                           C.c_int, C.c_int, _llp]),
+    ("geye_serve_submit_models_dev", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                               C.c_int, C.c_int, C.c_int, _llp]),
     ("geye_last_scores", C.c_int, [C.c_void_p, _fp, C.c_longlong, _llp]),
     ("geye_accumulated_score", C.c_int, [C.c_void_p, C.c_float, _fp]),
     ("geye_accumulated_score_dev", C.c_int, [C.c_void_p, C.c_float, C.c_void_p]),
@@ -264,6 +274,28 @@ class Server:
             raise BlockingIOError("serving queue full")
         _check(rc)
         return t.value
+
+    def submit_models(self, models, frame: np.ndarray) -> list:
+        """One frame for several models (one pyramid); one ticket per model, in order."""
+        H, W, _ = frame.shape
+        frame = np.ascontiguousarray(frame, dtype=np.uint8)
+        hs = (C.c_void_p * len(models))(*[m.h for m in models])
+        t = (C.c_longlong * len(models))()
+        rc = lib().geye_serve_submit_models(self.h, hs, len(models), frame.ctypes.data, W, H,
+                                            3 * W, t)
+        if rc == -6:
+            raise BlockingIOError("serving queue full")
+        _check(rc)
+        return list(t)
+
+    def submit_models_dev(self, models, d_frame, W: int, H: int) -> list:
+        hs = (C.c_void_p * len(models))(*[m.h for m in models])
+        t = (C.c_longlong * len(models))()
+        rc = lib().geye_serve_submit_models_dev(self.h, hs, len(models), d_frame, W, H, 3 * W, t)
+        if rc == -6:
+            raise BlockingIOError("serving queue full")
+        _check(rc)
+        return list(t)
 
     def wait(self, ticket: int):
         bp = _fp()
@@ -476,6 +508,37 @@ class Context:
     def graphs(self, on: bool) -> None:
         """Replay the per-frame front as a HIP graph (default) or launch kernels one by one."""
         _check(lib().geye_graph_enable(self.h, int(bool(on))))
+
+    def detect_models(self, models, frame: np.ndarray, thresh: float = -0.5,
+                      overlap: float = 0.5, precision: str = "fp32", d_frame=None) -> list:
+        """geye_detect_models: each model's NMS survivors on one frame, one shared pyramid
+        (frame: host (H, W, 3) uint8, or its shape with d_frame a device copy)."""
+        H, W, _ = frame.shape
+        n = len(models)
+        hs = (C.c_void_p * n)(*[m.h for m in models])
+        bps = (C.c_void_p * n)()
+        num = np.zeros(n, np.int32)
+        GL = np.zeros(n, np.int32)
+        if d_frame is None:
+            frame = np.ascontiguousarray(frame, dtype=np.uint8)
+            rc = lib().geye_detect_models(self.h, hs, n, frame.ctypes.data, W, H, 3 * W,
+                                          C.c_float(thresh), C.c_float(overlap),
+                                          PRECISIONS[precision], bps, _p(num, _ip), _p(GL, _ip))
+        else:
+            rc = lib().geye_detect_models_dev(self.h, hs, n, d_frame, W, H, 3 * W,
+                                              C.c_float(thresh), C.c_float(overlap),
+                                              PRECISIONS[precision], bps, _p(num, _ip),
+                                              _p(GL, _ip))
+        _check(rc)
+        out = []
+        for i in range(n):
+            k = int(num[i]) * int(GL[i])
+            bp = C.cast(bps[i], _fp)
+            rows = np.ctypeslib.as_array(bp, shape=(max(k, 1),))[:k].copy() if bps[i] else \
+                np.zeros(0, np.float32)
+            lib().geye_free(bps[i])
+            out.append(rows.reshape(int(num[i]), int(GL[i])))
+        return out
 
     def dev_upload(self, arr: np.ndarray) -> C.c_void_p:
         arr = np.ascontiguousarray(arr)

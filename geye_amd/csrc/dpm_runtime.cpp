@@ -481,14 +481,9 @@ int plan_upload(Plan &P, hipStream_t st) {
   CHK(upload(P.d_qtiles0, P.q_tiles[0], st));
   CHK(upload(P.d_qtiles1, P.q_tiles[1], st));
   CHK(ensure(P.d_qcounter, 256));
-  CHK(ensure(P.d_grid, sizeof(int2) * std::max<long long>(P.grid_total, 1)));
-  long long tmp = 0;
-  for (auto &R : P.rounds) tmp = std::max(tmp, R.tmp_total);
-  CHK(ensure(P.d_img, sizeof(float) * P.img_total));
-  (void)tmp;  // the fused resize keeps the row pass in LDS: no intermediate image
-  CHK(ensure(P.d_hist, sizeof(float) * P.hist_total));
-  CHK(ensure(P.d_norm, sizeof(float) * P.norm_total));
-  CHK(ensure(P.d_feat, sizeof(float) * P.feat_total));
+  // the pyramid's arenas (images, gradients, histograms, features) are allocated by the
+  // first pyramid run: a plan that convolves another plan's pyramid never needs them; the
+  // fused resize keeps the row pass in LDS, so there is no intermediate image
   CHK(ensure(P.d_conv, sizeof(float) * P.conv_total));
   CHK(ensure(P.d_tmpM, sizeof(float) * P.dt_total));
   CHK(ensure(P.d_tmpIy, sizeof(int) * P.dt_total));
@@ -626,6 +621,11 @@ bool dt_windowed(const geye_ctx *c, int precision) {
 
 int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, int hkind) {
   hipStream_t s = c->stream;
+  CHK(ensure(P.d_img, sizeof(float) * P.img_total));
+  CHK(ensure(P.d_grid, sizeof(int2) * std::max<long long>(P.grid_total, 1)));
+  CHK(ensure(P.d_hist, sizeof(float) * P.hist_total));
+  CHK(ensure(P.d_norm, sizeof(float) * P.norm_total));
+  if (!hkind) CHK(ensure(P.d_feat, sizeof(float) * P.feat_total));
   {
     KTimer t(c, "ingest");
     launch_ingest(s, d_bgr, P.W, P.H, ws, dp<float>(P.d_img) + P.img_off[0]);
@@ -663,14 +663,15 @@ int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, int hk
   return GEYE_OK;
 }
 
-int read_count(geye_ctx *c, Plan &P, int *ncand);
+int read_count(geye_ctx *c, Plan &P, int *ncand, int slot = kHostCandidates);
 
 std::mutex g_tune_mu;
 std::map<std::tuple<const Model *, long long, int>, int> g_tuned;  // autotuned MFMA schedules
 
 // MFMA convolution schedule `choice`: 0 = one launch per filter class (4 x 64 tiles), else
 // the work-queue launch over every class with 16 * MT row tiles and `bpc` workgroups per CU.
-int run_mfma(geye_ctx *c, Model &m, Plan &P, int precision, int choice, bool timed) {
+int run_mfma(geye_ctx *c, Model &m, Plan &P, const Plan &pyr, int precision, int choice,
+             bool timed) {
   hipStream_t s = c->stream;
   const void *filt = precision == GEYE_BF16 ? m.d_filt_bf16.p : m.d_filt_f16.p;
   const std::string tag = precision == GEYE_BF16 ? "conv_bf16_" : "conv_fp16_";
@@ -679,7 +680,7 @@ int run_mfma(geye_ctx *c, Model &m, Plan &P, int precision, int choice, bool tim
       if (mc.tile_count == 0) continue;
       KTimer t(c, timed ? tag + std::to_string(mc.h) + "x" + std::to_string(mc.w) : std::string());
       if (launch_conv_mfma(s, precision, mc.nt, dp<MfmaTile>(P.d_mfma_tiles) + mc.tile_begin,
-                           mc.tile_count, mc.h, mc.w, P.d_featb.p, filt,
+                           mc.tile_count, mc.h, mc.w, pyr.d_featb.p, filt,
                            dp<long long>(P.d_map_off), dp<int2>(P.d_map_dims),
                            dp<float>(P.d_conv)) != 0)
         return fail(GEYE_ERR_UNSUPPORTED, "filter class too large for the MFMA kernel's LDS");
@@ -692,7 +693,7 @@ int run_mfma(geye_ctx *c, Model &m, Plan &P, int precision, int choice, bool tim
   launch_fill(s, dp<int>(P.d_qcounter), 1, 0);
   KTimer t(c, timed ? tag + "queue" : std::string());
   if (launch_conv_mfma_queue(s, precision, MT, c->num_cu * bpc, P.q_lds[mi], (const MfmaTile *)qt.p,
-                             (int)P.q_tiles[mi].size(), dp<int>(P.d_qcounter), P.d_featb.p, filt,
+                             (int)P.q_tiles[mi].size(), dp<int>(P.d_qcounter), pyr.d_featb.p, filt,
                              dp<long long>(P.d_map_off), dp<int2>(P.d_map_dims),
                              dp<float>(P.d_conv)) != 0)
     return fail(GEYE_ERR_UNSUPPORTED, "filter classes too large for the work-queue kernel's LDS");
@@ -703,7 +704,7 @@ int run_mfma(geye_ctx *c, Model &m, Plan &P, int precision, int choice, bool tim
 // the frame being processed is already in HBM) and keeps the fastest.  All schedules give
 // the same bits, so the choice only changes speed.  Runs once per plan and precision,
 // before any graph capture.
-int tune_mfma(geye_ctx *c, Model &m, Plan &P, int precision) {
+int tune_mfma(geye_ctx *c, Model &m, Plan &P, const Plan &pyr, int precision) {
   std::vector<int> cands{0};
   for (int mi = 0; mi < 2; mi++)
     for (int bpc = 1; bpc <= 4; bpc++)
@@ -717,10 +718,11 @@ int tune_mfma(geye_ctx *c, Model &m, Plan &P, int precision) {
   int pick = 0;
   P.mfma_tuning[precision].clear();
   for (int cand : cands) {
-    int rc = run_mfma(c, m, P, precision, cand, false);  // warm-up (attributes, caches)
+    int rc = run_mfma(c, m, P, pyr, precision, cand, false);  // warm-up (attributes, caches)
     float ms = 0;
     if (rc == GEYE_OK && hipEventRecord(e0, c->stream) == hipSuccess) {
-      for (int k = 0; k < 3 && rc == GEYE_OK; k++) rc = run_mfma(c, m, P, precision, cand, false);
+      for (int k = 0; k < 3 && rc == GEYE_OK; k++)
+        rc = run_mfma(c, m, P, pyr, precision, cand, false);
       (void)hipEventRecord(e1, c->stream);
       (void)hipEventSynchronize(e1);
       (void)hipEventElapsedTime(&ms, e0, e1);
@@ -746,15 +748,20 @@ int tune_mfma(geye_ctx *c, Model &m, Plan &P, int precision) {
 }
 
 // Launches the convolution, DT, scoring and candidate compaction; with ncand non-null also
-// waits for the candidate count.
-int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int *ncand) {
+// waits for the candidate count.  pyr (default P) is the plan whose pyramid holds the
+// features: another model's plan over the same frame when the pyramid is shared (its levels
+// are a prefix of pyr's, at the same arena offsets).  slot is the mapped host slot that
+// receives the candidate count.
+int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int *ncand,
+               const Plan *pyr = nullptr, int slot = kHostCandidates) {
   hipStream_t s = c->stream;
+  const Plan &F = pyr ? *pyr : P;
   if (precision == GEYE_FP32) {
     for (auto &cc : P.conv_classes) {
       if (cc.tile_count == 0) continue;
       KTimer t(c, "conv_f32_" + std::to_string(cc.h) + "x" + std::to_string(cc.w));
       launch_conv_f32(s, cc.nf, dp<ConvTile>(P.d_conv_tiles) + cc.tile_begin, cc.tile_count,
-                      cc.h, cc.w, dp<float>(P.d_feat), dp<float>(m.d_filters),
+                      cc.h, cc.w, (const float *)F.d_feat.p, dp<float>(m.d_filters),
                       dp<long long>(P.d_map_off), dp<int2>(P.d_map_dims), dp<float>(P.d_conv));
     }
   } else if (precision == GEYE_BF16 || precision == GEYE_FP16) {
@@ -767,12 +774,12 @@ int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int 
         if (it != g_tuned.end()) P.mfma_choice[precision] = it->second;
       }
       if (P.mfma_choice[precision] < 0) {
-        CHK(tune_mfma(c, m, P, precision));
+        CHK(tune_mfma(c, m, P, F, precision));
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tuned[key] = P.mfma_choice[precision];
       }
     }
-    CHK(run_mfma(c, m, P, precision, P.mfma_choice[precision], true));
+    CHK(run_mfma(c, m, P, F, precision, P.mfma_choice[precision], true));
   } else {
     return fail(GEYE_ERR_ARG, "unknown precision");
   }
@@ -807,20 +814,20 @@ int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int 
     {
       KTimer t(c, "compact");
       launch_compact(s, dp<int>(P.d_flags), P.score_total, dp<int>(P.d_blocksum),
-                     dp<int>(P.d_count), dp<int>(P.d_cand), c->d_hcount + kHostCandidates);
+                     dp<int>(P.d_count), dp<int>(P.d_cand), c->d_hcount + slot);
     }
   }
   HIPCHK(hipGetLastError());
-  return ncand ? read_count(c, P, ncand) : GEYE_OK;
+  return ncand ? read_count(c, P, ncand, slot) : GEYE_OK;
 }
 
 // number of candidates the last scoring pass left in P.d_count (waits for it)
-int read_count(geye_ctx *c, Plan &P, int *ncand) {
+int read_count(geye_ctx *c, Plan &P, int *ncand, int slot) {
   *ncand = 0;
   if (P.score_total <= 0) return GEYE_OK;
   // the compaction's last workgroup wrote the count into the context's mapped host slot
   HIPCHK(hipStreamSynchronize(c->stream));
-  *ncand = ((volatile int *)c->h_count)[kHostCandidates];
+  *ncand = ((volatile int *)c->h_count)[slot];
   return GEYE_OK;
 }
 
@@ -1099,6 +1106,88 @@ int detect_common(geye_ctx *c, const geye_model *model, const unsigned char *bgr
   *GL = P->GL;
   flush_stats(c);
   return *boxes ? GEYE_OK : fail(GEYE_ERR_NOMEM, "malloc");
+}
+
+// Levels of B are a prefix of A's at the same feature arena offsets (the same frame size,
+// sbin and interval, and B's pyramid no deeper): B's convolution can read A's features.
+bool pyramid_prefix(const Plan &A, const Plan &B) {
+  if (A.W != B.W || A.H != B.H || A.sbin != B.sbin || A.interval != B.interval || B.LEN > A.LEN)
+    return false;
+  for (int l = 0; l < B.LEN; l++) {
+    const Level &a = A.lv[l], &b = B.lv[l];
+    if (a.img_h != b.img_h || a.img_w != b.img_w || a.sbin != b.sbin || a.F0 != b.F0 ||
+        a.F1 != b.F1 || a.scale != b.scale || A.feat_off[l] != B.feat_off[l])
+      return false;
+  }
+  return true;
+}
+
+// mapped host slot of model i's candidate count in a several-model detection
+constexpr int kMaxModels = 8, kHostModelCounts = 8;
+int count_slot(int i) { return i == 0 ? kHostCandidates : kHostModelCounts + i; }
+
+// Several models on one frame: the deepest model's plan builds the pyramid once and every
+// model whose levels are a prefix of it convolves those features; each model then scores,
+// runs its NMS and assembles its rows exactly as a detection of it alone does.
+int detect_models(geye_ctx *c, const geye_model *const *models, int n, const unsigned char *bgr,
+                  bool dev, int W, int H, int ws, float thresh, float overlap, int precision,
+                  float **boxes, int *num, int *GL) {
+  if (!c || !models || n < 1 || n > kMaxModels || !boxes || !num || !GL)
+    return fail(GEYE_ERR_ARG, "detect_models: null argument or model count outside 1..8");
+  for (int i = 0; i < n; i++) {
+    if (!models[i]) return fail(GEYE_ERR_ARG, "detect_models: null model");
+    boxes[i] = nullptr;
+    num[i] = GL[i] = 0;
+  }
+  if (precision != GEYE_FP32 && precision != GEYE_BF16 && precision != GEYE_FP16)
+    return fail(GEYE_ERR_ARG, "unknown precision");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(hipSetDevice(c->device));
+  (void)hipGetLastError();
+  std::vector<Plan *> P(n, nullptr);
+  for (int i = 0; i < n; i++)
+    CHK(get_plan(c, const_cast<Model &>(models[i]->m), W, H, &P[i], -1));
+  int owner = 0;
+  for (int i = 1; i < n; i++)
+    if (P[i]->LEN > P[owner]->LEN) owner = i;
+  const unsigned char *d_bgr = bgr;
+  if (!dev) CHK(frame_in(c, *P[owner], bgr, W, H, ws, &d_bgr));
+  const int hkind = precision == GEYE_FP32 ? 0 : precision;
+  CHK(run_pyramid(c, *P[owner], d_bgr, ws, hkind));
+  for (int i = 0; i < n; i++) {
+    Model &m = const_cast<Model &>(models[i]->m);
+    const bool shared = i == owner || pyramid_prefix(*P[owner], *P[i]);
+    if (!shared) CHK(run_pyramid(c, *P[i], d_bgr, ws, hkind));
+    CHK(run_scores(c, m, *P[i], thresh, precision, nullptr, shared ? P[owner] : nullptr,
+                   count_slot(i)));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int rc = GEYE_OK;
+  for (int i = 0; i < n && rc == GEYE_OK; i++) {
+    Model &m = const_cast<Model &>(models[i]->m);
+    const int ncand = P[i]->score_total > 0 ? ((volatile int *)c->h_count)[count_slot(i)] : 0;
+    int nk = 0;
+    std::vector<float> rows;
+    rc = run_nms(c, m, *P[i], ncand, overlap, &nk);
+    if (rc == GEYE_OK) rc = run_rows(c, m, *P[i], dp<int>(P[i]->n_kept), nk, dt_windowed(c, precision), rows);
+    if (rc == GEYE_OK) {
+      boxes[i] = to_malloc(rows);
+      num[i] = nk;
+      GL[i] = P[i]->GL;
+      if (!boxes[i]) rc = fail(GEYE_ERR_NOMEM, "malloc");
+    }
+  }
+  if (rc != GEYE_OK) {
+    for (int i = 0; i < n; i++) {
+      std::free(boxes[i]);
+      boxes[i] = nullptr;
+      num[i] = GL[i] = 0;
+    }
+    return rc;
+  }
+  c->last_plan = P[n - 1];
+  flush_stats(c);
+  return GEYE_OK;
 }
 
 // single-level plan pieces used by the stage entry points
@@ -1414,6 +1503,22 @@ int geye_detect_dev(geye_ctx *c, const geye_model *model, const void *d_bgr, int
                        thresh, overlap, precision, true, boxes, num, GL);
 }
 
+int geye_detect_models(geye_ctx *c, const geye_model *const *models, int nmodels,
+                       const unsigned char *bgr, int width, int height, int widthStep,
+                       float thresh, float overlap, int precision, float **boxes, int *num,
+                       int *GL) {
+  return detect_models(c, models, nmodels, bgr, false, width, height, widthStep, thresh, overlap,
+                       precision, boxes, num, GL);
+}
+
+int geye_detect_models_dev(geye_ctx *c, const geye_model *const *models, int nmodels,
+                           const void *d_bgr, int width, int height, int widthStep, float thresh,
+                           float overlap, int precision, float **boxes, int *num, int *GL) {
+  if (!d_bgr) return fail(GEYE_ERR_ARG, "null frame");
+  return detect_models(c, models, nmodels, (const unsigned char *)d_bgr, true, width, height,
+                       widthStep, thresh, overlap, precision, boxes, num, GL);
+}
+
 int geye_last_scores(geye_ctx *c, float *out, long long cap, long long *total) {
   if (!c || !total) return fail(GEYE_ERR_ARG, "null argument");
   std::lock_guard<std::mutex> lk(c->mu);
@@ -1565,6 +1670,7 @@ int geye_get_boxes_features(geye_ctx *c, const geye_model *model, int width, int
   auto lvl_n = [&](int l) { return 31LL * std::max(0, P->lv[l].F0) * std::max(0, P->lv[l].F1); };
   for (int l = 0; l < P->LEN; l++) total += lvl_n(l);
   if (nfeat != total) return fail(GEYE_ERR_ARG, "feature count does not match the level plan");
+  CHK(ensure(P->d_feat, sizeof(float) * std::max<long long>(P->feat_total, 1)));
   long long off = 0;
   for (int l = 0; l < P->LEN; l++) {
     const long long n = lvl_n(l);

@@ -35,8 +35,8 @@ double now_ms() {
 }
 
 struct Request {
-  long long ticket = 0;
-  const geye_model *model = nullptr;
+  std::vector<long long> tickets;          // one per model
+  std::vector<const geye_model *> models;  // several share the frame's pyramid
   std::vector<unsigned char> frame;        // host copy, or empty for a device frame
   const void *dev = nullptr;               // device-resident frame (caller-owned)
   int W = 0, H = 0, ws = 0;
@@ -128,29 +128,37 @@ void worker(geye_server *S, int s) {
     // the oldest request, once this worker can hold its working memory (arrival order is
     // kept: the head waits for memory rather than being overtaken)
     auto pick = S->queue.end();
-    long long need = 0;
-    bool here = false;
+    std::vector<long long> need;
+    std::vector<bool> here;
     while (!S->stop) {
       pick = S->queue.end();
       if (!S->queue.empty()) {
         auto it = S->queue.begin();
-        const Model *m = &it->model->m;
         const long long key = plan_key(it->W, it->H, -1);
-        need = need_bytes(S, s, m, it->W, it->H, key, &here);
-        if (!here && S->cfg.hbm_budget > 0 && need > S->cfg.hbm_budget) {
+        const size_t nm = it->models.size();
+        need.assign(nm, 0);
+        here.assign(nm, false);
+        long long total = 0;
+        for (size_t k = 0; k < nm; k++) {
+          bool h = false;
+          need[k] = need_bytes(S, s, &it->models[k]->m, it->W, it->H, key, &h);
+          here[k] = h;
+          if (!h) total += need[k];
+        }
+        if (S->cfg.hbm_budget > 0 && total > S->cfg.hbm_budget) {
           // can never fit: answer it now instead of blocking the queue
           Result res;
           res.rc = GEYE_ERR_NOMEM;
           res.err = "frame working memory exceeds the server's HBM budget";
           res.t_submit = it->t_submit;
           res.t_start = res.t_done = now_ms();
-          const long long t = it->ticket;
+          const std::vector<long long> tickets = it->tickets;
           S->queue.erase(it);
-          S->done[t] = std::move(res);
+          for (long long t : tickets) S->done[t] = res;
           S->cv_done.notify_all();
           continue;
         }
-        if (here || make_room(S, need)) pick = it;
+        if (total == 0 || make_room(S, total)) pick = it;
       }
       if (pick != S->queue.end()) break;
       S->cv_work.wait(lk);
@@ -160,51 +168,93 @@ void worker(geye_server *S, int s) {
     S->queue.erase(pick);
     S->busy[s] = true;
     const long long key = plan_key(req.W, req.H, -1);
-    if (!here) {
-      S->resident.push_back(Resident{s, &req.model->m, key, need, now_ms()});
-      S->held += need;
-      S->peak = std::max(S->peak, S->held);
-    }
+    for (size_t k = 0; k < req.models.size(); k++)
+      if (!here[k]) {
+        S->resident.push_back(Resident{s, &req.models[k]->m, key, need[k], now_ms()});
+        S->held += need[k];
+      }
+    S->peak = std::max(S->peak, S->held);
     lk.unlock();
 
-    Result res;
-    res.t_submit = req.t_submit;
-    res.t_start = now_ms();
-    res.slot = s;
-    const size_t fb = req.frame.size();
+    const int nm = (int)req.models.size();
+    std::vector<Result> res(nm);
+    const double t_start = now_ms();
+    std::vector<float *> rows(nm, nullptr);
+    std::vector<int> num(nm, 0), GL(nm, 0);
+    int rc = GEYE_OK;
     if (req.dev) {
-      res.rc = geye_detect_dev(S->ctx[s], req.model, req.dev, req.W, req.H, req.ws, S->cfg.thresh,
-                               S->cfg.overlap, S->cfg.precision, &res.rows, &res.num, &res.GL);
+      rc = geye_detect_models_dev(S->ctx[s], req.models.data(), nm, req.dev, req.W, req.H, req.ws,
+                                  S->cfg.thresh, S->cfg.overlap, S->cfg.precision, rows.data(),
+                                  num.data(), GL.data());
     } else {
-    if (S->pinned_bytes[s] < fb) {
-      if (S->pinned[s]) (void)hipHostFree(S->pinned[s]);
-      S->pinned[s] = nullptr;
-      S->pinned_bytes[s] = 0;
-      if (hipHostMalloc((void **)&S->pinned[s], fb, hipHostMallocDefault) == hipSuccess)
-        S->pinned_bytes[s] = fb;
+      const size_t fb = req.frame.size();
+      if (S->pinned_bytes[s] < fb) {
+        if (S->pinned[s]) (void)hipHostFree(S->pinned[s]);
+        S->pinned[s] = nullptr;
+        S->pinned_bytes[s] = 0;
+        if (hipHostMalloc((void **)&S->pinned[s], fb, hipHostMallocDefault) == hipSuccess)
+          S->pinned_bytes[s] = fb;
+      }
+      const unsigned char *src = req.frame.data();
+      if (S->pinned[s]) {
+        std::memcpy(S->pinned[s], req.frame.data(), fb);
+        src = S->pinned[s];
+      }
+      rc = geye_detect_models(S->ctx[s], req.models.data(), nm, src, req.W, req.H, 3 * req.W,
+                              S->cfg.thresh, S->cfg.overlap, S->cfg.precision, rows.data(),
+                              num.data(), GL.data());
     }
-    const unsigned char *src = req.frame.data();
-    if (S->pinned[s]) {
-      std::memcpy(S->pinned[s], req.frame.data(), fb);
-      src = S->pinned[s];
+    const std::string err = rc != GEYE_OK ? std::string(geye_last_error()) : std::string();
+    const double t_done = now_ms();
+    for (int k = 0; k < nm; k++) {
+      res[k].rc = rc;
+      res[k].err = err;
+      res[k].rows = rows[k];
+      res[k].num = num[k];
+      res[k].GL = GL[k];
+      res[k].slot = s;
+      res[k].t_submit = req.t_submit;
+      res[k].t_start = t_start;
+      res[k].t_done = t_done;
     }
-    res.rc = geye_detect(S->ctx[s], req.model, src, req.W, req.H, 3 * req.W, S->cfg.thresh,
-                         S->cfg.overlap, S->cfg.precision, &res.rows, &res.num, &res.GL);
-    }
-    if (res.rc != GEYE_OK) res.err = geye_last_error();
-    res.t_done = now_ms();
 
     lk.lock();
     for (auto &r : S->resident)
-      if (r.slot == s && r.model == &req.model->m && r.key == key) r.last_use = res.t_done;
+      if (r.slot == s && r.key == key)
+        for (const geye_model *m : req.models)
+          if (r.model == &m->m) r.last_use = t_done;
     S->busy[s] = false;
-    S->completed++;
-    S->latencies.push_back(res.t_done - res.t_submit);
-    S->t_last = std::max(S->t_last, res.t_done);
-    S->done[req.ticket] = std::move(res);
+    for (int k = 0; k < nm; k++) {
+      S->completed++;
+      S->latencies.push_back(t_done - req.t_submit);
+      S->done[req.tickets[k]] = std::move(res[k]);
+    }
+    S->t_last = std::max(S->t_last, t_done);
     S->cv_done.notify_all();
     S->cv_work.notify_all();
   }
+}
+
+// Queues a request for nm models on one frame; one ticket per model.
+int enqueue(geye_server *S, Request &&r, int nm, long long *tickets) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  if ((int)S->queue.size() >= S->cfg.queue_cap) return GEYE_ERR_BUSY;
+  r.t_submit = now_ms();
+  if (S->t_first < 0) S->t_first = r.t_submit;
+  for (int k = 0; k < nm; k++) {
+    r.tickets.push_back(S->next_ticket++);
+    tickets[k] = r.tickets.back();
+  }
+  S->queue.push_back(std::move(r));
+  S->cv_work.notify_all();
+  return GEYE_OK;
+}
+
+bool models_ok(const geye_model *const *models, int nm) {
+  if (!models || nm < 1 || nm > 8) return false;
+  for (int k = 0; k < nm; k++)
+    if (!models[k]) return false;
+  return true;
 }
 
 }  // namespace
@@ -237,47 +287,45 @@ int geye_serve_create(int device, const geye_serve_config *cfg, geye_server **ou
   return GEYE_OK;
 }
 
-int geye_serve_submit(geye_server *S, const geye_model *model, const unsigned char *bgr,
-                      int width, int height, int widthStep, long long *ticket) {
-  if (!S || !model || !bgr || !ticket || width < 8 || height < 8 || widthStep < 3 * width)
+int geye_serve_submit_models(geye_server *S, const geye_model *const *models, int nmodels,
+                             const unsigned char *bgr, int width, int height, int widthStep,
+                             long long *tickets) {
+  if (!S || !models_ok(models, nmodels) || !bgr || !tickets || width < 8 || height < 8 ||
+      widthStep < 3 * width)
     return GEYE_ERR_ARG;
   Request r;
-  r.model = model;
+  r.models.assign(models, models + nmodels);
   r.W = width;
   r.H = height;
   r.frame.resize((size_t)3 * width * height);
   for (int y = 0; y < height; y++)
     std::memcpy(&r.frame[(size_t)y * 3 * width], bgr + (size_t)y * widthStep, (size_t)3 * width);
-  std::lock_guard<std::mutex> lk(S->mu);
-  if ((int)S->queue.size() >= S->cfg.queue_cap) return GEYE_ERR_BUSY;
-  r.t_submit = now_ms();
-  if (S->t_first < 0) S->t_first = r.t_submit;
-  r.ticket = S->next_ticket++;
-  *ticket = r.ticket;
-  S->queue.push_back(std::move(r));
-  S->cv_work.notify_all();
-  return GEYE_OK;
+  return enqueue(S, std::move(r), nmodels, tickets);
 }
 
-int geye_serve_submit_dev(geye_server *S, const geye_model *model, const void *d_bgr, int width,
-                          int height, int widthStep, long long *ticket) {
-  if (!S || !model || !d_bgr || !ticket || width < 8 || height < 8 || widthStep < 3 * width)
+int geye_serve_submit_models_dev(geye_server *S, const geye_model *const *models, int nmodels,
+                                 const void *d_bgr, int width, int height, int widthStep,
+                                 long long *tickets) {
+  if (!S || !models_ok(models, nmodels) || !d_bgr || !tickets || width < 8 || height < 8 ||
+      widthStep < 3 * width)
     return GEYE_ERR_ARG;
   Request r;
-  r.model = model;
+  r.models.assign(models, models + nmodels);
   r.dev = d_bgr;
   r.W = width;
   r.H = height;
   r.ws = widthStep;
-  std::lock_guard<std::mutex> lk(S->mu);
-  if ((int)S->queue.size() >= S->cfg.queue_cap) return GEYE_ERR_BUSY;
-  r.t_submit = now_ms();
-  if (S->t_first < 0) S->t_first = r.t_submit;
-  r.ticket = S->next_ticket++;
-  *ticket = r.ticket;
-  S->queue.push_back(std::move(r));
-  S->cv_work.notify_all();
-  return GEYE_OK;
+  return enqueue(S, std::move(r), nmodels, tickets);
+}
+
+int geye_serve_submit(geye_server *S, const geye_model *model, const unsigned char *bgr,
+                      int width, int height, int widthStep, long long *ticket) {
+  return geye_serve_submit_models(S, &model, 1, bgr, width, height, widthStep, ticket);
+}
+
+int geye_serve_submit_dev(geye_server *S, const geye_model *model, const void *d_bgr, int width,
+                          int height, int widthStep, long long *ticket) {
+  return geye_serve_submit_models_dev(S, &model, 1, d_bgr, width, height, widthStep, ticket);
 }
 
 int geye_serve_wait(geye_server *S, long long ticket, float **boxes, int *num, int *GL,

@@ -112,6 +112,18 @@ int geye_detect(geye_ctx *ctx, const geye_model *model, const unsigned char *bgr
 int geye_detect_dev(geye_ctx *ctx, const geye_model *model, const void *d_bgr, int width,
                     int height, int widthStep, float thresh, float overlap, int precision,
                     float **boxes, int *num, int *GL);
+/* Several models on one frame (car_detection once per model, CT/detect.cpp:232-276): the
+ * models whose sbin and interval match share one HOG pyramid, built once to the deepest
+ * model's level count, and each model's detections are geye_detect's for it alone, bit for
+ * bit.  boxes, num and GL are arrays of nmodels entries (free each boxes[i] with geye_free);
+ * 1 <= nmodels <= 8.  The _dev form takes a device-resident frame. */
+int geye_detect_models(geye_ctx *ctx, const geye_model *const *models, int nmodels,
+                       const unsigned char *bgr, int width, int height, int widthStep,
+                       float thresh, float overlap, int precision, float **boxes, int *num,
+                       int *GL);
+int geye_detect_models_dev(geye_ctx *ctx, const geye_model *const *models, int nmodels,
+                           const void *d_bgr, int width, int height, int widthStep, float thresh,
+                           float overlap, int precision, float **boxes, int *num, int *GL);
 /* Root+bias-parts score maps of the last geye_get_boxes/geye_detect* call on ctx,
  * concatenated per (valid level, component) in the reference's order, column-major.
  * Writes at most cap floats; returns the total count through *total. */
@@ -213,6 +225,15 @@ int geye_serve_submit(geye_server *srv, const geye_model *model, const unsigned 
  * until its ticket has been waited for. */
 int geye_serve_submit_dev(geye_server *srv, const geye_model *model, const void *d_bgr,
                           int width, int height, int widthStep, long long *ticket);
+/* One frame for several models (geye_detect_models on the worker that takes it: the frame's
+ * pyramid is built once); tickets receives one ticket per model, in order, each waited for
+ * with geye_serve_wait.  1 <= nmodels <= 8. */
+int geye_serve_submit_models(geye_server *srv, const geye_model *const *models, int nmodels,
+                             const unsigned char *bgr, int width, int height, int widthStep,
+                             long long *tickets);
+int geye_serve_submit_models_dev(geye_server *srv, const geye_model *const *models, int nmodels,
+                                 const void *d_bgr, int width, int height, int widthStep,
+                                 long long *tickets);
 int geye_serve_wait(geye_server *srv, long long ticket, float **boxes, int *num, int *GL,
                     double *latency_ms);
 /* Completed requests, latency percentiles (ms, submission to result), throughput since the

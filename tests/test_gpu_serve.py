@@ -99,3 +99,67 @@ def test_queue_cap_reports_busy(ctx):
     assert busy > 0 and len(tickets) >= 2
     for r in res[1:]:
         assert np.array_equal(r.view(np.int32), res[0].view(np.int32))
+
+
+def _other_interval_model():
+    """A small model with interval 5: its pyramid differs from the interval-10 models'."""
+    m = synth.tiny_model(seed=9)
+    m.interval = 5
+    return m
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp32"])
+@pytest.mark.parametrize("W,H", [(640, 480), (1920, 1080)])
+def test_detect_models_equals_each_model_alone(ctx, precision, W, H):
+    """geye_detect_models: car and person share one pyramid (person's levels are a prefix of
+    car's), the interval-5 model builds its own; every model's survivors are its
+    car_detection alone, bit for bit, in either model order."""
+    say("detect models", precision, W, H)
+    models = [ctx.model(synth.synthetic_car()), ctx.model(synth.synthetic_person()),
+              ctx.model(_other_interval_model())]
+    frame = synth.make_frame(W, H, seed=77)
+    alone = [m.car_detection(frame, thresh=-0.5, overlap=0.5, precision=precision) for m in models]
+    for order in ([0, 1, 2], [1, 0, 2], [2, 1, 0]):
+        got = ctx.detect_models([models[i] for i in order], frame, thresh=-0.5, overlap=0.5,
+                                precision=precision)
+        for g, i in zip(got, order):
+            r = alone[i]
+            assert g.shape == r.shape and np.array_equal(g.view(np.int32), r.view(np.int32)), \
+                (order, i)
+    d = ctx.dev_upload(frame)
+    try:
+        got = ctx.detect_models(models, frame, thresh=-0.5, overlap=0.5, precision=precision,
+                                d_frame=d)
+    finally:
+        ctx.dev_free(d)
+    for g, r in zip(got, alone):
+        assert np.array_equal(g.view(np.int32), r.view(np.int32))
+    for m in models:
+        m.free()
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp32"])
+def test_served_model_pairs_identical_alone_and_in_the_mix(ctx, precision):
+    """Frames submitted once for car and person (geye_serve_submit_models: one pyramid per
+    frame), mixed with single-model requests, on the four camera sizes."""
+    say("serve model pairs", precision)
+    car, person = ctx.model(synth.synthetic_car()), ctx.model(synth.synthetic_person())
+    sizes = [(640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
+    frames = [synth.make_frame(w, h, seed=600 + i) for i, (w, h) in enumerate(sizes)]
+    ref = [(car.car_detection(f, -0.5, 0.5, precision),
+            person.car_detection(f, -0.5, 0.5, precision)) for f in frames]
+    with geye_amd.Server(slots=3, precision=precision, queue_cap=64) as srv:
+        pairs = [srv.submit_models([car, person], f) for f in frames]
+        singles = [srv.submit(person, f) for f in frames]
+        got_pairs = [[srv.wait(t)[0] for t in p] for p in pairs]
+        got_singles = [srv.wait(t)[0] for t in singles]
+        st = srv.stats()
+    say(st)
+    assert st["completed"] == 3 * len(frames)
+    for k in range(len(frames)):
+        for g, r in zip(got_pairs[k], ref[k]):
+            assert np.array_equal(g.view(np.int32), r.view(np.int32)), (sizes[k], "pair")
+        assert np.array_equal(got_singles[k].view(np.int32), ref[k][1].view(np.int32)), sizes[k]
+    car.free()
+    person.free()
+
