@@ -9,11 +9,18 @@ back.  Frames are resident in HBM before the timed region starts.
 
 Headline (`value`): the batched BF16-MFMA serving form -- the serving loop with several
 frames on the device at once (closed loop, `slots` requests in flight), the slot count
-being the fastest one whose p99 latency stays under 10 ms.  Beside it: the FP32 plain
-form (one frame at a time, one stream), the roofline of the dominant kernel (algorithmic
-bytes or FLOPs per launch / mean launch time from HIP events on the stream the kernel runs
-on), BF16 vs FP32 detection agreement, the two-model batch (16 x 1280x720, car + person),
-the four-camera Poisson serving mix, and the CPU oracle timed on this host's cores.
+being the fastest one whose p99 latency stays under 10 ms.  Beside it, every form timed the
+same way (HIP graphs on, no per-launch events):
+- the FP32 plain form, one frame at a time on one stream (the ratio's denominator), and the
+  FP32 form served like the headline;
+- the kernel breakdown of each form from a separate profiled pass (events around every
+  launch on the stream the kernels run on), the roofline of the dominant kernel and of the
+  MFMA convolution, and per-stage rooflines;
+- BF16 vs FP32 detection agreement, NMS host replays per leg;
+- the two-model batch (16 x 1280x720, car + person, one pyramid per frame) and the
+  four-camera Poisson serving mix;
+- the CPU oracle on this host's cores: C1 (640x480, the real car model, 3 warm-up + 20 timed
+  frames, p50/p99) and one frame of the headline workload.
 Prints ONE JSON line.
 """
 from __future__ import annotations
@@ -115,10 +122,52 @@ def stage_rooflines(plan, model, level_valid, W, H, per_frame: dict, precision: 
     return out
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def load_car_model():
+    """The reference's real car model (CT/car_*.csv) from the repository's data fixture."""
+    from geye_amd.synth import DPMModel
+
+    z = np.load(os.path.join(ROOT, "tests", "golden", "car_model.npz"), allow_pickle=False)
+    return DPMModel(
+        numcomponent=int(z["numcomponent"]), sbin=int(z["sbin"]), interval=int(z["interval"]),
+        max_Y=int(z["max_Y"]), max_X=int(z["max_X"]), ridx=z["ridx"], oidx=z["oidx"],
+        offw=z["offw"], rsize=z["rsize"], numpart=z["numpart"], pidx=z["pidx"], didx=z["didx"],
+        psize=z["psize"], deff=z["deff"], anchor=z["anchor"], root_size=z["root_size"],
+        rootfilters=[z[f"root{i}"] for i in range(int(z["NoR"]))], part_size=z["part_size"],
+        partfilters=[z[f"part{i}"] for i in range(int(z["NoP"]))], name="car")
+
+
 def cpu_baseline(model, frame, threads: int) -> dict:
-    """The oracle (CPU restatement of the reference path) on one frame of the workload."""
+    """The oracle (CPU restatement of the reference path, threaded as the reference: a thread
+    per pyramid level for HOG, a thread per filter for the convolution, DT / boxes / NMS on one
+    thread) on this host: C1 (640x480 rectangle-noise frame, the real car model, thresh -0.5,
+    NMS 0.5; 3 warm-up + 20 timed frames) and one frame of the headline workload."""
+    from geye_amd import synth
     from oracle import oracle as O
 
+    car = load_car_model()
+    c1_frame = synth.make_frame(640, 480, seed=1)
+    for _ in range(3):
+        O.nms(O.detect(car, c1_frame, -0.5, 0, threads), 0.5)
+    lat = []
+    for _ in range(20):
+        t = time.perf_counter()
+        kept1 = O.nms(O.detect(car, c1_frame, -0.5, 0, threads), 0.5)
+        lat.append((time.perf_counter() - t) * 1e3)
+    lat = np.array(lat)
+    c1 = {"workload": "C1: 640x480 rectangle-noise frame (seed 1), real car model, thresh -0.5, "
+                      "NMS 0.5", "frames": 20, "warmup": 3, "fps": float(1e3 / lat.mean()),
+          "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+          "boxes": int(len(kept1))}
     t0 = time.perf_counter()
     boxes = O.detect(model, frame, -0.5, 0, threads)
     t1 = time.perf_counter()
@@ -128,7 +177,9 @@ def cpu_baseline(model, frame, threads: int) -> dict:
             "sample": f"1 frame of the workload ({W}x{H}, synthetic car): detect {t1 - t0:.2f} s "
                       f"({threads} threads: thread per level HOG, thread per filter conv) + "
                       f"NMS {t2 - t1:.2f} s ({len(boxes)} candidates -> {len(kept)} boxes)",
-            "n_candidates": int(len(boxes)), "n_boxes": int(len(kept))}
+            "n_candidates": int(len(boxes)), "n_boxes": int(len(kept)),
+            "host": {"nproc": os.cpu_count(), "threads_used": threads, "cpu_model": cpu_model()},
+            "c1": c1}
 
 
 def iou_agreement(ref: np.ndarray, got: np.ndarray, thr: float = 0.99) -> dict:
@@ -164,36 +215,43 @@ def serve_closed_loop(geye_amd, gm, d_frame, precision, slots, steps, warmup):
         for t in inflight:
             lat.append(srv.wait(t)[1])
         total = time.perf_counter() - t0
+        fallbacks = srv.nms_fallbacks()
     lat = np.array(lat)
     return {"slots": slots, "fps": steps / total, "ms_per_step": total / steps * 1e3,
-            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "nms_host_replays": fallbacks}
 
 
 def two_model_batch(geye_amd, ctx, synth, slots):
-    """16 seeded 1280x720 frames, each detected with the car and the person model (BF16)."""
+    """16 seeded 1280x720 frames, each detected with the car and the person model (BF16),
+    submitted as one request per frame for both models (one pyramid per frame)."""
     car, person = ctx.model(synth.synthetic_car()), ctx.model(synth.synthetic_person())
     frames = [ctx.dev_upload(synth.make_frame(1280, 720, seed=300 + i)) for i in range(16)]
     with geye_amd.Server(slots=slots, precision="bf16", queue_cap=64) as srv:
         for _ in range(2):
-            for t in [srv.submit_dev(m, frames[0], 1280, 720) for m in (car, person)
-                      for _ in range(slots)]:
-                srv.wait(t)
+            for ts in [srv.submit_models_dev([car, person], frames[0], 1280, 720)
+                       for _ in range(slots)]:
+                for t in ts:
+                    srv.wait(t)
         t0 = time.perf_counter()
-        tickets = [srv.submit_dev(m, f, 1280, 720) for f in frames for m in (car, person)]
-        lat = [srv.wait(t)[1] for t in tickets]
+        tickets = [srv.submit_models_dev([car, person], f, 1280, 720) for f in frames]
+        lat = [max(srv.wait(a)[1], srv.wait(b)[1]) for a, b in tickets]
         total = time.perf_counter() - t0
+        fallbacks = srv.nms_fallbacks()
     for f in frames:
         ctx.dev_free(f)
     car.free()
     person.free()
-    return {"workload": "16 x 1280x720 frames x (synthetic car + synthetic person), bf16",
+    return {"workload": "16 x 1280x720 frames x (synthetic car + synthetic person), bf16, "
+                        "one shared pyramid per frame",
             "slots": slots, "frames_per_s": 16 / total, "batch_ms": total * 1e3,
-            "max_request_ms": float(max(lat))}
+            "max_request_ms": float(max(lat)), "nms_host_replays": fallbacks}
 
 
 def serving_mix(geye_amd, ctx, synth, slots, seconds: float = 2.0):
     """Four cameras (640x480, 1280x720, 1920x1080, 3840x2160), Poisson arrivals at 30 fps
-    each, every frame detected with the car and the person model (BF16, host frames)."""
+    each, every frame detected with the car and the person model (BF16, host frames, one
+    shared pyramid per frame)."""
     sizes = [(640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
     car, person = ctx.model(synth.synthetic_car()), ctx.model(synth.synthetic_person())
     frames = {sz: synth.make_frame(sz[0], sz[1], seed=500 + i) for i, sz in enumerate(sizes)}
@@ -209,9 +267,10 @@ def serving_mix(geye_amd, ctx, synth, slots, seconds: float = 2.0):
     events.sort()
     with geye_amd.Server(slots=slots, precision="bf16", queue_cap=1024) as srv:
         for _ in range(2):  # working memory of every (model, size) resident before the clock
-            for t in [srv.submit(m, frames[sz]) for sz in sizes for m in (car, person)
-                      for _ in range(slots)]:
-                srv.wait(t)
+            for ts in [srv.submit_models([car, person], frames[sz]) for sz in sizes
+                       for _ in range(slots)]:
+                for t in ts:
+                    srv.wait(t)
         srv.reset_stats()
         pending = []
         t0 = time.perf_counter()
@@ -219,18 +278,26 @@ def serving_mix(geye_amd, ctx, synth, slots, seconds: float = 2.0):
             dt = t - (time.perf_counter() - t0)
             if dt > 0:
                 time.sleep(dt)
-            pending.append((srv.submit(car, frames[sz]), srv.submit(person, frames[sz])))
-        lat = [max(srv.wait(a)[1], srv.wait(b)[1]) for a, b in pending]
+            pending.append((sz, srv.submit_models([car, person], frames[sz])))
+        lat, per_size = [], {}
+        for sz, (a, b) in pending:
+            l_ = max(srv.wait(a)[1], srv.wait(b)[1])
+            lat.append(l_)
+            per_size.setdefault(f"{sz[0]}x{sz[1]}", []).append(l_)
         total = time.perf_counter() - t0
         st = srv.stats()
+        fallbacks = srv.nms_fallbacks()
     car.free()
     person.free()
     lat = np.array(lat)
     return {"workload": "4 Poisson 30 fps cameras (640x480, 1280x720, 1920x1080, 3840x2160), "
-                        "car + person per frame, bf16, frames from host memory",
+                        "car + person per frame (one shared pyramid), bf16, frames from host "
+                        "memory",
             "slots": slots, "frames": int(len(events)), "offered_fps": len(events) / seconds,
             "frames_per_s": len(events) / total, "p50_ms": float(np.percentile(lat, 50)),
-            "p99_ms": float(np.percentile(lat, 99)), "peak_working_bytes": st["peak_bytes"]}
+            "p99_ms": float(np.percentile(lat, 99)),
+            "p99_ms_per_camera": {k: float(np.percentile(v, 99)) for k, v in per_size.items()},
+            "peak_working_bytes": st["peak_bytes"], "nms_host_replays": fallbacks}
 
 
 def probe(precision: str) -> int:
@@ -311,41 +378,61 @@ def main() -> int:
                     and plan["feat_w"][l] >= model.max_X) for l in range(plan["LEN"])]
 
     def single_stream(precision: str, steps: int) -> dict:
+        """One frame at a time on one stream, graphs on, no per-launch events."""
         for _ in range(args.warmup):
             boxes = gm.detect_dev(d_frame, W, H, -0.5, 0.5, precision)
         ctx.synchronize()
-        ctx.profile(True)
+        before = ctx.nms_fallbacks()
         lat = []
         t0 = time.perf_counter()
         for _ in range(steps):
-            s = time.perf_counter()
+            s_ = time.perf_counter()
             boxes = gm.detect_dev(d_frame, W, H, -0.5, 0.5, precision)
-            lat.append((time.perf_counter() - s) * 1e3)
+            lat.append((time.perf_counter() - s_) * 1e3)
         ctx.synchronize()
         total = time.perf_counter() - t0
-        stats = ctx.kernel_stats()
-        ctx.profile(False)
         lat = np.array(lat)
         return {"fps": steps / total, "ms_per_step": total / steps * 1e3,
                 "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                "stats": stats, "boxes": boxes, "steps": steps}
+                "boxes": boxes, "steps": steps, "nms_host_replays": ctx.nms_fallbacks() - before}
 
-    # one frame at a time: the kernel breakdown and the rooflines come from this run
-    single = single_stream(args.precision, 20)
+    def breakdown(precision: str, frames: int = 10) -> dict:
+        """Kernel time per class from a separate profiled pass (events around every launch on
+        the context's stream; launches run one by one)."""
+        ctx.profile(True)
+        for _ in range(frames):
+            gm.detect_dev(d_frame, W, H, -0.5, 0.5, precision)
+        ctx.synchronize()
+        stats = ctx.kernel_stats()
+        ctx.profile(False)
+        return {"stats": stats, "frames": frames}
+
+    def served_best(precision: str) -> tuple:
+        sweep = [serve_closed_loop(geye_amd, gm, d_frame, precision, sl, args.steps, args.warmup)
+                 for sl in ([args.slots] if args.slots else [1, 2, 3, 4])]
+        ok = [r for r in sweep if r["p99_ms"] < 10.0] or sweep
+        return max(ok, key=lambda r: r["fps"]), sweep
+
+    # one frame at a time, every form timed alike; the breakdowns come from separate passes
+    single = single_stream(args.precision, max(args.steps, 20))
     tuning = ctx.mfma_tuning(args.precision) if args.precision != "fp32" else None
-    plain = single_stream("fp32", 20) if args.precision != "fp32" else single
+    plain = single_stream("fp32", max(args.steps, 20)) if args.precision != "fp32" else single
+    ctx.dt_form("window")
+    try:
+        plain_wdt = single_stream("fp32", max(args.steps, 20))
+    finally:
+        ctx.dt_form("auto")
+    bd = breakdown(args.precision)
+    bd32 = breakdown("fp32") if args.precision != "fp32" else bd
 
-    # the batched serving form: the headline
-    sweep = []
-    for slots in ([args.slots] if args.slots else [1, 2, 3, 4]):
-        sweep.append(serve_closed_loop(geye_amd, gm, d_frame, args.precision, slots,
-                                       args.steps, args.warmup))
-    ok = [r for r in sweep if r["p99_ms"] < 10.0] or sweep
-    head = max(ok, key=lambda r: r["fps"])
+    # the batched serving forms: the headline and the FP32 form served the same way
+    head, sweep = served_best(args.precision)
+    head32, sweep32 = served_best("fp32") if args.precision != "fp32" else (head, sweep)
 
     # dominant kernel: the one with the most time per frame in the single-stream run
-    st = single["stats"]
-    per_frame = {k: v[0] / single["steps"] for k, v in st.items()}
+    st = bd["stats"]
+    per_frame = {k: v[0] / bd["frames"] for k, v in st.items()}
+    per_frame32 = {k: v[0] / bd32["frames"] for k, v in bd32["stats"].items()}
     kgroups = {"dt": ["dt_cols", "dt_rows"]}
     dt_ms = sum(st[k][0] for k in kgroups["dt"] if k in st)
     dt_launches = sum(st[k][1] for k in kgroups["dt"] if k in st)
@@ -357,7 +444,7 @@ def main() -> int:
                 part_cells += (int(plan["feat_h"][L]) - h + 1) * (int(plan["feat_w"][L]) - w + 1)
     roof = None
     if dt_launches:
-        launches_per_frame = dt_launches / single["steps"]
+        launches_per_frame = dt_launches / bd["frames"]
         bytes_per_launch = 16 * part_cells / launches_per_frame
         mean_s = dt_ms / dt_launches / 1e3
         tr = dt_traffic()
@@ -411,16 +498,40 @@ def main() -> int:
         "roofline": roof,
         "conv_roofline": conv_roof,
         "fp32_plain": {"fps": plain["fps"], "p50_ms": plain["p50_ms"], "p99_ms": plain["p99_ms"],
-                       "boxes": int(len(plain["boxes"])), "form": "one frame at a time, one stream"},
+                       "boxes": int(len(plain["boxes"])), "nms_host_replays": plain["nms_host_replays"],
+                       "form": "one frame at a time, one stream, HIP graph, divide-and-conquer DT "
+                               "(timed like every other form: no per-launch events)"},
         "speedup_vs_fp32_plain": head["fps"] / plain["fps"],
-        "single_stream": {"fps": single["fps"], "p50_ms": single["p50_ms"], "p99_ms": single["p99_ms"]},
+        "fp32_windowed_dt": {"fps": plain_wdt["fps"], "p50_ms": plain_wdt["p50_ms"],
+                             "form": "the FP32 form with the 16-bit forms' windowed DT "
+                                     "(bit-identical outputs)"},
+        "fp32_served": {"fps": head32["fps"], "slots": head32["slots"], "p50_ms": head32["p50_ms"],
+                        "p99_ms": head32["p99_ms"], "sweep": sweep32},
+        "speedup_vs_fp32_served": head["fps"] / head32["fps"],
+        "single_stream": {"fps": single["fps"], "p50_ms": single["p50_ms"], "p99_ms": single["p99_ms"],
+                          "nms_host_replays": single["nms_host_replays"]},
         "slot_sweep": sweep,
         "mfma_autotune": tuning,
         "detection_agreement_vs_fp32": iou_agreement(plain["boxes"], single["boxes"]),
         "kernel_ms_per_frame": {k: round(v, 4) for k, v in sorted(per_frame.items())},
+        "kernel_ms_per_frame_fp32": {k: round(v, 4) for k, v in sorted(per_frame32.items())},
         "stage_roofline": stage_rooflines(plan, model, level_valid, W, H, per_frame,
                                           args.precision, float(sum(fl.values()))),
     }
+    # C1's input on the GPU forms, for comparison with the CPU leg below
+    car = ctx.model(load_car_model())
+    c1 = synth.make_frame(640, 480, seed=1)
+    c1_gpu = {}
+    for prec in ("fp32", args.precision):
+        for _ in range(args.warmup):
+            car.car_detection(c1, -0.5, 0.5, prec)
+        t0 = time.perf_counter()
+        for _ in range(20):
+            car.car_detection(c1, -0.5, 0.5, prec)
+        c1_gpu[prec] = 20 / (time.perf_counter() - t0)
+    car.free()
+    line["c1_gpu_fps"] = {"workload": "C1 input (640x480, real car model, host frame), one "
+                                      "frame at a time", **c1_gpu}
     if not args.no_extra:
         line["two_model_batch"] = two_model_batch(geye_amd, ctx, synth, head["slots"])
         line["serving_mix"] = serving_mix(geye_amd, ctx, synth, max(head["slots"], 2))

@@ -106,6 +106,7 @@ _SIGS = [
     ("geye_serve_wait", C.c_int, [C.c_void_p, C.c_longlong, C.POINTER(_fp), _ip, _ip, _dp]),
     ("geye_serve_stats", C.c_int, [C.c_void_p, _llp, _dp, _dp, _dp, _llp, _llp]),
     ("geye_serve_reset_stats", C.c_int, [C.c_void_p]),
+    ("geye_serve_nms_fallbacks", C.c_int, [C.c_void_p, _llp]),
     ("geye_plan_bytes", C.c_int, [C.c_void_p, C.c_int, C.c_int, _llp]),
     ("geye_serve_destroy", None, [C.c_void_p]),
     ("geye_free", None, [C.c_void_p]),
@@ -318,6 +319,11 @@ class Server:
 
     def reset_stats(self):
         _check(lib().geye_serve_reset_stats(self.h))
+
+    def nms_fallbacks(self) -> int:
+        n = C.c_longlong()
+        _check(lib().geye_serve_nms_fallbacks(self.h, C.byref(n)))
+        return int(n.value)
 
     def close(self):
         if self.h:

@@ -241,6 +241,9 @@ int geye_serve_wait(geye_server *srv, long long ticket, float **boxes, int *num,
 int geye_serve_stats(geye_server *srv, long long *completed, double *p50_ms, double *p99_ms,
                      double *fps, long long *peak_bytes, long long *evictions);
 int geye_serve_reset_stats(geye_server *srv);
+/* NMS calls of the server's workers replayed on the host (geye_nms_fallbacks), since the
+ * server was created. */
+int geye_serve_nms_fallbacks(geye_server *srv, long long *count);
 /* Device bytes of one frame's working memory for this model and frame size, as the server
  * accounts it against hbm_budget (host-only; no device work). */
 int geye_plan_bytes(const geye_model *model, int width, int height, long long *bytes);
