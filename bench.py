@@ -300,9 +300,12 @@ def serving_mix(geye_amd, ctx, synth, slots, seconds: float = 2.0):
             "peak_working_bytes": st["peak_bytes"], "nms_host_replays": fallbacks}
 
 
-def probe(precision: str) -> int:
-    """Short run for `rocprofv3 --pmc`: two detections of the workload, then a plain copy of
-    256 MiB between device buffers (a kernel of known traffic for the counter scale)."""
+def probe(precision: str, frames: int) -> int:
+    """Short run for rocprofv3 (kernel statistics, or `--pmc` counters): `frames` detections
+    of the workload, then a plain copy of 256 MiB between device buffers (a kernel of known
+    traffic for the counter scale).  For the 16-bit forms one FP32 frame runs first and the
+    MFMA schedule is fixed to one launch per filter class, so the trace holds no autotuning
+    launches."""
     import torch
 
     import geye_amd
@@ -311,14 +314,18 @@ def probe(precision: str) -> int:
     ctx = geye_amd.Context(0)
     gm = ctx.model(synth.synthetic_car())
     d_frame = ctx.dev_upload(synth.make_frame(W, H, seed=2))
-    for _ in range(2):
+    if precision != "fp32":
+        gm.detect_dev(d_frame, W, H, -0.5, 0.5, "fp32")
+        ctx.mfma_schedule(precision, 0)
+    for k in range(frames):
         gm.detect_dev(d_frame, W, H, -0.5, 0.5, precision)
+        print("probe frame", k, flush=True)
     ctx.synchronize()
     a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     b.copy_(a)
     torch.cuda.synchronize()
-    print("probe done: 2 frames +", a.numel() * 4, "byte copy", flush=True)
+    print(f"probe done: {frames} {precision} frames +", a.numel() * 4, "byte copy", flush=True)
     ctx.dev_free(d_frame)
     gm.free()
     ctx.close()
@@ -355,11 +362,12 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true", help="skip the two-model and mix legs")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--probe", action="store_true",
-                    help="counter run: 2 frames of the workload, then a 256 MiB device copy "
-                         "whose counters scale FETCH_SIZE / WRITE_SIZE to bytes")
+                    help="profiler run: --probe-frames frames of the workload, then a 256 MiB "
+                         "device copy whose counters scale FETCH_SIZE / WRITE_SIZE to bytes")
+    ap.add_argument("--probe-frames", type=int, default=2)
     args = ap.parse_args()
     if args.probe:
-        return probe(args.precision)
+        return probe(args.precision, args.probe_frames)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is "
                                    "described in DESIGN.md and not run", "n_gpus": args.gpus}))
