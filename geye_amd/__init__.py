@@ -82,6 +82,12 @@ _SIGS = [
     ("geye_detect_models_dev", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                                          C.c_int, C.c_int, C.c_float, C.c_float, C.c_int,
                                          C.c_void_p, _ip, _ip]),
+    ("geye_detect_batch", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                    C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.c_void_p,
+                                    _ip, _ip]),
+    ("geye_detect_batch_dev", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                        C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int,
+                                        C.c_void_p, _ip, _ip]),
     ("geye_serve_submit_models", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                                            C.c_int, C.c_int, _llp]),
     ("geye_serve_submit_models_dev", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
@@ -544,6 +550,43 @@ class Context:
                 np.zeros(0, np.float32)
             lib().geye_free(bps[i])
             out.append(rows.reshape(int(num[i]), int(GL[i])))
+        return out
+
+    def detect_batch(self, models, frames, thresh: float = -0.5, overlap: float = 0.5,
+                     precision: str = "fp32", d_frames=None) -> list:
+        """geye_detect_batch: frames (list of (H, W, 3) uint8 host frames of one size, or their
+        device copies in d_frames with frames giving the shape) through every model; returns
+        result[frame][model], each frame's NMS survivors for that model."""
+        nf, nm = len(frames), len(models)
+        H, W, _ = frames[0].shape
+        hs = (C.c_void_p * nm)(*[m.h for m in models])
+        bps = (C.c_void_p * (nf * nm))()
+        num = np.zeros(nf * nm, np.int32)
+        GL = np.zeros(nf * nm, np.int32)
+        if d_frames is None:
+            keep = [np.ascontiguousarray(f, dtype=np.uint8) for f in frames]
+            fp = (C.c_void_p * nf)(*[f.ctypes.data for f in keep])
+            rc = lib().geye_detect_batch(self.h, hs, nm, fp, nf, W, H, 3 * W, C.c_float(thresh),
+                                         C.c_float(overlap), PRECISIONS[precision], bps,
+                                         _p(num, _ip), _p(GL, _ip))
+        else:
+            fp = (C.c_void_p * nf)(*[C.cast(d, C.c_void_p).value for d in d_frames])
+            rc = lib().geye_detect_batch_dev(self.h, hs, nm, fp, nf, W, H, 3 * W,
+                                             C.c_float(thresh), C.c_float(overlap),
+                                             PRECISIONS[precision], bps, _p(num, _ip),
+                                             _p(GL, _ip))
+        _check(rc)
+        out = []
+        for f in range(nf):
+            row = []
+            for i in range(nm):
+                k = f * nm + i
+                cnt = int(num[k]) * int(GL[k])
+                rows = (np.ctypeslib.as_array(C.cast(bps[k], _fp), shape=(max(cnt, 1),))[:cnt].copy()
+                        if bps[k] else np.zeros(0, np.float32))
+                lib().geye_free(bps[k])
+                row.append(rows.reshape(int(num[k]), int(GL[k])))
+            out.append(row)
         return out
 
     def dev_upload(self, arr: np.ndarray) -> C.c_void_p:

@@ -196,6 +196,13 @@ struct DeviceBuffer {
 struct Plan {
   int W = 0, H = 0, ws = 0;
   int max_scale = 0, LEN = 0, interval = 0, sbin = 0;
+  // A batch plan covers nb frames: every descriptor of a one-frame plan repeated per frame,
+  // frame f's arenas at f times the one-frame sizes below (its features at f * feat_stride /
+  // featb_stride, which may be a deeper model's one-frame size when the pyramid is shared).
+  int nb = 1;
+  long long img_one = 0, grid_one = 0, hist_one = 0, norm_one = 0, conv_one = 0, dt_one = 0,
+            score_one = 0, feat_stride = 0, featb_stride = 0;
+  int nmaps_one = 0, nparts_one = 0;
   std::vector<Level> lv;
   // images: index -> (h, w, arena offset)
   std::vector<int> img_h, img_w;
@@ -279,6 +286,12 @@ struct Plan {
       n_total, n_vals, n_ord, n_rank;
   long long nms_fallbacks = 0;
   std::map<long long, hipGraphExec_t> graphs;  // front of the frame, per (precision, ws, thresh)
+  // batch plans: per frame's NMS job its small buffers (counters, flags, totals), window
+  // bitmask, compaction counts and sort storage, two events, and mapped host slots (4 per job,
+  // then the nb + 1 frame offsets of the candidate list)
+  std::vector<DeviceBuffer> b_small, b_mask, b_blocksum, b_sort;
+  std::vector<hipEvent_t> b_ev;
+  int *b_host = nullptr, *b_dev = nullptr;
 };
 
 struct Model {
@@ -330,6 +343,7 @@ struct geye_ctx {
   geye::Plan *last_plan = nullptr;
   bool graphs = true;                 // replay the per-frame front as a HIP graph
   int dt_form = GEYE_DT_AUTO;         // geye_dt_form
+  hipStream_t job_streams[4] = {};    // streams of a batch's concurrent NMS jobs (created on use)
   int num_cu = 256;
   int *h_count = nullptr;             // pinned host slots for counts (coherent, mapped: the
   int *d_hcount = nullptr;            // compaction kernels write them directly; see kHost*)

@@ -5,7 +5,9 @@
 
 namespace geye {
 
-void launch_ingest(hipStream_t s, const unsigned char *bgr, int W, int H, int ws, float *out);
+// nframes frames, frame_bytes apart in bgr and out_stride floats apart in out
+void launch_ingest(hipStream_t s, const unsigned char *bgr, int W, int H, int ws, float *out,
+                   int nframes = 1, long long frame_bytes = 0, long long out_stride = 0);
 void launch_resize_rows(hipStream_t s, const ResizeOp *ops, int nops, int blocks,
                         const int2 *tab, const int *si, const float *alpha, const float *src,
                         float *dst);
@@ -40,6 +42,9 @@ int compact_blocks(long long n);
 // host_total (optional): mapped host memory that also receives the count
 void launch_compact(hipStream_t s, const int *flags, long long n, int *blocksum, int *total,
                     int *out, int *host_total = nullptr);
+// out[f] = first candidate of frame f of a batch, out[nf] = *count (out: mapped host memory)
+void launch_frame_offsets(hipStream_t s, const int *cand, const int *count, long long score_one,
+                          int nf, int *out);
 void launch_rows(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,
                  const float *score, const int *cand, const int *sel, int ncand,
                  const int *tmpIx, const int *tmpIy, int half, int GL, int padx, int pady,

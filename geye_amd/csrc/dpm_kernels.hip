@@ -29,12 +29,30 @@ __device__ inline int xcd_local_block(int b, int nb) {
 }
 
 
+// Index of the descriptor whose block range holds block bx: the last of n descriptors (in
+// ascending blocks_begin order) with blocks_begin <= bx.  A binary search, uniform across the
+// workgroup: a batch of frames lays many frames' descriptors end to end.
+template <typename T>
+__device__ inline int find_blocks(const T *__restrict__ d, int n, int bx) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].blocks_begin <= bx) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // ---------------------------------------------------------------------------------------
 // Ingest: BGR interleaved uint8 -> planar float [3][H][W] (row-major).  Plane 0 is B, as in
 // Ipl_to_FLOAT (CT/featurepyramid.cpp.new2:816-850); the conversion is exact.
 // ---------------------------------------------------------------------------------------
-__global__ void k_ingest(const unsigned char *__restrict__ bgr, int W, int H, int ws,
-                         float *__restrict__ out) {
+// Frame blockIdx.y of a batch: its bytes frame_bytes after the previous frame's, its planes
+// out_stride floats after the previous frame's.
+__global__ void k_ingest(const unsigned char *__restrict__ bgr, long long frame_bytes, int W,
+                         int H, int ws, float *__restrict__ out, long long out_stride) {
+  bgr += blockIdx.y * frame_bytes;
+  out += blockIdx.y * out_stride;
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long SQ = (long long)W * H;
   if (i >= SQ) return;
@@ -52,9 +70,7 @@ __global__ void k_ingest(const unsigned char *__restrict__ bgr, int W, int H, in
 // cols pass: out[c][y][dx] = sum_k alpha_k * in[c][y][si_k]
 // ---------------------------------------------------------------------------------------
 __device__ inline int find_op(const ResizeOp *ops, int nops, int b) {
-  int k = 0;
-  while (k + 1 < nops && ops[k + 1].blocks_begin <= b) k++;
-  return k;
+  return find_blocks(ops, nops, b);
 }
 
 __global__ __launch_bounds__(256) void k_resize_rows(const ResizeOp *__restrict__ ops, int nops,
@@ -115,9 +131,7 @@ __global__ __launch_bounds__(256) void k_resize_fused(const FusedResize *__restr
   __shared__ int s_rsi[kResizeTY][MAXTAP], s_csi[MAXTAP][kResizeTX];
   __shared__ float s_ral[kResizeTY][MAXTAP], s_cal[MAXTAP][kResizeTX];
   const int bx = xcd_local_block(blockIdx.x, gridDim.x);
-  int k = 0;
-  while (k + 1 < nops && ops[k + 1].blocks_begin <= bx) k++;
-  const FusedResize op = ops[k];
+  const FusedResize op = ops[find_blocks(ops, nops, bx)];
   const int tile = bx - op.blocks_begin;
   const int tyi = tile / op.tiles_x, txi = tile - tyi * op.tiles_x;
   const int dy0 = tyi * kResizeTY, dx0 = txi * kResizeTX;
@@ -284,9 +298,7 @@ __global__ __launch_bounds__(256) void k_hog_grad(const GradImg *__restrict__ im
   constexpr int TXP = 64, TYP = kGradTY, SX = TXP + 2, SY = TYP + 2;
   __shared__ float s_t[3][SY * SX];
   const int bx = xcd_local_block(blockIdx.x, gridDim.x);
-  int ii = 0;
-  while (ii + 1 < nimg && imgs[ii + 1].blocks_begin <= bx) ii++;
-  const GradImg G = imgs[ii];
+  const GradImg G = imgs[find_blocks(imgs, nimg, bx)];
   const int tile = bx - G.blocks_begin;
   const int tx = tile / G.tiles_y, ty = tile - tx * G.tiles_y;
   const int x0 = tx * TXP, y0 = ty * TYP;
@@ -412,9 +424,7 @@ __global__ __launch_bounds__(64) void k_hog_votes(const HogLevel *__restrict__ l
   __shared__ int2 s_col[kHogVotePad];
   __shared__ float s_acc[18 * R];
   const int bx = xcd_local_block(blockIdx.x, gridDim.x);
-  int li = 0;
-  while (li + 1 < nlv && lv[li + 1].blocks_begin <= bx) li++;
-  const HogLevel L = lv[li];
+  const HogLevel L = lv[find_blocks(lv, nlv, bx)];
   const int tile = bx - L.blocks_begin;
   const int cx = tile / L.tiles_y, cy0 = (tile - cx * L.tiles_y) * R;
   const int lane = threadIdx.x;
@@ -471,9 +481,7 @@ __global__ __launch_bounds__(256) void k_hog_feat(const HogLevel *__restrict__ l
                                                   float *__restrict__ feat_arena,
                                                   unsigned short *__restrict__ featb_arena,
                                                   int hkind) {
-  int li = 0;
-  while (li + 1 < nlv && lv[li + 1].blocks_begin <= (int)blockIdx.x) li++;
-  const HogLevel L = lv[li];
+  const HogLevel L = lv[find_blocks(lv, nlv, (int)blockIdx.x)];
   long long e = (long long)(blockIdx.x - L.blocks_begin) * blockDim.x + threadIdx.x;
   const long long OSQ = (long long)L.F0 * L.F1;
   if (e >= OSQ) return;
@@ -1271,10 +1279,11 @@ __global__ __launch_bounds__(256) void k_rows(const ScoreMap *__restrict__ maps,
 // ---------------------------------------------------------------------------------------
 // Host-side launchers
 // ---------------------------------------------------------------------------------------
-void launch_ingest(hipStream_t s, const unsigned char *bgr, int W, int H, int ws, float *out) {
+void launch_ingest(hipStream_t s, const unsigned char *bgr, int W, int H, int ws, float *out,
+                   int nframes, long long frame_bytes, long long out_stride) {
   long long n = (long long)W * H;
-  hipLaunchKernelGGL(k_ingest, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, bgr, W, H, ws,
-                     out);
+  hipLaunchKernelGGL(k_ingest, dim3((unsigned)((n + 255) / 256), (unsigned)nframes), dim3(256), 0,
+                     s, bgr, frame_bytes, W, H, ws, out, out_stride);
 }
 
 void launch_resize_rows(hipStream_t s, const ResizeOp *ops, int nops, int blocks,
@@ -1372,6 +1381,33 @@ void launch_compact(hipStream_t s, const int *flags, long long n, int *blocksum,
   hipLaunchKernelGGL(k_flag_count, dim3(nb), dim3(256), 0, s, flags, n, blocksum);
   hipLaunchKernelGGL(k_flag_scatter, dim3(nb), dim3(256), 0, s, flags, n, blocksum, total,
                      host_total, out);
+}
+
+// The first candidate of each frame of a batch (candidates ascend by flat score index, frame
+// f's cells from f * score_one on) and the total at [nf]: one thread per frame.
+__global__ void k_frame_offsets(const int *__restrict__ cand, const int *__restrict__ count,
+                                long long score_one, int nf, int *__restrict__ out) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f > nf) return;
+  const int n = *count;
+  if (f == nf) {
+    *(volatile int *)(out + f) = n;
+    return;
+  }
+  const long long key = (long long)f * score_one;
+  int lo = 0, hi = n;  // first position with cand >= key
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if ((long long)cand[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  *(volatile int *)(out + f) = lo;
+}
+
+void launch_frame_offsets(hipStream_t s, const int *cand, const int *count, long long score_one,
+                          int nf, int *out) {
+  hipLaunchKernelGGL(k_frame_offsets, dim3((nf + 1 + 255) / 256), dim3(256), 0, s, cand, count,
+                     score_one, nf, out);
 }
 
 void launch_rows(hipStream_t s, const ScoreMap *maps, int nmaps, const ScorePart *parts,

@@ -655,6 +655,166 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
   return GEYE_OK;
 }
 
+void replicate_plan(const Plan &one, int nb, long long feat_stride, long long featb_stride,
+                    Plan &P) {
+  P = Plan();
+  P.W = one.W;
+  P.H = one.H;
+  P.max_scale = one.max_scale;
+  P.LEN = one.LEN;
+  P.interval = one.interval;
+  P.sbin = one.sbin;
+  P.nb = nb;
+  P.img_one = one.img_total;
+  P.grid_one = one.grid_total;
+  P.hist_one = one.hist_total;
+  P.norm_one = one.norm_total;
+  P.conv_one = one.conv_total;
+  P.dt_one = one.dt_total;
+  P.score_one = one.score_total;
+  P.feat_stride = feat_stride > 0 ? feat_stride : one.feat_total;
+  P.featb_stride = featb_stride > 0 ? featb_stride : one.featb_total;
+  P.nmaps_one = (int)one.smaps.size();
+  P.nparts_one = (int)one.sparts.size();
+  P.lv = one.lv;
+  P.img_h = one.img_h;
+  P.img_w = one.img_w;
+  P.img_off = one.img_off;
+  P.feat_off = one.feat_off;
+  P.img_total = one.img_total * nb;
+  P.grid_total = one.grid_total * nb;
+  P.hist_total = one.hist_total * nb;
+  P.norm_total = one.norm_total * nb;
+  P.feat_total = P.feat_stride * nb;
+  P.featb_total = P.featb_stride * nb;
+  P.conv_total = one.conv_total * nb;
+  P.dt_total = one.dt_total * nb;
+  P.score_total = one.score_total * nb;
+  P.GL = one.GL;
+  // the shared tables (resize taps, DT node tables, accumulated-score columns)
+  P.resize_tab = one.resize_tab;
+  P.resize_si = one.resize_si;
+  P.resize_alpha = one.resize_alpha;
+  P.dt_nodes = one.dt_nodes;
+  P.dt_depths = one.dt_depths;
+  P.dt_len_tab = one.dt_len_tab;
+  P.ascore_tab = one.ascore_tab;
+  P.dt_lds1 = one.dt_lds1;
+  P.dt_lds2 = one.dt_lds2;
+  P.dt_lds1w = one.dt_lds1w;
+  P.dt_lds2w = one.dt_lds2w;
+  // resize rounds, gradients, HOG votes and features
+  P.fused.assign(one.fused.size(), {});
+  P.fused_blocks.assign(one.fused.size(), 0);
+  for (size_t r = 0; r < one.fused.size(); r++) {
+    for (int f = 0; f < nb; f++)
+      for (FusedResize op : one.fused[r]) {
+        op.in_off += f * one.img_total;
+        op.out_off += f * one.img_total;
+        op.blocks_begin += f * one.fused_blocks[r];
+        P.fused[r].push_back(op);
+      }
+    P.fused_blocks[r] = one.fused_blocks[r] * nb;
+  }
+  for (int f = 0; f < nb; f++)
+    for (GradImg g : one.grad) {
+      g.img_off += f * one.img_total;
+      g.grid_off += f * one.grid_total;
+      g.blocks_begin += f * one.grad_blocks;
+      P.grad.push_back(g);
+    }
+  P.grad_blocks = one.grad_blocks * nb;
+  const int nvote = one.hog16_n + one.hog8_n;
+  auto shift_hog = [&](HogLevel h, int f, int blocks) {
+    h.img_off += f * one.img_total;
+    h.grid_off += f * one.grid_total;
+    h.hist_off += f * one.hist_total;
+    h.norm_off += f * one.norm_total;
+    h.feat_off += f * P.feat_stride;
+    h.featb_off += f * P.featb_stride;
+    h.blocks_begin += f * blocks;
+    return h;
+  };
+  for (int f = 0; f < nb; f++)
+    for (int k = 0; k < nvote; k++)
+      P.hog.push_back(shift_hog(one.hog[k], f, one.hog_hist_blocks + one.hog8_blocks));
+  for (int f = 0; f < nb; f++)
+    for (int k = nvote; k < nvote + one.hogf_n; k++)
+      P.hog.push_back(shift_hog(one.hog[k], f, one.hog_feat_blocks));
+  P.hog16_n = nvote * nb;
+  P.hog8_n = 0;
+  P.hogf_n = one.hogf_n * nb;
+  P.hog_hist_blocks = (one.hog_hist_blocks + one.hog8_blocks) * nb;
+  P.hog8_blocks = 0;
+  P.hog_feat_blocks = one.hog_feat_blocks * nb;
+  // convolution: each class's tiles for every frame, contiguous; output slots per frame
+  const int nslots = (int)one.map_off.size();
+  for (int f = 0; f < nb; f++)
+    for (int k = 0; k < nslots; k++) {
+      P.map_off.push_back(one.map_off[k] + f * one.conv_total);
+      P.map_dims.push_back(one.map_dims[k]);
+    }
+  for (const Plan::ConvClass &cc : one.conv_classes) {
+    Plan::ConvClass c2 = cc;
+    c2.tile_begin = (int)P.conv_tiles.size();
+    for (int f = 0; f < nb; f++)
+      for (int k = cc.tile_begin; k < cc.tile_begin + cc.tile_count; k++) {
+        ConvTile t = one.conv_tiles[k];
+        t.feat_off += f * P.feat_stride;
+        t.out_slot += f * nslots;
+        P.conv_tiles.push_back(t);
+      }
+    c2.tile_count = cc.tile_count * nb;
+    P.conv_classes.push_back(c2);
+  }
+  for (const Plan::MfmaClass &mc : one.mfma_classes) {
+    Plan::MfmaClass c2 = mc;
+    c2.tile_begin = (int)P.mfma_tiles.size();
+    for (int f = 0; f < nb; f++)
+      for (int k = mc.tile_begin; k < mc.tile_begin + mc.tile_count; k++) {
+        MfmaTile t = one.mfma_tiles[k];
+        t.featb_off += f * P.featb_stride;
+        t.out_slot += f * nslots;
+        P.mfma_tiles.push_back(t);
+      }
+    c2.tile_count = mc.tile_count * nb;
+    P.mfma_classes.push_back(c2);
+  }
+  // a batch runs one launch per filter class (the schedule the autotuner picks for 1080p)
+  for (int k = 0; k < 3; k++) P.mfma_choice[k] = 0;
+  // DT jobs
+  // (pass 1 reads the convolution arena, pass 2 the first pass's output in the DT arena)
+  auto rep_jobs = [&](const std::vector<DtJob> &src, std::vector<DtJob> &dst, long long src_frame) {
+    for (int f = 0; f < nb; f++)
+      for (DtJob j : src) {
+        j.src_off += f * src_frame;
+        j.dst_off += f * one.dt_total;
+        dst.push_back(j);
+      }
+  };
+  rep_jobs(one.dt_pass1, P.dt_pass1, one.conv_total);
+  rep_jobs(one.dt_pass2, P.dt_pass2, one.dt_total);
+  rep_jobs(one.dt_pass1w, P.dt_pass1w, one.conv_total);
+  rep_jobs(one.dt_pass2w, P.dt_pass2w, one.dt_total);
+  // score maps and parts
+  for (int f = 0; f < nb; f++) {
+    for (ScoreMap m : one.smaps) {
+      m.root_off += f * one.conv_total;
+      m.score_off += f * one.score_total;
+      m.part_first += f * (int)one.sparts.size();
+      m.blocks_begin += f * one.score_blocks;
+      P.smaps.push_back(m);
+    }
+    for (ScorePart q : one.sparts) {
+      q.M_off += f * one.dt_total;
+      q.Ix_off += f * one.dt_total;
+      q.Iy_off += f * one.dt_total;
+      P.sparts.push_back(q);
+    }
+  }
+  P.score_blocks = one.score_blocks * nb;
+}
+
 int dt_line_geometry(int n, bool window, int *G, size_t *lds) {
   const int g = window ? dt_pick_G_window(n) : dt_pick_G(n);
   if (g < 0) return GEYE_ERR_UNSUPPORTED;

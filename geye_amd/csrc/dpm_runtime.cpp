@@ -95,16 +95,19 @@ T *dp(DeviceBuffer &b) {
 struct KTimer {
   geye_ctx *c;
   std::string name;
+  hipStream_t s;
   hipEvent_t a = nullptr, b = nullptr;
-  KTimer(geye_ctx *ctx, std::string n) : c(ctx), name(std::move(n)) {
+  // s: the stream the timed launches go to (the context's by default)
+  KTimer(geye_ctx *ctx, std::string n, hipStream_t st = nullptr)
+      : c(ctx), name(std::move(n)), s(st ? st : ctx->stream) {
     if (!c->profile || name.empty()) return;
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
-    (void)hipEventRecord(a, c->stream);
+    (void)hipEventRecord(a, s);
   }
   ~KTimer() {
     if (!c->profile || name.empty()) return;
-    (void)hipEventRecord(b, c->stream);
+    (void)hipEventRecord(b, s);
     c->pending.push_back({name, {a, b}});
   }
 };
@@ -500,6 +503,12 @@ int plan_upload(Plan &P, hipStream_t st) {
 void plan_release(Plan &P) {
   for (auto &kv : P.graphs) (void)hipGraphExecDestroy(kv.second);
   P.graphs.clear();
+  for (auto *v : {&P.b_small, &P.b_mask, &P.b_blocksum, &P.b_sort})
+    for (DeviceBuffer &b : *v) release(b);
+  for (hipEvent_t e : P.b_ev) (void)hipEventDestroy(e);
+  P.b_ev.clear();
+  if (P.b_host) (void)hipHostFree(P.b_host);
+  P.b_host = P.b_dev = nullptr;
   for (DeviceBuffer *b : {&P.d_resize_ops, &P.d_resize_tab, &P.d_resize_si, &P.d_resize_alpha,
                           &P.d_resize_si4, &P.d_resize_al4,
                           &P.d_hog, &P.d_conv_tiles, &P.d_mfma_tiles, &P.d_map_dims,
@@ -560,6 +569,77 @@ int get_plan(geye_ctx *c, Model &m, int W, int H, Plan **out, int max_scale) {
   *out = P.release();
   std::lock_guard<std::mutex> lk(g_plan_mu);
   g_plans[key] = *out;
+  return GEYE_OK;
+}
+
+// The plan of a batch of nb frames for model m (plan_key with nb and the feature stride's
+// owner depth in its high bits): the one-frame plan's descriptors repeated per frame, plus
+// the resources of nb concurrent NMS jobs.  feat_len > 0 lays each frame's features out as a
+// model with feat_len levels would (the deepest model of a set sharing one pyramid).
+int get_batch_plan(geye_ctx *c, Model &m, int W, int H, int nb, int feat_len, Plan **out) {
+  const long long key = plan_key(W, H, 0) | ((long long)nb << 40) | ((long long)feat_len << 52) |
+                        (1LL << 62);
+  const auto k = std::make_tuple((const geye_ctx *)c, (const Model *)&m, key);
+  {
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto it = g_plans.find(k);
+    if (it != g_plans.end()) {
+      *out = it->second;
+      return GEYE_OK;
+    }
+  }
+  Plan one;
+  std::string err;
+  int rc = build_plan(m, W, H, one, err);
+  if (rc != GEYE_OK) return fail(rc, err);
+  long long fs = 0, fbs = 0;
+  if (feat_len > 0 && feat_len != one.LEN) {
+    // the deeper model's one-frame feature sizes (this model's levels are a prefix of its):
+    // every level with cells, in level order, as the plan lays them out
+    std::vector<Level> lv;
+    std::vector<int> ih, iw, isrc;
+    std::vector<float> isc;
+    level_geometry(feat_len - m.interval, m.sbin, m.interval, W, H, lv, ih, iw, isrc, isc);
+    for (const Level &L : lv)
+      if (L.F0 > 0 && L.F1 > 0) {
+        fs += 31LL * L.F0 * L.F1;
+        fbs += 32LL * L.F0 * L.F1;
+      }
+  }
+  std::unique_ptr<Plan> P(new Plan());
+  replicate_plan(one, nb, fs, fbs, *P);
+  rc = plan_upload(*P, c->stream);
+  const long long cells = std::max<long long>(one.score_total, 1);
+  P->b_small.resize(nb);
+  P->b_mask.resize(nb);
+  P->b_blocksum.resize(nb);
+  P->b_sort.resize(nb);
+  for (int f = 0; f < nb && rc == GEYE_OK; f++) {
+    rc = ensure(P->b_small[f], 64);
+    if (rc == GEYE_OK)
+      rc = ensure(P->b_mask[f], sizeof(unsigned long long) * kNmsWindow * (kNmsWindow / 64));
+    if (rc == GEYE_OK) rc = ensure(P->b_blocksum[f], sizeof(int) * (compact_blocks(cells) + 1));
+    if (rc == GEYE_OK) rc = ensure(P->b_sort[f], nms_sort_temp_bytes((int)cells));
+    for (int e = 0; e < 2 && rc == GEYE_OK; e++) {
+      hipEvent_t ev = nullptr;
+      if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess)
+        rc = fail(GEYE_ERR_HIP, "batch events");
+      else
+        P->b_ev.push_back(ev);
+    }
+  }
+  if (rc == GEYE_OK &&
+      (hipHostMalloc((void **)&P->b_host, sizeof(int) * (4 * nb + nb + 1),
+                     hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+       hipHostGetDevicePointer((void **)&P->b_dev, P->b_host, 0) != hipSuccess))
+    rc = fail(GEYE_ERR_HIP, "batch host slots");
+  if (rc != GEYE_OK) {
+    plan_release(*P);
+    return rc;
+  }
+  *out = P.release();
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  g_plans[k] = *out;
   return GEYE_OK;
 }
 
@@ -628,7 +708,8 @@ int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, int hk
   if (!hkind) CHK(ensure(P.d_feat, sizeof(float) * P.feat_total));
   {
     KTimer t(c, "ingest");
-    launch_ingest(s, d_bgr, P.W, P.H, ws, dp<float>(P.d_img) + P.img_off[0]);
+    launch_ingest(s, d_bgr, P.W, P.H, ws, dp<float>(P.d_img) + P.img_off[0], P.nb,
+                  (long long)ws * P.H, P.img_one);
   }
   int fbase = 0;
   for (size_t r = 0; r < P.fused.size(); r++) {
@@ -878,102 +959,222 @@ int nms_buffers(Plan &P, int n) {
   return GEYE_OK;
 }
 
+// One greedy NMS over a segment of a plan's NMS arrays: candidates [off, off + n) of the box,
+// key and rank arrays (the whole candidate set, or one frame of a batch).  A job has its own
+// stream, events, mapped host slots and small buffers, so several jobs run at once; their
+// rounds are driven together (nms_run), each one round ahead of the host.
+struct NmsJob {
+  Plan *P = nullptr;
+  long long off = 0;
+  int n = 0;
+  hipStream_t s = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  volatile int *h_round[2] = {nullptr, nullptr}, *h_surv = nullptr, *h_tie = nullptr;
+  int *d_round[2] = {nullptr, nullptr}, *d_surv = nullptr, *d_tie = nullptr;
+  int *counters = nullptr, *flags = nullptr, *total = nullptr, *blocksum = nullptr;
+  unsigned long long *mask = nullptr;
+  void *sort_temp = nullptr;
+  size_t sort_bytes = 0;
+  bool use_maps = false;
+  NmsMaps maps{};
+  const int *cand = nullptr;        // candidate -> flat score index, segment base (maps path)
+  long long rank_off = 0, rank_n = 0;  // the score cells of rank_of_flat this job owns
+  // state of the rounds
+  int na = 0, cur = 0, slot = 0, nk = 0;
+  int *alive = nullptr, *alive_next = nullptr;
+  bool done = true, waiting = false;
+  template <typename T>
+  T *seg(DeviceBuffer &b) const {
+    return (T *)b.p + off;
+  }
+  int *na_buf(int k) const { return total + 2 * k; }
+};
+
+// Sorts the job's keys, gathers the boxes in that order and enqueues the first round.
+int nms_job_start(geye_ctx *c, NmsJob &J, float overlap) {
+  Plan &P = *J.P;
+  J.done = J.n <= 0;
+  J.waiting = false;
+  J.nk = 0;
+  if (J.done) return GEYE_OK;
+  hipStream_t s = J.s;
+  {
+    KTimer t(c, "nms_sort", s);
+    if (launch_nms_sort(s, J.sort_temp, J.sort_bytes, J.seg<unsigned int>(P.n_keys),
+                        J.seg<unsigned int>(P.n_keys2), J.seg<int>(P.n_vals),
+                        J.seg<int>(P.n_ord), J.n) != 0)
+      return fail(GEYE_ERR_HIP, "rocprim radix sort failed");
+  }
+  if (J.use_maps) launch_fill(s, J.maps.rank_of_flat + J.rank_off, J.rank_n, -1);
+  J.cur = 0;
+  J.slot = 0;
+  launch_nms_gather(s, J.seg<int>(P.n_ord), J.seg<NmsBox>(P.n_box), J.n, J.seg<NmsBox>(P.n_sbox),
+                    J.seg<int>(P.n_alive0), J.seg<int>(P.n_status), J.seg<int>(P.n_fsup), J.cand,
+                    J.use_maps ? J.maps.rank_of_flat : nullptr, J.counters, J.na_buf(0), J.flags,
+                    J.seg<int>(P.n_keep_orig));
+  J.alive = J.seg<int>(P.n_alive0);
+  J.alive_next = J.seg<int>(P.n_alive1);
+  J.na = J.n;
+  KTimer t(c, "nms_round", s);
+  if (launch_nms_round(s, J.seg<NmsBox>(P.n_sbox), J.alive, J.na_buf(J.cur), J.na, overlap, J.mask,
+                       J.seg<int>(P.n_status), J.seg<int>(P.n_fsup), J.seg<int>(P.n_picks),
+                       J.counters, J.blocksum, J.use_maps ? &J.maps : nullptr) != 0)
+    return fail(GEYE_ERR_HIP, "NMS greedy kernel: dynamic LDS attribute not set");
+  return GEYE_OK;
+}
+
+// Rounds run one ahead of the host: after round k's compaction, round k+1 is enqueued with a
+// bound of round k's boxes after its window (its alive count can only be smaller) before the
+// host waits for round k's survivor count, so the GPU never idles on the read-back.  The
+// kernels read the true counts from device memory and do nothing past them.
+int nms_job_enqueue(geye_ctx *c, NmsJob &J, float overlap) {
+  if (J.done) return GEYE_OK;
+  Plan &P = *J.P;
+  const int rest = J.na - std::min(J.na, kNmsWindow);  // boxes after that round's window
+  if (rest <= 0) {                                      // it held every alive box
+    J.done = true;
+    return GEYE_OK;
+  }
+  launch_nms_compact(J.s, J.alive, J.na_buf(J.cur), J.seg<int>(P.n_fsup), J.blocksum, J.na,
+                     J.na_buf(J.cur ^ 1), J.alive_next, J.d_round[J.slot]);
+  HIPCHK(hipEventRecord(J.ev[J.slot], J.s));
+  std::swap(J.alive, J.alive_next);
+  J.cur ^= 1;
+  KTimer t(c, "nms_round", J.s);
+  if (launch_nms_round(J.s, J.seg<NmsBox>(P.n_sbox), J.alive, J.na_buf(J.cur), rest, overlap,
+                       J.mask, J.seg<int>(P.n_status), J.seg<int>(P.n_fsup),
+                       J.seg<int>(P.n_picks), J.counters, J.blocksum,
+                       J.use_maps ? &J.maps : nullptr) != 0)
+    return fail(GEYE_ERR_HIP, "NMS greedy kernel: dynamic LDS attribute not set");
+  J.waiting = true;
+  return GEYE_OK;
+}
+
+int nms_job_wait(NmsJob &J) {
+  if (!J.waiting) return GEYE_OK;
+  HIPCHK(hipEventSynchronize(J.ev[J.slot]));
+  J.na = *J.h_round[J.slot];  // the exact alive count of the round enqueued last
+  J.slot ^= 1;
+  J.waiting = false;
+  if (J.na == 0) J.done = true;
+  return GEYE_OK;
+}
+
+// Finalisation: the tie check and the survivor compaction are both enqueued (the pick count
+// stays on the device); the tie flag and the count arrive in the job's mapped host slots.
+void nms_job_finish(NmsJob &J) {
+  if (J.n <= 0) return;
+  Plan &P = *J.P;
+  launch_nms_check(J.s, J.seg<NmsBox>(P.n_sbox), J.n, J.seg<int>(P.n_status),
+                   J.seg<int>(P.n_fsup), J.seg<int>(P.n_picks), J.counters, J.flags);
+  launch_nms_keep(J.s, J.seg<NmsBox>(P.n_sbox), J.seg<int>(P.n_picks), J.counters, J.n, J.flags,
+                  J.seg<int>(P.n_keep_orig), J.d_tie);
+  launch_compact(J.s, J.seg<int>(P.n_keep_orig), (long long)J.n, J.blocksum, J.total + 1,
+                 J.seg<int>(P.n_kept), J.d_surv);
+}
+
+// After the job's stream has finished: its survivor count, or, when equal scores decided an
+// outcome, the host replay of the reference's exact quicksort order (its survivors written
+// to the same place).
+int nms_job_collect(geye_ctx *c, NmsJob &J, float overlap) {
+  J.nk = 0;
+  if (J.n <= 0) return GEYE_OK;
+  Plan &P = *J.P;
+  if (*J.h_tie == 0) {
+    J.nk = *J.h_surv;
+    return GEYE_OK;
+  }
+  P.nms_fallbacks++;
+  c->nms_fallbacks++;
+  c->stats["nms_host_fallback"].count++;
+  const size_t N = (size_t)J.n;
+  std::vector<NmsBox> hb(N);
+  HIPCHK(hipMemcpyAsync(hb.data(), J.seg<NmsBox>(P.n_box), sizeof(NmsBox) * N,
+                        hipMemcpyDeviceToHost, J.s));
+  HIPCHK(hipStreamSynchronize(J.s));
+  std::vector<float> rows(N * 7), out(N * 7);
+  for (size_t i = 0; i < N; i++) {
+    float *r = &rows[i * 7];
+    r[0] = hb[i].y1; r[1] = hb[i].x1; r[2] = hb[i].y2; r[3] = hb[i].x2;
+    r[4] = (float)i; r[5] = hb[i].score; r[6] = 0.0f;
+  }
+  const int nk = nms_host(rows.data(), J.n, 7, overlap, out.data());
+  std::vector<int> kept(nk);
+  for (int k = 0; k < nk; k++) kept[k] = (int)out[(size_t)k * 7 + 4];
+  if (nk > 0) {
+    HIPCHK(hipMemcpyAsync(J.seg<int>(P.n_kept), kept.data(), sizeof(int) * nk,
+                          hipMemcpyHostToDevice, J.s));
+    HIPCHK(hipStreamSynchronize(J.s));
+  }
+  J.nk = nk;
+  return GEYE_OK;
+}
+
+// Runs the jobs' NMS together: each job's boxes, keys and values are in place (a prep
+// kernel on its stream); leaves each job's survivors' local candidate positions, ascending,
+// in its segment of P.n_kept and their count in nk.
+int nms_run(geye_ctx *c, std::vector<NmsJob> &jobs, float overlap) {
+  for (NmsJob &J : jobs) CHK(nms_job_start(c, J, overlap));
+  for (;;) {
+    bool any = false;
+    for (NmsJob &J : jobs) {
+      CHK(nms_job_enqueue(c, J, overlap));
+      any = any || J.waiting;
+    }
+    if (!any) break;
+    for (NmsJob &J : jobs) CHK(nms_job_wait(J));
+  }
+  for (NmsJob &J : jobs) nms_job_finish(J);
+  for (NmsJob &J : jobs) HIPCHK(hipStreamSynchronize(J.s));
+  for (NmsJob &J : jobs) CHK(nms_job_collect(c, J, overlap));
+  HIPCHK(hipGetLastError());
+  return GEYE_OK;
+}
+
+// The whole candidate set of a one-frame plan as one job on the context's stream, with the
+// context's events and mapped host slots.
+NmsJob single_job(geye_ctx *c, Plan &P, int n, const NmsMaps *maps, const int *cand) {
+  NmsJob J;
+  J.P = &P;
+  J.off = 0;
+  J.n = n;
+  J.s = c->stream;
+  J.ev[0] = c->ev_count[0];
+  J.ev[1] = c->ev_count[1];
+  volatile int *h = (volatile int *)c->h_count;
+  J.h_round[0] = h;
+  J.h_round[1] = h + 1;
+  J.h_surv = h + kHostSurvivors;
+  J.h_tie = h + kHostTie;
+  J.d_round[0] = c->d_hcount;
+  J.d_round[1] = c->d_hcount + 1;
+  J.d_surv = c->d_hcount + kHostSurvivors;
+  J.d_tie = c->d_hcount + kHostTie;
+  J.counters = dp<int>(P.n_counters);
+  J.flags = dp<int>(P.n_flags);
+  J.total = dp<int>(P.n_total);
+  J.blocksum = dp<int>(P.d_blocksum);
+  J.mask = dp<unsigned long long>(P.n_mask);
+  J.sort_temp = P.n_sort.p;
+  J.sort_bytes = P.n_sort.bytes;
+  J.use_maps = maps != nullptr;
+  if (maps) J.maps = *maps;
+  J.cand = cand;
+  J.rank_off = 0;
+  J.rank_n = P.score_total;
+  return J;
+}
+
 // Greedy NMS over P.n_box / P.n_keys / P.n_vals (filled by a prep kernel, original order);
 // leaves the survivors' original indices, ascending, in P.n_kept.  maps (optional) enables
 // the scatter form of the suppression pass; cand/rank_of_flat go with it.
 int nms_core(geye_ctx *c, Plan &P, int n, float overlap, const NmsMaps *maps, const int *cand,
              int *nkept) {
-  hipStream_t s = c->stream;
-  const size_t N = (size_t)n;
   *nkept = 0;
   if (n <= 0) return GEYE_OK;
-  {
-    KTimer t(c, "nms_sort");
-    if (launch_nms_sort(s, P.n_sort.p, P.n_sort.bytes, dp<unsigned int>(P.n_keys),
-                        dp<unsigned int>(P.n_keys2), dp<int>(P.n_vals), dp<int>(P.n_ord), n) != 0)
-      return fail(GEYE_ERR_HIP, "rocprim radix sort failed");
-  }
-  if (maps) launch_fill(s, maps->rank_of_flat, P.score_total, -1);
-  // alive counts, double-buffered: round k reads na_buf[cur], its compaction writes the other
-  int *na_buf[2] = {dp<int>(P.n_total), dp<int>(P.n_total) + 2};
-  int cur = 0;
-  launch_nms_gather(s, dp<int>(P.n_ord), dp<NmsBox>(P.n_box), n, dp<NmsBox>(P.n_sbox),
-                    dp<int>(P.n_alive0), dp<int>(P.n_status), dp<int>(P.n_fsup), cand,
-                    maps ? maps->rank_of_flat : nullptr, dp<int>(P.n_counters), na_buf[0],
-                    dp<int>(P.n_flags), dp<int>(P.n_keep_orig));
-  int *alive = dp<int>(P.n_alive0), *alive_next = dp<int>(P.n_alive1);
-  int *blocksum = dp<int>(P.d_blocksum);
-  int *d_sel_total = dp<int>(P.n_total) + 1;  // count of the final survivor compaction
-  // Rounds run one ahead of the host: after round k's compaction, round k+1 is enqueued with
-  // a bound of round k's boxes after its window (its alive count can only be smaller) before
-  // the host waits for round k's survivor count, so the GPU never idles on the read-back.
-  // The kernels read the true counts from device memory and do nothing past them.
-  auto round = [&](int bound) {
-    KTimer t(c, "nms_round");
-    return launch_nms_round(s, dp<NmsBox>(P.n_sbox), alive, na_buf[cur], bound, overlap,
-                            dp<unsigned long long>(P.n_mask), dp<int>(P.n_status),
-                            dp<int>(P.n_fsup), dp<int>(P.n_picks), dp<int>(P.n_counters),
-                            blocksum, maps) == 0
-               ? GEYE_OK
-               : fail(GEYE_ERR_HIP, "NMS greedy kernel: dynamic LDS attribute not set");
-  };
-  int na = n;  // exact alive count of the round last enqueued
-  int slot = 0;
-  CHK(round(na));
-  for (;;) {
-    const int rest = na - std::min(na, kNmsWindow);  // boxes after that round's window
-    if (rest <= 0) break;                           // it held every alive box
-    launch_nms_compact(s, alive, na_buf[cur], dp<int>(P.n_fsup), blocksum, na,
-                       na_buf[cur ^ 1], alive_next, c->d_hcount + slot);
-    HIPCHK(hipEventRecord(c->ev_count[slot], s));
-    std::swap(alive, alive_next);
-    cur ^= 1;
-    CHK(round(rest));                               // next round, bounded by rest
-    HIPCHK(hipEventSynchronize(c->ev_count[slot]));
-    na = ((volatile int *)c->h_count)[slot];        // its exact alive count
-    slot ^= 1;
-    if (na == 0) break;
-  }
-  // Finalisation with one wait: the tie check and the survivor compaction are both enqueued
-  // (the pick count stays on the device); the tie flag and the count arrive in the context's
-  // mapped host slots.
-  // A tie flag discards the device result for the host replay below.
-  launch_nms_check(s, dp<NmsBox>(P.n_sbox), n, dp<int>(P.n_status), dp<int>(P.n_fsup),
-                   dp<int>(P.n_picks), dp<int>(P.n_counters), dp<int>(P.n_flags));
-  launch_nms_keep(s, dp<NmsBox>(P.n_sbox), dp<int>(P.n_picks), dp<int>(P.n_counters), n,
-                  dp<int>(P.n_flags), dp<int>(P.n_keep_orig), c->d_hcount + kHostTie);
-  launch_compact(s, dp<int>(P.n_keep_orig), (long long)N, dp<int>(P.d_blocksum), d_sel_total,
-                 dp<int>(P.n_kept), c->d_hcount + kHostSurvivors);
-  HIPCHK(hipStreamSynchronize(s));
-  const int tie = ((volatile int *)c->h_count)[kHostTie];
-  const int nk_dev = ((volatile int *)c->h_count)[kHostSurvivors];
-  int nk = 0;
-  if (tie != 0) {
-    // equal scores decided an outcome: replay the reference's exact quicksort order on the host
-    P.nms_fallbacks++;
-    c->nms_fallbacks++;
-    c->stats["nms_host_fallback"].count++;
-    std::vector<NmsBox> hb(N);
-    HIPCHK(hipMemcpyAsync(hb.data(), P.n_box.p, sizeof(NmsBox) * N, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    std::vector<float> rows(N * 7), out(N * 7);
-    for (size_t i = 0; i < N; i++) {
-      float *r = &rows[i * 7];
-      r[0] = hb[i].y1; r[1] = hb[i].x1; r[2] = hb[i].y2; r[3] = hb[i].x2;
-      r[4] = (float)i; r[5] = hb[i].score; r[6] = 0.0f;
-    }
-    nk = nms_host(rows.data(), n, 7, overlap, out.data());
-    std::vector<int> kept(nk);
-    for (int k = 0; k < nk; k++) kept[k] = (int)out[(size_t)k * 7 + 4];
-    if (nk > 0) {
-      HIPCHK(hipMemcpyAsync(P.n_kept.p, kept.data(), sizeof(int) * nk, hipMemcpyHostToDevice, s));
-      HIPCHK(hipStreamSynchronize(s));
-    }
-  } else {
-    nk = nk_dev;
-  }
-  HIPCHK(hipGetLastError());
-  *nkept = nk;
+  std::vector<NmsJob> jobs{single_job(c, P, n, maps, cand)};
+  CHK(nms_run(c, jobs, overlap));
+  *nkept = jobs[0].nk;
   return GEYE_OK;
 }
 
@@ -1122,6 +1323,27 @@ bool pyramid_prefix(const Plan &A, const Plan &B) {
   return true;
 }
 
+// The level geometry and feature offsets of model m's plan for a W x H frame, nothing else
+// (what pyramid_prefix compares).
+void level_only_plan(const Model &m, int W, int H, Plan &P) {
+  P.W = W;
+  P.H = H;
+  P.sbin = m.sbin;
+  P.interval = m.interval;
+  P.max_scale = compute_max_scale(m, W, H);
+  P.LEN = P.max_scale + m.interval;
+  std::vector<int> isrc;
+  std::vector<float> isc;
+  level_geometry(P.max_scale, m.sbin, m.interval, W, H, P.lv, P.img_h, P.img_w, isrc, isc);
+  P.feat_off.assign(P.LEN, -1);
+  long long fo = 0;
+  for (int l = 0; l < P.LEN; l++)
+    if (P.lv[l].F0 > 0 && P.lv[l].F1 > 0) {
+      P.feat_off[l] = fo;
+      fo += 31LL * P.lv[l].F0 * P.lv[l].F1;
+    }
+}
+
 // mapped host slot of model i's candidate count in a several-model detection
 constexpr int kMaxModels = 8, kHostModelCounts = 8;
 int count_slot(int i) { return i == 0 ? kHostCandidates : kHostModelCounts + i; }
@@ -1186,6 +1408,177 @@ int detect_models(geye_ctx *c, const geye_model *const *models, int n, const uns
     return rc;
   }
   c->last_plan = P[n - 1];
+  flush_stats(c);
+  return GEYE_OK;
+}
+
+// nf frames of one size through nm models: per model a batch plan (its one-frame plan's
+// descriptors repeated per frame), the deepest model's pyramid built once for every frame
+// and read by every model whose levels are a prefix of it, each model's convolution, DT and
+// scoring in one launch sequence for all frames; then every (frame, model) NMS as a job, the
+// jobs running together on the context's job streams.  boxes/num/GL: nf * nm entries, frame
+// major.  Every result equals that frame's detection alone.
+int detect_batch(geye_ctx *c, const geye_model *const *models, int nm,
+                 const unsigned char *const *frames, bool dev, int nf, int W, int H, int ws,
+                 float thresh, float overlap, int precision, float **boxes, int *num, int *GL) {
+  if (!c || !models || nm < 1 || nm > kMaxModels || !frames || nf < 1 || nf > 64 || !boxes ||
+      !num || !GL)
+    return fail(GEYE_ERR_ARG, "detect_batch: null argument, 1..8 models or 1..64 frames");
+  if (ws < 3 * W || W < 8 || H < 8) return fail(GEYE_ERR_ARG, "detect_batch: frame geometry");
+  for (int i = 0; i < nm; i++)
+    if (!models[i]) return fail(GEYE_ERR_ARG, "detect_batch: null model");
+  for (int f = 0; f < nf; f++)
+    if (!frames[f]) return fail(GEYE_ERR_ARG, "detect_batch: null frame");
+  if (precision != GEYE_FP32 && precision != GEYE_BF16 && precision != GEYE_FP16)
+    return fail(GEYE_ERR_ARG, "unknown precision");
+  for (int k = 0; k < nf * nm; k++) {
+    boxes[k] = nullptr;
+    num[k] = GL[k] = 0;
+  }
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(hipSetDevice(c->device));
+  (void)hipGetLastError();
+  for (hipStream_t &js : c->job_streams)
+    if (!js) HIPCHK(hipStreamCreateWithFlags(&js, hipStreamNonBlocking));
+  // the level plans decide which models share the deepest model's pyramid
+  std::vector<Plan> lvl(nm);
+  for (int i = 0; i < nm; i++) level_only_plan(models[i]->m, W, H, lvl[i]);
+  int owner = 0;
+  for (int i = 1; i < nm; i++)
+    if (lvl[i].LEN > lvl[owner].LEN) owner = i;
+  std::vector<bool> shared(nm);
+  std::vector<Plan *> P(nm, nullptr);
+  for (int i = 0; i < nm; i++) {
+    shared[i] = i == owner || pyramid_prefix(lvl[owner], lvl[i]);
+    CHK(get_batch_plan(c, const_cast<Model &>(models[i]->m), W, H, nf,
+                       shared[i] ? lvl[owner].LEN : 0, &P[i]));
+  }
+  // frames into the owner's frame buffer, one after another
+  const size_t fbytes = (size_t)ws * H;
+  CHK(ensure_frame(*P[owner], fbytes * nf));
+  unsigned char *fr = (unsigned char *)P[owner]->d_frame.p;
+  for (int f = 0; f < nf; f++)
+    HIPCHK(hipMemcpyAsync(fr + fbytes * f, frames[f], fbytes,
+                          dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+  const int hkind = precision == GEYE_FP32 ? 0 : precision;
+  CHK(run_pyramid(c, *P[owner], fr, ws, hkind));
+  for (int i = 0; i < nm; i++) {
+    Model &m = const_cast<Model &>(models[i]->m);
+    if (!shared[i]) {
+      CHK(ensure_frame(*P[i], fbytes * nf));
+      HIPCHK(hipMemcpyAsync(P[i]->d_frame.p, fr, fbytes * nf, hipMemcpyDeviceToDevice, c->stream));
+      CHK(run_pyramid(c, *P[i], (const unsigned char *)P[i]->d_frame.p, ws, hkind));
+    }
+    CHK(run_scores(c, m, *P[i], thresh, precision, nullptr, shared[i] ? P[owner] : nullptr,
+                   count_slot(i)));
+    if (P[i]->score_total > 0)
+      launch_frame_offsets(c->stream, dp<int>(P[i]->d_cand), dp<int>(P[i]->d_count),
+                           P[i]->score_one, nf, P[i]->b_dev + 4 * nf);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // every (frame, model) NMS as one job
+  const bool window = dt_windowed(c, precision);
+  std::vector<NmsJob> jobs;
+  std::vector<std::pair<int, int>> who;  // (frame, model) of each job
+  for (int i = 0; i < nm; i++) {
+    Model &m = const_cast<Model &>(models[i]->m);
+    Plan &B = *P[i];
+    const int ncand = B.score_total > 0 ? ((volatile int *)c->h_count)[count_slot(i)] : 0;
+    CHK(nms_buffers(B, std::max(ncand, 1)));
+    CHK(ensure(B.n_rank, sizeof(int) * (size_t)std::max<long long>(B.score_total, 1)));
+    const volatile int *off = (const volatile int *)B.b_host + 4 * nf;
+    for (int f = 0; f < nf; f++) {
+      NmsJob J;
+      J.P = &B;
+      J.off = B.score_total > 0 ? off[f] : 0;
+      J.n = B.score_total > 0 ? off[f + 1] - off[f] : 0;
+      J.s = c->job_streams[jobs.size() % 4];
+      J.ev[0] = B.b_ev[2 * f];
+      J.ev[1] = B.b_ev[2 * f + 1];
+      volatile int *h = (volatile int *)B.b_host + 4 * f;
+      int *d = B.b_dev + 4 * f;
+      J.h_round[0] = h;
+      J.h_round[1] = h + 1;
+      J.h_surv = h + 2;
+      J.h_tie = h + 3;
+      J.d_round[0] = d;
+      J.d_round[1] = d + 1;
+      J.d_surv = d + 2;
+      J.d_tie = d + 3;
+      int *small = dp<int>(B.b_small[f]);
+      J.counters = small;
+      J.flags = small + 4;
+      J.total = small + 8;
+      J.blocksum = dp<int>(B.b_blocksum[f]);
+      J.mask = dp<unsigned long long>(B.b_mask[f]);
+      J.sort_temp = B.b_sort[f].p;
+      J.sort_bytes = B.b_sort[f].bytes;
+      J.use_maps = B.nmaps_one <= kNmsMaxMaps;
+      J.cand = dp<int>(B.d_cand) + J.off;
+      J.maps = NmsMaps{dp<ScoreMap>(B.d_smaps) + (size_t)f * B.nmaps_one, B.nmaps_one,
+                       dp<int>(m.d_rsize), m.padx, m.pady, dp<int>(B.n_rank), J.cand};
+      J.rank_off = (long long)f * B.score_one;
+      J.rank_n = B.score_one;
+      if (J.n > 0) {
+        KTimer t(c, "nms_prep", J.s);
+        launch_nms_prep(J.s, dp<ScoreMap>(B.d_smaps), (int)B.smaps.size(), dp<float>(B.d_score),
+                        J.cand, J.n, m.padx, m.pady, dp<int>(m.d_rsize), J.seg<NmsBox>(B.n_box),
+                        J.seg<unsigned int>(B.n_keys), J.seg<int>(B.n_vals));
+      }
+      jobs.push_back(J);
+      who.push_back({f, i});
+    }
+  }
+  CHK(nms_run(c, jobs, overlap));
+  // rows of every job's survivors, then one copy back per model
+  int rc = GEYE_OK;
+  for (int i = 0; i < nm && rc == GEYE_OK; i++) {
+    Model &m = const_cast<Model &>(models[i]->m);
+    Plan &B = *P[i];
+    long long total = 0;
+    for (size_t j = 0; j < jobs.size(); j++)
+      if (who[j].second == i) total += jobs[j].nk;
+    rc = ensure(B.d_rows, sizeof(float) * (size_t)std::max<long long>(total, 1) * B.GL);
+    std::vector<float> rows((size_t)total * B.GL);
+    long long at = 0;
+    for (size_t j = 0; j < jobs.size() && rc == GEYE_OK; j++) {
+      if (who[j].second != i || jobs[j].nk == 0) continue;
+      NmsJob &J = jobs[j];
+      launch_rows(J.s, dp<ScoreMap>(B.d_smaps), (int)B.smaps.size(), dp<ScorePart>(B.d_sparts),
+                  dp<float>(B.d_score), J.cand, J.seg<int>(B.n_kept), J.nk, dp<int>(B.d_tmpIx),
+                  dp<int>(B.d_tmpIy), window ? 1 : 0, B.GL, m.padx, m.pady, dp<int>(m.d_rsize),
+                  dp<float>(B.d_rows) + at * B.GL);
+      at += J.nk;
+    }
+    for (hipStream_t js : c->job_streams)
+      if (rc == GEYE_OK && hipStreamSynchronize(js) != hipSuccess)
+        rc = fail(GEYE_ERR_HIP, "batch rows");
+    if (rc == GEYE_OK && total > 0 &&
+        hipMemcpy(rows.data(), B.d_rows.p, sizeof(float) * rows.size(), hipMemcpyDeviceToHost) !=
+            hipSuccess)
+      rc = fail(GEYE_ERR_HIP, "batch rows copy");
+    at = 0;
+    for (size_t j = 0; j < jobs.size() && rc == GEYE_OK; j++) {
+      if (who[j].second != i) continue;
+      const int k = who[j].first * nm + i;
+      std::vector<float> r(rows.begin() + at * B.GL, rows.begin() + (at + jobs[j].nk) * B.GL);
+      boxes[k] = to_malloc(r);
+      num[k] = jobs[j].nk;
+      GL[k] = B.GL;
+      if (!boxes[k]) rc = fail(GEYE_ERR_NOMEM, "malloc");
+      at += jobs[j].nk;
+    }
+  }
+  if (rc != GEYE_OK) {
+    for (int k = 0; k < nf * nm; k++) {
+      std::free(boxes[k]);
+      boxes[k] = nullptr;
+      num[k] = GL[k] = 0;
+    }
+    return rc;
+  }
+  c->last_plan = P[nm - 1];
   flush_stats(c);
   return GEYE_OK;
 }
@@ -1365,6 +1758,8 @@ void geye_shutdown(geye_ctx *c) {
   if (c->h_count) (void)hipHostFree(c->h_count);
   for (hipEvent_t e : c->ev_count)
     if (e) (void)hipEventDestroy(e);
+  for (hipStream_t js : c->job_streams)
+    if (js) (void)hipStreamDestroy(js);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1517,6 +1912,22 @@ int geye_detect_models_dev(geye_ctx *c, const geye_model *const *models, int nmo
   if (!d_bgr) return fail(GEYE_ERR_ARG, "null frame");
   return detect_models(c, models, nmodels, (const unsigned char *)d_bgr, true, width, height,
                        widthStep, thresh, overlap, precision, boxes, num, GL);
+}
+
+int geye_detect_batch(geye_ctx *c, const geye_model *const *models, int nmodels,
+                      const unsigned char *const *bgr, int nframes, int width, int height,
+                      int widthStep, float thresh, float overlap, int precision, float **boxes,
+                      int *num, int *GL) {
+  return detect_batch(c, models, nmodels, bgr, false, nframes, width, height, widthStep, thresh,
+                      overlap, precision, boxes, num, GL);
+}
+
+int geye_detect_batch_dev(geye_ctx *c, const geye_model *const *models, int nmodels,
+                          const void *const *d_bgr, int nframes, int width, int height,
+                          int widthStep, float thresh, float overlap, int precision,
+                          float **boxes, int *num, int *GL) {
+  return detect_batch(c, models, nmodels, (const unsigned char *const *)d_bgr, true, nframes,
+                      width, height, widthStep, thresh, overlap, precision, boxes, num, GL);
 }
 
 int geye_last_scores(geye_ctx *c, float *out, long long cap, long long *total) {

@@ -12,6 +12,11 @@ void level_geometry(int max_scale, int sbin, int interval, int W, int H, std::ve
                     std::vector<int> &img_h, std::vector<int> &img_w,
                     std::vector<int> &img_src, std::vector<float> &img_scale);
 int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_override = -1);
+// The plan of nb frames from a one-frame plan: every descriptor repeated per frame with the
+// frame's arena offsets; the features of frame f at f * feat_stride (FP32) / featb_stride
+// (16-bit) elements (0: one's own sizes).
+void replicate_plan(const Plan &one, int nb, long long feat_stride, long long featb_stride,
+                    Plan &P);
 
 void resize_tables(int sheight, int dheight, std::vector<int2> &tab, std::vector<int> &si,
                    std::vector<float> &alpha);

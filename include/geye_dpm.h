@@ -124,6 +124,21 @@ int geye_detect_models(geye_ctx *ctx, const geye_model *const *models, int nmode
 int geye_detect_models_dev(geye_ctx *ctx, const geye_model *const *models, int nmodels,
                            const void *d_bgr, int width, int height, int widthStep, float thresh,
                            float overlap, int precision, float **boxes, int *num, int *GL);
+/* A batch of nframes frames of one size through nmodels models (geye_detect_models for each
+ * frame, batched): every stage before the NMS runs once for all frames -- each launch covers
+ * every frame's tiles, lines and maps -- the models share the deepest one's pyramid as in
+ * geye_detect_models, and the frames' NMS passes run concurrently.  Every result is that
+ * frame's geye_detect for that model, bit for bit.  boxes, num and GL hold nframes * nmodels
+ * entries, frame major (entry f * nmodels + m); free each boxes[k] with geye_free.
+ * 1 <= nframes <= 64, 1 <= nmodels <= 8.  The _dev form takes device-resident frames. */
+int geye_detect_batch(geye_ctx *ctx, const geye_model *const *models, int nmodels,
+                      const unsigned char *const *bgr, int nframes, int width, int height,
+                      int widthStep, float thresh, float overlap, int precision, float **boxes,
+                      int *num, int *GL);
+int geye_detect_batch_dev(geye_ctx *ctx, const geye_model *const *models, int nmodels,
+                          const void *const *d_bgr, int nframes, int width, int height,
+                          int widthStep, float thresh, float overlap, int precision,
+                          float **boxes, int *num, int *GL);
 /* Root+bias-parts score maps of the last geye_get_boxes/geye_detect* call on ctx,
  * concatenated per (valid level, component) in the reference's order, column-major.
  * Writes at most cap floats; returns the total count through *total. */

@@ -163,3 +163,45 @@ def test_served_model_pairs_identical_alone_and_in_the_mix(ctx, precision):
     car.free()
     person.free()
 
+
+@pytest.mark.parametrize("precision,W,H,nf,thresh", [("bf16", 640, 480, 5, -0.5),
+                                                     ("fp32", 640, 480, 5, -0.5),
+                                                     ("fp16", 800, 600, 2, -0.5),
+                                                     ("bf16", 1920, 1080, 3, -0.5),
+                                                     ("fp32", 640, 480, 3, 1.5)])
+def test_detect_batch_equals_each_frame_alone(ctx, precision, W, H, nf, thresh):
+    """geye_detect_batch: nf frames through car, person (sharing car's pyramid) and an
+    interval-5 model (its own pyramid), every launch before the NMS covering all frames and the
+    frames' NMS passes running together; each (frame, model) result is that frame's
+    car_detection alone, bit for bit (thresh 1.5 leaves frames with few or no candidates)."""
+    say("detect batch", precision, W, H, nf, thresh)
+    models = [ctx.model(synth.synthetic_car()), ctx.model(synth.synthetic_person()),
+              ctx.model(_other_interval_model())]
+    frames = [synth.make_frame(W, H, seed=900 + k) for k in range(nf)]
+    before = ctx.nms_fallbacks()
+    alone = [[m.car_detection(f, thresh=thresh, overlap=0.5, precision=precision) for m in models]
+             for f in frames]
+    replays = ctx.nms_fallbacks() - before  # score ties of the small model's NMS, if any
+    for rep in range(2):
+        before = ctx.nms_fallbacks()
+        got = ctx.detect_batch(models, frames, thresh=thresh, overlap=0.5, precision=precision)
+        for f in range(nf):
+            for i in range(len(models)):
+                g, r = got[f][i], alone[f][i]
+                assert g.shape == r.shape and np.array_equal(g.view(np.int32), r.view(np.int32)), \
+                    (rep, f, i, g.shape, r.shape)
+        # the batch takes the host replay exactly where the frames alone took it
+        assert ctx.nms_fallbacks() - before == replays
+    d = [ctx.dev_upload(f) for f in frames]
+    try:
+        got = ctx.detect_batch(models[:2], frames, thresh=thresh, overlap=0.5, precision=precision,
+                               d_frames=d)
+    finally:
+        for p in d:
+            ctx.dev_free(p)
+    for f in range(nf):
+        for i in range(2):
+            assert np.array_equal(got[f][i].view(np.int32), alone[f][i].view(np.int32)), (f, i)
+    for m in models:
+        m.free()
+
