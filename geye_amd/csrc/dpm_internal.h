@@ -48,6 +48,7 @@ struct FusedResize {
   int in_h, in_w, out_h, out_w;
   int rtab_off, ctab_off;
   int tiles_x, tiles_y, blocks_begin;
+  int frame;                   // frame of a batch (the first round reads that frame's bytes)
 };
 // fused resize tile: 8 x 64 outputs (13 KB of LDS, so a CU holds as many workgroups as its
 // wave slots allow; on 1080p 8 rows beat 16 and 32); span: source columns

@@ -16,9 +16,12 @@ void launch_resize_cols(hipStream_t s, const ResizeOp *ops, int nops, int blocks
                         float *dst);
 // HOG histogram in two launches: gradients per pyramid image, then votes per level
 // (HogLevel tiles: one cell column of kHogVoteRows cells per 64-lane workgroup)
+// bgr (optional): the round reads the frames' BGR bytes (rows ws bytes apart, frames
+// frame_bytes apart) in place of the planar source image (the first round)
 void launch_resize_fused(hipStream_t s, const FusedResize *ops, int nops, int blocks,
                          const int2 *tab, const int4 *si4, const float4 *al4, const float *src,
-                         float *dst);
+                         float *dst, const unsigned char *bgr = nullptr, int ws = 0,
+                         long long frame_bytes = 0);
 void launch_hog_grad(hipStream_t s, const GradImg *imgs, int nimg, int blocks, const float *img,
                      int2 *grid);
 void launch_hog_votes(hipStream_t s, const HogLevel *lv, int nlv, int blocks, const int2 *grid,

@@ -118,11 +118,19 @@ __global__ __launch_bounds__(256) void k_resize_cols(const ResizeOp *__restrict_
 // pass for the tile's output rows over the source columns its 64 outputs tap goes to LDS, the
 // column pass reads it there.  Each element is the same ordered tap sum (from 0, table
 // order) as the two separate passes, so the image is bit-identical.
+//
+// BGR: the first round, whose source is the frame itself -- the row pass reads the frame's
+// BGR bytes (frame op.frame of a batch, frame_bytes apart, rows ws bytes apart) instead of
+// the planar float copy: a quarter of the bytes, the same values (the conversion is exact),
+// so the same bits.
+template <bool BGR>
 __global__ __launch_bounds__(256) void k_resize_fused(const FusedResize *__restrict__ ops,
                                                       int nops, const int2 *__restrict__ tab,
                                                       const int4 *__restrict__ si4,
                                                       const float4 *__restrict__ al4,
                                                       const float *__restrict__ src,
+                                                      const unsigned char *__restrict__ bgr,
+                                                      int ws, long long frame_bytes,
                                                       float *__restrict__ dst) {
   constexpr int MAXTAP = 4;  // scales >= 0.5: at most 4 source samples per output
   __shared__ float s_t[3][kResizeTY][kResizeSpan];
@@ -165,6 +173,7 @@ __global__ __launch_bounds__(256) void k_resize_fused(const FusedResize *__restr
   // row pass, the three planes of an element together: up to 12 loads in flight
   const long long plane = (long long)op.in_h * op.in_w;
   const float *in = src + op.in_off + c0;
+  const unsigned char *inb = bgr + (BGR ? op.frame * frame_bytes + 3LL * c0 : 0);
   constexpr int U = 2;  // elements per thread per step
   for (int i0 = tid; i0 < ny * span; i0 += U * 256) {
     float v[U][3][MAXTAP];
@@ -178,9 +187,15 @@ __global__ __launch_bounds__(256) void k_resize_fused(const FusedResize *__restr
 #pragma unroll
       for (int q = 0; q < MAXTAP; q++)
         if (q < nn[u]) {
-          const float *p = in + (long long)s_rsi[rr[u]][q] * op.in_w + xx[u];
+          if (BGR) {
+            const unsigned char *p = inb + (long long)s_rsi[rr[u]][q] * ws + 3 * xx[u];
 #pragma unroll
-          for (int c = 0; c < 3; c++) v[u][c][q] = p[c * plane];
+            for (int c = 0; c < 3; c++) v[u][c][q] = (float)p[c];
+          } else {
+            const float *p = in + (long long)s_rsi[rr[u]][q] * op.in_w + xx[u];
+#pragma unroll
+            for (int c = 0; c < 3; c++) v[u][c][q] = p[c * plane];
+          }
         }
     }
 #pragma unroll
@@ -216,10 +231,14 @@ __global__ __launch_bounds__(256) void k_resize_fused(const FusedResize *__restr
 
 void launch_resize_fused(hipStream_t s, const FusedResize *ops, int nops, int blocks,
                          const int2 *tab, const int4 *si4, const float4 *al4, const float *src,
-                         float *dst) {
-  if (blocks > 0)
-    hipLaunchKernelGGL(k_resize_fused, dim3(blocks), dim3(256), 0, s, ops, nops, tab, si4, al4,
-                       src, dst);
+                         float *dst, const unsigned char *bgr, int ws, long long frame_bytes) {
+  if (blocks <= 0) return;
+  if (bgr)
+    hipLaunchKernelGGL(k_resize_fused<true>, dim3(blocks), dim3(256), 0, s, ops, nops, tab, si4,
+                       al4, src, bgr, ws, frame_bytes, dst);
+  else
+    hipLaunchKernelGGL(k_resize_fused<false>, dim3(blocks), dim3(256), 0, s, ops, nops, tab, si4,
+                       al4, src, nullptr, 0, 0LL, dst);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -877,7 +896,10 @@ __global__ __launch_bounds__(256) void k_dt(const DtJob *__restrict__ jobs,
 // Each output then takes the first least cost over the 2T + 1 sources of its window in
 // ascending order, with the reference's cost expression and strict comparison.
 // ---------------------------------------------------------------------------------------
-template <int T, bool ROWS>
+// B0: b == 0 (either sign), where b*t is a zero and (x + Q) + b*t == x + Q for every x, Q
+// with Q >= 0 (a sum that is exactly zero rounds to +0, and +0 + -0 = +0), so each window
+// position costs one add less.
+template <int T, bool ROWS, bool B0>
 __device__ inline void dt_window(const DtJob &J, const float *s_f, int LSf,
                                  float *__restrict__ dst_arena, int *__restrict__ ptr_arena) {
   constexpr int NW = 2 * T + 1;
@@ -933,7 +955,7 @@ __device__ inline void dt_window(const DtJob &J, const float *s_f, int LSf,
       int bj = T;
 #pragma unroll
       for (int j = 0; j < NW; j++) {
-        const float c = (w[j] + Q[j]) + B[j];
+        const float c = B0 ? w[j] + Q[j] : (w[j] + Q[j]) + B[j];
         if (best > c) {
           best = c;
           bj = j;
@@ -1048,7 +1070,7 @@ __global__ __launch_bounds__(256) void k_dt_window(const DtJob *__restrict__ job
     const float x = (b1 + sqrtf(b1 * b1 + 4.0f * a1 * R)) / (2.0f * a1);
     if (x < 16.0f) {
       const int need = (int)floorf(x * (1.0f + 1.52587890625e-5f) + 9.765625e-4f) + 1;
-      const int Tw = need <= 8 ? 8 : need <= 10 ? 10 : need <= 12 ? 12 : need <= 14 ? 14 : 16;
+      const int Tw = need < 8 ? 8 : need;  // 8 .. 16
       // monotone argmins: a >= 2 eps with eps = 2^-23 (2 max|f| + 4 a T^2 + 2 |b| T) (twice
       // the rounding bound above)
       const float ft = (float)Tw;
@@ -1079,14 +1101,23 @@ __global__ __launch_bounds__(256) void k_dt_window(const DtJob *__restrict__ job
     return;
   }
   const bool rows = J.elem_stride != 1;
-#define GEYE_DT_WINDOW(TT)                                                 \
-  (rows ? dt_window<TT, true>(J, smem, LSf, dst_arena, ptr_arena)           \
-        : dt_window<TT, false>(J, smem, LSf, dst_arena, ptr_arena))
-  if (T == 8) GEYE_DT_WINDOW(8);
-  else if (T == 10) GEYE_DT_WINDOW(10);
-  else if (T == 12) GEYE_DT_WINDOW(12);
-  else if (T == 14) GEYE_DT_WINDOW(14);
-  else GEYE_DT_WINDOW(16);
+  const bool b0 = b == 0.0f;
+#define GEYE_DT_WINDOW(TT)                                                                \
+  (rows ? (b0 ? dt_window<TT, true, true>(J, smem, LSf, dst_arena, ptr_arena)              \
+              : dt_window<TT, true, false>(J, smem, LSf, dst_arena, ptr_arena))            \
+        : (b0 ? dt_window<TT, false, true>(J, smem, LSf, dst_arena, ptr_arena)             \
+              : dt_window<TT, false, false>(J, smem, LSf, dst_arena, ptr_arena)))
+  switch (T) {
+    case 8: GEYE_DT_WINDOW(8); break;
+    case 9: GEYE_DT_WINDOW(9); break;
+    case 10: GEYE_DT_WINDOW(10); break;
+    case 11: GEYE_DT_WINDOW(11); break;
+    case 12: GEYE_DT_WINDOW(12); break;
+    case 13: GEYE_DT_WINDOW(13); break;
+    case 14: GEYE_DT_WINDOW(14); break;
+    case 15: GEYE_DT_WINDOW(15); break;
+    default: GEYE_DT_WINDOW(16); break;
+  }
 #undef GEYE_DT_WINDOW
 }
 

@@ -501,7 +501,10 @@ __global__ __launch_bounds__(256) void k_nms_scatter(const NmsBox *__restrict__ 
       b.y2 = (b.y1 + s_fh[km]) - 1.0f;  // == (float)((double)(y1 + rh*s) - 1.0)
       b.x2 = (b.x1 + s_fw[km]) - 1.0f;
       b.area = (float)(((double)(b.x2 - b.x1) + 1.0) * ((double)(b.y2 - b.y1) + 1.0));
-      if (suppresses(P, b, overlap)) atomicMin(&sup[r], rp);
+      // sup only decreases: a box that already names an earlier suppressor needs no atomic
+      if (suppresses(P, b, overlap) &&
+          __hip_atomic_load(&sup[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > rp)
+        atomicMin(&sup[r], rp);
     }
   }
 }

@@ -319,6 +319,7 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
       f.tiles_x = (f.out_w + kResizeTX - 1) / kResizeTX;
       f.tiles_y = (f.out_h + kResizeTY - 1) / kResizeTY;
       f.blocks_begin = P.fused_blocks[r];
+      f.frame = 0;
       P.fused_blocks[r] += f.tiles_x * f.tiles_y;
       // every tile's source-column span must fit the kernel's LDS rows
       for (int tx = 0; tx < f.tiles_x; tx++) {
@@ -712,6 +713,7 @@ void replicate_plan(const Plan &one, int nb, long long feat_stride, long long fe
         op.in_off += f * one.img_total;
         op.out_off += f * one.img_total;
         op.blocks_begin += f * one.fused_blocks[r];
+        op.frame = f;
         P.fused[r].push_back(op);
       }
     P.fused_blocks[r] = one.fused_blocks[r] * nb;

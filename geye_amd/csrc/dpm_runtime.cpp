@@ -714,9 +714,11 @@ int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, int hk
   int fbase = 0;
   for (size_t r = 0; r < P.fused.size(); r++) {
     KTimer t(c, "resize");
+    // round 0 resizes the frame itself: it reads the BGR bytes
     launch_resize_fused(s, dp<FusedResize>(P.d_fused) + fbase, (int)P.fused[r].size(),
                         P.fused_blocks[r], dp<int2>(P.d_resize_tab), dp<int4>(P.d_resize_si4),
-                        dp<float4>(P.d_resize_al4), dp<float>(P.d_img), dp<float>(P.d_img));
+                        dp<float4>(P.d_resize_al4), dp<float>(P.d_img), dp<float>(P.d_img),
+                        r == 0 ? d_bgr : nullptr, ws, (long long)ws * P.H);
     fbase += (int)P.fused[r].size();
   }
   const HogLevel *hog = dp<HogLevel>(P.d_hog);
