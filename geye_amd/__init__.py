@@ -114,6 +114,7 @@ _SIGS = [
     ("geye_serve_reset_stats", C.c_int, [C.c_void_p]),
     ("geye_serve_nms_fallbacks", C.c_int, [C.c_void_p, _llp]),
     ("geye_plan_bytes", C.c_int, [C.c_void_p, C.c_int, C.c_int, _llp]),
+    ("geye_plan_bytes_desc", C.c_int, [C.c_void_p, C.c_int, C.c_int, _llp]),
     ("geye_serve_destroy", None, [C.c_void_p]),
     ("geye_free", None, [C.c_void_p]),
     ("geye_resize", C.c_int, [C.c_void_p, _fp, _ip, C.c_float, _fp, _ip]),
@@ -136,6 +137,7 @@ _SIGS = [
     ("geye_profile_enable", C.c_int, [C.c_void_p, C.c_int]),
     ("geye_kernel_stats", C.c_int, [C.c_void_p, C.c_char_p, C.c_int, _dp, _llp, C.c_int, _ip]),
     ("geye_nms_fallbacks", C.c_int, [C.c_void_p, _llp]),
+    ("geye_csv_values", C.c_int, [C.c_char_p, C.POINTER(_fp), _llp]),
     ("geye_dt_form", C.c_int, [C.c_void_p, C.c_int]),
 ]
 

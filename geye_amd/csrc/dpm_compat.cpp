@@ -146,23 +146,11 @@ int GL_of(const MODEL *MO) { return (MO->MI->numpart[0] + 1) * 4 + 3; }
 
 // ---- CSV loader (formats of CT/load_model.cpp:57-530) -----------------------------------
 bool read_values(const char *path, std::vector<float> &v) {
-  FILE *f = std::fopen(path, "r");
-  if (!f) return false;
-  std::string s;
-  char buf[1 << 16];
-  size_t n;
-  while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) s.append(buf, n);
-  std::fclose(f);
-  const char *p = s.c_str();
-  while (*p) {
-    while (*p && std::strchr(", \t\r\n", *p)) p++;
-    if (!*p) break;
-    char *e;
-    const float x = std::strtof(p, &e);
-    if (e == p) return false;
-    v.push_back(x);
-    p = e;
-  }
+  float *vals = nullptr;
+  long long n = 0;
+  if (geye_csv_values(path, &vals, &n) != GEYE_OK) return false;
+  v.assign(vals, vals + n);
+  geye_free(vals);
   return true;
 }
 

@@ -16,31 +16,30 @@
 
 namespace geye {
 
+// Pyramid depth beyond the first octave (Model_info.max_scale as ini_scales sets it,
+// CT/featurepyramid.cpp.new2:179-219).  The frame's cell grid (H / sbin by W / sbin) shrinks
+// by 2^(-1/interval) per level; the depth ends one level before the first level (counted from
+// interval) whose grid is smaller than half the smallest root filter in either direction,
+// capped at 36 levels and never below interval.  The grid is tracked in float with the
+// reference's operations, because the level where the comparison first holds depends on
+// that rounding.
 int compute_max_scale(const Model &m, int W, int H) {
-  const int interval = m.interval;
-  const float sc = (float)std::pow(2.0, (1 / (double)interval));
-  int max_scale = 36;
-  const int L_NUM = interval + max_scale;
-  float MRY = (float)m.rsize[0], MRX = (float)m.rsize[1];
-  for (int kk = 1; kk < m.numcomponent; kk++) {
-    if ((float)m.rsize[kk * 2] < MRY) MRY = (float)m.rsize[kk * 2];
-    if ((float)m.rsize[kk * 2 + 1] < MRX) MRX = (float)m.rsize[kk * 2 + 1];
+  constexpr int kDepthCap = 36;
+  const float shrink = 1 / (float)std::pow(2.0, 1.0 / (double)m.interval);
+  float half_rows = (float)m.rsize[0], half_cols = (float)m.rsize[1];
+  for (int c = 1; c < m.numcomponent; c++) {
+    half_rows = std::min(half_rows, (float)m.rsize[2 * c]);
+    half_cols = std::min(half_cols, (float)m.rsize[2 * c + 1]);
   }
-  MRY /= 2;
-  MRX /= 2;
-  float height = (float)H / (float)m.sbin;
-  float width = (float)W / (float)m.sbin;
-  const float sc_step = 1 / sc;
-  for (int kk = 0; kk < L_NUM; kk++) {
-    height *= sc_step;
-    width *= sc_step;
-    if (height < MRY || width < MRX) {
-      max_scale = kk - interval - 1;
-      break;
-    }
+  half_rows /= 2;
+  half_cols /= 2;
+  float rows = (float)H / (float)m.sbin, cols = (float)W / (float)m.sbin;
+  for (int level = 0; level < m.interval + kDepthCap; level++) {
+    rows *= shrink;
+    cols *= shrink;
+    if (rows < half_rows || cols < half_cols) return std::max(level - m.interval - 1, m.interval);
   }
-  if (max_scale < interval) max_scale = interval;
-  return max_scale;
+  return std::max(kDepthCap, m.interval);
 }
 
 static int blocks_of(int n, int sbin) { return (int)std::floor(double(n) / double(sbin) + 0.5); }

@@ -151,10 +151,13 @@ unsigned short to_f16(float f) {
   return u;
 }
 
-int model_from_desc(geye_ctx *ctx, const geye_model_desc *d, Model &m) {
+}  // namespace
+
+// The host side of a model: fields, checks, filter classes and the packed filters (FP32,
+// BF16, FP16) the device copies are made from.  No device work.
+int geye::model_pack(const geye_model_desc *d, Model &m, ModelPacks &pk) {
   if (!d || d->numcomponent <= 0 || d->NoR <= 0 || d->sbin <= 0 || d->interval <= 0)
     return fail(GEYE_ERR_ARG, "model: empty or malformed description");
-  m.ctx = ctx;
   m.numcomponent = d->numcomponent;
   m.sbin = d->sbin;
   m.interval = d->interval;
@@ -247,8 +250,11 @@ int model_from_desc(geye_ctx *ctx, const geye_model_desc *d, Model &m) {
       m.classes.push_back(pc);
     }
   }
-  std::vector<float> packed;
-  std::vector<unsigned short> pbf, pf16;
+  std::vector<float> &packed = pk.f32;
+  std::vector<unsigned short> &pbf = pk.bf16, &pf16 = pk.f16;
+  packed.clear();
+  pbf.clear();
+  pf16.clear();
   for (Model::FilterClass &fc : m.classes) {
     const int nmem = (int)fc.members.size();
     fc.nf = pick_nf(nmem);
@@ -282,10 +288,6 @@ int model_from_desc(geye_ctx *ctx, const geye_model_desc *d, Model &m) {
                   pf16.push_back(to_f16(v));
                 }
   }
-  CHK(upload(m.d_filt_bf16, pbf, ctx->stream));
-  CHK(upload(m.d_filt_f16, pf16, ctx->stream));
-  CHK(upload(m.d_filters, packed, ctx->stream));
-  CHK(upload(m.d_rsize, m.rsize, ctx->stream));
   m.lsq.clear();
   m.lsq_idx.clear();
   if (d->lsq) {
@@ -296,12 +298,29 @@ int model_from_desc(geye_ctx *ctx, const geye_model_desc *d, Model &m) {
       m.lsq_idx[C + i] = ml;
       m.lsq.insert(m.lsq.end(), d->lsq + m.lsq.size(), d->lsq + m.lsq.size() + 4 * ml);
     }
+  }
+  return GEYE_OK;
+}
+
+namespace {
+
+int model_from_desc(geye_ctx *ctx, const geye_model_desc *d, Model &m) {
+  ModelPacks pk;
+  CHK(model_pack(d, m, pk));
+  m.ctx = ctx;
+  CHK(upload(m.d_filt_bf16, pk.bf16, ctx->stream));
+  CHK(upload(m.d_filt_f16, pk.f16, ctx->stream));
+  CHK(upload(m.d_filters, pk.f32, ctx->stream));
+  CHK(upload(m.d_rsize, m.rsize, ctx->stream));
+  if (!m.lsq.empty()) {
     CHK(upload(m.d_lsq, m.lsq, ctx->stream));
     CHK(upload(m.d_lsq_idx, m.lsq_idx, ctx->stream));
   }
   return GEYE_OK;
 }
 
+// Every number of a model CSV file (CT/load_model.cpp:57-530 reads them with fscanf, commas
+// and white space between); false when the file cannot be read or holds a non-number.
 bool read_csv(const char *path, std::vector<float> &v) {
   FILE *f = std::fopen(path, "r");
   if (!f) return false;
@@ -2540,6 +2559,26 @@ int geye_dt_form(geye_ctx *c, int form) {
     return fail(GEYE_ERR_ARG, "unknown DT form");
   std::lock_guard<std::mutex> lk(c->mu);
   c->dt_form = form;
+  return GEYE_OK;
+}
+
+int geye_plan_bytes_desc(const geye_model_desc *desc, int width, int height, long long *bytes) {
+  if (!desc || !bytes || width < 8 || height < 8) return fail(GEYE_ERR_ARG, "bad argument");
+  Model m;
+  ModelPacks pk;
+  CHK(model_pack(desc, m, pk));
+  return plan_bytes(m, width, height, bytes);
+}
+
+int geye_csv_values(const char *path, float **values, long long *count) {
+  if (!path || !values || !count) return fail(GEYE_ERR_ARG, "null argument");
+  *values = nullptr;
+  *count = 0;
+  std::vector<float> v;
+  if (!read_csv(path, v)) return fail(GEYE_ERR_IO, std::string("cannot read ") + path);
+  *values = to_malloc(v);
+  if (!*values) return fail(GEYE_ERR_NOMEM, "malloc");
+  *count = (long long)v.size();
   return GEYE_OK;
 }
 

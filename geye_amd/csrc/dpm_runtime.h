@@ -33,6 +33,14 @@ int dt_line_geometry(int n, bool window, int *G, size_t *lds);
 // device (dpm_noalloc.hip)
 int noalloc_setup(int device);
 
+// the packed filters of a model (FP32 [group][31][w][h][NF]; 16-bit for the MFMA kernels)
+struct ModelPacks {
+  std::vector<float> f32;
+  std::vector<unsigned short> bf16, f16;
+};
+// the host side of a model from its description (no device work)
+int model_pack(const geye_model_desc *d, Model &m, ModelPacks &pk);
+
 // sets the calling thread's geye_last_error() message; returns code
 int set_error(int code, const std::string &msg);
 

@@ -74,6 +74,10 @@ int geye_device_count(int *n);
 int geye_model_load_csv(geye_ctx *ctx, const char *comp, const char *root, const char *part,
                         geye_model **model);
 int geye_model_create(geye_ctx *ctx, const geye_model_desc *desc, geye_model **model);
+/* The numbers of one model CSV file in order (the format of CT/load_model.cpp:57-530:
+ * numbers separated by commas and white space), malloc'd (free with geye_free);
+ * GEYE_ERR_IO when the file cannot be read or holds something else.  Host only. */
+int geye_csv_values(const char *path, float **values, long long *count);
 void geye_model_free(geye_model *model);
 /* Reads back the loaded model's scalar fields and the first filter value (loader checks). */
 int geye_model_info(const geye_model *model, int *numcomponent, int *sbin, int *interval,
@@ -262,6 +266,8 @@ int geye_serve_nms_fallbacks(geye_server *srv, long long *count);
 /* Device bytes of one frame's working memory for this model and frame size, as the server
  * accounts it against hbm_budget (host-only; no device work). */
 int geye_plan_bytes(const geye_model *model, int width, int height, long long *bytes);
+/* The same from a model description, with no device at all (capacity planning on a host). */
+int geye_plan_bytes_desc(const geye_model_desc *desc, int width, int height, long long *bytes);
 void geye_serve_destroy(geye_server *srv);
 
 /* ---- device memory and timing (bench / serving plumbing) ------------------------------- */

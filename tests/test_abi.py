@@ -41,6 +41,48 @@ def test_error_path_without_device_work():
     assert b"" != L.geye_last_error()
 
 
+def test_model_file_missing_or_malformed_is_an_io_error(tmp_path):
+    """The model CSV reader (CT/load_model.cpp:67-71 exits on a missing file; here a status):
+    a missing file and a file holding a non-number are GEYE_ERR_IO with a message naming the
+    file; a well-formed file gives its numbers in order (commas, spaces and newlines between)."""
+    L = geye_amd.lib()
+    vals, n = C.POINTER(C.c_float)(), C.c_longlong()
+    missing = str(tmp_path / "no_such_comp.csv").encode()
+    assert L.geye_csv_values(missing, C.byref(vals), C.byref(n)) == -4
+    assert b"no_such_comp.csv" in L.geye_last_error()
+    bad = tmp_path / "bad.csv"
+    bad.write_text("2,8,10,\n6,13,abc\n")
+    assert L.geye_csv_values(str(bad).encode(), C.byref(vals), C.byref(n)) == -4
+    good = tmp_path / "good.csv"
+    good.write_text("2, 8,10\n-4.058444 ,1e-3\r\n\t7\n")
+    assert L.geye_csv_values(str(good).encode(), C.byref(vals), C.byref(n)) == 0
+    got = [vals[i] for i in range(n.value)]
+    L.geye_free(vals)
+    assert got == [2.0, 8.0, 10.0, np.float32(-4.058444), np.float32(1e-3), 7.0]
+
+
+def test_plan_bytes_from_a_description_needs_no_device(car_model):
+    """geye_plan_bytes_desc packs the model and lays out one frame's plan on the host: the
+    size is positive, grows with the frame, and a description naming a part filter that does
+    not exist is GEYE_ERR_ARG."""
+    L = geye_amd.lib()
+    d, keep = geye_amd.desc_from_model(car_model)
+    sizes = []
+    for W, H in [(320, 240), (640, 480), (1920, 1080), (3840, 2160)]:
+        b = C.c_longlong()
+        assert L.geye_plan_bytes_desc(C.byref(d), W, H, C.byref(b)) == 0, L.geye_last_error()
+        sizes.append(b.value)
+    assert sizes[0] > 0 and all(a < b for a, b in zip(sizes, sizes[1:]))
+    # 4K needs at least its own feature pyramid: 31 planes of the level-0 cells, in floats
+    assert sizes[-1] > 31 * (2160 // 8) * (3840 // 8) * 4
+    bad_pidx = np.array(car_model.pidx, np.int32)
+    bad_pidx[0] = car_model.NoP + 5
+    keep.append(bad_pidx)
+    d.pidx = bad_pidx.ctypes.data_as(C.POINTER(C.c_int))
+    b = C.c_longlong()
+    assert L.geye_plan_bytes_desc(C.byref(d), 640, 480, C.byref(b)) == -1
+
+
 @pytest.mark.parametrize("W,H", [(640, 480), (1280, 720), (1920, 1080), (3840, 2160), (200, 150),
                                  (90, 400)])
 def test_level_plan_matches_oracle(car_model, W, H):

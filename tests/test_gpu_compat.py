@@ -9,6 +9,7 @@ import subprocess
 import numpy as np
 import pytest
 
+import geye_amd
 from geye_amd import synth
 from oracle import oracle as O
 
@@ -121,3 +122,29 @@ def test_reference_call_sequence_end_to_end(tmp_path, car_model, which, W, H, th
         o_res = O.new_rects(m, o_rects, W, H, 1.0)
         for k in o_res:
             assert np.array_equal(res[k], o_res[k]), k
+
+
+def test_failures_are_status_codes_and_the_context_survives(ctx, tmp_path):
+    """Failure paths on the device: a model file that is missing (GEYE_ERR_IO, as
+    CT/load_model.cpp:67-71 reports with exit()), a device allocation beyond the GPU's memory
+    (GEYE_ERR_NOMEM) and a frame too small for a plan (GEYE_ERR_ARG); after each the same
+    context still detects, bit-exact with the oracle."""
+    L = geye_amd.lib()
+    h = C.c_void_p()
+    rc = L.geye_model_load_csv(ctx.h, str(tmp_path / "none_comp.csv").encode(),
+                               str(tmp_path / "none_root.csv").encode(),
+                               str(tmp_path / "none_part.csv").encode(), C.byref(h))
+    assert rc == -4 and b"none_comp.csv" in L.geye_last_error()
+    p = C.c_void_p()
+    assert L.geye_dev_alloc(ctx.h, C.c_size_t(1 << 60), C.byref(p)) == -3
+    m = synth.tiny_model()
+    gm = ctx.model(m)
+    tiny = synth.make_frame(6, 6, seed=1)
+    with pytest.raises(geye_amd.GeyeError):
+        gm.get_boxes(tiny, thresh=-0.5)
+    frame = synth.make_frame(200, 150, seed=21)
+    rows = gm.get_boxes(frame, thresh=-0.5)
+    ref = O.detect(m, frame, -0.5)
+    assert rows.shape == ref.shape and np.array_equal(rows.view(np.int32), ref.view(np.int32))
+    gm.free()
+
