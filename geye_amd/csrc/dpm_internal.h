@@ -284,13 +284,13 @@ struct Plan {
   // device NMS
   DeviceBuffer n_box, n_keys, n_keys2, n_sbox, n_alive0, n_alive1, n_status, n_fsup, n_picks,
       n_counters, n_mask, n_sort, n_flags, n_keep_orig, n_kept,
-      n_total, n_vals, n_ord, n_rank;
+      n_total, n_vals, n_ord, n_rank, n_dec, n_alist, n_acount, n_grid;
   long long nms_fallbacks = 0;
   std::map<long long, hipGraphExec_t> graphs;  // front of the frame, per (precision, ws, thresh)
   // batch plans: per frame's NMS job its small buffers (counters, flags, totals), window
-  // bitmask, compaction counts and sort storage, two events, and mapped host slots (4 per job,
-  // then the nb + 1 frame offsets of the candidate list)
-  std::vector<DeviceBuffer> b_small, b_mask, b_blocksum, b_sort;
+  // bitmask, compaction counts, sort storage and pick index, two events, and mapped host slots
+  // (4 per job, then the nb + 1 frame offsets of the candidate list)
+  std::vector<DeviceBuffer> b_small, b_mask, b_blocksum, b_sort, b_grid, b_acount;
   std::vector<hipEvent_t> b_ev;
   int *b_host = nullptr, *b_dev = nullptr;
 };

@@ -13,9 +13,9 @@
 //     pair of distinct scores);
 //  3. rounds: the first W unchecked boxes in sigma form a window whose greedy outcome is
 //     exact on its own (everything earlier is decided); a pairwise bitmask plus a one-wave
-//     sequential scan picks within the window, then every later unchecked box is tested
-//     against the window's picks in order (first suppressor recorded), and the unchecked
-//     list is compacted;
+//     sequential scan picks within the window, then every unchecked box is tested against
+//     the window's picks ranked before it (first suppressor recorded) -- with a spatial index
+//     of the picks, only the picks it names -- and the unchecked list is compacted;
 //  4. finalise: the last pick is dropped unless some box names it as first suppressor, and
 //     the picks are emitted in original order.
 // Tie order: sigma differs from the quicksort only among equal scores.  If no box is
@@ -23,6 +23,8 @@
 // every order of equal scores gives the same result (induction over score groups); when
 // that check fails the caller re-runs the exact host quicksort (a reported fallback).
 #include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
 
 #include "dpm_kernels.h"
 #include "dpm_nms.h"
@@ -94,19 +96,33 @@ __global__ __launch_bounds__(256) void k_nms_prep_rows(const float *__restrict__
   vals[i] = i;
 }
 
-// sorted order: sbox[r] = box[ord[r]]; optional rank lookup by flat score index
+// Tile of a coordinate along one axis: floor(v) / 2^shift, clamped to the grid (monotone in
+// v, so a point inside an interval falls in a tile of the interval's tile range).
+__device__ inline int nms_tile(float v, int shift, int g) {
+  const int t = (int)floorf(v) >> shift;
+  return t < 0 ? 0 : (t >= g ? g - 1 : t);
+}
+
+// fine tiles, coarse tiles, and the words of all three mask arrays per mask word
+__device__ __host__ inline int nms_coarse_tiles(const NmsGrid &g) {
+  return ((g.gx + 3) >> 2) * ((g.gy + 3) >> 2);
+}
+__device__ __host__ inline int nms_grid_tiles(const NmsGrid &g) {
+  return 2 * g.gx * g.gy + nms_coarse_tiles(g);
+}
+
+// sorted order: sbox[r] = box[ord[r]]; with a grid, each candidate's rank and decided flag,
+// and the first round's tile counts
 __global__ __launch_bounds__(256) void k_nms_gather(const int *__restrict__ ord,
                                                     const NmsBox *__restrict__ box, int n,
                                                     NmsBox *__restrict__ sbox,
                                                     int *__restrict__ alive,
                                                     int *__restrict__ status,
                                                     int *__restrict__ sup,
-                                                    const int *__restrict__ cand,
-                                                    int *__restrict__ rank_of_flat,
                                                     int *__restrict__ counters,
                                                     int *__restrict__ nalive0,
                                                     int *__restrict__ flags,
-                                                    int *__restrict__ keep) {
+                                                    int *__restrict__ keep, NmsGrid g) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   // the pass's other initial state: pick counters, the first round's alive count, the tie
   // flags and the survivor flags (written by the finalisation, after the rounds)
@@ -115,6 +131,7 @@ __global__ __launch_bounds__(256) void k_nms_gather(const int *__restrict__ ord,
     flags[r] = 0;
   }
   if (r == 0) *nalive0 = n;
+  if (g.ext && r < kNmsMaskWords * nms_grid_tiles(g)) g.ext[r] = 0;  // ext, ctr, ctrc
   if (r >= n) return;
   const int i = ord[r];
   sbox[r] = box[i];
@@ -122,7 +139,10 @@ __global__ __launch_bounds__(256) void k_nms_gather(const int *__restrict__ ord,
   status[r] = 0;
   sup[r] = kNoSup;
   keep[r] = 0;
-  if (rank_of_flat) rank_of_flat[cand[i]] = r;
+  if (g.ext) {
+    g.rank_of[i] = r;
+    g.dec[i] = 0;
+  }
 }
 
 // overlap test of CT/nms.cpp:145-175 (symmetric in a and b)
@@ -377,155 +397,271 @@ __global__ __launch_bounds__(256) void k_nms_suppress(const NmsBox *__restrict__
     }
 }
 
-// Map path: each pick scatters its suppression over the score-map cells whose root boxes
-// can intersect it (a conservative cell rectangle per map).  The workgroup lays the cells
-// of all maps end to end and every thread walks its share with a running map index, so the
-// rank loads are independent.  The candidate's box is rebuilt from its cell with the prep
-// kernel's arithmetic and the exact overlap test decides; the first suppressor is the
-// minimum pick rank.
-__global__ __launch_bounds__(256) void k_nms_scatter(const NmsBox *__restrict__ sbox,
-                                                     const int *__restrict__ picks,
-                                                     const int *__restrict__ counters,
-                                                     const ScoreMap *__restrict__ maps,
-                                                     int nmaps, const int *__restrict__ rsize,
-                                                     int padx, int pady,
-                                                     const int *__restrict__ rank_of_flat,
-                                                     float overlap, int *__restrict__ sup) {
-  // per-map tables in dynamic LDS sized to nmaps (40 B per map): a workgroup takes only what
-  // the plan's maps need, so more of them (and other kernels) share a CU
-  extern __shared__ long long s_dyn[];
-  long long *s_off = s_dyn;                                 // [nmaps]
-  int *s_xa = (int *)(s_off + nmaps);                       // [nmaps] each
-  int *s_ya = s_xa + nmaps, *s_h = s_ya + nmaps, *s_R0 = s_h + nmaps;
-  float *s_s = (float *)(s_R0 + nmaps), *s_fh = s_s + nmaps, *s_fw = s_fh + nmaps;
-  int *s_beg = (int *)(s_fw + nmaps);                       // [nmaps + 1]
-  __shared__ int s_part[256];
-  const int p0 = counters[1], p1 = counters[0];
-  if ((int)blockIdx.x >= p1 - p0) return;
-  const int rp = picks[p0 + blockIdx.x];
-  const NmsBox P = sbox[rp];
-  const int tid = threadIdx.x;
-  // per-map rectangles; thread t owns maps [t*per, (t+1)*per).  A cell's box B (width
-  // bw = rw*s, height bh = rh*s) can be suppressed only if its overlap with the pick, ix by iy,
-  // exceeds overlap * R with R the smaller area; iy <= min(bh, ph) then needs ix > tx =
-  // overlap * R / min(bh, ph) (and likewise iy > ty), which bounds B's corner on both sides.
-  // The bounds are relaxed by 0.1% and one cell each way; the exact test decides.
-  const int per = (nmaps + 255) / 256;
-  const float pw = P.x2 - P.x1 + 1.0f, ph = P.y2 - P.y1 + 1.0f;
-  int mycnt = 0;
-  for (int k = tid * per; k < min(nmaps, (tid + 1) * per); k++) {
-    const ScoreMap m = maps[k];
-    const float s = m.scale;
-    const int rh = rsize[m.comp * 2], rw = rsize[m.comp * 2 + 1];
-    const float bw = (float)rw * s, bh = (float)rh * s;
-    const float R = fminf(bw * bh, P.area) * 0.999f;
-    const float tx = overlap > 0.0f ? overlap * R / fminf(bh, ph) : 0.0f;
-    const float ty = overlap > 0.0f ? overlap * R / fminf(bw, pw) : 0.0f;
-    const int xa = max((int)floorf((P.x1 - bw + tx) / s) + padx - 2, 0);
-    const int xb = min((int)ceilf((P.x2 + 1.0f - tx) / s) + padx + 1, m.R1 - 1);
-    const int ya = max((int)floorf((P.y1 - bh + ty) / s) + pady - 2, 0);
-    const int yb = min((int)ceilf((P.y2 + 1.0f - ty) / s) + pady + 1, m.R0 - 1);
-    const int h = yb - ya + 1, w = xb - xa + 1;
-    s_xa[k] = xa;
-    s_ya[k] = ya;
-    s_h[k] = h > 0 ? h : 1;
-    s_R0[k] = m.R0;
-    s_off[k] = m.score_off;
-    s_s[k] = s;
-    s_fh[k] = (float)rh * s;
-    s_fw[k] = (float)rw * s;
-    s_beg[k] = (h > 0 && w > 0) ? w * h : 0;  // count for now
-    mycnt += s_beg[k];
+// Index of the round's picks, one wave per pick: the pick's box (its rank in .pad) to pbox,
+// its candidate marked decided, its bit in the centre masks of its centre's tile and coarse
+// tile and in the extent mask of every tile its grown box touches.
+__global__ __launch_bounds__(256) void k_nms_grid_build(const NmsBox *__restrict__ sbox,
+                                                        const int *__restrict__ picks,
+                                                        const int *__restrict__ counters,
+                                                        NmsGrid g, int round) {
+  const int p0 = counters[1], np = counters[0] - p0;
+  const int pl = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (pl >= np) return;
+  const int rp = picks[p0 + pl];
+  NmsBox P = sbox[rp];
+  const int nt = g.gx * g.gy, cgx = (g.gx + 3) >> 2, nct = nms_coarse_tiles(g);
+  const int w = pl >> 6;
+  const unsigned long long bit = 1ull << (pl & 63);
+  if (lane == 0) {
+    g.dec[P.idx] = 1;
+    const int ty = nms_tile((P.y1 + P.y2) * 0.5f, g.shift, g.gy);
+    const int tx = nms_tile((P.x1 + P.x2) * 0.5f, g.shift, g.gx);
+    atomicOr(&g.ctr[(size_t)w * nt + ty * g.gx + tx], bit);
+    atomicOr(&g.ctrc[(size_t)w * nct + (ty >> 2) * cgx + (tx >> 2)], bit);
+    P.pad = rp;
+    g.pbox[pl] = P;
   }
-  s_part[tid] = mycnt;
-  __syncthreads();
-  for (int o = 1; o < 256; o <<= 1) {  // inclusive scan of per-thread counts
-    const int v = tid >= o ? s_part[tid - o] : 0;
-    __syncthreads();
-    s_part[tid] += v;
-    __syncthreads();
+  const int ty0 = nms_tile(P.y1 - kNmsTileMargin, g.shift, g.gy);
+  const int ty1 = nms_tile(P.y2 + kNmsTileMargin, g.shift, g.gy);
+  const int tx0 = nms_tile(P.x1 - kNmsTileMargin, g.shift, g.gx);
+  const int tx1 = nms_tile(P.x2 + kNmsTileMargin, g.shift, g.gx);
+  const int tw = tx1 - tx0 + 1, cells = tw * (ty1 - ty0 + 1);
+  for (int k = lane; k < cells; k += 64) {
+    const int dy = k / tw;
+    atomicOr(&g.ext[(size_t)w * nt + (ty0 + dy) * g.gx + tx0 + (k - dy * tw)], bit);
   }
-  int run = s_part[tid] - mycnt;
-  for (int k = tid * per; k < min(nmaps, (tid + 1) * per); k++) {
-    const int c = s_beg[k];
-    s_beg[k] = run;
-    run += c;
+}
+
+constexpr int kNmsStageWords = 2048;  // mask words a suppression workgroup stages in LDS
+constexpr int kSupThreads = 512;      // threads (candidates) per suppression workgroup
+constexpr int kSupWaves = kSupThreads / 64;
+
+// Suppression pass over the grid, one thread per undecided candidate, in candidate order
+// (neighbouring score cells, so a workgroup's boxes share tiles): round 0 takes every
+// candidate, a later round the list of the candidates the previous rounds left undecided.  An
+// undecided box gets the least rank of the round's picks ranked before it that suppress it.
+// The workgroup stages the round's pick boxes and, when they fit, the masks of the tiles its
+// boxes read (the bounding rectangle of their grown boxes) in LDS; each box then ORs, one
+// mask word at a time, the extent mask of its centre's tile with the centre masks of the tiles
+// its grown box touches (from global memory when the rectangle does not fit, over coarse tiles
+// for a box touching more than 16), and tests the picks named there in pick order, which is
+// rank order: the first that suppresses it, or the first ranked after it, ends the scan.  The
+// boxes left undecided go, in thread order, to the workgroup's kSupThreads places in slot
+// (the round's end compacts them into the next list).
+// Completeness for overlap >= 0.5: suppression needs w h / R > overlap with R the smaller
+// area, and the intersection's width and height are at most the smaller box's, so each
+// exceeds half of the smaller box's (up to the rounding of w, h and R, well under a pixel
+// here); an interval inside the smaller box's span and longer than half of it reaches to
+// within half a pixel of that span's centre.  So the smaller box's centre lies in the larger
+// box grown by half a pixel: a pick that suppresses B is in the extent mask of B's centre
+// tile (B the smaller) or in the centre mask of a tile B's grown box touches (the pick the
+// smaller), and the exact test of CT/nms.cpp decides.
+__global__ __launch_bounds__(kSupThreads) void k_nms_grid_sup(const int *__restrict__ counters,
+                                                              NmsGrid g, float overlap,
+                                                              int round,
+                                                              int *__restrict__ sup) {
+  __shared__ float4 s_b[kNmsWindow];  // y1, x1, y2, x2
+  __shared__ float s_a[kNmsWindow];
+  __shared__ int s_r[kNmsWindow];
+  __shared__ unsigned long long s_m[kNmsStageWords];  // [word][ext, ctr][rectangle tile]
+  __shared__ int s_rect[4], s_cnt[kSupWaves];
+  const int cnt = round == 0 ? g.n : *g.lcount;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = -1;
+  if (t < cnt) i = round == 0 ? t : g.list[t];
+  const bool act = i >= 0 && !g.dec[i];
+  const int nact = __syncthreads_count(act);
+  if (nact == 0) {
+    if (threadIdx.x == 0) g.bsum[blockIdx.x] = 0;
+    return;
   }
-  if (tid == 255) s_beg[nmaps] = s_part[255];
-  __syncthreads();
-  const int total = s_beg[nmaps];
-  // each wave takes one contiguous chunk of the cells laid end to end (lanes consecutive,
-  // so the rank loads stay coalesced) and crosses only the maps that chunk spans; the
-  // chunk's first map comes from a binary search of the offsets
-  constexpr int B = 4;  // cells per batch: their rank loads are in flight together
-  const int lane = tid & 63;
-  const int nw = 4 * gridDim.y, wv = blockIdx.y * 4 + (tid >> 6);
-  const int chunk = (total + nw * 64 * B - 1) / (nw * 64 * B) * (64 * B);
-  const int c0 = wv * chunk, c1 = min(total, c0 + chunk);
-  int k = 0;
-  {
-    int lo = 0, hi = nmaps - 1;  // last map with s_beg[k] <= c0
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (s_beg[mid] <= c0) lo = mid; else hi = mid - 1;
+  // a workgroup with few undecided boxes (the list's last) reads masks and picks from global
+  // memory: staging costs it more than it saves
+  const bool dense = nact > kSupThreads / 4;
+  if (threadIdx.x == 0) {
+    s_rect[0] = s_rect[1] = 0x7fffffff;
+    s_rect[2] = s_rect[3] = -1;
+  }
+  const int p0 = counters[1], np = counters[0] - p0;
+  const int nt = g.gx * g.gy, nw = (np + 63) >> 6;
+  if (dense)
+    for (int k = threadIdx.x; k < np; k += blockDim.x) {
+      const NmsBox P = g.pbox[k];
+      s_b[k] = make_float4(P.y1, P.x1, P.y2, P.x2);
+      s_a[k] = P.area;
+      s_r[k] = P.pad;
     }
-    k = lo;
+  int r = 0, ty0 = 0, ty1 = -1, tx0 = 0, tx1 = -1, c = 0;
+  NmsBox B;
+  if (act) {
+    r = g.rank_of[i];
+    B = g.box[i];
+    ty0 = nms_tile(B.y1 - kNmsTileMargin, g.shift, g.gy);
+    ty1 = nms_tile(B.y2 + kNmsTileMargin, g.shift, g.gy);
+    tx0 = nms_tile(B.x1 - kNmsTileMargin, g.shift, g.gx);
+    tx1 = nms_tile(B.x2 + kNmsTileMargin, g.shift, g.gx);
+    c = nms_tile((B.y1 + B.y2) * 0.5f, g.shift, g.gy) * g.gx +
+        nms_tile((B.x1 + B.x2) * 0.5f, g.shift, g.gx);  // inside the grown box's tiles
   }
-  const int stride = 64;
-  for (int base = c0 + lane; base < c1; base += B * stride) {
-    int kk[B], xs[B], ys[B], rr[B];
+  int RY0 = 0, RX0 = 0, rw = 0, nrect = 0;
+  bool staged = false;
+  if (dense) {
+    // the workgroup's rectangle: wave reductions, then one LDS atomic per wave and bound
+    int a0 = act ? ty0 : 0x7fffffff, a1 = act ? tx0 : 0x7fffffff, a2 = act ? ty1 : -1,
+        a3 = act ? tx1 : -1;
+    for (int o = 1; o < 64; o <<= 1) {
+      a0 = min(a0, __shfl_xor(a0, o));
+      a1 = min(a1, __shfl_xor(a1, o));
+      a2 = max(a2, __shfl_xor(a2, o));
+      a3 = max(a3, __shfl_xor(a3, o));
+    }
+    __syncthreads();  // s_rect initialised
+    if ((threadIdx.x & 63) == 0 && a2 >= 0) {
+      atomicMin(&s_rect[0], a0);
+      atomicMin(&s_rect[1], a1);
+      atomicMax(&s_rect[2], a2);
+      atomicMax(&s_rect[3], a3);
+    }
+    __syncthreads();
+    RY0 = s_rect[0];
+    RX0 = s_rect[1];
+    rw = s_rect[3] - RX0 + 1;
+    nrect = rw * (s_rect[2] - RY0 + 1);
+    staged = 2 * nw * nrect <= kNmsStageWords;
+    if (staged) {
+      // [word][ext, ctr][tile]: up to 4 loads per thread, all in flight before the stores
+      constexpr int U = kNmsStageWords / kSupThreads;
+      const int total = 2 * nw * nrect;
+      unsigned long long v[U];
 #pragma unroll
-    for (int u = 0; u < B; u++) {
-      const int idx = base + u * stride;
-      rr[u] = -1;
-      if (idx < c1) {
-        while (s_beg[k + 1] <= idx) k++;
-        const int off = idx - s_beg[k];
-        const int h = s_h[k];
-        const int xo = off / h;
-        kk[u] = k;
-        xs[u] = s_xa[k] + xo;
-        ys[u] = s_ya[k] + (off - xo * h);
-        rr[u] = rank_of_flat[s_off[k] + (long long)xs[u] * s_R0[k] + ys[u]];
+      for (int u = 0; u < U; u++) {
+        const int k = threadIdx.x + u * kSupThreads;
+        v[u] = 0ull;
+        if (k < total) {
+          const int wk = k / nrect, lt = k - wk * nrect, dy = lt / rw;
+          const int tile = (RY0 + dy) * g.gx + RX0 + (lt - dy * rw);
+          v[u] = ((wk & 1) ? g.ctr : g.ext)[(size_t)(wk >> 1) * nt + tile];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int k = threadIdx.x + u * kSupThreads;
+        if (k < total) s_m[k] = v[u];
       }
     }
-#pragma unroll
-    for (int u = 0; u < B; u++) {
-      const int r = rr[u];
-      if (r <= rp) continue;  // not a candidate (-1), or ranked before the pick
-      const int km = kk[u];
-      const float s = s_s[km];
-      NmsBox b;
-      b.y1 = ((float)ys[u] - (float)pady + 1.0f) * s;
-      b.x1 = ((float)xs[u] - (float)padx + 1.0f) * s;
-      b.y2 = (b.y1 + s_fh[km]) - 1.0f;  // == (float)((double)(y1 + rh*s) - 1.0)
-      b.x2 = (b.x1 + s_fw[km]) - 1.0f;
-      b.area = (float)(((double)(b.x2 - b.x1) + 1.0) * ((double)(b.y2 - b.y1) + 1.0));
-      // sup only decreases: a box that already names an earlier suppressor needs no atomic
-      if (suppresses(P, b, overlap) &&
-          __hip_atomic_load(&sup[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > rp)
-        atomicMin(&sup[r], rp);
+    __syncthreads();
+  }
+  int best = kNoSup;
+  if (act) {
+    // global path: a box touching more than 16 tiles reads the coarse centre masks
+    const unsigned long long *cm = g.ctr;
+    int rowlen = g.gx, ntl = nt, qy0 = ty0, qy1 = ty1, qx0 = tx0, qx1 = tx1;
+    if (!staged && (ty1 - ty0 + 1) * (tx1 - tx0 + 1) > 16) {
+      qy0 >>= 2; qy1 >>= 2; qx0 >>= 2; qx1 >>= 2;
+      cm = g.ctrc;
+      rowlen = (g.gx + 3) >> 2;
+      ntl = nms_coarse_tiles(g);
     }
+    const int cy = c / g.gx, lc = (cy - RY0) * rw + (c - cy * g.gx - RX0);
+    bool done = false;
+    for (int w = 0; w < nw && !done; w++) {
+      unsigned long long bits;
+      if (staged) {
+        const unsigned long long *e = s_m + (size_t)(2 * w) * nrect, *m = e + nrect;
+        bits = e[lc];
+        for (int ty = qy0; ty <= qy1; ty++) {
+          const unsigned long long *row = m + (ty - RY0) * rw - RX0;
+          for (int tx = qx0; tx <= qx1; tx++) bits |= row[tx];
+        }
+      } else {
+        bits = g.ext[(size_t)w * nt + c];
+        const unsigned long long *m = cm + (size_t)w * ntl;
+        for (int ty = qy0; ty <= qy1; ty++)
+          for (int tx = qx0; tx <= qx1; tx++) bits |= m[ty * rowlen + tx];
+      }
+      while (bits && !done) {
+        const int pl = w * 64 + __builtin_ctzll(bits);
+        bits &= bits - 1;
+        NmsBox P;
+        if (dense) {
+          const float4 q = s_b[pl];
+          P.y1 = q.x;
+          P.x1 = q.y;
+          P.y2 = q.z;
+          P.x2 = q.w;
+          P.area = s_a[pl];
+          P.pad = s_r[pl];
+        } else {
+          P = g.pbox[pl];
+        }
+        if (P.pad >= r) {
+          done = true;  // this pick and every later one rank after the box
+        } else if (suppresses(P, B, overlap)) {
+          best = P.pad;
+          done = true;
+        }
+      }
+    }
+    if (best != kNoSup) {
+      sup[r] = best;
+      g.dec[i] = 1;
+    }
+  }
+  // the boxes still undecided, in thread order, to the workgroup's places in slot
+  const bool keep = act && best == kNoSup;
+  const unsigned long long bal = __ballot(keep);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) s_cnt[wave] = __popcll(bal);
+  __syncthreads();
+  if (keep) {
+    int pos = blockIdx.x * kSupThreads + __popcll(bal & ((1ull << lane) - 1ull));
+    for (int w = 0; w < wave; w++) pos += s_cnt[w];
+    g.slot[pos] = i;
+  }
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int w = 0; w < kSupWaves; w++) tot += s_cnt[w];
+    g.bsum[blockIdx.x] = tot;
   }
 }
 
 
 constexpr int kNmsCompactBlock = 1024;  // positions per workgroup of the per-round counts
 
-// End of a round, fused: every alive position t < na is either decided (a window box, or a
-// later box some pick suppressed), whose score cell then leaves rank_of_flat (-1, when the
-// map path is on), or still alive.  Each workgroup counts the alive later boxes of its
-// kNmsCompactBlock positions (4 consecutive per thread) into blocksum; positions past the
-// true count add nothing, so a grid over a host bound counts exactly.
-
+// End of a round: every alive position t < na is either decided (a window box, or a later
+// box some pick suppressed) or still alive.  Each workgroup counts the alive later boxes of
+// its kNmsCompactBlock positions (4 consecutive per thread) into blocksum; positions past the
+// true count add nothing, so a grid over a host bound counts exactly.  With a grid, the
+// workgroups also clear its masks for the next round's index (the suppression pass that read
+// them has finished: the words the round's picks used), and workgroups past the first nb
+// compact the boxes the pass left undecided into the next round's candidate list.
 __global__ __launch_bounds__(256) void k_nms_post(const int *__restrict__ alive,
                                                   const int *__restrict__ nalive,
                                                   const int *__restrict__ sup,
-                                                  const NmsBox *__restrict__ sbox,
-                                                  const int *__restrict__ cand,
-                                                  int *__restrict__ rank_of_flat,
-                                                  int *__restrict__ blocksum) {
+                                                  const int *__restrict__ counters, NmsGrid g,
+                                                  int nb, int *__restrict__ blocksum) {
   __shared__ int s_w[4];
+  if (g.ext) {
+    const int nt = g.gx * g.gy, nct = nms_coarse_tiles(g);
+    const int nw = (counters[0] - counters[1] + 63) >> 6;
+    const int step = gridDim.x * blockDim.x;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nw * nt; k += step) {
+      g.ext[k] = 0ull;
+      g.ctr[k] = 0ull;
+    }
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nw * nct; k += step) g.ctrc[k] = 0ull;
+    // workgroups nb and up: the suppression pass's workgroup b = blockIdx.x - nb moves its
+    // undecided boxes to the next list, after those of the workgroups before it
+    if ((int)blockIdx.x >= nb) {
+      const int b = blockIdx.x - nb;
+      const int before = preceding_count(g.bsum, b, s_w);
+      const int kb = g.bsum[b];
+      for (int k = threadIdx.x; k < kb; k += blockDim.x)
+        g.list[before + k] = g.slot[b * kSupThreads + k];
+      if (b == (int)gridDim.x - nb - 1 && threadIdx.x == 0) *g.lcount = before + kb;
+      return;
+    }
+  }
   const int na = *nalive;
   const int W = min(na, kNmsWindow);
   const int t0 = blockIdx.x * kNmsCompactBlock + threadIdx.x * 4;
@@ -536,24 +672,8 @@ __global__ __launch_bounds__(256) void k_nms_post(const int *__restrict__ alive,
 #pragma unroll
   for (int k = 0; k < 4; k++) su[k] = r[k] >= 0 ? sup[r[k]] : kNoSup;
   int c = 0;
-  bool ret[4];
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const bool stay = r[k] >= 0 && t0 + k >= W && su[k] == kNoSup;
-    c += stay ? 1 : 0;
-    ret[k] = r[k] >= 0 && !stay;
-  }
-  if (rank_of_flat) {
-    int ci[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) ci[k] = ret[k] ? sbox[r[k]].idx : -1;
-    int fl[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) fl[k] = ci[k] >= 0 ? cand[ci[k]] : -1;
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-      if (fl[k] >= 0) rank_of_flat[fl[k]] = -1;
-  }
+  for (int k = 0; k < 4; k++) c += (r[k] >= 0 && t0 + k >= W && su[k] == kNoSup) ? 1 : 0;
   for (int m = 1; m < 64; m <<= 1) c += __shfl_xor(c, m);
   if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
   __syncthreads();
@@ -698,16 +818,52 @@ int launch_nms_sort(hipStream_t s, void *temp, size_t temp_bytes, unsigned int *
 }
 
 void launch_nms_gather(hipStream_t s, const int *ord, const NmsBox *box, int n, NmsBox *sbox,
-                       int *alive, int *status, int *sup, const int *cand, int *rank_of_flat,
-                       int *counters, int *nalive0, int *flags, int *keep) {
-  hipLaunchKernelGGL(k_nms_gather, dim3((n + 255) / 256), dim3(256), 0, s, ord, box, n, sbox,
-                     alive, status, sup, cand, rank_of_flat, counters, nalive0, flags, keep);
+                       int *alive, int *status, int *sup, int *counters, int *nalive0,
+                       int *flags, int *keep, const NmsGrid *grid) {
+  NmsGrid g;
+  if (grid) g = *grid;
+  const int threads = grid ? std::max(n, kNmsMaskWords * nms_grid_tiles(g)) : n;
+  hipLaunchKernelGGL(k_nms_gather, dim3((threads + 255) / 256), dim3(256), 0, s, ord, box, n,
+                     sbox, alive, status, sup, counters, nalive0, flags, keep, g);
+}
+
+void nms_grid_shape(int W, int H, int *gx, int *gy, int *shift) {
+  int sh = 6;  // 64-pixel tiles, coarser when the frame would need more than kNmsMaxTiles
+  for (;;) {
+    const long long x = ((long long)std::max(W, 1) + (1LL << sh) - 1) >> sh;
+    const long long y = ((long long)std::max(H, 1) + (1LL << sh) - 1) >> sh;
+    if (x * y <= kNmsMaxTiles) {
+      *gx = (int)x;
+      *gy = (int)y;
+      *shift = sh;
+      return;
+    }
+    sh++;
+  }
+}
+
+size_t nms_grid_bytes(int gx, int gy) {
+  NmsGrid g;
+  g.gx = gx;
+  g.gy = gy;
+  return sizeof(NmsBox) * kNmsWindow +
+         sizeof(unsigned long long) * kNmsMaskWords * (size_t)nms_grid_tiles(g);
+}
+
+// layout: pick boxes, then the extent, centre and coarse centre masks (contiguous, so the
+// gather kernel clears them as one array)
+void nms_grid_bind(NmsGrid &g, void *buf) {
+  const size_t nt = (size_t)g.gx * g.gy;
+  g.pbox = (NmsBox *)buf;
+  g.ext = (unsigned long long *)(g.pbox + kNmsWindow);
+  g.ctr = g.ext + kNmsMaskWords * nt;
+  g.ctrc = g.ctr + kNmsMaskWords * nt;
 }
 
 int launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const int *nalive,
                      int nalive_host, float overlap, unsigned long long *mask, int *status,
                       int *sup, int *picks, int *counters, int *blocksum,
-                      const NmsMaps *maps) {
+                      const NmsGrid *grid, int round) {
   const int W = nalive_host < kNmsWindow ? nalive_host : kNmsWindow;
   const size_t glds = sizeof(unsigned long long) * kNmsWindow * (kNmsWindow / 64) + 4 * kNmsWindow;
   // set once per process (a function-local static is initialised once, also when serving
@@ -725,21 +881,23 @@ int launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const 
                      sup, picks, counters, rest <= 0 ? 1 : 0);
   if (rest <= 0) return 0;
   const int nb = (nalive_host + kNmsCompactBlock - 1) / kNmsCompactBlock;
-  if (maps) {
-    // s_off 8 B, seven 4-byte tables and s_beg (nmaps + 1 entries) per map
-    const size_t slds = 40 * (size_t)maps->nmaps + 4;
-    hipLaunchKernelGGL(k_nms_scatter, dim3(W, kNmsScatterSplit), dim3(256), slds, s, sbox, picks,
-                       counters, maps->maps,
-                       maps->nmaps, maps->rsize, maps->padx, maps->pady, maps->rank_of_flat,
-                       overlap, sup);
+  int nbc = 0;  // suppression workgroups whose undecided boxes the round's end compacts
+  if (grid) {
+    hipLaunchKernelGGL(k_nms_grid_build, dim3((W + 3) / 4), dim3(256), 0, s, sbox, picks,
+                       counters, *grid, round);
+    // round 0 takes every candidate; a later round's list holds the alive boxes, at most the
+    // host's bound
+    nbc = ((round == 0 ? grid->n : std::min(grid->n, nalive_host)) + kSupThreads - 1) /
+          kSupThreads;
+    hipLaunchKernelGGL(k_nms_grid_sup, dim3(nbc), dim3(kSupThreads), 0, s, counters, *grid,
+                       overlap, round, sup);
   } else {
     hipLaunchKernelGGL(k_nms_suppress, dim3((nalive_host + 255) / 256), dim3(256), 0, s, sbox,
                        alive, nalive, picks, counters, overlap, sup);
   }
-  const bool retire = maps && maps->cand;
-  hipLaunchKernelGGL(k_nms_post, dim3(nb), dim3(256), 0, s, alive, nalive, sup, sbox,
-                     retire ? maps->cand : nullptr, retire ? maps->rank_of_flat : nullptr,
-                     blocksum);
+  NmsGrid none;
+  hipLaunchKernelGGL(k_nms_post, dim3(nb + nbc), dim3(256), 0, s, alive, nalive, sup, counters,
+                     grid ? *grid : none, nb, blocksum);
   return 0;
 }
 

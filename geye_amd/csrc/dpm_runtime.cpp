@@ -522,7 +522,7 @@ int plan_upload(Plan &P, hipStream_t st) {
 void plan_release(Plan &P) {
   for (auto &kv : P.graphs) (void)hipGraphExecDestroy(kv.second);
   P.graphs.clear();
-  for (auto *v : {&P.b_small, &P.b_mask, &P.b_blocksum, &P.b_sort})
+  for (auto *v : {&P.b_small, &P.b_mask, &P.b_blocksum, &P.b_sort, &P.b_grid, &P.b_acount})
     for (DeviceBuffer &b : *v) release(b);
   for (hipEvent_t e : P.b_ev) (void)hipEventDestroy(e);
   P.b_ev.clear();
@@ -540,7 +540,9 @@ void plan_release(Plan &P) {
                           &P.n_status, &P.n_fsup, &P.n_picks, &P.n_counters, &P.n_mask,
                           &P.n_sort, &P.n_flags, &P.n_keep_orig,
                           &P.n_kept, &P.n_total, &P.n_vals, &P.n_ord,
-                          &P.n_rank, &P.d_ascore_tab, &P.d_ascore, &P.d_grad, &P.d_grid,
+                          &P.n_rank, &P.n_dec, &P.n_alist, &P.n_acount, &P.n_grid, &P.d_ascore_tab,
+                          &P.d_ascore,
+                          &P.d_grad, &P.d_grid,
                           &P.d_qtiles0, &P.d_qtiles1, &P.d_qcounter, &P.d_fused})
     release(*b);
 }
@@ -633,6 +635,8 @@ int get_batch_plan(geye_ctx *c, Model &m, int W, int H, int nb, int feat_len, Pl
   P->b_mask.resize(nb);
   P->b_blocksum.resize(nb);
   P->b_sort.resize(nb);
+  P->b_grid.resize(nb);
+  P->b_acount.resize(nb);
   for (int f = 0; f < nb && rc == GEYE_OK; f++) {
     rc = ensure(P->b_small[f], 64);
     if (rc == GEYE_OK)
@@ -996,12 +1000,11 @@ struct NmsJob {
   unsigned long long *mask = nullptr;
   void *sort_temp = nullptr;
   size_t sort_bytes = 0;
-  bool use_maps = false;
-  NmsMaps maps{};
-  const int *cand = nullptr;        // candidate -> flat score index, segment base (maps path)
-  long long rank_off = 0, rank_n = 0;  // the score cells of rank_of_flat this job owns
+  bool use_grid = false;  // spatial index of each round's picks (overlap >= 0.5)
+  NmsGrid grid{};
+  const int *cand = nullptr;  // candidate -> flat score index, segment base (batch rows)
   // state of the rounds
-  int na = 0, cur = 0, slot = 0, nk = 0;
+  int na = 0, cur = 0, slot = 0, nk = 0, round = 0;
   int *alive = nullptr, *alive_next = nullptr;
   bool done = true, waiting = false;
   template <typename T>
@@ -1026,20 +1029,20 @@ int nms_job_start(geye_ctx *c, NmsJob &J, float overlap) {
                         J.seg<int>(P.n_ord), J.n) != 0)
       return fail(GEYE_ERR_HIP, "rocprim radix sort failed");
   }
-  if (J.use_maps) launch_fill(s, J.maps.rank_of_flat + J.rank_off, J.rank_n, -1);
   J.cur = 0;
   J.slot = 0;
   launch_nms_gather(s, J.seg<int>(P.n_ord), J.seg<NmsBox>(P.n_box), J.n, J.seg<NmsBox>(P.n_sbox),
-                    J.seg<int>(P.n_alive0), J.seg<int>(P.n_status), J.seg<int>(P.n_fsup), J.cand,
-                    J.use_maps ? J.maps.rank_of_flat : nullptr, J.counters, J.na_buf(0), J.flags,
-                    J.seg<int>(P.n_keep_orig));
+                    J.seg<int>(P.n_alive0), J.seg<int>(P.n_status), J.seg<int>(P.n_fsup),
+                    J.counters, J.na_buf(0), J.flags, J.seg<int>(P.n_keep_orig),
+                    J.use_grid ? &J.grid : nullptr);
   J.alive = J.seg<int>(P.n_alive0);
   J.alive_next = J.seg<int>(P.n_alive1);
   J.na = J.n;
+  J.round = 0;
   KTimer t(c, "nms_round", s);
   if (launch_nms_round(s, J.seg<NmsBox>(P.n_sbox), J.alive, J.na_buf(J.cur), J.na, overlap, J.mask,
                        J.seg<int>(P.n_status), J.seg<int>(P.n_fsup), J.seg<int>(P.n_picks),
-                       J.counters, J.blocksum, J.use_maps ? &J.maps : nullptr) != 0)
+                       J.counters, J.blocksum, J.use_grid ? &J.grid : nullptr, 0) != 0)
     return fail(GEYE_ERR_HIP, "NMS greedy kernel: dynamic LDS attribute not set");
   return GEYE_OK;
 }
@@ -1061,11 +1064,12 @@ int nms_job_enqueue(geye_ctx *c, NmsJob &J, float overlap) {
   HIPCHK(hipEventRecord(J.ev[J.slot], J.s));
   std::swap(J.alive, J.alive_next);
   J.cur ^= 1;
+  J.round++;
   KTimer t(c, "nms_round", J.s);
   if (launch_nms_round(J.s, J.seg<NmsBox>(P.n_sbox), J.alive, J.na_buf(J.cur), rest, overlap,
                        J.mask, J.seg<int>(P.n_status), J.seg<int>(P.n_fsup),
                        J.seg<int>(P.n_picks), J.counters, J.blocksum,
-                       J.use_maps ? &J.maps : nullptr) != 0)
+                       J.use_grid ? &J.grid : nullptr, J.round) != 0)
     return fail(GEYE_ERR_HIP, "NMS greedy kernel: dynamic LDS attribute not set");
   J.waiting = true;
   return GEYE_OK;
@@ -1154,7 +1158,7 @@ int nms_run(geye_ctx *c, std::vector<NmsJob> &jobs, float overlap) {
 
 // The whole candidate set of a one-frame plan as one job on the context's stream, with the
 // context's events and mapped host slots.
-NmsJob single_job(geye_ctx *c, Plan &P, int n, const NmsMaps *maps, const int *cand) {
+NmsJob single_job(geye_ctx *c, Plan &P, int n, const NmsGrid *grid) {
   NmsJob J;
   J.P = &P;
   J.off = 0;
@@ -1178,22 +1182,42 @@ NmsJob single_job(geye_ctx *c, Plan &P, int n, const NmsMaps *maps, const int *c
   J.mask = dp<unsigned long long>(P.n_mask);
   J.sort_temp = P.n_sort.p;
   J.sort_bytes = P.n_sort.bytes;
-  J.use_maps = maps != nullptr;
-  if (maps) J.maps = *maps;
-  J.cand = cand;
-  J.rank_off = 0;
-  J.rank_n = P.score_total;
+  J.use_grid = grid != nullptr;
+  if (grid) J.grid = *grid;
   return J;
 }
 
+// The indexed suppression pass's buffers for the n candidates of a one-frame plan (P.n_box
+// filled) over a W x H frame: each candidate's rank and decided flag, and the tile lists.
+// Only for overlap >= 0.5, where the index is complete.
+int nms_grid_buffers(Plan &P, int n, int W, int H, NmsGrid *g) {
+  nms_grid_shape(W, H, &g->gx, &g->gy, &g->shift);
+  const size_t N = (size_t)std::max(n, 1);
+  CHK(ensure(P.n_rank, sizeof(int) * N));
+  CHK(ensure(P.n_dec, N));
+  const size_t NB = (N + 1023) / 1024;
+  CHK(ensure(P.n_alist, sizeof(int) * (N + 1024 * NB)));       // list, then slot
+  CHK(ensure(P.n_acount, sizeof(int) * (4 * NB + 64)));         // lcount, then bsum
+  CHK(ensure(P.n_grid, nms_grid_bytes(g->gx, g->gy)));
+  nms_grid_bind(*g, P.n_grid.p);
+  g->rank_of = dp<int>(P.n_rank);
+  g->dec = (unsigned char *)P.n_dec.p;
+  g->box = dp<NmsBox>(P.n_box);
+  g->list = dp<int>(P.n_alist);
+  g->slot = g->list + N;
+  g->lcount = dp<int>(P.n_acount);
+  g->bsum = g->lcount + 64;
+  g->n = n;
+  return GEYE_OK;
+}
+
 // Greedy NMS over P.n_box / P.n_keys / P.n_vals (filled by a prep kernel, original order);
-// leaves the survivors' original indices, ascending, in P.n_kept.  maps (optional) enables
-// the scatter form of the suppression pass; cand/rank_of_flat go with it.
-int nms_core(geye_ctx *c, Plan &P, int n, float overlap, const NmsMaps *maps, const int *cand,
-             int *nkept) {
+// leaves the survivors' original indices, ascending, in P.n_kept.  grid (optional) selects
+// the indexed suppression pass.
+int nms_core(geye_ctx *c, Plan &P, int n, float overlap, const NmsGrid *grid, int *nkept) {
   *nkept = 0;
   if (n <= 0) return GEYE_OK;
-  std::vector<NmsJob> jobs{single_job(c, P, n, maps, cand)};
+  std::vector<NmsJob> jobs{single_job(c, P, n, grid)};
   CHK(nms_run(c, jobs, overlap));
   *nkept = jobs[0].nk;
   return GEYE_OK;
@@ -1212,15 +1236,11 @@ int run_nms(geye_ctx *c, Model &m, Plan &P, int n, float overlap, int *nkept) {
                     dp<int>(m.d_rsize), dp<NmsBox>(P.n_box), dp<unsigned int>(P.n_keys),
                     dp<int>(P.n_vals));
   }
-  NmsMaps mp{};
-  const bool use_maps = (int)P.smaps.size() <= kNmsMaxMaps;
-  if (use_maps) {
-    CHK(ensure(P.n_rank, sizeof(int) * (size_t)std::max<long long>(P.score_total, 1)));
-    mp = NmsMaps{dp<ScoreMap>(P.d_smaps), (int)P.smaps.size(), dp<int>(m.d_rsize), m.padx,
-                 m.pady, dp<int>(P.n_rank), dp<int>(P.d_cand)};
-  }
+  NmsGrid g;
+  const bool use_grid = overlap >= 0.5f;
+  if (use_grid) CHK(nms_grid_buffers(P, n, P.W, P.H, &g));
   int nk = 0;
-  CHK(nms_core(c, P, n, overlap, use_maps ? &mp : nullptr, dp<int>(P.d_cand), &nk));
+  CHK(nms_core(c, P, n, overlap, use_grid ? &g : nullptr, &nk));
   HIPCHK(hipGetLastError());
   *nkept = nk;
   return GEYE_OK;
@@ -1507,7 +1527,9 @@ int detect_batch(geye_ctx *c, const geye_model *const *models, int nm,
     Plan &B = *P[i];
     const int ncand = B.score_total > 0 ? ((volatile int *)c->h_count)[count_slot(i)] : 0;
     CHK(nms_buffers(B, std::max(ncand, 1)));
-    CHK(ensure(B.n_rank, sizeof(int) * (size_t)std::max<long long>(B.score_total, 1)));
+    NmsGrid g0;
+    const bool use_grid = overlap >= 0.5f;
+    if (use_grid) CHK(nms_grid_buffers(B, std::max(ncand, 1), W, H, &g0));
     const volatile int *off = (const volatile int *)B.b_host + 4 * nf;
     for (int f = 0; f < nf; f++) {
       NmsJob J;
@@ -1535,17 +1557,31 @@ int detect_batch(geye_ctx *c, const geye_model *const *models, int nm,
       J.mask = dp<unsigned long long>(B.b_mask[f]);
       J.sort_temp = B.b_sort[f].p;
       J.sort_bytes = B.b_sort[f].bytes;
-      J.use_maps = B.nmaps_one <= kNmsMaxMaps;
       J.cand = dp<int>(B.d_cand) + J.off;
-      J.maps = NmsMaps{dp<ScoreMap>(B.d_smaps) + (size_t)f * B.nmaps_one, B.nmaps_one,
-                       dp<int>(m.d_rsize), m.padx, m.pady, dp<int>(B.n_rank), J.cand};
-      J.rank_off = (long long)f * B.score_one;
-      J.rank_n = B.score_one;
+      // frame f's grid: its own tile lists, its segment of the rank and decided arrays
+      J.use_grid = use_grid;
+      if (use_grid) {
+        J.grid = g0;
+        CHK(ensure(B.b_grid[f], nms_grid_bytes(g0.gx, g0.gy)));
+        nms_grid_bind(J.grid, B.b_grid[f].p);
+        J.grid.rank_of = g0.rank_of + J.off;
+        J.grid.dec = g0.dec + J.off;
+        J.grid.box = g0.box + J.off;
+        // the job's own list, slots and counts
+        const size_t nbj = ((size_t)std::max(J.n, 1) + 1023) / 1024;
+        CHK(ensure(B.b_acount[f], sizeof(int) * (2 * 1024 * nbj + 4 * nbj + 64)));
+        J.grid.list = dp<int>(B.b_acount[f]);
+        J.grid.slot = J.grid.list + 1024 * nbj;
+        J.grid.lcount = J.grid.slot + 1024 * nbj;
+        J.grid.bsum = J.grid.lcount + 64;
+        J.grid.n = J.n;
+      }
       if (J.n > 0) {
         KTimer t(c, "nms_prep", J.s);
         launch_nms_prep(J.s, dp<ScoreMap>(B.d_smaps), (int)B.smaps.size(), dp<float>(B.d_score),
-                        J.cand, J.n, m.padx, m.pady, dp<int>(m.d_rsize), J.seg<NmsBox>(B.n_box),
-                        J.seg<unsigned int>(B.n_keys), J.seg<int>(B.n_vals));
+                        J.cand, J.n, m.padx, m.pady, dp<int>(m.d_rsize),
+                        J.seg<NmsBox>(B.n_box), J.seg<unsigned int>(B.n_keys),
+                        J.seg<int>(B.n_vals));
       }
       jobs.push_back(J);
       who.push_back({f, i});
@@ -2145,7 +2181,22 @@ int geye_nms(geye_ctx *c, const float *boxes, int num, int GL, float overlap, fl
   if (rc == GEYE_OK) {
     launch_nms_prep_rows(c->stream, dp<float>(rows), num, GL, dp<NmsBox>(B.n_box),
                          dp<unsigned int>(B.n_keys), dp<int>(B.n_vals));
-    rc = nms_core(c, B, num, overlap, nullptr, nullptr, &nk);
+    // the indexed pass needs overlap >= 0.5 and boxes with x2 >= x1, y2 >= y1 (then a box
+    // that contains another also intersects it); the frame is the boxes' extent
+    bool grid_ok = overlap >= 0.5f;
+    float ex = 0.0f, ey = 0.0f;
+    for (int i = 0; i < num && grid_ok; i++) {
+      const float *r = boxes + (size_t)i * GL;
+      grid_ok = r[2] >= r[0] && r[3] >= r[1] && std::isfinite(r[2]) && std::isfinite(r[3]) &&
+                std::isfinite(r[0]) && std::isfinite(r[1]);
+      ey = std::max(ey, r[2]);
+      ex = std::max(ex, r[3]);
+    }
+    NmsGrid g;
+    if (grid_ok)
+      rc = nms_grid_buffers(B, num, (int)std::min(ex + 2.0f, 1e6f), (int)std::min(ey + 2.0f, 1e6f),
+                            &g);
+    if (rc == GEYE_OK) rc = nms_core(c, B, num, overlap, grid_ok ? &g : nullptr, &nk);
   }
   std::vector<int> kept(nk);
   if (rc == GEYE_OK && nk > 0) {

@@ -240,8 +240,8 @@ def test_car_detection_synthetic(ctx):
 
 @pytest.mark.parametrize("overlap", [0.3, 0.5, 0.7, 0.95])
 def test_car_detection_overlaps(ctx, overlap):
-    """Device NMS in the detect path (score-map scatter suppression) against the oracle at
-    several overlaps, from loose to nearly disjoint-only."""
+    """Device NMS in the detect path against the oracle at several overlaps, from loose (the
+    every-pick suppression pass) to nearly disjoint-only (the tile-indexed pass, 0.5 and up)."""
     say("car_detection synthetic car 480x360 overlap", overlap)
     m = synth.synthetic_car()
     frame = synth.make_frame(480, 360, seed=13)
@@ -279,6 +279,52 @@ def test_nms_matches_oracle(ctx, n, ties):
     rows = _random_boxes(n, seed=n + ties, ties=ties)
     g = ctx.nms(rows, 0.5)
     o = O.nms(rows, 0.5)
+    _rows_equal(g, o)
+
+
+def _hard_geometry_boxes(seed, GL=11):
+    """Boxes built to stress the tile index of the suppression pass: a dense cluster of small
+    boxes in one tile (so windows hold hundreds of picks and every mask word is used), boxes
+    reaching far outside the boxes' extent on every side (negative coordinates included),
+    boxes as large as the whole extent, thin boxes, boxes nested in one another, and exact
+    duplicates of other boxes with their own scores."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    # cluster: 1500 boxes of 3..9 px inside a 60 x 60 px area (mostly disjoint at 0.5)
+    n = 1500
+    y = rng.uniform(300, 360, n); x = rng.uniform(500, 560, n)
+    h = rng.integers(3, 10, n); w = rng.integers(3, 10, n)
+    parts.append(np.stack([y, x, y + h - 1, x + w - 1], 1))
+    # reaching outside: centres near the border, sizes up to 900 px
+    n = 400
+    y = rng.uniform(-300, 1000, n); x = rng.uniform(-300, 1800, n)
+    h = rng.integers(20, 900, n); w = rng.integers(20, 900, n)
+    parts.append(np.stack([y, x, y + h - 1, x + w - 1], 1))
+    # whole-extent and thin boxes
+    parts.append(np.array([[-50.0, -50.0, 1100.0, 1900.0], [0.0, 0.0, 1079.0, 1919.0],
+                           [10.0, 0.0, 12.0, 1919.0], [0.0, 700.0, 1079.0, 701.0]]))
+    # nested: shrinking squares around one point
+    k = np.arange(1, 60, dtype=np.float64)
+    parts.append(np.stack([500 - 4 * k, 900 - 4 * k, 500 + 4 * k, 900 + 4 * k], 1))
+    b = np.concatenate(parts).astype(np.float32)
+    dup = b[rng.integers(0, len(b), 200)]
+    b = np.concatenate([b, dup])
+    rows = np.zeros((len(b), GL), np.float32)
+    rows[:, :4] = b
+    rows[:, GL - 3] = rng.integers(0, 6, len(b))
+    rows[:, GL - 2] = rng.permutation(len(b)).astype(np.float32) * 0.001 - 0.5  # distinct
+    rows[:, GL - 1] = 8.0
+    return rows
+
+
+@pytest.mark.parametrize("overlap", [0.5, 0.6, 0.9, 0.999])
+def test_nms_indexed_pass_hard_geometry(ctx, overlap):
+    """The tile-indexed suppression pass (overlap >= 0.5) on geometry built against it, bit
+    for bit against the oracle, scores distinct (so the device path must decide everything)."""
+    say("nms hard geometry overlap", overlap)
+    rows = _hard_geometry_boxes(seed=int(overlap * 1000))
+    g = ctx.nms(rows, overlap)
+    o = O.nms(rows, overlap)
     _rows_equal(g, o)
 
 
