@@ -166,6 +166,35 @@ def test_dt_window_matches_oracle(ctx, d0, d1, defs, scale, ties):
     assert np.array_equal(Ix, oIx) and np.array_equal(Iy, oIy)
 
 
+@pytest.mark.parametrize("kind", ["ramp", "wells", "spike", "vertex", "checker"])
+def test_dt_window_structured_maps(ctx, kind):
+    """The windowed DT on structured maps: smooth ramps (argmins at the window's edge or
+    centre), isolated deep wells (distant sources win across a flat map), a single spike far
+    below a line, b large against a (the cost's vertex away from the centre), and a checker of
+    two values (exact ties at every offset).  All bit for bit against the oracle."""
+    say("dt window structured", kind)
+    rng = np.random.default_rng(len(kind))
+    d0, d1 = 97, 83
+    defs = (0.01, 0.0, 0.01, 0.0)
+    if kind == "ramp":
+        v = (np.add.outer(np.arange(d1), np.arange(d0)) * 0.003).astype(np.float32)
+    elif kind == "wells":
+        v = np.zeros((d1, d0), np.float32)
+        v[rng.integers(0, d1, 40), rng.integers(0, d0, 40)] = -0.9
+    elif kind == "spike":
+        v = (rng.standard_normal((d1, d0)) * 0.05).astype(np.float32)
+        v[::7, 11] = -1.4
+    elif kind == "vertex":
+        v = (rng.standard_normal((d1, d0)) * 0.2).astype(np.float32)
+        defs = (0.01, 0.09, 0.012, -0.07)
+    else:
+        v = np.where(np.add.outer(np.arange(d1), np.arange(d0)) % 2 == 0, 0.25, -0.25).astype(np.float32)
+    M, Ix, Iy = ctx.dt(v, *defs, window=True)
+    oM, oIx, oIy = O.dt(v, *defs)
+    assert bits_equal(M, oM)
+    assert np.array_equal(Ix, oIx) and np.array_equal(Iy, oIy)
+
+
 def test_dt_window_matches_reference_vectors(ctx):
     """The windowed DT on the reference's own dt() vectors (random maps, plateaus, ties,
     all-zero maps with a = 0, large coefficients)."""
