@@ -704,44 +704,60 @@ __device__ inline void dt_scan(const float *ls, float a, float b, int d, int p0,
 
 // Stages the job's lines into LDS, line l at s_src[l * LS + H0 + p], negated when the job
 // says so.  MINMAX also returns this thread's minimum and maximum of the staged values.
+// The dimension contiguous in memory (positions of a line, or the lines at one position) is
+// read four elements per load (16 bytes at 4-byte alignment, which gfx950 global loads
+// accept), U such loads in flight per thread, so a job's lines arrive in few round trips.
+struct __attribute__((packed, aligned(4))) DtQuad {
+  float v0, v1, v2, v3;
+};
+
 template <bool MINMAX>
 __device__ inline void dt_stage(const DtJob &J, const float *__restrict__ src_arena, float *s_src,
                                 int LS, int H0, float &vmin, float &vmax) {
   const int n = J.n, tid = threadIdx.x;
   const float *src = src_arena + J.src_off + (long long)J.line0 * J.line_stride;
-  // element i of the job is (line l, position p), p fastest for lines contiguous in memory
-  // and l fastest otherwise; (l, p) advance by 256 elements without a division
-  // U loads are issued before the first is used, so their latencies overlap
-  constexpr int U = 8;
+  constexpr int U = 4;
   const bool cols = J.elem_stride == 1;
-  const int inner = cols ? n : J.nlines;
-  const int total = J.nlines * n;
-  const int sq = 256 / inner, so = 256 - sq * inner;
-  int q = tid / inner, o = tid - q * inner;
-  for (int i0 = tid; i0 < total; i0 += 256 * U) {
-    float v[U];
-    int at[U];
+  // inner: the contiguous dimension (positions for columns, lines for rows); outer: the other
+  const int inner = cols ? n : J.nlines, outer = cols ? J.nlines : n;
+  const long long ostride = cols ? J.line_stride : J.elem_stride;
+  const int qn = (inner + 3) >> 2, total = outer * qn;
+  const float sgn = J.neg ? -1.0f : 1.0f;
+  for (int q0 = tid; q0 < total; q0 += 256 * U) {
+    float v[U][4];
+    int oo[U], ii[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int l = cols ? q : o, p = cols ? o : q;
-      const bool ok = i0 + u * 256 < total;
-      at[u] = ok ? l * LS + H0 + p : -1;
-      v[u] = ok ? src[(long long)l * J.line_stride + (long long)p * J.elem_stride] : 0.0f;
-      q += sq;
-      o += so;
-      if (o >= inner) {
-        o -= inner;
-        q++;
+      const int q = q0 + u * 256;
+      const int o = q < total ? q / qn : 0;
+      const int i = q < total ? 4 * (q - o * qn) : inner;  // i == inner: nothing to load
+      oo[u] = o;
+      ii[u] = i;
+      const float *p = src + (long long)o * ostride + i;
+      if (i + 3 < inner) {
+        const DtQuad t = *(const DtQuad *)p;
+        v[u][0] = t.v0;
+        v[u][1] = t.v1;
+        v[u][2] = t.v2;
+        v[u][3] = t.v3;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[u][k] = i + k < inner ? p[k] : 0.0f;
       }
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      if (at[u] < 0) continue;
-      const float x = J.neg ? v[u] * -1.0f : v[u];
-      s_src[at[u]] = x;
-      if (MINMAX) {
-        vmin = fminf(vmin, x);
-        vmax = fmaxf(vmax, x);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int i = ii[u] + k;
+        if (i >= inner) continue;
+        const int l = cols ? oo[u] : i, p = cols ? i : oo[u];
+        const float x = v[u][k] * sgn;
+        s_src[l * LS + H0 + p] = x;
+        if (MINMAX) {
+          vmin = fminf(vmin, x);
+          vmax = fmaxf(vmax, x);
+        }
       }
     }
   }
