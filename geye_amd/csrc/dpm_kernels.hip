@@ -174,7 +174,7 @@ __global__ __launch_bounds__(256) void k_resize_fused(const FusedResize *__restr
   const long long plane = (long long)op.in_h * op.in_w;
   const float *in = src + op.in_off + c0;
   const unsigned char *inb = bgr + (BGR ? op.frame * frame_bytes + 3LL * c0 : 0);
-  constexpr int U = 2;  // elements per thread per step
+  constexpr int U = 3;  // elements per thread per step
   for (int i0 = tid; i0 < ny * span; i0 += U * 256) {
     float v[U][3][MAXTAP];
     int rr[U], xx[U], nn[U];
@@ -323,13 +323,30 @@ __global__ __launch_bounds__(256) void k_hog_grad(const GradImg *__restrict__ im
   const int x0 = tx * TXP, y0 = ty * TYP;
   const int h = G.h, w = G.w;
   const float *img = img_arena + G.img_off;
-  for (int i = threadIdx.x; i < SY * SX; i += 256) {
-    const int r = i / SX, q = i - r * SX;
-    const int yy = min(max(y0 - 1 + r, 0), h - 1), xx = min(max(x0 - 1 + q, 0), w - 1);
-    const long long o = (long long)yy * w + xx;
-    s_t[0][i] = img[o];
-    s_t[1][i] = img[o + (long long)h * w];
-    s_t[2][i] = img[o + 2LL * h * w];
+  // every element's three loads are issued before the first store (a thread stages up to
+  // U elements), so the tile arrives in one round trip
+  constexpr int U = (SY * SX + 255) / 256;
+  {
+    float v[U][3];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = min((int)threadIdx.x + 256 * u, SY * SX - 1);
+      const int r = i / SX, q = i - r * SX;
+      const int yy = min(max(y0 - 1 + r, 0), h - 1), xx = min(max(x0 - 1 + q, 0), w - 1);
+      const long long o = (long long)yy * w + xx;
+      v[u][0] = img[o];
+      v[u][1] = img[o + (long long)h * w];
+      v[u][2] = img[o + 2LL * h * w];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = (int)threadIdx.x + 256 * u;
+      if (i < SY * SX) {
+        s_t[0][i] = v[u][0];
+        s_t[1][i] = v[u][1];
+        s_t[2][i] = v[u][2];
+      }
+    }
   }
   __syncthreads();
   const int ly = threadIdx.x & (TYP - 1);
