@@ -26,8 +26,11 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgeye_amd.so")
 
-GEYE_FP32, GEYE_BF16, GEYE_FP16 = 0, 1, 2
-PRECISIONS = {"fp32": GEYE_FP32, "bf16": GEYE_BF16, "fp16": GEYE_FP16}
+# convolution arithmetic: FP32 (the reference's), one MFMA per tap on BF16 / FP16 operands, or
+# three MFMAs per tap on hi/lo operand pairs (bf16x3, fp16x3)
+GEYE_FP32, GEYE_BF16, GEYE_FP16, GEYE_BF16X3, GEYE_FP16X3 = 0, 1, 2, 3, 4
+PRECISIONS = {"fp32": GEYE_FP32, "bf16": GEYE_BF16, "fp16": GEYE_FP16, "bf16x3": GEYE_BF16X3,
+              "fp16x3": GEYE_FP16X3}
 
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int)

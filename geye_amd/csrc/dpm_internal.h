@@ -249,8 +249,8 @@ struct Plan {
   size_t q_lds[2] = {0, 0};
   // autotuned MFMA schedule per precision (index 1 = bf16, 2 = fp16): -1 = not tuned yet,
   // 0 = one launch per class, else 1 + (MT index) * 8 + blocks per CU - 1
-  int mfma_choice[3] = {-1, -1, -1};
-  std::vector<std::pair<std::string, float>> mfma_tuning[3];
+  int mfma_choice[5] = {-1, -1, -1, -1, -1};
+  std::vector<std::pair<std::string, float>> mfma_tuning[5];
   std::vector<long long> map_off;      // output map slot -> offset in conv arena
   std::vector<int2> map_dims;          // output map slot -> (H, W)
   long long conv_total = 0;
@@ -321,7 +321,8 @@ struct Model {
   // least-squares box coefficients (x1, y1, x2, y2 per component), may be empty
   std::vector<float> lsq;
   std::vector<int> lsq_idx;                       // [offsets C][lengths C]
-  DeviceBuffer d_filters, d_rsize, d_filt_bf16, d_filt_f16, d_lsq, d_lsq_idx;
+  DeviceBuffer d_filters, d_rsize, d_lsq, d_lsq_idx;
+  DeviceBuffer d_filt16[5];  // MFMA filter packs, indexed by precision (GEYE_BF16 .. FP16X3)
 };
 
 struct KernelStat {

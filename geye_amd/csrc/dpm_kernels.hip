@@ -493,7 +493,10 @@ __global__ __launch_bounds__(64) void k_hog_votes(const HogLevel *__restrict__ l
 // 407-492).  The four normalisers are 1/sqrt((double)sum + 1e-4) rounded to float and the
 // texture features 0.2357*t in double, as the reference's promotions compute them.
 // Writes [31][F1][F0] float when feat_arena is non-null and, when featb_arena is non-null,
-// [F1][F0][32] 16-bit (the 16-bit forms read only the latter).
+// the MFMA forms' channel-last copy (they read only that): hkind 1 / 2 = [F1][F0][32] BF16 /
+// FP16 (the rounding of each feature, channel 31 = 0); hkind 3 / 4 = [F1][F0][64], the split
+// operand pair of each feature -- channels 0..31 hold hi = round16(f), channels 32..63 hold
+// lo = round16(f - hi) (f - hi is exact in FP32) -- at twice the cell offsets.
 // ---------------------------------------------------------------------------------------
 __device__ inline float min_2(float x) { return (x < 0.2f) ? x : 0.2f; }
 
@@ -503,7 +506,7 @@ __device__ inline float norm_factor(const float *p, int b0) {
 }
 
 __device__ inline unsigned short half_bits(float f, int kind) {
-  if (kind == 1) {
+  if (kind == 1 || kind == 3) {
     unsigned int u = __float_as_uint(f);
     u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite values)
     return (unsigned short)(u >> 16);
@@ -565,7 +568,7 @@ __global__ __launch_bounds__(256) void k_hog_feat(const HogLevel *__restrict__ l
 #pragma unroll
     for (int o = 0; o < 31; o++) dst[o * OSQ] = fv[o];
   }
-  if (featb_arena) {
+  if (featb_arena && hkind <= 2) {
     // 32 16-bit channels (the last one 0) as four 16-byte stores; featb_off is a multiple
     // of 32 elements, so every cell starts on a 64-byte boundary
     uint4 *db = (uint4 *)(featb_arena + L.featb_off + e * 32);
@@ -578,6 +581,27 @@ __global__ __launch_bounds__(256) void k_hog_feat(const HogLevel *__restrict__ l
         const unsigned int lo = half_bits(fv[c], hkind);
         const unsigned int hi = c + 1 < 31 ? half_bits(fv[c + 1], hkind) : 0u;
         w[r] = lo | (hi << 16);
+      }
+      db[q] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+  } else if (featb_arena) {
+    // the split pair: 64 channels as eight 16-byte stores, 128 bytes per cell
+    uint4 *db = (uint4 *)(featb_arena + 2 * L.featb_off + e * 64);
+    auto hl = [&](int c, bool want_lo) -> unsigned int {
+      if (c >= 31) return 0u;
+      const unsigned short h = half_bits(fv[c], hkind);
+      if (!want_lo) return h;
+      const float hv = hkind == 3 ? __uint_as_float((unsigned int)h << 16)
+                                  : __half2float(__ushort_as_half(h));
+      return half_bits(fv[c] - hv, hkind);
+    };
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      unsigned int w[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int c = (q & 3) * 8 + r * 2;
+        w[r] = hl(c, q >= 4) | (hl(c + 1, q >= 4) << 16);
       }
       db[q] = make_uint4(w[0], w[1], w[2], w[3]);
     }

@@ -5,18 +5,20 @@
 
 namespace geye {
 
-size_t mfma_lds_bytes(int fh, int fw, int nt);
+// LDS bytes of the per-class kernel with nbuf filter buffers (split: 8 chunks per cell).
+size_t mfma_lds_bytes(int fh, int fw, int nt, bool split, int nbuf);
 // Work-queue form: one launch for every tile of every filter class (tiles carry their class
 // shape), `blocks` persistent workgroups taking tiles from *counter (0 on entry) until the
 // list is exhausted; tiles are 4 columns x 16*MT rows.  lds = the largest class's need.
-size_t mfma_q_lds_bytes(int fh, int fw, int nt, int MT);
-int launch_conv_mfma_queue(hipStream_t s, int kind, int MT, int blocks, size_t lds,
+size_t mfma_q_lds_bytes(int fh, int fw, int nt, int MT, bool split);
+int launch_conv_mfma_queue(hipStream_t s, int kind, bool split, int MT, int blocks, size_t lds,
                            const MfmaTile *tiles, int ntiles, int *counter, const void *feat,
                            const void *filt, const long long *map_off, const int2 *map_dims,
                            float *out);
-// kind: 1 = bf16, 2 = fp16.  Returns 0 on success, -1 when the shape does not fit.
-int launch_conv_mfma(hipStream_t s, int kind, int nt, const MfmaTile *tiles, int ntiles, int fh,
-                     int fw, const void *feat, const void *filt, const long long *map_off,
-                     const int2 *map_dims, float *out);
+// kind: 1 = bf16, 2 = fp16; split: hi/lo operand pairs, three MFMAs per tap.  Returns 0 on
+// success, -1 when the shape does not fit.
+int launch_conv_mfma(hipStream_t s, int kind, bool split, int nt, const MfmaTile *tiles,
+                     int ntiles, int fh, int fw, const void *feat, const void *filt,
+                     const long long *map_off, const int2 *map_dims, float *out);
 
 }  // namespace geye

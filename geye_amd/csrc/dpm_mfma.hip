@@ -12,6 +12,12 @@
 // double-buffered.  Each wave owns one column: 4 M-tiles x NT N-tiles of accumulators.
 // Operand layout (per lane l of the wave): A row l%16, channels 8*(l/16)..+8; B filter
 // l%16, the same channels; accumulator rows 4*(l/16)..+4 of column l%16.
+//
+// Split forms (SPLIT = true): every operand x is the pair hi = round16(x), lo = round16(x -
+// hi).  A cell then holds 64 channels ([hi 32][lo 32], 8 chunks of 16 bytes) and a filter
+// tap 8 chunks per 16 filters; each tap runs three MFMAs into the same accumulator, hi*hi,
+// hi*lo, lo*hi (lo*lo, below 2^-16 of the product, is dropped).  CPC below is the number of
+// 16-byte chunks per cell or per filter tap: 4, or 8 when split.
 #include "dpm_mfma.h"
 
 #include <algorithm>
@@ -42,33 +48,53 @@ __device__ inline f32x4 mfma16(u32x4 a, u32x4 b, f32x4 c) {
 // gx mod AX, each slot AY = 64 + fh - 1 cells of 4 16-byte chunks.  The chunk index of cell c
 // (c = slot * AY + row) is XOR-swizzled by (c >> 1) & 3, so the 8 consecutive cells that 8
 // lanes read in one LDS cycle fall on 8 distinct 16-byte bank groups.
-__device__ inline int conv_cell_chunk(int cell, int part) { return cell * 4 + (part ^ ((cell >> 1) & 3)); }
+// With 8 chunks per cell (split forms) the chunk index is XOR-swizzled by cell & 7.
+template <int CPC>
+__device__ inline int conv_cell_chunk(int cell, int part) {
+  if constexpr (CPC == 4) return cell * 4 + (part ^ ((cell >> 1) & 3));
+  else return cell * 8 + (part ^ (cell & 7));
+}
 
-// Global chunk of feature column gx, chunk rem (= row * 4 + part) of the tile's rows; the
+// Global chunk of feature column gx, chunk rem (= row * CPC + part) of the tile's rows; the
 // load is unconditional from a clamped in-level address and `valid` says whether it is real.
+template <int CPC>
 __device__ inline u32x4 conv_feat_load(const u32x4 *__restrict__ F, const MfmaTile &t, int gx,
                                        int rem, bool &valid) {
-  const int gy = t.y0 + (rem >> 2);
+  const int gy = t.y0 + rem / CPC;
   valid = gx < t.F1 && gy < t.F0;
   const int cx = min(gx, t.F1 - 1), cy = min(gy, t.F0 - 1);
-  return F[(unsigned)((cx * t.F0 + cy) * 4 + (rem & 3))];   // a level holds < 2^31 chunks
+  return F[(unsigned)((cx * t.F0 + cy) * CPC + (rem & (CPC - 1)))];  // a level holds < 2^31 chunks
+}
+
+// Feature arena and filter pack of a tile in 16-byte chunks (the tile's offsets count the
+// 16-bit elements of the 32-channel forms; the split forms' cells and filter taps are twice
+// as long, at twice the offsets).
+template <int CPC>
+__device__ inline const u32x4 *conv_feat_base(const u32x4 *feat, const MfmaTile &t) {
+  return feat + t.featb_off * (CPC / 4) / 8;
+}
+template <int CPC>
+__device__ inline const u32x4 *conv_filt_base(const u32x4 *filt, const MfmaTile &t) {
+  return filt + t.filt_off * (CPC / 4) / 8;
 }
 
 // Registers per thread: PN chunks of the next tile's 4 new feature columns, PRE chunks of the
-// next tap column's filters.
-template <int NT> struct ConvRegs {
-  static constexpr int PN = 5;
-  static constexpr int PRE = NT == 3 ? 5 : 4;
+// next tap column's filters (NTH threads per workgroup).
+template <int NT, bool SPLIT, int NTH> struct ConvRegs {
+  static constexpr int PN = SPLIT ? (NTH == 512 ? 5 : 10) : 5;
+  static constexpr int PRE = SPLIT && NTH == 256 ? (NT == 3 ? 9 : 8) : (NT == 3 ? 5 : 4);
 };
 
 // Stages all AX feature columns of tile t into the ring (8 loads in flight per thread; a
-// column holds at least 256 chunks, so one step of 256 crosses at most one column boundary).
+// column holds at least NTH chunks -- AY >= 64 cells of CPC chunks, NTH <= 64 CPC -- so one
+// step of NTH crosses at most one column boundary).
+template <int CPC, int NTH>
 __device__ inline void conv_stage_tile(u32x4 *As, const u32x4 *__restrict__ feat,
                                        const MfmaTile &t, int AX, int AY, int tid) {
-  const int CS = AY * 4, NA = AX * CS;
-  const u32x4 *F = feat + t.featb_off / 8;     // featb_off counts 16-bit elements
+  const int CS = AY * CPC, NA = AX * CS;
+  const u32x4 *F = conv_feat_base<CPC>(feat, t);
   int c = 0, rem = tid, slot = t.x0 % AX;
-  for (int i0 = 0; i0 < NA; i0 += 8 * 256) {
+  for (int i0 = 0; i0 < NA; i0 += 8 * NTH) {
     u32x4 v[8];
     bool ok[8];
     int cc[8], rr[8];
@@ -76,8 +102,8 @@ __device__ inline void conv_stage_tile(u32x4 *As, const u32x4 *__restrict__ feat
     for (int u = 0; u < 8; u++) {
       cc[u] = slot;
       rr[u] = rem;
-      v[u] = conv_feat_load(F, t, t.x0 + c, rem, ok[u]);
-      rem += 256;
+      v[u] = conv_feat_load<CPC>(F, t, t.x0 + c, rem, ok[u]);
+      rem += NTH;
       if (rem >= CS) {
         rem -= CS;
         c++;
@@ -86,8 +112,8 @@ __device__ inline void conv_stage_tile(u32x4 *As, const u32x4 *__restrict__ feat
     }
 #pragma unroll
     for (int u = 0; u < 8; u++)
-      if (i0 + u * 256 + tid < NA)
-        As[conv_cell_chunk(cc[u] * AY + (rr[u] >> 2), rr[u] & 3)] =
+      if (i0 + u * NTH + tid < NA)
+        As[conv_cell_chunk<CPC>(cc[u] * AY + rr[u] / CPC, rr[u] & (CPC - 1))] =
             ok[u] ? v[u] : (u32x4){0u, 0u, 0u, 0u};
   }
 }
@@ -95,63 +121,84 @@ __device__ inline void conv_stage_tile(u32x4 *As, const u32x4 *__restrict__ feat
 // Loads the 4 feature columns tile tn adds to its predecessor (tn's relative columns
 // AX-4..AX-1) into registers.  live == false still issues the loads, all from one cell, so
 // that the code around them stays free of branches and the wait counts stay exact.
-template <int PN>
+template <int CPC, int PN, int NTH>
 __device__ inline void conv_fetch_new(u32x4 (&nf)[PN], unsigned &vmask,
                                       const u32x4 *__restrict__ feat, const MfmaTile &tn,
                                       bool live, int AX, int CS, int tid) {
-  const u32x4 *F = feat + tn.featb_off / 8;
+  const u32x4 *F = conv_feat_base<CPC>(feat, tn);
   int c = 0, rem = tid;
   vmask = 0;
 #pragma unroll
   for (int k = 0; k < PN; k++) {
     bool v;
-    nf[k] = conv_feat_load(F, tn, live ? tn.x0 + AX - 4 + c : tn.x0, live ? rem : 0, v);
+    nf[k] = conv_feat_load<CPC>(F, tn, live ? tn.x0 + AX - 4 + c : tn.x0, live ? rem : 0, v);
     vmask |= (unsigned)(v && live) << k;
-    rem += 256;
+    rem += NTH;
     if (rem >= CS) { rem -= CS; c++; }
   }
 }
 
 // Writes those columns into the slots of the current tile's relative columns 0..3 (sx is the
 // slot of its relative column 0): feature column x0 + AX + k lives in slot (x0 + k) mod AX.
-template <int PN>
+template <int CPC, int PN, int NTH>
 __device__ inline void conv_put_new(u32x4 *As, const u32x4 (&nf)[PN], unsigned vmask, int sx,
                                     int AX, int AY, int tid) {
-  const int CS = AY * 4;
+  const int CS = AY * CPC;
   int c = 0, rem = tid;
 #pragma unroll
   for (int k = 0; k < PN; k++) {
-    if (k * 256 + tid < 4 * CS) {
+    if (k * NTH + tid < 4 * CS) {
       int s2 = sx + c;
       if (s2 >= AX) s2 -= AX;
-      As[conv_cell_chunk(s2 * AY + (rem >> 2), rem & 3)] =
+      As[conv_cell_chunk<CPC>(s2 * AY + rem / CPC, rem & (CPC - 1))] =
           ((vmask >> k) & 1) ? nf[k] : (u32x4){0u, 0u, 0u, 0u};
     }
-    rem += 256;
+    rem += NTH;
     if (rem >= CS) { rem -= CS; c++; }
   }
 }
 
-// One tap column for one wave: fh taps x 4 M-tiles x NT N-tiles.  Bcol is the column's
-// filters in LDS ([fh][NT][4 chunks][16 filters]); cell0 the ring cell of the wave's row r.
-// Two register sets: tap yp + 1's fragments are read while tap yp's MFMAs run (the
+// One tap column for one wave: fh taps x MT M-tiles x NT N-tiles.  Bcol is the column's
+// filters in LDS ([fh][NT][CPC chunks][16 filters]); cell0 the ring cell of the wave's first
+// row.  Two register sets: tap yp + 1's fragments are read while tap yp's MFMAs run (the
 // scheduling barriers keep the compiler from sinking the reads next to their use); the reads
-// are unconditional (the last ones repeat tap fh - 1) so the wait counts are fixed.
-template <int KIND, int NT>
-__device__ inline void conv_column(f32x4 (&acc)[4][NT], const u32x4 *Bcol, const u32x4 *As,
+// are unconditional (the last ones repeat tap fh - 1) so the wait counts are fixed.  Split:
+// fragments [0, NT) / [0, MT) are the hi halves, [NT, 2NT) / [MT, 2MT) the lo halves; the
+// three products of a tap go to every accumulator in the order hi*hi, hi*lo, lo*hi, each
+// product over all MT x NT accumulators before the next (independent MFMAs between
+// dependent ones).
+template <int KIND, int NT, bool SPLIT, int MT>
+__device__ inline void conv_column(f32x4 (&acc)[MT][NT], const u32x4 *Bcol, const u32x4 *As,
                                    int cell0, int fh, int r, int chunk) {
-  u32x4 b0[NT], a0[4], b1[NT], a1[4];
+  constexpr int CPC = SPLIT ? 8 : 4, NH = SPLIT ? 2 : 1;
+  u32x4 b0[NH * NT], a0[NH * MT], b1[NH * NT], a1[NH * MT];
   auto load = [&](u32x4 *b, u32x4 *a, int yp) {
 #pragma unroll
-    for (int nt = 0; nt < NT; nt++) b[nt] = Bcol[((yp * NT + nt) * 4 + chunk) * 16 + r];
+    for (int h = 0; h < NH; h++)
 #pragma unroll
-    for (int mt = 0; mt < 4; mt++) a[mt] = As[conv_cell_chunk(cell0 + mt * 16 + yp, chunk)];
+      for (int nt = 0; nt < NT; nt++)
+        b[h * NT + nt] = Bcol[((yp * NT + nt) * CPC + 4 * h + chunk) * 16 + r];
+#pragma unroll
+    for (int h = 0; h < NH; h++)
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++)
+        a[h * MT + mt] = As[conv_cell_chunk<CPC>(cell0 + mt * 16 + yp, 4 * h + chunk)];
   };
   auto mma = [&](const u32x4 *b, const u32x4 *a) {
 #pragma unroll
-    for (int mt = 0; mt < 4; mt++)
+    for (int mt = 0; mt < MT; mt++)
 #pragma unroll
       for (int nt = 0; nt < NT; nt++) acc[mt][nt] = mfma16<KIND>(a[mt], b[nt], acc[mt][nt]);
+    if constexpr (SPLIT) {
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) acc[mt][nt] = mfma16<KIND>(a[mt], b[NT + nt], acc[mt][nt]);
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) acc[mt][nt] = mfma16<KIND>(a[MT + mt], b[nt], acc[mt][nt]);
+    }
   };
   load(b0, a0, 0);
   for (int yp = 0; yp < fh; yp += 2) {
@@ -166,14 +213,13 @@ __device__ inline void conv_column(f32x4 (&acc)[4][NT], const u32x4 *Bcol, const
   }
 }
 
-// Writes one wave's outputs (column x0 + wave, 64 rows, the group's filters).  A lane holds 4
-// consecutive rows of one filter's map: one 16-byte store (4-byte aligned, which gfx950
-// global stores accept) unless the map ends inside them.
-template <int NT>
-__device__ inline void conv_store(const f32x4 (&acc)[4][NT], const MfmaTile &t, int wave, int r,
-                                  int chunk, const long long *__restrict__ map_off,
+// Writes one wave's outputs (column x, rows y0 .. y0 + 16 MT - 1, the group's filters).  A
+// lane holds 4 consecutive rows of one filter's map: one 16-byte store (4-byte aligned, which
+// gfx950 global stores accept) unless the map ends inside them.
+template <int NT, int MT>
+__device__ inline void conv_store(const f32x4 (&acc)[MT][NT], const MfmaTile &t, int x, int y0,
+                                  int r, int chunk, const long long *__restrict__ map_off,
                                   const int2 *__restrict__ map_dims, float *__restrict__ out) {
-  const int x = t.x0 + wave;
 #pragma unroll
   for (int nt = 0; nt < NT; nt++) {
     const int n = nt * 16 + r;
@@ -183,8 +229,8 @@ __device__ inline void conv_store(const f32x4 (&acc)[4][NT], const MfmaTile &t, 
     if (x >= hw.y) continue;
     float *o = out + map_off[slot] + (long long)x * hw.x;
 #pragma unroll
-    for (int mt = 0; mt < 4; mt++) {
-      const int y = t.y0 + mt * 16 + chunk * 4;
+    for (int mt = 0; mt < MT; mt++) {
+      const int y = y0 + mt * 16 + chunk * 4;
       if (y + 3 < hw.x) {
         *(RowQuad *)(o + y) = RowQuad{acc[mt][nt][0], acc[mt][nt][1], acc[mt][nt][2],
                                       acc[mt][nt][3]};
@@ -203,23 +249,33 @@ __device__ inline void conv_store(const f32x4 (&acc)[4][NT], const MfmaTile &t, 
 // this tile's first tap column and written into the ring slots this tile has finished with
 // (relative columns 0..3, last read at tap column min(3, fw - 1)); the next tile's first
 // filter column is loaded by this tile's last tap column.  A tile that does not continue its
-// predecessor stages its whole feature tile first.  Waves own one output column each:
-// 4 M-tiles x NT N-tiles of accumulators.
-template <int KIND, int NT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv_mfma(
-    const MfmaTile *__restrict__ tiles, int ntiles, int fh, int fw,
+// predecessor stages its whole feature tile first.  WPC waves share each of the 4 output
+// columns: wave w takes column w % 4 and rows 64 (w / 4) / WPC .. + 64 / WPC, i.e. MT =
+// 4 / WPC M-tiles x NT N-tiles of accumulators.  The split forms run WPC = 2: their LDS tile
+// leaves room for one workgroup per CU, and two waves per SIMD keep the matrix unit fed while
+// one waits.  The per-output order of the products is the same for every WPC.  nbuf = 2
+// double-buffers the filter columns; nbuf = 1 (a split-form class whose double buffer would
+// not fit the LDS) writes the next column after a barrier that ends this column's reads.
+template <int KIND, int NT, bool SPLIT, int WPC>
+__global__ __launch_bounds__(256 * WPC)
+__attribute__((amdgpu_waves_per_eu(SPLIT ? WPC : 2, SPLIT ? WPC : 2))) void k_conv_mfma(
+    const MfmaTile *__restrict__ tiles, int ntiles, int fh, int fw, int nbuf,
     const u32x4 *__restrict__ feat, const u32x4 *__restrict__ filt,
     const long long *__restrict__ map_off, const int2 *__restrict__ map_dims,
     float *__restrict__ out) {
   extern __shared__ u32x4 lds[];
-  constexpr int PN = ConvRegs<NT>::PN, PRE = ConvRegs<NT>::PRE;
+  constexpr int NTH = 256 * WPC, MT = 4 / WPC;
+  constexpr int PN = ConvRegs<NT, SPLIT, NTH>::PN, PRE = ConvRegs<NT, SPLIT, NTH>::PRE;
+  constexpr int CPC = SPLIT ? 8 : 4;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int AX = 4 + fw - 1, AY = 64 + fh - 1, CS = AY * 4, NA = AX * CS;
-  u32x4 *As = lds;                               // [AX slots][AY][4 chunks], swizzled
-  u32x4 *Bs = lds + NA;                          // [2][fh][NT][4][16]
-  const int bstage = fh * NT * 64;               // 16-byte chunks per tap column
-  const bool in_regs = bstage <= PRE * 256;
-  const bool ring = 4 * CS <= PN * 256;
+  const int wcol = wave & 3, wrow = (wave >> 2) * 16 * MT;  // the wave's column and first row
+  const int AX = 4 + fw - 1, AY = 64 + fh - 1, CS = AY * CPC, NA = AX * CS;
+  u32x4 *As = lds;                               // [AX slots][AY][CPC chunks], swizzled
+  u32x4 *Bs = lds + NA;                          // [nbuf][fh][NT][CPC][16]
+  const int bstage = fh * NT * CPC * 16;         // 16-byte chunks per tap column
+  const bool in_regs = bstage <= PRE * NTH;
+  const bool ring = 4 * CS <= PN * NTH;
+  const int bmask = nbuf - 1;
 
   int bx = blockIdx.x;
   if ((int)gridDim.x == ntiles) {
@@ -244,62 +300,68 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const MfmaTile tn = tiles[has_next ? ti + 1 : ti];
     const bool cont = has_next && ring && tn.featb_off == t.featb_off && tn.y0 == t.y0 &&
                       tn.x0 == t.x0 + 4;
-    const u32x4 *Wg = filt + t.filt_off / 8;   // offsets count 16-bit elements
+    const u32x4 *Wg = conv_filt_base<CPC>(filt, t);
     const int sx = t.x0 % AX;                  // slot of the tile's relative column 0
     // the previous tile ended with a barrier after its last reads of As
-    if (!have_feat) conv_stage_tile(As, feat, t, AX, AY, tid);
+    if (!have_feat) conv_stage_tile<CPC, NTH>(As, feat, t, AX, AY, tid);
     if (!have_filt)
-      for (int i = tid; i < bstage; i += 256) Bs[bpar * bstage + i] = Wg[i];
+      for (int i = tid; i < bstage; i += NTH) Bs[bpar * bstage + i] = Wg[i];
     if (!have_feat || !have_filt) __syncthreads();
 
-    f32x4 acc[4][NT];
+    f32x4 acc[MT][NT];
 #pragma unroll
-    for (int mt = 0; mt < 4; mt++)
+    for (int mt = 0; mt < MT; mt++)
 #pragma unroll
       for (int nt = 0; nt < NT; nt++) acc[mt][nt] = (f32x4){0.f, 0.f, 0.f, 0.f};
 
     u32x4 nf[PN];
     unsigned vmask = 0;
     auto step = [&](int xp, bool first) {
-      const int buf = (bpar + xp) & 1;
+      const int buf = (bpar + xp) & bmask;
       // the next filter column (this tile's xp + 1, or the next tile's column 0) is loaded into
       // registers before this column's MFMAs and written to LDS after them (the register copy
       // runs even with nothing to copy -- last column of the last tile -- into the idle
       // buffer, and alongside the plain copy, writing the same values, when a column exceeds
       // the registers: branch-free, so the wait counts stay exact)
       const bool more = xp + 1 < fw;
-      const u32x4 *src = more ? Wg + (long long)(xp + 1) * bstage : filt + tn.filt_off / 8;
-      u32x4 *dst = Bs + (buf ^ 1) * bstage;
+      const u32x4 *src = more ? Wg + (long long)(xp + 1) * bstage : conv_filt_base<CPC>(filt, tn);
+      u32x4 *dst = Bs + ((buf ^ 1) & bmask) * bstage;
       u32x4 pre[PRE];
 #pragma unroll
-      for (int c = 0; c < PRE; c++) pre[c] = src[(unsigned)min(tid + c * 256, bstage - 1)];
-      if (!in_regs && (more || has_next))
-        for (int i = tid; i < bstage; i += 256) dst[i] = src[i];
+      for (int c = 0; c < PRE; c++) pre[c] = src[(unsigned)min(tid + c * NTH, bstage - 1)];
+      if (nbuf == 2 && !in_regs && (more || has_next))
+        for (int i = tid; i < bstage; i += NTH) dst[i] = src[i];
       // issued after the filter loads, the next tile's feature loads may stay in flight past
       // this column's wait for the filters
-      if (first) conv_fetch_new<PN>(nf, vmask, feat, tn, cont, AX, CS, tid);
-      int slot = sx + wave + xp;
+      if (first) conv_fetch_new<CPC, PN, NTH>(nf, vmask, feat, tn, cont, AX, CS, tid);
+      int slot = sx + wcol + xp;
       if (slot >= AX) slot -= AX;
-      conv_column<KIND, NT>(acc, Bs + buf * bstage, As, slot * AY + r, fh, r, chunk);
+      conv_column<KIND, NT, SPLIT, MT>(acc, Bs + buf * bstage, As, slot * AY + wrow + r, fh, r,
+                                       chunk);
+      if (nbuf == 1) {
+        __syncthreads();                       // every wave's reads of the one buffer done
+        if (!in_regs && (more || has_next))
+          for (int i = tid; i < bstage; i += NTH) dst[i] = src[i];
+      }
 #pragma unroll
-      for (int c = 0; c < PRE; c++) dst[(unsigned)min(tid + c * 256, bstage - 1)] = pre[c];
+      for (int c = 0; c < PRE; c++) dst[(unsigned)min(tid + c * NTH, bstage - 1)] = pre[c];
       __syncthreads();
     };
     // tap column xp reads relative columns xp..xp+3, so from tap column 4 on (or after the
     // last one) the slots of relative columns 0..3 take the next tile's new columns
     step(0, true);
     for (int xp = 1; xp < fw; xp++) {
-      if (xp == 4 && cont) conv_put_new<PN>(As, nf, vmask, sx, AX, AY, tid);
+      if (xp == 4 && cont) conv_put_new<CPC, PN, NTH>(As, nf, vmask, sx, AX, AY, tid);
       step(xp, false);
     }
     if (fw <= 4 && cont) {
-      conv_put_new<PN>(As, nf, vmask, sx, AX, AY, tid);
+      conv_put_new<CPC, PN, NTH>(As, nf, vmask, sx, AX, AY, tid);
       __syncthreads();
     }
-    conv_store<NT>(acc, t, wave, r, chunk, map_off, map_dims, out);
+    conv_store<NT, MT>(acc, t, t.x0 + wcol, t.y0 + wrow, r, chunk, map_off, map_dims, out);
     have_feat = cont;
     have_filt = has_next;
-    bpar = (bpar + fw) & 1;
+    bpar = (bpar + fw) & bmask;
     t = tn;
   }
 }
@@ -310,8 +372,9 @@ constexpr int kQueueMaxLds = 160 * 1024 - 1024;  // dynamic LDS cap of the queue
 // is exhausted (every workgroup reaches that exit).  The tile carries its filter class's
 // shape (fh, fw) and its number of 16-filter tiles nt (1..3), so one launch covers every
 // level of every class; per output the taps are accumulated in the same order as the
-// per-class kernel, so both give the same bits.  Waves own one column of 16*MT rows each.
-template <int KIND, int MT>
+// per-class kernel (split: hi*hi, hi*lo, lo*hi per tap), so both give the same bits.  Waves
+// own one column of 16*MT rows each.
+template <int KIND, int MT, bool SPLIT>
 __global__ __launch_bounds__(256) void k_conv_mfma_q(const MfmaTile *__restrict__ tiles,
                                                      int ntiles, int *__restrict__ counter,
                                                      const u32x4 *__restrict__ feat,
@@ -330,19 +393,20 @@ __global__ __launch_bounds__(256) void k_conv_mfma_q(const MfmaTile *__restrict_
     __syncthreads();
     if (ti >= ntiles) break;
     const MfmaTile t = tiles[ti];
+    constexpr int CPC = SPLIT ? 8 : 4;
     const int fh = t.fh, fw = t.fw, nt = t.nt;
     const int AX = 4 + fw - 1, AY = 16 * MT + fh - 1;
-    u32x4 *As = lds;                               // [AX][AY][4 chunks]
-    u32x4 *Bs = lds + AX * AY * 4;                 // [2][fh][nt][4][16]
-    const int bstage = fh * nt * 64;
-    const u32x4 *F = feat + t.featb_off / 8;
-    const u32x4 *Wg = filt + t.filt_off / 8;
-    for (int i = tid; i < AX * AY * 4; i += 256) {
-      const int cell = i >> 2, part = i & 3;
+    u32x4 *As = lds;                               // [AX][AY][CPC chunks]
+    u32x4 *Bs = lds + AX * AY * CPC;               // [2][fh][nt][CPC][16]
+    const int bstage = fh * nt * CPC * 16;
+    const u32x4 *F = conv_feat_base<CPC>(feat, t);
+    const u32x4 *Wg = conv_filt_base<CPC>(filt, t);
+    for (int i = tid; i < AX * AY * CPC; i += 256) {
+      const int cell = i / CPC, part = i & (CPC - 1);
       const int ax = cell / AY, ay = cell - ax * AY;
       const int gx = t.x0 + ax, gy = t.y0 + ay;
       u32x4 v = {0u, 0u, 0u, 0u};
-      if (gx < t.F1 && gy < t.F0) v = F[((long long)gx * t.F0 + gy) * 4 + part];
+      if (gx < t.F1 && gy < t.F0) v = F[((long long)gx * t.F0 + gy) * CPC + part];
       As[i] = v;
     }
     for (int i = tid; i < bstage; i += 256) Bs[i] = Wg[i];
@@ -360,17 +424,33 @@ __global__ __launch_bounds__(256) void k_conv_mfma_q(const MfmaTile *__restrict_
         for (int i = tid; i < bstage; i += 256) dst[i] = src[i];
       }
       const u32x4 *Bb = Bs + buf * bstage;
-      const u32x4 *Ab = As + ((wave + xp) * AY) * 4 + chunk;
+      const u32x4 *Ab = As + ((wave + xp) * AY) * CPC + chunk;
       for (int yp = 0; yp < fh; yp++) {
-        u32x4 b0 = Bb[((yp * nt + 0) * 4 + chunk) * 16 + r], b1 = b0, b2 = b0;
-        if (nt > 1) b1 = Bb[((yp * nt + 1) * 4 + chunk) * 16 + r];
-        if (nt > 2) b2 = Bb[((yp * nt + 2) * 4 + chunk) * 16 + r];
+        u32x4 b[2][3];
+#pragma unroll
+        for (int h = 0; h < (SPLIT ? 2 : 1); h++) {
+          b[h][0] = Bb[((yp * nt + 0) * CPC + 4 * h + chunk) * 16 + r];
+          b[h][1] = b[h][2] = b[h][0];
+          if (nt > 1) b[h][1] = Bb[((yp * nt + 1) * CPC + 4 * h + chunk) * 16 + r];
+          if (nt > 2) b[h][2] = Bb[((yp * nt + 2) * CPC + 4 * h + chunk) * 16 + r];
+        }
 #pragma unroll
         for (int mt = 0; mt < MT; mt++) {
-          const u32x4 a = Ab[(mt * 16 + r + yp) * 4];
-          acc[mt][0] = mfma16<KIND>(a, b0, acc[mt][0]);
-          if (nt > 1) acc[mt][1] = mfma16<KIND>(a, b1, acc[mt][1]);
-          if (nt > 2) acc[mt][2] = mfma16<KIND>(a, b2, acc[mt][2]);
+          // the cell's hi chunk (its lo chunk lies 4 further): this kernel's ring is a plain
+          // [AX][AY][CPC] array
+          const u32x4 a = Ab[(mt * 16 + r + yp) * CPC];
+#pragma unroll
+          for (int n = 0; n < 3; n++)
+            if (n < nt) acc[mt][n] = mfma16<KIND>(a, b[0][n], acc[mt][n]);
+          if constexpr (SPLIT) {
+            const u32x4 al = Ab[(mt * 16 + r + yp) * CPC + 4];
+#pragma unroll
+            for (int n = 0; n < 3; n++)
+              if (n < nt) acc[mt][n] = mfma16<KIND>(a, b[1][n], acc[mt][n]);
+#pragma unroll
+            for (int n = 0; n < 3; n++)
+              if (n < nt) acc[mt][n] = mfma16<KIND>(al, b[0][n], acc[mt][n]);
+          }
         }
       }
       __syncthreads();
@@ -396,55 +476,81 @@ __global__ __launch_bounds__(256) void k_conv_mfma_q(const MfmaTile *__restrict_
   }
 }
 
-size_t mfma_q_lds_bytes(int fh, int fw, int nt, int MT) {
-  return (size_t)16 * ((size_t)(4 + fw - 1) * (16 * MT + fh - 1) * 4 + 2 * (size_t)fh * nt * 64);
+size_t mfma_q_lds_bytes(int fh, int fw, int nt, int MT, bool split) {
+  const size_t cpc = split ? 8 : 4;
+  return (size_t)16 * cpc * ((size_t)(4 + fw - 1) * (16 * MT + fh - 1) + 2 * (size_t)fh * nt * 16);
 }
 
-template <int KIND, int MT>
+template <int KIND, int MT, bool SPLIT>
 static int launch_q(hipStream_t s, int blocks, size_t lds, const MfmaTile *tiles, int ntiles,
                     int *counter, const void *feat, const void *filt, const long long *map_off,
                     const int2 *map_dims, float *out) {
   // the kernel's static LDS (the tile index) comes out of the same 160 KB
-  static const bool attr = hipFuncSetAttribute((const void *)k_conv_mfma_q<KIND, MT>,
+  static const bool attr = hipFuncSetAttribute((const void *)k_conv_mfma_q<KIND, MT, SPLIT>,
                                                hipFuncAttributeMaxDynamicSharedMemorySize,
                                                kQueueMaxLds) == hipSuccess;
   if (!attr) return -1;
-  hipLaunchKernelGGL((k_conv_mfma_q<KIND, MT>), dim3(blocks), dim3(256), lds, s, tiles, ntiles,
-                     counter, (const u32x4 *)feat, (const u32x4 *)filt, map_off, map_dims, out);
+  hipLaunchKernelGGL((k_conv_mfma_q<KIND, MT, SPLIT>), dim3(blocks), dim3(256), lds, s, tiles,
+                     ntiles, counter, (const u32x4 *)feat, (const u32x4 *)filt, map_off, map_dims,
+                     out);
   return 0;
 }
 
-int launch_conv_mfma_queue(hipStream_t s, int kind, int MT, int blocks, size_t lds,
+// kind: 1 = BF16, 2 = FP16; SPLIT: the three-product form on hi/lo operand pairs.
+template <bool SPLIT>
+static int launch_q_kind(hipStream_t s, int kind, int MT, int blocks, size_t lds,
+                         const MfmaTile *tiles, int ntiles, int *counter, const void *feat,
+                         const void *filt, const long long *map_off, const int2 *map_dims,
+                         float *out) {
+  if (kind == 1)
+    return MT == 2 ? launch_q<1, 2, SPLIT>(s, blocks, lds, tiles, ntiles, counter, feat, filt, map_off, map_dims, out)
+                   : launch_q<1, 4, SPLIT>(s, blocks, lds, tiles, ntiles, counter, feat, filt, map_off, map_dims, out);
+  return MT == 2 ? launch_q<2, 2, SPLIT>(s, blocks, lds, tiles, ntiles, counter, feat, filt, map_off, map_dims, out)
+                 : launch_q<2, 4, SPLIT>(s, blocks, lds, tiles, ntiles, counter, feat, filt, map_off, map_dims, out);
+}
+
+int launch_conv_mfma_queue(hipStream_t s, int kind, bool split, int MT, int blocks, size_t lds,
                            const MfmaTile *tiles, int ntiles, int *counter, const void *feat,
                            const void *filt, const long long *map_off, const int2 *map_dims,
                            float *out) {
   if (ntiles <= 0) return 0;
-  if (lds > (size_t)kQueueMaxLds || blocks <= 0) return -1;
-  if (kind == 1)
-    return MT == 2 ? launch_q<1, 2>(s, blocks, lds, tiles, ntiles, counter, feat, filt, map_off, map_dims, out)
-                   : launch_q<1, 4>(s, blocks, lds, tiles, ntiles, counter, feat, filt, map_off, map_dims, out);
-  return MT == 2 ? launch_q<2, 2>(s, blocks, lds, tiles, ntiles, counter, feat, filt, map_off, map_dims, out)
-                 : launch_q<2, 4>(s, blocks, lds, tiles, ntiles, counter, feat, filt, map_off, map_dims, out);
+  if (lds > (size_t)kQueueMaxLds || blocks <= 0 || (kind != 1 && kind != 2)) return -1;
+  return split ? launch_q_kind<true>(s, kind, MT, blocks, lds, tiles, ntiles, counter, feat, filt, map_off, map_dims, out)
+               : launch_q_kind<false>(s, kind, MT, blocks, lds, tiles, ntiles, counter, feat, filt, map_off, map_dims, out);
 }
 
-size_t mfma_lds_bytes(int fh, int fw, int nt) {
-  return (size_t)16 * ((size_t)(4 + fw - 1) * (64 + fh - 1) * 4 + 2 * (size_t)fh * nt * 64);
+size_t mfma_lds_bytes(int fh, int fw, int nt, bool split, int nbuf) {
+  const size_t cpc = split ? 8 : 4;
+  return (size_t)16 * cpc * ((size_t)(4 + fw - 1) * (64 + fh - 1) + (size_t)nbuf * fh * nt * 16);
+}
+
+// Filter buffers of the per-class kernel: 2, or 1 for a split class whose double buffer
+// would not fit the 160 KB of LDS; 0 = unsupported (a split class whose 64-row feature tile
+// and one tap column of filters exceed the LDS, e.g. 12x12 filters in groups of 48).
+static int mfma_nbuf(int fh, int fw, int nt, bool split) {
+  if (mfma_lds_bytes(fh, fw, nt, split, 2) <= 160 * 1024) return 2;
+  if (split && mfma_lds_bytes(fh, fw, nt, split, 1) <= 160 * 1024) return 1;
+  return 0;
 }
 
 // Launches the persistent kernel with every workgroup resident at once (its tile ranges are
 // static).  With fewer than 4 tiles per resident workgroup a static split would leave the
 // last round uneven, so each tile gets its own workgroup and the hardware balances them.
-template <int KIND, int NT>
+template <int KIND, int NT, bool SPLIT>
 static int launch_one(hipStream_t s, const MfmaTile *tiles, int ntiles, int fh, int fw,
                       const void *feat, const void *filt, const long long *map_off,
                       const int2 *map_dims, float *out) {
-  const size_t lds = mfma_lds_bytes(fh, fw, NT);
+  constexpr int WPC = SPLIT ? 2 : 1;           // waves per output column
+  const void *kern = (const void *)k_conv_mfma<KIND, NT, SPLIT, WPC>;
+  const int nbuf = mfma_nbuf(fh, fw, NT, SPLIT);
+  if (nbuf == 0) return -1;
+  const size_t lds = mfma_lds_bytes(fh, fw, NT, SPLIT, nbuf);
   // once per process (function-local statics are initialised once, also when serving
   // workers launch concurrently): the LDS attribute and the CU count, 0 on failure
-  static const int ncu = [] {
+  static const int ncu = [kern] {
     int dev = 0, n = 0;
-    if (hipFuncSetAttribute((const void *)k_conv_mfma<KIND, NT>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess ||
+    if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) !=
+            hipSuccess ||
         hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
       return 0;
@@ -452,29 +558,38 @@ static int launch_one(hipStream_t s, const MfmaTile *tiles, int ntiles, int fh, 
   }();
   if (ncu <= 0) return -1;
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_conv_mfma<KIND, NT>,
-                                                   256, lds) != hipSuccess || per_cu < 1)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256 * WPC, lds) != hipSuccess ||
+      per_cu < 1)
     return -1;
   const int full = per_cu * ncu;
   const int blocks = ntiles <= 4 * full ? ntiles : full;
-  hipLaunchKernelGGL((k_conv_mfma<KIND, NT>), dim3(blocks), dim3(256), lds, s, tiles, ntiles, fh,
-                     fw, (const u32x4 *)feat, (const u32x4 *)filt, map_off, map_dims, out);
+  hipLaunchKernelGGL((k_conv_mfma<KIND, NT, SPLIT, WPC>), dim3(blocks), dim3(256 * WPC), lds, s,
+                     tiles, ntiles, fh, fw, nbuf, (const u32x4 *)feat, (const u32x4 *)filt,
+                     map_off, map_dims, out);
   return 0;
 }
 
-int launch_conv_mfma(hipStream_t s, int kind, int nt, const MfmaTile *tiles, int ntiles, int fh,
-                     int fw, const void *feat, const void *filt, const long long *map_off,
-                     const int2 *map_dims, float *out) {
-  if (ntiles <= 0) return 0;
-  if (mfma_lds_bytes(fh, fw, nt) > 160 * 1024) return -1;
+template <bool SPLIT>
+static int launch_kind(hipStream_t s, int kind, int nt, const MfmaTile *tiles, int ntiles, int fh,
+                       int fw, const void *feat, const void *filt, const long long *map_off,
+                       const int2 *map_dims, float *out) {
   if (kind == 1) {
-    if (nt == 3) return launch_one<1, 3>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
-    if (nt == 2) return launch_one<1, 2>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
-    return launch_one<1, 1>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+    if (nt == 3) return launch_one<1, 3, SPLIT>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+    if (nt == 2) return launch_one<1, 2, SPLIT>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+    return launch_one<1, 1, SPLIT>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
   }
-  if (nt == 3) return launch_one<2, 3>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
-  if (nt == 2) return launch_one<2, 2>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
-  return launch_one<2, 1>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+  if (nt == 3) return launch_one<2, 3, SPLIT>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+  if (nt == 2) return launch_one<2, 2, SPLIT>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+  return launch_one<2, 1, SPLIT>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+}
+
+int launch_conv_mfma(hipStream_t s, int kind, bool split, int nt, const MfmaTile *tiles,
+                     int ntiles, int fh, int fw, const void *feat, const void *filt,
+                     const long long *map_off, const int2 *map_dims, float *out) {
+  if (ntiles <= 0) return 0;
+  if ((kind != 1 && kind != 2) || nt < 1 || nt > 3 || mfma_nbuf(fh, fw, nt, split) == 0) return -1;
+  return split ? launch_kind<true>(s, kind, nt, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out)
+               : launch_kind<false>(s, kind, nt, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
 }
 
 }  // namespace geye

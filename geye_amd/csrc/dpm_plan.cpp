@@ -536,7 +536,7 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
               t.cost = fc.h * fc.w * fc.nt * std::min(rows, Hm - y0);
               P.q_tiles[mi].push_back(t);
             }
-          P.q_lds[mi] = std::max(P.q_lds[mi], mfma_q_lds_bytes(fc.h, fc.w, fc.nt, mi == 0 ? 2 : 4));
+          P.q_lds[mi] = std::max(P.q_lds[mi], mfma_q_lds_bytes(fc.h, fc.w, fc.nt, mi == 0 ? 2 : 4, false));
         }
       }
     }
@@ -782,7 +782,7 @@ void replicate_plan(const Plan &one, int nb, long long feat_stride, long long fe
     P.mfma_classes.push_back(c2);
   }
   // a batch runs one launch per filter class (the schedule the autotuner picks for 1080p)
-  for (int k = 0; k < 3; k++) P.mfma_choice[k] = 0;
+  for (int k = 0; k < 5; k++) P.mfma_choice[k] = 0;
   // DT jobs
   // (pass 1 reads the convolution arena, pass 2 the first pass's output in the DT arena)
   auto rep_jobs = [&](const std::vector<DtJob> &src, std::vector<DtJob> &dst, long long src_frame) {

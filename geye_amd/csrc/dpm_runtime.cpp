@@ -151,10 +151,35 @@ unsigned short to_f16(float f) {
   return u;
 }
 
+float from_bf16(unsigned short b) {
+  const unsigned int u = (unsigned int)b << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+float from_f16(unsigned short b) {
+  _Float16 h;
+  std::memcpy(&h, &b, 2);
+  return (float)h;
+}
+
+// The 16-bit operand of an MFMA form for v: its rounding (BF16, FP16), or for the split forms
+// the pair hi = round16(v), lo = round16(v - hi) (v - hi is exact in FP32).
+void mfma_operand(float v, int precision, unsigned short &hi, unsigned short &lo) {
+  const bool bf = mfma_kind(precision) == 1;
+  hi = bf ? to_bf16(v) : to_f16(v);
+  lo = 0;
+  if (split_precision(precision)) {
+    const float h = bf ? from_bf16(hi) : from_f16(hi);
+    lo = bf ? to_bf16(v - h) : to_f16(v - h);
+  }
+}
+
 }  // namespace
 
-// The host side of a model: fields, checks, filter classes and the packed filters (FP32,
-// BF16, FP16) the device copies are made from.  No device work.
+// The host side of a model: fields, checks, filter classes and the packed filters (FP32 and
+// one 16-bit pack per MFMA form) the device copies are made from.  No device work.
 int geye::model_pack(const geye_model_desc *d, Model &m, ModelPacks &pk) {
   if (!d || d->numcomponent <= 0 || d->NoR <= 0 || d->sbin <= 0 || d->interval <= 0)
     return fail(GEYE_ERR_ARG, "model: empty or malformed description");
@@ -251,10 +276,8 @@ int geye::model_pack(const geye_model_desc *d, Model &m, ModelPacks &pk) {
     }
   }
   std::vector<float> &packed = pk.f32;
-  std::vector<unsigned short> &pbf = pk.bf16, &pf16 = pk.f16;
   packed.clear();
-  pbf.clear();
-  pf16.clear();
+  for (auto &h : pk.h16) h.clear();
   for (Model::FilterClass &fc : m.classes) {
     const int nmem = (int)fc.members.size();
     fc.nf = pick_nf(nmem);
@@ -275,18 +298,25 @@ int geye::model_pack(const geye_model_desc *d, Model &m, ModelPacks &pk) {
         for (int xp = 0; xp < fc.w; xp++)
           for (int yp = 0; yp < fc.h; yp++)
             for (int n = 0; n < fc.nf; n++) packed.push_back(weight(g * fc.nf + n, f, xp, yp));
-    fc.hoff = (long long)pbf.size();
-    for (int g = 0; g < g16; g++)
-      for (int xp = 0; xp < fc.w; xp++)
-        for (int yp = 0; yp < fc.h; yp++)
-          for (int t = 0; t < fc.nt; t++)
-            for (int ch = 0; ch < 4; ch++)
-              for (int n = 0; n < 16; n++)
-                for (int j = 0; j < 8; j++) {
-                  const float v = weight(g * 16 * fc.nt + t * 16 + n, ch * 8 + j, xp, yp);
-                  pbf.push_back(to_bf16(v));
-                  pf16.push_back(to_f16(v));
-                }
+    // [group][xp][yp][nt][chunk][16 filters][8]: 4 chunks of 8 channels, or 8 in the split
+    // forms (the hi halves, then the lo halves); the split packs' offsets are twice hoff
+    fc.hoff = (long long)pk.h16[GEYE_BF16].size();
+    for (int p = GEYE_BF16; p <= GEYE_FP16X3; p++) {
+      const int nh = split_precision(p) ? 2 : 1;
+      for (int g = 0; g < g16; g++)
+        for (int xp = 0; xp < fc.w; xp++)
+          for (int yp = 0; yp < fc.h; yp++)
+            for (int t = 0; t < fc.nt; t++)
+              for (int h = 0; h < nh; h++)
+                for (int ch = 0; ch < 4; ch++)
+                  for (int n = 0; n < 16; n++)
+                    for (int j = 0; j < 8; j++) {
+                      unsigned short hi, lo;
+                      mfma_operand(weight(g * 16 * fc.nt + t * 16 + n, ch * 8 + j, xp, yp), p,
+                                   hi, lo);
+                      pk.h16[p].push_back(h ? lo : hi);
+                    }
+    }
   }
   m.lsq.clear();
   m.lsq_idx.clear();
@@ -308,8 +338,7 @@ int model_from_desc(geye_ctx *ctx, const geye_model_desc *d, Model &m) {
   ModelPacks pk;
   CHK(model_pack(d, m, pk));
   m.ctx = ctx;
-  CHK(upload(m.d_filt_bf16, pk.bf16, ctx->stream));
-  CHK(upload(m.d_filt_f16, pk.f16, ctx->stream));
+  for (int p = GEYE_BF16; p <= GEYE_FP16X3; p++) CHK(upload(m.d_filt16[p], pk.h16[p], ctx->stream));
   CHK(upload(m.d_filters, pk.f32, ctx->stream));
   CHK(upload(m.d_rsize, m.rsize, ctx->stream));
   if (!m.lsq.empty()) {
@@ -679,7 +708,7 @@ int plan_bytes(const Model &m, int W, int H, long long *bytes) {
   (void)tmp;
   long long b = 4 * (P.img_total + P.hist_total + P.norm_total + P.feat_total + P.conv_total +
                      4 * P.dt_total + 3 * P.score_total) +
-                2 * P.featb_total + 8 * P.grid_total + (long long)P.W * P.H * (3 + 4) +
+                4 * P.featb_total + 8 * P.grid_total + (long long)P.W * P.H * (3 + 4) +
                 N * (2 * (long long)sizeof(NmsBox) + 16 * 4) + (long long)P.GL * 4 * N;
   *bytes = b;
   return GEYE_OK;
@@ -755,7 +784,8 @@ int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, int hk
     launch_hog_votes(s, hog, P.hog16_n + P.hog8_n, P.hog_hist_blocks + P.hog8_blocks,
                      dp<int2>(P.d_grid), dp<float>(P.d_hist), dp<float>(P.d_norm));
   }
-  if (hkind) CHK(ensure(P.d_featb, 2 * P.featb_total));
+  // the MFMA forms' channel-last copy: 32 16-bit channels per cell, 64 in the split forms
+  if (hkind) CHK(ensure(P.d_featb, (split_precision(hkind) ? 4 : 2) * P.featb_total));
   if (P.hogf_n > 0) {
     KTimer t(c, "hog_feat");
     // the 16-bit forms convolve the channel-last copy only, so the FP32 features are not
@@ -779,13 +809,16 @@ std::map<std::tuple<const Model *, long long, int>, int> g_tuned;  // autotuned 
 int run_mfma(geye_ctx *c, Model &m, Plan &P, const Plan &pyr, int precision, int choice,
              bool timed) {
   hipStream_t s = c->stream;
-  const void *filt = precision == GEYE_BF16 ? m.d_filt_bf16.p : m.d_filt_f16.p;
-  const std::string tag = precision == GEYE_BF16 ? "conv_bf16_" : "conv_fp16_";
+  const void *filt = m.d_filt16[precision].p;
+  static const char *const tags[5] = {"", "conv_bf16_", "conv_fp16_", "conv_bf16x3_", "conv_fp16x3_"};
+  const std::string tag = tags[precision];
+  const int kind = mfma_kind(precision);
+  const bool split = split_precision(precision);
   if (choice == 0) {
     for (auto &mc : P.mfma_classes) {
       if (mc.tile_count == 0) continue;
       KTimer t(c, timed ? tag + std::to_string(mc.h) + "x" + std::to_string(mc.w) : std::string());
-      if (launch_conv_mfma(s, precision, mc.nt, dp<MfmaTile>(P.d_mfma_tiles) + mc.tile_begin,
+      if (launch_conv_mfma(s, kind, split, mc.nt, dp<MfmaTile>(P.d_mfma_tiles) + mc.tile_begin,
                            mc.tile_count, mc.h, mc.w, pyr.d_featb.p, filt,
                            dp<long long>(P.d_map_off), dp<int2>(P.d_map_dims),
                            dp<float>(P.d_conv)) != 0)
@@ -798,7 +831,8 @@ int run_mfma(geye_ctx *c, Model &m, Plan &P, const Plan &pyr, int precision, int
   const DeviceBuffer &qt = mi == 0 ? P.d_qtiles0 : P.d_qtiles1;
   launch_fill(s, dp<int>(P.d_qcounter), 1, 0);
   KTimer t(c, timed ? tag + "queue" : std::string());
-  if (launch_conv_mfma_queue(s, precision, MT, c->num_cu * bpc, P.q_lds[mi], (const MfmaTile *)qt.p,
+  if (launch_conv_mfma_queue(s, kind, split, MT, c->num_cu * bpc, P.q_lds[mi] * (split ? 2 : 1),
+                             (const MfmaTile *)qt.p,
                              (int)P.q_tiles[mi].size(), dp<int>(P.d_qcounter), pyr.d_featb.p, filt,
                              dp<long long>(P.d_map_off), dp<int2>(P.d_map_dims),
                              dp<float>(P.d_conv)) != 0)
@@ -812,9 +846,11 @@ int run_mfma(geye_ctx *c, Model &m, Plan &P, const Plan &pyr, int precision, int
 // before any graph capture.
 int tune_mfma(geye_ctx *c, Model &m, Plan &P, const Plan &pyr, int precision) {
   std::vector<int> cands{0};
+  const size_t qscale = split_precision(precision) ? 2 : 1;  // the split forms' LDS per tile
   for (int mi = 0; mi < 2; mi++)
     for (int bpc = 1; bpc <= 4; bpc++)
-      if (!P.q_tiles[mi].empty() && P.q_lds[mi] * bpc <= 159 * 1024) cands.push_back(1 + mi * 8 + bpc - 1);
+      if (!P.q_tiles[mi].empty() && P.q_lds[mi] * qscale * bpc <= 159 * 1024)
+        cands.push_back(1 + mi * 8 + bpc - 1);
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -870,7 +906,7 @@ int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int 
                       cc.h, cc.w, (const float *)F.d_feat.p, dp<float>(m.d_filters),
                       dp<long long>(P.d_map_off), dp<int2>(P.d_map_dims), dp<float>(P.d_conv));
     }
-  } else if (precision == GEYE_BF16 || precision == GEYE_FP16) {
+  } else if (mfma_precision(precision)) {
     if (P.mfma_choice[precision] < 0) {
       // a schedule tuned for the same model and frame geometry on another context is reused
       const auto key = std::make_tuple((const Model *)&m, plan_key(P.W, P.H, P.max_scale), precision);
@@ -1402,8 +1438,7 @@ int detect_models(geye_ctx *c, const geye_model *const *models, int n, const uns
     boxes[i] = nullptr;
     num[i] = GL[i] = 0;
   }
-  if (precision != GEYE_FP32 && precision != GEYE_BF16 && precision != GEYE_FP16)
-    return fail(GEYE_ERR_ARG, "unknown precision");
+  if (!valid_precision(precision)) return fail(GEYE_ERR_ARG, "unknown precision");
   std::lock_guard<std::mutex> lk(c->mu);
   HIPCHK(hipSetDevice(c->device));
   (void)hipGetLastError();
@@ -1470,8 +1505,7 @@ int detect_batch(geye_ctx *c, const geye_model *const *models, int nm,
     if (!models[i]) return fail(GEYE_ERR_ARG, "detect_batch: null model");
   for (int f = 0; f < nf; f++)
     if (!frames[f]) return fail(GEYE_ERR_ARG, "detect_batch: null frame");
-  if (precision != GEYE_FP32 && precision != GEYE_BF16 && precision != GEYE_FP16)
-    return fail(GEYE_ERR_ARG, "unknown precision");
+  if (!valid_precision(precision)) return fail(GEYE_ERR_ARG, "unknown precision");
   for (int k = 0; k < nf * nm; k++) {
     boxes[k] = nullptr;
     num[k] = GL[k] = 0;
@@ -1863,8 +1897,7 @@ void geye_model_free(geye_model *gm) {
   }
   release(m.d_filters);
   release(m.d_rsize);
-  release(m.d_filt_bf16);
-  release(m.d_filt_f16);
+  for (auto &b : m.d_filt16) release(b);
   release(m.d_lsq);
   release(m.d_lsq_idx);
   delete gm;
@@ -2374,8 +2407,7 @@ int geye_fconv(geye_ctx *c, const float *feat, int F0, int F1, const float *filt
   if (h < 1 || w < 1 || h > 16 || w > 16) return fail(GEYE_ERR_UNSUPPORTED, "filter size outside 1..16");
   const int H = F0 - h + 1, W = F1 - w + 1;
   if (H < 1 || W < 1) return fail(GEYE_ERR_ARG, "filter larger than the feature map");
-  if (precision != GEYE_FP32 && precision != GEYE_BF16 && precision != GEYE_FP16)
-    return fail(GEYE_ERR_ARG, "unknown precision");
+  if (!valid_precision(precision)) return fail(GEYE_ERR_ARG, "unknown precision");
   std::lock_guard<std::mutex> lk(c->mu);
   HIPCHK(hipSetDevice(c->device));
   (void)hipGetLastError();
@@ -2416,24 +2448,28 @@ int geye_fconv(geye_ctx *c, const float *feat, int F0, int F1, const float *filt
     launch_conv_f32(c->stream, nf, dp<ConvTile>(dt), (int)tiles.size(), h, w, dp<float>(dfe),
                     dp<float>(dfi), dp<long long>(dm), dp<int2>(dd), dp<float>(dout));
   } else {
-    const bool bf = precision == GEYE_BF16;
     const int nt = std::min(3, (n + 15) / 16), gsz = 16 * nt, ng = (n + gsz - 1) / gsz;
-    std::vector<unsigned short> packed, fb((size_t)F0 * F1 * 32, 0);
+    const bool split = split_precision(precision);
+    const int nh = split ? 2 : 1, cw = 32 * nh;   // channels per cell: [hi 32][lo 32] if split
+    std::vector<unsigned short> packed, fb((size_t)F0 * F1 * cw, 0);
+    unsigned short hi, lo;
     for (int g = 0; g < ng; g++)
       for (int xp = 0; xp < w; xp++)
         for (int yp = 0; yp < h; yp++)
           for (int t = 0; t < nt; t++)
-            for (int ch = 0; ch < 4; ch++)
-              for (int k = 0; k < 16; k++)
-                for (int j = 0; j < 8; j++) {
-                  const float v = wt(g * gsz + t * 16 + k, ch * 8 + j, xp, yp);
-                  packed.push_back(bf ? to_bf16(v) : to_f16(v));
-                }
+            for (int hh = 0; hh < nh; hh++)
+              for (int ch = 0; ch < 4; ch++)
+                for (int k = 0; k < 16; k++)
+                  for (int j = 0; j < 8; j++) {
+                    mfma_operand(wt(g * gsz + t * 16 + k, ch * 8 + j, xp, yp), precision, hi, lo);
+                    packed.push_back(hh ? lo : hi);
+                  }
     for (int x = 0; x < F1; x++)
       for (int y = 0; y < F0; y++)
         for (int f = 0; f < 31; f++) {
-          const float v = feat[(size_t)f * F0 * F1 + (size_t)x * F0 + y];
-          fb[((size_t)x * F0 + y) * 32 + f] = bf ? to_bf16(v) : to_f16(v);
+          mfma_operand(feat[(size_t)f * F0 * F1 + (size_t)x * F0 + y], precision, hi, lo);
+          fb[((size_t)x * F0 + y) * cw + f] = hi;
+          if (split) fb[((size_t)x * F0 + y) * cw + 32 + f] = lo;
         }
     std::vector<MfmaTile> tiles;
     for (int k = 0; k < ng * gsz; k++) {
@@ -2451,7 +2487,8 @@ int geye_fconv(geye_ctx *c, const float *feat, int F0, int F1, const float *filt
     CHK(upload(dm, moff, c->stream));
     CHK(upload(dd, mdims, c->stream));
     CHK(ensure(dout, sizeof(float) * (size_t)n * H * W));
-    if (launch_conv_mfma(c->stream, precision, nt, dp<MfmaTile>(dt), (int)tiles.size(), h, w,
+    if (launch_conv_mfma(c->stream, mfma_kind(precision), split, nt, dp<MfmaTile>(dt),
+                         (int)tiles.size(), h, w,
                          dfe.p, dfi.p, dp<long long>(dm), dp<int2>(dd), dp<float>(dout)) != 0)
       return fail(GEYE_ERR_UNSUPPORTED, "filter too large for the MFMA kernel's LDS");
   }
@@ -2554,7 +2591,7 @@ int geye_synchronize(geye_ctx *c) {
 }
 
 int geye_mfma_tuning(geye_ctx *c, int precision, int *choice, char *text, int cap) {
-  if (!c || (precision != GEYE_BF16 && precision != GEYE_FP16)) return fail(GEYE_ERR_ARG, "bad argument");
+  if (!c || !mfma_precision(precision)) return fail(GEYE_ERR_ARG, "bad argument");
   std::lock_guard<std::mutex> lk(c->mu);
   Plan *P = c->last_plan;
   if (!P) return fail(GEYE_ERR_ARG, "no detection has run on this context");
@@ -2573,13 +2610,14 @@ int geye_mfma_tuning(geye_ctx *c, int precision, int *choice, char *text, int ca
 }
 
 int geye_mfma_schedule(geye_ctx *c, int precision, int choice) {
-  if (!c || (precision != GEYE_BF16 && precision != GEYE_FP16)) return fail(GEYE_ERR_ARG, "bad argument");
+  if (!c || !mfma_precision(precision)) return fail(GEYE_ERR_ARG, "bad argument");
   std::lock_guard<std::mutex> lk(c->mu);
   Plan *P = c->last_plan;
   if (!P) return fail(GEYE_ERR_ARG, "no detection has run on this context");
   if (choice != 0) {
     const int mi = (choice - 1) / 8, bpc = (choice - 1) % 8 + 1;
-    if (choice < 0 || mi > 1 || P->q_tiles[mi].empty() || P->q_lds[mi] * bpc > 159 * 1024)
+    const size_t lds = P->q_lds[mi] * (split_precision(precision) ? 2 : 1);
+    if (choice < 0 || mi > 1 || P->q_tiles[mi].empty() || lds * bpc > 159 * 1024)
       return fail(GEYE_ERR_ARG, "no such MFMA schedule for this plan");
   }
   P->mfma_choice[precision] = choice;

@@ -33,11 +33,19 @@ int dt_line_geometry(int n, bool window, int *G, size_t *lds);
 // device (dpm_noalloc.hip)
 int noalloc_setup(int device);
 
-// the packed filters of a model (FP32 [group][31][w][h][NF]; 16-bit for the MFMA kernels)
+// the packed filters of a model (FP32 [group][31][w][h][NF]; 16-bit for the MFMA kernels,
+// indexed by precision GEYE_BF16 .. GEYE_FP16X3)
 struct ModelPacks {
   std::vector<float> f32;
-  std::vector<unsigned short> bf16, f16;
+  std::vector<unsigned short> h16[5];
 };
+
+// precision codes: valid at all, an MFMA form, and its 16-bit kind (1 = BF16, 2 = FP16) and
+// whether it splits every operand into a hi/lo pair
+inline bool valid_precision(int p) { return p >= GEYE_FP32 && p <= GEYE_FP16X3; }
+inline bool mfma_precision(int p) { return p >= GEYE_BF16 && p <= GEYE_FP16X3; }
+inline int mfma_kind(int p) { return p == GEYE_BF16 || p == GEYE_BF16X3 ? 1 : 2; }
+inline bool split_precision(int p) { return p == GEYE_BF16X3 || p == GEYE_FP16X3; }
 // the host side of a model from its description (no device work)
 int model_pack(const geye_model_desc *d, Model &m, ModelPacks &pk);
 

@@ -264,7 +264,7 @@ extern "C" {
 int geye_serve_create(int device, const geye_serve_config *cfg, geye_server **out) {
   if (!cfg || !out) return GEYE_ERR_ARG;
   if (cfg->slots < 1 || cfg->slots > 16 || cfg->queue_cap < 1 ||
-      (cfg->precision != GEYE_FP32 && cfg->precision != GEYE_BF16 && cfg->precision != GEYE_FP16))
+      !valid_precision(cfg->precision))
     return GEYE_ERR_ARG;
   geye_server *S = new geye_server();
   S->cfg = *cfg;

@@ -31,7 +31,13 @@ enum {
   GEYE_ERR_BUSY = -6       /* serving: no slot free for the request */
 };
 
-enum { GEYE_FP32 = 0, GEYE_BF16 = 1, GEYE_FP16 = 2 };
+/* Arithmetic of the filter convolution; every other step is the same in all forms.
+ *   GEYE_FP32:   FP32 FMA chains in the reference's order (bit-exact).
+ *   GEYE_BF16, GEYE_FP16: one MFMA per filter tap on 16-bit operands, FP32 accumulation.
+ *   GEYE_BF16X3, GEYE_FP16X3: split operands x = hi + lo (hi the 16-bit rounding of x, lo
+ *     that of x - hi), three MFMAs per tap (hi*hi + hi*lo + lo*hi) with FP32 accumulation:
+ *     16 (BF16) or 22 (FP16) significant bits per operand. */
+enum { GEYE_FP32 = 0, GEYE_BF16 = 1, GEYE_FP16 = 2, GEYE_BF16X3 = 3, GEYE_FP16X3 = 4 };
 
 typedef struct geye_ctx geye_ctx;
 typedef struct geye_model geye_model;
@@ -104,8 +110,8 @@ int geye_nms_host(const float *boxes, int num, int GL, float overlap, float *out
  *   [root box][part boxes][component][score][scale], in the reference's order.
  * geye_detect: replaces car_detection()'s detect + nms (CT/detect.cpp:232-276,
  *   CT/nms.cpp:94-217): the NMS survivors in original candidate order.
- * Output buffers are malloc'd; free with geye_free.  precision: GEYE_FP32 / BF16 / FP16
- * (the MFMA forms change only the filter convolution). */
+ * Output buffers are malloc'd; free with geye_free.  precision: GEYE_FP32 / BF16 / FP16 /
+ * BF16X3 / FP16X3 (the MFMA forms change only the filter convolution). */
 int geye_get_boxes(geye_ctx *ctx, const geye_model *model, const unsigned char *bgr, int width,
                    int height, int widthStep, float thresh, int precision, float **boxes,
                    int *num, int *GL);
@@ -192,7 +198,7 @@ int geye_fconv(geye_ctx *ctx, const float *feat, int F0, int F1, const float *fi
                int h, int w, int precision, float *out);
 int geye_dt(geye_ctx *ctx, const float *vals, float ax, float bx, float ay, float by,
             const int *dims, float *M, int *Ix, int *Iy);
-/* The DT of the 16-bit forms (GEYE_BF16, GEYE_FP16), same arguments and the same outputs,
+/* The DT of the MFMA forms (GEYE_BF16 ... GEYE_FP16X3), same arguments and the same outputs,
  * bit for bit, as geye_dt (CT/dt.cpp:101).  Each output takes the first least cost in a window
  * bounded by the line block's value range, which holds dt()'s argmin when ax, ay > 0 and the
  * costs' rounding cannot reorder the argmins along a line; where either fails, or the window
@@ -233,7 +239,7 @@ typedef struct {
   int slots;               /* 1..16 frames on the device at once */
   long long hbm_budget;    /* bytes of per-frame working memory held, 0 = unlimited */
   int queue_cap;           /* requests waiting before submit returns GEYE_ERR_BUSY */
-  int precision;           /* GEYE_FP32 / GEYE_BF16 / GEYE_FP16 */
+  int precision;           /* GEYE_FP32 / GEYE_BF16 / GEYE_FP16 / GEYE_BF16X3 / GEYE_FP16X3 */
   float thresh, overlap;
 } geye_serve_config;
 
@@ -292,7 +298,7 @@ int geye_graph_enable(geye_ctx *ctx, int on);
 int geye_mfma_tuning(geye_ctx *ctx, int precision, int *choice, char *text, int cap);
 /* The DT form whole-frame detection runs on ctx: GEYE_DT_AUTO (default) takes the
  * divide and conquer of dt() (CT/dt.cpp:61-136) for GEYE_FP32 and the windowed minimum of
- * geye_dt_window for the 16-bit forms; GEYE_DT_DIVIDE / GEYE_DT_WINDOW take one of them for
+ * geye_dt_window for the MFMA forms; GEYE_DT_DIVIDE / GEYE_DT_WINDOW take one of them for
  * every precision.  Both give the same bits. */
 enum { GEYE_DT_AUTO = 0, GEYE_DT_DIVIDE = 1, GEYE_DT_WINDOW = 2 };
 int geye_dt_form(geye_ctx *ctx, int form);

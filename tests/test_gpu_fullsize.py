@@ -3,10 +3,11 @@
 C2: one 1920x1080 rectangle-noise frame (seed 2), the synthetic car model, FP32, threshold
     -0.5, NMS overlap 0.5 -- every candidate row (about a million) and every NMS survivor
     bit-identical to the oracle (CT/get_boxes.cpp:1497-1570, CT/nms.cpp:94-217).
-C3: the same frame with the convolution in BF16 and FP16 on MFMA (FP32 accumulation): score
-    maps against the oracle's FP64 scoring (max-abs and relative error printed and bounded),
-    the windowed DT's rows equal to the divide and conquer's on the scores the form computed,
-    and the post-NMS detection set against the FP32 form's at IoU >= 0.99.
+C3: the same frame with the convolution in BF16 and FP16 on MFMA (FP32 accumulation), and in
+    the split forms (hi/lo operand pairs, three MFMAs per tap): score maps against the
+    oracle's FP64 scoring (max-abs and relative error printed and bounded), the windowed DT's
+    rows equal to the divide and conquer's on the scores the form computed, and the post-NMS
+    detection set against the FP32 form's at IoU >= 0.99 (all of it for the split forms).
 C5: the four camera sizes 640x480, 1280x720, 1920x1080 and 3840x2160, each frame detected
     with the car and the person model: every result identical alone and in the serving mix.
 """
@@ -95,8 +96,10 @@ def c3_fp64(c2):
 
 
 # Bounds: the measured errors of each form with headroom.  BF16 keeps 8 significant bits
-# per operand, FP16 11; the scores span about -4..6.
-@pytest.mark.parametrize("prec,max_abs,min_agree", [("bf16", 0.03, 0.97), ("fp16", 0.005, 0.98)])
+# per operand, FP16 11, the split forms 16 and 22; the scores span about -4..6.  The split
+# forms must reproduce the FP32 detection set: every box matched at IoU >= 0.99.
+@pytest.mark.parametrize("prec,max_abs,min_agree", [("bf16", 0.03, 0.97), ("fp16", 0.005, 0.98),
+                                                    ("bf16x3", 2e-4, 1.0), ("fp16x3", 1e-4, 1.0)])
 def test_c3_precision_1080p(ctx, c2, c3_fp64, prec, max_abs, min_agree):
     model, frame, _, kept32 = c2
     gm = ctx.model(model)
@@ -129,7 +132,7 @@ def test_c3_precision_1080p(ctx, c2, c3_fp64, prec, max_abs, min_agree):
 C5_STREAMS = [(640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
 
 
-@pytest.mark.parametrize("precision", ["bf16", "fp32"])
+@pytest.mark.parametrize("precision", ["bf16", "bf16x3", "fp32"])
 def test_c5_camera_sizes_alone_and_in_the_mix(ctx, precision):
     say("serve C5 sizes", precision)
     models = [ctx.model(synth.synthetic_car()), ctx.model(synth.synthetic_person())]

@@ -32,7 +32,7 @@ def _frames(n_per_stream, seed0=100):
     return frames
 
 
-@pytest.mark.parametrize("precision", ["bf16", "fp32"])
+@pytest.mark.parametrize("precision", ["bf16", "bf16x3", "fp32"])
 def test_results_identical_alone_and_in_the_mix(ctx, precision):
     say("serve mix", precision)
     models = [ctx.model(synth.synthetic_car()), ctx.model(synth.synthetic_person())]
@@ -108,7 +108,7 @@ def _other_interval_model():
     return m
 
 
-@pytest.mark.parametrize("precision", ["bf16", "fp32"])
+@pytest.mark.parametrize("precision", ["bf16", "bf16x3", "fp32"])
 @pytest.mark.parametrize("W,H", [(640, 480), (1920, 1080)])
 def test_detect_models_equals_each_model_alone(ctx, precision, W, H):
     """geye_detect_models: car and person share one pyramid (person's levels are a prefix of
@@ -138,7 +138,7 @@ def test_detect_models_equals_each_model_alone(ctx, precision, W, H):
         m.free()
 
 
-@pytest.mark.parametrize("precision", ["bf16", "fp32"])
+@pytest.mark.parametrize("precision", ["bf16", "bf16x3", "fp32"])
 def test_served_model_pairs_identical_alone_and_in_the_mix(ctx, precision):
     """Frames submitted once for car and person (geye_serve_submit_models: one pyramid per
     frame), mixed with single-model requests, on the four camera sizes."""
@@ -168,6 +168,8 @@ def test_served_model_pairs_identical_alone_and_in_the_mix(ctx, precision):
                                                      ("fp32", 640, 480, 5, -0.5),
                                                      ("fp16", 800, 600, 2, -0.5),
                                                      ("bf16", 1920, 1080, 3, -0.5),
+                                                     ("bf16x3", 1280, 720, 3, -0.5),
+                                                     ("fp16x3", 640, 480, 2, -0.5),
                                                      ("fp32", 640, 480, 3, 1.5)])
 def test_detect_batch_equals_each_frame_alone(ctx, precision, W, H, nf, thresh):
     """geye_detect_batch: nf frames through car, person (sharing car's pyramid) and an
