@@ -224,10 +224,11 @@ def serve_closed_loop(geye_amd, gm, d_frame, precision, slots, steps, warmup):
             lat.append(srv.wait(t)[1])
         total = time.perf_counter() - t0
         fallbacks = srv.nms_fallbacks()
+        repairs = srv.nms_tie_repairs()
     lat = np.array(lat)
     return {"slots": slots, "fps": steps / total, "ms_per_step": total / steps * 1e3,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "nms_host_replays": fallbacks}
+            "nms_host_replays": fallbacks, "nms_tie_repairs": repairs}
 
 
 def two_model_batch(geye_amd, ctx, synth, slots, precision):
@@ -246,6 +247,7 @@ def two_model_batch(geye_amd, ctx, synth, slots, precision):
         lat = [max(srv.wait(a)[1], srv.wait(b)[1]) for a, b in tickets]
         total = time.perf_counter() - t0
         fallbacks = srv.nms_fallbacks()
+        repairs = srv.nms_tie_repairs()
     for f in frames:
         ctx.dev_free(f)
     car.free()
@@ -253,7 +255,8 @@ def two_model_batch(geye_amd, ctx, synth, slots, precision):
     return {"workload": f"16 x 1280x720 frames x (synthetic car + synthetic person), "
                         f"{precision}, one shared pyramid per frame",
             "slots": slots, "frames_per_s": 16 / total, "batch_ms": total * 1e3,
-            "max_request_ms": float(max(lat)), "nms_host_replays": fallbacks}
+            "max_request_ms": float(max(lat)), "nms_host_replays": fallbacks,
+            "nms_tie_repairs": repairs}
 
 
 def serving_mix(geye_amd, ctx, synth, slots, precision, seconds: float = 2.0):
@@ -295,6 +298,7 @@ def serving_mix(geye_amd, ctx, synth, slots, precision, seconds: float = 2.0):
         total = time.perf_counter() - t0
         st = srv.stats()
         fallbacks = srv.nms_fallbacks()
+        repairs = srv.nms_tie_repairs()
     car.free()
     person.free()
     lat = np.array(lat)
@@ -305,7 +309,8 @@ def serving_mix(geye_amd, ctx, synth, slots, precision, seconds: float = 2.0):
             "frames_per_s": len(events) / total, "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)),
             "p99_ms_per_camera": {k: float(np.percentile(v, 99)) for k, v in per_size.items()},
-            "peak_working_bytes": st["peak_bytes"], "nms_host_replays": fallbacks}
+            "peak_working_bytes": st["peak_bytes"], "nms_host_replays": fallbacks,
+            "nms_tie_repairs": repairs}
 
 
 def probe(precision: str, frames: int) -> int:

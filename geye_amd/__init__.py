@@ -116,6 +116,7 @@ _SIGS = [
     ("geye_serve_stats", C.c_int, [C.c_void_p, _llp, _dp, _dp, _dp, _llp, _llp]),
     ("geye_serve_reset_stats", C.c_int, [C.c_void_p]),
     ("geye_serve_nms_fallbacks", C.c_int, [C.c_void_p, _llp]),
+    ("geye_serve_nms_tie_repairs", C.c_int, [C.c_void_p, _llp]),
     ("geye_plan_bytes", C.c_int, [C.c_void_p, C.c_int, C.c_int, _llp]),
     ("geye_plan_bytes_desc", C.c_int, [C.c_void_p, C.c_int, C.c_int, _llp]),
     ("geye_serve_destroy", None, [C.c_void_p]),
@@ -140,6 +141,7 @@ _SIGS = [
     ("geye_profile_enable", C.c_int, [C.c_void_p, C.c_int]),
     ("geye_kernel_stats", C.c_int, [C.c_void_p, C.c_char_p, C.c_int, _dp, _llp, C.c_int, _ip]),
     ("geye_nms_fallbacks", C.c_int, [C.c_void_p, _llp]),
+    ("geye_nms_tie_repairs", C.c_int, [C.c_void_p, _llp]),
     ("geye_csv_values", C.c_int, [C.c_char_p, C.POINTER(_fp), _llp]),
     ("geye_dt_form", C.c_int, [C.c_void_p, C.c_int]),
 ]
@@ -336,6 +338,11 @@ class Server:
         _check(lib().geye_serve_nms_fallbacks(self.h, C.byref(n)))
         return int(n.value)
 
+    def nms_tie_repairs(self) -> int:
+        n = C.c_longlong()
+        _check(lib().geye_serve_nms_tie_repairs(self.h, C.byref(n)))
+        return int(n.value)
+
     def close(self):
         if self.h:
             lib().geye_serve_destroy(self.h)
@@ -498,9 +505,17 @@ class Context:
         _check(lib().geye_dt_form(self.h, {"auto": 0, "divide": 1, "window": 2}[form]))
 
     def nms_fallbacks(self) -> int:
-        """NMS calls on this context that were replayed on the host (score ties)."""
+        """NMS calls on this context that were replayed on the host (score ties the device
+        order could not settle)."""
         n = C.c_longlong()
         _check(lib().geye_nms_fallbacks(self.h, C.byref(n)))
+        return int(n.value)
+
+    def nms_tie_repairs(self) -> int:
+        """NMS calls on this context that ran again with tied scores in the reference's
+        quicksort order (set by the host's partial quicksort)."""
+        n = C.c_longlong()
+        _check(lib().geye_nms_tie_repairs(self.h, C.byref(n)))
         return int(n.value)
 
     def synchronize(self):

@@ -284,7 +284,7 @@ struct Plan {
   // device NMS
   DeviceBuffer n_box, n_keys, n_keys2, n_sbox, n_alive0, n_alive1, n_status, n_fsup, n_picks,
       n_counters, n_mask, n_sort, n_flags, n_keep_orig, n_kept,
-      n_total, n_vals, n_ord, n_rank, n_dec, n_alist, n_acount, n_grid;
+      n_total, n_vals, n_ord, n_rank, n_dec, n_alist, n_acount, n_grid, n_tie;
   long long nms_fallbacks = 0;
   std::map<long long, hipGraphExec_t> graphs;  // front of the frame, per (precision, ws, thresh)
   // batch plans: per frame's NMS job its small buffers (counters, flags, totals), window
@@ -351,6 +351,7 @@ struct geye_ctx {
   int *d_hcount = nullptr;            // compaction kernels write them directly; see kHost*)
   hipEvent_t ev_count[2] = {nullptr, nullptr};
   long long nms_fallbacks = 0;
+  long long nms_tie_repairs = 0;  // NMS calls whose tie order the host set by a partial quicksort
 };
 
 struct geye_model {

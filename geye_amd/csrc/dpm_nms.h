@@ -7,6 +7,12 @@ namespace geye {
 
 constexpr int kNmsWindow = 1024;
 constexpr int kNoSup = 0x7fffffff;
+// tie records of the finalisation check (a hash set of the scores whose equal-score order
+// decided an outcome, empty slots kNmsTieEmpty, a NaN no score takes) and the scores whose
+// equal-score order the host has set to the reference's
+constexpr int kNmsTieCap = 256;
+constexpr int kNmsResolvedCap = 32;
+constexpr unsigned int kNmsTieEmpty = 0x7fc00001u;
 
 struct NmsBox {
   float y1, x1, y2, x2, area, score;
@@ -54,11 +60,11 @@ void launch_nms_prep_rows(hipStream_t s, const float *rows, int n, int GL, NmsBo
                           unsigned int *keys, int *vals);
 int launch_nms_sort(hipStream_t s, void *temp, size_t temp_bytes, unsigned int *keys_in,
                     unsigned int *keys_out, int *vals_in, int *vals_out, int n);
-// also zeroes counters[0..3], flags[0..3] and keep[0..n), and sets *nalive0 = n; with a grid,
-// fills its rank_of, clears dec and the masks
+// also zeroes counters[0..3], flags[0..3] and keep[0..n), empties the tie set (tie, when
+// non-null) and sets *nalive0 = n; with a grid, fills its rank_of, clears dec and the masks
 void launch_nms_gather(hipStream_t s, const int *ord, const NmsBox *box, int n, NmsBox *sbox,
                        int *alive, int *status, int *sup, int *counters, int *nalive0,
-                       int *flags, int *keep, const NmsGrid *grid);
+                       int *flags, int *keep, const NmsGrid *grid, float *tie);
 // One round: window bitmask, greedy, suppression of the later boxes, then the fused
 // retire-and-count pass that leaves per-block counts of the surviving later boxes in
 // blocksum.  nalive_host is a host upper bound of *nalive: the kernels read the true count
@@ -77,9 +83,14 @@ int launch_nms_round(hipStream_t s, const NmsBox *sbox, const int *alive, const 
 void launch_nms_compact(hipStream_t s, const int *alive, const int *nalive, const int *sup,
                         int *blocksum, int bound, int *nalive_out, int *alive_out,
                         int *host_total);
-// the pick count is read from counters[0] on the device; bound >= that count
+// the pick count is read from counters[0] on the device; bound >= that count.  A tie that
+// decided an outcome sets flags[0] unless its score is one of the nresolved scores in
+// resolved (their equal-score runs are in the reference's order); each such score is also
+// entered in the tie set tie_vals (flags[2] counts the distinct scores entered, flags[3] is
+// set when the set is full).
 void launch_nms_check(hipStream_t s, const NmsBox *sbox, int n, const int *status, const int *sup,
-                      const int *picks, const int *counters, int *flags);
+                      const int *picks, const int *counters, int *flags, const float *resolved,
+                      int nresolved, float *tie_vals);
 void launch_nms_keep(hipStream_t s, const NmsBox *sbox, const int *picks, const int *counters,
                      int bound, const int *flags, int *keep_orig,
                      int *host_tie);

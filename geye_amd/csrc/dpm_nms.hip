@@ -122,14 +122,16 @@ __global__ __launch_bounds__(256) void k_nms_gather(const int *__restrict__ ord,
                                                     int *__restrict__ counters,
                                                     int *__restrict__ nalive0,
                                                     int *__restrict__ flags,
-                                                    int *__restrict__ keep, NmsGrid g) {
+                                                    int *__restrict__ keep, NmsGrid g,
+                                                    float *__restrict__ tie) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   // the pass's other initial state: pick counters, the first round's alive count, the tie
-  // flags and the survivor flags (written by the finalisation, after the rounds)
+  // flags and set, and the survivor flags (written by the finalisation, after the rounds)
   if (r < 4) {
     counters[r] = 0;
     flags[r] = 0;
   }
+  if (tie && r < kNmsTieCap) ((unsigned int *)tie)[r] = kNmsTieEmpty;  // the grid has >= 256
   if (r == 0) *nalive0 = n;
   if (g.ext && r < kNmsMaskWords * nms_grid_tiles(g)) g.ext[r] = 0;  // ext, ctr, ctrc
   if (r >= n) return;
@@ -731,12 +733,36 @@ __global__ __launch_bounds__(256) void k_nms_compact(const int *__restrict__ ali
 
 
 // finalisation: conflicts, last-pick emission, survivors in original order
+// a tie on score v decided an outcome: flag it unless v's equal-score run is already in the
+// reference's order, and enter v in the tie set for the host (open addressing on its bits)
+__device__ inline void nms_tie(int *flags, int bit, float v, const float *resolved, int nresolved,
+                               float *tie_vals) {
+  for (int k = 0; k < nresolved; k++)
+    if (resolved[k] == v) return;
+  atomicOr(&flags[0], bit);
+  const unsigned int b = __float_as_uint(v);
+  unsigned int *tab = (unsigned int *)tie_vals;
+  unsigned int h = (b * 2654435761u) >> 24;  // kNmsTieCap = 256 slots
+  for (int k = 0; k < kNmsTieCap; k++, h = (h + 1) & (kNmsTieCap - 1)) {
+    const unsigned int old = atomicCAS(&tab[h], kNmsTieEmpty, b);
+    if (old == kNmsTieEmpty) {
+      atomicAdd(&flags[2], 1);
+      return;
+    }
+    if (old == b) return;
+  }
+  atomicOr(&flags[3], 1);  // the set is full
+}
+
 __global__ __launch_bounds__(256) void k_nms_check(const NmsBox *__restrict__ sbox, int n,
                                                    const int *__restrict__ status,
                                                    const int *__restrict__ sup,
                                                    const int *__restrict__ picks,
                                                    const int *__restrict__ counters,
-                                                   int *__restrict__ flags) {
+                                                   int *__restrict__ flags,
+                                                   const float *__restrict__ resolved,
+                                                   int nresolved,
+                                                   float *__restrict__ tie_vals) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int npick = counters[0];
   if (r >= n || npick == 0) return;
@@ -747,10 +773,12 @@ __global__ __launch_bounds__(256) void k_nms_check(const NmsBox *__restrict__ sb
       atomicOr(&flags[0], 4);  // undecided box: cannot happen; forces the exact host path
       return;
     }
-    if (sbox[f].score == sbox[r].score) atomicOr(&flags[0], 1);  // a tie decided the outcome
-    if (f == last) atomicOr(&flags[1], 1);                      // last pick suppressed something
+    if (sbox[f].score == sbox[r].score)  // a tie decided the outcome
+      nms_tie(flags, 1, sbox[r].score, resolved, nresolved, tie_vals);
+    if (f == last) atomicOr(&flags[1], 1);  // last pick suppressed something
   } else if (r != last && sbox[r].score == sbox[last].score) {
-    atomicOr(&flags[0], 2);  // the identity of the last pick depends on the tie order
+    // the identity of the last pick depends on the tie order
+    nms_tie(flags, 2, sbox[r].score, resolved, nresolved, tie_vals);
   }
 }
 
@@ -819,12 +847,13 @@ int launch_nms_sort(hipStream_t s, void *temp, size_t temp_bytes, unsigned int *
 
 void launch_nms_gather(hipStream_t s, const int *ord, const NmsBox *box, int n, NmsBox *sbox,
                        int *alive, int *status, int *sup, int *counters, int *nalive0,
-                       int *flags, int *keep, const NmsGrid *grid) {
+                       int *flags, int *keep, const NmsGrid *grid, float *tie) {
   NmsGrid g;
   if (grid) g = *grid;
-  const int threads = grid ? std::max(n, kNmsMaskWords * nms_grid_tiles(g)) : n;
+  const int threads = std::max(grid ? std::max(n, kNmsMaskWords * nms_grid_tiles(g)) : n,
+                               kNmsTieCap);
   hipLaunchKernelGGL(k_nms_gather, dim3((threads + 255) / 256), dim3(256), 0, s, ord, box, n,
-                     sbox, alive, status, sup, counters, nalive0, flags, keep, g);
+                     sbox, alive, status, sup, counters, nalive0, flags, keep, g, tie);
 }
 
 void nms_grid_shape(int W, int H, int *gx, int *gy, int *shift) {
@@ -910,9 +939,10 @@ void launch_nms_compact(hipStream_t s, const int *alive, const int *nalive, cons
 }
 
 void launch_nms_check(hipStream_t s, const NmsBox *sbox, int n, const int *status, const int *sup,
-                      const int *picks, const int *counters, int *flags) {
+                      const int *picks, const int *counters, int *flags, const float *resolved,
+                      int nresolved, float *tie_vals) {
   hipLaunchKernelGGL(k_nms_check, dim3((n + 255) / 256), dim3(256), 0, s, sbox, n, status, sup,
-                     picks, counters, flags);
+                     picks, counters, flags, resolved, nresolved, tie_vals);
 }
 
 void launch_nms_keep(hipStream_t s, const NmsBox *sbox, const int *picks, const int *counters,

@@ -569,7 +569,8 @@ void plan_release(Plan &P) {
                           &P.n_status, &P.n_fsup, &P.n_picks, &P.n_counters, &P.n_mask,
                           &P.n_sort, &P.n_flags, &P.n_keep_orig,
                           &P.n_kept, &P.n_total, &P.n_vals, &P.n_ord,
-                          &P.n_rank, &P.n_dec, &P.n_alist, &P.n_acount, &P.n_grid, &P.d_ascore_tab,
+                          &P.n_rank, &P.n_dec, &P.n_alist, &P.n_acount, &P.n_grid, &P.n_tie,
+                          &P.d_ascore_tab,
                           &P.d_ascore,
                           &P.d_grad, &P.d_grid,
                           &P.d_qtiles0, &P.d_qtiles1, &P.d_qcounter, &P.d_fused})
@@ -667,7 +668,7 @@ int get_batch_plan(geye_ctx *c, Model &m, int W, int H, int nb, int feat_len, Pl
   P->b_grid.resize(nb);
   P->b_acount.resize(nb);
   for (int f = 0; f < nb && rc == GEYE_OK; f++) {
-    rc = ensure(P->b_small[f], 64);
+    rc = ensure(P->b_small[f], 64 + sizeof(float) * (kNmsTieCap + kNmsResolvedCap));
     if (rc == GEYE_OK)
       rc = ensure(P->b_mask[f], sizeof(unsigned long long) * kNmsWindow * (kNmsWindow / 64));
     if (rc == GEYE_OK) rc = ensure(P->b_blocksum[f], sizeof(int) * (compact_blocks(cells) + 1));
@@ -1014,6 +1015,7 @@ int nms_buffers(Plan &P, int n) {
   CHK(ensure(P.n_counters, 16));
   CHK(ensure(P.n_flags, 16));
   CHK(ensure(P.n_total, 16));
+  CHK(ensure(P.n_tie, sizeof(float) * (kNmsTieCap + kNmsResolvedCap)));
   CHK(ensure(P.n_mask, sizeof(unsigned long long) * kNmsWindow * (kNmsWindow / 64)));
   CHK(ensure(P.d_blocksum, sizeof(int) * (compact_blocks((long long)N) + 1)));
   CHK(ensure(P.n_sort, nms_sort_temp_bytes(n)));
@@ -1033,6 +1035,10 @@ struct NmsJob {
   volatile int *h_round[2] = {nullptr, nullptr}, *h_surv = nullptr, *h_tie = nullptr;
   int *d_round[2] = {nullptr, nullptr}, *d_surv = nullptr, *d_tie = nullptr;
   int *counters = nullptr, *flags = nullptr, *total = nullptr, *blocksum = nullptr;
+  // tie records ([kNmsTieCap]) then the resolved scores ([kNmsResolvedCap]) on the device;
+  // the resolved scores on the host
+  float *tie = nullptr;
+  std::vector<float> resolved;
   unsigned long long *mask = nullptr;
   void *sort_temp = nullptr;
   size_t sort_bytes = 0;
@@ -1050,15 +1056,16 @@ struct NmsJob {
   int *na_buf(int k) const { return total + 2 * k; }
 };
 
-// Sorts the job's keys, gathers the boxes in that order and enqueues the first round.
-int nms_job_start(geye_ctx *c, NmsJob &J, float overlap) {
+// Sorts the job's keys (unless sort is false: the order is already in place), gathers the
+// boxes in that order and enqueues the first round.
+int nms_job_start(geye_ctx *c, NmsJob &J, float overlap, bool sort = true) {
   Plan &P = *J.P;
   J.done = J.n <= 0;
   J.waiting = false;
   J.nk = 0;
   if (J.done) return GEYE_OK;
   hipStream_t s = J.s;
-  {
+  if (sort) {
     KTimer t(c, "nms_sort", s);
     if (launch_nms_sort(s, J.sort_temp, J.sort_bytes, J.seg<unsigned int>(P.n_keys),
                         J.seg<unsigned int>(P.n_keys2), J.seg<int>(P.n_vals),
@@ -1070,7 +1077,7 @@ int nms_job_start(geye_ctx *c, NmsJob &J, float overlap) {
   launch_nms_gather(s, J.seg<int>(P.n_ord), J.seg<NmsBox>(P.n_box), J.n, J.seg<NmsBox>(P.n_sbox),
                     J.seg<int>(P.n_alive0), J.seg<int>(P.n_status), J.seg<int>(P.n_fsup),
                     J.counters, J.na_buf(0), J.flags, J.seg<int>(P.n_keep_orig),
-                    J.use_grid ? &J.grid : nullptr);
+                    J.use_grid ? &J.grid : nullptr, J.tie);
   J.alive = J.seg<int>(P.n_alive0);
   J.alive_next = J.seg<int>(P.n_alive1);
   J.na = J.n;
@@ -1127,24 +1134,94 @@ void nms_job_finish(NmsJob &J) {
   if (J.n <= 0) return;
   Plan &P = *J.P;
   launch_nms_check(J.s, J.seg<NmsBox>(P.n_sbox), J.n, J.seg<int>(P.n_status),
-                   J.seg<int>(P.n_fsup), J.seg<int>(P.n_picks), J.counters, J.flags);
+                   J.seg<int>(P.n_fsup), J.seg<int>(P.n_picks), J.counters, J.flags,
+                   J.tie + kNmsTieCap, (int)J.resolved.size(), J.tie);
   launch_nms_keep(J.s, J.seg<NmsBox>(P.n_sbox), J.seg<int>(P.n_picks), J.counters, J.n, J.flags,
                   J.seg<int>(P.n_keep_orig), J.d_tie);
   launch_compact(J.s, J.seg<int>(P.n_keep_orig), (long long)J.n, J.blocksum, J.total + 1,
                  J.seg<int>(P.n_kept), J.d_surv);
 }
 
-// After the job's stream has finished: its survivor count, or, when equal scores decided an
-// outcome, the host replay of the reference's exact quicksort order (its survivors written
-// to the same place).
-int nms_job_collect(geye_ctx *c, NmsJob &J, float overlap) {
+// After the job's stream has finished: its survivor count (returns 0), or 1 when equal scores
+// decided an outcome and the order of their runs has been set to the reference's (the job
+// must run again from its gather), or 2 when that cannot be done here (the host replay must
+// run).  The reference's quicksort is unstable: equal scores end in an order fixed by the
+// whole array.  The sort of the device keeps them in candidate order, which is the
+// reference's order wherever no tie decides anything; the finalisation check records each
+// score whose run's order did decide something.  For those scores the host runs the
+// reference's quicksort on the scores in candidate order, restricted to the subarrays that
+// can hold them (O(N) work rather than the host replay's O(N^2)), and writes each run in the
+// order found into the device order; the check of the next run skips those scores.
+int nms_job_collect(geye_ctx *c, NmsJob &J) {
   J.nk = 0;
-  if (J.n <= 0) return GEYE_OK;
+  if (J.n <= 0) return 0;
   Plan &P = *J.P;
-  if (*J.h_tie == 0) {
+  const int tie = *J.h_tie;
+  if (tie == 0) {
     J.nk = *J.h_surv;
-    return GEYE_OK;
+    return 0;
   }
+  if ((tie & 4) || (int)J.resolved.size() >= kNmsResolvedCap) return 2;
+  int fl[2] = {0, 0};
+  std::vector<unsigned int> tab(kNmsTieCap);
+  HIPCHK(hipMemcpyAsync(fl, J.flags + 2, sizeof(fl), hipMemcpyDeviceToHost, J.s));
+  HIPCHK(hipMemcpyAsync(tab.data(), J.tie, sizeof(float) * kNmsTieCap, hipMemcpyDeviceToHost,
+                        J.s));
+  HIPCHK(hipStreamSynchronize(J.s));
+  if (fl[0] <= 0 || fl[1] != 0) return 2;  // nothing recorded, or the set overflowed
+  std::vector<float> vals;
+  for (unsigned int b : tab)
+    if (b != kNmsTieEmpty) {
+      float v;
+      std::memcpy(&v, &b, 4);
+      vals.push_back(v);
+    }
+  for (float v : vals)
+    if (v == 0.0f || v != v) return 2;  // +0 and -0 are one run for the reference, two here
+  for (float v : vals)
+    if (std::find(J.resolved.begin(), J.resolved.end(), v) == J.resolved.end())
+      J.resolved.push_back(v);
+  if ((int)J.resolved.size() > kNmsResolvedCap) return 2;
+  // the scores in candidate order, from the sort keys (descending order of the float bits)
+  const size_t N = (size_t)J.n;
+  std::vector<unsigned int> keys(N);
+  HIPCHK(hipMemcpyAsync(keys.data(), J.seg<unsigned int>(P.n_keys), 4 * N, hipMemcpyDeviceToHost,
+                        J.s));
+  HIPCHK(hipStreamSynchronize(J.s));
+  std::vector<float> sc(N);
+  std::vector<int> order(N);
+  for (size_t i = 0; i < N; i++) {
+    unsigned int u = ~keys[i];
+    u = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    std::memcpy(&sc[i], &u, 4);
+    order[i] = (int)i;
+  }
+  // each resolved score's run starts after every greater score in the device order
+  std::vector<int> start(J.resolved.size(), 0), count(J.resolved.size(), 0);
+  for (size_t i = 0; i < N; i++)
+    for (size_t k = 0; k < J.resolved.size(); k++) {
+      start[k] += sc[i] > J.resolved[k];
+      count[k] += sc[i] == J.resolved[k];
+    }
+  quicksort_host(sc.data(), order.data(), 0, (int)N - 1, &J.resolved);
+  for (size_t k = 0; k < J.resolved.size(); k++) {
+    const float v = J.resolved[k];
+    const int a = start[k], b = a + count[k];
+    for (int p = a; p < b; p++)
+      if (p >= (int)N || sc[p] != v) return 2;  // the run is not where the counts put it
+    HIPCHK(hipMemcpyAsync(J.seg<int>(P.n_ord) + a, order.data() + a, sizeof(int) * (b - a),
+                          hipMemcpyHostToDevice, J.s));
+  }
+  HIPCHK(hipMemcpyAsync(J.tie + kNmsTieCap, J.resolved.data(), sizeof(float) * J.resolved.size(),
+                        hipMemcpyHostToDevice, J.s));
+  HIPCHK(hipStreamSynchronize(J.s));
+  return 1;
+}
+
+// The host replay of the reference's NMS (its exact quicksort order); the survivors are
+// written where the device's would be.
+int nms_job_host_replay(geye_ctx *c, NmsJob &J, float overlap) {
+  Plan &P = *J.P;
   P.nms_fallbacks++;
   c->nms_fallbacks++;
   c->stats["nms_host_fallback"].count++;
@@ -1171,23 +1248,48 @@ int nms_job_collect(geye_ctx *c, NmsJob &J, float overlap) {
   return GEYE_OK;
 }
 
-// Runs the jobs' NMS together: each job's boxes, keys and values are in place (a prep
-// kernel on its stream); leaves each job's survivors' local candidate positions, ascending,
-// in its segment of P.n_kept and their count in nk.
-int nms_run(geye_ctx *c, std::vector<NmsJob> &jobs, float overlap) {
-  for (NmsJob &J : jobs) CHK(nms_job_start(c, J, overlap));
+// Drives started jobs' rounds together to their finalisation and waits for their streams.
+int nms_drive(geye_ctx *c, std::vector<NmsJob *> &jobs, float overlap) {
   for (;;) {
     bool any = false;
-    for (NmsJob &J : jobs) {
-      CHK(nms_job_enqueue(c, J, overlap));
-      any = any || J.waiting;
+    for (NmsJob *J : jobs) {
+      CHK(nms_job_enqueue(c, *J, overlap));
+      any = any || J->waiting;
     }
     if (!any) break;
-    for (NmsJob &J : jobs) CHK(nms_job_wait(J));
+    for (NmsJob *J : jobs) CHK(nms_job_wait(*J));
   }
-  for (NmsJob &J : jobs) nms_job_finish(J);
-  for (NmsJob &J : jobs) HIPCHK(hipStreamSynchronize(J.s));
-  for (NmsJob &J : jobs) CHK(nms_job_collect(c, J, overlap));
+  for (NmsJob *J : jobs) nms_job_finish(*J);
+  for (NmsJob *J : jobs) HIPCHK(hipStreamSynchronize(J->s));
+  return GEYE_OK;
+}
+
+// Runs the jobs' NMS together: each job's boxes, keys and values are in place (a prep
+// kernel on its stream); leaves each job's survivors' local candidate positions, ascending,
+// in its segment of P.n_kept and their count in nk.  A job whose outcome a tie decided runs
+// again with the reference's order of the tied runs (up to 4 times), else on the host.
+int nms_run(geye_ctx *c, std::vector<NmsJob> &jobs, float overlap) {
+  std::vector<NmsJob *> all;
+  for (NmsJob &J : jobs) {
+    J.resolved.clear();
+    CHK(nms_job_start(c, J, overlap));
+    all.push_back(&J);
+  }
+  CHK(nms_drive(c, all, overlap));
+  for (NmsJob &J : jobs) {
+    int st = nms_job_collect(c, J);
+    for (int attempt = 0; st == 1 && attempt < 4; attempt++) {
+      if (attempt == 0) {
+        c->nms_tie_repairs++;
+        c->stats["nms_tie_repair"].count++;
+      }
+      std::vector<NmsJob *> one{&J};
+      CHK(nms_job_start(c, J, overlap, false));
+      CHK(nms_drive(c, one, overlap));
+      st = nms_job_collect(c, J);
+    }
+    if (st != 0) CHK(nms_job_host_replay(c, J, overlap));
+  }
   HIPCHK(hipGetLastError());
   return GEYE_OK;
 }
@@ -1214,6 +1316,7 @@ NmsJob single_job(geye_ctx *c, Plan &P, int n, const NmsGrid *grid) {
   J.counters = dp<int>(P.n_counters);
   J.flags = dp<int>(P.n_flags);
   J.total = dp<int>(P.n_total);
+  J.tie = dp<float>(P.n_tie);
   J.blocksum = dp<int>(P.d_blocksum);
   J.mask = dp<unsigned long long>(P.n_mask);
   J.sort_temp = P.n_sort.p;
@@ -1587,6 +1690,7 @@ int detect_batch(geye_ctx *c, const geye_model *const *models, int nm,
       J.counters = small;
       J.flags = small + 4;
       J.total = small + 8;
+      J.tie = (float *)(small + 16);
       J.blocksum = dp<int>(B.b_blocksum[f]);
       J.mask = dp<unsigned long long>(B.b_mask[f]);
       J.sort_temp = B.b_sort[f].p;
@@ -1695,7 +1799,7 @@ struct Scratch {
 // =======================================================================================
 namespace geye {
 
-void quicksort_host(float *ary, int *order, int first, int last) {
+void quicksort_host(float *ary, int *order, int first, int last, const std::vector<float> *only) {
   while (true) {
     int i = first, j = last;
     const float key = ary[(first + last) / 2];
@@ -1708,15 +1812,27 @@ void quicksort_host(float *ary, int *order, int first, int last) {
       i++;
       j--;
     }
+    // [first, i-1] holds values >= key and [j+1, last] values <= key (i <= j + 1, so the two
+    // ranges are disjoint); with `only`, a side is sorted further only if one of those
+    // values can lie in it
+    bool left = first < i - 1, right = last > j + 1;
+    if (only) {
+      bool ge = false, le = false;
+      for (float v : *only) {
+        ge = ge || v >= key;
+        le = le || v <= key;
+      }
+      left = left && ge;
+      right = right && le;
+    }
     // recurse into the smaller side first to bound the stack; the order of the two
     // recursive calls does not change the result (they touch disjoint ranges)
-    const bool left = first < i - 1, right = last > j + 1;
     if (left && right) {
       if ((i - 1 - first) < (last - j - 1)) {
-        quicksort_host(ary, order, first, i - 1);
+        quicksort_host(ary, order, first, i - 1, only);
         first = j + 1;
       } else {
-        quicksort_host(ary, order, j + 1, last);
+        quicksort_host(ary, order, j + 1, last, only);
         last = i - 1;
       }
     } else if (left) {
@@ -2668,6 +2784,13 @@ int geye_csv_values(const char *path, float **values, long long *count) {
   *values = to_malloc(v);
   if (!*values) return fail(GEYE_ERR_NOMEM, "malloc");
   *count = (long long)v.size();
+  return GEYE_OK;
+}
+
+int geye_nms_tie_repairs(geye_ctx *c, long long *count) {
+  if (!c || !count) return fail(GEYE_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  *count = c->nms_tie_repairs;
   return GEYE_OK;
 }
 

@@ -60,6 +60,9 @@ void release_plans(const geye_ctx *c, const Model *m, long long key);
 
 // host NMS with the reference's exact semantics (CT/nms.cpp:70-217)
 int nms_host(const float *boxes, int n, int GL, float overlap, float *out);
-void quicksort_host(float *ary, int *order, int first, int last);
+// the reference's quicksort (CT/nms.cpp:70-84); with `only`, restricted to the subarrays that
+// can hold one of those values (their elements end where the whole sort puts them)
+void quicksort_host(float *ary, int *order, int first, int last,
+                    const std::vector<float> *only = nullptr);
 
 }  // namespace geye

@@ -377,6 +377,17 @@ int geye_serve_nms_fallbacks(geye_server *S, long long *count) {
   return GEYE_OK;
 }
 
+int geye_serve_nms_tie_repairs(geye_server *S, long long *count) {
+  if (!S || !count) return GEYE_ERR_ARG;
+  long long total = 0;
+  for (geye_ctx *c : S->ctx) {
+    long long k = 0;
+    if (geye_nms_tie_repairs(c, &k) == GEYE_OK) total += k;
+  }
+  *count = total;
+  return GEYE_OK;
+}
+
 int geye_plan_bytes(const geye_model *model, int width, int height, long long *bytes) {
   if (!model || !bytes || width < 8 || height < 8) return GEYE_ERR_ARG;
   return plan_bytes(model->m, width, height, bytes);

@@ -266,9 +266,10 @@ int geye_serve_wait(geye_server *srv, long long ticket, float **boxes, int *num,
 int geye_serve_stats(geye_server *srv, long long *completed, double *p50_ms, double *p99_ms,
                      double *fps, long long *peak_bytes, long long *evictions);
 int geye_serve_reset_stats(geye_server *srv);
-/* NMS calls of the server's workers replayed on the host (geye_nms_fallbacks), since the
- * server was created. */
+/* NMS calls of the server's workers replayed on the host (geye_nms_fallbacks) and with tie
+ * runs reordered (geye_nms_tie_repairs), since the server was created. */
 int geye_serve_nms_fallbacks(geye_server *srv, long long *count);
+int geye_serve_nms_tie_repairs(geye_server *srv, long long *count);
 /* Device bytes of one frame's working memory for this model and frame size, as the server
  * accounts it against hbm_budget (host-only; no device work). */
 int geye_plan_bytes(const geye_model *model, int width, int height, long long *bytes);
@@ -305,11 +306,15 @@ int geye_dt_form(geye_ctx *ctx, int form);
 int geye_mfma_schedule(geye_ctx *ctx, int precision, int choice);
 int geye_kernel_stats(geye_ctx *ctx, char *names, int names_cap, double *ms, long long *count,
                       int cap, int *n);
-/* NMS runs on the device; where equal scores could change its outcome (the reference's
- * quicksort, CT/nms.cpp:70-84, orders ties in its own way) that call is replayed on the host
- * with the reference's exact order.  Reports how many NMS calls on ctx took that host path
- * since the context was created. */
+/* NMS runs on the device.  The reference's quicksort (CT/nms.cpp:70-84) orders equal scores
+ * in its own way; where that order decides an outcome, the host runs the quicksort on the
+ * scores restricted to the subarrays holding those scores, sets the device order of their
+ * runs to the result and the device NMS runs again (geye_nms_tie_repairs counts such calls).
+ * When that cannot settle it (a zero score, more than 32 tied scores), the call is replayed
+ * on the host with the reference's exact order: geye_nms_fallbacks counts those.  Both
+ * count NMS calls on ctx since the context was created. */
 int geye_nms_fallbacks(geye_ctx *ctx, long long *count);
+int geye_nms_tie_repairs(geye_ctx *ctx, long long *count);
 
 #ifdef __cplusplus
 }

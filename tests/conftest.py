@@ -14,7 +14,9 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) and the built library")
-    config.addinivalue_line("markers", "nms_ties: the test's NMS call must take the host replay")
+    config.addinivalue_line("markers", "nms_ties: the test's NMS call must reorder tied runs on "
+                                       "the device order (no host replay)")
+    config.addinivalue_line("markers", "nms_replay: the test's NMS call must take the host replay")
 
 
 @pytest.fixture(scope="session")

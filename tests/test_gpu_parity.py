@@ -22,16 +22,19 @@ def say(*a):
 
 @pytest.fixture(autouse=True)
 def nms_path(request, ctx):
-    """Every test here runs its NMS on the device: the host replay that score ties trigger
-    (geye_nms_fallbacks) must not have run, except in the test built to create ties, whose
-    one NMS call must have taken it."""
-    before = ctx.nms_fallbacks()
+    """Every test here runs its NMS on the device: the host replay (geye_nms_fallbacks) must
+    not have run, except in the test built to force it (a tie on a zero score), whose one NMS
+    call must have taken it.  The tests built to create ties must have set the order of the
+    tied runs on the device (geye_nms_tie_repairs) instead."""
+    before, rep0 = ctx.nms_fallbacks(), ctx.nms_tie_repairs()
     yield
-    ran = ctx.nms_fallbacks() - before
-    if request.node.get_closest_marker("nms_ties"):
+    ran, rep = ctx.nms_fallbacks() - before, ctx.nms_tie_repairs() - rep0
+    if request.node.get_closest_marker("nms_replay"):
         assert ran == 1, f"host NMS replays: {ran}"
     else:
         assert ran == 0, f"host NMS replays: {ran}"
+    if request.node.get_closest_marker("nms_ties"):
+        assert rep >= 1, f"tie repairs: {rep}"
 
 
 def bits_equal(a, b):
@@ -303,10 +306,40 @@ def _random_boxes(n, seed, ties=False, GL=11):
                                     (20000, False)])
 def test_nms_matches_oracle(ctx, n, ties):
     """Device NMS against the oracle.  The ties case (scores rounded to quarters) is the test
-    of the host replay: equal scores decide outcomes there, so the call must take the host
-    path with the reference's quicksort order (checked by the nms_path fixture)."""
+    of the tie repair: equal scores decide outcomes there, so the tied runs must take the
+    reference's quicksort order on the device (checked by the nms_path fixture)."""
     say("nms", n, ties)
     rows = _random_boxes(n, seed=n + ties, ties=ties)
+    g = ctx.nms(rows, 0.5)
+    o = O.nms(rows, 0.5)
+    _rows_equal(g, o)
+
+
+@pytest.mark.nms_ties
+@pytest.mark.parametrize("n,levels,seed", [(4000, 3, 1), (20000, 7, 2), (1500, 1, 3),
+                                           (60000, 24, 4)])
+def test_nms_tie_runs_match_oracle(ctx, n, levels, seed):
+    """Scores drawn from a handful of values (one value: every box tied), so runs of
+    thousands of equal scores decide outcomes: the host's restricted quicksort must put each
+    run in the reference's order (bit for bit against the oracle's whole quicksort)."""
+    say("nms tie runs", n, levels, seed)
+    rows = _random_boxes(n, seed=seed)
+    rng = np.random.default_rng(seed + 100)
+    rows[:, -2] = (rng.integers(1, levels + 1, n) * 0.375 - 2.0).astype(np.float32)
+    rows[rows[:, -2] == 0.0, -2] = 0.125  # no zero score: that run goes to the host replay
+    g = ctx.nms(rows, 0.5)
+    o = O.nms(rows, 0.5)
+    _rows_equal(g, o)
+
+
+@pytest.mark.nms_replay
+def test_nms_zero_score_tie_takes_host_replay(ctx):
+    """A tie on score zero (+0 and -0 are one run for the reference's comparisons but two
+    for the sort keys) is left to the host replay, which must match the oracle."""
+    say("nms zero-score ties")
+    rows = _random_boxes(3000, seed=9)
+    rows[:, -2] = 0.0
+    rows[1::3, -2] = -0.0
     g = ctx.nms(rows, 0.5)
     o = O.nms(rows, 0.5)
     _rows_equal(g, o)
