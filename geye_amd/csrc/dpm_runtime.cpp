@@ -785,8 +785,17 @@ int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, int hk
     launch_hog_votes(s, hog, P.hog16_n + P.hog8_n, P.hog_hist_blocks + P.hog8_blocks,
                      dp<int2>(P.d_grid), dp<float>(P.d_hist), dp<float>(P.d_norm));
   }
-  // the MFMA forms' channel-last copy: 32 16-bit channels per cell, 64 in the split forms
-  if (hkind) CHK(ensure(P.d_featb, (split_precision(hkind) ? 4 : 2) * P.featb_total));
+  // the MFMA forms' channel-last copy: 32 16-bit channels per cell, 64 in the split forms.
+  // Sized for the split forms whatever the form, so that switching forms never moves it
+  // under a captured graph; should it move all the same, the plan's graphs are dropped.
+  if (hkind) {
+    const void *old = P.d_featb.p;
+    CHK(ensure(P.d_featb, 4 * P.featb_total));
+    if (old && P.d_featb.p != old) {
+      for (auto &kv : P.graphs) (void)hipGraphExecDestroy(kv.second);
+      P.graphs.clear();
+    }
+  }
   if (P.hogf_n > 0) {
     KTimer t(c, "hog_feat");
     // the 16-bit forms convolve the channel-last copy only, so the FP32 features are not

@@ -589,6 +589,28 @@ def test_graph_replays_match_eager(ctx, prec):
         ctx.dev_free(d)
 
 
+def test_graphs_survive_precision_switches(ctx):
+    """One plan replays its captured graphs of every form in turn: switching between the
+    16-bit and the split forms (whose feature copy is twice as wide) must not move a buffer
+    under another form's graph -- every replay equals that form's first result."""
+    say("graphs across precision switches")
+    m = ctx.model(synth.synthetic_car())
+    frame = synth.make_frame(400, 300, seed=41)
+    ctx.graphs(True)
+    order = ["bf16", "fp32", "bf16x3", "fp16", "fp16x3", "bf16", "bf16x3", "fp32", "fp16x3"]
+    first = {}
+    for prec in order:
+        for _ in range(2):  # warm-up + capture, then a replay
+            got = m.car_detection(frame, thresh=-0.5, overlap=0.5, precision=prec)
+        scores = m.last_scores().copy()
+        if prec not in first:
+            first[prec] = (got, scores)
+        else:
+            assert bits_equal(got, first[prec][0]), prec
+            assert bits_equal(scores, first[prec][1]), prec
+    m.free()
+
+
 @pytest.mark.parametrize("prec", ["bf16", "fp16", "bf16x3", "fp16x3"])
 def test_every_mfma_schedule_gives_the_same_scores(ctx, prec):
     """The autotuner's candidates (one launch per class; the persistent work-queue launch at
