@@ -520,91 +520,98 @@ __global__ __launch_bounds__(256) void k_hog_feat(const HogLevel *__restrict__ l
                                                   float *__restrict__ feat_arena,
                                                   unsigned short *__restrict__ featb_arena,
                                                   int hkind) {
+  // the block's channel-last cells, staged so that the global stores of the block's
+  // contiguous region are contiguous across lanes: cell c's chunk q at c * CPC + (q ^ swz(c))
+  __shared__ uint4 s_cells[256 * 8];
   const HogLevel L = lv[find_blocks(lv, nlv, (int)blockIdx.x)];
-  long long e = (long long)(blockIdx.x - L.blocks_begin) * blockDim.x + threadIdx.x;
+  const long long e0 = (long long)(blockIdx.x - L.blocks_begin) * blockDim.x;
+  const long long e = e0 + threadIdx.x;
   const long long OSQ = (long long)L.F0 * L.F1;
-  if (e >= OSQ) return;
-  const int x = (int)(e / L.F0), y = (int)(e - (long long)x * L.F0);
-  const int b0 = L.b0;
-  const long long BSQ = (long long)b0 * L.b1;
-  // the 3x3 norm neighbourhood (x..x+2, y..y+2) once; each normaliser sums its 2x2 window
-  // in the reference's order p[0] + p[1] + p[b0] + p[b0 + 1]
-  const float *nb = norm_arena + L.norm_off + (long long)x * b0 + y;
-  float nv[3][3];
+  const int cpc = hkind <= 2 ? 4 : 8;               // 16-byte chunks per cell
+  auto swz = [&](long long c) { return cpc == 4 ? (int)((c >> 1) & 3) : (int)(c & 7); };
+  if (e < OSQ) {
+    const int x = (int)(e / L.F0), y = (int)(e - (long long)x * L.F0);
+    const int b0 = L.b0;
+    const long long BSQ = (long long)b0 * L.b1;
+    // the 3x3 norm neighbourhood (x..x+2, y..y+2) once; each normaliser sums its 2x2 window
+    // in the reference's order p[0] + p[1] + p[b0] + p[b0 + 1]
+    const float *nb = norm_arena + L.norm_off + (long long)x * b0 + y;
+    float nv[3][3];
 #pragma unroll
-  for (int i = 0; i < 3; i++)
+    for (int i = 0; i < 3; i++)
 #pragma unroll
-    for (int j = 0; j < 3; j++) nv[i][j] = nb[(long long)i * b0 + j];
-  auto nf = [&](int i, int j) {
-    const float sum = nv[i][j] + nv[i][j + 1] + nv[i + 1][j] + nv[i + 1][j + 1];
-    return (float)(1.0 / sqrt((double)sum + 0.0001));
-  };
-  const float n1 = nf(1, 1), n2 = nf(1, 0), n3 = nf(0, 1), n4 = nf(0, 0);
-  const float *src = hist_arena + L.hist_off + (long long)(x + 1) * b0 + (y + 1);
-  float fv[31];
-  float t1 = 0.0f, t2 = 0.0f, t3 = 0.0f, t4 = 0.0f;
-  float hs[18];
-#pragma unroll
-  for (int o = 0; o < 18; o++) hs[o] = src[o * BSQ];
-#pragma unroll
-  for (int o = 0; o < 18; o++) {
-    const float hv = hs[o];
-    float h1 = min_2(hv * n1), h2 = min_2(hv * n2), h3 = min_2(hv * n3), h4 = min_2(hv * n4);
-    fv[o] = 0.5f * (h1 + h2 + h3 + h4);
-    t1 += h1; t2 += h2; t3 += h3; t4 += h4;
-  }
-#pragma unroll
-  for (int o = 0; o < 9; o++) {
-    float sum = hs[o] + hs[o + 9];
-    float h1 = min_2(sum * n1), h2 = min_2(sum * n2), h3 = min_2(sum * n3), h4 = min_2(sum * n4);
-    fv[18 + o] = 0.5f * (h1 + h2 + h3 + h4);
-  }
-  fv[27] = (float)(0.2357 * (double)t1);
-  fv[28] = (float)(0.2357 * (double)t2);
-  fv[29] = (float)(0.2357 * (double)t3);
-  fv[30] = (float)(0.2357 * (double)t4);
-  if (feat_arena) {
-    float *dst = feat_arena + L.feat_off + e;
-#pragma unroll
-    for (int o = 0; o < 31; o++) dst[o * OSQ] = fv[o];
-  }
-  if (featb_arena && hkind <= 2) {
-    // 32 16-bit channels (the last one 0) as four 16-byte stores; featb_off is a multiple
-    // of 32 elements, so every cell starts on a 64-byte boundary
-    uint4 *db = (uint4 *)(featb_arena + L.featb_off + e * 32);
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      unsigned int w[4];
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int c = q * 8 + r * 2;
-        const unsigned int lo = half_bits(fv[c], hkind);
-        const unsigned int hi = c + 1 < 31 ? half_bits(fv[c + 1], hkind) : 0u;
-        w[r] = lo | (hi << 16);
-      }
-      db[q] = make_uint4(w[0], w[1], w[2], w[3]);
-    }
-  } else if (featb_arena) {
-    // the split pair: 64 channels as eight 16-byte stores, 128 bytes per cell
-    uint4 *db = (uint4 *)(featb_arena + 2 * L.featb_off + e * 64);
-    auto hl = [&](int c, bool want_lo) -> unsigned int {
-      if (c >= 31) return 0u;
-      const unsigned short h = half_bits(fv[c], hkind);
-      if (!want_lo) return h;
-      const float hv = hkind == 3 ? __uint_as_float((unsigned int)h << 16)
-                                  : __half2float(__ushort_as_half(h));
-      return half_bits(fv[c] - hv, hkind);
+      for (int j = 0; j < 3; j++) nv[i][j] = nb[(long long)i * b0 + j];
+    auto nf = [&](int i, int j) {
+      const float sum = nv[i][j] + nv[i][j + 1] + nv[i + 1][j] + nv[i + 1][j + 1];
+      return (float)(1.0 / sqrt((double)sum + 0.0001));
     };
+    const float n1 = nf(1, 1), n2 = nf(1, 0), n3 = nf(0, 1), n4 = nf(0, 0);
+    const float *src = hist_arena + L.hist_off + (long long)(x + 1) * b0 + (y + 1);
+    float fv[31];
+    float t1 = 0.0f, t2 = 0.0f, t3 = 0.0f, t4 = 0.0f;
+    float hs[18];
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      unsigned int w[4];
+    for (int o = 0; o < 18; o++) hs[o] = src[o * BSQ];
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int c = (q & 3) * 8 + r * 2;
-        w[r] = hl(c, q >= 4) | (hl(c + 1, q >= 4) << 16);
-      }
-      db[q] = make_uint4(w[0], w[1], w[2], w[3]);
+    for (int o = 0; o < 18; o++) {
+      const float hv = hs[o];
+      float h1 = min_2(hv * n1), h2 = min_2(hv * n2), h3 = min_2(hv * n3), h4 = min_2(hv * n4);
+      fv[o] = 0.5f * (h1 + h2 + h3 + h4);
+      t1 += h1; t2 += h2; t3 += h3; t4 += h4;
     }
+#pragma unroll
+    for (int o = 0; o < 9; o++) {
+      float sum = hs[o] + hs[o + 9];
+      float h1 = min_2(sum * n1), h2 = min_2(sum * n2), h3 = min_2(sum * n3), h4 = min_2(sum * n4);
+      fv[18 + o] = 0.5f * (h1 + h2 + h3 + h4);
+    }
+    fv[27] = (float)(0.2357 * (double)t1);
+    fv[28] = (float)(0.2357 * (double)t2);
+    fv[29] = (float)(0.2357 * (double)t3);
+    fv[30] = (float)(0.2357 * (double)t4);
+    if (feat_arena) {
+      float *dst = feat_arena + L.feat_off + e;
+#pragma unroll
+      for (int o = 0; o < 31; o++) dst[o * OSQ] = fv[o];
+    }
+    if (featb_arena) {
+      // 16-bit channels 0..31 (the last one 0), and for the split forms the lo halves as
+      // channels 32..63; featb_off is a multiple of 32 elements, so every cell starts on a
+      // 64-byte (128-byte) boundary
+      unsigned short hb[32], lb[32];
+#pragma unroll
+      for (int c = 0; c < 32; c++) {
+        hb[c] = c < 31 ? half_bits(fv[c], hkind) : (unsigned short)0;
+        lb[c] = 0;
+      }
+      if (hkind > 2) {
+#pragma unroll
+        for (int c = 0; c < 31; c++) {
+          const float hv = hkind == 3 ? __uint_as_float((unsigned int)hb[c] << 16)
+                                      : __half2float(__ushort_as_half(hb[c]));
+          lb[c] = half_bits(fv[c] - hv, hkind);
+        }
+      }
+      const int t = threadIdx.x;
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        if (q >= cpc) break;
+        const unsigned short *h = q < 4 ? hb : lb;
+        const int c0 = (q & 3) * 8;
+        s_cells[t * cpc + (q ^ swz(t))] =
+            make_uint4(h[c0] | ((unsigned)h[c0 + 1] << 16), h[c0 + 2] | ((unsigned)h[c0 + 3] << 16),
+                       h[c0 + 4] | ((unsigned)h[c0 + 5] << 16), h[c0 + 6] | ((unsigned)h[c0 + 7] << 16));
+      }
+    }
+  }
+  if (!featb_arena) return;  // uniform over the launch
+  __syncthreads();
+  // the block's cells [e0, e0 + 256) are contiguous in the arena: write them chunk by chunk
+  uint4 *db = (uint4 *)(featb_arena + (cpc == 4 ? L.featb_off : 2 * L.featb_off)) + e0 * cpc;
+  const long long ncells = OSQ - e0 < 256 ? OSQ - e0 : 256;
+  for (int k = threadIdx.x; k < (int)ncells * cpc; k += 256) {
+    const int c = k / cpc, q = k - c * cpc;
+    db[k] = s_cells[c * cpc + (q ^ swz(c))];
   }
 }
 
