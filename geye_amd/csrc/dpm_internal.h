@@ -238,9 +238,10 @@ struct Plan {
     int h, w, nf, tile_begin, tile_count, filt_off;
   };
   std::vector<ConvClass> conv_classes;
-  std::vector<MfmaTile> mfma_tiles;
+  std::vector<MfmaTile> mfma_tiles, mfma_tiles8;  // 4-column tiles; the same as 8-column tiles
   struct MfmaClass {
     int h, w, nt, tile_begin, tile_count;
+    int tile8_begin = 0, tile8_count = 0;
   };
   std::vector<MfmaClass> mfma_classes;
   // work-queue form: every class's tiles in one list, largest first; one list per tile
@@ -274,7 +275,7 @@ struct Plan {
   std::vector<int> ascore_tab;         // per valid level: Xn[W] then Yn[H]
   // device copies of the tables
   DeviceBuffer d_resize_ops, d_resize_tab, d_resize_si, d_resize_alpha, d_hog, d_conv_tiles,
-      d_mfma_tiles, d_map_dims, d_map_off, d_dt1, d_dt2, d_dt1w, d_dt2w, d_resize_si4,
+      d_mfma_tiles, d_mfma_tiles8, d_map_dims, d_map_off, d_dt1, d_dt2, d_dt1w, d_dt2w, d_resize_si4,
       d_resize_al4, d_nodes, d_depths, d_smaps, d_sparts,
       d_ascore_tab, d_grad, d_grid, d_qtiles0, d_qtiles1, d_qcounter, d_fused;
   // device arenas

@@ -80,8 +80,9 @@ __device__ inline const u32x4 *conv_filt_base(const u32x4 *filt, const MfmaTile 
 
 // Registers per thread: PN chunks of the next tile's 4 new feature columns, PRE chunks of the
 // next tap column's filters (NTH threads per workgroup).
-template <int NT, bool SPLIT, int NTH> struct ConvRegs {
-  static constexpr int PN = SPLIT ? (NTH == 512 ? 5 : 10) : 5;
+template <int NT, bool SPLIT, int NTH, int TW> struct ConvRegs {
+  // TW new columns of up to 79 cells (filter height <= 16)
+  static constexpr int PN = (TW * 79 * (SPLIT ? 8 : 4) + NTH - 1) / NTH;
   static constexpr int PRE = SPLIT && NTH == 256 ? (NT == 3 ? 9 : 8) : (NT == 3 ? 5 : 4);
 };
 
@@ -118,10 +119,10 @@ __device__ inline void conv_stage_tile(u32x4 *As, const u32x4 *__restrict__ feat
   }
 }
 
-// Loads the 4 feature columns tile tn adds to its predecessor (tn's relative columns
-// AX-4..AX-1) into registers.  live == false still issues the loads, all from one cell, so
+// Loads the TW feature columns tile tn adds to its predecessor (tn's relative columns
+// AX-TW..AX-1) into registers.  live == false still issues the loads, all from one cell, so
 // that the code around them stays free of branches and the wait counts stay exact.
-template <int CPC, int PN, int NTH>
+template <int CPC, int PN, int NTH, int TW>
 __device__ inline void conv_fetch_new(u32x4 (&nf)[PN], unsigned &vmask,
                                       const u32x4 *__restrict__ feat, const MfmaTile &tn,
                                       bool live, int AX, int CS, int tid) {
@@ -131,23 +132,23 @@ __device__ inline void conv_fetch_new(u32x4 (&nf)[PN], unsigned &vmask,
 #pragma unroll
   for (int k = 0; k < PN; k++) {
     bool v;
-    nf[k] = conv_feat_load<CPC>(F, tn, live ? tn.x0 + AX - 4 + c : tn.x0, live ? rem : 0, v);
+    nf[k] = conv_feat_load<CPC>(F, tn, live ? tn.x0 + AX - TW + c : tn.x0, live ? rem : 0, v);
     vmask |= (unsigned)(v && live) << k;
     rem += NTH;
     if (rem >= CS) { rem -= CS; c++; }
   }
 }
 
-// Writes those columns into the slots of the current tile's relative columns 0..3 (sx is the
-// slot of its relative column 0): feature column x0 + AX + k lives in slot (x0 + k) mod AX.
-template <int CPC, int PN, int NTH>
+// Writes those columns into the slots of the current tile's relative columns 0..TW-1 (sx is
+// the slot of its relative column 0): feature column x0 + AX + k lives in slot (x0 + k) mod AX.
+template <int CPC, int PN, int NTH, int TW>
 __device__ inline void conv_put_new(u32x4 *As, const u32x4 (&nf)[PN], unsigned vmask, int sx,
                                     int AX, int AY, int tid) {
   const int CS = AY * CPC;
   int c = 0, rem = tid;
 #pragma unroll
   for (int k = 0; k < PN; k++) {
-    if (k * NTH + tid < 4 * CS) {
+    if (k * NTH + tid < TW * CS) {
       int s2 = sx + c;
       if (s2 >= AX) s2 -= AX;
       As[conv_cell_chunk<CPC>(s2 * AY + rem / CPC, rem & (CPC - 1))] =
@@ -244,37 +245,39 @@ __device__ inline void conv_store(const f32x4 (&acc)[MT][NT], const MfmaTile &t,
 }
 
 // Per-class kernel, persistent: workgroup b of G takes the contiguous tile range
-// [b*n/G, (b+1)*n/G) of a list ordered along x within (level, group, row of tiles).  When the
-// next tile is the x + 4 neighbour, its 4 new feature columns are loaded into registers during
-// this tile's first tap column and written into the ring slots this tile has finished with
-// (relative columns 0..3, last read at tap column min(3, fw - 1)); the next tile's first
-// filter column is loaded by this tile's last tap column.  A tile that does not continue its
-// predecessor stages its whole feature tile first.  WPC waves share each of the 4 output
-// columns: wave w takes column w % 4 and rows 64 (w / 4) / WPC .. + 64 / WPC, i.e. MT =
-// 4 / WPC M-tiles x NT N-tiles of accumulators.  The split forms run WPC = 2: their LDS tile
-// leaves room for one workgroup per CU, and two waves per SIMD keep the matrix unit fed while
-// one waits.  The per-output order of the products is the same for every WPC.  nbuf = 2
-// double-buffers the filter columns; nbuf = 1 (a split-form class whose double buffer would
-// not fit the LDS) writes the next column after a barrier that ends this column's reads.
-template <int KIND, int NT, bool SPLIT, int WPC>
-__global__ __launch_bounds__(256 * WPC)
-__attribute__((amdgpu_waves_per_eu(SPLIT ? WPC : 2, SPLIT ? WPC : 2))) void k_conv_mfma(
+// [b*n/G, (b+1)*n/G) of a list of TW-column tiles ordered along x within (level, group, row of
+// tiles).  When the next tile is the x + TW neighbour, its TW new feature columns are loaded
+// into registers during this tile's first tap column and written into the ring slots this
+// tile has finished with (relative columns 0..TW-1, last read at tap column
+// min(TW - 1, fw - 1)); the next tile's first filter column is loaded by this tile's last tap
+// column.  A tile that does not continue its predecessor stages its whole feature tile
+// first.  WPC waves share each of the TW output columns: wave w takes column w % TW and rows
+// 64 (w / TW) / WPC .. + 64 / WPC, i.e. MT = 4 / WPC M-tiles x NT N-tiles of accumulators.
+// The split forms' LDS tile leaves room for one workgroup per CU, which runs 8 waves: 8
+// columns (TW = 8, half the filter traffic and LDS filter reads per output) where that tile
+// fits the LDS, else 4 columns of two waves each (WPC = 2).  The per-output order of the
+// products is the same for every TW and WPC.  nbuf = 2 double-buffers the filter columns;
+// nbuf = 1 (a class whose double buffer would not fit the LDS) writes the next column after a
+// barrier that ends this column's reads.
+template <int KIND, int NT, bool SPLIT, int TW, int WPC>
+__global__ __launch_bounds__(64 * TW * WPC)
+__attribute__((amdgpu_waves_per_eu(2, 2))) void k_conv_mfma(
     const MfmaTile *__restrict__ tiles, int ntiles, int fh, int fw, int nbuf,
     const u32x4 *__restrict__ feat, const u32x4 *__restrict__ filt,
     const long long *__restrict__ map_off, const int2 *__restrict__ map_dims,
     float *__restrict__ out) {
   extern __shared__ u32x4 lds[];
-  constexpr int NTH = 256 * WPC, MT = 4 / WPC;
-  constexpr int PN = ConvRegs<NT, SPLIT, NTH>::PN, PRE = ConvRegs<NT, SPLIT, NTH>::PRE;
+  constexpr int NTH = 64 * TW * WPC, MT = 4 / WPC;
+  constexpr int PN = ConvRegs<NT, SPLIT, NTH, TW>::PN, PRE = ConvRegs<NT, SPLIT, NTH, TW>::PRE;
   constexpr int CPC = SPLIT ? 8 : 4;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wcol = wave & 3, wrow = (wave >> 2) * 16 * MT;  // the wave's column and first row
-  const int AX = 4 + fw - 1, AY = 64 + fh - 1, CS = AY * CPC, NA = AX * CS;
+  const int wcol = wave % TW, wrow = (wave / TW) * 16 * MT;  // the wave's column and first row
+  const int AX = TW + fw - 1, AY = 64 + fh - 1, CS = AY * CPC, NA = AX * CS;
   u32x4 *As = lds;                               // [AX slots][AY][CPC chunks], swizzled
   u32x4 *Bs = lds + NA;                          // [nbuf][fh][NT][CPC][16]
   const int bstage = fh * NT * CPC * 16;         // 16-byte chunks per tap column
   const bool in_regs = bstage <= PRE * NTH;
-  const bool ring = 4 * CS <= PN * NTH;
+  const bool ring = TW * CS <= PN * NTH;
   const int bmask = nbuf - 1;
 
   int bx = blockIdx.x;
@@ -299,7 +302,7 @@ __attribute__((amdgpu_waves_per_eu(SPLIT ? WPC : 2, SPLIT ? WPC : 2))) void k_co
     const bool has_next = ti + 1 < te;
     const MfmaTile tn = tiles[has_next ? ti + 1 : ti];
     const bool cont = has_next && ring && tn.featb_off == t.featb_off && tn.y0 == t.y0 &&
-                      tn.x0 == t.x0 + 4;
+                      tn.x0 == t.x0 + TW;
     const u32x4 *Wg = conv_filt_base<CPC>(filt, t);
     const int sx = t.x0 % AX;                  // slot of the tile's relative column 0
     // the previous tile ended with a barrier after its last reads of As
@@ -333,7 +336,7 @@ __attribute__((amdgpu_waves_per_eu(SPLIT ? WPC : 2, SPLIT ? WPC : 2))) void k_co
         for (int i = tid; i < bstage; i += NTH) dst[i] = src[i];
       // issued after the filter loads, the next tile's feature loads may stay in flight past
       // this column's wait for the filters
-      if (first) conv_fetch_new<CPC, PN, NTH>(nf, vmask, feat, tn, cont, AX, CS, tid);
+      if (first) conv_fetch_new<CPC, PN, NTH, TW>(nf, vmask, feat, tn, cont, AX, CS, tid);
       int slot = sx + wcol + xp;
       if (slot >= AX) slot -= AX;
       conv_column<KIND, NT, SPLIT, MT>(acc, Bs + buf * bstage, As, slot * AY + wrow + r, fh, r,
@@ -347,15 +350,19 @@ __attribute__((amdgpu_waves_per_eu(SPLIT ? WPC : 2, SPLIT ? WPC : 2))) void k_co
       for (int c = 0; c < PRE; c++) dst[(unsigned)min(tid + c * NTH, bstage - 1)] = pre[c];
       __syncthreads();
     };
-    // tap column xp reads relative columns xp..xp+3, so from tap column 4 on (or after the
-    // last one) the slots of relative columns 0..3 take the next tile's new columns
-    step(0, true);
+    // tap column xp reads relative columns xp..xp+TW-1, so from tap column TW on (or after the
+    // last one) the slots of relative columns 0..TW-1 take the next tile's new columns.  With
+    // 8-column tiles (fw <= 8, so always after the last one) the new columns are loaded only
+    // then, when the tap columns' registers are free.
+    step(0, TW == 4);
     for (int xp = 1; xp < fw; xp++) {
-      if (xp == 4 && cont) conv_put_new<CPC, PN, NTH>(As, nf, vmask, sx, AX, AY, tid);
+      if (TW == 4 && xp == TW && cont)
+        conv_put_new<CPC, PN, NTH, TW>(As, nf, vmask, sx, AX, AY, tid);
       step(xp, false);
     }
-    if (fw <= 4 && cont) {
-      conv_put_new<CPC, PN, NTH>(As, nf, vmask, sx, AX, AY, tid);
+    if (fw <= TW && cont) {
+      if constexpr (TW == 8) conv_fetch_new<CPC, PN, NTH, TW>(nf, vmask, feat, tn, cont, AX, CS, tid);
+      conv_put_new<CPC, PN, NTH, TW>(As, nf, vmask, sx, AX, AY, tid);
       __syncthreads();
     }
     conv_store<NT, MT>(acc, t, t.x0 + wcol, t.y0 + wrow, r, chunk, map_off, map_dims, out);
@@ -519,32 +526,37 @@ int launch_conv_mfma_queue(hipStream_t s, int kind, bool split, int MT, int bloc
                : launch_q_kind<false>(s, kind, MT, blocks, lds, tiles, ntiles, counter, feat, filt, map_off, map_dims, out);
 }
 
-size_t mfma_lds_bytes(int fh, int fw, int nt, bool split, int nbuf) {
+size_t mfma_lds_bytes(int fh, int fw, int nt, bool split, int nbuf, int tw) {
   const size_t cpc = split ? 8 : 4;
-  return (size_t)16 * cpc * ((size_t)(4 + fw - 1) * (64 + fh - 1) + (size_t)nbuf * fh * nt * 16);
+  return (size_t)16 * cpc * ((size_t)(tw + fw - 1) * (64 + fh - 1) + (size_t)nbuf * fh * nt * 16);
 }
 
-// Filter buffers of the per-class kernel: 2, or 1 for a split class whose double buffer
-// would not fit the 160 KB of LDS; 0 = unsupported (a split class whose 64-row feature tile
-// and one tap column of filters exceed the LDS, e.g. 12x12 filters in groups of 48).
-static int mfma_nbuf(int fh, int fw, int nt, bool split) {
-  if (mfma_lds_bytes(fh, fw, nt, split, 2) <= 160 * 1024) return 2;
-  if (split && mfma_lds_bytes(fh, fw, nt, split, 1) <= 160 * 1024) return 1;
+// Filter buffers of the per-class kernel at tile width tw: 2, or 1 for a split class whose
+// double buffer would not fit the 160 KB of LDS; 0 = unsupported (a split class whose 64-row
+// feature tile and one tap column of filters exceed the LDS, e.g. 12x12 filters in groups of
+// 48).
+static int mfma_nbuf(int fh, int fw, int nt, bool split, int tw) {
+  if (mfma_lds_bytes(fh, fw, nt, split, 2, tw) <= 160 * 1024) return 2;
+  if (split && mfma_lds_bytes(fh, fw, nt, split, 1, tw) <= 160 * 1024) return 1;
   return 0;
+}
+
+bool mfma_wide_tiles(int fh, int fw, int nt, bool split) {
+  return split && fw <= 8 && mfma_nbuf(fh, fw, nt, split, 8) > 0;
 }
 
 // Launches the persistent kernel with every workgroup resident at once (its tile ranges are
 // static).  With fewer than 4 tiles per resident workgroup a static split would leave the
 // last round uneven, so each tile gets its own workgroup and the hardware balances them.
-template <int KIND, int NT, bool SPLIT>
+template <int KIND, int NT, bool SPLIT, int TW>
 static int launch_one(hipStream_t s, const MfmaTile *tiles, int ntiles, int fh, int fw,
                       const void *feat, const void *filt, const long long *map_off,
                       const int2 *map_dims, float *out) {
-  constexpr int WPC = SPLIT ? 2 : 1;           // waves per output column
-  const void *kern = (const void *)k_conv_mfma<KIND, NT, SPLIT, WPC>;
-  const int nbuf = mfma_nbuf(fh, fw, NT, SPLIT);
+  constexpr int WPC = SPLIT && TW == 4 ? 2 : 1;  // waves per output column
+  const void *kern = (const void *)k_conv_mfma<KIND, NT, SPLIT, TW, WPC>;
+  const int nbuf = mfma_nbuf(fh, fw, NT, SPLIT, TW);
   if (nbuf == 0) return -1;
-  const size_t lds = mfma_lds_bytes(fh, fw, NT, SPLIT, nbuf);
+  const size_t lds = mfma_lds_bytes(fh, fw, NT, SPLIT, nbuf, TW);
   // once per process (function-local statics are initialised once, also when serving
   // workers launch concurrently): the LDS attribute and the CU count, 0 on failure
   static const int ncu = [kern] {
@@ -558,38 +570,44 @@ static int launch_one(hipStream_t s, const MfmaTile *tiles, int ntiles, int fh, 
   }();
   if (ncu <= 0) return -1;
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256 * WPC, lds) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * TW * WPC, lds) !=
+          hipSuccess ||
       per_cu < 1)
     return -1;
   const int full = per_cu * ncu;
   const int blocks = ntiles <= 4 * full ? ntiles : full;
-  hipLaunchKernelGGL((k_conv_mfma<KIND, NT, SPLIT, WPC>), dim3(blocks), dim3(256 * WPC), lds, s,
-                     tiles, ntiles, fh, fw, nbuf, (const u32x4 *)feat, (const u32x4 *)filt,
-                     map_off, map_dims, out);
+  hipLaunchKernelGGL((k_conv_mfma<KIND, NT, SPLIT, TW, WPC>), dim3(blocks), dim3(64 * TW * WPC),
+                     lds, s, tiles, ntiles, fh, fw, nbuf, (const u32x4 *)feat,
+                     (const u32x4 *)filt, map_off, map_dims, out);
   return 0;
 }
 
-template <bool SPLIT>
+template <bool SPLIT, int TW>
 static int launch_kind(hipStream_t s, int kind, int nt, const MfmaTile *tiles, int ntiles, int fh,
                        int fw, const void *feat, const void *filt, const long long *map_off,
                        const int2 *map_dims, float *out) {
   if (kind == 1) {
-    if (nt == 3) return launch_one<1, 3, SPLIT>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
-    if (nt == 2) return launch_one<1, 2, SPLIT>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
-    return launch_one<1, 1, SPLIT>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+    if (nt == 3) return launch_one<1, 3, SPLIT, TW>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+    if (nt == 2) return launch_one<1, 2, SPLIT, TW>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+    return launch_one<1, 1, SPLIT, TW>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
   }
-  if (nt == 3) return launch_one<2, 3, SPLIT>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
-  if (nt == 2) return launch_one<2, 2, SPLIT>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
-  return launch_one<2, 1, SPLIT>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+  if (nt == 3) return launch_one<2, 3, SPLIT, TW>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+  if (nt == 2) return launch_one<2, 2, SPLIT, TW>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+  return launch_one<2, 1, SPLIT, TW>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
 }
 
-int launch_conv_mfma(hipStream_t s, int kind, bool split, int nt, const MfmaTile *tiles,
-                     int ntiles, int fh, int fw, const void *feat, const void *filt,
-                     const long long *map_off, const int2 *map_dims, float *out) {
-  if (ntiles <= 0) return 0;
-  if ((kind != 1 && kind != 2) || nt < 1 || nt > 3 || mfma_nbuf(fh, fw, nt, split) == 0) return -1;
-  return split ? launch_kind<true>(s, kind, nt, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out)
-               : launch_kind<false>(s, kind, nt, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+int launch_conv_mfma(hipStream_t s, int kind, bool split, int nt, const MfmaTile *tiles4,
+                     int ntiles4, const MfmaTile *tiles8, int ntiles8, int fh, int fw,
+                     const void *feat, const void *filt, const long long *map_off,
+                     const int2 *map_dims, float *out) {
+  if (ntiles4 <= 0) return 0;
+  if ((kind != 1 && kind != 2) || nt < 1 || nt > 3) return -1;
+  if (tiles8 && ntiles8 > 0 && mfma_wide_tiles(fh, fw, nt, split))
+    return launch_kind<true, 8>(s, kind, nt, tiles8, ntiles8, fh, fw, feat, filt, map_off,
+                                map_dims, out);
+  if (mfma_nbuf(fh, fw, nt, split, 4) == 0) return -1;
+  return split ? launch_kind<true, 4>(s, kind, nt, tiles4, ntiles4, fh, fw, feat, filt, map_off, map_dims, out)
+               : launch_kind<false, 4>(s, kind, nt, tiles4, ntiles4, fh, fw, feat, filt, map_off, map_dims, out);
 }
 
 }  // namespace geye

@@ -424,6 +424,7 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
   P.conv_tiles.clear();
   P.conv_classes.clear();
   P.mfma_tiles.clear();
+  P.mfma_tiles8.clear();
   P.mfma_classes.clear();
   P.q_tiles[0].clear();
   P.q_tiles[1].clear();
@@ -442,6 +443,7 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
     const int nmem = (int)fc.members.size();
     Plan::ConvClass cc{fc.h, fc.w, fc.nf, (int)P.conv_tiles.size(), 0, (int)ci};
     Plan::MfmaClass mc{fc.h, fc.w, fc.nt, (int)P.mfma_tiles.size(), 0};
+    mc.tile8_begin = (int)P.mfma_tiles8.size();
     for (int l = interval; l < LEN; l++) {
       if (!P.lv[l].valid) continue;
       const int fl = fc.root ? l : l - interval;
@@ -498,24 +500,25 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
         int Hm, Wm;
         group_extent(g * gsz, gsz, Hm, Wm);
         // rows of tiles along x: the MFMA kernel carries the shared feature columns from
-        // one tile to the next
-        for (int y0 = 0; y0 < Hm; y0 += 64)
-          for (int x0 = 0; x0 < Wm; x0 += 4) {
-            MfmaTile t;
-            t.featb_off = featb_off[fl];
-            t.F0 = FL.F0;
-            t.F1 = FL.F1;
-            t.x0 = x0;
-            t.y0 = y0;
-            t.filt_off = fc.hoff + (long long)g * fc.w * fc.h * fc.nt * 512;
-            t.out_slot = base + g * gsz;
-            t.nvalid = std::min(gsz, nmem - g * gsz);
-            t.fh = fc.h;
-            t.fw = fc.w;
-            t.nt = fc.nt;
-            t.cost = 0;
-            P.mfma_tiles.push_back(t);
-          }
+        // one tile to the next; the same outputs as 4- and as 8-column tiles
+        for (int tw = 4; tw <= 8; tw += 4)
+          for (int y0 = 0; y0 < Hm; y0 += 64)
+            for (int x0 = 0; x0 < Wm; x0 += tw) {
+              MfmaTile t;
+              t.featb_off = featb_off[fl];
+              t.F0 = FL.F0;
+              t.F1 = FL.F1;
+              t.x0 = x0;
+              t.y0 = y0;
+              t.filt_off = fc.hoff + (long long)g * fc.w * fc.h * fc.nt * 512;
+              t.out_slot = base + g * gsz;
+              t.nvalid = std::min(gsz, nmem - g * gsz);
+              t.fh = fc.h;
+              t.fw = fc.w;
+              t.nt = fc.nt;
+              t.cost = 0;
+              (tw == 4 ? P.mfma_tiles : P.mfma_tiles8).push_back(t);
+            }
         // work-queue tiles: 4 columns x 32 or 64 rows
         for (int mi = 0; mi < 2; mi++) {
           const int rows = mi == 0 ? 32 : 64;
@@ -542,6 +545,7 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
     }
     cc.tile_count = (int)P.conv_tiles.size() - cc.tile_begin;
     mc.tile_count = (int)P.mfma_tiles.size() - mc.tile_begin;
+    mc.tile8_count = (int)P.mfma_tiles8.size() - mc.tile8_begin;
     P.conv_classes.push_back(cc);
     P.mfma_classes.push_back(mc);
   }
@@ -779,6 +783,15 @@ void replicate_plan(const Plan &one, int nb, long long feat_stride, long long fe
         P.mfma_tiles.push_back(t);
       }
     c2.tile_count = mc.tile_count * nb;
+    c2.tile8_begin = (int)P.mfma_tiles8.size();
+    for (int f = 0; f < nb; f++)
+      for (int k = mc.tile8_begin; k < mc.tile8_begin + mc.tile8_count; k++) {
+        MfmaTile t = one.mfma_tiles8[k];
+        t.featb_off += f * P.featb_stride;
+        t.out_slot += f * nslots;
+        P.mfma_tiles8.push_back(t);
+      }
+    c2.tile8_count = mc.tile8_count * nb;
     P.mfma_classes.push_back(c2);
   }
   // a batch runs one launch per filter class (the schedule the autotuner picks for 1080p)

@@ -512,6 +512,7 @@ int plan_upload(Plan &P, hipStream_t st) {
   CHK(upload(P.d_hog, P.hog, st));
   CHK(upload(P.d_conv_tiles, P.conv_tiles, st));
   CHK(upload(P.d_mfma_tiles, P.mfma_tiles, st));
+  CHK(upload(P.d_mfma_tiles8, P.mfma_tiles8, st));
   CHK(upload(P.d_map_dims, P.map_dims, st));
   CHK(upload(P.d_map_off, P.map_off, st));
   CHK(upload(P.d_dt1, P.dt_pass1, st));
@@ -559,7 +560,7 @@ void plan_release(Plan &P) {
   P.b_host = P.b_dev = nullptr;
   for (DeviceBuffer *b : {&P.d_resize_ops, &P.d_resize_tab, &P.d_resize_si, &P.d_resize_alpha,
                           &P.d_resize_si4, &P.d_resize_al4,
-                          &P.d_hog, &P.d_conv_tiles, &P.d_mfma_tiles, &P.d_map_dims,
+                          &P.d_hog, &P.d_conv_tiles, &P.d_mfma_tiles, &P.d_mfma_tiles8, &P.d_map_dims,
                           &P.d_map_off, &P.d_dt1, &P.d_dt2, &P.d_dt1w, &P.d_dt2w,
                           &P.d_nodes, &P.d_depths, &P.d_smaps, &P.d_sparts, &P.d_frame,
                           &P.d_img, &P.d_tmp, &P.d_hist, &P.d_norm, &P.d_feat, &P.d_featb,
@@ -829,7 +830,8 @@ int run_mfma(geye_ctx *c, Model &m, Plan &P, const Plan &pyr, int precision, int
       if (mc.tile_count == 0) continue;
       KTimer t(c, timed ? tag + std::to_string(mc.h) + "x" + std::to_string(mc.w) : std::string());
       if (launch_conv_mfma(s, kind, split, mc.nt, dp<MfmaTile>(P.d_mfma_tiles) + mc.tile_begin,
-                           mc.tile_count, mc.h, mc.w, pyr.d_featb.p, filt,
+                           mc.tile_count, dp<MfmaTile>(P.d_mfma_tiles8) + mc.tile8_begin,
+                           mc.tile8_count, mc.h, mc.w, pyr.d_featb.p, filt,
                            dp<long long>(P.d_map_off), dp<int2>(P.d_map_dims),
                            dp<float>(P.d_conv)) != 0)
         return fail(GEYE_ERR_UNSUPPORTED, "filter class too large for the MFMA kernel's LDS");
@@ -2596,24 +2598,28 @@ int geye_fconv(geye_ctx *c, const float *feat, int F0, int F1, const float *filt
           fb[((size_t)x * F0 + y) * cw + f] = hi;
           if (split) fb[((size_t)x * F0 + y) * cw + 32 + f] = lo;
         }
-    std::vector<MfmaTile> tiles;
+    std::vector<MfmaTile> tiles;  // 4-column tiles, then the same outputs as 8-column tiles
     for (int k = 0; k < ng * gsz; k++) {
       moff.push_back((long long)std::min(k, n - 1) * H * W);
       mdims.push_back(k < n ? int2{H, W} : int2{0, 0});
     }
-    for (int g = 0; g < ng; g++)
-      for (int y0 = 0; y0 < H; y0 += 64)
-        for (int x0 = 0; x0 < W; x0 += 4)
-          tiles.push_back(MfmaTile{0, F0, F1, x0, y0, (long long)g * w * h * nt * 512, g * gsz,
-                                   std::min(gsz, n - g * gsz)});
+    int n4 = 0;
+    for (int tw = 4; tw <= 8; tw += 4) {
+      for (int g = 0; g < ng; g++)
+        for (int y0 = 0; y0 < H; y0 += 64)
+          for (int x0 = 0; x0 < W; x0 += tw)
+            tiles.push_back(MfmaTile{0, F0, F1, x0, y0, (long long)g * w * h * nt * 512,
+                                     g * gsz, std::min(gsz, n - g * gsz)});
+      if (tw == 4) n4 = (int)tiles.size();
+    }
     CHK(upload(dfe, fb, c->stream));
     CHK(upload(dfi, packed, c->stream));
     CHK(upload(dt, tiles, c->stream));
     CHK(upload(dm, moff, c->stream));
     CHK(upload(dd, mdims, c->stream));
     CHK(ensure(dout, sizeof(float) * (size_t)n * H * W));
-    if (launch_conv_mfma(c->stream, mfma_kind(precision), split, nt, dp<MfmaTile>(dt),
-                         (int)tiles.size(), h, w,
+    if (launch_conv_mfma(c->stream, mfma_kind(precision), split, nt, dp<MfmaTile>(dt), n4,
+                         dp<MfmaTile>(dt) + n4, (int)tiles.size() - n4, h, w,
                          dfe.p, dfi.p, dp<long long>(dm), dp<int2>(dd), dp<float>(dout)) != 0)
       return fail(GEYE_ERR_UNSUPPORTED, "filter too large for the MFMA kernel's LDS");
   }
