@@ -288,6 +288,9 @@ struct Plan {
       n_total, n_vals, n_ord, n_rank, n_dec, n_alist, n_acount, n_grid, n_tie;
   long long nms_fallbacks = 0;
   std::map<long long, hipGraphExec_t> graphs;  // front of the frame, per (precision, ws, thresh)
+  // generation: new whenever the plan's graphs are dropped (a buffer they use moved) or the
+  // plan is made; graphs over several plans (a batch's front) record it
+  unsigned long long gen = 0;
   // batch plans: per frame's NMS job its small buffers (counters, flags, totals), window
   // bitmask, compaction counts, sort storage and pick index, two events, and mapped host slots
   // (4 per job, then the nb + 1 frame offsets of the candidate list)
@@ -353,6 +356,10 @@ struct geye_ctx {
   hipEvent_t ev_count[2] = {nullptr, nullptr};
   long long nms_fallbacks = 0;
   long long nms_tie_repairs = 0;  // NMS calls whose tie order the host set by a partial quicksort
+  // batch fronts captured as one graph, per (form, frames, models' plans): the graph and the
+  // generations of the plans it was captured against
+  std::map<std::vector<long long>, std::pair<hipGraphExec_t, std::vector<unsigned long long>>>
+      batch_graphs;
 };
 
 struct geye_model {

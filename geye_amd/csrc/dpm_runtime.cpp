@@ -6,6 +6,7 @@
 // (CT/get_boxes.cpp:583).  Unlike the reference, filters, plans and every intermediate stay
 // resident in HBM: a frame crosses PCIe once (in) and only its boxes come back.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -549,9 +550,18 @@ int plan_upload(Plan &P, hipStream_t st) {
   return GEYE_OK;
 }
 
-void plan_release(Plan &P) {
+std::atomic<unsigned long long> g_plan_gen{0};
+
+// Drops the plan's captured graphs (a buffer they use moved, or the plan goes); the new
+// generation also retires every batch graph captured against the plan.
+void drop_graphs(Plan &P) {
   for (auto &kv : P.graphs) (void)hipGraphExecDestroy(kv.second);
   P.graphs.clear();
+  P.gen = ++g_plan_gen;
+}
+
+void plan_release(Plan &P) {
+  drop_graphs(P);
   for (auto *v : {&P.b_small, &P.b_mask, &P.b_blocksum, &P.b_sort, &P.b_grid, &P.b_acount})
     for (DeviceBuffer &b : *v) release(b);
   for (hipEvent_t e : P.b_ev) (void)hipEventDestroy(e);
@@ -610,6 +620,7 @@ int get_plan(geye_ctx *c, Model &m, int W, int H, Plan **out, int max_scale) {
     }
   }
   std::unique_ptr<Plan> P(new Plan());
+  P->gen = ++g_plan_gen;
   std::string err;
   int rc = build_plan(m, W, H, *P, err, max_scale > 0 ? max_scale : -1);
   if (rc != GEYE_OK) return fail(rc, err);
@@ -659,6 +670,7 @@ int get_batch_plan(geye_ctx *c, Model &m, int W, int H, int nb, int feat_len, Pl
       }
   }
   std::unique_ptr<Plan> P(new Plan());
+  P->gen = ++g_plan_gen;
   replicate_plan(one, nb, fs, fbs, *P);
   rc = plan_upload(*P, c->stream);
   const long long cells = std::max<long long>(one.score_total, 1);
@@ -792,10 +804,7 @@ int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, int hk
   if (hkind) {
     const void *old = P.d_featb.p;
     CHK(ensure(P.d_featb, 4 * P.featb_total));
-    if (old && P.d_featb.p != old) {
-      for (auto &kv : P.graphs) (void)hipGraphExecDestroy(kv.second);
-      P.graphs.clear();
-    }
+    if (old && P.d_featb.p != old) drop_graphs(P);
   }
   if (P.hogf_n > 0) {
     KTimer t(c, "hog_feat");
@@ -1400,10 +1409,7 @@ int run_nms(geye_ctx *c, Model &m, Plan &P, int n, float overlap, int *nkept) {
 int ensure_frame(Plan &P, size_t bytes) {
   const void *old = P.d_frame.p;
   CHK(ensure(P.d_frame, bytes));
-  if (P.d_frame.p != old) {
-    for (auto &kv : P.graphs) (void)hipGraphExecDestroy(kv.second);
-    P.graphs.clear();
-  }
+  if (P.d_frame.p != old) drop_graphs(P);
   return GEYE_OK;
 }
 
@@ -1650,19 +1656,68 @@ int detect_batch(geye_ctx *c, const geye_model *const *models, int nm,
     HIPCHK(hipMemcpyAsync(fr + fbytes * f, frames[f], fbytes,
                           dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
   const int hkind = precision == GEYE_FP32 ? 0 : precision;
-  CHK(run_pyramid(c, *P[owner], fr, ws, hkind));
-  for (int i = 0; i < nm; i++) {
-    Model &m = const_cast<Model &>(models[i]->m);
-    if (!shared[i]) {
-      CHK(ensure_frame(*P[i], fbytes * nf));
-      HIPCHK(hipMemcpyAsync(P[i]->d_frame.p, fr, fbytes * nf, hipMemcpyDeviceToDevice, c->stream));
-      CHK(run_pyramid(c, *P[i], (const unsigned char *)P[i]->d_frame.p, ws, hkind));
+  // every launch before the NMS, for every frame and model
+  auto front = [&]() -> int {
+    CHK(run_pyramid(c, *P[owner], fr, ws, hkind));
+    for (int i = 0; i < nm; i++) {
+      Model &m = const_cast<Model &>(models[i]->m);
+      if (!shared[i]) {
+        CHK(ensure_frame(*P[i], fbytes * nf));
+        HIPCHK(hipMemcpyAsync(P[i]->d_frame.p, fr, fbytes * nf, hipMemcpyDeviceToDevice,
+                              c->stream));
+        CHK(run_pyramid(c, *P[i], (const unsigned char *)P[i]->d_frame.p, ws, hkind));
+      }
+      CHK(run_scores(c, m, *P[i], thresh, precision, nullptr, shared[i] ? P[owner] : nullptr,
+                     count_slot(i)));
+      if (P[i]->score_total > 0)
+        launch_frame_offsets(c->stream, dp<int>(P[i]->d_cand), dp<int>(P[i]->d_count),
+                             P[i]->score_one, nf, P[i]->b_dev + 4 * nf);
     }
-    CHK(run_scores(c, m, *P[i], thresh, precision, nullptr, shared[i] ? P[owner] : nullptr,
-                   count_slot(i)));
-    if (P[i]->score_total > 0)
-      launch_frame_offsets(c->stream, dp<int>(P[i]->d_cand), dp<int>(P[i]->d_count),
-                           P[i]->score_one, nf, P[i]->b_dev + 4 * nf);
+    return GEYE_OK;
+  };
+  if (c->profile || !c->graphs) {
+    CHK(front());
+  } else {
+    // one HIP graph for the whole batch front, per (form, threshold, frames, models' plans),
+    // captured after a warm-up run (allocations and one-time attributes) and replayed while
+    // none of the plans has dropped its graphs since
+    unsigned int tb;
+    std::memcpy(&tb, &thresh, 4);
+    std::vector<long long> key{precision, dt_windowed(c, precision) ? 1 : 0, ws, (long long)tb,
+                               nf, nm, W, H};
+    std::vector<unsigned long long> gens;
+    for (int i = 0; i < nm; i++) {
+      key.push_back((long long)(uintptr_t)P[i]);
+      gens.push_back(P[i]->gen);
+    }
+    auto it = c->batch_graphs.find(key);
+    if (it != c->batch_graphs.end() && it->second.second != gens) {
+      (void)hipGraphExecDestroy(it->second.first);
+      c->batch_graphs.erase(it);
+      it = c->batch_graphs.end();
+    }
+    if (it == c->batch_graphs.end()) {
+      CHK(front());
+      HIPCHK(hipStreamSynchronize(c->stream));
+      for (int i = 0; i < nm; i++) gens[i] = P[i]->gen;  // the warm-up may have grown buffers
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = front();
+      const hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (rc != GEYE_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess)
+        return fail(GEYE_ERR_HIP, std::string("batch graph capture: ") + hipGetErrorString(e));
+      hipGraphExec_t ex = nullptr;
+      const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess)
+        return fail(GEYE_ERR_HIP, std::string("batch graph instantiate: ") + hipGetErrorString(ei));
+      it = c->batch_graphs.emplace(key, std::make_pair(ex, gens)).first;
+    }
+    HIPCHK(hipGraphLaunch(it->second.first, c->stream));
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1971,6 +2026,8 @@ void geye_shutdown(geye_ctx *c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   flush_stats(c);
+  for (auto &kv : c->batch_graphs) (void)hipGraphExecDestroy(kv.second.first);
+  c->batch_graphs.clear();
   release_plans(c, nullptr, -1);
   for (auto &b : c->scratch_bufs) release(b);
   if (c->h_count) (void)hipHostFree(c->h_count);
@@ -2752,8 +2809,7 @@ int geye_mfma_schedule(geye_ctx *c, int precision, int choice) {
       return fail(GEYE_ERR_ARG, "no such MFMA schedule for this plan");
   }
   P->mfma_choice[precision] = choice;
-  for (auto &kv : P->graphs) (void)hipGraphExecDestroy(kv.second);
-  P->graphs.clear();
+  drop_graphs(*P);
   return GEYE_OK;
 }
 
