@@ -30,10 +30,13 @@ struct ResizeOp {
 };
 
 // Gradient grid of one pyramid image: per pixel (x, y), 1 <= x <= w-2, 1 <= y <= h-2, the
-// strongest channel's gradient magnitude and snapped orientation, column-major int2
-// {float bits of v, o} at grid_off + x*h + y.  Tiles of 64 (x) x kGradTY (y) pixels (16 rows
-// beat 32 on 1080p pyramids: 0.086 against 0.098 ms per frame).
-constexpr int kGradTY = 16;
+// strongest channel's gradient magnitude v (float) and snapped orientation o (one byte), in
+// two column-major planes of the grid arena -- v at mag[grid_off + x*gs + y], o at
+// ori[grid_off + x*gs + y] -- with the column stride gs = grid_stride(h), h rounded up to
+// kGridAlign so that every 4-row group starts 16-byte aligned.  Tiles of 64 (x) x kGradTY (y)
+// pixels.
+constexpr int kGradTY = 16, kGridAlign = 32;
+__host__ __device__ inline int grid_stride(int h) { return (h + kGridAlign - 1) & ~(kGridAlign - 1); }
 struct GradImg {
   long long img_off, grid_off;
   int h, w;
@@ -54,8 +57,10 @@ struct FusedResize {
 // wave slots allow; on 1080p 8 rows beat 16 and 32); span: source columns
 constexpr int kResizeTY = 8, kResizeTX = 64, kResizeSpan = 136;
 
-// cell rows per HOG vote workgroup (one wave, one lane per cell of a cell column)
-constexpr int kHogVoteRows = 64;
+// HOG vote workgroup: kHogVoteCols adjacent cell columns (one wave each) of kHogVoteRows
+// cells (one lane per cell); one column per workgroup measured fastest (2 and 4 share the
+// staged pixel columns but leave waves idle between barriers)
+constexpr int kHogVoteRows = 64, kHogVoteCols = 1;
 
 struct HogLevel {
   long long img_off;           // image arena offset ([3][h][w] row-major)
@@ -65,7 +70,7 @@ struct HogLevel {
   long long feat_off;          // feature arena offset ([31][F1][F0])
   long long featb_off;         // bf16 channel-last arena offset ([F1][F0][32])
   int h, w, sbin, b0, b1, F0, F1;
-  int tiles_x, tiles_y;        // vote tiles: b1 cell columns x ceil(b0 / kHogVoteRows)
+  int tiles_x, tiles_y;        // vote tiles: ceil(b1 / kHogVoteCols) x ceil(b0 / kHogVoteRows)
   int blocks_begin;            // first flat block (hist or feat launch)
   int T;                       // cell rows per vote tile (kHogVoteRows)
 };
@@ -227,7 +232,7 @@ struct Plan {
   std::vector<HogLevel> hog;          // [T16 levels][T8 levels][feature levels]
   std::vector<GradImg> grad;          // one per pyramid image
   int grad_blocks = 0;
-  long long grid_total = 0;           // int2 entries
+  long long grid_total = 0;           // pixels of the gradient planes (5 bytes each)
   int hog16_n = 0, hog8_n = 0, hogf_n = 0;
   int hog_hist_blocks = 0, hog8_blocks = 0, hog_feat_blocks = 0;
   long long hist_total = 0, norm_total = 0, feat_total = 0, featb_total = 0;

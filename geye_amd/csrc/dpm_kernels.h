@@ -15,7 +15,8 @@ void launch_resize_cols(hipStream_t s, const ResizeOp *ops, int nops, int blocks
                         const int2 *tab, const int *si, const float *alpha, const float *src,
                         float *dst);
 // HOG histogram in two launches: gradients per pyramid image, then votes per level
-// (HogLevel tiles: one cell column of kHogVoteRows cells per 64-lane workgroup)
+// (HogLevel tiles: kHogVoteCols cell columns of kHogVoteRows cells per workgroup).  grid holds
+// the magnitude plane (4 * grid_total bytes) followed by the orientation plane (grid_total).
 // bgr (optional): the round reads the frames' BGR bytes (rows ws bytes apart, frames
 // frame_bytes apart) in place of the planar source image (the first round)
 void launch_resize_fused(hipStream_t s, const FusedResize *ops, int nops, int blocks,
@@ -23,9 +24,9 @@ void launch_resize_fused(hipStream_t s, const FusedResize *ops, int nops, int bl
                          float *dst, const unsigned char *bgr = nullptr, int ws = 0,
                          long long frame_bytes = 0);
 void launch_hog_grad(hipStream_t s, const GradImg *imgs, int nimg, int blocks, const float *img,
-                     int2 *grid);
-void launch_hog_votes(hipStream_t s, const HogLevel *lv, int nlv, int blocks, const int2 *grid,
-                      float *hist, float *norm);
+                     void *grid, long long grid_total);
+void launch_hog_votes(hipStream_t s, const HogLevel *lv, int nlv, int blocks, const void *grid,
+                      long long grid_total, float *hist, float *norm);
 void launch_hog_feat(hipStream_t s, const HogLevel *lv, int nlv, int blocks, const float *hist,
                      const float *norm, float *feat, void *featb, int hkind);
 void launch_conv_f32(hipStream_t s, int nf, const ConvTile *tiles, int ntiles, int fh, int fw,

@@ -308,13 +308,17 @@ __device__ inline int2 hog_grad_at(const float *t0, const float *t1, const float
 }
 
 // One block = 64 (x) x kGradTY (y) pixels of one image: the 66 x (kGradTY + 2) x 3
-// neighbourhood is read row-major (coalesced) into LDS, and the results go out column-major,
-// kGradTY consecutive y per column.  Pixels outside [1, w-2] x [1, h-2] are not written: readers clamp to that range,
-// exactly as the reference clamps the pixel it differentiates.
+// neighbourhood is read row-major (coalesced) into LDS.  Each thread then takes 4 consecutive
+// y of one column and writes their magnitudes as one 16-byte store and their orientations as
+// one 4-byte store into the column-major planes.  Columns outside [1, w-2] are not written,
+// and rows outside [1, h-2] hold zeros: readers clamp to that range, exactly as the reference
+// clamps the pixel it differentiates.
 __global__ __launch_bounds__(256) void k_hog_grad(const GradImg *__restrict__ imgs, int nimg,
                                                   const float *__restrict__ img_arena,
-                                                  int2 *__restrict__ grid_arena) {
-  constexpr int TXP = 64, TYP = kGradTY, SX = TXP + 2, SY = TYP + 2;
+                                                  float *__restrict__ mag_arena,
+                                                  unsigned char *__restrict__ ori_arena) {
+  constexpr int TXP = 64, TYP = kGradTY, SX = TXP + 2, SY = TYP + 2, QY = TYP / 4;
+  static_assert(kGridAlign % TYP == 0 && 256 % QY == 0 && TXP * QY % 256 == 0, "grad tile");
   __shared__ float s_t[3][SY * SX];
   const int bx = xcd_local_block(blockIdx.x, gridDim.x);
   const GradImg G = imgs[find_blocks(imgs, nimg, bx)];
@@ -349,32 +353,53 @@ __global__ __launch_bounds__(256) void k_hog_grad(const GradImg *__restrict__ im
     }
   }
   __syncthreads();
-  const int ly = threadIdx.x & (TYP - 1);
+  const int q = threadIdx.x % QY;
+  const long long gs = grid_stride(h);
 #pragma unroll
-  for (int k = 0; k < TXP * TYP / 256; k++) {
-    const int lx = (threadIdx.x / TYP) + (256 / TYP) * k;
-    const int x = x0 + lx, y = y0 + ly;
-    if (x < 1 || y < 1 || x > w - 2 || y > h - 2) continue;
-    const int i = (ly + 1) * SX + (lx + 1);
-    grid_arena[G.grid_off + (long long)x * h + y] = hog_grad_at(s_t[0], s_t[1], s_t[2], i, SX);
+  for (int k = 0; k < TXP * QY / 256; k++) {
+    const int lx = (int)threadIdx.x / QY + (256 / QY) * k;
+    const int x = x0 + lx;
+    if (x < 1 || x > w - 2) continue;
+    float m[4];
+    unsigned int o = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int ly = 4 * q + j, y = y0 + ly;
+      int2 g = make_int2(0, 0);
+      if (y >= 1 && y <= h - 2) g = hog_grad_at(s_t[0], s_t[1], s_t[2], (ly + 1) * SX + lx + 1, SX);
+      m[j] = __int_as_float(g.x);
+      o |= (unsigned int)g.y << (8 * j);
+    }
+    // y0 + 4q + 3 < gs: tiles_y * kGradTY <= grid_stride(h)
+    const long long e = G.grid_off + (long long)x * gs + y0 + 4 * q;
+    *reinterpret_cast<float4 *>(mag_arena + e) = make_float4(m[0], m[1], m[2], m[3]);
+    *reinterpret_cast<unsigned int *>(ori_arena + e) = o;
   }
 }
 
-// Orientation histogram + block energy, one lane per cell of a column of kHogVoteRows cells
-// (one wave per workgroup).  Each cell adds its bilinear votes in the reference's pixel order,
-// x outer and y inner (CT/featurepyramid.cpp.new2:323-378), so every bin sees the same
-// sequence of float additions as the reference's scatter loop.  Then hist[o][x][y] and
-// norm[x][y] = sum_{o<9} (h_o + h_{o+9})^2 (CT/featurepyramid.cpp.new2:391-404).
+// Orientation histogram + block energy.  A workgroup owns kHogVoteCols adjacent cell columns
+// of kHogVoteRows cells, one wave per cell column and one lane per cell.  Each cell adds its
+// bilinear votes in the reference's pixel order, x outer and y inner
+// (CT/featurepyramid.cpp.new2:323-378), so every bin sees the same sequence of float additions
+// as the reference's scatter loop.  Then hist[o][x][y] and norm[x][y] = sum_{o<9} (h_o +
+// h_{o+9})^2 (CT/featurepyramid.cpp.new2:391-404).
 //
-// The cells of one column share their pixel columns, so the wave stages each pixel column of
-// its rows in LDS with contiguous loads (the grid is column-major) and the lanes read their
-// rows from there; the next column's loads are in flight while the current one votes.  A
-// lane's bins live at s_acc[o][lane] (LDS float atomics, tried in their place, ran 5x slower;
-// reading a column's bins together and forwarding repeats in registers ran slower than this
-// read-add-write per vote, for its registers).
+// The workgroup walks the union of its cell columns' pixel columns ((kHogVoteCols + 1) sbin
+// columns): all its threads stage a pixel column of its rows in LDS with contiguous loads from
+// the two planes, and the waves whose cell column the pixel column votes into read their rows
+// from there; the next column's loads are in flight while the current one votes.  A lane's
+// bins live at s_acc[o][lane] of its wave.  Measured on 1080p pyramids, the kernel is bound
+// neither by HBM nor by the read-add-write chains nor by the loads' latency: the 5-byte grid
+// (8 bytes before) left its time unchanged; reading a half column's bins together and
+// forwarding repeats in registers ran 20% slower (its compares), two columns in flight 5%
+// slower, LDS float atomics 5x slower.  Sharing the staging across 2 or 4 cell columns (each
+// pixel column read once instead of twice) ran 10% and 23% slower: only the waves whose
+// columns a pixel column votes into work between the barriers.
 constexpr int kHogVoteJM = 16;                                         // 2 * sbin <= 16
-constexpr int kHogVoteLoads = (kHogVoteRows * kHogVoteJM / 2 + kHogVoteJM / 2 + 1 + 63) / 64;
-constexpr int kHogVoteStage = kHogVoteLoads * 64;                      // staged rows (>= (R + 1) sbin + 1)
+constexpr int kHogVoteThreads = 64 * kHogVoteCols;
+constexpr int kHogVoteStage =                                          // staged rows (>= (R + 1) sbin + 1)
+    (kHogVoteRows * kHogVoteJM / 2 + kHogVoteJM / 2 + 1 + 63) / 64 * 64;
+constexpr int kHogVoteLoads = (kHogVoteStage + kHogVoteThreads - 1) / kHogVoteThreads;
 constexpr int kHogVotePad = kHogVoteStage + kHogVoteStage / 8;         // row r at r + r / 8
 
 __device__ inline int hog_stage_index(int r) { return r + (r >> 3); }
@@ -395,14 +420,20 @@ __device__ inline void hog_vote_column(const int2 *s_col, float *acc, float wx, 
 }
 
 template <int NJ>
-__device__ inline void hog_votes_cells(const HogLevel &L, const int2 *__restrict__ grid, int cx,
-                                       int cy0, int2 *s_col, float *s_acc) {
+__device__ inline void hog_votes_cells(const HogLevel &L, const float *__restrict__ mag,
+                                       const unsigned char *__restrict__ ori, int cx0, int cy0,
+                                       int2 *s_col, float *s_acc) {
   constexpr int R = kHogVoteRows, NL = kHogVoteLoads;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63, cx = cx0 + (int)(threadIdx.x >> 6);
   const int sbin = L.sbin, h = L.h, w = L.w;
+  const long long gs = grid_stride(h);
   const int vis0 = L.b0 * sbin, vis1 = L.b1 * sbin;
   const int cy = cy0 + lane;
-  // pixels voting into a cell: interpolation cell in {c - 1, c}, within [1, vis - 2]
+  // pixels voting into a cell: interpolation cell in {c - 1, c}, within [1, vis - 2]; the
+  // workgroup's columns run from its first cell column's first to its last one's last
+  const int cxl = min(cx0 + kHogVoteCols - 1, L.b1 - 1);
+  const int Xa = max(1, first_pix(cx0 - 1, sbin)), Xb = min(vis1 - 2, first_pix(cxl + 1, sbin) - 1);
+  const bool col_live = cx <= cxl;
   const int xa = max(1, first_pix(cx - 1, sbin)), xb = min(vis1 - 2, first_pix(cx + 1, sbin) - 1);
   const int cyl = min(cy0 + R - 1, L.b0 - 1);
   const int Ya = max(1, first_pix(cy0 - 1, sbin));
@@ -427,56 +458,69 @@ __device__ inline void hog_votes_cells(const HogLevel &L, const int2 *__restrict
   const int ry0 = min(max(y0 - Ya, 0), kHogVoteStage - 1);
   const int ry1 = min(max(y1 - Ya, 0), kHogVoteStage - 1);
   // staged row i of a column is grid row min(Ya + min(i, ns - 1), h - 2), as the reference
-  // clamps the pixel it differentiates; every lane issues the same NL loads per column, and
-  // the next column's loads are in flight while one votes (two columns ahead ran slower)
-  if (xa > xb) return;
+  // clamps the pixel it differentiates; every thread issues the same NL loads per column
+  // (per plane), and the next column's loads are in flight while one votes
+  if (Xa > Xb) return;  // uniform across the workgroup
   const int ylast = max(ns - 1, 0);
-  int2 v[NL];
+  float vm[NL];
+  unsigned int vo[NL];
   auto fetch = [&](int x) {
-    const int2 *col = grid + (long long)min(x, w - 2) * h;
+    const long long col = (long long)min(x, w - 2) * gs;
 #pragma unroll
-    for (int k = 0; k < NL; k++) v[k] = col[max(0, min(Ya + min(lane + 64 * k, ylast), h - 2))];
+    for (int k = 0; k < NL; k++) {
+      const long long e =
+          col + max(0, min(Ya + min((int)threadIdx.x + kHogVoteThreads * k, ylast), h - 2));
+      vm[k] = mag[e];
+      vo[k] = ori[e];
+    }
   };
-  fetch(xa);
-  for (int x = xa; x <= xb; x++) {
+  fetch(Xa);
+  for (int x = Xa; x <= Xb; x++) {
 #pragma unroll
-    for (int k = 0; k < NL; k++) s_col[hog_stage_index(lane + 64 * k)] = v[k];
+    for (int k = 0; k < NL; k++) {
+      const int i = (int)threadIdx.x + kHogVoteThreads * k;
+      if (i < kHogVoteStage) s_col[hog_stage_index(i)] = make_int2(__float_as_int(vm[k]), (int)vo[k]);
+    }
     __syncthreads();
-    if (x < xb) fetch(x + 1);
-    int ix;
-    float v0, v1;
-    interp_coord(x, sbin, &ix, &v0, &v1);
-    const float wx = (ix == cx) ? v1 : v0;
-    hog_vote_column<NJ>(s_col, s_acc + lane, wx, ry0, ry1, wy, ny);
+    if (x < Xb) fetch(x + 1);
+    if (col_live && x >= xa && x <= xb) {
+      int ix;
+      float v0, v1;
+      interp_coord(x, sbin, &ix, &v0, &v1);
+      const float wx = (ix == cx) ? v1 : v0;
+      hog_vote_column<NJ>(s_col, s_acc + lane, wx, ry0, ry1, wy, ny);
+    }
     __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(64) void k_hog_votes(const HogLevel *__restrict__ lv, int nlv,
-                                                  const int2 *__restrict__ grid_arena,
-                                                  float *__restrict__ hist_arena,
-                                                  float *__restrict__ norm_arena) {
+__global__ __launch_bounds__(kHogVoteThreads) void k_hog_votes(
+    const HogLevel *__restrict__ lv, int nlv, const float *__restrict__ mag_arena,
+    const unsigned char *__restrict__ ori_arena, float *__restrict__ hist_arena,
+    float *__restrict__ norm_arena) {
   constexpr int R = kHogVoteRows;
   __shared__ int2 s_col[kHogVotePad];
-  __shared__ float s_acc[18 * R];
+  __shared__ float s_acc[kHogVoteCols][18 * R];
   const int bx = xcd_local_block(blockIdx.x, gridDim.x);
   const HogLevel L = lv[find_blocks(lv, nlv, bx)];
   const int tile = bx - L.blocks_begin;
-  const int cx = tile / L.tiles_y, cy0 = (tile - cx * L.tiles_y) * R;
-  const int lane = threadIdx.x;
+  const int cx0 = tile / L.tiles_y * kHogVoteCols, cy0 = (tile % L.tiles_y) * R;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float *acc = s_acc[wave];
 #pragma unroll
-  for (int o = 0; o < 18; o++) s_acc[o * R + lane] = 0.0f;
-  const int2 *grid = grid_arena + L.grid_off;
+  for (int o = 0; o < 18; o++) acc[o * R + lane] = 0.0f;
+  const float *mag = mag_arena + L.grid_off;
+  const unsigned char *ori = ori_arena + L.grid_off;
   if (2 * L.sbin <= kHogVoteJM / 2)
-    hog_votes_cells<kHogVoteJM / 2>(L, grid, cx, cy0, s_col, s_acc);
+    hog_votes_cells<kHogVoteJM / 2>(L, mag, ori, cx0, cy0, s_col, acc);
   else
-    hog_votes_cells<kHogVoteJM>(L, grid, cx, cy0, s_col, s_acc);
+    hog_votes_cells<kHogVoteJM>(L, mag, ori, cx0, cy0, s_col, acc);
   __syncthreads();
-  const int cy = cy0 + lane;
-  if (cy >= L.b0) return;
+  const int cx = cx0 + wave, cy = cy0 + lane;
+  if (cx >= L.b1 || cy >= L.b0) return;
   const long long BSQ = (long long)L.b0 * L.b1;
   float *hist = hist_arena + L.hist_off + (long long)cx * L.b0 + cy;
-  const float *acc = s_acc + lane;
+  acc += lane;
   float nrm = 0.0f;
 #pragma unroll
   for (int o = 0; o < 18; o++) hist[o * BSQ] = acc[o * R];
@@ -1396,15 +1440,18 @@ void launch_resize_cols(hipStream_t s, const ResizeOp *ops, int nops, int blocks
 }
 
 void launch_hog_grad(hipStream_t s, const GradImg *imgs, int nimg, int blocks, const float *img,
-                     int2 *grid) {
+                     void *grid, long long grid_total) {
   if (blocks > 0)
-    hipLaunchKernelGGL(k_hog_grad, dim3(blocks), dim3(256), 0, s, imgs, nimg, img, grid);
+    hipLaunchKernelGGL(k_hog_grad, dim3(blocks), dim3(256), 0, s, imgs, nimg, img,
+                       (float *)grid, (unsigned char *)grid + 4 * grid_total);
 }
 
-void launch_hog_votes(hipStream_t s, const HogLevel *lv, int nlv, int blocks, const int2 *grid,
-                      float *hist, float *norm) {
+void launch_hog_votes(hipStream_t s, const HogLevel *lv, int nlv, int blocks, const void *grid,
+                      long long grid_total, float *hist, float *norm) {
   if (blocks > 0)
-    hipLaunchKernelGGL(k_hog_votes, dim3(blocks), dim3(64), 0, s, lv, nlv, grid, hist, norm);
+    hipLaunchKernelGGL(k_hog_votes, dim3(blocks), dim3(kHogVoteThreads), 0, s, lv, nlv,
+                       (const float *)grid, (const unsigned char *)grid + 4 * grid_total, hist,
+                       norm);
 }
 
 void launch_hog_feat(hipStream_t s, const HogLevel *lv, int nlv, int blocks, const float *hist,

@@ -361,7 +361,7 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
       g.blocks_begin = gb;
       gb += g.tiles_x * g.tiles_y;
       grid_of[i] = go;
-      go += (long long)g.h * g.w;
+      go += (long long)grid_stride(g.h) * g.w;
       P.grad.push_back(g);
     }
     P.grad_blocks = gb;
@@ -396,7 +396,7 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
     fo += 31LL * L.F0 * L.F1;
     fbo += 32LL * L.F0 * L.F1;
     h.T = kHogVoteRows;
-    h.tiles_x = h.b1;
+    h.tiles_x = (h.b1 + kHogVoteCols - 1) / kHogVoteCols;
     h.tiles_y = (h.b0 + h.T - 1) / h.T;
     HogLevel hh = h;
     hh.blocks_begin = b16;

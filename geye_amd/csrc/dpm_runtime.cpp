@@ -722,7 +722,7 @@ int plan_bytes(const Model &m, int W, int H, long long *bytes) {
   (void)tmp;
   long long b = 4 * (P.img_total + P.hist_total + P.norm_total + P.feat_total + P.conv_total +
                      4 * P.dt_total + 3 * P.score_total) +
-                4 * P.featb_total + 8 * P.grid_total + (long long)P.W * P.H * (3 + 4) +
+                4 * P.featb_total + 5 * P.grid_total + (long long)P.W * P.H * (3 + 4) +
                 N * (2 * (long long)sizeof(NmsBox) + 16 * 4) + (long long)P.GL * 4 * N;
   *bytes = b;
   return GEYE_OK;
@@ -768,7 +768,7 @@ bool dt_windowed(const geye_ctx *c, int precision) {
 int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, int hkind) {
   hipStream_t s = c->stream;
   CHK(ensure(P.d_img, sizeof(float) * P.img_total));
-  CHK(ensure(P.d_grid, sizeof(int2) * std::max<long long>(P.grid_total, 1)));
+  CHK(ensure(P.d_grid, 5 * std::max<long long>(P.grid_total, 1)));
   CHK(ensure(P.d_hist, sizeof(float) * P.hist_total));
   CHK(ensure(P.d_norm, sizeof(float) * P.norm_total));
   if (!hkind) CHK(ensure(P.d_feat, sizeof(float) * P.feat_total));
@@ -791,12 +791,12 @@ int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, int hk
   {
     KTimer t(c, "hog_grad");
     launch_hog_grad(s, dp<GradImg>(P.d_grad), (int)P.grad.size(), P.grad_blocks,
-                    dp<float>(P.d_img), dp<int2>(P.d_grid));
+                    dp<float>(P.d_img), P.d_grid.p, P.grid_total);
   }
   if (P.hog16_n + P.hog8_n > 0) {
     KTimer t(c, "hog_hist");
     launch_hog_votes(s, hog, P.hog16_n + P.hog8_n, P.hog_hist_blocks + P.hog8_blocks,
-                     dp<int2>(P.d_grid), dp<float>(P.d_hist), dp<float>(P.d_norm));
+                     P.d_grid.p, P.grid_total, dp<float>(P.d_hist), dp<float>(P.d_norm));
   }
   // the MFMA forms' channel-last copy: 32 16-bit channels per cell, 64 in the split forms.
   // Sized for the split forms whatever the form, so that switching forms never moves it
@@ -2513,7 +2513,7 @@ int geye_calc_feature(geye_ctx *c, const float *img, int height, int width, int 
   L.img_off = 0;
   L.h = h; L.w = w; L.sbin = sbin; L.b0 = b0; L.b1 = b1; L.F0 = *F0; L.F1 = *F1;
   L.T = kHogVoteRows;
-  L.tiles_x = b1;
+  L.tiles_x = (b1 + kHogVoteCols - 1) / kHogVoteCols;
   L.tiles_y = (b0 + L.T - 1) / L.T;
   L.blocks_begin = 0;
   L.grid_off = 0;
@@ -2528,13 +2528,14 @@ int geye_calc_feature(geye_ctx *c, const float *img, int height, int width, int 
   CHK(upload(dl, std::vector<HogLevel>{L}, c->stream));
   CHK(upload(dg, std::vector<GradImg>{G}, c->stream));
   CHK(upload(dimg, in, c->stream));
-  CHK(ensure(dgrid, sizeof(int2) * (size_t)h * w));
+  const long long grid_total = (long long)grid_stride(h) * w;
+  CHK(ensure(dgrid, 5 * grid_total));
   CHK(ensure(dh, sizeof(float) * 18LL * b0 * b1));
   CHK(ensure(dn, sizeof(float) * (size_t)b0 * b1));
   CHK(ensure(df, sizeof(float) * 31LL * *F0 * *F1));
   launch_hog_grad(c->stream, dp<GradImg>(dg), 1, G.tiles_x * G.tiles_y, dp<float>(dimg),
-                  dp<int2>(dgrid));
-  launch_hog_votes(c->stream, dp<HogLevel>(dl), 1, L.tiles_x * L.tiles_y, dp<int2>(dgrid),
+                  dgrid.p, grid_total);
+  launch_hog_votes(c->stream, dp<HogLevel>(dl), 1, L.tiles_x * L.tiles_y, dgrid.p, grid_total,
                    dp<float>(dh), dp<float>(dn));
   launch_hog_feat(c->stream, dp<HogLevel>(dl), 1, (int)((1LL * *F0 * *F1 + 255) / 256),
                   dp<float>(dh), dp<float>(dn), dp<float>(df), nullptr, 0);
