@@ -386,8 +386,10 @@ def dt_traffic(name: str = "k_dt_window_traffic_1080p_synthetic_car.txt") -> dic
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=3)
+    # 500 steps: about 0.7 s of timed work per serving point, so that start-up transients of
+    # a new server's workers stay out of its rate and p99 is a percentile of 500 latencies
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--precision", default="bf16x3", choices=list(CONV_TAG))
     ap.add_argument("--slots", type=int, default=0, help="serving slots (0 = sweep 1..4)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
