@@ -42,6 +42,7 @@ struct GradImg {
   int h, w;
   int tiles_x, tiles_y;
   int blocks_begin;
+  int frame;                   // >= 0: the image is that frame of the batch, read as BGR bytes
 };
 
 // One fused resize of a whole image: out = cols(rows(in)), 16 x 64 output tiles.  rtab /

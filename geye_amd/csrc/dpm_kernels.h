@@ -5,9 +5,6 @@
 
 namespace geye {
 
-// nframes frames, frame_bytes apart in bgr and out_stride floats apart in out
-void launch_ingest(hipStream_t s, const unsigned char *bgr, int W, int H, int ws, float *out,
-                   int nframes = 1, long long frame_bytes = 0, long long out_stride = 0);
 void launch_resize_rows(hipStream_t s, const ResizeOp *ops, int nops, int blocks,
                         const int2 *tab, const int *si, const float *alpha, const float *src,
                         float *dst);
@@ -23,8 +20,10 @@ void launch_resize_fused(hipStream_t s, const FusedResize *ops, int nops, int bl
                          const int2 *tab, const int4 *si4, const float4 *al4, const float *src,
                          float *dst, const unsigned char *bgr = nullptr, int ws = 0,
                          long long frame_bytes = 0);
+// bgr / ws / frame_bytes: the frames of the images whose GradImg::frame is >= 0
 void launch_hog_grad(hipStream_t s, const GradImg *imgs, int nimg, int blocks, const float *img,
-                     void *grid, long long grid_total);
+                     void *grid, long long grid_total, const unsigned char *bgr = nullptr,
+                     int ws = 0, long long frame_bytes = 0);
 void launch_hog_votes(hipStream_t s, const HogLevel *lv, int nlv, int blocks, const void *grid,
                       long long grid_total, float *hist, float *norm);
 void launch_hog_feat(hipStream_t s, const HogLevel *lv, int nlv, int blocks, const float *hist,

@@ -44,26 +44,6 @@ __device__ inline int find_blocks(const T *__restrict__ d, int n, int bx) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Ingest: BGR interleaved uint8 -> planar float [3][H][W] (row-major).  Plane 0 is B, as in
-// Ipl_to_FLOAT (CT/featurepyramid.cpp.new2:816-850); the conversion is exact.
-// ---------------------------------------------------------------------------------------
-// Frame blockIdx.y of a batch: its bytes frame_bytes after the previous frame's, its planes
-// out_stride floats after the previous frame's.
-__global__ void k_ingest(const unsigned char *__restrict__ bgr, long long frame_bytes, int W,
-                         int H, int ws, float *__restrict__ out, long long out_stride) {
-  bgr += blockIdx.y * frame_bytes;
-  out += blockIdx.y * out_stride;
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long SQ = (long long)W * H;
-  if (i >= SQ) return;
-  int y = (int)(i / W), x = (int)(i - (long long)y * W);
-  const unsigned char *p = bgr + (long long)y * ws + 3 * x;
-  out[i] = (float)p[0];
-  out[SQ + i] = (float)p[1];
-  out[2 * SQ + i] = (float)p[2];
-}
-
-// ---------------------------------------------------------------------------------------
 // Resize, one separable pass.  Each output sums alpha*src over its table entries in table
 // order starting from 0, exactly the per-destination sequence alphacopy produces.
 // rows pass: out[c][dy][x] = sum_k alpha_k * in[c][si_k][x]   (in: [3][in_h][in_w])
@@ -308,13 +288,18 @@ __device__ inline int2 hog_grad_at(const float *t0, const float *t1, const float
 }
 
 // One block = 64 (x) x kGradTY (y) pixels of one image: the 66 x (kGradTY + 2) x 3
-// neighbourhood is read row-major (coalesced) into LDS.  Each thread then takes 4 consecutive
+// neighbourhood is read row-major (coalesced) into LDS -- from the float image, or for the
+// frame itself (G.frame >= 0) from its BGR bytes, plane c being byte c of a pixel as in
+// Ipl_to_FLOAT (CT/featurepyramid.cpp.new2:816-850; the conversion is exact, so the values are
+// the planar copy's).  Each thread then takes 4 consecutive
 // y of one column and writes their magnitudes as one 16-byte store and their orientations as
 // one 4-byte store into the column-major planes.  Columns outside [1, w-2] are not written,
 // and rows outside [1, h-2] hold zeros: readers clamp to that range, exactly as the reference
 // clamps the pixel it differentiates.
 __global__ __launch_bounds__(256) void k_hog_grad(const GradImg *__restrict__ imgs, int nimg,
                                                   const float *__restrict__ img_arena,
+                                                  const unsigned char *__restrict__ bgr, int ws,
+                                                  long long frame_bytes,
                                                   float *__restrict__ mag_arena,
                                                   unsigned char *__restrict__ ori_arena) {
   constexpr int TXP = 64, TYP = kGradTY, SX = TXP + 2, SY = TYP + 2, QY = TYP / 4;
@@ -327,6 +312,7 @@ __global__ __launch_bounds__(256) void k_hog_grad(const GradImg *__restrict__ im
   const int x0 = tx * TXP, y0 = ty * TYP;
   const int h = G.h, w = G.w;
   const float *img = img_arena + G.img_off;
+  const unsigned char *fr = bgr + (G.frame >= 0 ? G.frame * frame_bytes : 0);
   // every element's three loads are issued before the first store (a thread stages up to
   // U elements), so the tile arrives in one round trip
   constexpr int U = (SY * SX + 255) / 256;
@@ -337,10 +323,17 @@ __global__ __launch_bounds__(256) void k_hog_grad(const GradImg *__restrict__ im
       const int i = min((int)threadIdx.x + 256 * u, SY * SX - 1);
       const int r = i / SX, q = i - r * SX;
       const int yy = min(max(y0 - 1 + r, 0), h - 1), xx = min(max(x0 - 1 + q, 0), w - 1);
-      const long long o = (long long)yy * w + xx;
-      v[u][0] = img[o];
-      v[u][1] = img[o + (long long)h * w];
-      v[u][2] = img[o + 2LL * h * w];
+      if (G.frame >= 0) {
+        const unsigned char *p = fr + (long long)yy * ws + 3 * xx;
+        v[u][0] = (float)p[0];
+        v[u][1] = (float)p[1];
+        v[u][2] = (float)p[2];
+      } else {
+        const long long o = (long long)yy * w + xx;
+        v[u][0] = img[o];
+        v[u][1] = img[o + (long long)h * w];
+        v[u][2] = img[o + 2LL * h * w];
+      }
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
@@ -1418,13 +1411,6 @@ __global__ __launch_bounds__(256) void k_rows(const ScoreMap *__restrict__ maps,
 // ---------------------------------------------------------------------------------------
 // Host-side launchers
 // ---------------------------------------------------------------------------------------
-void launch_ingest(hipStream_t s, const unsigned char *bgr, int W, int H, int ws, float *out,
-                   int nframes, long long frame_bytes, long long out_stride) {
-  long long n = (long long)W * H;
-  hipLaunchKernelGGL(k_ingest, dim3((unsigned)((n + 255) / 256), (unsigned)nframes), dim3(256), 0,
-                     s, bgr, frame_bytes, W, H, ws, out, out_stride);
-}
-
 void launch_resize_rows(hipStream_t s, const ResizeOp *ops, int nops, int blocks,
                         const int2 *tab, const int *si, const float *alpha, const float *src,
                         float *dst) {
@@ -1440,10 +1426,11 @@ void launch_resize_cols(hipStream_t s, const ResizeOp *ops, int nops, int blocks
 }
 
 void launch_hog_grad(hipStream_t s, const GradImg *imgs, int nimg, int blocks, const float *img,
-                     void *grid, long long grid_total) {
+                     void *grid, long long grid_total, const unsigned char *bgr, int ws,
+                     long long frame_bytes) {
   if (blocks > 0)
-    hipLaunchKernelGGL(k_hog_grad, dim3(blocks), dim3(256), 0, s, imgs, nimg, img,
-                       (float *)grid, (unsigned char *)grid + 4 * grid_total);
+    hipLaunchKernelGGL(k_hog_grad, dim3(blocks), dim3(256), 0, s, imgs, nimg, img, bgr, ws,
+                       frame_bytes, (float *)grid, (unsigned char *)grid + 4 * grid_total);
 }
 
 void launch_hog_votes(hipStream_t s, const HogLevel *lv, int nlv, int blocks, const void *grid,

@@ -260,11 +260,13 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
   }
   // ---- images and resize rounds
   const int nimg = (int)P.img_h.size();
+  // image 0 is the frame itself: its readers (the first resize round and its gradients) read
+  // the frame's BGR bytes, so it takes no room in the arena
   P.img_off.assign(nimg, 0);
   long long off = 0;
   for (int i = 0; i < nimg; i++) {
     P.img_off[i] = off;
-    off += 3LL * P.img_h[i] * P.img_w[i];
+    if (img_src[i] >= 0) off += 3LL * P.img_h[i] * P.img_w[i];
   }
   P.img_total = off;
   // round of an image = depth of its resize chain
@@ -359,6 +361,7 @@ int build_plan(Model &m, int W, int H, Plan &P, std::string &err, int max_scale_
       g.tiles_x = (g.w + 63) / 64;
       g.tiles_y = (g.h + kGradTY - 1) / kGradTY;
       g.blocks_begin = gb;
+      g.frame = img_src[i] < 0 ? 0 : -1;
       gb += g.tiles_x * g.tiles_y;
       grid_of[i] = go;
       go += (long long)grid_stride(g.h) * g.w;
@@ -726,6 +729,7 @@ void replicate_plan(const Plan &one, int nb, long long feat_stride, long long fe
       g.img_off += f * one.img_total;
       g.grid_off += f * one.grid_total;
       g.blocks_begin += f * one.grad_blocks;
+      if (g.frame >= 0) g.frame = f;
       P.grad.push_back(g);
     }
   P.grad_blocks = one.grad_blocks * nb;

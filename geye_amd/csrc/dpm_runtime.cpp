@@ -767,16 +767,13 @@ bool dt_windowed(const geye_ctx *c, int precision) {
 
 int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, int hkind) {
   hipStream_t s = c->stream;
-  CHK(ensure(P.d_img, sizeof(float) * P.img_total));
+  CHK(ensure(P.d_img, sizeof(float) * std::max<long long>(P.img_total, 1)));
   CHK(ensure(P.d_grid, 5 * std::max<long long>(P.grid_total, 1)));
   CHK(ensure(P.d_hist, sizeof(float) * P.hist_total));
   CHK(ensure(P.d_norm, sizeof(float) * P.norm_total));
   if (!hkind) CHK(ensure(P.d_feat, sizeof(float) * P.feat_total));
-  {
-    KTimer t(c, "ingest");
-    launch_ingest(s, d_bgr, P.W, P.H, ws, dp<float>(P.d_img) + P.img_off[0], P.nb,
-                  (long long)ws * P.H, P.img_one);
-  }
+  // the frame itself is never copied to float planes: the first resize round and its
+  // gradients read its BGR bytes
   int fbase = 0;
   for (size_t r = 0; r < P.fused.size(); r++) {
     KTimer t(c, "resize");
@@ -791,7 +788,8 @@ int run_pyramid(geye_ctx *c, Plan &P, const unsigned char *d_bgr, int ws, int hk
   {
     KTimer t(c, "hog_grad");
     launch_hog_grad(s, dp<GradImg>(P.d_grad), (int)P.grad.size(), P.grad_blocks,
-                    dp<float>(P.d_img), P.d_grid.p, P.grid_total);
+                    dp<float>(P.d_img), P.d_grid.p, P.grid_total, d_bgr, ws,
+                    (long long)ws * P.H);
   }
   if (P.hog16_n + P.hog8_n > 0) {
     KTimer t(c, "hog_hist");
@@ -1413,7 +1411,7 @@ int ensure_frame(Plan &P, size_t bytes) {
   return GEYE_OK;
 }
 
-// The per-frame work before the first host wait -- ingest, resize rounds, HOG, filter
+// The per-frame work before the first host wait -- resize rounds, HOG, filter
 // convolution, DT, scoring and candidate compaction, about 30 launches -- runs as one HIP
 // graph per (plan, precision, threshold, widthStep): captured on the first frame after a
 // warm-up run (which makes every allocation and one-time attribute call), then replayed.
@@ -2522,6 +2520,7 @@ int geye_calc_feature(geye_ctx *c, const float *img, int height, int width, int 
   G.w = w;
   G.tiles_x = (w + 63) / 64;
   G.tiles_y = (h + kGradTY - 1) / kGradTY;
+  G.frame = -1;
   Scratch S;
   DeviceBuffer &dl = S.get(), &dimg = S.get(), &dh = S.get(), &dn = S.get(), &df = S.get(),
                &dg = S.get(), &dgrid = S.get();
