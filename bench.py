@@ -606,8 +606,10 @@ def main() -> int:
     if not args.no_extra:
         line["two_model_batch"] = two_model_batch(geye_amd, ctx, synth, head["slots"],
                                                   args.precision)
-        line["serving_mix"] = serving_mix(geye_amd, ctx, synth, max(head["slots"], 2),
-                                          args.precision)
+        # one slot per camera; the same arrivals with 2 slots beside it (fewer frames share
+        # the GPU, more wait in the queue)
+        line["serving_mix"] = serving_mix(geye_amd, ctx, synth, 4, args.precision)
+        line["serving_mix_2_slots"] = serving_mix(geye_amd, ctx, synth, 2, args.precision)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(model, frame, threads)

@@ -271,6 +271,28 @@ def test_car_detection_synthetic(ctx):
     _rows_equal(g, o)
 
 
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_padded_width_step_reads_the_same_pixels(ctx, precision):
+    """IplImage rows are padded to 4 bytes (widthStep > 3 W for W = 165).  The first resize
+    round and the frame's gradients read the BGR bytes in place: with padding bytes of 255
+    between the rows, every candidate row equals the unpadded frame's, bit for bit."""
+    import ctypes as C
+
+    W, H = 165, 123
+    ws = (3 * W + 3) & ~3
+    say("get_boxes widthStep", ws, "for", 3 * W, precision)
+    frame = synth.make_frame(W, H, seed=21)
+    padded = np.full((H, ws), 255, np.uint8)
+    padded[:, :3 * W] = frame.reshape(H, 3 * W)
+    gm = ctx.model(synth.synthetic_car())
+    ref = gm.get_boxes(frame, thresh=-0.8, precision=precision)
+    got = gm._rows(geye_amd.lib().geye_get_boxes, ctx.h, gm.h, padded.ctypes.data, W, H, ws,
+                   C.c_float(-0.8), geye_amd.PRECISIONS[precision])
+    assert len(ref) > 0
+    _rows_equal(got, ref)
+    gm.free()
+
+
 @pytest.mark.parametrize("overlap", [0.3, 0.5, 0.7, 0.95])
 def test_car_detection_overlaps(ctx, overlap):
     """Device NMS in the detect path against the oracle at several overlaps, from loose (the
