@@ -1688,6 +1688,31 @@ int detect_batch(geye_ctx *c, const geye_model *const *models, int nm,
       key.push_back((long long)(uintptr_t)P[i]);
       gens.push_back(P[i]->gen);
     }
+    // graphs captured against plans that are gone (freed models, evicted plans) or whose
+    // buffers have moved since are never replayed again: destroy them here
+    {
+      std::vector<std::pair<const Plan *, unsigned long long>> live;
+      {
+        std::lock_guard<std::mutex> lk(g_plan_mu);
+        for (const auto &kv : g_plans)
+          if (std::get<0>(kv.first) == c) live.emplace_back(kv.second, kv.second->gen);
+      }
+      for (auto g = c->batch_graphs.begin(); g != c->batch_graphs.end();) {
+        const std::vector<long long> &k = g->first;  // the plans from position 8 on
+        bool stale = false;
+        for (size_t j = 8; j < k.size() && !stale; j++) {
+          const std::pair<const Plan *, unsigned long long> pg(
+              (const Plan *)(uintptr_t)k[j], g->second.second[j - 8]);
+          stale = std::find(live.begin(), live.end(), pg) == live.end();
+        }
+        if (stale) {
+          (void)hipGraphExecDestroy(g->second.first);
+          g = c->batch_graphs.erase(g);
+        } else {
+          ++g;
+        }
+      }
+    }
     auto it = c->batch_graphs.find(key);
     if (it != c->batch_graphs.end() && it->second.second != gens) {
       (void)hipGraphExecDestroy(it->second.first);

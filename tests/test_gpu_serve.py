@@ -207,3 +207,24 @@ def test_detect_batch_equals_each_frame_alone(ctx, precision, W, H, nf, thresh):
     for m in models:
         m.free()
 
+
+
+def test_detect_batch_after_models_are_freed_and_made_again(ctx):
+    """The batch front's captured graph belongs to the models' plans: freeing the models
+    releases those plans, and a batch of new models (whose plans may sit at the same
+    addresses) captures its own graph.  Every cycle's result is each frame's alone."""
+    W, H, nf = 320, 240, 2
+    frames = [synth.make_frame(W, H, seed=950 + k) for k in range(nf)]
+    for cycle in range(3):
+        say("detect batch cycle", cycle)
+        models = [ctx.model(synth.synthetic_car()), ctx.model(synth.synthetic_person())]
+        alone = [[m.car_detection(f, thresh=-0.5, overlap=0.5, precision="bf16x3")
+                  for m in models] for f in frames]
+        for rep in range(2):  # the first call captures, the second replays
+            got = ctx.detect_batch(models, frames, thresh=-0.5, overlap=0.5, precision="bf16x3")
+            for f in range(nf):
+                for i in range(len(models)):
+                    assert np.array_equal(got[f][i].view(np.int32), alone[f][i].view(np.int32)), \
+                        (cycle, rep, f, i)
+        for m in models:
+            m.free()
