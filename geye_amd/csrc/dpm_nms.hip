@@ -143,7 +143,7 @@ __global__ __launch_bounds__(256) void k_nms_gather(const int *__restrict__ ord,
   keep[r] = 0;
   if (g.ext) {
     g.rank_of[i] = r;
-    g.dec[i] = 0;
+    g.dec[r] = 0;  // every candidate's flag cleared: r runs over all of them, in order
   }
 }
 
