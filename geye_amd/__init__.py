@@ -26,14 +26,6 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgeye_amd.so")
 
-# Each serving slot runs on its own HIP stream, and HIP spreads a process's streams over at
-# most GPU_MAX_HW_QUEUES hardware queues (4 unless set).  With 4 slots beside the context's
-# own stream, two slots share a queue and run one after the other: at 1920x1080 in bf16x3,
-# 4 slots gave 613-686 frames/s (p99 8-9 ms) on 4 queues and 777-780 (p99 6.1 ms) on 8.  The
-# package asks for 8 queues unless the variable is set; it takes effect when the package is
-# imported before the process's first HIP call.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
-
 # convolution arithmetic: FP32 (the reference's), one MFMA per tap on BF16 / FP16 operands, or
 # three MFMAs per tap on hi/lo operand pairs (bf16x3, fp16x3)
 GEYE_FP32, GEYE_BF16, GEYE_FP16, GEYE_BF16X3, GEYE_FP16X3 = 0, 1, 2, 3, 4

@@ -234,9 +234,8 @@ int geye_dt_noalloc(geye_ctx *ctx, const float *d_vals, float ax, float bx, floa
  * returns at once (GEYE_ERR_BUSY when queue_cap requests are waiting); geye_serve_wait blocks
  * for one ticket and returns its NMS survivors like geye_detect (free with geye_free) and
  * its latency from submission.  Results do not depend on the other frames in flight.
- * HIP spreads a process's streams over at most GPU_MAX_HW_QUEUES hardware queues (4 unless
- * set): set it to at least slots + 2 before the process's first HIP call, or workers share a
- * queue and run one after the other (the Python package sets 8 unless the variable is set). */
+ * Every context's stream, a worker's included, has a hardware queue of its own, so workers
+ * never wait on one another's queue. */
 typedef struct geye_server geye_server;
 typedef struct {
   int slots;               /* 1..16 frames on the device at once */

@@ -34,23 +34,6 @@ def test_library_exports_every_declared_symbol():
     assert sorted(geye_amd.exported_symbols()) == syms
 
 
-def test_package_asks_for_enough_hardware_queues():
-    """Importing the package sets GPU_MAX_HW_QUEUES to 8 unless the caller set it, so that 4
-    serving slots do not share a hardware queue; a caller's value is kept."""
-    import subprocess
-    import sys
-
-    code = "import os, geye_amd; print(os.environ['GPU_MAX_HW_QUEUES'])"
-    env = {k: v for k, v in os.environ.items() if k != "GPU_MAX_HW_QUEUES"}
-    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=env, capture_output=True,
-                         text=True, check=True).stdout.split()
-    assert out[-1] == "8"
-    env["GPU_MAX_HW_QUEUES"] = "6"
-    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=env, capture_output=True,
-                         text=True, check=True).stdout.split()
-    assert out[-1] == "6"
-
-
 def test_error_path_without_device_work():
     L = geye_amd.lib()
     rc = L.geye_level_plan_dims(None, 0, 8, 10, 640, 480, None, None, None, None, None)
