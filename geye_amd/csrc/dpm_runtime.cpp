@@ -2023,11 +2023,16 @@ int geye_init(int device, geye_ctx **ctx) {
   // queues (4 unless set): a server's four workers beside the caller's context were five
   // streams on four queues, and the two sharing one ran one after the other (1920x1080 bf16x3
   // at 4 slots: 697 frames/s with plain streams, 795 with masked ones, every process).
+  // (A plain stream is the fallback where the masked one cannot be made.)
   std::vector<uint32_t> cu_all((c->num_cu + 31) / 32, 0xffffffffu);
   hipError_t e = hipExtStreamCreateWithCUMask(&c->stream, (uint32_t)cu_all.size(), cu_all.data());
   if (e != hipSuccess) {
+    (void)hipGetLastError();
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  }
+  if (e != hipSuccess) {
     delete c;
-    return fail(GEYE_ERR_HIP, std::string("hipExtStreamCreateWithCUMask: ") + hipGetErrorString(e));
+    return fail(GEYE_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
   // the NMS rounds' alive counts: written by the compaction kernel into coherent pinned
   // memory, read by the host after the round's event
