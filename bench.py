@@ -399,9 +399,25 @@ def main() -> int:
                     help="profiler run: --probe-frames frames of the workload, then a 256 MiB "
                          "device copy whose counters scale FETCH_SIZE / WRITE_SIZE to bytes")
     ap.add_argument("--probe-frames", type=int, default=2)
+    ap.add_argument("--serve-only", action="store_true",
+                    help="profiler run: only the closed serving loop (--slots, --steps, --warmup)")
     args = ap.parse_args()
     if args.probe:
         return probe(args.precision, args.probe_frames)
+    if args.serve_only:
+        import geye_amd
+        from geye_amd import synth
+
+        ctx = geye_amd.Context(0)
+        gm = ctx.model(synth.synthetic_car())
+        d_frame = ctx.dev_upload(synth.make_frame(W, H, seed=2))
+        r = serve_closed_loop(geye_amd, gm, d_frame, args.precision, args.slots or 4, args.steps,
+                              args.warmup)
+        print(json.dumps(r), flush=True)
+        ctx.dev_free(d_frame)
+        gm.free()
+        ctx.close()
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is "
                                    "described in DESIGN.md and not run", "n_gpus": args.gpus}))
