@@ -357,6 +357,10 @@ struct geye_ctx {
   int dt_form = GEYE_DT_AUTO;         // geye_dt_form
   hipStream_t job_streams[4] = {};    // streams of a batch's concurrent NMS jobs (created on use)
   int num_cu = 256;
+  // eighths of the device's resident workgroups the persistent MFMA convolution takes (8:
+  // all).  A serving worker leaves a quarter to the other frames' kernels, which cannot share
+  // a CU with a split-form convolution workgroup (its LDS fills the CU)
+  int conv_eighths = 8;
   int *h_count = nullptr;             // pinned host slots for counts (coherent, mapped: the
   int *d_hcount = nullptr;            // compaction kernels write them directly; see kHost*)
   hipEvent_t ev_count[2] = {nullptr, nullptr};

@@ -21,10 +21,11 @@ int launch_conv_mfma_queue(hipStream_t s, int kind, bool split, int MT, int bloc
                            float *out);
 // kind: 1 = bf16, 2 = fp16; split: hi/lo operand pairs, three MFMAs per tap.  tiles4 are the
 // class's 4-column tiles, tiles8 (optional) the same outputs as 8-column tiles, taken where
-// mfma_wide_tiles says so.  Returns 0 on success, -1 when the shape does not fit.
+// mfma_wide_tiles says so.  The persistent grid takes eighths / 8 of the workgroups the
+// device holds at once (8: all).  Returns 0 on success, -1 when the shape does not fit.
 int launch_conv_mfma(hipStream_t s, int kind, bool split, int nt, const MfmaTile *tiles4,
                      int ntiles4, const MfmaTile *tiles8, int ntiles8, int fh, int fw,
                      const void *feat, const void *filt, const long long *map_off,
-                     const int2 *map_dims, float *out);
+                     const int2 *map_dims, float *out, int eighths = 8);
 
 }  // namespace geye

@@ -551,7 +551,7 @@ bool mfma_wide_tiles(int fh, int fw, int nt, bool split) {
 template <int KIND, int NT, bool SPLIT, int TW>
 static int launch_one(hipStream_t s, const MfmaTile *tiles, int ntiles, int fh, int fw,
                       const void *feat, const void *filt, const long long *map_off,
-                      const int2 *map_dims, float *out) {
+                      const int2 *map_dims, float *out, int eighths) {
   constexpr int WPC = SPLIT && TW == 4 ? 2 : 1;  // waves per output column
   const void *kern = (const void *)k_conv_mfma<KIND, NT, SPLIT, TW, WPC>;
   const int nbuf = mfma_nbuf(fh, fw, NT, SPLIT, TW);
@@ -574,7 +574,7 @@ static int launch_one(hipStream_t s, const MfmaTile *tiles, int ntiles, int fh, 
           hipSuccess ||
       per_cu < 1)
     return -1;
-  const int full = per_cu * ncu;
+  const int full = std::max(1, per_cu * ncu * std::min(std::max(eighths, 1), 8) / 8);
   const int blocks = ntiles <= 4 * full ? ntiles : full;
   hipLaunchKernelGGL((k_conv_mfma<KIND, NT, SPLIT, TW, WPC>), dim3(blocks), dim3(64 * TW * WPC),
                      lds, s, tiles, ntiles, fh, fw, nbuf, (const u32x4 *)feat,
@@ -585,29 +585,29 @@ static int launch_one(hipStream_t s, const MfmaTile *tiles, int ntiles, int fh, 
 template <bool SPLIT, int TW>
 static int launch_kind(hipStream_t s, int kind, int nt, const MfmaTile *tiles, int ntiles, int fh,
                        int fw, const void *feat, const void *filt, const long long *map_off,
-                       const int2 *map_dims, float *out) {
+                       const int2 *map_dims, float *out, int mb) {
   if (kind == 1) {
-    if (nt == 3) return launch_one<1, 3, SPLIT, TW>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
-    if (nt == 2) return launch_one<1, 2, SPLIT, TW>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
-    return launch_one<1, 1, SPLIT, TW>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+    if (nt == 3) return launch_one<1, 3, SPLIT, TW>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out, mb);
+    if (nt == 2) return launch_one<1, 2, SPLIT, TW>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out, mb);
+    return launch_one<1, 1, SPLIT, TW>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out, mb);
   }
-  if (nt == 3) return launch_one<2, 3, SPLIT, TW>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
-  if (nt == 2) return launch_one<2, 2, SPLIT, TW>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
-  return launch_one<2, 1, SPLIT, TW>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out);
+  if (nt == 3) return launch_one<2, 3, SPLIT, TW>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out, mb);
+  if (nt == 2) return launch_one<2, 2, SPLIT, TW>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out, mb);
+  return launch_one<2, 1, SPLIT, TW>(s, tiles, ntiles, fh, fw, feat, filt, map_off, map_dims, out, mb);
 }
 
 int launch_conv_mfma(hipStream_t s, int kind, bool split, int nt, const MfmaTile *tiles4,
                      int ntiles4, const MfmaTile *tiles8, int ntiles8, int fh, int fw,
                      const void *feat, const void *filt, const long long *map_off,
-                     const int2 *map_dims, float *out) {
+                     const int2 *map_dims, float *out, int eighths) {
   if (ntiles4 <= 0) return 0;
   if ((kind != 1 && kind != 2) || nt < 1 || nt > 3) return -1;
   if (tiles8 && ntiles8 > 0 && mfma_wide_tiles(fh, fw, nt, split))
     return launch_kind<true, 8>(s, kind, nt, tiles8, ntiles8, fh, fw, feat, filt, map_off,
-                                map_dims, out);
+                                map_dims, out, eighths);
   if (mfma_nbuf(fh, fw, nt, split, 4) == 0) return -1;
-  return split ? launch_kind<true, 4>(s, kind, nt, tiles4, ntiles4, fh, fw, feat, filt, map_off, map_dims, out)
-               : launch_kind<false, 4>(s, kind, nt, tiles4, ntiles4, fh, fw, feat, filt, map_off, map_dims, out);
+  return split ? launch_kind<true, 4>(s, kind, nt, tiles4, ntiles4, fh, fw, feat, filt, map_off, map_dims, out, eighths)
+               : launch_kind<false, 4>(s, kind, nt, tiles4, ntiles4, fh, fw, feat, filt, map_off, map_dims, out, eighths);
 }
 
 }  // namespace geye

@@ -840,7 +840,7 @@ int run_mfma(geye_ctx *c, Model &m, Plan &P, const Plan &pyr, int precision, int
                            mc.tile_count, dp<MfmaTile>(P.d_mfma_tiles8) + mc.tile8_begin,
                            mc.tile8_count, mc.h, mc.w, pyr.d_featb.p, filt,
                            dp<long long>(P.d_map_off), dp<int2>(P.d_map_dims),
-                           dp<float>(P.d_conv)) != 0)
+                           dp<float>(P.d_conv), c->conv_eighths) != 0)
         return fail(GEYE_ERR_UNSUPPORTED, "filter class too large for the MFMA kernel's LDS");
     }
     return GEYE_OK;

@@ -277,6 +277,10 @@ int geye_serve_create(int device, const geye_serve_config *cfg, geye_server **ou
       delete S;
       return rc;
     }
+    // with other frames in flight, each frame's MFMA convolution keeps 3/4 of the CUs
+    // (1920x1080 bf16x3 at 4 slots: 798-801 frames/s with every CU, 852-855 with 3/4, 815-825
+    // with 5/8, 808 with 7/8; one frame alone is faster with every CU)
+    if (cfg->slots >= 2) c->conv_eighths = 6;
     S->ctx.push_back(c);
   }
   S->pinned.assign(cfg->slots, nullptr);
