@@ -821,6 +821,10 @@ int read_count(geye_ctx *c, Plan &P, int *ncand, int slot = kHostCandidates);
 
 std::mutex g_tune_mu;
 std::map<std::tuple<const Model *, long long, int>, int> g_tuned;  // autotuned MFMA schedules
+// held while a schedule is tuned: one context tunes, the others (a server's workers starting
+// together) wait and take its result, so the candidates are timed without each other's
+// kernels on the device
+std::mutex g_tune_run_mu;
 
 // MFMA convolution schedule `choice`: 0 = one launch per filter class (4 x 64 tiles), else
 // the work-queue launch over every class with 16 * MT row tiles and `bpc` workgroups per CU.
@@ -880,14 +884,20 @@ int tune_mfma(geye_ctx *c, Model &m, Plan &P, const Plan &pyr, int precision) {
   P.mfma_tuning[precision].clear();
   for (int cand : cands) {
     int rc = run_mfma(c, m, P, pyr, precision, cand, false);  // warm-up (attributes, caches)
-    float ms = 0;
-    if (rc == GEYE_OK && hipEventRecord(e0, c->stream) == hipSuccess) {
-      for (int k = 0; k < 3 && rc == GEYE_OK; k++)
-        rc = run_mfma(c, m, P, pyr, precision, cand, false);
+    // the fastest of three runs: a run that shared the device with other work (another
+    // context's kernels still in flight) is not the schedule's own time
+    float ms = 1e30f;
+    for (int k = 0; k < 3 && rc == GEYE_OK; k++) {
+      float t = 0;
+      if (hipEventRecord(e0, c->stream) != hipSuccess) {
+        rc = GEYE_ERR_HIP;
+        break;
+      }
+      rc = run_mfma(c, m, P, pyr, precision, cand, false);
       (void)hipEventRecord(e1, c->stream);
       (void)hipEventSynchronize(e1);
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      ms /= 3;
+      (void)hipEventElapsedTime(&t, e0, e1);
+      ms = std::min(ms, t);
     }
     if (rc != GEYE_OK) continue;
     const int mi = (cand - 1) / 8, bpc = (cand - 1) % 8 + 1;
@@ -929,15 +939,20 @@ int run_scores(geye_ctx *c, Model &m, Plan &P, float thresh, int precision, int 
     if (P.mfma_choice[precision] < 0) {
       // a schedule tuned for the same model and frame geometry on another context is reused
       const auto key = std::make_tuple((const Model *)&m, plan_key(P.W, P.H, P.max_scale), precision);
-      {
+      auto lookup = [&] {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tuned.find(key);
         if (it != g_tuned.end()) P.mfma_choice[precision] = it->second;
-      }
+      };
+      lookup();
       if (P.mfma_choice[precision] < 0) {
-        CHK(tune_mfma(c, m, P, F, precision));
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tuned[key] = P.mfma_choice[precision];
+        std::lock_guard<std::mutex> run(g_tune_run_mu);
+        lookup();  // another context may have tuned it meanwhile
+        if (P.mfma_choice[precision] < 0) {
+          CHK(tune_mfma(c, m, P, F, precision));
+          std::lock_guard<std::mutex> lk(g_tune_mu);
+          g_tuned[key] = P.mfma_choice[precision];
+        }
       }
     }
     CHK(run_mfma(c, m, P, F, precision, P.mfma_choice[precision], true));
