@@ -2009,20 +2009,7 @@ int nms_host(const float *boxes, int NUM, int GL, float overlap, float *out) {
 
 }  // namespace geye
 
-// =======================================================================================
-// C ABI
-// =======================================================================================
-extern "C" {
-
-const char *geye_last_error(void) { return g_err.c_str(); }
-
-int geye_device_count(int *n) {
-  if (!n) return fail(GEYE_ERR_ARG, "null");
-  HIPCHK(hipGetDeviceCount(n));
-  return GEYE_OK;
-}
-
-int geye_init(int device, geye_ctx **ctx) {
+int geye::ctx_create(int device, bool own_queue, geye_ctx **ctx) {
   if (!ctx) return fail(GEYE_ERR_ARG, "null ctx");
   int n = 0;
   HIPCHK(hipGetDeviceCount(&n));
@@ -2033,18 +2020,21 @@ int geye_init(int device, geye_ctx **ctx) {
   if (hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
       c->num_cu <= 0)
     c->num_cu = 256;
-  // The context's stream is created with a CU mask (every CU enabled), which gives it a
-  // hardware queue of its own.  Plain streams share the process's pool of GPU_MAX_HW_QUEUES
-  // queues (4 unless set): a server's four workers beside the caller's context were five
-  // streams on four queues, and the two sharing one ran one after the other (1920x1080 bf16x3
-  // at 4 slots: 697 frames/s with plain streams, 795 with masked ones, every process).
-  // (A plain stream is the fallback where the masked one cannot be made.)
-  std::vector<uint32_t> cu_all((c->num_cu + 31) / 32, 0xffffffffu);
-  hipError_t e = hipExtStreamCreateWithCUMask(&c->stream, (uint32_t)cu_all.size(), cu_all.data());
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
-    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  // own_queue (a serving worker's context): the stream is created with a CU mask (every CU
+  // enabled), which gives it a hardware queue of its own.  Plain streams share the process's
+  // pool of GPU_MAX_HW_QUEUES queues (4 unless set): a server's four workers beside the
+  // caller's context were five streams on four queues, and the two sharing one ran one after
+  // the other (1920x1080 bf16x3 at 4 slots: 697 frames/s with plain streams, 795 with masked
+  // ones, every process).  Other contexts keep a plain stream, which leaves the device's
+  // queues to the workers (4 slots: 829-833 frames/s with every context masked, 847-855
+  // with the workers' alone).  A plain stream is also the fallback.
+  hipError_t e = hipErrorNotSupported;
+  if (own_queue) {
+    std::vector<uint32_t> cu_all((c->num_cu + 31) / 32, 0xffffffffu);
+    e = hipExtStreamCreateWithCUMask(&c->stream, (uint32_t)cu_all.size(), cu_all.data());
+    if (e != hipSuccess) (void)hipGetLastError();
   }
+  if (e != hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete c;
     return fail(GEYE_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
@@ -2070,6 +2060,21 @@ int geye_init(int device, geye_ctx **ctx) {
   *ctx = c;
   return GEYE_OK;
 }
+
+// =======================================================================================
+// C ABI
+// =======================================================================================
+extern "C" {
+
+const char *geye_last_error(void) { return g_err.c_str(); }
+
+int geye_device_count(int *n) {
+  if (!n) return fail(GEYE_ERR_ARG, "null");
+  HIPCHK(hipGetDeviceCount(n));
+  return GEYE_OK;
+}
+
+int geye_init(int device, geye_ctx **ctx) { return geye::ctx_create(device, false, ctx); }
 
 void geye_shutdown(geye_ctx *c) {
   if (!c) return;

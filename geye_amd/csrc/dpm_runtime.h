@@ -8,6 +8,9 @@
 namespace geye {
 
 int compute_max_scale(const Model &m, int W, int H);
+// geye_init; own_queue: the context's stream gets a hardware queue of its own (serving
+// workers, whose streams run concurrently)
+int ctx_create(int device, bool own_queue, geye_ctx **ctx);
 void level_geometry(int max_scale, int sbin, int interval, int W, int H, std::vector<Level> &lv,
                     std::vector<int> &img_h, std::vector<int> &img_w,
                     std::vector<int> &img_src, std::vector<float> &img_scale);

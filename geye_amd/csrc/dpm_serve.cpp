@@ -271,7 +271,7 @@ int geye_serve_create(int device, const geye_serve_config *cfg, geye_server **ou
   S->device = device;
   for (int s = 0; s < cfg->slots; s++) {
     geye_ctx *c = nullptr;
-    const int rc = geye_init(device, &c);
+    const int rc = ctx_create(device, true, &c);
     if (rc != GEYE_OK) {
       for (geye_ctx *x : S->ctx) geye_shutdown(x);
       delete S;

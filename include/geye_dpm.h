@@ -234,9 +234,9 @@ int geye_dt_noalloc(geye_ctx *ctx, const float *d_vals, float ax, float bx, floa
  * returns at once (GEYE_ERR_BUSY when queue_cap requests are waiting); geye_serve_wait blocks
  * for one ticket and returns its NMS survivors like geye_detect (free with geye_free) and
  * its latency from submission.  Results do not depend on the other frames in flight.
- * Every context's stream, a worker's included, has a hardware queue of its own, so workers
- * never wait on one another's queue; with 2 or more slots a worker's MFMA convolution keeps
- * 3/4 of the CUs, so that the other frames' kernels find room beside it. */
+ * Each worker's stream has a hardware queue of its own, so workers never wait on one
+ * another's queue; with 2 or more slots a worker's MFMA convolution keeps 3/4 of the CUs,
+ * so that the other frames' kernels find room beside it. */
 typedef struct geye_server geye_server;
 typedef struct {
   int slots;               /* 1..16 frames on the device at once */
