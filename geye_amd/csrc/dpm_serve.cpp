@@ -277,10 +277,11 @@ int geye_serve_create(int device, const geye_serve_config *cfg, geye_server **ou
       delete S;
       return rc;
     }
-    // with other frames in flight, each frame's MFMA convolution keeps 3/4 of the CUs
-    // (1920x1080 bf16x3 at 4 slots: 798-801 frames/s with every CU, 852-855 with 3/4, 815-825
-    // with 5/8, 808 with 7/8; one frame alone is faster with every CU)
-    if (cfg->slots >= 2) c->conv_eighths = 6;
+    // with other frames in flight, each frame's MFMA convolution keeps part of the CUs:
+    // 1920x1080 bf16x3 at 4 slots gave 798-801 frames/s with every CU, 841-855 with 6/8,
+    // 813-825 with 5/8, 808-820 with 7/8; at 3 slots 5/8 was best (814-831 against 797-806
+    // with 6/8).  One frame alone is faster with every CU.
+    if (cfg->slots >= 2) c->conv_eighths = cfg->slots >= 4 ? 6 : 5;
     S->ctx.push_back(c);
   }
   S->pinned.assign(cfg->slots, nullptr);

@@ -235,8 +235,8 @@ int geye_dt_noalloc(geye_ctx *ctx, const float *d_vals, float ax, float bx, floa
  * for one ticket and returns its NMS survivors like geye_detect (free with geye_free) and
  * its latency from submission.  Results do not depend on the other frames in flight.
  * Each worker's stream has a hardware queue of its own, so workers never wait on one
- * another's queue; with 2 or more slots a worker's MFMA convolution keeps 3/4 of the CUs,
- * so that the other frames' kernels find room beside it. */
+ * another's queue; with 2 or more slots a worker's MFMA convolution keeps 5/8 to 3/4 of the
+ * CUs, so that the other frames' kernels find room beside it. */
 typedef struct geye_server geye_server;
 typedef struct {
   int slots;               /* 1..16 frames on the device at once */
